@@ -1,0 +1,204 @@
+/*
+ * y3g.h -- C ABI of the MI355X (gfx950) cluster-observable integration
+ * library (liby3g.so).
+ *
+ * This is the thin layer between the host C++ (the CosmoSIS module template,
+ * modules/y3g_module.hpp) and the HIP kernels.  Plain pointers and sizes
+ * only; every function returns 0 on success and a non-zero y3g_status
+ * otherwise, with a message available from y3g_last_error().
+ *
+ * Reference interfaces each entry point replaces (reference paths relative
+ * to the estevesjh/y3_cluster_cpp tree):
+ *   y3g_runtime_create      DeviceInitializer, src/utils/CosmoSISSICUDAModule.hh:59-83
+ *                           (cudaGetDeviceCount + cudaSetDevice(local_rank))
+ *   y3g_points_upload /     the per-sample functor construction of
+ *   y3g_tables_build        set_sample(), e.g. hmf_t.cuh:44-52 (mass_function
+ *                           table -> quad::Interp2D on the device),
+ *                           dv_do_dz_t.cuh:13-17, and the table copies
+ *                           quad::Interp* make inside gpuintegration
+ *   y3g_integrate           MultiDimensionalIntegrator<ndim>::integrate,
+ *                           src/utils/gpu_integrator.cuh:142-156 ("pagani" ->
+ *                           Workspace::integrate, "mcubes" -> mcubes::integrate),
+ *                           called once per (volume, grid point) by the serial
+ *                           loops of CosmoSISSICUDAModule.hh:282-335; here the
+ *                           whole list is one device submission
+ *   y3g_result              numint::integration_result fields used at
+ *                           CosmoSISSICUDAModule.hh:299-309
+ *                           (estimate, errorest, chi_sq, neval, nregions, status)
+ *   y3g_eval_functor        the host-side functor calls of the reference's
+ *                           Catch2 tests (e.g. test/roffset_t.test.cc:40-46),
+ *                           here evaluated by the device functors
+ */
+#ifndef Y3G_H
+#define Y3G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  Y3G_OK = 0,
+  Y3G_ERR_ARG = 1,      /* invalid argument                              */
+  Y3G_ERR_HIP = 2,      /* HIP runtime error                             */
+  Y3G_ERR_MEMORY = 3,   /* allocation failed / over the HBM budget       */
+  Y3G_ERR_NODEVICE = 4, /* no gfx950 device                              */
+  Y3G_ERR_STATE = 5     /* call out of order                             */
+} y3g_status;
+
+/* Integrand kinds of the hot path. */
+typedef enum {
+  Y3G_COUNT3D = 0,  /* (t, zt, lnM), lambda_obs bin analytic (erf)        */
+  Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM)                               */
+  Y3G_DSIGMA6D = 2  /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
+} y3g_kind;
+
+/* Integrators: the names of gpu_integrator.cuh:54-65. */
+typedef enum {
+  Y3G_PAGANI = 0,   /* Genz-Malik adaptive cubature                       */
+  Y3G_MCUBES = 1    /* VEGAS with m-Cubes stratification                  */
+} y3g_algorithm;
+
+/* Arithmetic of the integrand. */
+typedef enum {
+  Y3G_FP32SPLIT = 0, /* FP32 integrand, FP64 sums / errors / grid updates  */
+  Y3G_FP64 = 1       /* FP64 throughout                                   */
+} y3g_precision;
+
+/* Per-parameter-point physics.  Same field order as the oracle's
+ * y3o_params. */
+typedef struct {
+  double omega_m, h;
+  double hmf_amp, hmf_A, hmf_a, hmf_c;
+  double mor_lnl0, mor_slope, mor_zslope, mor_sigma_ln;
+  double f_prj, prj_scale;
+  double f_mis, mis_rlambda;
+  double z_source, sigz0;
+} y3g_params;
+
+/* Uniform table grids of one parameter point. */
+typedef struct {
+  double lnm_lo, lnm_hi;
+  int n_lnm;
+  double z_lo, z_hi;
+  int n_z;
+  double da_zmax;
+  int n_da;
+  double sci_zmax;
+  int n_sci;
+} y3g_table_spec;
+
+/* One integral.  lows/highs are the box of the integration variables
+ * (the standardised MOR variable t replaces lambda_true; lt_lo/lt_hi keep
+ * the lambda_true range as an indicator). */
+typedef struct {
+  int kind;
+  int ndim;
+  int point;          /* parameter point (table set) index               */
+  int algorithm;      /* y3g_algorithm                                   */
+  double lo_lo, lo_hi;
+  double zo_lo, zo_hi;
+  double radius;
+  double sigma_mis;
+  double rmis_norm;
+  double lt_lo, lt_hi;
+  double lows[8], highs[8];
+  double eps_rel, eps_abs;
+  long long max_eval;
+  uint64_t seed;      /* VEGAS Philox key                                */
+} y3g_integral;
+
+/* The reference's integration_result. */
+typedef struct {
+  double value, error, prob;
+  long long neval;
+  int nregions, status;   /* status 0 = converged, 1 = limit reached     */
+} y3g_result;
+
+/* Integrator tuning (defaults from y3g_options_default). */
+typedef struct {
+  int precision;          /* y3g_precision                               */
+  int max_regions;        /* per-integral region capacity (cubature)     */
+  int max_split_per_step; /* regions split per step (cubature)           */
+  int init_split;         /* initial pieces per axis, 0 = by dimension   */
+  int block_threads;      /* threads per workgroup                       */
+  int slots;              /* resident workgroups (0 = by device)         */
+  long long vegas_calls;  /* VEGAS calls per iteration                   */
+  int vegas_itmax, vegas_itskip, vegas_nbins;
+  double vegas_alpha;
+  int single;             /* 1: one integral per device-wide launch
+                             sequence (the plain path)                   */
+} y3g_options;
+
+typedef struct y3g_runtime y3g_runtime;
+
+const char* y3g_last_error(void);
+const char* y3g_version(void);
+
+/* Bind a device (DeviceInitializer semantics: device = local rank). */
+int y3g_runtime_create(int device, y3g_runtime** out);
+int y3g_runtime_destroy(y3g_runtime* rt);
+int y3g_device_info(y3g_runtime* rt, int* n_cu, double* clock_ghz,
+                    size_t* hbm_bytes, char* name, int name_len);
+
+void y3g_options_default(y3g_options* o);
+void y3g_default_params(y3g_params* p);
+void y3g_spec_config0(y3g_table_spec* s);
+void y3g_spec_wide(y3g_table_spec* s);
+
+/* Build the tables of n_points parameter points on the device (HMF
+ * 200x32, D_A 512, Sigma_crit^-1 128 per point, FP64 and FP32 copies).
+ * Replaces any previous point set. */
+int y3g_tables_build(y3g_runtime* rt, const y3g_params* params,
+                     const y3g_table_spec* spec, int n_points);
+
+/* Upload host tables for point `point` instead of building them (the
+ * DataBlock path: mass_function/dndlnmh etc.); arrays sized by the spec. */
+int y3g_tables_upload(y3g_runtime* rt, int point, const double* hmf,
+                      const double* da, const double* sci);
+
+/* Copy the device tables of one point back (tests). */
+int y3g_tables_download(y3g_runtime* rt, int point, double* hmf, double* da,
+                        double* sci);
+
+/* Fill an integral descriptor for the synthetic workloads. */
+void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
+                       double lo_lo, double lo_hi, double zo_lo, double zo_hi,
+                       double radius, y3g_integral* out);
+
+/* Integrate n integrals (one device submission, synchronous). */
+int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                  const y3g_options* opt, y3g_result* out);
+
+/* Time of the last y3g_integrate's kernels in milliseconds (HIP events on
+ * the library's stream) and the number of integrand evaluations they made. */
+int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
+
+/* Evaluate a device functor on n inputs (tests of the model functors).
+ * which: see y3g_functor; in[n * nin], out[n]. */
+typedef enum {
+  Y3G_F_ROFFSET_GAMMA = 0, /* (x; tau)                          roffset_t   */
+  Y3G_F_LO_LC = 1,         /* (lo, lc, R; alpha, a, b, R_lambda) lo_lc_t    */
+  Y3G_F_ZO_ZT_DES = 2,     /* (zo, zt)                          zo_zt_des_t */
+  Y3G_F_INT_ZO_ZT_DES = 3, /* (zomin, zomax, zt)          int_zo_zt_des_t   */
+  Y3G_F_SIGMA_PHOTOZ = 4,  /* (zt)                        sigma_photoz_des  */
+  Y3G_F_DV_DO_DZ = 5,      /* (zt) of point 0                 dv_do_dz_t    */
+  Y3G_F_HMF = 6,           /* (lnM, zt) of point 0                hmf_t     */
+  Y3G_F_MOR = 7,           /* (lt, lnM, zt) of point 0    lognormal MOR     */
+  Y3G_F_LO_LT_PDF = 8,     /* (lo, lt) of point 0   Gauss + EMG tail        */
+  Y3G_F_LO_LT_BIN = 9,     /* (lo_lo, lo_hi, lt) of point 0                 */
+  Y3G_F_NFW_DSIGMA = 10,   /* (R, lnM, rho, conc)  centred NFW Delta-Sigma  */
+  Y3G_F_DSIGMA_OFFSET = 11,/* (R, rmis, theta, lnM, rho, conc) tangential    */
+  Y3G_F_SCI = 12,          /* (zt) of point 0   Sigma_crit^-1              */
+  Y3G_F_INTEGRAND = 13     /* (y[ndim]) mapped integrand of integral `aux` */
+} y3g_functor;
+int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
+                     const double* in, int nin, size_t n, const double* par,
+                     int npar, const y3g_integral* aux, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

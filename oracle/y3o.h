@@ -1,0 +1,232 @@
+/*
+ * y3o.h -- FP64 CPU oracle for the cluster-observable integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * by nothing else.  The product path (y3_cluster_cpp_amd, the HIP library and
+ * the CosmoSIS modules) never links or calls it.
+ *
+ * It restates, in plain C99 and double precision:
+ *   - the reference model functors (reference form, for the reference's own
+ *     known-answer fixtures) and their synthetic-workload parameterisations;
+ *   - the three integrands of the hot path (3-D count, 5-D count, 6-D
+ *     miscentred Delta-Sigma);
+ *   - a structure-exploiting nested Gauss-Legendre / Gauss-Kronrod "golden"
+ *     integrator (truth to ~1e-9);
+ *   - CPU restatements of the two integrators the GPU library implements
+ *     (Genz-Malik adaptive cubature, VEGAS with m-Cubes stratification), with
+ *     the same region / grid algorithms and the same Philox4x32-10 streams.
+ *
+ * Parity pins (which check pins which function) are listed in the header
+ * comment of y3o.c next to each function.
+ */
+#ifndef Y3O_H
+#define Y3O_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- parameters ------------------------------------------- */
+
+/* Per-parameter-point physics (one "cosmology" of the sampler).  Field order
+ * is shared with include/y3g.h (y3g_params) so one marshalling serves both. */
+typedef struct {
+  double omega_m;        /* flat LCDM, Omega_Lambda = 1 - omega_m          */
+  double h;              /* H0 / 100                                        */
+  double hmf_amp;        /* sigma(M,z) amplitude (0.8)                      */
+  double hmf_A;          /* 0.2                                             */
+  double hmf_a;          /* 1.5                                             */
+  double hmf_c;          /* 1.2                                             */
+  double mor_lnl0;       /* ln lambda-bar at pivot (3.2)                    */
+  double mor_slope;      /* d ln lambda / d ln M (0.9)                      */
+  double mor_zslope;     /* d ln lambda / d ln(1+z) (0.5)                   */
+  double mor_sigma_ln;   /* lognormal scatter (0.25)                        */
+  double f_prj;          /* projection-tail fraction (0.2; 0 for 3-D)       */
+  double prj_scale;      /* exponential excess scale / lambda_true (0.1)    */
+  double f_mis;          /* miscentred fraction (0.2)                       */
+  double mis_rlambda;    /* sigma_Rayleigh / R_lambda (0.2)                 */
+  double z_source;       /* source plane for Sigma_crit^-1 (1.2)            */
+  double sigz0;          /* photo-z scatter sigma_z = sigz0 (1+z) (0.01)    */
+} y3o_params;
+
+/* Table geometry (uniform grids). */
+typedef struct {
+  double lnm_lo, lnm_hi;  int n_lnm;   /* HMF lnM axis, 200              */
+  double z_lo, z_hi;      int n_z;     /* HMF z axis, 32                 */
+  double da_zmax;         int n_da;    /* D_A(z) on [0, da_zmax], 512    */
+  double sci_zmax;        int n_sci;   /* Sigma_crit^-1(z_l), 128        */
+} y3o_table_spec;
+
+/* Tables of one parameter point, FP64.  hmf is stored x-fastest:
+ * hmf[iz * n_lnm + im]  (ndarray of shape (n_z, n_lnM)). */
+typedef struct {
+  y3o_table_spec spec;
+  double *lnm, *z, *hmf;     /* n_lnm, n_z, n_z*n_lnm */
+  double *da_z, *da;         /* n_da, Mpc             */
+  double *sci_z, *sci;       /* n_sci                 */
+} y3o_tables;
+
+/* Integrand kinds. */
+enum { Y3O_COUNT3D = 0, Y3O_COUNT5D = 1, Y3O_DSIGMA6D = 2 };
+
+/* One integral of the hot path. */
+typedef struct {
+  int kind;
+  int ndim;
+  double lo_lo, lo_hi;     /* lambda_obs bin                                */
+  double zo_lo, zo_hi;     /* z_obs bin                                     */
+  double radius;           /* R [Mpc/h] for Delta-Sigma                     */
+  double sigma_mis;        /* Rayleigh sigma [Mpc/h] for Delta-Sigma        */
+  double rmis_norm;        /* 1 / (1 - exp(-rmax^2 / 2 sigma^2))            */
+  double lt_lo, lt_hi;     /* lambda_true range [1, 300]                    */
+  double lows[8], highs[8];  /* box of the integration variables y        */
+} y3o_integral;
+
+/* Integration result: the fields of the reference's integration_result. */
+typedef struct {
+  double value, error, prob;
+  long long neval;
+  int nregions, status;
+} y3o_result;
+
+/* ---------------- primitives and reference-form functors --------------- */
+double y3o_gaussian(double x, double mu, double sigma);
+double y3o_roffset_gamma(double x, double tau);
+double y3o_lo_lc(double lo, double lc, double R_mis, double alpha, double a,
+                 double b, double R_lambda);
+double y3o_sigma_photoz_des(double zt);
+double y3o_zo_zt_des(double zo, double zt);
+double y3o_int_zo_zt_des(double zomin, double zomax, double zt);
+double y3o_ez(double z, double om, double ol, double ok);
+int    y3o_interp1d(const double* xs, const double* ys, int n, double x,
+                    double* out);
+double y3o_interp1d_clamp(const double* xs, const double* ys, int n, double x);
+int    y3o_interp2d(const double* xs, int nx, const double* ys, int ny,
+                    const double* zs, double x, double y, double* out);
+double y3o_interp2d_clamp(const double* xs, int nx, const double* ys, int ny,
+                          const double* zs, double x, double y);
+double y3o_dv_do_dz(double zt, const double* da_z, const double* da, int n,
+                    double om, double ol, double ok, double h);
+double y3o_lc_lt_eq15(double lc, double lt, double mu, double sigma,
+                      double tau, double fmsk, double fprj);
+
+/* ---------------- synthetic-workload functors --------------------------- */
+double y3o_hmf_formula(const y3o_params* p, double lnM, double z);
+double y3o_mor_lognormal(const y3o_params* p, double lt, double lnM, double zt);
+double y3o_lo_lt_pdf(const y3o_params* p, double lo, double lt);
+double y3o_lo_lt_bin(const y3o_params* p, double lo_lo, double lo_hi, double lt);
+double y3o_emg_pdf(double x, double mu, double sigma, double tau);
+double y3o_emg_cdf(double x, double mu, double sigma, double tau);
+double y3o_zo_pdf(const y3o_params* p, double zo, double zt);
+double y3o_zo_bin(const y3o_params* p, double zo_lo, double zo_hi, double zt);
+double y3o_comoving_distance(double om, double h, double z); /* Mpc */
+double y3o_nfw_conc(double lnM);
+double y3o_nfw_g(double x);  /* Delta-Sigma / (rho_s r_s) */
+double y3o_nfw_sigma_unit(double x); /* Sigma / (rho_s r_s) */
+double y3o_nfw_dsigma(double R, double lnM, double rho, double conc);
+double y3o_dsigma_offset(double R, double rmis, double theta, double lnM,
+                         double rho, double conc);
+double y3o_dsigma_mis_ring(double R, double rmis, double lnM, double rho,
+                           double conc, double epsrel);
+double y3o_dsigma_mis_gamma(double R, double tau, double lnM, double rho,
+                            double conc, double epsrel);
+double y3o_dsigma_mis_rayleigh(double R, double sigma, double rmax,
+                               double lnM, double rho, double conc,
+                               double epsrel);
+
+/* ---------------- tables ------------------------------------------------ */
+void y3o_default_params(y3o_params* p);
+void y3o_spec_config0(y3o_table_spec* s);
+void y3o_spec_wide(y3o_table_spec* s);
+int  y3o_tables_build(const y3o_params* p, const y3o_table_spec* s,
+                      y3o_tables* t);
+void y3o_tables_free(y3o_tables* t);
+double y3o_hmf(const y3o_tables* t, double lnM, double zt);
+double y3o_dvdz(const y3o_params* p, const y3o_tables* t, double zt);
+double y3o_sci(const y3o_tables* t, double zt);
+
+/* ---------------- integrands ------------------------------------------ */
+double y3o_integrand(const y3o_params* p, const y3o_tables* t,
+                     const y3o_integral* ig, const double* x);
+double y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
+                            const y3o_integral* ig, const double* y);
+void   y3o_integrand_mapped_batch(const y3o_params* p, const y3o_tables* t,
+                                  const y3o_integral* ig, const double* y,
+                                  long n, double* out);
+void   y3o_make_integral(const y3o_params* p, const y3o_table_spec* s,
+                         int kind, double lo_lo, double lo_hi, double zo_lo,
+                         double zo_hi, double radius, y3o_integral* ig);
+
+/* ---------------- quadrature ------------------------------------------ */
+double y3o_gk21_adapt(double (*f)(double, void*), void* ctx, double a,
+                      double b, double epsrel, double epsabs, double* err);
+void   y3o_gl_rule(int n, double* x, double* w);
+const double* y3o_gk21_nodes(void);
+const double* y3o_gk21_wk(void);
+const double* y3o_gk21_wg(void);
+
+/* Golden values: counts[n_lo*n_zo] (config 0 when kind == COUNT3D and
+ * n_zo == 0) and, when n_r > 0, dsig[n_lo*n_zo*n_r] (bin-major, radius
+ * fastest).  Returns 0 on success. */
+int y3o_golden(const y3o_params* p, const y3o_tables* t, int count_kind,
+               const double* lo_edges, int n_lo, const double* zo_edges,
+               int n_zo, const double* radii, int n_r, double* counts,
+               double* dsig);
+
+void y3o_golden_set_orders(int ngz, int ngm);
+
+/* Genz-Malik adaptive cubature, the CPU restatement of the device
+ * algorithm (csrc/y3g_kernels.hip k_cubature). */
+typedef struct {
+  double eps_rel, eps_abs;
+  long long max_eval;
+  int max_regions;
+  int max_split_per_step;
+  int init_split;           /* pieces per axis of the initial partition */
+} y3o_cub_opts;
+void y3o_cubature(const y3o_params* p, const y3o_tables* t,
+                  const y3o_integral* ig, const y3o_cub_opts* o,
+                  y3o_result* r);
+/* Generic: integrate f over [lows, highs] (no table snapping). */
+void y3o_cubature_fn(double (*f)(const double*, void*), void* ctx, int ndim,
+                     const double* lows, const double* highs,
+                     const y3o_cub_opts* o, y3o_result* r);
+void y3o_cubature_fn_snap(double (*f)(const double*, void*), void* ctx,
+                          int ndim, const double* lows, const double* highs,
+                          const double* snap_x0, const double* snap_dx,
+                          const y3o_cub_opts* o, y3o_result* r);
+/* Genz-Malik rule of degree 7 / 5 on one region; returns R7, sets *r5 and
+ * the fourth differences fourth[ndim]. */
+double y3o_gm_rule(double (*f)(const double*, void*), void* ctx, int ndim,
+                   const double* center, const double* halfw, double* r5,
+                   double* fourth);
+int y3o_gm_num_nodes(int ndim);
+
+/* VEGAS / m-Cubes. */
+typedef struct {
+  double eps_rel, eps_abs;
+  long long max_eval;       /* maxcalls */
+  long long calls_per_iter;
+  int itmax, itskip, nbins;
+  double alpha;
+  uint64_t seed;
+} y3o_vegas_opts;
+void y3o_vegas(const y3o_params* p, const y3o_tables* t,
+               const y3o_integral* ig, const y3o_vegas_opts* o, y3o_result* r);
+void y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int ndim,
+                  const double* lows, const double* highs,
+                  const y3o_vegas_opts* o, y3o_result* r);
+
+/* Philox4x32-10 (Salmon et al. 2011). */
+void y3o_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Test integrands of the reference's integrator known-answer tests. */
+double y3o_test_fn(int which, const double* x, int ndim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,706 @@
+/*
+ * y3o_models.c -- FP64 restatement of the model functors, tables and
+ * integrands of the cluster-observable integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): linked by tests/, smoke() and the
+ * bench's cpu_baseline leg; never by the product library.
+ *
+ * Each function cites the reference file it restates.  Pins (what checks each
+ * function) are in tests/test_oracle_*.py:
+ *   y3o_gaussian            primitives.hh:60-65        via every fixture below
+ *   y3o_roffset_gamma       roffset_t.hh:18-23         data/test_Rmis.txt (1e-6)
+ *   y3o_lo_lc               lo_lc_t.hh:28-39           data/test_lolc_t.txt subsample (1e-6)
+ *   y3o_sigma_photoz_des    sigma_photoz_des.hh        data/test_int_zo_zt_des.txt (1e-6)
+ *   y3o_zo_zt_des           zo_zt_des_t.hh:16-21       data/test_zo_zt_des.txt (1e-6)
+ *   y3o_int_zo_zt_des       int_zo_zt_des_t.hh:14-21   data/test_int_zo_zt_des.txt (1e-6)
+ *   y3o_interp1d/2d         interp_1d.cc, interp_2d.cc interp_2d.test.cc closed forms
+ *   y3o_dv_do_dz            dv_do_dz_t.hh:23-31        data/test_dv_do_dz.txt (1e-3)
+ *   y3o_lc_lt_eq15          lc_lt_t.hh:31-63           normalisation (lc_lt_probability_unity)
+ *   y3o_emg_*               emg_des_t.cuh:136-193      CDF' == pdf, CDF(inf) == 1
+ *   y3o_nfw_*               nfw_dsigma_mis.hh:95-114   line-of-sight Sigma integral;
+ *                                                      cluster_toolkit values of
+ *                                                      nfw_dsigma_mis.test.cu:32-49 (5e-3)
+ *   y3o_comoving_distance   ez.hh / dv_do_dz fixture   data/d_a_test.txt (1e-4)
+ */
+#include "y3o.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define Y3O_PI 3.14159265358979323846
+#define Y3O_INV_SQRT2PI 0.3989422804014327
+#define Y3O_SQRT2 1.4142135623730951
+#define Y3O_RHOC 2.77533742639e+11 /* h^2 Msun / Mpc^3, nfw_dsigma_mis.hh:26 */
+#define Y3O_DH 2997.92458          /* c / H0 in Mpc/h                     */
+#define Y3O_DV_CONST 2997.92       /* dv_do_dz_t.hh:29                    */
+#define Y3O_SCI_CONST 6.0134795e-7 /* 4 pi G / c^2 [pc/Msun] * 1e6 pc/Mpc */
+#define Y3O_LN_MPIVOT_HMF 32.236191301916641 /* ln 1e14 */
+#define Y3O_LN_MPIVOT_MOR 33.334803590584751 /* ln 3e14 */
+
+/* ---------------- primitives (primitives.hh:60-65) ---------------------- */
+double
+y3o_gaussian(double x, double mu, double sigma)
+{
+  double const z = (x - mu) / sigma;
+  return exp(-z * z / 2.0) * Y3O_INV_SQRT2PI / sigma;
+}
+
+/* roffset_t.hh:18-23: x / tau^2 exp(-x / tau) (Gamma, k = 2). */
+double
+y3o_roffset_gamma(double x, double tau)
+{
+  return x / tau / tau * exp(-x / tau);
+}
+
+/* lo_lc_t.hh:28-39 */
+double
+y3o_lo_lc(double lo, double lc, double R_mis, double alpha, double a, double b,
+          double R_lambda)
+{
+  double const x = R_mis / R_lambda;
+  double const y = lo / lc;
+  double const mu_y = exp(-x * x / alpha / alpha);
+  double const sigma_y = a * atan(b * x);
+  return y3o_gaussian(y, mu_y, sigma_y) / lc;
+}
+
+/* sigma_photoz_des.hh: degree-10 Horner polynomial in (z - 0.4) on the
+ * clamped range [0.15, 0.7].  The coefficients are the reference's fit. */
+double
+y3o_sigma_photoz_des(double zt)
+{
+  static const double c[11] = {-40358.8315,  2798.08304,   9333.80185,
+                               -657.348248,  -840.565610,  46.8506649,
+                               37.8839498,   -0.868811858, -0.808928182,
+                               0.00890199353, 0.0139811265};
+  double z = zt < 0.15 ? 0.15 : (zt > 0.7 ? 0.7 : zt);
+  double s = 0.0;
+  z -= 0.4;
+  for (int i = 0; i < 10; ++i) s = (c[i] + s) * z;
+  return s + c[10];
+}
+
+/* zo_zt_des_t.hh:16-21 */
+double
+y3o_zo_zt_des(double zo, double zt)
+{
+  return y3o_gaussian(zo, zt, y3o_sigma_photoz_des(zt));
+}
+
+/* int_zo_zt_des_t.hh:14-21 */
+double
+y3o_int_zo_zt_des(double zomin, double zomax, double zt)
+{
+  double const base = Y3O_SQRT2 * y3o_sigma_photoz_des(zt);
+  return (erf((zomax - zt) / base) - erf((zomin - zt) / base)) / 2.0;
+}
+
+/* ez.hh:19-24 / ez_sq.hh:11-18 */
+double
+y3o_ez(double z, double om, double ol, double ok)
+{
+  double const zp = 1.0 + z;
+  return sqrt(om * zp * zp * zp + ok * zp * zp + ol);
+}
+
+/* GSL's bracketing search (index_lo <= result < index_hi), the lookup used
+ * by interp_1d.cc:70 and interp_2d.cc:123 through gsl_interp*_eval_e. */
+static int
+bsearch_idx(const double* xa, double x, int lo, int hi)
+{
+  while (hi > lo + 1) {
+    int const i = (hi + lo) / 2;
+    if (xa[i] > x)
+      hi = i;
+    else
+      lo = i;
+  }
+  return lo;
+}
+
+/* interp_1d.cc:62-72: linear interpolation, error outside [x0, xn]. */
+int
+y3o_interp1d(const double* xs, const double* ys, int n, double x, double* out)
+{
+  if (x < xs[0] || x > xs[n - 1]) return 1;
+  int const i = bsearch_idx(xs, x, 0, n - 1);
+  double const dx = xs[i + 1] - xs[i];
+  *out = dx > 0.0 ? ys[i] + (x - xs[i]) * (ys[i + 1] - ys[i]) / dx : ys[i];
+  return 0;
+}
+
+/* interp_1d.hh clamp(): std::clamp then evaluate. */
+double
+y3o_interp1d_clamp(const double* xs, const double* ys, int n, double x)
+{
+  double v = 0.0;
+  double const xc = x < xs[0] ? xs[0] : (x > xs[n - 1] ? xs[n - 1] : x);
+  y3o_interp1d(xs, ys, n, xc, &v);
+  return v;
+}
+
+/* interp_2d.cc:115-127: GSL bilinear, z stored x-fastest (z[j*nx+i]). */
+int
+y3o_interp2d(const double* xs, int nx, const double* ys, int ny,
+             const double* zs, double x, double y, double* out)
+{
+  if (x < xs[0] || x > xs[nx - 1] || y < ys[0] || y > ys[ny - 1]) return 1;
+  int const i = bsearch_idx(xs, x, 0, nx - 1);
+  int const j = bsearch_idx(ys, y, 0, ny - 1);
+  double const t = (x - xs[i]) / (xs[i + 1] - xs[i]);
+  double const u = (y - ys[j]) / (ys[j + 1] - ys[j]);
+  double const z00 = zs[j * nx + i], z10 = zs[j * nx + i + 1];
+  double const z01 = zs[(j + 1) * nx + i], z11 = zs[(j + 1) * nx + i + 1];
+  *out = (1.0 - t) * (1.0 - u) * z00 + t * (1.0 - u) * z10 +
+         (1.0 - t) * u * z01 + t * u * z11;
+  return 0;
+}
+
+/* interp_2d.hh:82-86 clamp(): clamp both coordinates, then evaluate. */
+double
+y3o_interp2d_clamp(const double* xs, int nx, const double* ys, int ny,
+                   const double* zs, double x, double y)
+{
+  double v = 0.0;
+  double const xc = x < xs[0] ? xs[0] : (x > xs[nx - 1] ? xs[nx - 1] : x);
+  double const yc = y < ys[0] ? ys[0] : (y > ys[ny - 1] ? ys[ny - 1] : y);
+  y3o_interp2d(xs, nx, ys, ny, zs, xc, yc, &v);
+  return v;
+}
+
+/* dv_do_dz_t.hh:23-31 (D_A in Mpc, result (Mpc/h)^3 / sr). */
+double
+y3o_dv_do_dz(double zt, const double* da_z, const double* da, int n, double om,
+             double ol, double ok, double h)
+{
+  double da_z_v = 0.0;
+  y3o_interp1d(da_z, da, n, zt, &da_z_v);
+  return Y3O_DV_CONST * (1.0 + zt) * (1.0 + zt) * da_z_v * h * da_z_v * h /
+         y3o_ez(zt, om, ol, ok);
+}
+
+/* lc_lt_t.hh:31-63 (Costanzi et al. 2018 eq. 15) with the five table
+ * lookups replaced by explicit parameters. */
+double
+y3o_lc_lt_eq15(double lc, double lt, double mu, double sigma, double tau,
+               double fmsk, double fprj)
+{
+  double const exptau =
+    exp(tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lc) / 2.0);
+  double const r2s = Y3O_SQRT2 * sigma;
+  double const mts = mu + tau * sigma * sigma;
+  return (1.0 - fmsk) * (1.0 - fprj) * y3o_gaussian(lc, mu, sigma) +
+         0.5 * ((1.0 - fmsk) * fprj * tau + fmsk * fprj / lt) * exptau *
+           erfc((mts - lc) / r2s) +
+         0.5 * fmsk / lt * (erfc((lc - mu) / r2s) - erfc((lc + lt - mu) / r2s)) -
+         0.5 * fmsk * fprj / lt *
+           (exp(-tau * lt) * exptau * erfc((mts - (lc + lt)) / r2s));
+}
+
+/* ---------------- erfcx (FP64) ------------------------------------------ */
+static double
+y3o_erfcx(double u)
+{
+  if (u < 10.0) return exp(u * u) * erfc(u);
+  /* asymptotic series 1/(u sqrt(pi)) sum_k (-1)^k (2k-1)!! / (2u^2)^k */
+  double const inv2u2 = 1.0 / (2.0 * u * u);
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 60; ++k) {
+    term *= -(2.0 * k - 1.0) * inv2u2;
+    sum += term;
+    if (fabs(term) < 1e-17 * fabs(sum)) break;
+  }
+  return sum / (u * sqrt(Y3O_PI));
+}
+
+/* Exponentially-modified Gaussian pdf, the projection tail of
+ * lc_lt_t.hh:59-60 / emg_des_t.cuh:172-193, in the overflow-free form:
+ * exp(A) erfc(B) = exp(-(x-mu)^2 / 2 sigma^2) erfcx(B) for B >= 0. */
+double
+y3o_emg_pdf(double x, double mu, double sigma, double tau)
+{
+  double const B = (mu + tau * sigma * sigma - x) / (Y3O_SQRT2 * sigma);
+  if (B >= 0.0) {
+    double const d = (x - mu) / sigma;
+    return 0.5 * tau * exp(-0.5 * d * d) * y3o_erfcx(B);
+  }
+  double const A = tau * (mu - x) + 0.5 * tau * tau * sigma * sigma;
+  return 0.5 * tau * exp(A) * erfc(B);
+}
+
+/* emg_des_t.cuh:136-168: F(x) = Phi(z) - exp(A) Phi(z - tau sigma). */
+double
+y3o_emg_cdf(double x, double mu, double sigma, double tau)
+{
+  double const z = (x - mu) / sigma;
+  double const phi = 0.5 * erfc(-z / Y3O_SQRT2);
+  double const u = (tau * sigma - z) / Y3O_SQRT2;
+  double tail;
+  if (u >= 0.0)
+    tail = 0.5 * exp(-0.5 * z * z) * y3o_erfcx(u);
+  else
+    tail = 0.5 * exp(-tau * (x - mu) + 0.5 * tau * tau * sigma * sigma) *
+           erfc(u);
+  return phi - tail;
+}
+
+/* ---------------- synthetic-workload functors --------------------------- */
+
+void
+y3o_default_params(y3o_params* p)
+{
+  p->omega_m = 0.3;
+  p->h = 0.7;
+  p->hmf_amp = 0.8;
+  p->hmf_A = 0.2;
+  p->hmf_a = 1.5;
+  p->hmf_c = 1.2;
+  p->mor_lnl0 = 3.2;
+  p->mor_slope = 0.9;
+  p->mor_zslope = 0.5;
+  p->mor_sigma_ln = 0.25;
+  p->f_prj = 0.2;
+  p->prj_scale = 0.1;
+  p->f_mis = 0.2;
+  p->mis_rlambda = 0.2;
+  p->z_source = 1.2;
+  p->sigz0 = 0.01;
+}
+
+/* dn/dlnM = A (sigma^-a + 1) exp(-c / sigma^2),
+ * sigma = amp (M / 1e14)^-0.25 / (1+z)^0.8 : the values the HMF_t table
+ * (hmf_t.hh:34-43, mass_function/dndlnmh) holds for the synthetic runs. */
+double
+y3o_hmf_formula(const y3o_params* p, double lnM, double z)
+{
+  double const sig =
+    p->hmf_amp * exp(-0.25 * (lnM - Y3O_LN_MPIVOT_HMF)) * pow(1.0 + z, -0.8);
+  return p->hmf_A * (pow(sig, -p->hmf_a) + 1.0) *
+         exp(-p->hmf_c / (sig * sig));
+}
+
+/* Lognormal P(lambda_true | M, z), in the MOR slot mor(lt, lnM, zt)
+ * (mor_t.hh:29-39 call signature):
+ * ln lbar = lnl0 + slope ln(M / 3e14) + zslope ln((1+z)/1.35). */
+double
+y3o_mor_lognormal(const y3o_params* p, double lt, double lnM, double zt)
+{
+  double const lnlbar = p->mor_lnl0 +
+                        p->mor_slope * (lnM - Y3O_LN_MPIVOT_MOR) +
+                        p->mor_zslope * log((1.0 + zt) / 1.35);
+  double const s = p->mor_sigma_ln;
+  double const d = (log(lt) - lnlbar) / s;
+  return exp(-0.5 * d * d) * Y3O_INV_SQRT2PI / (s * lt);
+}
+
+/* P(lambda_obs | lambda_true): (1-f_prj) N(lt, sqrt(lt)) + f_prj EMG with
+ * tau = 1 / (prj_scale lt); lc_lt_t.hh:54-67 with fmsk = 0. */
+double
+y3o_lo_lt_pdf(const y3o_params* p, double lo, double lt)
+{
+  double const sigma = sqrt(lt);
+  double g = y3o_gaussian(lo, lt, sigma);
+  if (p->f_prj == 0.0) return g;
+  double const tau = 1.0 / (p->prj_scale * lt);
+  return (1.0 - p->f_prj) * g + p->f_prj * y3o_emg_pdf(lo, lt, sigma, tau);
+}
+
+/* Integral of y3o_lo_lt_pdf over [lo_lo, lo_hi]: erf difference
+ * (int_zo_zt_des_t.hh:14-21 pattern) plus the EMG CDF difference
+ * (emg_des_t.cuh:136-168). */
+double
+y3o_lo_lt_bin(const y3o_params* p, double lo_lo, double lo_hi, double lt)
+{
+  double const sigma = sqrt(lt);
+  double const base = Y3O_SQRT2 * sigma;
+  double const g = 0.5 * (erf((lo_hi - lt) / base) - erf((lo_lo - lt) / base));
+  if (p->f_prj == 0.0) return g;
+  double const tau = 1.0 / (p->prj_scale * lt);
+  double const e =
+    y3o_emg_cdf(lo_hi, lt, sigma, tau) - y3o_emg_cdf(lo_lo, lt, sigma, tau);
+  return (1.0 - p->f_prj) * g + p->f_prj * e;
+}
+
+/* z_obs ~ N(z_true, sigz0 (1 + z_true)): zo_zt_des_t.hh:16-21 form. */
+double
+y3o_zo_pdf(const y3o_params* p, double zo, double zt)
+{
+  return y3o_gaussian(zo, zt, p->sigz0 * (1.0 + zt));
+}
+
+/* int_zo_zt_des_t.hh:14-21 form. */
+double
+y3o_zo_bin(const y3o_params* p, double zo_lo, double zo_hi, double zt)
+{
+  double const base = Y3O_SQRT2 * p->sigz0 * (1.0 + zt);
+  return 0.5 * (erf((zo_hi - zt) / base) - erf((zo_lo - zt) / base));
+}
+
+/* Gauss-Legendre 8-point nodes on [-1, 1] (positive half). */
+static const double gl8_x[4] = {0.1834346424956498, 0.5255324099163290,
+                                0.7966664774136267, 0.9602898564975363};
+static const double gl8_w[4] = {0.3626837833783620, 0.3137066458778873,
+                                0.2223810344533745, 0.1012285362903763};
+
+/* Comoving distance chi(z) in Mpc/h for flat LCDM by composite 8-point
+ * Gauss-Legendre with 64 panels; the device table builder runs the same
+ * sum.  h is unused (Mpc/h units) but kept for the call shape of EZ. */
+double
+y3o_comoving_distance(double om, double h, double z)
+{
+  (void)h;
+  int const np = 64;
+  double const dz = z / np;
+  double s = 0.0;
+  for (int ip = 0; ip < np; ++ip) {
+    double const c = (ip + 0.5) * dz, hw = 0.5 * dz;
+    double ps = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      double const zp = c + hw * gl8_x[k], zm = c - hw * gl8_x[k];
+      ps += gl8_w[k] * (1.0 / y3o_ez(zp, om, 1.0 - om, 0.0) +
+                        1.0 / y3o_ez(zm, om, 1.0 - om, 0.0));
+    }
+    s += ps * hw;
+  }
+  return Y3O_DH * s;
+}
+
+/* c(M) = 5 (M / 1e14)^-0.1 */
+double
+y3o_nfw_conc(double lnM)
+{
+  return 5.0 * exp(-0.1 * (lnM - Y3O_LN_MPIVOT_HMF));
+}
+
+/* F(x) of the projected NFW profile and h(x) = (1 - F) / (x^2 - 1), both
+ * regular at x = 1 through the series in t = x^2 - 1:
+ * F = sum_k (-t)^k / (2k+1). */
+static void
+nfw_F_h(double x, double* F, double* h)
+{
+  double const t = x * x - 1.0;
+  if (fabs(t) < 0.1) {
+    /* h = 1/3 - t/5 + t^2/7 - ... */
+    double hs = 0.0, tp = 1.0;
+    for (int k = 1; k < 40; ++k) {
+      double const term = tp / (2.0 * k + 1.0);
+      hs += (k & 1) ? term : -term;
+      tp *= t;
+      if (fabs(tp) < 1e-18) break;
+    }
+    *h = hs;
+    *F = 1.0 - t * hs;
+    return;
+  }
+  if (t < 0.0) {
+    double const s = sqrt(-t);
+    *F = log((1.0 + s) / x) / s; /* atanh(s) / s */
+  } else {
+    double const s = sqrt(t);
+    *F = atan(s) / s;
+  }
+  *h = (1.0 - *F) / t;
+}
+
+/* Sigma / (rho_s r_s) = 2 h(x) (Wright & Brainerd 2000). */
+double
+y3o_nfw_sigma_unit(double x)
+{
+  double F, h;
+  nfw_F_h(x, &F, &h);
+  return 2.0 * h;
+}
+
+/* Delta-Sigma / (rho_s r_s) = 4 m(x) - 2 h(x), m = (ln(x/2) + F) / x^2,
+ * with the cancellation-free form of m for x < 1. */
+double
+y3o_nfw_g(double x)
+{
+  double F, h, m;
+  nfw_F_h(x, &F, &h);
+  if (x < 0.9) {
+    double const s = sqrt(1.0 - x * x);
+    double const q = x * x / (2.0 * (1.0 + s));
+    double const phi = -log1p(-q) / q;
+    m = (log(2.0 / x) / (1.0 + s) - 0.5 * phi / (1.0 + s)) / s;
+  } else {
+    m = (log(0.5 * x) + F) / (x * x);
+  }
+  return 4.0 * m - 2.0 * h;
+}
+
+/* Centred NFW Delta-Sigma [h Msun / pc^2] with Delta = 200 rho:
+ * r200 = cbrt(3M / (800 pi rho)), r_s = r200 / c,
+ * delta_c = (200/3) c^3 / (ln(1+c) - c/(1+c))  (nfw_dsigma_mis.hh:95-114). */
+double
+y3o_nfw_dsigma(double R, double lnM, double rho, double conc)
+{
+  double const c = conc > 0.0 ? conc : y3o_nfw_conc(lnM);
+  double const r200 = cbrt(3.0 * exp(lnM) / (800.0 * Y3O_PI * rho));
+  double const rs = r200 / c;
+  double const dc = (200.0 / 3.0) * c * c * c / (log(1.0 + c) - c / (1.0 + c));
+  return 1e-12 * dc * rho * rs * y3o_nfw_g(R / rs);
+}
+
+/* Tangential Delta-Sigma about a centre offset by rmis, at polar angle theta:
+ * R' = sqrt(R^2 + rmis^2 + 2 R rmis cos(theta))
+ * (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:176-178) and the spin-2
+ * projection cos(2 beta), cos(beta) = (R + rmis cos(theta)) / R'. */
+double
+y3o_dsigma_offset(double R, double rmis, double theta, double lnM, double rho,
+                  double conc)
+{
+  double const ct = cos(theta);
+  double const rp2 = R * R + rmis * rmis + 2.0 * R * rmis * ct;
+  if (rp2 <= 1e-300) return 0.0;
+  double const a = R + rmis * ct;
+  double const c2b = 2.0 * a * a / rp2 - 1.0;
+  return y3o_nfw_dsigma(sqrt(rp2), lnM, rho, conc) * c2b;
+}
+
+/* ---------------- tables ------------------------------------------------ */
+
+void
+y3o_spec_config0(y3o_table_spec* s)
+{
+  s->lnm_lo = 29.933606208922594; /* ln 1e13     */
+  s->lnm_hi = 35.690174623927322; /* 15.5 ln 10  */
+  s->n_lnm = 200;
+  s->z_lo = 0.20;
+  s->z_hi = 0.35;
+  s->n_z = 32;
+  s->da_zmax = 1.0;
+  s->n_da = 512;
+  s->sci_zmax = 1.0;
+  s->n_sci = 128;
+}
+
+void
+y3o_spec_wide(y3o_table_spec* s)
+{
+  y3o_spec_config0(s);
+  s->z_lo = 0.10;
+  s->z_hi = 0.75;
+}
+
+static double
+lin_knot(double a, double b, int n, int i)
+{
+  /* identical on host and device: a + i * ((b - a) / (n - 1)) */
+  return a + (double)i * ((b - a) / (double)(n - 1));
+}
+
+int
+y3o_tables_build(const y3o_params* p, const y3o_table_spec* s, y3o_tables* t)
+{
+  memset(t, 0, sizeof(*t));
+  t->spec = *s;
+  t->lnm = (double*)malloc(sizeof(double) * s->n_lnm);
+  t->z = (double*)malloc(sizeof(double) * s->n_z);
+  t->hmf = (double*)malloc(sizeof(double) * s->n_lnm * s->n_z);
+  t->da_z = (double*)malloc(sizeof(double) * s->n_da);
+  t->da = (double*)malloc(sizeof(double) * s->n_da);
+  t->sci_z = (double*)malloc(sizeof(double) * s->n_sci);
+  t->sci = (double*)malloc(sizeof(double) * s->n_sci);
+  if (!t->lnm || !t->z || !t->hmf || !t->da_z || !t->da || !t->sci_z ||
+      !t->sci)
+    return 1;
+  for (int i = 0; i < s->n_lnm; ++i)
+    t->lnm[i] = lin_knot(s->lnm_lo, s->lnm_hi, s->n_lnm, i);
+  for (int j = 0; j < s->n_z; ++j)
+    t->z[j] = lin_knot(s->z_lo, s->z_hi, s->n_z, j);
+  for (int j = 0; j < s->n_z; ++j)
+    for (int i = 0; i < s->n_lnm; ++i)
+      t->hmf[j * s->n_lnm + i] = y3o_hmf_formula(p, t->lnm[i], t->z[j]);
+  for (int i = 0; i < s->n_da; ++i) {
+    double const z = lin_knot(0.0, s->da_zmax, s->n_da, i);
+    t->da_z[i] = z;
+    /* D_A in Mpc: chi [Mpc/h] / h / (1+z) */
+    t->da[i] = y3o_comoving_distance(p->omega_m, p->h, z) / p->h / (1.0 + z);
+  }
+  double const chis = y3o_comoving_distance(p->omega_m, p->h, p->z_source);
+  for (int i = 0; i < s->n_sci; ++i) {
+    double const z = lin_knot(0.0, s->sci_zmax, s->n_sci, i);
+    double const chil = y3o_comoving_distance(p->omega_m, p->h, z);
+    t->sci_z[i] = z;
+    t->sci[i] = chil < chis ? Y3O_SCI_CONST * (1.0 + z) * chil *
+                                (chis - chil) / chis
+                            : 0.0;
+  }
+  return 0;
+}
+
+void
+y3o_tables_free(y3o_tables* t)
+{
+  free(t->lnm);
+  free(t->z);
+  free(t->hmf);
+  free(t->da_z);
+  free(t->da);
+  free(t->sci_z);
+  free(t->sci);
+  memset(t, 0, sizeof(*t));
+}
+
+/* HMF_t::operator() (hmf_t.hh:45-50) with s = 0, q = 1: the clamped
+ * bilinear table lookup. */
+double
+y3o_hmf(const y3o_tables* t, double lnM, double zt)
+{
+  return y3o_interp2d_clamp(t->lnm, t->spec.n_lnm, t->z, t->spec.n_z, t->hmf,
+                            lnM, zt);
+}
+
+double
+y3o_dvdz(const y3o_params* p, const y3o_tables* t, double zt)
+{
+  return y3o_dv_do_dz(zt, t->da_z, t->da, t->spec.n_da, p->omega_m,
+                      1.0 - p->omega_m, 0.0, p->h);
+}
+
+double
+y3o_sci(const y3o_tables* t, double zt)
+{
+  return y3o_interp1d_clamp(t->sci_z, t->sci, t->spec.n_sci, zt);
+}
+
+/* ---------------- integrands -------------------------------------------- */
+
+/* Argument orders follow the reference integrands:
+ *   COUNT3D  (lt, zt, lnM)                 cf. NC_cent_y1_mort_cuda_scalarintegrand.cu:164-175
+ *   COUNT5D  (lo, zo, lt, zt, lnM)         cf. numberCounts.cu:159-174
+ *   DSIGMA6D (lo, lt, zt, lnM, rmis, theta) cf. sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182
+ */
+double
+y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
+              const double* x)
+{
+  switch (ig->kind) {
+    case Y3O_COUNT3D: {
+      double const lt = x[0], zt = x[1], lnM = x[2];
+      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+             y3o_lo_lt_bin(p, ig->lo_lo, ig->lo_hi, lt) * y3o_dvdz(p, t, zt);
+    }
+    case Y3O_COUNT5D: {
+      double const lo = x[0], zo = x[1], lt = x[2], zt = x[3], lnM = x[4];
+      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+             y3o_lo_lt_pdf(p, lo, lt) * y3o_zo_pdf(p, zo, zt) *
+             y3o_dvdz(p, t, zt);
+    }
+    case Y3O_DSIGMA6D: {
+      double const lo = x[0], lt = x[1], zt = x[2], lnM = x[3], rmis = x[4],
+                   th = x[5];
+      double const rho = p->omega_m * Y3O_RHOC;
+      double const s2 = ig->sigma_mis * ig->sigma_mis;
+      double const pr =
+        rmis / s2 * exp(-0.5 * rmis * rmis / s2) * ig->rmis_norm;
+      double const prof =
+        (1.0 - p->f_mis) * y3o_nfw_dsigma(ig->radius, lnM, rho, 0.0) +
+        p->f_mis * y3o_dsigma_offset(ig->radius, rmis, th, lnM, rho, 0.0);
+      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+             y3o_lo_lt_pdf(p, lo, lt) *
+             y3o_zo_bin(p, ig->zo_lo, ig->zo_hi, zt) * y3o_dvdz(p, t, zt) *
+             y3o_sci(t, zt) * pr / (2.0 * Y3O_PI) * prof;
+    }
+  }
+  return 0.0;
+}
+
+/* Integration variables.  The integrators see the box of y; the MOR
+ * variable is integrated in its standardised form
+ *   t = (ln lt - ln lbar(lnM, zt)) / sigma_ln,  lt = lbar e^{sigma_ln t},
+ *   MOR(lt) dlt = phi(t) dt,
+ * over t in [-Y3O_T_MAX, Y3O_T_MAX] with the lambda_true range [1, 300]
+ * kept as an indicator, so the integral is the same one over lt in [1, 300]
+ * (the truncated Gaussian mass beyond |t| = 7 is 2.6e-12).  Axis order
+ * follows the physical integrands with lt replaced by t. */
+#define Y3O_T_MAX 7.0
+
+double
+y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
+                     const y3o_integral* ig, const double* y)
+{
+  double x[8];
+  int it, iz, im;
+  switch (ig->kind) {
+    case Y3O_COUNT3D: it = 0, iz = 1, im = 2; break;
+    case Y3O_COUNT5D: it = 2, iz = 3, im = 4; break;
+    default: it = 1, iz = 2, im = 3; break;
+  }
+  for (int k = 0; k < ig->ndim; ++k) x[k] = y[k];
+  double const zt = y[iz], lnM = y[im], tt = y[it];
+  double const s = p->mor_sigma_ln;
+  double const lnlbar = p->mor_lnl0 + p->mor_slope * (lnM - Y3O_LN_MPIVOT_MOR) +
+                        p->mor_zslope * log((1.0 + zt) / 1.35);
+  double const lt = exp(lnlbar + s * tt);
+  if (lt < ig->lt_lo || lt > ig->lt_hi) return 0.0;
+  x[it] = lt;
+  /* integrand(lt) * dlt/dt, with MOR(lt) * lt * s = phi(t) */
+  return y3o_integrand(p, t, ig, x) * lt * s;
+}
+
+#define Y3O_ZT_MARGIN 0.1
+#define Y3O_RMIS_NSIGMA 6.0
+
+void
+y3o_make_integral(const y3o_params* p, const y3o_table_spec* s, int kind,
+                  double lo_lo, double lo_hi, double zo_lo, double zo_hi,
+                  double radius, y3o_integral* ig)
+{
+  memset(ig, 0, sizeof(*ig));
+  ig->kind = kind;
+  ig->lo_lo = lo_lo;
+  ig->lo_hi = lo_hi;
+  ig->zo_lo = zo_lo;
+  ig->zo_hi = zo_hi;
+  ig->radius = radius;
+  ig->lt_lo = 1.0;
+  ig->lt_hi = 300.0;
+  double const zt_lo = kind == Y3O_COUNT3D ? s->z_lo : zo_lo - Y3O_ZT_MARGIN;
+  double const zt_hi = kind == Y3O_COUNT3D ? s->z_hi : zo_hi + Y3O_ZT_MARGIN;
+  switch (kind) {
+    case Y3O_COUNT3D:
+      ig->ndim = 3;
+      ig->lows[0] = -Y3O_T_MAX, ig->highs[0] = Y3O_T_MAX;
+      ig->lows[1] = zt_lo, ig->highs[1] = zt_hi;
+      ig->lows[2] = s->lnm_lo, ig->highs[2] = s->lnm_hi;
+      break;
+    case Y3O_COUNT5D:
+      ig->ndim = 5;
+      ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
+      ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
+      ig->lows[2] = -Y3O_T_MAX, ig->highs[2] = Y3O_T_MAX;
+      ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
+      ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;
+      break;
+    case Y3O_DSIGMA6D: {
+      ig->ndim = 6;
+      /* R_lambda from the lambda_obs bin centre (gamma_1h_nfw.cuh:100-107,
+       * lensing_weights.hh:172). */
+      double const rl = pow(0.5 * (lo_lo + lo_hi) / 100.0, 0.2);
+      ig->sigma_mis = p->mis_rlambda * rl;
+      double const rmax = Y3O_RMIS_NSIGMA * ig->sigma_mis;
+      ig->rmis_norm =
+        1.0 / (1.0 - exp(-0.5 * Y3O_RMIS_NSIGMA * Y3O_RMIS_NSIGMA));
+      ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
+      ig->lows[1] = -Y3O_T_MAX, ig->highs[1] = Y3O_T_MAX;
+      ig->lows[2] = zt_lo, ig->highs[2] = zt_hi;
+      ig->lows[3] = s->lnm_lo, ig->highs[3] = s->lnm_hi;
+      ig->lows[4] = 0.0, ig->highs[4] = rmax;
+      ig->lows[5] = 0.0, ig->highs[5] = 2.0 * Y3O_PI;
+      break;
+    }
+  }
+}
+
+/* Batch evaluation of the mapped integrand at n points y[n][ndim]. */
+void
+y3o_integrand_mapped_batch(const y3o_params* p, const y3o_tables* t,
+                           const y3o_integral* ig, const double* y, long n,
+                           double* out)
+{
+  for (long i = 0; i < n; ++i)
+    out[i] = y3o_integrand_mapped(p, t, ig, y + i * ig->ndim);
+}

@@ -1,0 +1,257 @@
+"""MI355X (gfx950) cluster-observable integration: Python binding of liby3g.
+
+The product is the HIP library ``liby3g.so`` (C ABI in ``include/y3g.h``)
+and the CosmoSIS modules built on it.  This module is a ctypes binding used
+by tests, the benchmark and Python callers; it loads the in-tree library and
+raises if it is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liby3g.so")
+
+COUNT3D, COUNT5D, DSIGMA6D = 0, 1, 2
+PAGANI, MCUBES = 0, 1
+FP32SPLIT, FP64 = 0, 1
+
+(F_ROFFSET_GAMMA, F_LO_LC, F_ZO_ZT_DES, F_INT_ZO_ZT_DES, F_SIGMA_PHOTOZ,
+ F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
+ F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND) = range(14)
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "omega_m", "h", "hmf_amp", "hmf_A", "hmf_a", "hmf_c", "mor_lnl0",
+        "mor_slope", "mor_zslope", "mor_sigma_ln", "f_prj", "prj_scale",
+        "f_mis", "mis_rlambda", "z_source", "sigz0")]
+
+
+class TableSpec(C.Structure):
+    _fields_ = [("lnm_lo", C.c_double), ("lnm_hi", C.c_double),
+                ("n_lnm", C.c_int), ("z_lo", C.c_double),
+                ("z_hi", C.c_double), ("n_z", C.c_int),
+                ("da_zmax", C.c_double), ("n_da", C.c_int),
+                ("sci_zmax", C.c_double), ("n_sci", C.c_int)]
+
+
+class Integral(C.Structure):
+    _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("point", C.c_int),
+                ("algorithm", C.c_int),
+                ("lo_lo", C.c_double), ("lo_hi", C.c_double),
+                ("zo_lo", C.c_double), ("zo_hi", C.c_double),
+                ("radius", C.c_double), ("sigma_mis", C.c_double),
+                ("rmis_norm", C.c_double), ("lt_lo", C.c_double),
+                ("lt_hi", C.c_double),
+                ("lows", C.c_double * 8), ("highs", C.c_double * 8),
+                ("eps_rel", C.c_double), ("eps_abs", C.c_double),
+                ("max_eval", C.c_longlong), ("seed", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("value", C.c_double), ("error", C.c_double),
+                ("prob", C.c_double), ("neval", C.c_longlong),
+                ("nregions", C.c_int), ("status", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_regions", C.c_int),
+                ("max_split_per_step", C.c_int), ("init_split", C.c_int),
+                ("block_threads", C.c_int), ("slots", C.c_int),
+                ("vegas_calls", C.c_longlong), ("vegas_itmax", C.c_int),
+                ("vegas_itskip", C.c_int), ("vegas_nbins", C.c_int),
+                ("vegas_alpha", C.c_double), ("single", C.c_int)]
+
+
+RESULT_DTYPE = np.dtype([("value", "f8"), ("error", "f8"), ("prob", "f8"),
+                         ("neval", "i8"), ("nregions", "i4"), ("status", "i4")])
+assert RESULT_DTYPE.itemsize == C.sizeof(Result)
+
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+
+# exported symbols and signatures of include/y3g.h
+SIGNATURES = {
+    "y3g_last_error": (C.c_char_p, []),
+    "y3g_version": (C.c_char_p, []),
+    "y3g_runtime_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "y3g_runtime_destroy": (C.c_int, [C.c_void_p]),
+    "y3g_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _PD,
+                                  C.POINTER(C.c_size_t), C.c_char_p, C.c_int]),
+    "y3g_options_default": (None, [C.POINTER(Options)]),
+    "y3g_default_params": (None, [C.POINTER(Params)]),
+    "y3g_spec_config0": (None, [C.POINTER(TableSpec)]),
+    "y3g_spec_wide": (None, [C.POINTER(TableSpec)]),
+    "y3g_tables_build": (C.c_int, [C.c_void_p, C.POINTER(Params),
+                                   C.POINTER(TableSpec), C.c_int]),
+    "y3g_tables_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, _PD, _PD]),
+    "y3g_tables_download": (C.c_int, [C.c_void_p, C.c_int, _PD, _PD, _PD]),
+    "y3g_make_integral": (None, [C.POINTER(Params), C.POINTER(TableSpec), C.c_int,
+                                 _D, _D, _D, _D, _D, C.POINTER(Integral)]),
+    "y3g_integrate": (C.c_int, [C.c_void_p, C.POINTER(Integral), C.c_size_t,
+                                C.POINTER(Options), C.c_void_p]),
+    "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
+    "y3g_eval_functor": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _PD, C.c_int,
+                                   C.c_size_t, _PD, C.c_int,
+                                   C.POINTER(Integral), _PD]),
+}
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"liby3g.so not found at {path}: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def last_error():
+    return lib().y3g_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def default_params(**kw):
+    p = Params()
+    lib().y3g_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def spec_config0():
+    s = TableSpec()
+    lib().y3g_spec_config0(C.byref(s))
+    return s
+
+
+def spec_wide():
+    s = TableSpec()
+    lib().y3g_spec_wide(C.byref(s))
+    return s
+
+
+def options(**kw):
+    o = Options()
+    lib().y3g_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def make_integral(params, spec, kind, lo_lo=20.0, lo_hi=30.0, zo_lo=0.2,
+                  zo_hi=0.35, radius=1.0, point=0, eps_rel=1e-4,
+                  algorithm=PAGANI, max_eval=1 << 40, seed=0x59334331):
+    ig = Integral()
+    lib().y3g_make_integral(C.byref(params), C.byref(spec), kind, lo_lo, lo_hi,
+                            zo_lo, zo_hi, radius, C.byref(ig))
+    ig.point = point
+    ig.eps_rel = eps_rel
+    ig.algorithm = algorithm
+    ig.max_eval = max_eval
+    ig.seed = seed
+    return ig
+
+
+class Runtime:
+    """One device binding (y3g_runtime)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        _check(lib().y3g_runtime_create(device, C.byref(self._h)), "y3g_runtime_create")
+
+    def close(self):
+        if self._h:
+            lib().y3g_runtime_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        n = C.c_int()
+        clk = C.c_double()
+        hbm = C.c_size_t()
+        name = C.create_string_buffer(64)
+        _check(lib().y3g_device_info(self._h, C.byref(n), C.byref(clk), C.byref(hbm),
+                                     name, 64), "y3g_device_info")
+        return dict(n_cu=n.value, clock_ghz=clk.value, hbm_bytes=hbm.value,
+                    name=name.value.decode())
+
+    def tables_build(self, params, spec):
+        if isinstance(params, Params):
+            params = [params]
+        arr = (Params * len(params))(*params)
+        _check(lib().y3g_tables_build(self._h, arr, C.byref(spec), len(params)),
+               "y3g_tables_build")
+
+    def tables_upload(self, point, hmf=None, da=None, sci=None):
+        def p(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            return a.ctypes.data_as(_PD)
+        _check(lib().y3g_tables_upload(self._h, point, p(hmf), p(da), p(sci)),
+               "y3g_tables_upload")
+
+    def tables_download(self, point, spec):
+        hmf = np.zeros(spec.n_lnm * spec.n_z)
+        da = np.zeros(spec.n_da)
+        sci = np.zeros(spec.n_sci)
+        _check(lib().y3g_tables_download(self._h, point, hmf.ctypes.data_as(_PD),
+                                         da.ctypes.data_as(_PD), sci.ctypes.data_as(_PD)),
+               "y3g_tables_download")
+        return hmf, da, sci
+
+    def integrate(self, integrals, opts=None):
+        n = len(integrals)
+        arr = integrals if isinstance(integrals, C.Array) else (Integral * n)(*integrals)
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        o = opts if opts is not None else options()
+        _check(lib().y3g_integrate(self._h, arr, n, C.byref(o),
+                                   out.ctypes.data_as(C.c_void_p)), "y3g_integrate")
+        return out
+
+    def last_timing(self):
+        ms = C.c_double()
+        nev = C.c_longlong()
+        _check(lib().y3g_last_timing(self._h, C.byref(ms), C.byref(nev)), "y3g_last_timing")
+        return ms.value, nev.value
+
+    def eval_functor(self, which, x, par=(), precision=FP64, aux=None):
+        x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))
+        if x.shape[0] == 1 and x.shape[1] > 1 and which not in (F_INTEGRAND,):
+            pass
+        n, nin = x.shape
+        par = np.ascontiguousarray(np.asarray(par, dtype=np.float64).reshape(-1))
+        if par.size == 0:
+            par = np.zeros(1)
+        out = np.zeros(n)
+        _check(lib().y3g_eval_functor(self._h, which, precision, x.ctypes.data_as(_PD), nin, n,
+                                      par.ctypes.data_as(_PD), par.size,
+                                      C.byref(aux) if aux is not None else None,
+                                      out.ctypes.data_as(_PD)), "y3g_eval_functor")
+        return out

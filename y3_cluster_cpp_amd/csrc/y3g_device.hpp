@@ -1,0 +1,483 @@
+// y3g_device.hpp -- device model functors and the three hot-path integrands
+// for gfx950, templated on the compute precision T (float for the FP32
+// integrand of the split path, double for the plain FP64 path).
+//
+// Functor call signatures follow the reference functors:
+//   hmf(lnM, zt)                   src/models/hmf_t.cuh:54-60
+//   mor(lt, lnM, zt)               src/models/mor_t.cuh:70-81 (lognormal form)
+//   lc_lt(lc, lt, zt)              src/models/lc_lt_t.cuh:36-68 (fmsk = 0)
+//   lo_lc(lo, lc, R)               src/models/lo_lc_t.cuh:20-30
+//   int_zo_zt(zomin, zomax, zt)    src/models/int_zo_zt_des_t.cuh:11-17
+//   dv_do_dz(zt)                   src/models/dv_do_dz_t.cuh:19-27
+//   roffset(x)                     src/models/roffset_t.cuh:15-21
+//   nfw_dsigma_mis(r, rmis, lnM)   src/models/nfw_dsigma_mis.cuh:104-123
+// Integrand bodies: NC_cent_y1_mort_cuda_scalarintegrand.cu:164-175 (3-D),
+// numberCounts.cu:159-174 (5-D), sigma_miscent_y1_mort_cuda_scalarintegrand.cu
+// :161-182 (6-D).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "y3g.h"
+
+namespace y3g {
+
+constexpr int kMaxDim = 6;
+constexpr int kMaxHmf = 200 * 32;
+constexpr int kMaxDa = 512;
+constexpr int kMaxSci = 128;
+constexpr double kLnPivotHmf = 32.236191301916641;  // ln 1e14
+constexpr double kLnPivotMor = 33.334803590584751;  // ln 3e14
+constexpr double kRhoCrit = 2.77533742639e+11;      // h^2 Msun / Mpc^3
+constexpr double kDvConst = 2997.92;                // dv_do_dz_t.cuh:25
+constexpr double kDh = 2997.92458;                  // c / H0 [Mpc/h]
+constexpr double kSciConst = 6.0134795e-7;          // 4 pi G/c^2 * 1e6 pc/Mpc
+constexpr double kTMax = 7.0;
+
+// Per-parameter-point constants (host-prepared).
+struct PointDev {
+  y3g_params p;
+  double lnm0, dlnm, inv_dlnm;
+  double z0, dz, inv_dz;
+  double inv_dzda, inv_dzsci;
+  double rho_m;
+  int nm, nz, nda, nsci;
+};
+
+// ---------------------------------------------------------------- math
+template <typename T> struct M;
+template <> struct M<float> {
+  static __device__ __forceinline__ float exp(float x) { return expf(x); }
+  static __device__ __forceinline__ float log(float x) { return logf(x); }
+  static __device__ __forceinline__ float log1p(float x) { return log1pf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float erf(float x) { return erff(x); }
+  static __device__ __forceinline__ float erfc(float x) { return erfcf(x); }
+  static __device__ __forceinline__ float erfcx(float x) { return erfcxf(x); }
+  static __device__ __forceinline__ float cos(float x) { return cosf(x); }
+  static __device__ __forceinline__ float atan(float x) { return atanf(x); }
+  static __device__ __forceinline__ float cbrt(float x) { return cbrtf(x); }
+  static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
+  static __device__ __forceinline__ float pow(float x, float y) { return powf(x, y); }
+};
+template <> struct M<double> {
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double log(double x) { return ::log(x); }
+  static __device__ __forceinline__ double log1p(double x) { return ::log1p(x); }
+  static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double erf(double x) { return ::erf(x); }
+  static __device__ __forceinline__ double erfc(double x) { return ::erfc(x); }
+  static __device__ __forceinline__ double erfcx(double x) { return ::erfcx(x); }
+  static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+  static __device__ __forceinline__ double atan(double x) { return ::atan(x); }
+  static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
+  static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
+  static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+};
+
+#define Y3G_HD __device__ __forceinline__
+
+// Tables of one point as seen by an integrand evaluation (LDS or global).
+template <typename T> struct Tabs {
+  const T* hmf;
+  const T* da;
+  const T* sci;
+};
+
+// ------------------------------------------------------------ functors
+// primitives.hh:60-65
+template <typename T> Y3G_HD T gaussian(T x, T mu, T sigma)
+{
+  T const z = (x - mu) / sigma;
+  return M<T>::exp(T(-0.5) * z * z) * T(0.3989422804014327) / sigma;
+}
+
+// roffset_t.cuh:15-21 (Gamma k = 2 kernel of the reference).
+template <typename T> Y3G_HD T roffset_gamma(T x, T tau)
+{
+  return x / tau / tau * M<T>::exp(-x / tau);
+}
+
+// lo_lc_t.cuh:20-30
+template <typename T>
+Y3G_HD T lo_lc(T lo, T lc, T R, T alpha, T a, T b, T R_lambda)
+{
+  T const x = R / R_lambda;
+  T const y = lo / lc;
+  T const mu_y = M<T>::exp(-x * x / alpha / alpha);
+  T const sigma_y = a * M<T>::atan(b * x);
+  return gaussian<T>(y, mu_y, sigma_y) / lc;
+}
+
+// sigma_photoz_des.cuh:7-36: Horner in (clamp(z, 0.15, 0.7) - 0.4).
+template <typename T> Y3G_HD T sigma_photoz_des(T zt)
+{
+  const T c[11] = {T(-40358.8315),  T(2798.08304),   T(9333.80185),
+                   T(-657.348248),  T(-840.565610),  T(46.8506649),
+                   T(37.8839498),   T(-0.868811858), T(-0.808928182),
+                   T(0.00890199353), T(0.0139811265)};
+  T z = zt < T(0.15) ? T(0.15) : (zt > T(0.7) ? T(0.7) : zt);
+  z -= T(0.4);
+  T s = T(0);
+  for (int i = 0; i < 10; ++i) s = (c[i] + s) * z;
+  return s + c[10];
+}
+
+// Half the difference erf(b) - erf(a), b >= a, in the cancellation-free
+// form (complementary functions on the tails).
+template <typename T> Y3G_HD T half_erf_diff(T a, T b)
+{
+  if (a > T(0)) return T(0.5) * (M<T>::erfc(a) - M<T>::erfc(b));
+  if (b < T(0)) return T(0.5) * (M<T>::erfc(-b) - M<T>::erfc(-a));
+  return T(0.5) * (M<T>::erf(b) - M<T>::erf(a));
+}
+
+// int_zo_zt_des_t.cuh:11-17 with the DES photo-z polynomial.
+template <typename T> Y3G_HD T int_zo_zt_des(T zomin, T zomax, T zt)
+{
+  T const base = T(1.4142135623730951) * sigma_photoz_des<T>(zt);
+  return half_erf_diff<T>((zomin - zt) / base, (zomax - zt) / base);
+}
+
+// z_obs ~ N(zt, sigz0 (1+zt)): pdf and bin integral.
+template <typename T> Y3G_HD T zo_pdf(T zo, T zt, T sigz0)
+{
+  return gaussian<T>(zo, zt, sigz0 * (T(1) + zt));
+}
+template <typename T> Y3G_HD T zo_bin(T zlo, T zhi, T zt, T sigz0)
+{
+  T const base = T(1.4142135623730951) * sigz0 * (T(1) + zt);
+  return half_erf_diff<T>((zlo - zt) / base, (zhi - zt) / base);
+}
+
+// Exponentially-modified Gaussian pdf (lc_lt_t.cuh:56-61 second term with
+// fmsk = 0; emg_des_t.cuh:172-193), overflow-free through erfcx.
+template <typename T> Y3G_HD T emg_pdf(T x, T mu, T sigma, T tau)
+{
+  T const B = (mu + tau * sigma * sigma - x) / (T(1.4142135623730951) * sigma);
+  if (B >= T(0)) {
+    T const d = (x - mu) / sigma;
+    return T(0.5) * tau * M<T>::exp(T(-0.5) * d * d) * M<T>::erfcx(B);
+  }
+  T const A = tau * (mu - x) + T(0.5) * tau * tau * sigma * sigma;
+  return T(0.5) * tau * M<T>::exp(A) * M<T>::erfc(B);
+}
+
+// EMG CDF (emg_des_t.cuh:136-168).
+template <typename T> Y3G_HD T emg_cdf(T x, T mu, T sigma, T tau)
+{
+  T const z = (x - mu) / sigma;
+  T const phi = T(0.5) * M<T>::erfc(-z / T(1.4142135623730951));
+  T const u = (tau * sigma - z) / T(1.4142135623730951);
+  T tail;
+  if (u >= T(0))
+    tail = T(0.5) * M<T>::exp(T(-0.5) * z * z) * M<T>::erfcx(u);
+  else
+    tail = T(0.5) *
+           M<T>::exp(-tau * (x - mu) + T(0.5) * tau * tau * sigma * sigma) *
+           M<T>::erfc(u);
+  return phi - tail;
+}
+
+// P(lo | lt) = (1 - f_prj) N(lt, sqrt lt) + f_prj EMG(lt, sqrt lt, 1/(s lt)).
+template <typename T> Y3G_HD T lo_lt_pdf(T lo, T lt, T fprj, T pscale)
+{
+  T const sigma = M<T>::sqrt(lt);
+  T const g = gaussian<T>(lo, lt, sigma);
+  if (fprj == T(0)) return g;
+  T const tau = T(1) / (pscale * lt);
+  return (T(1) - fprj) * g + fprj * emg_pdf<T>(lo, lt, sigma, tau);
+}
+
+// Integral of lo_lt_pdf over [lo_lo, lo_hi].
+template <typename T> Y3G_HD T lo_lt_bin(T lo_lo, T lo_hi, T lt, T fprj, T pscale)
+{
+  T const sigma = M<T>::sqrt(lt);
+  T const base = T(1.4142135623730951) * sigma;
+  T const g = half_erf_diff<T>((lo_lo - lt) / base, (lo_hi - lt) / base);
+  if (fprj == T(0)) return g;
+  T const tau = T(1) / (pscale * lt);
+  T const e = emg_cdf<T>(lo_hi, lt, sigma, tau) - emg_cdf<T>(lo_lo, lt, sigma, tau);
+  return (T(1) - fprj) * g + fprj * e;
+}
+
+// Uniform-grid linear interpolation with clamp (interp_1d.hh clamp()).
+template <typename T>
+Y3G_HD T lerp_clamp(const T* ys, int n, double x_off, double inv_dx)
+{
+  double u = x_off * inv_dx;
+  u = u < 0.0 ? 0.0 : (u > double(n - 1) ? double(n - 1) : u);
+  int i = int(u);
+  if (i > n - 2) i = n - 2;
+  T const t = T(u - double(i));
+  return ys[i] + t * (ys[i + 1] - ys[i]);
+}
+
+// HMF_t::operator() (hmf_t.cuh:54-60) with s = 0, q = 1: clamped bilinear
+// lookup of the (n_z, n_lnM) table, x-fastest.
+template <typename T>
+Y3G_HD T hmf(const PointDev& P, const T* tab, double lnM, double zt)
+{
+  double u = (lnM - P.lnm0) * P.inv_dlnm;
+  double v = (zt - P.z0) * P.inv_dz;
+  u = u < 0.0 ? 0.0 : (u > double(P.nm - 1) ? double(P.nm - 1) : u);
+  v = v < 0.0 ? 0.0 : (v > double(P.nz - 1) ? double(P.nz - 1) : v);
+  int i = int(u), j = int(v);
+  if (i > P.nm - 2) i = P.nm - 2;
+  if (j > P.nz - 2) j = P.nz - 2;
+  T const t = T(u - double(i)), s = T(v - double(j));
+  const T* r0 = tab + j * P.nm + i;
+  const T* r1 = r0 + P.nm;
+  T const a = r0[0] + t * (r0[1] - r0[0]);
+  T const b = r1[0] + t * (r1[1] - r1[0]);
+  return a + s * (b - a);
+}
+
+// DV_DO_DZ_t::operator() (dv_do_dz_t.cuh:19-27).
+template <typename T>
+Y3G_HD T dv_do_dz(const PointDev& P, const T* da, double zt)
+{
+  T const d = lerp_clamp<T>(da, P.nda, zt, P.inv_dzda);
+  T const zp = T(1) + T(zt);
+  T const om = T(P.p.omega_m);
+  T const ez = M<T>::sqrt(om * zp * zp * zp + (T(1) - om));
+  T const h = T(P.p.h);
+  return T(kDvConst) * zp * zp * d * h * d * h / ez;
+}
+
+// NFW: F and h(x) = (1 - F) / (x^2 - 1), series in t = x^2 - 1 near x = 1.
+template <typename T> Y3G_HD void nfw_F_h(T x, T& F, T& h)
+{
+  T const t = x * x - T(1);
+  if (M<T>::fabs(t) < T(0.1)) {
+    // h = sum_k (-t)^(k-1) / (2k+1), k >= 1
+    T hs = T(0), tp = T(1);
+    constexpr int nt = sizeof(T) == 4 ? 9 : 18;
+#pragma unroll
+    for (int k = 1; k <= nt; ++k) {
+      T const term = tp / T(2 * k + 1);
+      hs += (k & 1) ? term : -term;
+      tp *= t;
+    }
+    h = hs;
+    F = T(1) - t * hs;
+    return;
+  }
+  if (t < T(0)) {
+    T const s = M<T>::sqrt(-t);
+    F = M<T>::log((T(1) + s) / x) / s;
+  } else {
+    T const s = M<T>::sqrt(t);
+    F = M<T>::atan(s) / s;
+  }
+  h = (T(1) - F) / t;
+}
+
+// Delta-Sigma / (rho_s r_s) of NFW (Wright & Brainerd 2000).
+template <typename T> Y3G_HD T nfw_g(T x)
+{
+  T F, h, m;
+  nfw_F_h<T>(x, F, h);
+  if (x < T(0.9)) {
+    T const s = M<T>::sqrt(T(1) - x * x);
+    T const q = x * x / (T(2) * (T(1) + s));
+    T const phi = -M<T>::log1p(-q) / q;
+    m = (M<T>::log(T(2) / x) / (T(1) + s) - T(0.5) * phi / (T(1) + s)) / s;
+  } else {
+    m = (M<T>::log(T(0.5) * x) + F) / (x * x);
+  }
+  return T(4) * m - T(2) * h;
+}
+
+// Scale radius, amplitude 1e-12 delta_c rho r_s of an NFW halo with
+// Delta = 200 rho and c = conc (> 0) or c(M) = 5 (M/1e14)^-0.1
+// (nfw_dsigma_mis.cuh:104-123 normalisation).
+template <typename T> struct NfwHalo {
+  T rs, amp;
+};
+template <typename T>
+Y3G_HD NfwHalo<T> nfw_halo(double lnM, T rho, T conc)
+{
+  T const dl = T(lnM - kLnPivotHmf);
+  T const c = conc > T(0) ? conc : T(5) * M<T>::exp(T(-0.1) * dl);
+  // r200 = cbrt(3 M / (800 pi rho)), M = 1e14 e^dl
+  T const r200 = M<T>::cbrt(T(3e14 / (800.0 * 3.14159265358979323846)) *
+                            M<T>::exp(dl) / rho);
+  T const rs = r200 / c;
+  T const dc = T(200.0 / 3.0) * c * c * c /
+               (M<T>::log(T(1) + c) - c / (T(1) + c));
+  NfwHalo<T> out;
+  out.rs = rs;
+  out.amp = T(1e-12) * dc * rho * rs;
+  return out;
+}
+
+template <typename T> Y3G_HD T nfw_dsigma(T R, const NfwHalo<T>& H)
+{
+  return H.amp * nfw_g<T>(R / H.rs);
+}
+
+// Tangential Delta-Sigma about a centre offset by rmis at angle theta:
+// R' (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:176-178), cos(2 beta).
+template <typename T>
+Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
+{
+  T const ct = M<T>::cos(theta);
+  T const rp2 = R * R + rmis * rmis + T(2) * R * rmis * ct;
+  if (!(rp2 > T(1e-20))) return T(0);
+  T const a = R + rmis * ct;
+  T const c2b = T(2) * a * a / rp2 - T(1);
+  return nfw_dsigma<T>(M<T>::sqrt(rp2), H) * c2b;
+}
+
+// ------------------------------------------------------- integrands
+// Mapped integrand: y are the integration variables of the box
+// (t = standardised MOR variable in place of lambda_true).  Returns the
+// physical integrand times dlt/dt, with MOR(lt) dlt = phi(t) dt.
+template <typename T>
+Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                   const double* y)
+{
+  int it, iz, im;
+  switch (ig.kind) {
+    case Y3G_COUNT3D: it = 0, iz = 1, im = 2; break;
+    case Y3G_COUNT5D: it = 2, iz = 3, im = 4; break;
+    default: it = 1, iz = 2, im = 3; break;
+  }
+  double const zt = y[iz], lnM = y[im];
+  T const tt = T(y[it]);
+  T const sln = T(P.p.mor_sigma_ln);
+  T const zp = T(1) + T(zt);
+  T const lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
+                   T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
+  T const lt = M<T>::exp(lnlbar + sln * tt);
+  if (lt < T(ig.lt_lo) || lt > T(ig.lt_hi)) return T(0);
+  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327);
+  T const common = hmf<T>(P, tb.hmf, lnM, zt) * phi * dv_do_dz<T>(P, tb.da, zt);
+  T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
+  switch (ig.kind) {
+    case Y3G_COUNT3D:
+      return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+    case Y3G_COUNT5D: {
+      T const lo = T(y[0]), zo = T(y[1]);
+      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+             zo_pdf<T>(zo, T(zt), T(P.p.sigz0));
+    }
+    default: {
+      T const lo = T(y[0]);
+      T const rmis = T(y[4]), th = T(y[5]);
+      T const rho = T(P.rho_m);
+      NfwHalo<T> const H = nfw_halo<T>(lnM, rho, T(0));
+      T const R = T(ig.radius);
+      T const fmis = T(P.p.f_mis);
+      T const prof = (T(1) - fmis) * nfw_dsigma<T>(R, H) +
+                     fmis * dsigma_offset<T>(R, rmis, th, H);
+      T const s = T(ig.sigma_mis);
+      T const pr = rmis / (s * s) * M<T>::exp(T(-0.5) * rmis * rmis / (s * s)) *
+                   T(ig.rmis_norm);
+      T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
+      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+             zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
+             pr * T(0.15915494309189535) * prof;
+    }
+  }
+}
+
+// ------------------------------------------------ Genz-Malik rule nodes
+constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
+constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)
+constexpr double kL5 = 0.68824720161168529;  // sqrt(9/19)
+
+__host__ __device__ constexpr int gm_num_nodes(int d)
+{
+  return 1 + 4 * d + 2 * d * (d - 1) + (1 << d);
+}
+
+// Generator class (1..5) of node j, and its unit offset on axis a.  Node
+// enumeration: [0] centre; +-l2 e_i; +-l3 e_i; +-l4 e_i +-l4 e_k (i<k);
+// the 2^d +-l5 corners.
+Y3G_HD int gm_class(int d, int j)
+{
+  if (j == 0) return 1;
+  j -= 1;
+  if (j < 2 * d) return 2;
+  j -= 2 * d;
+  if (j < 2 * d) return 3;
+  j -= 2 * d;
+  if (j < 2 * d * (d - 1)) return 4;
+  return 5;
+}
+
+Y3G_HD void gm_offsets(int d, int j, double* u)
+{
+#pragma unroll
+  for (int i = 0; i < kMaxDim; ++i) u[i] = 0.0;
+  if (j == 0) return;
+  j -= 1;
+  if (j < 2 * d) {
+    u[j >> 1] = (j & 1) ? -kL2 : kL2;
+    return;
+  }
+  j -= 2 * d;
+  if (j < 2 * d) {
+    u[j >> 1] = (j & 1) ? -kL3 : kL3;
+    return;
+  }
+  j -= 2 * d;
+  if (j < 2 * d * (d - 1)) {
+    int pair = j >> 2, sg = j & 3, a = 0, b = 1, cnt = 0;
+    for (a = 0; a < d; ++a) {
+      int const np = d - 1 - a;
+      if (pair < cnt + np) {
+        b = a + 1 + (pair - cnt);
+        break;
+      }
+      cnt += np;
+    }
+    u[a] = (sg & 1) ? -kL3 : kL3;
+    u[b] = (sg & 2) ? -kL3 : kL3;
+    return;
+  }
+  j -= 2 * d * (d - 1);
+  for (int i = 0; i < d; ++i) u[i] = ((j >> i) & 1) ? -kL5 : kL5;
+}
+
+__host__ __device__ inline void gm_weights(int d, double* w7, double* w5)
+{
+  double const dd = d;
+  w7[0] = 0.0;
+  w5[0] = 0.0;
+  w7[1] = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
+  w7[2] = 980.0 / 6561.0;
+  w7[3] = (1820.0 - 400.0 * dd) / 19683.0;
+  w7[4] = 200.0 / 19683.0;
+  w7[5] = 6859.0 / 19683.0 / double(1 << d);
+  w5[1] = (729.0 - 950.0 * dd + 50.0 * dd * dd) / 729.0;
+  w5[2] = 245.0 / 486.0;
+  w5[3] = (265.0 - 100.0 * dd) / 1458.0;
+  w5[4] = 25.0 / 729.0;
+  w5[5] = 0.0;
+}
+
+// --------------------------------------------------------- Philox4x32-10
+Y3G_HD void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t const hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t const lo0 = 0xD2511F53u * c[0];
+    uint32_t const hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t const lo1 = 0xCD9E8D57u * c[2];
+    uint32_t const n0 = hi1 ^ c[1] ^ k0;
+    uint32_t const n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+}  // namespace y3g

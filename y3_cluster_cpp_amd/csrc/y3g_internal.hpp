@@ -1,0 +1,82 @@
+// y3g_internal.hpp -- argument blocks shared by the kernels
+// (y3g_kernels.hip) and the host runtime (y3g_runtime.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "y3g.h"
+#include "y3g_device.hpp"
+
+namespace y3g {
+
+struct TableArgs {
+  const PointDev* points;
+  y3g_table_spec spec;
+  double* hmf64;
+  float* hmf32;
+  int hmf_stride;
+  double* da64;
+  float* da32;
+  int da_stride;
+  double* sci64;
+  float* sci32;
+  int sci_stride;
+};
+
+struct KernArgs {
+  const y3g_integral* igs;
+  int n;
+  const int* order;
+  const PointDev* points;
+  const float* hmf32;
+  const double* hmf64;
+  int hmf_stride;
+  const float* da32;
+  const double* da64;
+  int da_stride;
+  const float* sci32;
+  const double* sci64;
+  int sci_stride;
+  // cubature region slab: per slot lo/hi [kMaxDim][cap], est/err/axis [cap]
+  double* slab_lo;
+  double* slab_hi;
+  double* slab_est;
+  double* slab_err;
+  int8_t* slab_ax;
+  int cap;
+  int kmax;
+  int init_split;
+  // VEGAS
+  long long vegas_calls;
+  int vegas_itmax, vegas_itskip, vegas_nbins;
+  double vegas_alpha;
+  int* queue;
+  y3g_result* out;
+  unsigned long long* nevals;
+};
+
+struct FunctorArgs {
+  int which;
+  const double* in;
+  int nin;
+  size_t n;
+  const double* par;
+  const PointDev* points;  // the point the table functors read
+  const void* hmf;
+  const void* da;
+  const void* sci;
+  const y3g_integral* ig;
+  double* out;
+};
+
+size_t cubature_smem(int precision, int block);
+hipError_t launch_cubature(const KernArgs& A, int precision, int block,
+                           int slots, hipStream_t s);
+hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
+                        hipStream_t s);
+hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s);
+hipError_t launch_functor(const FunctorArgs& F, int precision, hipStream_t s);
+
+}  // namespace y3g

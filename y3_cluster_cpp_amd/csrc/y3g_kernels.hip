@@ -1,0 +1,888 @@
+// y3g_kernels.hip -- gfx950 kernels of the cluster-observable integration
+// path: per-point table builder, the persistent workgroup-per-integral
+// Genz-Malik cubature, VEGAS with m-Cubes stratification, and the functor
+// evaluation kernel used by the parity tests.
+//
+// Work distribution (both integrators): one launch, one workgroup per
+// resident slot, integrals taken from a device work queue (atomic counter)
+// in the order given by the host (largest expected cost first).  Each
+// integral is processed start to finish by one workgroup, with a fixed
+// reduction order, so its result does not depend on what else is in the
+// batch.  Sums, variances, error estimates, the split decision and the
+// VEGAS grid refinement are FP64; the integrand runs in T.
+#include <hip/hip_runtime.h>
+
+#include "y3g_internal.hpp"
+
+namespace y3g {
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int BLOCK>
+__device__ double block_sum(double v, double* red)
+{
+  constexpr int NW = BLOCK / 64;
+  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) s += red[i];
+  return s;
+}
+
+template <int BLOCK>
+__device__ void block_sum2(double& a, double& b, double* red)
+{
+  constexpr int NW = BLOCK / 64;
+  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if (lane == 0) red[w] = a, red[NW + w] = b;
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) sa += red[i], sb += red[NW + i];
+  a = sa;
+  b = sb;
+}
+
+template <int BLOCK>
+__device__ int block_sum_i(int v, int* red)
+{
+  constexpr int NW = BLOCK / 64;
+  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_sum_i(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) s += red[i];
+  return s;
+}
+
+// --------------------------------------------------------- table builder
+__constant__ double c_gl8_x[4] = {0.1834346424956498, 0.5255324099163290,
+                                  0.7966664774136267, 0.9602898564975363};
+__constant__ double c_gl8_w[4] = {0.3626837833783620, 0.3137066458778873,
+                                  0.2223810344533745, 0.1012285362903763};
+
+// chi(z) [Mpc/h], composite 8-point Gauss-Legendre on 64 panels (the same
+// sum, in the same order, as the oracle's y3o_comoving_distance).
+__device__ double comoving_distance(double om, double z)
+{
+  int const np = 64;
+  double const dz = z / np;
+  double s = 0.0;
+  for (int ip = 0; ip < np; ++ip) {
+    double const c = (ip + 0.5) * dz, hw = 0.5 * dz;
+    double ps = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      double const zp = c + hw * c_gl8_x[k], zm = c - hw * c_gl8_x[k];
+      double const ep = sqrt(om * (1 + zp) * (1 + zp) * (1 + zp) + (1 - om));
+      double const em = sqrt(om * (1 + zm) * (1 + zm) * (1 + zm) + (1 - om));
+      ps += c_gl8_w[k] * (1.0 / ep + 1.0 / em);
+    }
+    s += ps * hw;
+  }
+  return kDh * s;
+}
+
+__device__ __forceinline__ double lin_knot(double a, double b, int n, int i)
+{
+  return a + double(i) * ((b - a) / double(n - 1));
+}
+
+// One workgroup per parameter point: HMF (n_z x n_lnM), D_A (n_da),
+// Sigma_crit^-1 (n_sci), FP64 and FP32 copies.
+__global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
+{
+  int const pt = blockIdx.x;
+  PointDev const P = A.points[pt];
+  y3g_table_spec const& s = A.spec;
+  double* hmf64 = A.hmf64 + (size_t)pt * A.hmf_stride;
+  float* hmf32 = A.hmf32 + (size_t)pt * A.hmf_stride;
+  for (int k = threadIdx.x; k < s.n_lnm * s.n_z; k += blockDim.x) {
+    int const i = k % s.n_lnm, j = k / s.n_lnm;
+    double const lnM = lin_knot(s.lnm_lo, s.lnm_hi, s.n_lnm, i);
+    double const z = lin_knot(s.z_lo, s.z_hi, s.n_z, j);
+    double const sig = P.p.hmf_amp * exp(-0.25 * (lnM - kLnPivotHmf)) *
+                       pow(1.0 + z, -0.8);
+    double const v = P.p.hmf_A * (pow(sig, -P.p.hmf_a) + 1.0) *
+                     exp(-P.p.hmf_c / (sig * sig));
+    hmf64[k] = v;
+    hmf32[k] = float(v);
+  }
+  double* da64 = A.da64 + (size_t)pt * A.da_stride;
+  float* da32 = A.da32 + (size_t)pt * A.da_stride;
+  for (int i = threadIdx.x; i < s.n_da; i += blockDim.x) {
+    double const z = lin_knot(0.0, s.da_zmax, s.n_da, i);
+    double const v = comoving_distance(P.p.omega_m, z) / P.p.h / (1.0 + z);
+    da64[i] = v;
+    da32[i] = float(v);
+  }
+  double const chis = comoving_distance(P.p.omega_m, P.p.z_source);
+  double* sci64 = A.sci64 + (size_t)pt * A.sci_stride;
+  float* sci32 = A.sci32 + (size_t)pt * A.sci_stride;
+  for (int i = threadIdx.x; i < s.n_sci; i += blockDim.x) {
+    double const z = lin_knot(0.0, s.sci_zmax, s.n_sci, i);
+    double const chil = comoving_distance(P.p.omega_m, z);
+    double const v =
+      chil < chis ? kSciConst * (1.0 + z) * chil * (chis - chil) / chis : 0.0;
+    sci64[i] = v;
+    sci32[i] = float(v);
+  }
+}
+
+// Table pointers of precision T.
+template <typename T> struct TabSel;
+template <> struct TabSel<float> {
+  static __device__ const float* hmf(const KernArgs& A) { return A.hmf32; }
+  static __device__ const float* da(const KernArgs& A) { return A.da32; }
+  static __device__ const float* sci(const KernArgs& A) { return A.sci32; }
+};
+template <> struct TabSel<double> {
+  static __device__ const double* hmf(const KernArgs& A) { return A.hmf64; }
+  static __device__ const double* da(const KernArgs& A) { return A.da64; }
+  static __device__ const double* sci(const KernArgs& A) { return A.sci64; }
+};
+
+// Stage the tables of point pt into LDS.
+template <typename T>
+__device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
+                             T* s_sci)
+{
+  const T* g_hmf = TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
+  const T* g_da = TabSel<T>::da(A) + (size_t)pt * A.da_stride;
+  const T* g_sci = TabSel<T>::sci(A) + (size_t)pt * A.sci_stride;
+  for (int k = threadIdx.x; k < A.hmf_stride; k += blockDim.x) s_hmf[k] = g_hmf[k];
+  for (int k = threadIdx.x; k < A.da_stride; k += blockDim.x) s_da[k] = g_da[k];
+  for (int k = threadIdx.x; k < A.sci_stride; k += blockDim.x) s_sci[k] = g_sci[k];
+}
+
+// ------------------------------------------------------------- cubature
+//
+// Region algorithm (identical to the oracle's y3o_cubature restatement):
+// initial k^d uniform partition; per region R7, R5, err = |R7 - R5| and the
+// split axis (largest fourth difference; widest relative edge if all zero);
+// each step splits the largest-error regions holding half of E (and at least
+// max(64, N/8) of them, at most kmax; select_bins) at the midpoint of their
+// axis, child A in the parent slot, child B appended; if
+// |R7_A + R7_B - R7_parent| exceeds the children's combined error, their
+// errors are scaled up to that difference; stop when
+// E <= max(epsabs, epsrel |I|) (status 0) or when the region or evaluation
+// budget would be exceeded (status 1).
+
+constexpr int kChunk = 32;                      // regions per evaluation chunk
+constexpr double kAlpha = 0.5;                  // error mass split per step
+constexpr int kMinSplit = 64;
+
+// Error bin: binary exponent of E minus that of err (0 = largest), 63 for
+// err == 0.
+__host__ __device__ inline int err_bin(double err, double E, int eE)
+{
+  if (!(err > 0.0)) return 63;
+  int e;
+  (void)frexp(err, &e);
+  int const b = eE - e;
+  return b < 0 ? 0 : (b > 62 ? 62 : b);
+}
+
+// err / E in 2^-40 fixed point.
+__host__ __device__ inline unsigned long long err_q(double err, double E)
+{
+  double const q = floor(err / E * 1099511627776.0);
+  return q > 0.0 ? (unsigned long long)q : 0ull;
+}
+
+// Split bin B: the first bin where the cumulative mass reaches kAlpha and
+// the first where the cumulative count reaches max(kMinSplit, N/8) (capped
+// by N and room); all of bins < B up to room (lim0), then bin B (lim1).
+__host__ __device__ inline void select_bins(const int* hc,
+                                            const unsigned long long* hq,
+                                            int N, int room, int& B, int& lim0,
+                                            int& lim1)
+{
+  unsigned long long const qa =
+    (unsigned long long)(kAlpha * 1099511627776.0);
+  int km = N / 8 > kMinSplit ? N / 8 : kMinSplit;
+  if (km > room) km = room;
+  if (km > N) km = N;
+  int Ba = 63, Bk = 63;
+  unsigned long long cq = 0;
+  int cc = 0;
+  for (int b = 0; b < 64; ++b) {
+    cq += hq[b];
+    cc += hc[b];
+    if (Ba == 63 && cq >= qa) Ba = b;
+    if (Bk == 63 && cc >= km) Bk = b;
+  }
+  B = Ba > Bk ? Ba : Bk;
+  int below = 0;
+  for (int b = 0; b < B; ++b) below += hc[b];
+  lim0 = below < room ? below : room;
+  int const inb = B < 63 ? hc[B] : 0;
+  lim1 = room - lim0 < inb ? room - lim0 : inb;
+}
+constexpr int kMaxNodes = gm_num_nodes(kMaxDim);  // 149
+constexpr int kKmax = 1024;
+
+template <typename T, int BLOCK> struct CubShared {
+  T hmf[kMaxHmf];
+  T da[kMaxDa];
+  T sci[kMaxSci];
+  T f[kChunk * kMaxNodes];
+  double c[kChunk][kMaxDim], h[kChunk][kMaxDim];
+  double est[kChunk], err[kChunk];
+  int ax[kChunk];
+  int idx[kChunk];
+  double pest[kChunk / 2], perr[kChunk / 2];
+  int sel[kKmax];
+  double red[2 * (BLOCK / 64)];
+  int ired[BLOCK / 64];
+  int hc[64];
+  unsigned long long hq[64];
+  int selB, lim0, lim1;
+  double w7[6], w5[6];
+  y3g_integral ig;
+  PointDev P;
+  int item, point, ncount;
+};
+
+// Evaluate the m regions whose centre / half-widths are in S.c / S.h:
+// S.est, S.err, S.ax.
+template <typename T, int BLOCK>
+__device__ void eval_chunk(CubShared<T, BLOCK>& S, int m)
+{
+  int const d = S.ig.ndim;
+  int const nn = gm_num_nodes(d);
+  Tabs<T> tb{S.hmf, S.da, S.sci};
+  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+    int const r = q / nn, j = q - r * nn;
+    double u[kMaxDim], y[kMaxDim];
+    gm_offsets(d, j, u);
+#pragma unroll
+    for (int a = 0; a < kMaxDim; ++a)
+      if (a < d) y[a] = S.c[r][a] + S.h[r][a] * u[a];
+    S.f[q] = integrand<T>(S.ig, S.P, tb, y);
+  }
+  __syncthreads();
+  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int NW = BLOCK / 64;
+  for (int r = w; r < m; r += NW) {
+    const T* fr = S.f + r * nn;
+    double a7 = 0.0, a5 = 0.0;
+    for (int j = lane; j < nn; j += 64) {
+      int const g = gm_class(d, j);
+      double const fv = double(fr[j]);
+      a7 += S.w7[g] * fv;
+      a5 += S.w5[g] * fv;
+    }
+    a7 = wave_sum(a7);
+    a5 = wave_sum(a5);
+    // fourth differences, one lane per axis
+    double four = -1.0;
+    if (lane < d) {
+      double const f0 = double(fr[0]);
+      double const f2 = double(fr[1 + 2 * lane]) + double(fr[2 + 2 * lane]);
+      double const f3 =
+        double(fr[1 + 2 * d + 2 * lane]) + double(fr[2 + 2 * d + 2 * lane]);
+      four = fabs(f2 - 2.0 * f0 - (f3 - 2.0 * f0) / 7.0);
+    }
+    // argmax over lanes 0..d-1 (ties -> lowest axis)
+    int best = 0;
+    double bv = __shfl(four, 0, 64);
+    for (int a = 1; a < d; ++a) {
+      double const v = __shfl(four, a, 64);
+      if (v > bv) bv = v, best = a;
+    }
+    if (lane == 0) {
+      double vol = 1.0;
+      for (int a = 0; a < d; ++a) vol *= 2.0 * S.h[r][a];
+      if (!(bv > 0.0)) {
+        double bw = -1.0;
+        for (int a = 0; a < d; ++a) {
+          double const rel = 2.0 * S.h[r][a] / (S.ig.highs[a] - S.ig.lows[a]);
+          if (rel > bw) bw = rel, best = a;
+        }
+      }
+      S.est[r] = a7 * vol;
+      S.err[r] = fabs(a7 - a5) * vol;
+      S.ax[r] = best;
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
+{
+  extern __shared__ char smem_raw[];
+  CubShared<T, BLOCK>& S = *reinterpret_cast<CubShared<T, BLOCK>*>(smem_raw);
+  int const tid = threadIdx.x;
+  int const slot = blockIdx.x;
+  int const cap = A.cap;
+  double* lo = A.slab_lo + (size_t)slot * kMaxDim * cap;
+  double* hi = A.slab_hi + (size_t)slot * kMaxDim * cap;
+  double* est = A.slab_est + (size_t)slot * cap;
+  double* err = A.slab_err + (size_t)slot * cap;
+  int8_t* axs = A.slab_ax + (size_t)slot * cap;
+  if (tid == 0) S.point = -1;
+  unsigned long long nev_total = 0;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) S.item = atomicAdd(A.queue, 1);
+    __syncthreads();
+    int const item = S.item;
+    if (item >= A.n) break;
+    int const iid = A.order ? A.order[item] : item;
+    if (tid == 0) {
+      S.ig = A.igs[iid];
+      gm_weights(S.ig.ndim, S.w7, S.w5);
+    }
+    __syncthreads();
+    int const pt = S.ig.point;
+    if (pt != S.point) {
+      if (tid == 0) S.P = A.points[pt];
+      stage_tables<T>(A, pt, S.hmf, S.da, S.sci);
+      __syncthreads();
+      if (tid == 0) S.point = pt;
+    }
+    int const d = S.ig.ndim;
+    int const nn = gm_num_nodes(d);
+    int k = A.init_split > 0 ? A.init_split : (d <= 3 ? 4 : 2);
+    int N = 1;
+    for (int a = 0; a < d; ++a) N *= k;
+    if (N > cap) k = 1, N = 1;
+    // initial partition, region r -> digits (base k) per axis
+    for (int r = tid; r < N; r += BLOCK) {
+      int rr = r;
+      for (int a = 0; a < d; ++a) {
+        int const q = rr % k;
+        rr /= k;
+        double const L = S.ig.lows[a], H = S.ig.highs[a];
+        double const w = (H - L) / k;
+        lo[(size_t)a * cap + r] = q == 0 ? L : L + q * w;
+        hi[(size_t)a * cap + r] = q == k - 1 ? H : L + (q + 1) * w;
+      }
+    }
+    __syncthreads();
+    // evaluate the initial regions
+    for (int base = 0; base < N; base += kChunk) {
+      int const m = min(kChunk, N - base);
+      for (int q = tid; q < m * d; q += BLOCK) {
+        int const r = q / d, a = q - r * d;
+        double const l = lo[(size_t)a * cap + base + r];
+        double const h = hi[(size_t)a * cap + base + r];
+        S.c[r][a] = 0.5 * (l + h);
+        S.h[r][a] = 0.5 * (h - l);
+      }
+      __syncthreads();
+      eval_chunk<T, BLOCK>(S, m);
+      if (tid < m) {
+        est[base + tid] = S.est[tid];
+        err[base + tid] = S.err[tid];
+        axs[base + tid] = (int8_t)S.ax[tid];
+      }
+      __syncthreads();
+    }
+    long long neval = (long long)N * nn;
+    int status = 1;
+    double I = 0.0, E = 0.0;
+    for (;;) {
+      double a = 0.0, b = 0.0;
+      for (int r = tid; r < N; r += BLOCK) a += est[r], b += err[r];
+      block_sum2<BLOCK>(a, b, S.red);
+      I = a;
+      E = b;
+      if (E <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
+        status = 0;
+        break;
+      }
+      // selection: error bins by binary exponent relative to E (bin 0 holds
+      // the largest errors), error mass per bin in 2^-40 E fixed point;
+      // split bins 0..B-1 and, in index order, bin B, where B covers at
+      // least kAlpha of E and at least max(64, N/8) regions, at most `room`
+      int const room = min(A.kmax, cap - N);
+      if (room <= 0) break;
+      int eE;
+      (void)frexp(E, &eE);
+      if (tid < 64) S.hc[tid] = 0, S.hq[tid] = 0ull;
+      __syncthreads();
+      for (int r = tid; r < N; r += BLOCK) {
+        int const b = err_bin(err[r], E, eE);
+        atomicAdd(&S.hc[b], 1);
+        atomicAdd(&S.hq[b], err_q(err[r], E));
+      }
+      __syncthreads();
+      if (tid == 0) select_bins(S.hc, S.hq, N, room, S.selB, S.lim0, S.lim1);
+      __syncthreads();
+      int const B = S.selB, lim0 = S.lim0, lim1 = S.lim1;
+      int const ns = lim0 + lim1;
+      if (ns <= 0 || neval + 2LL * ns * nn > S.ig.max_eval) break;
+      // ordered compaction: bins < B (up to lim0), then bin B (up to lim1)
+      if (tid == 0) S.ncount = 0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        int const limit = pass == 0 ? lim0 : lim0 + lim1;
+        for (int base = 0; base < N; base += BLOCK) {
+          int const r = base + tid;
+          bool f = false;
+          if (r < N) {
+            int const b = err_bin(err[r], E, eE);
+            f = pass == 0 ? (b < B) : (b == B);
+          }
+          unsigned long long const bal = __ballot(f);
+          int const lane = tid & 63, w = tid >> 6;
+          int const before = __popcll(bal & ((1ull << lane) - 1ull));
+          if (lane == 0) S.ired[w] = __popcll(bal);
+          __syncthreads();
+          int off = S.ncount;
+          for (int i = 0; i < w; ++i) off += S.ired[i];
+          if (f && off + before < limit) S.sel[off + before] = r;
+          __syncthreads();
+          if (tid == 0) {
+            int t = 0;
+            for (int i = 0; i < BLOCK / 64; ++i) t += S.ired[i];
+            S.ncount = min(S.ncount + t, limit);
+          }
+          __syncthreads();
+          if (S.ncount >= limit) break;
+        }
+        __syncthreads();
+        if (tid == 0) S.ncount = min(S.ncount, limit);
+        __syncthreads();
+      }
+      // split and evaluate, kChunk/2 pairs at a time
+      for (int pb = 0; pb < ns; pb += kChunk / 2) {
+        int const np = min(kChunk / 2, ns - pb);
+        for (int q = tid; q < np * d; q += BLOCK) {
+          int const j = q / d, a = q - j * d;
+          int const p = S.sel[pb + j];
+          int const ax = axs[p];
+          double const l = lo[(size_t)a * cap + p];
+          double const h = hi[(size_t)a * cap + p];
+          if (a == ax) {
+            double const mid = 0.5 * (l + h);
+            S.c[2 * j][a] = 0.5 * (l + mid);
+            S.h[2 * j][a] = 0.5 * (mid - l);
+            S.c[2 * j + 1][a] = 0.5 * (mid + h);
+            S.h[2 * j + 1][a] = 0.5 * (h - mid);
+            lo[(size_t)a * cap + N + pb + j] = mid;
+            hi[(size_t)a * cap + N + pb + j] = h;
+            hi[(size_t)a * cap + p] = mid;
+          } else {
+            S.c[2 * j][a] = S.c[2 * j + 1][a] = 0.5 * (l + h);
+            S.h[2 * j][a] = S.h[2 * j + 1][a] = 0.5 * (h - l);
+            lo[(size_t)a * cap + N + pb + j] = l;
+            hi[(size_t)a * cap + N + pb + j] = h;
+          }
+        }
+        if (tid < np) {
+          int const p = S.sel[pb + tid];
+          S.pest[tid] = est[p];
+          S.perr[tid] = err[p];
+        }
+        __syncthreads();
+        eval_chunk<T, BLOCK>(S, 2 * np);
+        if (tid < np) {
+          int const p = S.sel[pb + tid];
+          double ea = S.err[2 * tid], eb = S.err[2 * tid + 1];
+          double const delta =
+            fabs(S.est[2 * tid] + S.est[2 * tid + 1] - S.pest[tid]);
+          double const se = ea + eb;
+          if (delta > se) {
+            if (se > 0.0) {
+              double const f = delta / se;
+              ea *= f;
+              eb *= f;
+            } else {
+              ea = eb = 0.5 * delta;
+            }
+          }
+          est[p] = S.est[2 * tid];
+          err[p] = ea;
+          axs[p] = (int8_t)S.ax[2 * tid];
+          est[N + pb + tid] = S.est[2 * tid + 1];
+          err[N + pb + tid] = eb;
+          axs[N + pb + tid] = (int8_t)S.ax[2 * tid + 1];
+        }
+        __syncthreads();
+      }
+      N += ns;
+      neval += 2LL * ns * nn;
+    }
+    if (tid == 0) {
+      y3g_result res;
+      res.value = I;
+      res.error = E;
+      res.prob = 0.0;
+      res.neval = neval;
+      res.nregions = N;
+      res.status = status;
+      A.out[iid] = res;
+      nev_total += (unsigned long long)neval;
+    }
+  }
+  if (tid == 0 && nev_total) atomicAdd(A.nevals, nev_total);
+}
+
+// ------------------------------------------------------------------ VEGAS
+//
+// m-Cubes stratification + VEGAS importance grid (identical to the oracle's
+// y3o_vegas restatement): g = floor((calls/2)^(1/d)) cubes per axis,
+// p = max(2, calls/g^d) samples per cube, Philox4x32-10 with key = seed and
+// counter (cube, sample, iteration, block), 53-bit uniforms (FP64) or 24-bit
+// uniforms (FP32 path); Lepage refinement with 3-point smoothing and damping
+// alpha; iterations < itskip only adapt; inverse-variance combination; error
+// inflated by sqrt(chi2/dof) when that exceeds 1.
+
+constexpr int kNbMax = 128;
+
+template <typename T, int BLOCK> struct VegShared {
+  T hmf[kMaxHmf];
+  T da[kMaxDa];
+  T sci[kMaxSci];
+  double xg[kMaxDim][kNbMax + 1];
+  double hist[kMaxDim][kNbMax];
+  double d1[kMaxDim][kNbMax];
+  double red[2 * (BLOCK / 64)];
+  y3g_integral ig;
+  PointDev P;
+  int item, point, stop;
+};
+
+template <typename T> struct Uniform;
+template <> struct Uniform<double> {
+  static constexpr int words_per_dim = 2;
+  static __device__ double get(const uint32_t* w, int k)
+  {
+    return (double(w[2 * k] >> 5) * 67108864.0 + double(w[2 * k + 1] >> 6)) *
+           (1.0 / 9007199254740992.0);
+  }
+};
+template <> struct Uniform<float> {
+  static constexpr int words_per_dim = 1;
+  static __device__ double get(const uint32_t* w, int k)
+  {
+    return double(w[k] >> 8) * (1.0 / 16777216.0) + (0.5 / 16777216.0);
+  }
+};
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
+{
+  extern __shared__ char smem_raw[];
+  VegShared<T, BLOCK>& S = *reinterpret_cast<VegShared<T, BLOCK>*>(smem_raw);
+  int const tid = threadIdx.x;
+  if (tid == 0) S.point = -1;
+  unsigned long long nev_total = 0;
+  int const nb = A.vegas_nbins;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) S.item = atomicAdd(A.queue, 1);
+    __syncthreads();
+    int const item = S.item;
+    if (item >= A.n) break;
+    int const iid = A.order ? A.order[item] : item;
+    if (tid == 0) S.ig = A.igs[iid];
+    __syncthreads();
+    int const pt = S.ig.point;
+    if (pt != S.point) {
+      if (tid == 0) S.P = A.points[pt];
+      stage_tables<T>(A, pt, S.hmf, S.da, S.sci);
+      __syncthreads();
+      if (tid == 0) S.point = pt;
+    }
+    int const d = S.ig.ndim;
+    Tabs<T> tb{S.hmf, S.da, S.sci};
+    for (int q = tid; q < d * (nb + 1); q += BLOCK) {
+      int const a = q / (nb + 1), b = q - a * (nb + 1);
+      S.xg[a][b] = double(b) / nb;
+    }
+    long long const calls = A.vegas_calls;
+    int g = (int)floor(pow(calls / 2.0, 1.0 / d));
+    if (g < 1) g = 1;
+    long long ncubes = 1;
+    for (int a = 0; a < d; ++a) ncubes *= g;
+    long long p = calls / ncubes;
+    if (p < 2) p = 2;
+    double range[kMaxDim], vol = 1.0;
+    for (int a = 0; a < d; ++a)
+      range[a] = S.ig.highs[a] - S.ig.lows[a], vol *= range[a];
+    uint32_t const k0 = (uint32_t)S.ig.seed, k1 = (uint32_t)(S.ig.seed >> 32);
+    int const nblk = (Uniform<T>::words_per_dim * d + 3) / 4;
+    double wsum = 0.0, isum = 0.0, i2sum = 0.0, I = 0.0, sig = 0.0;
+    int nacc = 0, status = 1;
+    long long neval = 0;
+    if (tid == 0) S.stop = 0;
+    __syncthreads();
+    for (int it = 0; it < A.vegas_itmax; ++it) {
+      for (int q = tid; q < d * nb; q += BLOCK) S.hist[q / nb][q % nb] = 0.0;
+      __syncthreads();
+      double sI = 0.0, sV = 0.0;
+      for (long long c = tid; c < ncubes; c += BLOCK) {
+        int ck[kMaxDim];
+        long long cc = c;
+        for (int a = 0; a < d; ++a) ck[a] = (int)(cc % g), cc /= g;
+        double s1 = 0.0, s2 = 0.0;
+        for (long long j = 0; j < p; ++j) {
+          uint32_t w[12];
+          for (int blk = 0; blk < nblk; ++blk) {
+            uint32_t ctr[4] = {(uint32_t)c, (uint32_t)j, (uint32_t)it,
+                               (uint32_t)blk};
+            philox(ctr, k0, k1);
+            w[4 * blk] = ctr[0], w[4 * blk + 1] = ctr[1];
+            w[4 * blk + 2] = ctr[2], w[4 * blk + 3] = ctr[3];
+          }
+          double y[kMaxDim], jac = vol;
+          int bins[kMaxDim];
+          for (int a = 0; a < d; ++a) {
+            double const u = Uniform<T>::get(w, a);
+            double const z = (ck[a] + u) / g * nb;
+            int b = (int)z;
+            if (b > nb - 1) b = nb - 1;
+            double const wb = S.xg[a][b + 1] - S.xg[a][b];
+            y[a] = S.ig.lows[a] + (S.xg[a][b] + (z - b) * wb) * range[a];
+            jac *= nb * wb;
+            bins[a] = b;
+          }
+          double const fx = double(integrand<T>(S.ig, S.P, tb, y)) * jac;
+          s1 += fx;
+          s2 += fx * fx;
+          for (int a = 0; a < d; ++a) atomicAdd(&S.hist[a][bins[a]], fx * fx);
+        }
+        double const m = s1 / p;
+        double const v = (s2 - s1 * m) / (double(p) * (p - 1));
+        sI += m;
+        sV += v > 0.0 ? v : 0.0;
+      }
+      block_sum2<BLOCK>(sI, sV, S.red);
+      neval += ncubes * p;
+      double const Ii = sI / ncubes;
+      double const Vi = sV / (double(ncubes) * ncubes);
+      if (it >= A.vegas_itskip) {
+        double const wi = Vi > 0.0 ? 1.0 / Vi : 1e300;
+        wsum += wi;
+        isum += wi * Ii;
+        i2sum += wi * Ii * Ii;
+        ++nacc;
+        I = isum / wsum;
+        sig = sqrt(1.0 / wsum);
+        double const dof = nacc - 1;
+        double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
+        double const sigr = chi > 1.0 ? sig * sqrt(chi) : sig;
+        if (sigr <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
+          status = 0;
+          break;
+        }
+      }
+      if (neval + ncubes * p > S.ig.max_eval) break;
+      if (it + 1 < A.vegas_itmax) {
+        // Lepage refinement, one thread per axis
+        if (tid < d) {
+          int const a = tid;
+          double* hs = S.hist[a];
+          double* dd = S.d1[a];
+          double sum = 0.0;
+          if (nb == 1) {
+            dd[0] = hs[0];
+          } else {
+            dd[0] = 0.5 * (hs[0] + hs[1]);
+            dd[nb - 1] = 0.5 * (hs[nb - 2] + hs[nb - 1]);
+            for (int b = 1; b < nb - 1; ++b)
+              dd[b] = (hs[b - 1] + hs[b] + hs[b + 1]) / 3.0;
+          }
+          for (int b = 0; b < nb; ++b) sum += dd[b];
+          if (sum > 0.0) {
+            double rs = 0.0;
+            for (int b = 0; b < nb; ++b) {
+              double const q = dd[b] / sum;
+              dd[b] = (q > 0.0 && q < 1.0) ? pow((q - 1.0) / log(q), A.vegas_alpha)
+                                           : 0.0;
+              rs += dd[b];
+            }
+            if (rs > 0.0) {
+              double const target = rs / nb;
+              double xn[kNbMax + 1];
+              int kk = -1;
+              double dr = 0.0;
+              xn[0] = 0.0;
+              for (int i = 1; i < nb; ++i) {
+                while (dr < target && kk < nb - 1) {
+                  ++kk;
+                  dr += dd[kk];
+                }
+                dr -= target;
+                xn[i] = S.xg[a][kk + 1] -
+                        (S.xg[a][kk + 1] - S.xg[a][kk]) * dr / dd[kk];
+              }
+              xn[nb] = 1.0;
+              for (int i = 0; i <= nb; ++i) S.xg[a][i] = xn[i];
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (tid == 0) {
+      double const dof = nacc - 1;
+      double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
+      y3g_result res;
+      res.value = I;
+      res.error = chi > 1.0 ? sig * sqrt(chi) : sig;
+      res.prob = chi;
+      res.neval = neval;
+      res.nregions = 0;
+      res.status = status;
+      A.out[iid] = res;
+      nev_total += (unsigned long long)neval;
+    }
+  }
+  if (tid == 0 && nev_total) atomicAdd(A.nevals, nev_total);
+}
+
+// ------------------------------------------------------- functor eval
+template <typename T>
+__global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
+{
+  size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F.n) return;
+  const double* x = F.in + i * F.nin;
+  const double* q = F.par;
+  PointDev const& P = F.points[0];
+  Tabs<T> tb{F.hmf ? (const T*)F.hmf : nullptr, (const T*)F.da, (const T*)F.sci};
+  T r = T(0);
+  switch (F.which) {
+    case Y3G_F_ROFFSET_GAMMA: r = roffset_gamma<T>(T(x[0]), T(q[0])); break;
+    case Y3G_F_LO_LC:
+      r = lo_lc<T>(T(x[0]), T(x[1]), T(x[2]), T(q[0]), T(q[1]), T(q[2]), T(q[3]));
+      break;
+    case Y3G_F_ZO_ZT_DES:
+      r = gaussian<T>(T(x[0]), T(x[1]), sigma_photoz_des<T>(T(x[1])));
+      break;
+    case Y3G_F_INT_ZO_ZT_DES:
+      r = int_zo_zt_des<T>(T(x[0]), T(x[1]), T(x[2]));
+      break;
+    case Y3G_F_SIGMA_PHOTOZ: r = sigma_photoz_des<T>(T(x[0])); break;
+    case Y3G_F_DV_DO_DZ: r = dv_do_dz<T>(P, tb.da, x[0]); break;
+    case Y3G_F_HMF: r = hmf<T>(P, tb.hmf, x[0], x[1]); break;
+    case Y3G_F_MOR: {
+      T const s = T(P.p.mor_sigma_ln);
+      T const lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(x[1] - kLnPivotMor) +
+                       T(P.p.mor_zslope) * M<T>::log((T(1) + T(x[2])) * T(1.0 / 1.35));
+      T const dd = (M<T>::log(T(x[0])) - lnlbar) / s;
+      r = M<T>::exp(T(-0.5) * dd * dd) * T(0.3989422804014327) / (s * T(x[0]));
+      break;
+    }
+    case Y3G_F_LO_LT_PDF:
+      r = lo_lt_pdf<T>(T(x[0]), T(x[1]), T(P.p.f_prj), T(P.p.prj_scale));
+      break;
+    case Y3G_F_LO_LT_BIN:
+      r = lo_lt_bin<T>(T(x[0]), T(x[1]), T(x[2]), T(P.p.f_prj), T(P.p.prj_scale));
+      break;
+    case Y3G_F_NFW_DSIGMA: {
+      NfwHalo<T> const H = nfw_halo<T>(x[1], T(x[2]), T(x[3]));
+      r = nfw_dsigma<T>(T(x[0]), H);
+      break;
+    }
+    case Y3G_F_DSIGMA_OFFSET: {
+      NfwHalo<T> const H = nfw_halo<T>(x[3], T(x[4]), T(x[5]));
+      r = dsigma_offset<T>(T(x[0]), T(x[1]), T(x[2]), H);
+      break;
+    }
+    case Y3G_F_SCI: r = lerp_clamp<T>(tb.sci, P.nsci, x[0], P.inv_dzsci); break;
+    case Y3G_F_INTEGRAND: r = integrand<T>(*F.ig, P, tb, x); break;
+  }
+  F.out[i] = double(r);
+}
+
+// ----------------------------------------------------------- launchers
+template <typename T, int BLOCK>
+static hipError_t launch_cub(const KernArgs& A, int slots, hipStream_t s)
+{
+  size_t const smem = sizeof(CubShared<T, BLOCK>);
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void*>(&k_cubature<T, BLOCK>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_cubature<T, BLOCK>), dim3(slots), dim3(BLOCK), smem,
+                     s, A);
+  return hipGetLastError();
+}
+
+template <typename T, int BLOCK>
+static hipError_t launch_veg(const KernArgs& A, int slots, hipStream_t s)
+{
+  size_t const smem = sizeof(VegShared<T, BLOCK>);
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void*>(&k_vegas<T, BLOCK>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_vegas<T, BLOCK>), dim3(slots), dim3(BLOCK), smem, s,
+                     A);
+  return hipGetLastError();
+}
+
+size_t cubature_smem(int precision, int block)
+{
+  if (precision == Y3G_FP64)
+    return block == 256 ? sizeof(CubShared<double, 256>)
+                        : sizeof(CubShared<double, 512>);
+  return block == 256 ? sizeof(CubShared<float, 256>)
+                      : sizeof(CubShared<float, 512>);
+}
+
+hipError_t launch_cubature(const KernArgs& A, int precision, int block,
+                           int slots, hipStream_t s)
+{
+  if (precision == Y3G_FP64)
+    return block == 256 ? launch_cub<double, 256>(A, slots, s)
+                        : launch_cub<double, 512>(A, slots, s);
+  return block == 256 ? launch_cub<float, 256>(A, slots, s)
+                      : launch_cub<float, 512>(A, slots, s);
+}
+
+hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
+                        hipStream_t s)
+{
+  if (precision == Y3G_FP64)
+    return block == 256 ? launch_veg<double, 256>(A, slots, s)
+                        : launch_veg<double, 512>(A, slots, s);
+  return block == 256 ? launch_veg<float, 256>(A, slots, s)
+                      : launch_veg<float, 512>(A, slots, s);
+}
+
+hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_tables_build, dim3(n_points), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_functor(const FunctorArgs& F, int precision, hipStream_t s)
+{
+  unsigned const nb = (unsigned)((F.n + 255) / 256);
+  if (precision == Y3G_FP64)
+    hipLaunchKernelGGL(k_eval_functor<double>, dim3(nb), dim3(256), 0, s, F);
+  else
+    hipLaunchKernelGGL(k_eval_functor<float>, dim3(nb), dim3(256), 0, s, F);
+  return hipGetLastError();
+}
+
+}  // namespace y3g

@@ -1,0 +1,615 @@
+// y3g_runtime.hip -- host side of liby3g.so: the C ABI of include/y3g.h.
+//
+// One y3g_runtime per process and device (DeviceInitializer of
+// src/utils/CosmoSISSICUDAModule.hh:59-83 binds device = local rank).  It
+// owns the per-point tables, the cubature region slab and the integral /
+// result buffers, all resident in HBM across calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "y3g.h"
+#include "y3g_internal.hpp"
+
+using namespace y3g;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, std::string const& msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                           \
+  do {                                                                      \
+    hipError_t const e_ = (x);                                              \
+    if (e_ != hipSuccess)                                                   \
+      return fail(Y3G_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T> int dalloc(T** p, size_t n)
+{
+  if (n == 0) n = 1;
+  hipError_t const e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(Y3G_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+template <typename T> void dfree(T*& p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct y3g_runtime {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0;
+  double clock_ghz = 0.0;
+  size_t hbm = 0;
+  std::string name;
+  // tables
+  int n_points = 0;
+  y3g_table_spec spec{};
+  std::vector<PointDev> hpoints;
+  PointDev* points = nullptr;
+  double *hmf64 = nullptr, *da64 = nullptr, *sci64 = nullptr;
+  float *hmf32 = nullptr, *da32 = nullptr, *sci32 = nullptr;
+  int hmf_stride = 0, da_stride = 0, sci_stride = 0;
+  // cubature slab
+  double *slab_lo = nullptr, *slab_hi = nullptr, *slab_est = nullptr,
+         *slab_err = nullptr;
+  int8_t* slab_ax = nullptr;
+  int slab_slots = 0, slab_cap = 0;
+  // batch buffers
+  y3g_integral* d_igs = nullptr;
+  y3g_result* d_res = nullptr;
+  int* d_order = nullptr;
+  size_t batch_cap = 0;
+  int* d_queue = nullptr;
+  unsigned long long* d_nevals = nullptr;
+  double last_ms = 0.0;
+  long long last_nevals = 0;
+};
+
+extern "C" {
+
+const char* y3g_last_error(void) { return g_err.c_str(); }
+const char* y3g_version(void) { return "y3g 0.1 gfx950"; }
+
+void y3g_options_default(y3g_options* o)
+{
+  o->precision = Y3G_FP32SPLIT;
+  o->max_regions = 65536;
+  o->max_split_per_step = 1024;
+  o->init_split = 0;
+  o->block_threads = 512;
+  o->slots = 0;
+  o->vegas_calls = 1000000;
+  o->vegas_itmax = 5;
+  o->vegas_itskip = 1;
+  o->vegas_nbins = 64;
+  o->vegas_alpha = 1.5;
+  o->single = 0;
+}
+
+void y3g_default_params(y3g_params* p)
+{
+  p->omega_m = 0.3;
+  p->h = 0.7;
+  p->hmf_amp = 0.8;
+  p->hmf_A = 0.2;
+  p->hmf_a = 1.5;
+  p->hmf_c = 1.2;
+  p->mor_lnl0 = 3.2;
+  p->mor_slope = 0.9;
+  p->mor_zslope = 0.5;
+  p->mor_sigma_ln = 0.25;
+  p->f_prj = 0.2;
+  p->prj_scale = 0.1;
+  p->f_mis = 0.2;
+  p->mis_rlambda = 0.2;
+  p->z_source = 1.2;
+  p->sigz0 = 0.01;
+}
+
+void y3g_spec_config0(y3g_table_spec* s)
+{
+  s->lnm_lo = 29.933606208922594;  // ln 1e13
+  s->lnm_hi = 35.690174623927322;  // 15.5 ln 10
+  s->n_lnm = 200;
+  s->z_lo = 0.20;
+  s->z_hi = 0.35;
+  s->n_z = 32;
+  s->da_zmax = 1.0;
+  s->n_da = 512;
+  s->sci_zmax = 1.0;
+  s->n_sci = 128;
+}
+
+void y3g_spec_wide(y3g_table_spec* s)
+{
+  y3g_spec_config0(s);
+  s->z_lo = 0.10;
+  s->z_hi = 0.75;
+}
+
+void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
+                       double lo_lo, double lo_hi, double zo_lo, double zo_hi,
+                       double radius, y3g_integral* ig)
+{
+  std::memset(ig, 0, sizeof(*ig));
+  ig->kind = kind;
+  ig->lo_lo = lo_lo;
+  ig->lo_hi = lo_hi;
+  ig->zo_lo = zo_lo;
+  ig->zo_hi = zo_hi;
+  ig->radius = radius;
+  ig->lt_lo = 1.0;
+  ig->lt_hi = 300.0;
+  ig->eps_rel = 1e-4;
+  ig->eps_abs = 1e-300;
+  ig->max_eval = 1LL << 40;
+  ig->seed = 0x59334331ull;
+  double const zt_lo = kind == Y3G_COUNT3D ? s->z_lo : zo_lo - 0.1;
+  double const zt_hi = kind == Y3G_COUNT3D ? s->z_hi : zo_hi + 0.1;
+  switch (kind) {
+    case Y3G_COUNT3D:
+      ig->ndim = 3;
+      ig->lows[0] = -kTMax, ig->highs[0] = kTMax;
+      ig->lows[1] = zt_lo, ig->highs[1] = zt_hi;
+      ig->lows[2] = s->lnm_lo, ig->highs[2] = s->lnm_hi;
+      break;
+    case Y3G_COUNT5D:
+      ig->ndim = 5;
+      ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
+      ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
+      ig->lows[2] = -kTMax, ig->highs[2] = kTMax;
+      ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
+      ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;
+      break;
+    default: {
+      ig->ndim = 6;
+      double const rl = std::pow(0.5 * (lo_lo + lo_hi) / 100.0, 0.2);
+      ig->sigma_mis = p->mis_rlambda * rl;
+      ig->rmis_norm = 1.0 / (1.0 - std::exp(-0.5 * 6.0 * 6.0));
+      ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
+      ig->lows[1] = -kTMax, ig->highs[1] = kTMax;
+      ig->lows[2] = zt_lo, ig->highs[2] = zt_hi;
+      ig->lows[3] = s->lnm_lo, ig->highs[3] = s->lnm_hi;
+      ig->lows[4] = 0.0, ig->highs[4] = 6.0 * ig->sigma_mis;
+      ig->lows[5] = 0.0, ig->highs[5] = 2.0 * 3.14159265358979323846;
+      break;
+    }
+  }
+}
+
+int y3g_runtime_create(int device, y3g_runtime** out)
+{
+  if (!out) return fail(Y3G_ERR_ARG, "y3g_runtime_create: null out");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(Y3G_ERR_NODEVICE, "no HIP device");
+  if (device < 0 || device >= ndev)
+    return fail(Y3G_ERR_ARG, "device " + std::to_string(device) + " of " +
+                               std::to_string(ndev));
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(Y3G_ERR_NODEVICE,
+                std::string("liby3g is built for gfx950, device is ") +
+                  prop.gcnArchName);
+  auto* rt = new y3g_runtime;
+  rt->device = device;
+  rt->n_cu = prop.multiProcessorCount;
+  rt->clock_ghz = prop.clockRate / 1e6;
+  rt->hbm = prop.totalGlobalMem;
+  rt->name = prop.gcnArchName;
+  if (hipStreamCreateWithFlags(&rt->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&rt->ev0) != hipSuccess ||
+      hipEventCreate(&rt->ev1) != hipSuccess ||
+      dalloc(&rt->d_queue, 1) || dalloc(&rt->d_nevals, 1)) {
+    y3g_runtime_destroy(rt);
+    return fail(Y3G_ERR_HIP, "runtime resources");
+  }
+  *out = rt;
+  return 0;
+}
+
+int y3g_runtime_destroy(y3g_runtime* rt)
+{
+  if (!rt) return 0;
+  (void)hipSetDevice(rt->device);
+  if (rt->stream) (void)hipStreamSynchronize(rt->stream);
+  dfree(rt->points);
+  dfree(rt->hmf64), dfree(rt->da64), dfree(rt->sci64);
+  dfree(rt->hmf32), dfree(rt->da32), dfree(rt->sci32);
+  dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
+    dfree(rt->slab_err), dfree(rt->slab_ax);
+  dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
+  dfree(rt->d_queue), dfree(rt->d_nevals);
+  if (rt->ev0) (void)hipEventDestroy(rt->ev0);
+  if (rt->ev1) (void)hipEventDestroy(rt->ev1);
+  if (rt->stream) (void)hipStreamDestroy(rt->stream);
+  delete rt;
+  return 0;
+}
+
+int y3g_device_info(y3g_runtime* rt, int* n_cu, double* clock_ghz,
+                    size_t* hbm_bytes, char* name, int name_len)
+{
+  if (!rt) return fail(Y3G_ERR_ARG, "null runtime");
+  if (n_cu) *n_cu = rt->n_cu;
+  if (clock_ghz) *clock_ghz = rt->clock_ghz;
+  if (hbm_bytes) *hbm_bytes = rt->hbm;
+  if (name && name_len > 0) {
+    std::strncpy(name, rt->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return 0;
+}
+
+static PointDev make_point(const y3g_params& p, const y3g_table_spec& s)
+{
+  PointDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.p = p;
+  P.lnm0 = s.lnm_lo;
+  P.dlnm = (s.lnm_hi - s.lnm_lo) / (s.n_lnm - 1);
+  P.inv_dlnm = 1.0 / P.dlnm;
+  P.z0 = s.z_lo;
+  P.dz = (s.z_hi - s.z_lo) / (s.n_z - 1);
+  P.inv_dz = 1.0 / P.dz;
+  P.inv_dzda = (s.n_da - 1) / s.da_zmax;
+  P.inv_dzsci = (s.n_sci - 1) / s.sci_zmax;
+  P.rho_m = p.omega_m * kRhoCrit;
+  P.nm = s.n_lnm;
+  P.nz = s.n_z;
+  P.nda = s.n_da;
+  P.nsci = s.n_sci;
+  return P;
+}
+
+static int check_spec(const y3g_table_spec* s)
+{
+  if (s->n_lnm < 2 || s->n_z < 2 || s->n_lnm * s->n_z > kMaxHmf ||
+      s->n_da < 2 || s->n_da > kMaxDa || s->n_sci < 2 || s->n_sci > kMaxSci ||
+      !(s->lnm_hi > s->lnm_lo) || !(s->z_hi > s->z_lo) || !(s->da_zmax > 0) ||
+      !(s->sci_zmax > 0))
+    return fail(Y3G_ERR_ARG, "table spec out of range (n_lnm*n_z <= 6400, "
+                             "n_da <= 512, n_sci <= 128)");
+  return 0;
+}
+
+static int alloc_tables(y3g_runtime* rt, const y3g_table_spec* spec, int n)
+{
+  dfree(rt->points);
+  dfree(rt->hmf64), dfree(rt->da64), dfree(rt->sci64);
+  dfree(rt->hmf32), dfree(rt->da32), dfree(rt->sci32);
+  rt->n_points = 0;
+  rt->spec = *spec;
+  rt->hmf_stride = spec->n_lnm * spec->n_z;
+  rt->da_stride = spec->n_da;
+  rt->sci_stride = spec->n_sci;
+  if (dalloc(&rt->points, n) || dalloc(&rt->hmf64, (size_t)n * rt->hmf_stride) ||
+      dalloc(&rt->hmf32, (size_t)n * rt->hmf_stride) ||
+      dalloc(&rt->da64, (size_t)n * rt->da_stride) ||
+      dalloc(&rt->da32, (size_t)n * rt->da_stride) ||
+      dalloc(&rt->sci64, (size_t)n * rt->sci_stride) ||
+      dalloc(&rt->sci32, (size_t)n * rt->sci_stride))
+    return Y3G_ERR_MEMORY;
+  rt->n_points = n;
+  return 0;
+}
+
+int y3g_tables_build(y3g_runtime* rt, const y3g_params* params,
+                     const y3g_table_spec* spec, int n_points)
+{
+  if (!rt || !params || !spec || n_points <= 0)
+    return fail(Y3G_ERR_ARG, "y3g_tables_build: bad arguments");
+  if (int rc = check_spec(spec)) return rc;
+  HIPCHK(hipSetDevice(rt->device));
+  if (rt->n_points != n_points || std::memcmp(&rt->spec, spec, sizeof(*spec))) {
+    if (int rc = alloc_tables(rt, spec, n_points)) return rc;
+  }
+  rt->hpoints.resize(n_points);
+  for (int k = 0; k < n_points; ++k) rt->hpoints[k] = make_point(params[k], *spec);
+  HIPCHK(hipMemcpyAsync(rt->points, rt->hpoints.data(),
+                        sizeof(PointDev) * n_points, hipMemcpyHostToDevice,
+                        rt->stream));
+  TableArgs A;
+  A.points = rt->points;
+  A.spec = *spec;
+  A.hmf64 = rt->hmf64, A.hmf32 = rt->hmf32, A.hmf_stride = rt->hmf_stride;
+  A.da64 = rt->da64, A.da32 = rt->da32, A.da_stride = rt->da_stride;
+  A.sci64 = rt->sci64, A.sci32 = rt->sci32, A.sci_stride = rt->sci_stride;
+  HIPCHK(launch_tables(A, n_points, rt->stream));
+  HIPCHK(hipStreamSynchronize(rt->stream));
+  return 0;
+}
+
+int y3g_tables_upload(y3g_runtime* rt, int point, const double* hmf,
+                      const double* da, const double* sci)
+{
+  if (!rt || point < 0 || point >= rt->n_points)
+    return fail(Y3G_ERR_ARG, "y3g_tables_upload: no such point");
+  HIPCHK(hipSetDevice(rt->device));
+  auto up = [&](const double* h, double* d64, float* d32, int n) -> int {
+    if (!h) return 0;
+    std::vector<float> f(h, h + n);
+    HIPCHK(hipMemcpy(d64 + (size_t)point * n, h, sizeof(double) * n,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d32 + (size_t)point * n, f.data(), sizeof(float) * n,
+                     hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (int rc = up(hmf, rt->hmf64, rt->hmf32, rt->hmf_stride)) return rc;
+  if (int rc = up(da, rt->da64, rt->da32, rt->da_stride)) return rc;
+  if (int rc = up(sci, rt->sci64, rt->sci32, rt->sci_stride)) return rc;
+  return 0;
+}
+
+int y3g_tables_download(y3g_runtime* rt, int point, double* hmf, double* da,
+                        double* sci)
+{
+  if (!rt || point < 0 || point >= rt->n_points)
+    return fail(Y3G_ERR_ARG, "y3g_tables_download: no such point");
+  HIPCHK(hipSetDevice(rt->device));
+  if (hmf)
+    HIPCHK(hipMemcpy(hmf, rt->hmf64 + (size_t)point * rt->hmf_stride,
+                     sizeof(double) * rt->hmf_stride, hipMemcpyDeviceToHost));
+  if (da)
+    HIPCHK(hipMemcpy(da, rt->da64 + (size_t)point * rt->da_stride,
+                     sizeof(double) * rt->da_stride, hipMemcpyDeviceToHost));
+  if (sci)
+    HIPCHK(hipMemcpy(sci, rt->sci64 + (size_t)point * rt->sci_stride,
+                     sizeof(double) * rt->sci_stride, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+static int ensure_batch(y3g_runtime* rt, size_t n)
+{
+  if (n <= rt->batch_cap) return 0;
+  dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
+  rt->batch_cap = 0;
+  size_t const c = std::max<size_t>(n, 1024);
+  if (dalloc(&rt->d_igs, c) || dalloc(&rt->d_res, c) || dalloc(&rt->d_order, c))
+    return Y3G_ERR_MEMORY;
+  rt->batch_cap = c;
+  return 0;
+}
+
+static int ensure_slab(y3g_runtime* rt, int slots, int cap)
+{
+  if (slots <= rt->slab_slots && cap == rt->slab_cap) return 0;
+  dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
+    dfree(rt->slab_err), dfree(rt->slab_ax);
+  rt->slab_slots = rt->slab_cap = 0;
+  size_t const n = (size_t)slots * cap;
+  if (dalloc(&rt->slab_lo, n * kMaxDim) || dalloc(&rt->slab_hi, n * kMaxDim) ||
+      dalloc(&rt->slab_est, n) || dalloc(&rt->slab_err, n) ||
+      dalloc(&rt->slab_ax, n))
+    return Y3G_ERR_MEMORY;
+  rt->slab_slots = slots;
+  rt->slab_cap = cap;
+  return 0;
+}
+
+// Largest expected cost first: dimension, then tighter tolerance.
+static void lpt_order(const y3g_integral* igs, const std::vector<int>& ids,
+                      std::vector<int>& order)
+{
+  order = ids;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (igs[a].ndim != igs[b].ndim) return igs[a].ndim > igs[b].ndim;
+    return igs[a].eps_rel < igs[b].eps_rel;
+  });
+}
+
+static int run_group(y3g_runtime* rt, const y3g_integral* igs,
+                     const std::vector<int>& ids, const y3g_options& o,
+                     int algorithm, KernArgs A)
+{
+  if (ids.empty()) return 0;
+  std::vector<int> order;
+  lpt_order(igs, ids, order);
+  HIPCHK(hipMemcpyAsync(rt->d_order, order.data(), sizeof(int) * order.size(),
+                        hipMemcpyHostToDevice, rt->stream));
+  HIPCHK(hipMemsetAsync(rt->d_queue, 0, sizeof(int), rt->stream));
+  A.n = (int)order.size();
+  A.order = rt->d_order;
+  int const block = o.block_threads == 256 ? 256 : 512;
+  int slots = o.slots;
+  if (slots <= 0) {
+    int per_cu = 1;
+    if (algorithm == Y3G_PAGANI) {
+      size_t const smem = cubature_smem(o.precision, block);
+      per_cu = std::max(1, (int)((160 * 1024) / (smem + 1024)));
+      per_cu = std::min(per_cu, 2048 / block);
+    } else {
+      per_cu = 2048 / block;
+    }
+    slots = rt->n_cu * per_cu;
+  }
+  if (o.single) slots = 1;
+  slots = std::max(1, std::min<int>(slots, (int)ids.size()));
+  if (algorithm == Y3G_PAGANI) {
+    if (int rc = ensure_slab(rt, std::max(slots, rt->slab_slots), A.cap))
+      return rc;
+    A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
+    A.slab_est = rt->slab_est, A.slab_err = rt->slab_err;
+    A.slab_ax = rt->slab_ax;
+    HIPCHK(launch_cubature(A, o.precision, block, slots, rt->stream));
+  } else {
+    HIPCHK(launch_vegas(A, o.precision, block, slots, rt->stream));
+  }
+  return 0;
+}
+
+int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                  const y3g_options* opt, y3g_result* out)
+{
+  if (!rt || (!igs && n) || (!out && n))
+    return fail(Y3G_ERR_ARG, "y3g_integrate: bad arguments");
+  y3g_options o;
+  y3g_options_default(&o);
+  if (opt) o = *opt;
+  if (n == 0) return 0;
+  if (rt->n_points <= 0)
+    return fail(Y3G_ERR_STATE, "y3g_integrate: no tables (call y3g_tables_build)");
+  if (o.max_regions < 64 || o.max_split_per_step < 1 ||
+      o.max_split_per_step > 1024 || o.vegas_nbins < 1 || o.vegas_nbins > 128 ||
+      o.vegas_calls < 2 || o.vegas_itmax < 1)
+    return fail(Y3G_ERR_ARG, "y3g_integrate: options out of range");
+  std::vector<int> cub, veg;
+  for (size_t i = 0; i < n; ++i) {
+    y3g_integral const& g = igs[i];
+    if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
+        g.point >= rt->n_points || g.kind < 0 || g.kind > 2 ||
+        !(g.eps_rel >= 0) || g.max_eval < 1)
+      return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
+                                 " is malformed");
+    for (int a = 0; a < g.ndim; ++a)
+      if (!(g.highs[a] > g.lows[a]))
+        return fail(Y3G_ERR_ARG, "y3g_integrate: empty volume in integral " +
+                                   std::to_string(i));
+    (g.algorithm == Y3G_MCUBES ? veg : cub).push_back((int)i);
+  }
+  HIPCHK(hipSetDevice(rt->device));
+  if (int rc = ensure_batch(rt, n)) return rc;
+  HIPCHK(hipMemcpyAsync(rt->d_igs, igs, sizeof(y3g_integral) * n,
+                        hipMemcpyHostToDevice, rt->stream));
+  HIPCHK(hipMemsetAsync(rt->d_nevals, 0, sizeof(unsigned long long), rt->stream));
+  KernArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.igs = rt->d_igs;
+  A.points = rt->points;
+  A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
+  A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
+  A.sci32 = rt->sci32, A.sci64 = rt->sci64, A.sci_stride = rt->sci_stride;
+  A.cap = o.max_regions;
+  A.kmax = o.max_split_per_step;
+  A.init_split = o.init_split;
+  A.vegas_calls = o.vegas_calls;
+  A.vegas_itmax = o.vegas_itmax;
+  A.vegas_itskip = o.vegas_itskip;
+  A.vegas_nbins = o.vegas_nbins;
+  A.vegas_alpha = o.vegas_alpha;
+  A.queue = rt->d_queue;
+  A.out = rt->d_res;
+  A.nevals = rt->d_nevals;
+  HIPCHK(hipEventRecord(rt->ev0, rt->stream));
+  if (o.single) {
+    // plain path: one integral per launch, serial over integrals
+    for (size_t i = 0; i < n; ++i) {
+      std::vector<int> one{(int)i};
+      if (int rc = run_group(rt, igs, one, o, igs[i].algorithm, A)) return rc;
+    }
+  } else {
+    if (int rc = run_group(rt, igs, cub, o, Y3G_PAGANI, A)) return rc;
+    if (int rc = run_group(rt, igs, veg, o, Y3G_MCUBES, A)) return rc;
+  }
+  HIPCHK(hipEventRecord(rt->ev1, rt->stream));
+  HIPCHK(hipMemcpyAsync(out, rt->d_res, sizeof(y3g_result) * n,
+                        hipMemcpyDeviceToHost, rt->stream));
+  unsigned long long nev = 0;
+  HIPCHK(hipMemcpyAsync(&nev, rt->d_nevals, sizeof(nev), hipMemcpyDeviceToHost,
+                        rt->stream));
+  HIPCHK(hipStreamSynchronize(rt->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
+  rt->last_ms = ms;
+  rt->last_nevals = (long long)nev;
+  return 0;
+}
+
+int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals)
+{
+  if (!rt) return fail(Y3G_ERR_ARG, "null runtime");
+  if (kernel_ms) *kernel_ms = rt->last_ms;
+  if (nevals) *nevals = rt->last_nevals;
+  return 0;
+}
+
+int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
+                     const double* in, int nin, size_t n, const double* par,
+                     int npar, const y3g_integral* aux, double* out)
+{
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INTEGRAND)
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(rt->device));
+  bool const needs_tables = which == Y3G_F_DV_DO_DZ || which == Y3G_F_HMF ||
+                            which == Y3G_F_MOR || which == Y3G_F_LO_LT_PDF ||
+                            which == Y3G_F_LO_LT_BIN || which == Y3G_F_SCI ||
+                            which == Y3G_F_INTEGRAND;
+  if (needs_tables && rt->n_points <= 0)
+    return fail(Y3G_ERR_STATE, "y3g_eval_functor: no tables");
+  if (which == Y3G_F_INTEGRAND && (!aux || aux->ndim != nin))
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: integrand needs aux integral");
+  int const pt = (which == Y3G_F_INTEGRAND) ? aux->point : 0;
+  if (needs_tables && (pt < 0 || pt >= rt->n_points))
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad point");
+  double *d_in = nullptr, *d_out = nullptr, *d_par = nullptr;
+  y3g_integral* d_ig = nullptr;
+  int rc = 0;
+  if ((rc = dalloc(&d_in, n * nin)) || (rc = dalloc(&d_out, n)) ||
+      (rc = dalloc(&d_par, npar > 0 ? npar : 1)) || (rc = dalloc(&d_ig, 1))) {
+    dfree(d_in), dfree(d_out), dfree(d_par), dfree(d_ig);
+    return rc;
+  }
+  auto cleanup = [&]() { dfree(d_in), dfree(d_out), dfree(d_par), dfree(d_ig); };
+  if (hipMemcpy(d_in, in, sizeof(double) * n * nin, hipMemcpyHostToDevice) ||
+      (npar > 0 &&
+       hipMemcpy(d_par, par, sizeof(double) * npar, hipMemcpyHostToDevice)) ||
+      (aux && hipMemcpy(d_ig, aux, sizeof(*aux), hipMemcpyHostToDevice))) {
+    cleanup();
+    return fail(Y3G_ERR_HIP, "y3g_eval_functor: copy in");
+  }
+  FunctorArgs F;
+  F.which = which;
+  F.in = d_in;
+  F.nin = nin;
+  F.n = n;
+  F.par = d_par;
+  F.points = rt->points ? rt->points + (needs_tables ? pt : 0) : nullptr;
+  bool const f64 = precision == Y3G_FP64;
+  if (rt->n_points > 0) {
+    F.hmf = f64 ? (const void*)(rt->hmf64 + (size_t)pt * rt->hmf_stride)
+                : (const void*)(rt->hmf32 + (size_t)pt * rt->hmf_stride);
+    F.da = f64 ? (const void*)(rt->da64 + (size_t)pt * rt->da_stride)
+               : (const void*)(rt->da32 + (size_t)pt * rt->da_stride);
+    F.sci = f64 ? (const void*)(rt->sci64 + (size_t)pt * rt->sci_stride)
+                : (const void*)(rt->sci32 + (size_t)pt * rt->sci_stride);
+  } else {
+    F.hmf = F.da = F.sci = nullptr;
+  }
+  F.ig = d_ig;
+  F.out = d_out;
+  hipError_t e = launch_functor(F, precision, rt->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess)
+    return fail(Y3G_ERR_HIP, std::string("y3g_eval_functor: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"
