@@ -121,6 +121,9 @@ def _load():
                                      _D, _D, _D, _D, _D, C.POINTER(Integral)]),
         "y3o_gk21_adapt": (_D, [_FN1, C.c_void_p, _D, _D, _D, _D, _PD]),
         "y3o_gl_rule": (None, [C.c_int, _PD, _PD]),
+        "y3o_gk21_nodes": (_PD, []),
+        "y3o_gk21_wk": (_PD, []),
+        "y3o_gk21_wg": (_PD, []),
         "y3o_golden": (C.c_int, [C.POINTER(Params), C.POINTER(Tables), C.c_int, _PD, C.c_int,
                                  _PD, C.c_int, _PD, C.c_int, _PD, _PD]),
         "y3o_golden_set_orders": (None, [C.c_int, C.c_int]),

@@ -38,6 +38,8 @@
 #define Y3O_LN_MPIVOT_HMF 32.236191301916641 /* ln 1e14 */
 #define Y3O_LN_MPIVOT_MOR 33.334803590584751 /* ln 3e14 */
 
+double y3o_erfcx(double u);
+
 /* ---------------- primitives (primitives.hh:60-65) ---------------------- */
 double
 y3o_gaussian(double x, double mu, double sigma)
@@ -199,7 +201,7 @@ y3o_lc_lt_eq15(double lc, double lt, double mu, double sigma, double tau,
 }
 
 /* ---------------- erfcx (FP64) ------------------------------------------ */
-static double
+double
 y3o_erfcx(double u)
 {
   if (u < 10.0) return exp(u * u) * erfc(u);
@@ -243,6 +245,42 @@ y3o_emg_cdf(double x, double mu, double sigma, double tau)
     tail = 0.5 * exp(-tau * (x - mu) + 0.5 * tau * tau * sigma * sigma) *
            erfc(u);
   return phi - tail;
+}
+
+/* Half the difference erf(b) - erf(a) (b >= a) without cancellation: the
+ * complementary function on either tail. */
+static double
+half_erf_diff(double a, double b)
+{
+  if (a > 0.0) return 0.5 * (erfc(a) - erfc(b));
+  if (b < 0.0) return 0.5 * (erfc(-b) - erfc(-a));
+  return 0.5 * (erf(b) - erf(a));
+}
+
+/* 1 - F_EMG(x) = Phi(-z) + exp(A) Phi(z - tau sigma), small when F ~ 1. */
+static double
+emg_sf(double x, double mu, double sigma, double tau)
+{
+  double const z = (x - mu) / sigma;
+  double const q = 0.5 * erfc(z / Y3O_SQRT2);
+  double const u = (tau * sigma - z) / Y3O_SQRT2;
+  double tail;
+  if (u >= 0.0)
+    tail = 0.5 * exp(-0.5 * z * z) * y3o_erfcx(u);
+  else
+    tail = 0.5 * exp(-tau * (x - mu) + 0.5 * tau * tau * sigma * sigma) *
+           erfc(u);
+  return q + tail;
+}
+
+/* F(hi) - F(lo) of the EMG, from the survival function when F(lo) > 1/2. */
+static double
+emg_cdf_diff(double lo, double hi, double mu, double sigma, double tau)
+{
+  double const flo = y3o_emg_cdf(lo, mu, sigma, tau);
+  if (flo > 0.5)
+    return emg_sf(lo, mu, sigma, tau) - emg_sf(hi, mu, sigma, tau);
+  return y3o_emg_cdf(hi, mu, sigma, tau) - flo;
 }
 
 /* ---------------- synthetic-workload functors --------------------------- */
@@ -314,11 +352,10 @@ y3o_lo_lt_bin(const y3o_params* p, double lo_lo, double lo_hi, double lt)
 {
   double const sigma = sqrt(lt);
   double const base = Y3O_SQRT2 * sigma;
-  double const g = 0.5 * (erf((lo_hi - lt) / base) - erf((lo_lo - lt) / base));
+  double const g = half_erf_diff((lo_lo - lt) / base, (lo_hi - lt) / base);
   if (p->f_prj == 0.0) return g;
   double const tau = 1.0 / (p->prj_scale * lt);
-  double const e =
-    y3o_emg_cdf(lo_hi, lt, sigma, tau) - y3o_emg_cdf(lo_lo, lt, sigma, tau);
+  double const e = emg_cdf_diff(lo_lo, lo_hi, lt, sigma, tau);
   return (1.0 - p->f_prj) * g + p->f_prj * e;
 }
 
@@ -334,7 +371,7 @@ double
 y3o_zo_bin(const y3o_params* p, double zo_lo, double zo_hi, double zt)
 {
   double const base = Y3O_SQRT2 * p->sigz0 * (1.0 + zt);
-  return 0.5 * (erf((zo_hi - zt) / base) - erf((zo_lo - zt) / base));
+  return half_erf_diff((zo_lo - zt) / base, (zo_hi - zt) / base);
 }
 
 /* Gauss-Legendre 8-point nodes on [-1, 1] (positive half). */

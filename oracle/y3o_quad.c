@@ -963,8 +963,11 @@ y3o_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
  *   Importance grid: nbins per axis, Lepage refinement with smoothing and
  *   damping exponent alpha after every iteration.  Iterations < itskip only
  *   adapt; later ones are combined with inverse-variance weights; the
- *   reported error is inflated by sqrt(chi2/dof) when that exceeds 1.
+ *   reported error is Y3O_VEGAS_NSIG standard deviations, inflated by
+ *   sqrt(chi2/dof) when that exceeds 1, so that it bounds the true error
+ *   with ~99.7% probability (a 1-sigma estimate does not bound it).
  */
+#define Y3O_VEGAS_NSIG 3.0
 static void
 vegas_refine(double* xg, const double* hist, int nb, double alpha)
 {
@@ -1083,7 +1086,7 @@ y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
       sig = sqrt(1.0 / wsum);
       double const dof = nacc - 1;
       double const chi = dof > 0 ? chisum / dof : 0.0;
-      double const sigr = chi > 1.0 ? sig * sqrt(chi) : sig;
+      double const sigr = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig);
       if (sigr <= fmax(o->eps_abs, o->eps_rel * fabs(I))) {
         status = 0;
         break;
@@ -1097,7 +1100,7 @@ y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
   double const dof = nacc - 1;
   double const chi = dof > 0 ? chisum / dof : 0.0;
   res->value = I;
-  res->error = chi > 1.0 ? sig * sqrt(chi) : sig;
+  res->error = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig);
   res->prob = chi;
   res->neval = neval;
   res->nregions = 0;

@@ -1,0 +1,268 @@
+"""GPU parity: the HIP path (through the C ABI of include/y3g.h) against the
+oracle, the reference's fixtures and the golden values.
+
+Tolerances: FP64 functors 1e-12 relative to the oracle (1e-6 / 1e-3 for the
+reference fixtures, as in its Catch2 tests); FP32 functors 2e-5 relative;
+integrals within their requested eps_rel of the golden value with a
+reported error that bounds the true error; FP64 integrals equal to the
+oracle's CPU restatement of the same algorithm to 1e-11.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import y3_cluster_cpp_amd as y3
+from conftest import GOLDEN, load_columns
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "golden_fiducial.json")) as f:
+    G = json.load(f)
+LO, ZO, RADII = G["lo_edges"], G["zo_edges"], G["radii"]
+
+
+def fid(rt, wide=True):
+    p = y3.default_params() if wide else y3.default_params(f_prj=0.0)
+    s = y3.spec_wide() if wide else y3.spec_config0()
+    rt.tables_build(p, s)
+    return p, s
+
+
+def test_device_info(runtime):
+    info = runtime.device_info()
+    assert info["name"].startswith("gfx950")
+    assert info["n_cu"] > 0
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_tables_match_oracle(runtime, wide):
+    p, s = fid(runtime, wide)
+    hmf, da, sci = runtime.tables_download(0, s)
+    op = O.default_params() if wide else O.default_params(f_prj=0.0)
+    ot = O.OracleTables(op, O.spec_wide() if wide else O.spec_config0())
+    assert np.max(np.abs(hmf / ot.hmf - 1)) < 1e-12
+    assert np.max(np.abs(da[1:] / ot.da[1:] - 1)) < 1e-13 and da[0] == 0.0
+    m = ot.sci > 0
+    assert np.max(np.abs(sci[m] / ot.sci[m] - 1)) < 1e-12
+
+
+def _fixture_ok(v, ref, prec):
+    """FP64: the reference tests' own tolerance (Approx epsilon 1e-6,
+    margin 1e-12).  FP32: 1e-4 relative plus 1e-7 of the largest value (the
+    FP32 argument rounding of far Gaussian tails)."""
+    if prec == y3.FP64:
+        return np.abs(v - ref) <= np.maximum(1e-12, 1e-6 * np.abs(ref))
+    return np.abs(v - ref) <= 1e-4 * np.abs(ref) + 1e-7 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec", [y3.FP64, y3.FP32SPLIT])
+def test_reference_fixtures_on_device(runtime, prec):
+    # roffset_t.test.cc, lo_lc_t.test.cc, zo_zt_des_t.test.cc,
+    # int_zo_zt_des_t.test.cc evaluated by the device functors
+    d = load_columns("test_Rmis.txt", 2)
+    v = runtime.eval_functor(y3.F_ROFFSET_GAMMA, d[:, :1], [0.15], precision=prec)
+    assert np.all(_fixture_ok(v, d[:, 1], prec))
+    d = load_columns("test_lolc_t.txt", 2)
+    v = runtime.eval_functor(y3.F_LO_LC, d[:, :3], [1.66, 0.26, 1.43, 1.0], precision=prec)
+    assert np.all(_fixture_ok(v * d[:, 1], d[:, 3], prec))
+    d = load_columns("test_zo_zt_des.txt", 2)
+    v = runtime.eval_functor(y3.F_SIGMA_PHOTOZ, d[:, 1:2], precision=prec)
+    assert np.all(_fixture_ok(v, d[:, 2], prec))
+    v = runtime.eval_functor(y3.F_ZO_ZT_DES, d[:, :2], precision=prec)
+    assert np.all(_fixture_ok(v, d[:, 3], prec))
+    d = load_columns("test_int_zo_zt_des.txt", 2)
+    v = runtime.eval_functor(y3.F_INT_ZO_ZT_DES, d[:, :3], precision=prec)
+    assert np.all(_fixture_ok(v, d[:, 4], prec))
+
+
+def test_dv_do_dz_fixture_on_device(runtime):
+    # dv_do_dz_t.test.cc: the reference D_A table uploaded as the table of a
+    # point with Omega_m = 0.3, h = 0.7 (uniform 500-point grid on [0, 1])
+    z = load_columns("z_da_test.txt")[:, 0]
+    da = load_columns("d_a_test.txt")[:, 0]
+    assert np.allclose(np.diff(z), z[1] - z[0], rtol=1e-9)
+    s = y3.spec_wide()
+    s.n_da = len(z)
+    s.da_zmax = float(z[-1])
+    p = y3.default_params()
+    runtime.tables_build(p, s)
+    runtime.tables_upload(0, da=da)
+    d = load_columns("test_dv_do_dz.txt")
+    for prec in (y3.FP64, y3.FP32SPLIT):
+        v = runtime.eval_functor(y3.F_DV_DO_DZ, d[1:, :1], precision=prec)
+        assert np.all(np.abs(v - d[1:, 1]) <= 1e-3 * np.abs(d[1:, 1]) + 1e-12)
+
+
+def _oracle_fid():
+    op = O.default_params()
+    return op, O.OracleTables(op, O.spec_wide())
+
+
+@pytest.mark.parametrize("prec,tol", [(y3.FP64, 1e-12), (y3.FP32SPLIT, 2e-5)])
+def test_functors_match_oracle(runtime, prec, tol):
+    p, s = fid(runtime)
+    op, ot = _oracle_fid()
+    rng = np.random.default_rng(7)
+    n = 400
+    lnM = rng.uniform(s.lnm_lo, s.lnm_hi, n)
+    zt = rng.uniform(0.1, 0.75, n)
+    lt = np.exp(rng.uniform(0.5, 5.5, n))
+    lo = rng.uniform(20, 300, n)
+    cases = [
+        (y3.F_HMF, np.c_[lnM, zt], lambda r: O.lib.y3o_hmf(C.byref(ot.t), r[0], r[1])),
+        (y3.F_MOR, np.c_[lt, lnM, zt], lambda r: O.lib.y3o_mor_lognormal(C.byref(op), *r)),
+        (y3.F_DV_DO_DZ, np.c_[zt], lambda r: O.lib.y3o_dvdz(C.byref(op), C.byref(ot.t), r[0])),
+        (y3.F_SCI, np.c_[zt], lambda r: O.lib.y3o_sci(C.byref(ot.t), r[0])),
+        (y3.F_LO_LT_PDF, np.c_[lo, lt], lambda r: O.lib.y3o_lo_lt_pdf(C.byref(op), *r)),
+        (y3.F_LO_LT_BIN, np.c_[np.full(n, 20.), np.full(n, 30.), lt],
+         lambda r: O.lib.y3o_lo_lt_bin(C.byref(op), *r)),
+    ]
+    for which, x, ref in cases:
+        v = runtime.eval_functor(which, x, precision=prec)
+        r = np.array([ref(row) for row in x])
+        top = np.max(np.abs(r))
+        # relative tolerance on values above 1e-8 (FP64) / 1e-4 (FP32) of
+        # the largest, absolute below (far tails)
+        cut = (1e-8 if prec == y3.FP64 else 1e-4) * top
+        big = np.abs(r) > cut
+        rel = np.abs(v[big] / r[big] - 1)
+        assert np.all(rel <= tol), (which, rel.max())
+        assert np.all(np.abs(v[~big] - r[~big]) <= tol * cut + 1e-300), which
+
+
+@pytest.mark.parametrize("prec,tol", [(y3.FP64, 1e-11), (y3.FP32SPLIT, 5e-5)])
+def test_nfw_functors_match_oracle(runtime, prec, tol):
+    rng = np.random.default_rng(11)
+    n = 300
+    rho = 0.3 * 2.77533742639e11
+    R = np.exp(rng.uniform(np.log(0.05), np.log(40), n))
+    lnM = rng.uniform(30, 35.6, n)
+    rmis = rng.uniform(0, 2, n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    v = runtime.eval_functor(y3.F_NFW_DSIGMA, np.c_[R, lnM, np.full(n, rho), np.zeros(n)], precision=prec)
+    r = np.array([O.lib.y3o_nfw_dsigma(a, b, rho, 0.0) for a, b in zip(R, lnM)])
+    assert np.max(np.abs(v / r - 1)) < tol
+    v = runtime.eval_functor(y3.F_DSIGMA_OFFSET, np.c_[R, rmis, th, lnM, np.full(n, rho), np.zeros(n)],
+                             precision=prec)
+    r = np.array([O.lib.y3o_dsigma_offset(a, b, c, d, rho, 0.0) for a, b, c, d in zip(R, rmis, th, lnM)])
+    scale = np.array([O.lib.y3o_nfw_dsigma(a, d, rho, 0.0) for a, d in zip(R, lnM)])
+    assert np.max(np.abs(v - r) / np.maximum(np.abs(r), 1e-3 * np.abs(scale))) < 10 * tol
+
+
+@pytest.mark.parametrize("kind", [y3.COUNT3D, y3.COUNT5D, y3.DSIGMA6D])
+@pytest.mark.parametrize("prec,tol", [(y3.FP64, 1e-12), (y3.FP32SPLIT, 5e-5)])
+def test_mapped_integrand_matches_oracle(runtime, kind, prec, tol):
+    wide = kind != y3.COUNT3D
+    p, s = fid(runtime, wide)
+    op = O.default_params() if wide else O.default_params(f_prj=0.0)
+    ot = O.OracleTables(op, O.spec_wide() if wide else O.spec_config0())
+    ig = y3.make_integral(p, s, kind, 20., 30., 0.2, 0.35, 1.0)
+    oig = O.make_integral(op, O.spec_wide() if wide else O.spec_config0(), kind, 20., 30., 0.2, 0.35, 1.0)
+    rng = np.random.default_rng(kind)
+    Y = np.array([[rng.uniform(ig.lows[k], ig.highs[k]) for k in range(ig.ndim)] for _ in range(500)])
+    v = runtime.eval_functor(y3.F_INTEGRAND, Y, precision=prec, aux=ig)
+    r = np.array([O.lib.y3o_integrand_mapped(C.byref(op), C.byref(ot.t), C.byref(oig),
+                                             O.ptr(np.ascontiguousarray(y))) for y in Y])
+    m = np.abs(r) > 1e-6 * np.max(np.abs(r))
+    assert np.max(np.abs(v[m] / r[m] - 1)) < tol
+    assert np.max(np.abs(v - r)) < 1e-4 * np.max(np.abs(r))
+
+
+def _check(res, gold, eps):
+    rel = np.abs(res["value"] - gold) / np.abs(gold)
+    assert np.all(res["status"] == 0), res["status"]
+    assert np.all(rel <= eps), rel.max()
+    assert np.all(res["error"] >= np.abs(res["value"] - gold)), "error estimate below true error"
+
+
+@pytest.mark.parametrize("prec", [y3.FP64, y3.FP32SPLIT])
+@pytest.mark.parametrize("algo", [y3.PAGANI, y3.MCUBES])
+@pytest.mark.parametrize("eps", [1e-3, 1e-4])
+def test_config0_integral(runtime, prec, algo, eps):
+    p, s = fid(runtime, wide=False)
+    ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=eps, algorithm=algo)
+    res = runtime.integrate([ig], y3.options(precision=prec, vegas_calls=6_000_000, vegas_itmax=5))
+    _check(res, G["config0"]["value"], eps)
+
+
+def test_config0_fp64_equals_cpu_restatement(runtime):
+    p, s = fid(runtime, wide=False)
+    op = O.default_params(f_prj=0.0)
+    ot = O.OracleTables(op, O.spec_config0())
+    oig = O.make_integral(op, O.spec_config0(), O.COUNT3D)
+    for eps in (1e-3, 1e-5):
+        ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=eps)
+        g = runtime.integrate([ig], y3.options(precision=y3.FP64))[0]
+        c = O.cubature(op, ot, oig, O.cub_opts(eps_rel=eps, max_regions=262144, max_eval=1 << 40))
+        assert (g["nregions"], g["neval"]) == (c.nregions, c.neval), (eps, g, c.value, c.nregions)
+        assert abs(g["value"] / c.value - 1) < 1e-11
+    ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=1e-4, algorithm=y3.MCUBES)
+    g = runtime.integrate([ig], y3.options(precision=y3.FP64, vegas_calls=200000, vegas_itmax=5))[0]
+    c = O.vegas(op, ot, oig, O.vegas_opts(eps_rel=1e-4, calls_per_iter=200000, itmax=5,
+                                          seed=0x59334331))
+    assert g["neval"] == c.neval
+    assert abs(g["value"] / c.value - 1) < 1e-11
+
+
+@pytest.mark.parametrize("prec", [y3.FP64, y3.FP32SPLIT])
+def test_config1_all_bins(runtime, prec):
+    p, s = fid(runtime)
+    igs = [y3.make_integral(p, s, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-3)
+           for i in range(4) for j in range(3)]
+    res = runtime.integrate(igs, y3.options(precision=prec))
+    _check(res, np.array(G["config1"]["values"]).reshape(-1), 1e-3)
+
+
+def test_config1_tight(runtime):
+    p, s = fid(runtime)
+    gold = np.array(G["config1"]["values"])
+    igs = [y3.make_integral(p, s, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-4)
+           for (i, j) in [(0, 0), (1, 1), (2, 2)]]
+    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT))
+    _check(res, np.array([gold[0, 0], gold[1, 1], gold[2, 2]]), 1e-4)
+
+
+def test_config2_subset(runtime):
+    p, s = fid(runtime)
+    gold = np.array(G["config2"]["values"])
+    sel = [(0, 0, 3), (1, 1, 7), (3, 2, 12), (2, 0, 0)]
+    igs = [y3.make_integral(p, s, y3.DSIGMA6D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], RADII[k],
+                            eps_rel=1e-3) for (i, j, k) in sel]
+    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT))
+    _check(res, np.array([gold[i, j, k] for (i, j, k) in sel]), 1e-3)
+
+
+def test_results_independent_of_batch(runtime):
+    p, s = fid(runtime)
+    ig0 = y3.make_integral(p, s, y3.COUNT5D, LO[1], LO[2], ZO[1], ZO[2], eps_rel=1e-3)
+    alone = runtime.integrate([ig0], y3.options(precision=y3.FP32SPLIT))[0]
+    mix = [y3.make_integral(p, s, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1],
+                            eps_rel=1e-3 if (i + j) % 2 else 3e-4)
+           for i in range(4) for j in range(3)] * 3
+    mix[17] = ig0
+    res = runtime.integrate(mix, y3.options(precision=y3.FP32SPLIT))
+    assert res[17]["value"] == alone["value"] and res[17]["error"] == alone["error"]
+    assert res[17]["nregions"] == alone["nregions"]
+
+
+def test_error_paths(runtime):
+    p, s = fid(runtime)
+    ig = y3.make_integral(p, s, y3.COUNT5D, 20., 30., 0.2, 0.35)
+    bad = y3.make_integral(p, s, y3.COUNT5D, 20., 30., 0.2, 0.35)
+    bad.point = 5
+    with pytest.raises(RuntimeError, match="malformed"):
+        runtime.integrate([ig, bad])
+    bad = y3.make_integral(p, s, y3.COUNT5D, 20., 30., 0.2, 0.35)
+    bad.highs[0] = bad.lows[0]
+    with pytest.raises(RuntimeError, match="empty volume"):
+        runtime.integrate([bad])
+    with pytest.raises(RuntimeError, match="options"):
+        runtime.integrate([ig], y3.options(max_split_per_step=5000))
+    # a tiny evaluation budget ends with status 1 and an honest error
+    ig.max_eval = 20000
+    r = runtime.integrate([ig])[0]
+    assert r["status"] == 1 and r["neval"] <= 20000

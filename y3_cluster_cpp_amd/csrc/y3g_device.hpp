@@ -180,6 +180,30 @@ template <typename T> Y3G_HD T emg_cdf(T x, T mu, T sigma, T tau)
   return phi - tail;
 }
 
+// 1 - F_EMG(x) = Phi(-z) + exp(A) Phi(z - tau sigma).
+template <typename T> Y3G_HD T emg_sf(T x, T mu, T sigma, T tau)
+{
+  T const z = (x - mu) / sigma;
+  T const q = T(0.5) * M<T>::erfc(z / T(1.4142135623730951));
+  T const u = (tau * sigma - z) / T(1.4142135623730951);
+  T tail;
+  if (u >= T(0))
+    tail = T(0.5) * M<T>::exp(T(-0.5) * z * z) * M<T>::erfcx(u);
+  else
+    tail = T(0.5) *
+           M<T>::exp(-tau * (x - mu) + T(0.5) * tau * tau * sigma * sigma) *
+           M<T>::erfc(u);
+  return q + tail;
+}
+
+// F(hi) - F(lo) of the EMG, from the survival function when F(lo) > 1/2.
+template <typename T> Y3G_HD T emg_cdf_diff(T lo, T hi, T mu, T sigma, T tau)
+{
+  T const flo = emg_cdf<T>(lo, mu, sigma, tau);
+  if (flo > T(0.5)) return emg_sf<T>(lo, mu, sigma, tau) - emg_sf<T>(hi, mu, sigma, tau);
+  return emg_cdf<T>(hi, mu, sigma, tau) - flo;
+}
+
 // P(lo | lt) = (1 - f_prj) N(lt, sqrt lt) + f_prj EMG(lt, sqrt lt, 1/(s lt)).
 template <typename T> Y3G_HD T lo_lt_pdf(T lo, T lt, T fprj, T pscale)
 {
@@ -198,7 +222,7 @@ template <typename T> Y3G_HD T lo_lt_bin(T lo_lo, T lo_hi, T lt, T fprj, T pscal
   T const g = half_erf_diff<T>((lo_lo - lt) / base, (lo_hi - lt) / base);
   if (fprj == T(0)) return g;
   T const tau = T(1) / (pscale * lt);
-  T const e = emg_cdf<T>(lo_hi, lt, sigma, tau) - emg_cdf<T>(lo_lo, lt, sigma, tau);
+  T const e = emg_cdf_diff<T>(lo_lo, lo_hi, lt, sigma, tau);
   return (T(1) - fprj) * g + fprj * e;
 }
 

@@ -550,10 +550,12 @@ __global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
 // p = max(2, calls/g^d) samples per cube, Philox4x32-10 with key = seed and
 // counter (cube, sample, iteration, block), 53-bit uniforms (FP64) or 24-bit
 // uniforms (FP32 path); Lepage refinement with 3-point smoothing and damping
-// alpha; iterations < itskip only adapt; inverse-variance combination; error
-// inflated by sqrt(chi2/dof) when that exceeds 1.
+// alpha; iterations < itskip only adapt; inverse-variance combination; the
+// reported error is kVegasNsig standard deviations (inflated by
+// sqrt(chi2/dof) when that exceeds 1) so that it bounds the true error.
 
 constexpr int kNbMax = 128;
+constexpr double kVegasNsig = 3.0;
 
 template <typename T, int BLOCK> struct VegShared {
   T hmf[kMaxHmf];
@@ -687,7 +689,7 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
         sig = sqrt(1.0 / wsum);
         double const dof = nacc - 1;
         double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
-        double const sigr = chi > 1.0 ? sig * sqrt(chi) : sig;
+        double const sigr = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
         if (sigr <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
           status = 0;
           break;
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
       y3g_result res;
       res.value = I;
-      res.error = chi > 1.0 ? sig * sqrt(chi) : sig;
+      res.error = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
       res.prob = chi;
       res.neval = neval;
       res.nregions = 0;

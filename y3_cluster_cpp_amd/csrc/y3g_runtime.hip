@@ -93,7 +93,7 @@ const char* y3g_version(void) { return "y3g 0.1 gfx950"; }
 void y3g_options_default(y3g_options* o)
 {
   o->precision = Y3G_FP32SPLIT;
-  o->max_regions = 65536;
+  o->max_regions = 262144;
   o->max_split_per_step = 1024;
   o->init_split = 0;
   o->block_threads = 512;
