@@ -173,6 +173,7 @@ def test_mapped_integrand_matches_oracle(runtime, kind, prec, tol):
 
 
 def _check(res, gold, eps):
+    gold = np.asarray(gold)
     rel = np.abs(res["value"] - gold) / np.abs(gold)
     assert np.all(res["status"] == 0), res["status"]
     assert np.all(rel <= eps), rel.max()
@@ -266,3 +267,18 @@ def test_error_paths(runtime):
     ig.max_eval = 20000
     r = runtime.integrate([ig])[0]
     assert r["status"] == 1 and r["neval"] <= 20000
+
+
+@pytest.mark.parametrize("prec", [y3.FP64, y3.FP32SPLIT])
+def test_plain_path_device_wide(runtime, prec):
+    # one integral per launch sequence (the plain path), same tolerance bar
+    p, s = fid(runtime)
+    gold1 = np.array(G["config1"]["values"])
+    gold2 = np.array(G["config2"]["values"])
+    igs = [y3.make_integral(p, s, y3.COUNT5D, LO[0], LO[1], ZO[0], ZO[1], eps_rel=1e-4),
+           y3.make_integral(p, s, y3.COUNT5D, LO[3], LO[4], ZO[2], ZO[3], eps_rel=1e-3),
+           y3.make_integral(p, s, y3.DSIGMA6D, LO[1], LO[2], ZO[1], ZO[2], RADII[9], eps_rel=1e-3)]
+    res = runtime.integrate(igs, y3.options(precision=prec, single=1))
+    _check(res, np.array([gold1[0, 0], gold1[3, 2], gold2[1, 1, 9]]), np.array([1e-4, 1e-3, 1e-3]))
+    ms, nev = runtime.last_timing()
+    assert nev == res["neval"].sum()

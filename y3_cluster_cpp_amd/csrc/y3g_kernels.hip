@@ -12,70 +12,9 @@
 // VEGAS grid refinement are FP64; the integrand runs in T.
 #include <hip/hip_runtime.h>
 
-#include "y3g_internal.hpp"
+#include "y3g_cub_common.hpp"
 
 namespace y3g {
-
-// ------------------------------------------------------------ reductions
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v)
-{
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-template <int BLOCK>
-__device__ double block_sum(double v, double* red)
-{
-  constexpr int NW = BLOCK / 64;
-  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  v = wave_sum(v);
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) s += red[i];
-  return s;
-}
-
-template <int BLOCK>
-__device__ void block_sum2(double& a, double& b, double* red)
-{
-  constexpr int NW = BLOCK / 64;
-  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  __syncthreads();
-  if (lane == 0) red[w] = a, red[NW + w] = b;
-  __syncthreads();
-  double sa = 0.0, sb = 0.0;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) sa += red[i], sb += red[NW + i];
-  a = sa;
-  b = sb;
-}
-
-template <int BLOCK>
-__device__ int block_sum_i(int v, int* red)
-{
-  constexpr int NW = BLOCK / 64;
-  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  v = wave_sum_i(v);
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) s += red[i];
-  return s;
-}
 
 // --------------------------------------------------------- table builder
 __constant__ double c_gl8_x[4] = {0.1834346424956498, 0.5255324099163290,
@@ -148,186 +87,6 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
     sci64[i] = v;
     sci32[i] = float(v);
   }
-}
-
-// Table pointers of precision T.
-template <typename T> struct TabSel;
-template <> struct TabSel<float> {
-  static __device__ const float* hmf(const KernArgs& A) { return A.hmf32; }
-  static __device__ const float* da(const KernArgs& A) { return A.da32; }
-  static __device__ const float* sci(const KernArgs& A) { return A.sci32; }
-};
-template <> struct TabSel<double> {
-  static __device__ const double* hmf(const KernArgs& A) { return A.hmf64; }
-  static __device__ const double* da(const KernArgs& A) { return A.da64; }
-  static __device__ const double* sci(const KernArgs& A) { return A.sci64; }
-};
-
-// Stage the tables of point pt into LDS.
-template <typename T>
-__device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
-                             T* s_sci)
-{
-  const T* g_hmf = TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
-  const T* g_da = TabSel<T>::da(A) + (size_t)pt * A.da_stride;
-  const T* g_sci = TabSel<T>::sci(A) + (size_t)pt * A.sci_stride;
-  for (int k = threadIdx.x; k < A.hmf_stride; k += blockDim.x) s_hmf[k] = g_hmf[k];
-  for (int k = threadIdx.x; k < A.da_stride; k += blockDim.x) s_da[k] = g_da[k];
-  for (int k = threadIdx.x; k < A.sci_stride; k += blockDim.x) s_sci[k] = g_sci[k];
-}
-
-// ------------------------------------------------------------- cubature
-//
-// Region algorithm (identical to the oracle's y3o_cubature restatement):
-// initial k^d uniform partition; per region R7, R5, err = |R7 - R5| and the
-// split axis (largest fourth difference; widest relative edge if all zero);
-// each step splits the largest-error regions holding half of E (and at least
-// max(64, N/8) of them, at most kmax; select_bins) at the midpoint of their
-// axis, child A in the parent slot, child B appended; if
-// |R7_A + R7_B - R7_parent| exceeds the children's combined error, their
-// errors are scaled up to that difference; stop when
-// E <= max(epsabs, epsrel |I|) (status 0) or when the region or evaluation
-// budget would be exceeded (status 1).
-
-constexpr int kChunk = 32;                      // regions per evaluation chunk
-constexpr double kAlpha = 0.5;                  // error mass split per step
-constexpr int kMinSplit = 64;
-
-// Error bin: binary exponent of E minus that of err (0 = largest), 63 for
-// err == 0.
-__host__ __device__ inline int err_bin(double err, double E, int eE)
-{
-  if (!(err > 0.0)) return 63;
-  int e;
-  (void)frexp(err, &e);
-  int const b = eE - e;
-  return b < 0 ? 0 : (b > 62 ? 62 : b);
-}
-
-// err / E in 2^-40 fixed point.
-__host__ __device__ inline unsigned long long err_q(double err, double E)
-{
-  double const q = floor(err / E * 1099511627776.0);
-  return q > 0.0 ? (unsigned long long)q : 0ull;
-}
-
-// Split bin B: the first bin where the cumulative mass reaches kAlpha and
-// the first where the cumulative count reaches max(kMinSplit, N/8) (capped
-// by N and room); all of bins < B up to room (lim0), then bin B (lim1).
-__host__ __device__ inline void select_bins(const int* hc,
-                                            const unsigned long long* hq,
-                                            int N, int room, int& B, int& lim0,
-                                            int& lim1)
-{
-  unsigned long long const qa =
-    (unsigned long long)(kAlpha * 1099511627776.0);
-  int km = N / 8 > kMinSplit ? N / 8 : kMinSplit;
-  if (km > room) km = room;
-  if (km > N) km = N;
-  int Ba = 63, Bk = 63;
-  unsigned long long cq = 0;
-  int cc = 0;
-  for (int b = 0; b < 64; ++b) {
-    cq += hq[b];
-    cc += hc[b];
-    if (Ba == 63 && cq >= qa) Ba = b;
-    if (Bk == 63 && cc >= km) Bk = b;
-  }
-  B = Ba > Bk ? Ba : Bk;
-  int below = 0;
-  for (int b = 0; b < B; ++b) below += hc[b];
-  lim0 = below < room ? below : room;
-  int const inb = B < 63 ? hc[B] : 0;
-  lim1 = room - lim0 < inb ? room - lim0 : inb;
-}
-constexpr int kMaxNodes = gm_num_nodes(kMaxDim);  // 149
-constexpr int kKmax = 1024;
-
-template <typename T, int BLOCK> struct CubShared {
-  T hmf[kMaxHmf];
-  T da[kMaxDa];
-  T sci[kMaxSci];
-  T f[kChunk * kMaxNodes];
-  double c[kChunk][kMaxDim], h[kChunk][kMaxDim];
-  double est[kChunk], err[kChunk];
-  int ax[kChunk];
-  int idx[kChunk];
-  double pest[kChunk / 2], perr[kChunk / 2];
-  int sel[kKmax];
-  double red[2 * (BLOCK / 64)];
-  int ired[BLOCK / 64];
-  int hc[64];
-  unsigned long long hq[64];
-  int selB, lim0, lim1;
-  double w7[6], w5[6];
-  y3g_integral ig;
-  PointDev P;
-  int item, point, ncount;
-};
-
-// Evaluate the m regions whose centre / half-widths are in S.c / S.h:
-// S.est, S.err, S.ax.
-template <typename T, int BLOCK>
-__device__ void eval_chunk(CubShared<T, BLOCK>& S, int m)
-{
-  int const d = S.ig.ndim;
-  int const nn = gm_num_nodes(d);
-  Tabs<T> tb{S.hmf, S.da, S.sci};
-  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
-    int const r = q / nn, j = q - r * nn;
-    double u[kMaxDim], y[kMaxDim];
-    gm_offsets(d, j, u);
-#pragma unroll
-    for (int a = 0; a < kMaxDim; ++a)
-      if (a < d) y[a] = S.c[r][a] + S.h[r][a] * u[a];
-    S.f[q] = integrand<T>(S.ig, S.P, tb, y);
-  }
-  __syncthreads();
-  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  constexpr int NW = BLOCK / 64;
-  for (int r = w; r < m; r += NW) {
-    const T* fr = S.f + r * nn;
-    double a7 = 0.0, a5 = 0.0;
-    for (int j = lane; j < nn; j += 64) {
-      int const g = gm_class(d, j);
-      double const fv = double(fr[j]);
-      a7 += S.w7[g] * fv;
-      a5 += S.w5[g] * fv;
-    }
-    a7 = wave_sum(a7);
-    a5 = wave_sum(a5);
-    // fourth differences, one lane per axis
-    double four = -1.0;
-    if (lane < d) {
-      double const f0 = double(fr[0]);
-      double const f2 = double(fr[1 + 2 * lane]) + double(fr[2 + 2 * lane]);
-      double const f3 =
-        double(fr[1 + 2 * d + 2 * lane]) + double(fr[2 + 2 * d + 2 * lane]);
-      four = fabs(f2 - 2.0 * f0 - (f3 - 2.0 * f0) / 7.0);
-    }
-    // argmax over lanes 0..d-1 (ties -> lowest axis)
-    int best = 0;
-    double bv = __shfl(four, 0, 64);
-    for (int a = 1; a < d; ++a) {
-      double const v = __shfl(four, a, 64);
-      if (v > bv) bv = v, best = a;
-    }
-    if (lane == 0) {
-      double vol = 1.0;
-      for (int a = 0; a < d; ++a) vol *= 2.0 * S.h[r][a];
-      if (!(bv > 0.0)) {
-        double bw = -1.0;
-        for (int a = 0; a < d; ++a) {
-          double const rel = 2.0 * S.h[r][a] / (S.ig.highs[a] - S.ig.lows[a]);
-          if (rel > bw) bw = rel, best = a;
-        }
-      }
-      S.est[r] = a7 * vol;
-      S.err[r] = fabs(a7 - a5) * vol;
-      S.ax[r] = best;
-    }
-  }
-  __syncthreads();
 }
 
 template <typename T, int BLOCK>

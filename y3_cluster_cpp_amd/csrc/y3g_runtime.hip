@@ -16,6 +16,7 @@
 
 #include "y3g.h"
 #include "y3g_internal.hpp"
+#include "y3g_single.hpp"
 
 using namespace y3g;
 
@@ -83,6 +84,10 @@ struct y3g_runtime {
   unsigned long long* d_nevals = nullptr;
   double last_ms = 0.0;
   long long last_nevals = 0;
+  // plain path workspace
+  SingleArgs sg{};
+  int sg_cap = 0;
+  y3g_integral* d_one = nullptr;
 };
 
 extern "C" {
@@ -244,6 +249,10 @@ int y3g_runtime_destroy(y3g_runtime* rt)
     dfree(rt->slab_err), dfree(rt->slab_ax);
   dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
   dfree(rt->d_queue), dfree(rt->d_nevals);
+  dfree(rt->sg.lo), dfree(rt->sg.hi), dfree(rt->sg.est), dfree(rt->sg.err);
+  dfree(rt->sg.ax), dfree(rt->sg.list), dfree(rt->sg.sel), dfree(rt->sg.pest);
+  dfree(rt->sg.perr), dfree(rt->sg.partial), dfree(rt->sg.IE), dfree(rt->sg.hc);
+  dfree(rt->sg.hq), dfree(rt->sg.blockcnt), dfree(rt->sg.blockoff), dfree(rt->d_one);
   if (rt->ev0) (void)hipEventDestroy(rt->ev0);
   if (rt->ev1) (void)hipEventDestroy(rt->ev1);
   if (rt->stream) (void)hipStreamDestroy(rt->stream);
@@ -411,6 +420,27 @@ static int ensure_slab(y3g_runtime* rt, int slots, int cap)
   return 0;
 }
 
+static int ensure_single(y3g_runtime* rt, int cap)
+{
+  if (cap == rt->sg_cap) return 0;
+  SingleArgs& g = rt->sg;
+  dfree(g.lo), dfree(g.hi), dfree(g.est), dfree(g.err), dfree(g.ax), dfree(g.list);
+  dfree(g.sel), dfree(g.pest), dfree(g.perr), dfree(g.partial), dfree(g.IE);
+  dfree(g.hc), dfree(g.hq), dfree(g.blockcnt), dfree(g.blockoff), dfree(rt->d_one);
+  rt->sg_cap = 0;
+  size_t const nt = (size_t)(cap + 4095) / 4096;
+  if (dalloc(&g.lo, (size_t)cap * kMaxDim) || dalloc(&g.hi, (size_t)cap * kMaxDim) ||
+      dalloc(&g.est, cap) || dalloc(&g.err, cap) || dalloc(&g.ax, cap) ||
+      dalloc(&g.list, 2 * (size_t)cap) || dalloc(&g.sel, cap) || dalloc(&g.pest, cap) ||
+      dalloc(&g.perr, cap) || dalloc(&g.partial, 512) || dalloc(&g.IE, 2) ||
+      dalloc(&g.hc, 64) || dalloc(&g.hq, 64) || dalloc(&g.blockcnt, 2 * nt) ||
+      dalloc(&g.blockoff, 4 * nt) || dalloc(&rt->d_one, 1))
+    return Y3G_ERR_MEMORY;
+  g.cap = cap;
+  rt->sg_cap = cap;
+  return 0;
+}
+
 // Largest expected cost first: dimension, then tighter tolerance.
 static void lpt_order(const y3g_integral* igs, const std::vector<int>& ids,
                       std::vector<int>& order)
@@ -515,12 +545,38 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.out = rt->d_res;
   A.nevals = rt->d_nevals;
   HIPCHK(hipEventRecord(rt->ev0, rt->stream));
+  long long extra_nev = 0;
   if (o.single) {
-    // plain path: one integral per launch, serial over integrals
+    // plain path: one integral per launch sequence, serial over integrals;
+    // cubature regions spread over the whole device
+    if (!cub.empty())
+      if (int rc = ensure_single(rt, o.max_regions)) return rc;
+    std::vector<y3g_result> host(n);
+    long long nev_single = 0;
     for (size_t i = 0; i < n; ++i) {
-      std::vector<int> one{(int)i};
-      if (int rc = run_group(rt, igs, one, o, igs[i].algorithm, A)) return rc;
+      if (igs[i].algorithm == Y3G_MCUBES) {
+        std::vector<int> one{(int)i};
+        if (int rc = run_group(rt, igs, one, o, Y3G_MCUBES, A)) return rc;
+        continue;
+      }
+      SingleArgs sa = rt->sg;
+      sa.K = A;
+      sa.ig = rt->d_one;
+      HIPCHK(hipMemcpyAsync(rt->d_one, &igs[i], sizeof(y3g_integral),
+                            hipMemcpyHostToDevice, rt->stream));
+      int const grid = rt->n_cu * (o.precision == Y3G_FP64 ? 1 : 2);
+      HIPCHK(run_single(sa, igs[i], o.precision,
+                        std::max(o.max_split_per_step, kSingleKmax), o.init_split, grid,
+                        rt->stream, &host[i], &nev_single));
     }
+    // the VEGAS results were written on the device; cubature ones on the host
+    HIPCHK(hipStreamSynchronize(rt->stream));
+    std::vector<y3g_result> dres(n);
+    HIPCHK(hipMemcpy(dres.data(), rt->d_res, sizeof(y3g_result) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i)
+      if (igs[i].algorithm != Y3G_MCUBES) dres[i] = host[i];
+    HIPCHK(hipMemcpy(rt->d_res, dres.data(), sizeof(y3g_result) * n, hipMemcpyHostToDevice));
+    extra_nev = nev_single;
   } else {
     if (int rc = run_group(rt, igs, cub, o, Y3G_PAGANI, A)) return rc;
     if (int rc = run_group(rt, igs, veg, o, Y3G_MCUBES, A)) return rc;
@@ -535,7 +591,7 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
   rt->last_ms = ms;
-  rt->last_nevals = (long long)nev;
+  rt->last_nevals = (long long)nev + extra_nev;
   return 0;
 }
 
