@@ -90,9 +90,11 @@ typedef struct {
   int n_sci;
 } y3g_table_spec;
 
-/* One integral.  lows/highs are the box of the integration variables
- * (the standardised MOR variable t replaces lambda_true; lt_lo/lt_hi keep
- * the lambda_true range as an indicator). */
+/* One integral.  lows/highs are the box of the integration variables: the
+ * MOR variable is integrated as v in [0, 1] across the window of the
+ * standardised t = (ln lt - ln lbar) / sigma_ln that lies inside
+ * [lnlt_lo, lnlt_hi] and |t| <= 7; for Delta-Sigma the offset radius as its
+ * Rayleigh CDF u and the angle on [0, pi]. */
 typedef struct {
   int kind;
   int ndim;
@@ -103,7 +105,7 @@ typedef struct {
   double radius;
   double sigma_mis;
   double rmis_norm;
-  double lt_lo, lt_hi;
+  double lnlt_lo, lnlt_hi;
   double lows[8], highs[8];
   double eps_rel, eps_abs;
   long long max_eval;
@@ -130,6 +132,8 @@ typedef struct {
   double vegas_alpha;
   int single;             /* 1: one integral per device-wide launch
                              sequence (the plain path)                   */
+  double err_c, err_smin; /* region error |R7-R5| * clamp(c r, smin, 1),
+                             r = |R7-R5| / |R5-R3|; c = 0 keeps |R7-R5|  */
 } y3g_options;
 
 typedef struct y3g_runtime y3g_runtime;

@@ -186,6 +186,7 @@ typedef struct {
   int max_regions;
   int max_split_per_step;
   int init_split;           /* pieces per axis of the initial partition */
+  double err_c, err_smin;   /* region error E1 clamp(c E1/E2, smin, 1)     */
 } y3o_cub_opts;
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,
@@ -204,6 +205,9 @@ double y3o_gm_rule(double (*f)(const double*, void*), void* ctx, int ndim,
                    const double* center, const double* halfw, double* r5,
                    double* fourth);
 int y3o_gm_num_nodes(int ndim);
+double y3o_gm_rule3(double (*f)(const double*, void*), void* ctx, int ndim,
+                    const double* center, const double* halfw, double* r5,
+                    double* r3, double* fourth);
 
 /* VEGAS / m-Cubes. */
 typedef struct {

@@ -649,10 +649,13 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
  * variable is integrated in its standardised form
  *   t = (ln lt - ln lbar(lnM, zt)) / sigma_ln,  lt = lbar e^{sigma_ln t},
  *   MOR(lt) dlt = phi(t) dt,
- * over t in [-Y3O_T_MAX, Y3O_T_MAX] with the lambda_true range [1, 300]
- * kept as an indicator, so the integral is the same one over lt in [1, 300]
- * (the truncated Gaussian mass beyond |t| = 7 is 2.6e-12).  Axis order
- * follows the physical integrands with lt replaced by t. */
+ * with t running over its window [max(-7, t(lt_lo)), min(7, t(lt_hi))]
+ * through v in [0, 1] (dt = (t_hi - t_lo) dv), so the integral is the same
+ * one over lt in [1, 300] (the Gaussian mass beyond |t| = 7 is 2.6e-12).  For the
+ * Delta-Sigma integrand the offset radius is integrated in its Rayleigh CDF
+ * u = 1 - exp(-rmis^2 / 2 sigma^2) (flat weight) and the angle on [0, pi]
+ * (the integrand is even about theta = pi).  Axis order follows the
+ * physical integrands with lt replaced by t, rmis by u. */
 #define Y3O_T_MAX 7.0
 
 double
@@ -667,15 +670,30 @@ y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
     default: it = 1, iz = 2, im = 3; break;
   }
   for (int k = 0; k < ig->ndim; ++k) x[k] = y[k];
-  double const zt = y[iz], lnM = y[im], tt = y[it];
+  double jac = 1.0;
+  if (ig->kind == Y3O_DSIGMA6D) {
+    /* y4 = u, the Rayleigh CDF: rmis = sigma sqrt(-2 ln(1 - u)) and
+     * P(rmis) drmis = du; y5 = theta on [0, pi], the integrand being even
+     * about pi: factor 2 */
+    double const s = ig->sigma_mis;
+    double const rmis = s * sqrt(-2.0 * log1p(-y[4]));
+    x[4] = rmis;
+    jac = 2.0 / (rmis / (s * s) * exp(-0.5 * rmis * rmis / (s * s)));
+    if (!(rmis > 0.0)) jac = 0.0;
+  }
+  double const zt = y[iz], lnM = y[im], v = y[it];
   double const s = p->mor_sigma_ln;
   double const lnlbar = p->mor_lnl0 + p->mor_slope * (lnM - Y3O_LN_MPIVOT_MOR) +
                         p->mor_zslope * log((1.0 + zt) / 1.35);
+  /* t window: [-T, T] clipped to the lambda_true range */
+  double const tlo = fmax(-Y3O_T_MAX, (log(ig->lt_lo) - lnlbar) / s);
+  double const thi = fmin(Y3O_T_MAX, (log(ig->lt_hi) - lnlbar) / s);
+  if (!(thi > tlo) || jac == 0.0) return 0.0;
+  double const tt = tlo + (thi - tlo) * v;
   double const lt = exp(lnlbar + s * tt);
-  if (lt < ig->lt_lo || lt > ig->lt_hi) return 0.0;
   x[it] = lt;
-  /* integrand(lt) * dlt/dt, with MOR(lt) * lt * s = phi(t) */
-  return y3o_integrand(p, t, ig, x) * lt * s;
+  /* integrand(lt) * dlt/dt * dt/dv, with MOR(lt) * lt * s = phi(t) */
+  return y3o_integrand(p, t, ig, x) * lt * s * (thi - tlo) * jac;
 }
 
 #define Y3O_ZT_MARGIN 0.1
@@ -700,7 +718,7 @@ y3o_make_integral(const y3o_params* p, const y3o_table_spec* s, int kind,
   switch (kind) {
     case Y3O_COUNT3D:
       ig->ndim = 3;
-      ig->lows[0] = -Y3O_T_MAX, ig->highs[0] = Y3O_T_MAX;
+      ig->lows[0] = 0.0, ig->highs[0] = 1.0;
       ig->lows[1] = zt_lo, ig->highs[1] = zt_hi;
       ig->lows[2] = s->lnm_lo, ig->highs[2] = s->lnm_hi;
       break;
@@ -708,7 +726,7 @@ y3o_make_integral(const y3o_params* p, const y3o_table_spec* s, int kind,
       ig->ndim = 5;
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
       ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
-      ig->lows[2] = -Y3O_T_MAX, ig->highs[2] = Y3O_T_MAX;
+      ig->lows[2] = 0.0, ig->highs[2] = 1.0;
       ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
       ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;
       break;
@@ -722,11 +740,12 @@ y3o_make_integral(const y3o_params* p, const y3o_table_spec* s, int kind,
       ig->rmis_norm =
         1.0 / (1.0 - exp(-0.5 * Y3O_RMIS_NSIGMA * Y3O_RMIS_NSIGMA));
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
-      ig->lows[1] = -Y3O_T_MAX, ig->highs[1] = Y3O_T_MAX;
+      ig->lows[1] = 0.0, ig->highs[1] = 1.0;
       ig->lows[2] = zt_lo, ig->highs[2] = zt_hi;
       ig->lows[3] = s->lnm_lo, ig->highs[3] = s->lnm_hi;
-      ig->lows[4] = 0.0, ig->highs[4] = rmax;
-      ig->lows[5] = 0.0, ig->highs[5] = 2.0 * Y3O_PI;
+      (void)rmax;
+      ig->lows[4] = 0.0, ig->highs[4] = -expm1(-0.5 * Y3O_RMIS_NSIGMA * Y3O_RMIS_NSIGMA);
+      ig->lows[5] = 0.0, ig->highs[5] = Y3O_PI;
       break;
     }
   }

@@ -606,20 +606,48 @@ gm_weights(int d, double* w7, double* w5)
   w5[5] = 0.0;
 }
 
+/* Region error from the degree-7, -5 and -3 rules (device: rule_error in
+ * y3g_cub_common.hpp): E1 = |R7 - R5| is the error of R5; when the rule
+ * sequence converges (r = E1 / |R5 - R3| small) the error of R7 is a
+ * fraction of it, E1 * clamp(c r, smin, 1); c = 0 keeps E1. */
+static double
+rule_error(double r7, double r5, double r3, double c, double smin)
+{
+  double const e1 = fabs(r7 - r5), e2 = fabs(r5 - r3);
+  if (!(c > 0.0) || !(e2 > 0.0)) return e1;
+  double f = c * e1 / e2;
+  f = f < smin ? smin : (f > 1.0 ? 1.0 : f);
+  return e1 * f;
+}
+
 double
 y3o_gm_rule(double (*f)(const double*, void*), void* ctx, int d,
             const double* c, const double* hw, double* r5, double* fourth)
 {
+  double r3;
+  return y3o_gm_rule3(f, ctx, d, c, hw, r5, &r3, fourth);
+}
+
+/* As y3o_gm_rule, also returning the degree-3 rule on the centre and the
+ * +-l3 axis nodes (weights 1/(6 l3^2) and 1 - 2d/(6 l3^2)). */
+double
+y3o_gm_rule3(double (*f)(const double*, void*), void* ctx, int d,
+             const double* c, const double* hw, double* r5, double* r3,
+             double* fourth)
+{
   double w7[6], w5[6], u[8], x[8];
   gm_weights(d, w7, w5);
+  double const w33 = 10.0 / 54.0, w31 = 1.0 - 2.0 * d * w33;
   int const n = y3o_gm_num_nodes(d);
-  double s7 = 0.0, s5 = 0.0, f0 = 0.0, f2p[8], f3p[8];
+  double s7 = 0.0, s5 = 0.0, s3 = 0.0, f0 = 0.0, f2p[8], f3p[8];
   for (int j = 0; j < n; ++j) {
     int const g = gm_node(d, j, u);
     for (int i = 0; i < d; ++i) x[i] = c[i] + hw[i] * u[i];
     double const fv = f(x, ctx);
     s7 += w7[g] * fv;
     s5 += w5[g] * fv;
+    if (g == 1) s3 += w31 * fv;
+    if (g == 3) s3 += w33 * fv;
     if (j == 0) f0 = fv;
     if (g == 2) {
       int const a = (j - 1) >> 1;
@@ -642,6 +670,7 @@ y3o_gm_rule(double (*f)(const double*, void*), void* ctx, int d,
     for (int i = 0; i < d; ++i)
       fourth[i] = fabs(f2p[i] - 2.0 * f0 - (f3p[i] - 2.0 * f0) / 7.0);
   *r5 = s5 * vol;
+  *r3 = s3 * vol;
   return s7 * vol;
 }
 
@@ -668,6 +697,7 @@ typedef struct {
   /* table axes: axis -> (x0, dx) of a uniform knot grid, or dx = 0; the
    * initial partition cuts these axes at every knot inside the domain */
   double snap_x0[8], snap_dx[8];
+  double err_c, err_smin;
 } cub_prob;
 
 typedef struct {
@@ -684,8 +714,9 @@ eval_region(const cub_prob* P, region_t* r)
     c[i] = 0.5 * (r->lo[i] + r->hi[i]);
     hw[i] = 0.5 * (r->hi[i] - r->lo[i]);
   }
-  r->est = y3o_gm_rule(P->f, P->ctx, P->d, c, hw, &r5, fourth);
-  r->err = fabs(r->est - r5);
+  double r3;
+  r->est = y3o_gm_rule3(P->f, P->ctx, P->d, c, hw, &r5, &r3, fourth);
+  r->err = rule_error(r->est, r5, r3, P->err_c, P->err_smin);
   int ax = 0;
   double best = fourth[0];
   for (int i = 1; i < P->d; ++i)
@@ -747,8 +778,12 @@ select_bins(const int* hc, const unsigned long long* hq, int N, int room,
 }
 
 static void
-cub_run(const cub_prob* P, const y3o_cub_opts* o, y3o_result* res)
+cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
 {
+  cub_prob Pc = *Pin;
+  Pc.err_c = o->err_c;
+  Pc.err_smin = o->err_smin;
+  const cub_prob* P = &Pc;
   int const d = P->d;
   int const nn = y3o_gm_num_nodes(d);
   int cap = o->max_regions > 0 ? o->max_regions : 8192;

@@ -37,8 +37,9 @@ def test_make_integral_matches_oracle():
             assert g.ndim == o.ndim
             for k in range(g.ndim):
                 assert g.lows[k] == o.lows[k] and g.highs[k] == o.highs[k]
-            for f in ("sigma_mis", "rmis_norm", "lt_lo", "lt_hi", "radius"):
+            for f in ("sigma_mis", "rmis_norm", "radius"):
                 assert getattr(g, f) == getattr(o, f), f
+            assert abs(g.lnlt_lo - np.log(o.lt_lo)) < 1e-15 and abs(g.lnlt_hi - np.log(o.lt_hi)) < 1e-15
 
 
 def test_params_and_specs_match_oracle():

@@ -185,19 +185,32 @@ def test_golden_vs_brute_force_dsigma6d():
 
 
 def test_mapped_integrand_equals_physical():
-    # the t-mapping: f_mapped(y) = f(x(y)) * dlt/dt with MOR dlt = phi(t) dt
+    # v -> t window -> lt: f_mapped(y) = f(x) * lt * sigma_ln * (t_hi - t_lo),
+    # with MOR(lt) dlt = phi(t) dt; Delta-Sigma: rmis from the Rayleigh CDF u
+    # and theta on [0, pi] (factor 2 / P(rmis))
     p = O.default_params()
     s = O.spec_wide()
     t = O.OracleTables(p, s)
-    ig = O.make_integral(p, s, O.COUNT5D, 20, 30, 0.2, 0.35)
+    L3 = 33.334803590584751
     rng = np.random.default_rng(3)
-    for _ in range(20):
-        y = np.array([rng.uniform(ig.lows[k], ig.highs[k]) for k in range(5)])
-        L3 = 33.334803590584751
-        lnlbar = p.mor_lnl0 + p.mor_slope * (y[4] - L3) + p.mor_zslope * math.log((1 + y[3]) / 1.35)
-        lt = math.exp(lnlbar + p.mor_sigma_ln * y[2])
-        x = y.copy()
-        x[2] = lt
-        phys = O.integrand(p, t, ig, x) if 1 <= lt <= 300 else 0.0
-        mapped = L.y3o_integrand_mapped(C.byref(p), C.byref(t.t), C.byref(ig), O.ptr(np.ascontiguousarray(y)))
-        assert abs(mapped - phys * lt * p.mor_sigma_ln) <= 1e-12 * abs(mapped) + 1e-300
+    for kind, (iv, iz, im) in [(O.COUNT5D, (2, 3, 4)), (O.DSIGMA6D, (1, 2, 3))]:
+        ig = O.make_integral(p, s, kind, 20, 30, 0.2, 0.35, 0.8)
+        for _ in range(20):
+            y = np.array([rng.uniform(ig.lows[k], ig.highs[k]) for k in range(ig.ndim)])
+            lnlbar = p.mor_slope * (y[im] - L3) + p.mor_lnl0 + p.mor_zslope * math.log((1 + y[iz]) / 1.35)
+            tlo = max(-7.0, (math.log(1.0) - lnlbar) / p.mor_sigma_ln)
+            thi = min(7.0, (math.log(300.0) - lnlbar) / p.mor_sigma_ln)
+            tt = tlo + (thi - tlo) * y[iv]
+            lt = math.exp(lnlbar + p.mor_sigma_ln * tt)
+            x = y.copy()
+            x[iv] = lt
+            jac = lt * p.mor_sigma_ln * (thi - tlo)
+            if kind == O.DSIGMA6D:
+                sg = ig.sigma_mis
+                rm = sg * math.sqrt(-2 * math.log1p(-y[4]))
+                x[4] = rm
+                jac *= 2.0 / (rm / sg ** 2 * math.exp(-0.5 * rm * rm / sg ** 2))
+            phys = O.integrand(p, t, ig, x)
+            mapped = L.y3o_integrand_mapped(C.byref(p), C.byref(t.t), C.byref(ig),
+                                            O.ptr(np.ascontiguousarray(y)))
+            assert abs(mapped - phys * jac) <= 1e-12 * abs(mapped) + 1e-300

@@ -43,8 +43,8 @@ class Integral(C.Structure):
                 ("lo_lo", C.c_double), ("lo_hi", C.c_double),
                 ("zo_lo", C.c_double), ("zo_hi", C.c_double),
                 ("radius", C.c_double), ("sigma_mis", C.c_double),
-                ("rmis_norm", C.c_double), ("lt_lo", C.c_double),
-                ("lt_hi", C.c_double),
+                ("rmis_norm", C.c_double), ("lnlt_lo", C.c_double),
+                ("lnlt_hi", C.c_double),
                 ("lows", C.c_double * 8), ("highs", C.c_double * 8),
                 ("eps_rel", C.c_double), ("eps_abs", C.c_double),
                 ("max_eval", C.c_longlong), ("seed", C.c_uint64)]
@@ -62,7 +62,8 @@ class Options(C.Structure):
                 ("block_threads", C.c_int), ("slots", C.c_int),
                 ("vegas_calls", C.c_longlong), ("vegas_itmax", C.c_int),
                 ("vegas_itskip", C.c_int), ("vegas_nbins", C.c_int),
-                ("vegas_alpha", C.c_double), ("single", C.c_int)]
+                ("vegas_alpha", C.c_double), ("single", C.c_int),
+                ("err_c", C.c_double), ("err_smin", C.c_double)]
 
 
 RESULT_DTYPE = np.dtype([("value", "f8"), ("error", "f8"), ("prob", "f8"),

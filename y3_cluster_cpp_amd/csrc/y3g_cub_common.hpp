@@ -111,6 +111,19 @@ __device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
 // budget would be exceeded (status 1).
 
 constexpr int kChunk = 32;                      // regions per evaluation chunk
+
+// Region error from the degree-7, -5 and -3 rules: E1 = |R7 - R5| is the
+// error of R5; when the rule sequence converges (r = E1 / |R5 - R3| small)
+// the error of R7 is a fraction of E1, taken as E1 * clamp(c r, smin, 1).
+__host__ __device__ inline double rule_error(double r7, double r5, double r3,
+                                             double c, double smin)
+{
+  double const e1 = fabs(r7 - r5), e2 = fabs(r5 - r3);
+  if (!(c > 0.0) || !(e2 > 0.0)) return e1;
+  double f = c * e1 / e2;
+  f = f < smin ? smin : (f > 1.0 ? 1.0 : f);
+  return e1 * f;
+}
 constexpr double kAlpha = 0.5;                  // error mass split per step
 constexpr int kMinSplit = 64;
 
@@ -180,7 +193,7 @@ template <typename T, int BLOCK> struct CubShared {
   int hc[64];
   unsigned long long hq[64];
   int selB, lim0, lim1;
-  double w7[6], w5[6];
+  double w7[6], w5[6], w3[6];
   y3g_integral ig;
   PointDev P;
   int item, point, ncount;
@@ -189,18 +202,17 @@ template <typename T, int BLOCK> struct CubShared {
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
 template <typename T, int BLOCK>
-__device__ void eval_chunk(CubShared<T, BLOCK>& S, int m)
+__device__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double err_smin)
 {
   int const d = S.ig.ndim;
   int const nn = gm_num_nodes(d);
   Tabs<T> tb{S.hmf, S.da, S.sci};
   for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
     int const r = q / nn, j = q - r * nn;
-    double u[kMaxDim], y[kMaxDim];
-    gm_offsets(d, j, u);
+    double y[kMaxDim];
 #pragma unroll
     for (int a = 0; a < kMaxDim; ++a)
-      if (a < d) y[a] = S.c[r][a] + S.h[r][a] * u[a];
+      y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
     S.f[q] = integrand<T>(S.ig, S.P, tb, y);
   }
   __syncthreads();
@@ -208,15 +220,17 @@ __device__ void eval_chunk(CubShared<T, BLOCK>& S, int m)
   constexpr int NW = BLOCK / 64;
   for (int r = w; r < m; r += NW) {
     const T* fr = S.f + r * nn;
-    double a7 = 0.0, a5 = 0.0;
+    double a7 = 0.0, a5 = 0.0, a3 = 0.0;
     for (int j = lane; j < nn; j += 64) {
       int const g = gm_class(d, j);
       double const fv = double(fr[j]);
       a7 += S.w7[g] * fv;
       a5 += S.w5[g] * fv;
+      a3 += S.w3[g] * fv;
     }
     a7 = wave_sum(a7);
     a5 = wave_sum(a5);
+    a3 = wave_sum(a3);
     // fourth differences, one lane per axis
     double four = -1.0;
     if (lane < d) {
@@ -244,12 +258,11 @@ __device__ void eval_chunk(CubShared<T, BLOCK>& S, int m)
         }
       }
       S.est[r] = a7 * vol;
-      S.err[r] = fabs(a7 - a5) * vol;
+      S.err[r] = rule_error(a7, a5, a3, err_c, err_smin) * vol;
       S.ax[r] = best;
     }
   }
   __syncthreads();
 }
-
 
 }  // namespace y3g

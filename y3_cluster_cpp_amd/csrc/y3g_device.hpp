@@ -356,9 +356,13 @@ Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 }
 
 // ------------------------------------------------------- integrands
-// Mapped integrand: y are the integration variables of the box
-// (t = standardised MOR variable in place of lambda_true).  Returns the
-// physical integrand times dlt/dt, with MOR(lt) dlt = phi(t) dt.
+// Mapped integrand: y are the integration variables of the box.  The MOR
+// variable is integrated in its standardised form
+//   t = (ln lt - ln lbar(lnM, zt)) / sigma_ln,  MOR(lt) dlt = phi(t) dt,
+// with t running over its window [max(-7, t(lt_lo)), min(7, t(lt_hi))]
+// through v in [0, 1]; for Delta-Sigma the offset radius is integrated in
+// its Rayleigh CDF u (P(rmis) drmis = du) and the angle on [0, pi] (the
+// integrand is even about pi).
 template <typename T>
 Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
                    const double* y)
@@ -370,14 +374,19 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
     default: it = 1, iz = 2, im = 3; break;
   }
   double const zt = y[iz], lnM = y[im];
-  T const tt = T(y[it]);
   T const sln = T(P.p.mor_sigma_ln);
   T const zp = T(1) + T(zt);
   T const lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
                    T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
+  T const isl = T(1) / sln;
+  T tlo = (T(ig.lnlt_lo) - lnlbar) * isl;
+  T thi = (T(ig.lnlt_hi) - lnlbar) * isl;
+  tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
+  thi = thi < T(kTMax) ? thi : T(kTMax);
+  if (!(thi > tlo)) return T(0);
+  T const tt = tlo + (thi - tlo) * T(y[it]);
   T const lt = M<T>::exp(lnlbar + sln * tt);
-  if (lt < T(ig.lt_lo) || lt > T(ig.lt_hi)) return T(0);
-  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327);
+  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (thi - tlo);
   T const common = hmf<T>(P, tb.hmf, lnM, zt) * phi * dv_do_dz<T>(P, tb.da, zt);
   T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
   switch (ig.kind) {
@@ -390,20 +399,21 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
     }
     default: {
       T const lo = T(y[0]);
-      T const rmis = T(y[4]), th = T(y[5]);
+      T const s = T(ig.sigma_mis);
+      T const one_minus_u = T(1.0 - y[4]);
+      T const rmis = s * M<T>::sqrt(T(-2) * M<T>::log(one_minus_u));
+      T const th = T(y[5]);
       T const rho = T(P.rho_m);
       NfwHalo<T> const H = nfw_halo<T>(lnM, rho, T(0));
       T const R = T(ig.radius);
       T const fmis = T(P.p.f_mis);
       T const prof = (T(1) - fmis) * nfw_dsigma<T>(R, H) +
                      fmis * dsigma_offset<T>(R, rmis, th, H);
-      T const s = T(ig.sigma_mis);
-      T const pr = rmis / (s * s) * M<T>::exp(T(-0.5) * rmis * rmis / (s * s)) *
-                   T(ig.rmis_norm);
       T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
+      // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
       return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
              zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
-             pr * T(0.15915494309189535) * prof;
+             T(ig.rmis_norm * 0.31830988618379067) * prof;
     }
   }
 }
@@ -433,45 +443,41 @@ Y3G_HD int gm_class(int d, int j)
   return 5;
 }
 
-Y3G_HD void gm_offsets(int d, int j, double* u)
+// Unit offset of node j on axis a (no local arrays: the node coordinates
+// stay in registers).
+Y3G_HD double gm_offset(int d, int j, int a)
 {
-#pragma unroll
-  for (int i = 0; i < kMaxDim; ++i) u[i] = 0.0;
-  if (j == 0) return;
+  if (j == 0) return 0.0;
   j -= 1;
-  if (j < 2 * d) {
-    u[j >> 1] = (j & 1) ? -kL2 : kL2;
-    return;
-  }
+  if (j < 2 * d) return (j >> 1) == a ? ((j & 1) ? -kL2 : kL2) : 0.0;
   j -= 2 * d;
-  if (j < 2 * d) {
-    u[j >> 1] = (j & 1) ? -kL3 : kL3;
-    return;
-  }
+  if (j < 2 * d) return (j >> 1) == a ? ((j & 1) ? -kL3 : kL3) : 0.0;
   j -= 2 * d;
   if (j < 2 * d * (d - 1)) {
-    int pair = j >> 2, sg = j & 3, a = 0, b = 1, cnt = 0;
-    for (a = 0; a < d; ++a) {
-      int const np = d - 1 - a;
-      if (pair < cnt + np) {
-        b = a + 1 + (pair - cnt);
-        break;
-      }
-      cnt += np;
-    }
-    u[a] = (sg & 1) ? -kL3 : kL3;
-    u[b] = (sg & 2) ? -kL3 : kL3;
-    return;
+    int const pair = j >> 2, sg = j & 3;
+    // pair -> (p, q), p < q, lexicographic
+    int p = 0, cnt = 0;
+    while (pair >= cnt + (d - 1 - p)) cnt += d - 1 - p, ++p;
+    int const q = p + 1 + (pair - cnt);
+    if (a == p) return (sg & 1) ? -kL3 : kL3;
+    if (a == q) return (sg & 2) ? -kL3 : kL3;
+    return 0.0;
   }
   j -= 2 * d * (d - 1);
-  for (int i = 0; i < d; ++i) u[i] = ((j >> i) & 1) ? -kL5 : kL5;
+  return ((j >> a) & 1) ? -kL5 : kL5;
 }
 
-__host__ __device__ inline void gm_weights(int d, double* w7, double* w5)
+// Weights per generator class (1..5) of the degree-7 rule, its embedded
+// degree-5 rule, and a degree-3 rule on the centre and the +-l3 axis nodes
+// (w3 = 1 / (6 l3^2) on those, 1 - 2d w3 at the centre).
+__host__ __device__ inline void gm_weights(int d, double* w7, double* w5, double* w3)
 {
   double const dd = d;
   w7[0] = 0.0;
   w5[0] = 0.0;
+  for (int g = 0; g < 6; ++g) w3[g] = 0.0;
+  w3[3] = 10.0 / 54.0;
+  w3[1] = 1.0 - 2.0 * dd * w3[3];
   w7[1] = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
   w7[2] = 980.0 / 6561.0;
   w7[3] = (1820.0 - 400.0 * dd) / 19683.0;

@@ -48,6 +48,7 @@ struct KernArgs {
   int cap;
   int kmax;
   int init_split;
+  double err_c, err_smin;
   // VEGAS
   long long vegas_calls;
   int vegas_itmax, vegas_itskip, vegas_nbins;

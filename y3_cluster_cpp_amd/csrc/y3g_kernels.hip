@@ -89,8 +89,12 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
   }
 }
 
+#ifndef Y3G_CUB_MIN_WAVES
+#define Y3G_CUB_MIN_WAVES 4
+#endif
+// FP32: at most 128 VGPRs, four waves per SIMD; FP64: the compiler's choice
 template <typename T, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
+__global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature(KernArgs A)
 {
   extern __shared__ char smem_raw[];
   CubShared<T, BLOCK>& S = *reinterpret_cast<CubShared<T, BLOCK>*>(smem_raw);
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
     int const iid = A.order ? A.order[item] : item;
     if (tid == 0) {
       S.ig = A.igs[iid];
-      gm_weights(S.ig.ndim, S.w7, S.w5);
+      gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
     }
     __syncthreads();
     int const pt = S.ig.point;
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
         S.h[r][a] = 0.5 * (h - l);
       }
       __syncthreads();
-      eval_chunk<T, BLOCK>(S, m);
+      eval_chunk<T, BLOCK>(S, m, A.err_c, A.err_smin);
       if (tid < m) {
         est[base + tid] = S.est[tid];
         err[base + tid] = S.err[tid];
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_cubature(KernArgs A)
           S.perr[tid] = err[p];
         }
         __syncthreads();
-        eval_chunk<T, BLOCK>(S, 2 * np);
+        eval_chunk<T, BLOCK>(S, 2 * np, A.err_c, A.err_smin);
         if (tid < np) {
           int const p = S.sel[pb + tid];
           double ea = S.err[2 * tid], eb = S.err[2 * tid + 1];

@@ -109,6 +109,8 @@ void y3g_options_default(y3g_options* o)
   o->vegas_nbins = 64;
   o->vegas_alpha = 1.5;
   o->single = 0;
+  o->err_c = 0.0;
+  o->err_smin = 1.0;
 }
 
 void y3g_default_params(y3g_params* p)
@@ -163,8 +165,8 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
   ig->zo_lo = zo_lo;
   ig->zo_hi = zo_hi;
   ig->radius = radius;
-  ig->lt_lo = 1.0;
-  ig->lt_hi = 300.0;
+  ig->lnlt_lo = 0.0;                   // ln 1
+  ig->lnlt_hi = 5.7037824746562009;    // ln 300
   ig->eps_rel = 1e-4;
   ig->eps_abs = 1e-300;
   ig->max_eval = 1LL << 40;
@@ -174,7 +176,7 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
   switch (kind) {
     case Y3G_COUNT3D:
       ig->ndim = 3;
-      ig->lows[0] = -kTMax, ig->highs[0] = kTMax;
+      ig->lows[0] = 0.0, ig->highs[0] = 1.0;
       ig->lows[1] = zt_lo, ig->highs[1] = zt_hi;
       ig->lows[2] = s->lnm_lo, ig->highs[2] = s->lnm_hi;
       break;
@@ -182,7 +184,7 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
       ig->ndim = 5;
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
       ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
-      ig->lows[2] = -kTMax, ig->highs[2] = kTMax;
+      ig->lows[2] = 0.0, ig->highs[2] = 1.0;
       ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
       ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;
       break;
@@ -192,11 +194,11 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
       ig->sigma_mis = p->mis_rlambda * rl;
       ig->rmis_norm = 1.0 / (1.0 - std::exp(-0.5 * 6.0 * 6.0));
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
-      ig->lows[1] = -kTMax, ig->highs[1] = kTMax;
+      ig->lows[1] = 0.0, ig->highs[1] = 1.0;
       ig->lows[2] = zt_lo, ig->highs[2] = zt_hi;
       ig->lows[3] = s->lnm_lo, ig->highs[3] = s->lnm_hi;
-      ig->lows[4] = 0.0, ig->highs[4] = 6.0 * ig->sigma_mis;
-      ig->lows[5] = 0.0, ig->highs[5] = 2.0 * 3.14159265358979323846;
+      ig->lows[4] = 0.0, ig->highs[4] = -std::expm1(-0.5 * 6.0 * 6.0);
+      ig->lows[5] = 0.0, ig->highs[5] = 3.14159265358979323846;
       break;
     }
   }
@@ -536,6 +538,8 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.cap = o.max_regions;
   A.kmax = o.max_split_per_step;
   A.init_split = o.init_split;
+  A.err_c = o.err_c;
+  A.err_smin = o.err_smin;
   A.vegas_calls = o.vegas_calls;
   A.vegas_itmax = o.vegas_itmax;
   A.vegas_itskip = o.vegas_itskip;

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
   if (tid == 0) {
     S.ig = *A.ig;
     S.P = A.K.points[S.ig.point];
-    gm_weights(S.ig.ndim, S.w7, S.w5);
+    gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
   }
   __syncthreads();
   stage_tables<T>(A.K, S.ig.point, S.hmf, S.da, S.sci);
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
       S.h[r][a] = 0.5 * (h - l);
     }
     __syncthreads();
-    eval_chunk<T, BLOCK>(S, m);
+    eval_chunk<T, BLOCK>(S, m, A.K.err_c, A.K.err_smin);
     if (!pairs) {
       if (tid < m) {
         int const g = A.list[base + tid];
