@@ -42,6 +42,12 @@ SEED = 0x59334331
 FLOPS_PER_EVAL = {0: 55, 1: 80, 2: 165}
 SFU_PER_EVAL = {0: 7, 1: 9, 2: 24}
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
+# HBM bytes per integrand evaluation of k_cubature<float,512>: rocprofv3
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 256 MiB copy) over
+# the evaluations of the same launch, profiles/k_cubature_prior_stream_pmc.md
+# (7.08e9 B / 1.829e9 evaluations).  Region-slab traffic; the integrand reads
+# only LDS-resident tables.
+HBM_BYTES_PER_EVAL = 7.0899e9 / 1.829061504e9
 
 
 def prior_points(n, seed=SEED):
@@ -149,7 +155,22 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=6)
     ap.add_argument("--cpu-sample", type=int, default=6)
+    ap.add_argument("--max-regions", type=int, default=0,
+                    help="per-integral region budget (0: library default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy (calibrates FETCH_SIZE / "
+                         "WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = 256 << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the "
               "integrals shard by parameter point with no exchange (see DESIGN.md); "
@@ -169,6 +190,8 @@ def main():
     rt = y3.Runtime(0)
     info = rt.device_info()
     opts = y3.options(precision=y3.FP32SPLIT)
+    if a.max_regions > 0:
+        opts.max_regions = a.max_regions
 
     def batch(step):
         ks = range(step * B, (step + 1) * B)
@@ -225,7 +248,9 @@ def main():
         idx = [v for pair in zip(cnt, dsg) for v in pair][: a.plain_sample]
         sub = (y3.Integral * len(idx))(*[arr[i] for i in idx])
         tp = time.perf_counter()
-        rp = rt.integrate(sub, y3.options(precision=y3.FP64, single=1))
+        popts = y3.options(precision=y3.FP64, single=1)
+        popts.max_regions = opts.max_regions
+        rp = rt.integrate(sub, popts)
         dtp = time.perf_counter() - tp
         okp, _ = count_ok(rp, ref[ks[0]][idx], eps[ks[0]][idx])
         plain = {"integrals_per_s": len(idx) / dtp, "counted_per_s": int(okp.sum()) / dtp,
@@ -264,7 +289,8 @@ def main():
                    "max_rel_err": float(rel_all.max()),
                    "device": info["name"]},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": nev_total / a.steps * HBM_BYTES_PER_EVAL,
                      "kernel": "k_cubature<float,512>", "kernel_ms_per_step": kern_ms / a.steps,
                      "evals_per_s": nev_total / (kern_ms * 1e-3) if kern_ms > 0 else 0.0},
         "cpu_baseline": cpu,
