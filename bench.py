@@ -47,7 +47,7 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # the evaluations of the same launch, profiles/k_cubature_prior_stream_pmc.md
 # (7.08e9 B / 1.829e9 evaluations).  Region-slab traffic; the integrand reads
 # only LDS-resident tables.
-HBM_BYTES_PER_EVAL = 7.0899e9 / 1.829061504e9
+HBM_BYTES_PER_EVAL = 7.0752e9 / 1.829061504e9
 
 
 def prior_points(n, seed=SEED):

@@ -3,7 +3,7 @@
  * library (liby3g.so).
  *
  * This is the thin layer between the host C++ (the CosmoSIS module template,
- * modules/y3g_module.hpp) and the HIP kernels.  Plain pointers and sizes
+ * y3_cluster_cpp_amd/modules/y3g_module.hpp) and the HIP kernels.  Plain pointers and sizes
  * only; every function returns 0 on success and a non-zero y3g_status
  * otherwise, with a message available from y3g_last_error().
  *
@@ -11,8 +11,8 @@
  * to the estevesjh/y3_cluster_cpp tree):
  *   y3g_runtime_create      DeviceInitializer, src/utils/CosmoSISSICUDAModule.hh:59-83
  *                           (cudaGetDeviceCount + cudaSetDevice(local_rank))
- *   y3g_points_upload /     the per-sample functor construction of
- *   y3g_tables_build        set_sample(), e.g. hmf_t.cuh:44-52 (mass_function
+ *   y3g_tables_build /      the per-sample functor construction of
+ *   y3g_tables_upload       set_sample(), e.g. hmf_t.cuh:44-52 (mass_function
  *                           table -> quad::Interp2D on the device),
  *                           dv_do_dz_t.cuh:13-17, and the table copies
  *                           quad::Interp* make inside gpuintegration
@@ -171,6 +171,23 @@ int y3g_tables_download(y3g_runtime* rt, int point, double* hmf, double* da,
 void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
                        double lo_lo, double lo_hi, double zo_lo, double zo_hi,
                        double radius, y3g_integral* out);
+
+/* Fill an integral descriptor from a physical integration volume and a grid
+ * point, as a CosmoSIS module reads them: the wall-of-numbers volumes of
+ * src/utils/make_cuda_integration_volumes.cuh:48-91 and the grid points of
+ * src/utils/make_grid_points.hh:267-284, consumed per integral by
+ * CosmoSISSICUDAModule.hh:295-300 (set_grid_point + integrate(volume)).
+ * Variables of the volume, in order (lc is the true richness):
+ *   Y3G_COUNT3D   (lc, zt, lnm)                  grid (lo_low, lo_high)
+ *   Y3G_COUNT5D   (lo, zo, lc, zt, lnm)          no grid
+ *   Y3G_DSIGMA6D  (lo, lc, zt, lnm, rmis, theta) grid (zo_low, zo_high, radius)
+ * rmis in Mpc/h (the offset pdf is integrated over [rmis_low, rmis_high]),
+ * theta in radians inside [0, 2 pi] (with the 1/2pi of the angular average,
+ * src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_scalarintegrand.cu:174-175).
+ * Host only; no device is needed. */
+int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
+                             int kind, const double* lows, const double* highs,
+                             const double* grid, int n_grid, y3g_integral* out);
 
 /* Integrate n integrals (one device submission, synchronous). */
 int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,

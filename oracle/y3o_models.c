@@ -502,7 +502,7 @@ void
 y3o_spec_config0(y3o_table_spec* s)
 {
   s->lnm_lo = 29.933606208922594; /* ln 1e13     */
-  s->lnm_hi = 35.690174623927322; /* 15.5 ln 10  */
+  s->lnm_hi = 35.690068941407709; /* 15.5 ln 10  */
   s->n_lnm = 200;
   s->z_lo = 0.20;
   s->z_hi = 0.35;

@@ -1,0 +1,178 @@
+"""GPU: the CosmoSIS modules end to end (setup / execute / cleanup through
+the C entry points, DataBlocks through the C API), against the golden
+values, against the library called directly on the same descriptors, and
+through properties of the integrals (additivity in the offset radius,
+theta symmetry), in the output layouts of CosmoSISSICUDAModule.hh:350-467.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import y3_cluster_cpp_amd as y3
+from conftest import GOLDEN
+from y3_cluster_cpp_amd import cosmosis as cs
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "golden_fiducial.json")) as f:
+    G = json.load(f)
+LO, ZO, RADII = G["lo_edges"], G["zo_edges"], G["radii"]
+SPEC = y3.spec_wide() if os.path.exists(y3.LIB_PATH) else None
+LNM = (SPEC.lnm_lo, SPEC.lnm_hi) if SPEC else (0, 0)
+
+
+def config(label, algorithm, eps_rel, cartesian, volumes, grid=None, extra=()):
+    c = cs.DataBlock()
+    for k, v in [("algorithm", algorithm), ("eps_rel", eps_rel), ("eps_abs", 0),
+                 ("max_eval", 100_000_000), ("use_cartesian_product", cartesian)] + list(extra):
+        c.put(label, k, v)
+    for var, (lo, hi) in volumes.items():
+        c.put(label, var + "_low", np.asarray(lo, dtype=float))
+        c.put(label, var + "_high", np.asarray(hi, dtype=float))
+    for var, vals in (grid or {}).items():
+        c.put(label, var, np.asarray(vals, dtype=float))
+    return c
+
+
+def sample(omega_m=0.3, **synthetic):
+    s = cs.DataBlock()
+    s.put("cosmological_parameters", "omega_m", omega_m)
+    for k, v in synthetic.items():
+        s.put("y3g_synthetic", k, float(v))
+    return s
+
+
+def run(label, cfg, smp):
+    m = cs.Module(label)
+    assert m.setup(cfg), "setup returned NULL"
+    rc = m.execute(smp)
+    assert rc == 0, rc
+    assert m.cleanup() == 0
+    return {k: smp.get_array(label, k) for k in ("vals", "errors", "probs", "status", "nregions")}
+
+
+def check(out, gold, eps):
+    gold = np.asarray(gold)
+    rel = np.abs(out["vals"] - gold) / np.abs(gold)
+    assert np.all(out["status"] == 0), out["status"]
+    assert np.all(rel <= eps), rel.max()
+    assert np.all(out["errors"] >= np.abs(out["vals"] - gold))
+
+
+def count5d_volumes(bins):
+    return {"lo": ([LO[i] for i, _ in bins], [LO[i + 1] for i, _ in bins]),
+            "zo": ([ZO[j] for _, j in bins], [ZO[j + 1] for _, j in bins]),
+            "lc": ([1.0] * len(bins), [300.0] * len(bins)),
+            "zt": ([ZO[j] - 0.1 for _, j in bins], [ZO[j + 1] + 0.1 for _, j in bins]),
+            "lnm": ([LNM[0]] * len(bins), [LNM[1]] * len(bins))}
+
+
+def test_counts5d_module_all_bins(runtime):
+    """The 12 number-count bins through the module (zipped, no grid) equal
+    the library on the standard descriptors and meet the golden values."""
+    L = "NumberCounts5DIntegrand"
+    bins = [(i, j) for i in range(4) for j in range(3)]
+    out = run(L, config(L, "pagani", 1e-4, False, count5d_volumes(bins)), sample())
+    assert out["vals"].shape == (12,) and np.all(out["probs"] == 0.0)
+    check(out, np.array(G["config1"]["values"]).reshape(-1), 1e-4)
+    p = y3.default_params()
+    runtime.tables_build(p, SPEC)
+    igs = [y3.make_integral(p, SPEC, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-4)
+           for i, j in bins]
+    for ig in igs:
+        ig.eps_abs = 0.0
+    direct = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT))
+    np.testing.assert_array_equal(out["vals"], direct["value"])
+    np.testing.assert_array_equal(out["nregions"], direct["nregions"])
+
+
+def test_counts3d_module_config0():
+    L = "NumberCounts3DIntegrand"
+    cfg = config(L, "pagani", 1e-4, True,
+                 {"lc": ([1.0], [300.0]), "zt": ([0.20], [0.35]), "lnm": ([LNM[0]], [LNM[1]])},
+                 {"lo_low": [20.0], "lo_high": [30.0]},
+                 extra=[("table_z_low", 0.20), ("table_z_high", 0.35)])
+    out = run(L, cfg, sample(f_prj=0.0))
+    assert out["vals"].shape == (1, 1)
+    check({k: v.reshape(-1) for k, v in out.items()}, [G["config0"]["value"]], 1e-4)
+
+
+def dsigma_volumes(ilo, rmis=(0.0, 1e300), theta=(0.0, 2 * np.pi), zt=(0.1, 0.45)):
+    return {"lo": ([LO[ilo]], [LO[ilo + 1]]), "lc": ([1.0], [300.0]), "zt": ([zt[0]], [zt[1]]),
+            "lnm": ([LNM[0]], [LNM[1]]), "rmis": ([rmis[0]], [rmis[1]]),
+            "theta": ([theta[0]], [theta[1]])}
+
+
+def test_dsigma_module_cartesian_layout_and_golden():
+    """One volume x 5 grid points (zo bin 0, five radii), cartesian: shapes
+    [1, 5], ivol, gridpoints, and the golden values within eps."""
+    L = "SigmaMiscent6DIntegrand"
+    ks = [0, 3, 7, 11, 14]
+    grid = {"zo_low": [ZO[0]] * 5, "zo_high": [ZO[1]] * 5, "radii": [RADII[k] for k in ks]}
+    cfg = config(L, "pagani", 1e-3, True, dsigma_volumes(1), grid)
+    smp = sample()
+    out = run(L, cfg, smp)
+    assert out["vals"].shape == (1, 5)
+    assert list(smp.get_array(L, "ivol").reshape(-1)) == [0] * 5
+    gp = smp.get_array(L, "gridpoints")
+    assert gp.shape == (5, 3) and np.allclose(gp[:, 2], grid["radii"])
+    gold = np.array(G["config2"]["values"])[1, 0, ks]
+    check({k: v.reshape(-1) for k, v in out.items()}, gold, 1e-3)
+
+
+def test_dsigma_module_offset_radius_additivity():
+    """Offset-radius ranges add: I[0, r1] + I[r1, inf) = I[0, inf)."""
+    L = "SigmaMiscent6DIntegrand"
+    r1 = 0.15
+    vols = [dsigma_volumes(0, rmis=r) for r in [(0.0, 1e300), (0.0, r1), (r1, 1e300)]]
+    merged = {k: ([v[k][0][0] for v in vols], [v[k][1][0] for v in vols]) for k in vols[0]}
+    grid = {"zo_low": [ZO[0]] * 3, "zo_high": [ZO[1]] * 3, "radii": [2.0] * 3}
+    out = run(L, config(L, "pagani", 1e-4, False, merged, grid), sample())
+    v, e = out["vals"], out["errors"]
+    assert np.all(out["status"] == 0)
+    assert 0 < v[1] < v[0] and 0 < v[2] < v[0]
+    assert abs(v[1] + v[2] - v[0]) <= e.sum(), (v, e)
+    assert abs(v[1] + v[2] - v[0]) <= 3e-4 * v[0]
+
+
+def test_dsigma_module_theta_half_circle():
+    """The tangential profile depends on cos(theta): the half circle [0, pi]
+    carries exactly half the angular weight of the full one."""
+    L = "SigmaMiscent6DIntegrand"
+    vols = [dsigma_volumes(2, theta=t) for t in [(0.0, 2 * np.pi), (0.0, np.pi), (np.pi, 2 * np.pi)]]
+    merged = {k: ([v[k][0][0] for v in vols], [v[k][1][0] for v in vols]) for k in vols[0]}
+    grid = {"zo_low": [ZO[1]] * 3, "zo_high": [ZO[2]] * 3, "radii": [2.0] * 3}
+    vols_zt = dict(merged)
+    vols_zt["zt"] = ([0.25] * 3, [0.6] * 3)
+    out = run(L, config(L, "pagani", 1e-4, False, vols_zt, grid), sample())
+    v, e = out["vals"], out["errors"]
+    assert np.all(out["status"] == 0)
+    for h, eh in ((v[1], e[1]), (v[2], e[2])):
+        assert abs(2 * h - v[0]) <= 2 * eh + e[0], (v, e)
+        assert abs(2 * h / v[0] - 1) <= 3e-4
+
+
+def test_module_mcubes_and_table_refresh():
+    """m-Cubes through the module: probs holds chi^2 in the cartesian
+    layout, values meet the golden ones within their reported errors, and a
+    second execute with another omega_m rebuilds the tables."""
+    L = "NumberCounts5DIntegrand"
+    bins = [(0, 0), (1, 1)]
+    cfg = config(L, "mcubes", 3e-3, True, count5d_volumes(bins),
+                 extra=[("vegas_calls", 2_000_000), ("vegas_itmax", 10)])
+    m = cs.Module(L)
+    assert m.setup(cfg)
+    s1, s2 = sample(0.3), sample(0.25)
+    assert m.execute(s1) == 0 and m.execute(s2) == 0
+    a, b = s1.get_array(L, "vals").reshape(-1), s2.get_array(L, "vals").reshape(-1)
+    assert s1.get_array(L, "vals").shape == (2, 1)
+    probs = s1.get_array(L, "probs").reshape(-1)
+    assert np.all(probs >= 0.0) and np.any(probs > 0.0)
+    gold = np.array(G["config1"]["values"])
+    ref = np.array([gold[0, 0], gold[1, 1]])
+    err = s1.get_array(L, "errors").reshape(-1)
+    assert np.all(np.abs(a - ref) <= err) and np.all(np.abs(a / ref - 1) <= 3e-3)
+    assert np.all(np.abs(b / a - 1) > 1e-2)   # omega_m changed the answer
+    assert m.cleanup() == 0

@@ -91,6 +91,8 @@ SIGNATURES = {
     "y3g_tables_download": (C.c_int, [C.c_void_p, C.c_int, _PD, _PD, _PD]),
     "y3g_make_integral": (None, [C.POINTER(Params), C.POINTER(TableSpec), C.c_int,
                                  _D, _D, _D, _D, _D, C.POINTER(Integral)]),
+    "y3g_make_integral_volume": (C.c_int, [C.POINTER(Params), C.POINTER(TableSpec), C.c_int,
+                                           _PD, _PD, _PD, C.c_int, C.POINTER(Integral)]),
     "y3g_integrate": (C.c_int, [C.c_void_p, C.POINTER(Integral), C.c_size_t,
                                 C.POINTER(Options), C.c_void_p]),
     "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
@@ -167,6 +169,25 @@ def make_integral(params, spec, kind, lo_lo=20.0, lo_hi=30.0, zo_lo=0.2,
     lib().y3g_make_integral(C.byref(params), C.byref(spec), kind, lo_lo, lo_hi,
                             zo_lo, zo_hi, radius, C.byref(ig))
     ig.point = point
+    ig.eps_rel = eps_rel
+    ig.algorithm = algorithm
+    ig.max_eval = max_eval
+    ig.seed = seed
+    return ig
+
+
+def make_integral_volume(params, spec, kind, lows, highs, grid=(), eps_rel=1e-4,
+                         algorithm=PAGANI, max_eval=1 << 40, seed=0x59334331):
+    """Integral descriptor from a physical volume and grid point (the module
+    path): see y3g_make_integral_volume in include/y3g.h for the variables."""
+    lo = np.ascontiguousarray(lows, dtype=np.float64)
+    hi = np.ascontiguousarray(highs, dtype=np.float64)
+    g = np.ascontiguousarray(grid if len(grid) else [0.0], dtype=np.float64)
+    ig = Integral()
+    _check(lib().y3g_make_integral_volume(C.byref(params), C.byref(spec), kind,
+                                          lo.ctypes.data_as(_PD), hi.ctypes.data_as(_PD),
+                                          g.ctypes.data_as(_PD), len(grid), C.byref(ig)),
+           "y3g_make_integral_volume")
     ig.eps_rel = eps_rel
     ig.algorithm = algorithm
     ig.max_eval = max_eval

@@ -136,7 +136,7 @@ void y3g_default_params(y3g_params* p)
 void y3g_spec_config0(y3g_table_spec* s)
 {
   s->lnm_lo = 29.933606208922594;  // ln 1e13
-  s->lnm_hi = 35.690174623927322;  // 15.5 ln 10
+  s->lnm_hi = 35.690068941407709;  // 15.5 ln 10
   s->n_lnm = 200;
   s->z_lo = 0.20;
   s->z_hi = 0.35;
@@ -202,6 +202,77 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
       break;
     }
   }
+}
+
+int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
+                             int kind, const double* lows, const double* highs,
+                             const double* grid, int n_grid, y3g_integral* ig)
+{
+  if (!p || !s || !lows || !highs || !ig)
+    return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: null argument");
+  int const nd = kind == Y3G_COUNT3D ? 3 : kind == Y3G_COUNT5D ? 5
+               : kind == Y3G_DSIGMA6D ? 6 : 0;
+  int const ng = kind == Y3G_COUNT3D ? 2 : kind == Y3G_COUNT5D ? 0 : 3;
+  if (nd == 0) return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: unknown kind");
+  if (n_grid != ng || (ng > 0 && !grid))
+    return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: kind " + std::to_string(kind) +
+                                 " takes " + std::to_string(ng) + " grid values, got " +
+                                 std::to_string(n_grid));
+  for (int i = 0; i < nd; ++i)
+    if (!(lows[i] <= highs[i]) || !std::isfinite(lows[i]))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: axis " + std::to_string(i) +
+                                   " has low > high or a non-finite low");
+  // physical axis positions of (lc, zt, lnm) per kind
+  int const ilc = kind == Y3G_COUNT3D ? 0 : kind == Y3G_COUNT5D ? 2 : 1;
+  int const izt = ilc + 1, ilm = ilc + 2;
+  double const zmax = kind == Y3G_DSIGMA6D ? std::min(s->da_zmax, s->sci_zmax) : s->da_zmax;
+  if (!(lows[ilc] > 0.0) || !std::isfinite(highs[ilc]))
+    return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: lc range must be finite and > 0");
+  if (!(lows[izt] >= 0.0) || !(highs[izt] <= zmax))
+    return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: zt range outside the distance "
+                             "tables [0, " + std::to_string(zmax) + "]");
+  if (!std::isfinite(highs[ilm]))
+    return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: lnm range must be finite");
+  if (kind == Y3G_COUNT3D) {
+    y3g_make_integral(p, s, kind, grid[0], grid[1], 0.0, 0.0, 0.0, ig);
+  } else if (kind == Y3G_COUNT5D) {
+    y3g_make_integral(p, s, kind, lows[0], highs[0], lows[1], highs[1], 0.0, ig);
+  } else {
+    if (!(grid[2] > 0.0) || !(grid[0] <= grid[1]))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: grid needs zo_low <= zo_high "
+                               "and radius > 0");
+    y3g_make_integral(p, s, kind, lows[0], highs[0], grid[0], grid[1], grid[2], ig);
+  }
+  ig->lnlt_lo = std::log(lows[ilc]);
+  ig->lnlt_hi = std::log(highs[ilc]);
+  ig->lows[izt] = lows[izt], ig->highs[izt] = highs[izt];
+  ig->lows[ilm] = lows[ilm], ig->highs[ilm] = highs[ilm];
+  if (kind == Y3G_DSIGMA6D) {
+    // offset radius as its Rayleigh CDF u = 1 - exp(-r^2 / 2 sigma^2), no
+    // renormalisation (the pdf is integrated over [rmis_low, rmis_high]),
+    // cut at u = 1 - exp(-18) (6 sigma); theta over [a, b] with the 1/2pi
+    // of the angular average, folded onto [0, pi] for the full circle
+    double const sg = ig->sigma_mis;
+    if (!(lows[4] >= 0.0))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: rmis_low must be >= 0");
+    double const twopi = 6.283185307179586;
+    if (!(lows[5] >= 0.0) || !(highs[5] <= twopi * (1.0 + 1e-12)))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: theta range must lie in [0, 2 pi]");
+    double const ucut = -std::expm1(-18.0);
+    auto cdf = [sg](double r) {
+      return std::isfinite(r) ? -std::expm1(-0.5 * (r / sg) * (r / sg)) : 1.0;
+    };
+    ig->lows[4] = std::min(cdf(lows[4]), ucut);
+    ig->highs[4] = std::min(cdf(highs[4]), ucut);
+    if (lows[5] == 0.0 && std::fabs(highs[5] - twopi) <= 1e-12 * twopi) {
+      ig->lows[5] = 0.0, ig->highs[5] = 3.14159265358979323846;
+      ig->rmis_norm = 1.0;
+    } else {
+      ig->lows[5] = lows[5], ig->highs[5] = highs[5];
+      ig->rmis_norm = 0.5;
+    }
+  }
+  return Y3G_OK;
 }
 
 int y3g_runtime_create(int device, y3g_runtime** out)

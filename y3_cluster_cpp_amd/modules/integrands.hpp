@@ -1,0 +1,153 @@
+// integrands.hpp -- the integrand classes of the CosmoSIS modules (one
+// module library per class, NumberCounts3DIntegrand.cc etc.).  Each class
+// models the reference's module-integrand concept: grid_t / grid_point_t,
+// a constructor from the config block, set_sample, set_grid_point,
+// module_label, make_integration_volumes and make_grid_points (reference:
+// src/utils/CosmoSISSICUDAModule.hh:99-110 and the modules under
+// src/modules/*/*_cuda_scalarintegrand.cu).  In place of the device
+// operator() they describe each integral for liby3g, whose device integrand
+// evaluates it.
+#ifndef Y3G_INTEGRANDS_HPP
+#define Y3G_INTEGRANDS_HPP
+
+#include "y3g_module.hpp"
+
+// NumberCounts3DIntegrand -- CosmoSIS module: cluster number counts in one
+// lambda_obs bin, integrated over (lc, zt, lnm) with the lambda_obs bin done
+// in closed form (lc is the true richness).
+//
+// Per integral:  int dlnM dzt dlc  n(lnM, zt) P(lc | lnM, zt) dV/dz(zt)
+//                    x [P(lo_high > lo > lo_low | lc)]
+// The integrand is the device integrand Y3G_COUNT3D of liby3g.  The module
+// concept mirrors the reference's count modules, e.g.
+// src/modules/mass_mort_y1/NC_cent_y1_mort_cuda_scalarintegrand.cu:164-175
+// (integrand body) and :190-196 (volumes).
+//
+// Config (<label> = NumberCounts3DIntegrand):
+//   algorithm, eps_rel, eps_abs, max_eval, use_cartesian_product
+//   lc_low/lc_high, zt_low/zt_high, lnm_low/lnm_high  -- volumes
+//   lo_low, lo_high                                   -- grid (lambda_obs bin)
+
+class NumberCounts3DIntegrand : public y3g_cosmosis::SyntheticSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<2>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit NumberCounts3DIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { bin_ = pt; }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_COUNT3D, v, bin_.data(), 2);
+  }
+  static char const* module_label() { return "NumberCounts3DIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lc", "zt", "lnm"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<2>(
+      cfg, module_label(), {"lo_low", "lo_high"});
+  }
+
+private:
+  grid_point_t bin_{};
+};
+
+// NumberCounts5DIntegrand -- CosmoSIS module: cluster number counts in a
+// (lambda_obs, z_obs) bin, integrated over (lo, zo, lc, zt, lnm).
+//
+// Per integral:  int dlo dzo dlc dzt dlnM  n(lnM, zt) P(lc | lnM, zt)
+//                    P(lo | lc) P(zo | zt) dV/dz(zt)
+// with P(lo | lc) the Gaussian + projection-tail (EMG) richness scatter and
+// P(zo | zt) the photometric-redshift Gaussian.  The integrand is the device
+// integrand Y3G_COUNT5D of liby3g; the module concept mirrors the reference's
+// src/modules/mass_mort_y1/NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-193.
+// The bins are the integration volumes themselves, so there is no grid.
+//
+// Config (<label> = NumberCounts5DIntegrand):
+//   algorithm, eps_rel, eps_abs, max_eval, use_cartesian_product
+//   lo_*, zo_*, lc_*, zt_*, lnm_* (_low / _high)      -- volumes
+
+class NumberCounts5DIntegrand : public y3g_cosmosis::SyntheticSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<0>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit NumberCounts5DIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t) {}
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_COUNT5D, v, nullptr, 0);
+  }
+  static char const* module_label() { return "NumberCounts5DIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lo", "zo", "lc", "zt", "lnm"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<0>(cfg, module_label(), {});
+  }
+};
+
+// SigmaMiscent6DIntegrand -- CosmoSIS module: the stacked, miscentred
+// excess surface density Delta-Sigma(R) of a (lambda_obs, z_obs) bin,
+// integrated over (lo, lc, zt, lnm, rmis, theta) with the z_obs bin in
+// closed form.
+//
+// Per integral:  int dlo dlc dzt dlnM drmis dtheta/2pi  n P(lc|lnM,zt)
+//                  P(lo|lc) [P(zo bin|zt)] dV/dz Sigma_crit^-1(zt) P(rmis)
+//                  [(1 - f_mis) DS_NFW(R) + f_mis DS_t(R; rmis, theta)]
+// where DS_t is the NFW Delta-Sigma at R' = sqrt(R^2 + rmis^2 + 2 R rmis
+// cos theta) projected on the tangential direction.  The integrand is the
+// device integrand Y3G_DSIGMA6D of liby3g.  Variable order, volumes and grid
+// follow the reference module
+// src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_scalarintegrand.cu:
+// operator() (lo, lc, zt, lnM, rmis, theta) :90-95 and :160-182, volumes
+// :190-203, grid (zo_low, zo_high, radii) :205-215.
+//
+// Config (<label> = SigmaMiscent6DIntegrand):
+//   algorithm, eps_rel, eps_abs, max_eval, use_cartesian_product
+//   lo_*, lc_*, zt_*, lnm_*, rmis_*, theta_* (_low / _high) -- volumes
+//   zo_low, zo_high, radii                                  -- grid
+
+class SigmaMiscent6DIntegrand : public y3g_cosmosis::SyntheticSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<3>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit SigmaMiscent6DIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_DSIGMA6D, v, gp_.data(), 3);
+  }
+  static char const* module_label() { return "SigmaMiscent6DIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lo", "lc", "zt", "lnm", "rmis", "theta"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<3>(
+      cfg, module_label(), {"zo_low", "zo_high", "radii"});
+  }
+
+private:
+  grid_point_t gp_{};
+};
+
+#endif

@@ -1,0 +1,621 @@
+// y3g_module.hpp -- host side of the CosmoSIS integration modules built on
+// liby3g (include/y3g.h).
+//
+// A module is one shared library exporting the three CosmoSIS entry points
+// setup / execute / cleanup (reference: src/utils/cuda_module_macros.cuh:20-45)
+// around HIPIntegrationModule<I>, the counterpart of the reference's
+// CosmoSISSICUDAModule<I> (src/utils/CosmoSISSICUDAModule.hh:99-467):
+//
+//   setup    reads <label>/{algorithm, eps_rel, eps_abs, max_eval,
+//            use_cartesian_product} and the integration volumes and grid
+//            points, and binds the GPU of the process's node-local rank;
+//   execute  reads the sample's parameters, (re)builds the per-sample tables
+//            on the device when they changed, integrates every
+//            (volume, grid point) pair in ONE device submission, and puts
+//            vals / errors / probs / status / nregions (+ ivol, gridpoints
+//            for the cartesian product) into the sample's <label> section;
+//   cleanup  deletes the module.
+//
+// Unlike the reference, exceptions never cross the extern "C" edge: setup
+// returns nullptr and execute a non-zero DATABLOCK_STATUS after printing the
+// error.  Every helper here is host code; the device work happens inside
+// liby3g.
+#ifndef Y3G_MODULE_HPP
+#define Y3G_MODULE_HPP
+
+#include "cosmosis/datablock/datablock.hh"
+#include "cosmosis/datablock/ndarray.hh"
+#include "y3g.h"
+
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <optional>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace y3g_cosmosis {
+
+  // ---------------------------------------------------------- config readers
+
+  // A std::vector<double> stored as vector<double>, double, vector<int> or
+  // int (reference: src/utils/datablock_reader.cc:3-28, same error text).
+  inline std::vector<double>
+  get_vector_double(cosmosis::DataBlock& db, std::string const& section,
+                    std::string const& name)
+  {
+    std::vector<double> v;
+    if (db.get_val(section, name, v) == DBS_SUCCESS) return v;
+    double x = 0.0;
+    if (db.get_val(section, name, x) == DBS_SUCCESS) return {x};
+    std::vector<int> iv;
+    if (db.get_val(section, name, iv) == DBS_SUCCESS)
+      return std::vector<double>(iv.begin(), iv.end());
+    int i = 0;
+    if (db.get_val(section, name, i) == DBS_SUCCESS) return {double(i)};
+    throw std::runtime_error("DataBlock contained no parameer in section " +
+                             section + " named " + name +
+                             " that could be coerced to type vector<double>");
+  }
+
+  // A scalar stored as double or int (reference:
+  // src/utils/CosmoSISSICUDAModule.hh:37-50).
+  inline double
+  view_as_double(cosmosis::DataBlock& cfg, std::string const& section,
+                 std::string const& name)
+  {
+    double x = 0.0;
+    if (cfg.get_val(section, name, x) == DBS_SUCCESS) return x;
+    int i = 0;
+    if (cfg.get_val(section, name, i) == DBS_SUCCESS) return double(i);
+    throw std::runtime_error("CosmoSISSICUDAModule: parameter " + section + "/" +
+                             name + " could not be read as double or int");
+  }
+
+  inline double
+  optional_double(cosmosis::DataBlock& db, std::string const& section,
+                  std::string const& name, double def)
+  {
+    return db.has_val(section, name) ? view_as_double(db, section, name) : def;
+  }
+
+  // ------------------------------------------------------ volumes and grids
+
+  struct volume_t {
+    std::vector<double> lows, highs;
+  };
+
+  // One volume per entry of <names[0]>_low; every <name>_low / _high list
+  // has that length (reference:
+  // src/utils/make_cuda_integration_volumes.cuh:48-91, 168-194).
+  inline std::vector<volume_t>
+  make_integration_volumes_wall_of_numbers(cosmosis::DataBlock& cfg,
+                                           std::string const& label,
+                                           std::vector<std::string> const& names)
+  {
+    std::vector<volume_t> out;
+    if (names.empty()) return out;
+    std::size_t nvol = 0;
+    for (std::size_t a = 0; a != names.size(); ++a) {
+      auto lo = get_vector_double(cfg, label, names[a] + "_low");
+      auto hi = get_vector_double(cfg, label, names[a] + "_high");
+      if (a == 0) {
+        nvol = lo.size();
+        out.assign(nvol, volume_t{std::vector<double>(names.size()),
+                                  std::vector<double>(names.size())});
+      }
+      if (lo.size() != nvol || hi.size() != nvol)
+        throw std::runtime_error(names[a] + " bad, bad user!");
+      for (std::size_t v = 0; v != nvol; ++v) {
+        out[v].lows[a] = lo[v];
+        out[v].highs[a] = hi[v];
+      }
+    }
+    return out;
+  }
+
+  // Odometer over lists, first list slowest (reference:
+  // src/utils/meta.hh:24-34).
+  inline std::vector<std::vector<double>>
+  cartesian_product(std::vector<std::vector<double>> const& axes)
+  {
+    std::vector<std::vector<double>> out;
+    if (axes.empty()) return out;
+    for (auto const& a : axes)
+      if (a.empty()) return out;
+    std::vector<std::size_t> k(axes.size(), 0);
+    while (true) {
+      std::vector<double> pt(axes.size());
+      for (std::size_t i = 0; i != axes.size(); ++i) pt[i] = axes[i][k[i]];
+      out.push_back(pt);
+      std::size_t i = axes.size();
+      while (i > 0) {
+        --i;
+        if (++k[i] < axes[i].size()) break;
+        k[i] = 0;
+        if (i == 0) return out;
+      }
+    }
+  }
+
+  // Lows and highs each taken as a cartesian product of the per-variable
+  // lists (reference: src/utils/make_cuda_integration_volumes.cuh:157-183,
+  // 196-222).
+  inline std::vector<volume_t>
+  make_integration_volumes_cartesian_product(cosmosis::DataBlock& cfg,
+                                             std::string const& label,
+                                             std::vector<std::string> const& names)
+  {
+    std::vector<std::vector<double>> lo, hi;
+    for (auto const& n : names) {
+      lo.push_back(get_vector_double(cfg, label, n + "_low"));
+      hi.push_back(get_vector_double(cfg, label, n + "_high"));
+    }
+    auto L = cartesian_product(lo), H = cartesian_product(hi);
+    if (L.size() != H.size())
+      throw std::runtime_error("cartesian product of volume bounds: lows and highs "
+                               "differ in number");
+    std::vector<volume_t> out(L.size());
+    for (std::size_t i = 0; i != L.size(); ++i) out[i] = {L[i], H[i]};
+    return out;
+  }
+
+  // Points of a space with NAXES named axes (reference:
+  // src/utils/make_grid_points.hh:18-60).
+  template <std::size_t NAXES>
+  struct grid_t {
+    using value_type = std::array<double, NAXES>;
+    std::vector<value_type> points;
+    std::vector<std::string> names;
+    std::size_t size() const { return points.size(); }
+    static constexpr std::size_t naxes() { return NAXES; }
+  };
+
+  // Axis lists zipped into points; unequal lengths are an error (reference:
+  // src/utils/make_grid_points.hh:224-240, 267-284).
+  template <std::size_t N>
+  grid_t<N>
+  make_grid_points_wall_of_numbers(cosmosis::DataBlock& cfg,
+                                   std::string const& label,
+                                   std::array<std::string, N> const& names)
+  {
+    grid_t<N> g;
+    g.names.assign(names.begin(), names.end());
+    if constexpr (N == 0) {
+      return g;
+    } else {
+      std::array<std::vector<double>, N> axes;
+      for (std::size_t i = 0; i != N; ++i)
+        axes[i] = get_vector_double(cfg, label, names[i]);
+      for (std::size_t i = 1; i != N; ++i)
+        if (axes[i].size() != axes[0].size())
+          throw std::invalid_argument("unequal length axes in make_grid_wall_of_numbers");
+      g.points.resize(axes[0].size());
+      for (std::size_t p = 0; p != axes[0].size(); ++p)
+        for (std::size_t i = 0; i != N; ++i) g.points[p][i] = axes[i][p];
+      return g;
+    }
+  }
+
+  // Cartesian product of the axis lists, first axis slowest (reference:
+  // src/utils/make_grid_points.hh:212-220, 247-265).
+  template <std::size_t N>
+  grid_t<N>
+  make_grid_points_cartesian_product(cosmosis::DataBlock& cfg,
+                                     std::string const& label,
+                                     std::array<std::string, N> const& names)
+  {
+    grid_t<N> g;
+    g.names.assign(names.begin(), names.end());
+    std::vector<std::vector<double>> axes;
+    for (auto const& n : names) axes.push_back(get_vector_double(cfg, label, n));
+    for (auto const& pt : cartesian_product(axes)) {
+      typename grid_t<N>::value_type v{};
+      for (std::size_t i = 0; i != N; ++i) v[i] = pt[i];
+      g.points.push_back(v);
+    }
+    return g;
+  }
+
+  // -------------------------------------------------------- process binding
+
+  // Node-local and global rank from the launcher's environment (the
+  // reference asks MPI, src/utils/mpi_support.cc:11-33; the variables below
+  // are what Open MPI, MPICH/PMI, Slurm and torchrun export).
+  struct rank_info {
+    int local_rank = 0;
+    int global_rank = 0;
+  };
+
+  inline int
+  env_int(std::initializer_list<char const*> names, int def)
+  {
+    for (char const* n : names)
+      if (char const* v = std::getenv(n)) return std::atoi(v);
+    return def;
+  }
+
+  inline rank_info
+  get_rank_info()
+  {
+    rank_info r;
+    r.local_rank = env_int({"OMPI_COMM_WORLD_LOCAL_RANK", "MPI_LOCALRANKID",
+                            "SLURM_LOCALID", "LOCAL_RANK"}, 0);
+    r.global_rank = env_int({"OMPI_COMM_WORLD_RANK", "PMI_RANK", "SLURM_PROCID",
+                             "RANK"}, 0);
+    return r;
+  }
+
+  inline void
+  record_timestamp(std::ostream& os, char const* label)
+  {
+    using namespace std::chrono;
+    double const t =
+      duration<double>(system_clock::now().time_since_epoch()).count();
+    os << label << '\t' << std::setprecision(17) << t << std::endl;
+  }
+
+  // One device binding per module object: the GPU whose index is the
+  // node-local rank (reference DeviceInitializer,
+  // src/utils/CosmoSISSICUDAModule.hh:59-83).
+  class DeviceBinding {
+  public:
+    explicit DeviceBinding(int device) : id(device)
+    {
+      if (y3g_runtime_create(device, &rt_) != Y3G_OK)
+        throw std::runtime_error(std::string("Failed to bind GPU ") +
+                                 std::to_string(device) + ": " + y3g_last_error());
+    }
+    ~DeviceBinding()
+    {
+      if (rt_) y3g_runtime_destroy(rt_);
+    }
+    DeviceBinding(DeviceBinding const&) = delete;
+    DeviceBinding& operator=(DeviceBinding const&) = delete;
+    y3g_runtime* get() const { return rt_; }
+    int id;
+
+  private:
+    y3g_runtime* rt_ = nullptr;
+  };
+
+  // ------------------------------------------------------- module settings
+
+  // Everything setup() reads, parsed before any device is touched.
+  template <class I>
+  struct module_config {
+    int algorithm = Y3G_PAGANI;
+    double eps_rel = 0.0, eps_abs = 0.0, max_eval = 0.0;
+    bool cartesian = false;
+    bool do_timing = false;
+    std::vector<volume_t> volumes;
+    typename I::grid_t grid;
+    y3g_options options{};
+    y3g_table_spec spec{};
+
+    // Number of integrals of one execute(): volumes x grid points for the
+    // cartesian product, else the zipped count.
+    std::size_t
+    num_results() const
+    {
+      std::size_t const ng = I::grid_t::naxes() == 0 ? 1 : grid.size();
+      return cartesian ? volumes.size() * ng : volumes.size();
+    }
+    std::size_t
+    num_grid_points() const
+    {
+      return I::grid_t::naxes() == 0 ? 1 : grid.size();
+    }
+  };
+
+  // "pagani" / "mcubes", anything else an error (reference:
+  // src/utils/gpu_integrator.cuh:54-65).
+  inline int
+  algorithm_from_name(std::string const& name)
+  {
+    if (name == "pagani") return Y3G_PAGANI;
+    if (name == "mcubes") return Y3G_MCUBES;
+    throw std::runtime_error("MultiDimensionalIntegrator::integrate called for "
+                             "unrecognized algorithm");
+  }
+
+  template <class I>
+  module_config<I>
+  parse_module_config(cosmosis::DataBlock& cfg)
+  {
+    std::string const L = I::module_label();
+    module_config<I> m;
+    m.algorithm = algorithm_from_name(cfg.view<std::string>(L, "algorithm"));
+    m.volumes = I::make_integration_volumes(cfg);
+    m.grid = I::make_grid_points(cfg);
+    m.eps_rel = view_as_double(cfg, L, "eps_rel");
+    m.eps_abs = view_as_double(cfg, L, "eps_abs");
+    m.cartesian = cfg.view<bool>(L, "use_cartesian_product");
+    m.max_eval = view_as_double(cfg, L, "max_eval");
+    if (m.max_eval <= 0.0)
+      throw std::runtime_error("MultiDimensionalIntegrator: maxcalls can not be negative\n");
+    if (!m.cartesian && I::grid_t::naxes() != 0 && m.volumes.size() != m.grid.size())
+      throw std::runtime_error(
+        "An integration module was configured to use a zipped sequence of "
+        "volumes and gridpoints,\nbut the number of volumes did not equal the "
+        "number of gridpoints.\n");
+    bool timing = false;
+    if (cfg.get_val(L, "do_timing", timing) == DBS_SUCCESS) m.do_timing = timing;
+
+    // device-side settings (defaults from the library)
+    y3g_options_default(&m.options);
+    std::string prec = "fp32split";
+    cfg.get_val(L, "precision", prec);
+    if (prec == "fp64")
+      m.options.precision = Y3G_FP64;
+    else if (prec == "fp32split")
+      m.options.precision = Y3G_FP32SPLIT;
+    else
+      throw std::runtime_error(L + "/precision must be fp32split or fp64, not " + prec);
+    m.options.max_regions =
+      int(optional_double(cfg, L, "max_regions", m.options.max_regions));
+    m.options.vegas_calls =
+      (long long)optional_double(cfg, L, "vegas_calls", double(m.options.vegas_calls));
+    m.options.vegas_itmax = int(optional_double(cfg, L, "vegas_itmax", m.options.vegas_itmax));
+
+    y3g_spec_wide(&m.spec);
+    m.spec.lnm_lo = optional_double(cfg, L, "table_lnm_low", m.spec.lnm_lo);
+    m.spec.lnm_hi = optional_double(cfg, L, "table_lnm_high", m.spec.lnm_hi);
+    m.spec.z_lo = optional_double(cfg, L, "table_z_low", m.spec.z_lo);
+    m.spec.z_hi = optional_double(cfg, L, "table_z_high", m.spec.z_hi);
+    return m;
+  }
+
+  // ------------------------------------------------------------- results
+
+  // The integral list of one execute(): volume-major over the grid for the
+  // cartesian product (reference: CosmoSISSICUDAModule.hh:282-313), else
+  // volume i with grid point i (:315-335).
+  template <class I>
+  std::vector<y3g_integral>
+  build_integrals(I& integrand, module_config<I> const& m)
+  {
+    std::vector<y3g_integral> out;
+    out.reserve(m.num_results());
+    auto add = [&](volume_t const& v) {
+      y3g_integral ig = integrand.describe(v);
+      ig.algorithm = m.algorithm;
+      ig.eps_rel = m.eps_rel;
+      ig.eps_abs = m.eps_abs;
+      // PAGANI takes no call limit (reference gpu_integrator.cuh:170); the
+      // region budget bounds it
+      ig.max_eval = m.algorithm == Y3G_MCUBES ? (long long)m.max_eval : (1LL << 62);
+      ig.point = 0;
+      out.push_back(ig);
+    };
+    std::size_t const ng = m.num_grid_points();
+    if (m.cartesian) {
+      for (auto const& v : m.volumes)
+        for (std::size_t g = 0; g != ng; ++g) {
+          if constexpr (I::grid_t::naxes() != 0) integrand.set_grid_point(m.grid.points[g]);
+          add(v);
+        }
+    } else {
+      for (std::size_t i = 0; i != m.volumes.size(); ++i) {
+        if constexpr (I::grid_t::naxes() != 0) integrand.set_grid_point(m.grid.points[i]);
+        add(m.volumes[i]);
+      }
+    }
+    return out;
+  }
+
+  // Put the result arrays into the sample (reference:
+  // CosmoSISSICUDAModule.hh:350-467).  probs is the chi^2 of m-Cubes in
+  // the cartesian layout and 0.0 in the zipped one, as the reference stores
+  // them (:306, :329).
+  template <class I>
+  void
+  put_results(cosmosis::DataBlock& sample, module_config<I> const& m,
+              std::vector<y3g_result> const& res)
+  {
+    using cosmosis::ndarray;
+    std::string const L = I::module_label();
+    std::size_t const n = m.num_results();
+    if (res.size() != n) throw std::logic_error("put_results: result count");
+    std::vector<double> vals(n), errs(n), probs(n);
+    std::vector<int> stat(n), nreg(n);
+    for (std::size_t i = 0; i != n; ++i) {
+      vals[i] = res[i].value;
+      errs[i] = res[i].error;
+      probs[i] = m.cartesian ? res[i].prob : 0.0;
+      stat[i] = res[i].status;
+      nreg[i] = res[i].nregions;
+    }
+    std::vector<std::size_t> ext;
+    if (m.cartesian)
+      ext = {m.volumes.size(), m.num_grid_points()};
+    else
+      ext = {n};
+    auto put = [&](char const* key, auto const& arr) {
+      if (sample.put_val(L, key, arr) != DBS_SUCCESS)
+        throw std::runtime_error(L + "/" + key + " could not be put into the sample");
+    };
+    put("vals", ndarray<double>(vals, ext));
+    put("errors", ndarray<double>(errs, ext));
+    put("probs", ndarray<double>(probs, ext));
+    put("status", ndarray<int>(stat, ext));
+    put("nregions", ndarray<int>(nreg, ext));
+    if (m.cartesian) {
+      std::size_t const ng = m.num_grid_points(), na = I::grid_t::naxes();
+      std::vector<int> ivol(n);
+      for (std::size_t v = 0; v != m.volumes.size(); ++v)
+        for (std::size_t g = 0; g != ng; ++g) ivol[v * ng + g] = int(v);
+      std::vector<double> gp(ng * na);
+      if constexpr (I::grid_t::naxes() != 0)
+        for (std::size_t g = 0; g != ng; ++g)
+          for (std::size_t a = 0; a != na; ++a) gp[g * na + a] = m.grid.points[g][a];
+      put("ivol", ndarray<int>(ivol, ext));
+      put("gridpoints", ndarray<double>(gp, {ng, na}));
+    }
+  }
+
+  // ------------------------------------------------------------- the module
+
+  template <class I>
+  class HIPIntegrationModule {
+  public:
+    explicit HIPIntegrationModule(cosmosis::DataBlock& cfg)
+      : cfg_(parse_module_config<I>(cfg)),
+        rank_(get_rank_info()),
+        device_(rank_.local_rank),
+        integrand_(cfg)
+    {
+      integrand_.set_spec(cfg_.spec);
+      if (cfg_.do_timing) {
+        std::ostringstream fn;
+        fn << "mpi_timing_" << I::module_label() << '_' << rank_.global_rank << ".txt";
+        timing_.emplace(fn.str());
+        record_timestamp(*timing_, "constructed");
+      }
+    }
+
+    ~HIPIntegrationModule()
+    {
+      if (timing_) record_timestamp(*timing_, "destroyed");
+    }
+
+    void
+    execute(cosmosis::DataBlock& sample)
+    {
+      if (timing_) record_timestamp(*timing_, "enter");
+      integrand_.set_sample(sample);
+      y3g_params const& p = integrand_.params();
+      if (!have_tables_ || std::memcmp(&p, &built_, sizeof(p)) != 0) {
+        if (y3g_tables_build(device_.get(), &p, &cfg_.spec, 1) != Y3G_OK)
+          throw std::runtime_error(std::string("table build: ") + y3g_last_error());
+        built_ = p;
+        have_tables_ = true;
+      }
+      std::vector<y3g_integral> igs = build_integrals(integrand_, cfg_);
+      std::vector<y3g_result> res(igs.size());
+      if (!igs.empty() &&
+          y3g_integrate(device_.get(), igs.data(), igs.size(), &cfg_.options,
+                        res.data()) != Y3G_OK)
+        throw std::runtime_error(std::string("integrate: ") + y3g_last_error());
+      put_results(sample, cfg_, res);
+      if (timing_) record_timestamp(*timing_, "exit");
+    }
+
+    module_config<I> const& config() const { return cfg_; }
+
+  private:
+    module_config<I> cfg_;
+    rank_info rank_;
+    DeviceBinding device_;
+    I integrand_;
+    y3g_params built_{};
+    bool have_tables_ = false;
+    std::optional<std::ofstream> timing_;
+  };
+
+  // ------------------------------------------------ the integrands' common part
+
+  // Per-sample physics of the synthetic integrands: omega_m (and h0) from
+  // cosmological_parameters, the rest from the optional y3g_synthetic section
+  // (defaults y3g_default_params).
+  class SyntheticSample {
+  public:
+    void
+    set_sample(cosmosis::DataBlock& sample)
+    {
+      y3g_default_params(&p_);
+      std::string const C = "cosmological_parameters", S = "y3g_synthetic";
+      p_.omega_m = view_as_double(sample, C, "omega_m");
+      p_.h = optional_double(sample, C, "h0", p_.h);
+      struct {
+        char const* key;
+        double* dst;
+      } const keys[] = {
+        {"hmf_amp", &p_.hmf_amp},         {"hmf_A", &p_.hmf_A},
+        {"hmf_a", &p_.hmf_a},             {"hmf_c", &p_.hmf_c},
+        {"mor_lnl0", &p_.mor_lnl0},       {"mor_slope", &p_.mor_slope},
+        {"mor_zslope", &p_.mor_zslope},   {"mor_sigma_ln", &p_.mor_sigma_ln},
+        {"f_prj", &p_.f_prj},             {"prj_scale", &p_.prj_scale},
+        {"f_mis", &p_.f_mis},             {"mis_rlambda", &p_.mis_rlambda},
+        {"z_source", &p_.z_source},       {"sigz0", &p_.sigz0}};
+      for (auto const& k : keys) *k.dst = optional_double(sample, S, k.key, *k.dst);
+    }
+    void set_spec(y3g_table_spec const& s) { spec_ = s; }
+    y3g_params const& params() const { return p_; }
+
+  protected:
+    y3g_integral
+    describe_kind(int kind, volume_t const& v, double const* grid, int ngrid) const
+    {
+      y3g_integral ig;
+      if (y3g_make_integral_volume(&p_, &spec_, kind, v.lows.data(), v.highs.data(),
+                                   grid, ngrid, &ig) != Y3G_OK)
+        throw std::runtime_error(y3g_last_error());
+      return ig;
+    }
+    y3g_params p_{};
+    y3g_table_spec spec_{};
+  };
+
+} // namespace y3g_cosmosis
+
+// The three CosmoSIS entry points for module class `klass` (reference:
+// src/utils/cuda_module_macros.cuh:20-45), with exceptions stopped at the
+// C boundary.
+#define DEFINE_COSMOSIS_HIP_INTEGRATION_MODULE(klass)                          \
+  using y3g_module_type = y3g_cosmosis::HIPIntegrationModule<klass>;           \
+  extern "C" {                                                                 \
+  void*                                                                        \
+  setup(cosmosis::DataBlock* cfg)                                              \
+  {                                                                            \
+    if (!cfg) return nullptr;                                                  \
+    try {                                                                      \
+      return new y3g_module_type(*cfg);                                        \
+    }                                                                          \
+    catch (std::exception const& e) {                                          \
+      std::cerr << "\nDuring construction of the module " << klass::module_label() \
+                << " an exception was thrown:\n" << e.what() << '\n';          \
+    }                                                                          \
+    catch (...) {                                                              \
+      std::cerr << "\nUnknown exception while constructing the module "        \
+                << klass::module_label() << "\n";                              \
+    }                                                                          \
+    return nullptr;                                                            \
+  }                                                                            \
+                                                                               \
+  DATABLOCK_STATUS                                                             \
+  execute(cosmosis::DataBlock* sample, void* module)                           \
+  {                                                                            \
+    if (!sample) return DBS_DATABLOCK_NULL;                                    \
+    if (!module) return DBS_LOGIC_ERROR;                                       \
+    try {                                                                      \
+      static_cast<y3g_module_type*>(module)->execute(*sample);                 \
+      return DBS_SUCCESS;                                                      \
+    }                                                                          \
+    catch (std::exception const& e) {                                          \
+      std::cerr << "\nModule " << klass::module_label() << " execute: "        \
+                << e.what() << '\n';                                           \
+    }                                                                          \
+    catch (...) {                                                              \
+      std::cerr << "\nModule " << klass::module_label()                        \
+                << " execute: unknown exception\n";                            \
+    }                                                                          \
+    return DBS_LOGIC_ERROR;                                                    \
+  }                                                                            \
+                                                                               \
+  int                                                                          \
+  cleanup(void* module)                                                        \
+  {                                                                            \
+    delete static_cast<y3g_module_type*>(module);                              \
+    return 0;                                                                  \
+  }                                                                            \
+  }
+
+#endif
