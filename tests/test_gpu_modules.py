@@ -74,12 +74,12 @@ def test_counts5d_module_all_bins(runtime):
     the library on the standard descriptors and meet the golden values."""
     L = "NumberCounts5DIntegrand"
     bins = [(i, j) for i in range(4) for j in range(3)]
-    out = run(L, config(L, "pagani", 1e-4, False, count5d_volumes(bins)), sample())
+    out = run(L, config(L, "pagani", 1e-3, False, count5d_volumes(bins)), sample())
     assert out["vals"].shape == (12,) and np.all(out["probs"] == 0.0)
-    check(out, np.array(G["config1"]["values"]).reshape(-1), 1e-4)
+    check(out, np.array(G["config1"]["values"]).reshape(-1), 1e-3)
     p = y3.default_params()
     runtime.tables_build(p, SPEC)
-    igs = [y3.make_integral(p, SPEC, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-4)
+    igs = [y3.make_integral(p, SPEC, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-3)
            for i, j in bins]
     for ig in igs:
         ig.eps_abs = 0.0
@@ -154,25 +154,43 @@ def test_dsigma_module_theta_half_circle():
         assert abs(2 * h / v[0] - 1) <= 3e-4
 
 
-def test_module_mcubes_and_table_refresh():
-    """m-Cubes through the module: probs holds chi^2 in the cartesian
-    layout, values meet the golden ones within their reported errors, and a
-    second execute with another omega_m rebuilds the tables."""
+def test_module_mcubes_and_table_refresh(runtime):
+    """m-Cubes through the module: probs holds the chi^2/dof the library
+    reports in the cartesian layout, values meet the golden ones within
+    their reported errors, and a second execute with another omega_m
+    rebuilds the tables."""
     L = "NumberCounts5DIntegrand"
     bins = [(0, 0), (1, 1)]
-    cfg = config(L, "mcubes", 3e-3, True, count5d_volumes(bins),
-                 extra=[("vegas_calls", 2_000_000), ("vegas_itmax", 10)])
+    vols = count5d_volumes(bins)
+    cfg = config(L, "mcubes", 2e-3, True, vols,
+                 extra=[("vegas_calls", 100_000), ("vegas_itmax", 40)])
     m = cs.Module(L)
     assert m.setup(cfg)
     s1, s2 = sample(0.3), sample(0.25)
     assert m.execute(s1) == 0 and m.execute(s2) == 0
+    assert m.cleanup() == 0
     a, b = s1.get_array(L, "vals").reshape(-1), s2.get_array(L, "vals").reshape(-1)
     assert s1.get_array(L, "vals").shape == (2, 1)
     probs = s1.get_array(L, "probs").reshape(-1)
-    assert np.all(probs >= 0.0) and np.any(probs > 0.0)
+    err = s1.get_array(L, "errors").reshape(-1)
     gold = np.array(G["config1"]["values"])
     ref = np.array([gold[0, 0], gold[1, 1]])
-    err = s1.get_array(L, "errors").reshape(-1)
-    assert np.all(np.abs(a - ref) <= err) and np.all(np.abs(a / ref - 1) <= 3e-3)
+    assert np.all(np.abs(a - ref) <= err) and np.all(np.abs(a / ref - 1) <= 2e-3)
     assert np.all(np.abs(b / a - 1) > 1e-2)   # omega_m changed the answer
-    assert m.cleanup() == 0
+    # the same descriptors through the library directly
+    p = y3.default_params()
+    runtime.tables_build(p, SPEC)
+    igs = []
+    for k in range(2):
+        lo = [vols[v][0][k] for v in ("lo", "zo", "lc", "zt", "lnm")]
+        hi = [vols[v][1][k] for v in ("lo", "zo", "lc", "zt", "lnm")]
+        ig = y3.make_integral_volume(p, SPEC, y3.COUNT5D, lo, hi, eps_rel=2e-3,
+                                     algorithm=y3.MCUBES, max_eval=100_000_000)
+        ig.eps_abs = 0.0
+        igs.append(ig)
+    direct = runtime.integrate(igs, y3.options(vegas_calls=100_000, vegas_itmax=40))
+    # the VEGAS histograms accumulate with LDS atomics, so grids (and values)
+    # agree to rounding, not bit for bit
+    np.testing.assert_allclose(a, direct["value"], rtol=1e-8)
+    np.testing.assert_allclose(probs, direct["prob"], rtol=1e-5, atol=1e-12)
+    assert np.all(probs >= 0.0)
