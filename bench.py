@@ -1,27 +1,38 @@
 """Throughput of the cluster-observable integration path on one MI355X.
 
-Workload: the batched prior-box stream -- parameter points drawn from the
-box Omega_m in [0.2, 0.4], HMF amplitude in [0.7, 0.9], MOR slope in
-[0.7, 1.1], sigma_ln in [0.1, 0.4], f_mis in [0, 0.4] (Philox key
-0x59334331), each point carrying 192 integrals: 12 five-dimensional number
-counts (4 lambda_obs x 3 z_obs bins) and 180 six-dimensional miscentred
-Delta-Sigma integrals (12 bins x 15 radii log-spaced in [0.1, 30] Mpc/h),
-per-integral eps_rel log-uniform in [1e-5, 1e-3], capped at 1e-4 for the
-headline.  Per-point tables (HMF 200x32, D_A 512, Sigma_crit^-1 128) are
-rebuilt on the device inside each step.
+Workload (y3_cluster_cpp_amd/workloads.py): the batched prior-box stream --
+parameter points drawn from Omega_m in [0.2, 0.4], HMF amplitude in
+[0.7, 0.9], MOR slope in [0.7, 1.1], sigma_ln in [0.1, 0.4], f_mis in
+[0, 0.4], each carrying 192 integrals (12 five-dimensional number counts,
+180 six-dimensional miscentred Delta-Sigma integrals), per-integral eps_rel
+log-uniform in [1e-5, 1e-3], capped at 1e-4 for the headline.  Per-point
+tables (HMF 200x32, D_A 512, Sigma_crit^-1 128) are rebuilt on the device
+inside each step.
 
-A step is one batch of parameter points taken in stream order.  An integral
-is counted when |I - I_oracle| <= eps_rel |I_oracle|, that error is <= 1e-4,
-and the reported error bounds it; I_oracle are the FP64 CPU oracle's golden
-values of the stream's first points (tests/golden/golden_points.json), so
-every integral of the timed steps is checked.
+A step is one batch of parameter points in stream order, integrated by one
+persistent cubature launch.  An integral is counted when
+|I - I_oracle| <= eps_rel |I_oracle|, that error is <= 1e-4, and the
+reported error bounds it; I_oracle are the FP64 oracle's golden values of
+the stream's first 64 points (tests/golden/golden_points.json), so every
+integral of the timed steps is checked.
+
+Beside the headline the JSON line carries
+  roofline      the cubature kernel's FP32 work rate against the FP32 peak
+                (HIP-event kernel time of the library's stream), and its
+                calibrated HBM traffic per launch;
+  cpu_baseline  the oracle's FP64 integrands under its CPU restatement of
+                the same cubature, OpenMP over integrals, on a bounded
+                sample of the first step's integrals;
+  plain_fp64    the plain path (one integral at a time, FP64, device-wide
+                cubature) on a larger sample of the same integrals;
+  serving       request latency p50/p99 and integrals/s of the request
+                server under Poisson arrivals of mock-MCMC requests.
 
     python bench.py [--steps K] [--warmup W] [--points-per-step B]
 """
 import argparse
 import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -31,10 +42,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-LO_EDGES = [20.0, 30.0, 45.0, 60.0, 300.0]
-ZO_EDGES = [0.20, 0.35, 0.50, 0.65]
-RADII = list(np.logspace(np.log10(0.1), np.log10(30.0), 15))
-SEED = 0x59334331
+from y3_cluster_cpp_amd.workloads import (  # noqa: E402
+    LO_EDGES, N_COUNTS, N_PER_POINT, RADII, SEED, ZO_EDGES, count_ok, eps_stream,
+    mcmc_requests, point_integrals, point_params)
 
 # FP32 operations per integrand evaluation of the device integrands
 # (+ - * / = 1, FMA = 2) and transcendental (SFU) calls, counted from
@@ -45,44 +55,10 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # HBM bytes per integrand evaluation of k_cubature<float,512>: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 256 MiB copy) over
 # the evaluations of the same launch, profiles/k_cubature_prior_stream_pmc.md
-# (7.08e9 B / 1.829e9 evaluations).  Region-slab traffic; the integrand reads
-# only LDS-resident tables.
+# (7.075e9 B / 1.829e9 evaluations).  Region-slab traffic; the integrand
+# reads only LDS-resident tables.
 HBM_BYTES_PER_EVAL = 7.0752e9 / 1.829061504e9
-
-
-def prior_points(n, seed=SEED):
-    rng = np.random.Generator(np.random.Philox(key=seed))
-    u = rng.random((n, 5))
-    lo = np.array([0.2, 0.7, 0.7, 0.1, 0.0])
-    hi = np.array([0.4, 0.9, 1.1, 0.4, 0.4])
-    return lo + u * (hi - lo)
-
-
-def eps_stream(n_points, seed=SEED):
-    """Per-integral eps_rel, log-uniform in [1e-5, 1e-3], 192 per point."""
-    rng = np.random.Generator(np.random.Philox(key=seed + 1))
-    return 10.0 ** rng.uniform(-5.0, -3.0, (n_points, 192))
-
-
-def point_integrals(y3, params, spec, k, eps_row, algorithm):
-    igs = []
-    n = 0
-    for i in range(4):
-        for j in range(3):
-            g = y3.make_integral(params, spec, y3.COUNT5D, LO_EDGES[i], LO_EDGES[i + 1],
-                                 ZO_EDGES[j], ZO_EDGES[j + 1], point=k,
-                                 eps_rel=float(eps_row[n]), algorithm=algorithm)
-            igs.append(g)
-            n += 1
-    for i in range(4):
-        for j in range(3):
-            for r in RADII:
-                g = y3.make_integral(params, spec, y3.DSIGMA6D, LO_EDGES[i], LO_EDGES[i + 1],
-                                     ZO_EDGES[j], ZO_EDGES[j + 1], r, point=k,
-                                     eps_rel=float(eps_row[n]), algorithm=algorithm)
-                igs.append(g)
-                n += 1
-    return igs
+DEFAULT_REGIONS = 32768
 
 
 def load_golden():
@@ -94,54 +70,115 @@ def load_golden():
     return pts, ref
 
 
-def count_ok(res, ref, eps):
-    err = np.abs(res["value"] - ref)
-    rel = err / np.abs(ref)
-    ok = (rel <= eps) & (rel <= 1e-4) & (res["error"] >= err)
-    return ok, rel
+def sample_indices(n_points, n, seed=SEED + 3):
+    """(point, integral) pairs in the workload's mix (1 count : 15
+    Delta-Sigma), drawn once with a fixed key; counts at positions 0, 16,
+    32, ... so every prefix keeps the mix."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    out = []
+    for s in range(n):
+        k = int(rng.integers(n_points))
+        if s % 16 == 0:
+            out.append((k, int(rng.integers(N_COUNTS))))
+        else:
+            out.append((k, N_COUNTS + int(rng.integers(N_PER_POINT - N_COUNTS))))
+    return out
 
 
-def cpu_baseline(points, eps, n_sample, threads):
+def cpu_baseline(pts, eps, sample, max_regions, threads):
     """The oracle's FP64 integrands under its CPU restatement of the same
-    cubature, one integral per thread (OpenMP), on a bounded sample."""
+    cubature, one integral per OpenMP thread, on `sample`."""
     import oracle as O
-    k = 0
-    om, amp, slope, sln, fmis = points[k]
-    op = O.default_params(omega_m=om, hmf_amp=amp, mor_slope=slope, mor_sigma_ln=sln,
-                          f_mis=fmis)
+    ks = sorted(set(k for k, _ in sample))
+    pidx = {k: i for i, k in enumerate(ks)}
+    params = [O.default_params(omega_m=pts[k][0], hmf_amp=pts[k][1], mor_slope=pts[k][2],
+                               mor_sigma_ln=pts[k][3], f_mis=pts[k][4]) for k in ks]
     spec = O.spec_wide()
-    igs = []
-    epsl = []
-    # sample: the 12 count integrals and the first Delta-Sigma radii, in order
-    n = 0
-    for i in range(4):
-        for j in range(3):
-            igs.append(O.make_integral(op, spec, O.COUNT5D, LO_EDGES[i], LO_EDGES[i + 1],
-                                       ZO_EDGES[j], ZO_EDGES[j + 1], 1.0))
-            epsl.append(eps[k, n])
-            n += 1
-    for i in range(4):
-        for j in range(3):
-            for r in RADII:
-                igs.append(O.make_integral(op, spec, O.DSIGMA6D, LO_EDGES[i], LO_EDGES[i + 1],
-                                           ZO_EDGES[j], ZO_EDGES[j + 1], r))
-                epsl.append(eps[k, n])
-                n += 1
-    cnt = list(range(0, 12, 2))
-    dsg = list(range(12, 192, 11))
-    idx = [v for pair in zip(cnt, dsg) for v in pair][:n_sample]
-    arr = (O.Integral * len(idx))(*[igs[i] for i in idx])
-    pt = (C.c_int * len(idx))(*([0] * len(idx)))
-    e = np.ascontiguousarray([epsl[i] for i in idx])
-    out = (O.Result * len(idx))()
-    pa = (O.Params * 1)(op)
-    sa = (O.TableSpec * 1)(spec)
+    igs, pt, el = [], [], []
+    for k, i in sample:
+        if i < N_COUNTS:
+            a, b = divmod(i, 3)
+            g = O.make_integral(params[pidx[k]], spec, O.COUNT5D, LO_EDGES[a], LO_EDGES[a + 1],
+                                ZO_EDGES[b], ZO_EDGES[b + 1], 1.0)
+        else:
+            a, r = divmod(i - N_COUNTS, len(RADII))
+            lo, zo = divmod(a, 3)
+            g = O.make_integral(params[pidx[k]], spec, O.DSIGMA6D, LO_EDGES[lo],
+                                LO_EDGES[lo + 1], ZO_EDGES[zo], ZO_EDGES[zo + 1], RADII[r])
+        igs.append(g)
+        pt.append(pidx[k])
+        el.append(eps[k, i])
+    n = len(igs)
+    arr = (O.Integral * n)(*igs)
+    pa = (O.Params * len(params))(*params)
+    sa = (O.TableSpec * len(params))(*([spec] * len(params)))
+    out = (O.Result * n)()
+    e = np.ascontiguousarray(el)
     t0 = time.perf_counter()
-    used = O.lib.y3o_batch_integrate(pa, sa, 1, arr, pt, O.ptr(e), len(idx), 0,
-                                     C.byref(O.cub_opts(max_regions=262144, max_eval=1 << 40)),
+    used = O.lib.y3o_batch_integrate(pa, sa, len(params), arr, (C.c_int * n)(*pt), O.ptr(e), n,
+                                     0, C.byref(O.cub_opts(max_regions=max_regions,
+                                                           max_eval=1 << 40)),
                                      C.byref(O.vegas_opts()), threads, out)
     dt = time.perf_counter() - t0
-    return len(idx) / dt, used, dt, len(idx)
+    res = np.array([(r.value, r.error) for r in out], dtype=[("value", "f8"), ("error", "f8")])
+    return res, used, dt
+
+
+def serving_run(y3, pts, ref, eps_all, rates, seconds, max_regions, lanes):
+    """Poisson arrivals of mock-MCMC requests at each rate; latency from the
+    server's own clock (submission to completion)."""
+    spec = y3.spec_wide()
+    cache = {}
+
+    def integrals(k, full):
+        key = (k, full)
+        if key not in cache:
+            p = point_params(y3, pts[k])
+            igs = point_integrals(y3, p, spec, 0, eps_all[k], y3.PAGANI, counts_only=not full)
+            cache[key] = (p, (y3.Integral * len(igs))(*igs))
+        return cache[key]
+
+    o = y3.server_options(lanes=lanes, cache_entries=0)
+    o.options.max_regions = max_regions
+    srv = y3.Server(0, o)
+    p, arr = integrals(0, True)  # first-launch costs outside the measurement
+    srv.wait(srv.submit(p, arr))
+    out = []
+    for rate in rates:
+        n = max(4, int(rate * seconds))
+        t_arr, reqs = mcmc_requests(pts, n, rate)
+        for k, full in reqs:
+            integrals(k, full)
+        sent = []
+        t0 = time.perf_counter()
+        for (k, full), ta in zip(reqs, t_arr):
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            p, arr = integrals(k, full)
+            sent.append((k, len(arr), srv.submit(p, arr)))
+        lat, n_int, n_ok = [], 0, 0
+        for k, m, t in sent:
+            res, lt = srv.wait(t)
+            lat.append(lt)
+            ok, _ = count_ok(res, ref[k, :m], eps_all[k, :m])
+            n_int += m
+            n_ok += int(ok.sum())
+        wall = time.perf_counter() - t0
+        lat = np.array(lat) * 1e3
+        out.append({"rate_req_per_s": rate, "requests": n, "integrals": n_int,
+                    "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)),
+                    "integrals_per_s": n_int / wall, "counted_per_s": n_ok / wall,
+                    "counted_frac": n_ok / n_int, "wall_s": wall})
+        print(f"serving: rate {rate}/s, {n} requests, p50 {out[-1]['p50_ms']:.1f} ms, "
+              f"p99 {out[-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    return {"rates": out, "lanes": st["lanes"], "max_regions": max_regions, "cache": "off",
+            "requests": "mock random-walk MCMC over the first 64 prior points, 30% counts + "
+                        "Delta-Sigma (192 integrals), 70% counts (12), eps_rel as streamed "
+                        "(log-uniform [1e-5, 1e-3]); counted with the headline rule"}
 
 
 def main():
@@ -151,12 +188,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points-per-step", type=int, default=16)
     ap.add_argument("--eps-cap", type=float, default=1e-4)
+    ap.add_argument("--max-regions", type=int, default=DEFAULT_REGIONS,
+                    help="per-integral region budget of the cubature")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-sample", type=int, default=6)
-    ap.add_argument("--cpu-sample", type=int, default=6)
-    ap.add_argument("--max-regions", type=int, default=0,
-                    help="per-integral region budget (0: library default)")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=24)
+    ap.add_argument("--serve-rates", default="4,12,24")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (calibrates FETCH_SIZE / "
                          "WRITE_SIZE counters) and exit")
@@ -173,8 +214,8 @@ def main():
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the "
-              "integrals shard by parameter point with no exchange (see DESIGN.md); "
-              "run with --gpus 1", file=sys.stderr)
+              "integrals shard by parameter point with no exchange; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import y3_cluster_cpp_amd as y3
@@ -183,15 +224,13 @@ def main():
     need = B * max(a.steps, 1)
     if need > len(pts):
         raise SystemExit(f"{need} points needed, {len(pts)} have oracle values")
-    eps = np.minimum(eps_stream(len(pts)), a.eps_cap)
+    eps_raw = eps_stream(len(pts))
+    eps = np.minimum(eps_raw, a.eps_cap)
     spec = y3.spec_wide()
-    params_all = [y3.default_params(omega_m=om, hmf_amp=amp, mor_slope=sl, mor_sigma_ln=sn,
-                                    f_mis=fm) for om, amp, sl, sn, fm in pts]
+    params_all = [point_params(y3, pt) for pt in pts]
     rt = y3.Runtime(0)
     info = rt.device_info()
-    opts = y3.options(precision=y3.FP32SPLIT)
-    if a.max_regions > 0:
-        opts.max_regions = a.max_regions
+    opts = y3.options(precision=y3.FP32SPLIT, max_regions=a.max_regions)
 
     def batch(step):
         ks = range(step * B, (step + 1) * B)
@@ -201,8 +240,7 @@ def main():
         return list(ks), (y3.Integral * len(igs))(*igs)
 
     batches = [batch(s) for s in range(a.steps)]
-    # warmup (untimed): the first batch
-    for _ in range(a.warmup):
+    for _ in range(a.warmup):  # untimed: the first batch
         ks, arr = batches[0]
         rt.tables_build([params_all[k] for k in ks], spec)
         rt.integrate(arr, opts)
@@ -223,48 +261,73 @@ def main():
         flops += float(sum(r["neval"][kinds == k].sum() * FLOPS_PER_EVAL[k] for k in (1, 2)))
         results.append(r)
     elapsed = time.perf_counter() - t0
+    print(f"timed steps: {a.steps} x {B} points in {elapsed:.2f} s", file=sys.stderr, flush=True)
 
-    ok_all = []
-    rel_all = []
+    ok_all, rel_all, status_all = [], [], []
     for s in range(a.steps):
         ks = list(range(s * B, (s + 1) * B))
         ok, rel = count_ok(results[s], ref[ks].reshape(-1), eps[ks].reshape(-1))
         ok_all.append(ok)
         rel_all.append(rel)
+        status_all.append(results[s]["status"])
     ok_all = np.concatenate(ok_all)
     rel_all = np.concatenate(rel_all)
+    status_all = np.concatenate(status_all)
     n_int = len(ok_all)
     counted = int(ok_all.sum())
     value = counted / elapsed
 
-    # plain FP64 path: one integral per launch sequence, serial over integrals
+    # samples of the first step's integrals for the two baselines
+    sample = sample_indices(B, max(a.plain_sample, a.cpu_sample))
     plain = None
     if not a.no_plain:
         ks, arr = batches[0]
         rt.tables_build([params_all[k] for k in ks], spec)
-        # sample: alternate count and Delta-Sigma integrals of point 0
-        cnt = list(range(0, 12, 2))
-        dsg = list(range(12, 192, 11))
-        idx = [v for pair in zip(cnt, dsg) for v in pair][: a.plain_sample]
-        sub = (y3.Integral * len(idx))(*[arr[i] for i in idx])
+        sub_idx = sample[: a.plain_sample]
+        sub = (y3.Integral * len(sub_idx))(*[arr[k * N_PER_POINT + i] for k, i in sub_idx])
+        sref = np.array([ref[k, i] for k, i in sub_idx])
+        seps = np.array([eps[k, i] for k, i in sub_idx])
+        popts = y3.options(precision=y3.FP64, single=1, max_regions=a.max_regions)
         tp = time.perf_counter()
-        popts = y3.options(precision=y3.FP64, single=1)
-        popts.max_regions = opts.max_regions
         rp = rt.integrate(sub, popts)
         dtp = time.perf_counter() - tp
-        okp, _ = count_ok(rp, ref[ks[0]][idx], eps[ks[0]][idx])
-        plain = {"integrals_per_s": len(idx) / dtp, "counted_per_s": int(okp.sum()) / dtp,
-                 "sample": f"{len(idx)} integrals of point 0 ({sum(1 for i in idx if i < 12)} "
-                           f"counts, {sum(1 for i in idx if i >= 12)} Delta-Sigma)",
-                 "seconds": dtp}
+        okp, _ = count_ok(rp, sref, seps)
+        # the batched path on the same sample, for a like-for-like ratio
+        tb = time.perf_counter()
+        rb = rt.integrate(sub, opts)
+        dtb = time.perf_counter() - tb
+        okb, _ = count_ok(rb, sref, seps)
+        plain = {"integrals_per_s": len(sub_idx) / dtp, "counted_per_s": int(okp.sum()) / dtp,
+                 "counted": int(okp.sum()), "seconds": dtp,
+                 "sample": f"{len(sub_idx)} integrals of the first step "
+                           f"({sum(1 for _, i in sub_idx if i < N_COUNTS)} counts), "
+                           "stream eps_rel, same region budget",
+                 "batched_same_sample_counted_per_s": int(okb.sum()) / dtb}
+        print(f"plain path: {len(sub_idx)} integrals in {dtp:.2f} s", file=sys.stderr,
+              flush=True)
 
     cpu = None
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        v, used, dt, n = cpu_baseline(pts, eps, a.cpu_sample, threads)
-        cpu = {"value": v, "unit": "integrals/s", "cores": int(used), "kind": "port",
-               "sample": f"{n} integrals of prior point 0 (counts and Delta-Sigma at their "
-                         f"stream eps_rel), FP64 oracle + CPU Genz-Malik cubature, {dt:.1f} s"}
+        sub_idx = sample[: a.cpu_sample]
+        res, used, dt = cpu_baseline(pts, eps, sub_idx, a.max_regions, threads)
+        okc, _ = count_ok(res, np.array([ref[k, i] for k, i in sub_idx]),
+                          np.array([eps[k, i] for k, i in sub_idx]))
+        cpu = {"value": int(okc.sum()) / dt, "unit": "integrals/s", "cores": int(used),
+               "kind": "port",
+               "sample": f"{len(sub_idx)} integrals of the first step "
+                         f"({sum(1 for _, i in sub_idx if i < N_COUNTS)} counts), FP64 oracle "
+                         f"integrands + its CPU Genz-Malik cubature, same region budget, "
+                         f"counted {int(okc.sum())}, {dt:.1f} s",
+               "integrals_per_s": len(sub_idx) / dt}
+        print(f"cpu baseline: {len(sub_idx)} integrals in {dt:.1f} s on {used} threads",
+              file=sys.stderr, flush=True)
+
+    serving = None
+    if not a.no_serve:
+        rates = [float(r) for r in a.serve_rates.split(",") if r]
+        serving = serving_run(y3, pts, ref, eps_raw, rates, a.serve_seconds, a.max_regions,
+                              a.serve_lanes)
 
     achieved = flops / (kern_ms * 1e-3) if kern_ms > 0 else 0.0
     out = {
@@ -285,7 +348,9 @@ def main():
                    "points_per_step": B, "integrals_per_step": n_int // a.steps,
                    "eps_rel": "log-uniform [1e-5,1e-3] capped at %g" % a.eps_cap,
                    "integrator": "Genz-Malik cubature, workgroup per integral",
+                   "max_regions": a.max_regions,
                    "counted": counted, "integrals": n_int,
+                   "converged": int((status_all == 0).sum()),
                    "max_rel_err": float(rel_all.max()),
                    "device": info["name"]},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
@@ -295,7 +360,9 @@ def main():
                      "evals_per_s": nev_total / (kern_ms * 1e-3) if kern_ms > 0 else 0.0},
         "cpu_baseline": cpu,
         "plain_fp64": plain,
-        "speedup_vs_plain_fp64": (value / plain["counted_per_s"]) if plain and plain["counted_per_s"] > 0 else None,
+        "speedup_vs_plain_fp64": (value / plain["counted_per_s"])
+        if plain and plain["counted_per_s"] > 0 else None,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -193,6 +193,12 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
 int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
                   const y3g_options* opt, y3g_result* out);
 
+/* HBM bytes of the cubature region slab a batched y3g_integrate with these
+ * options allocates on this runtime (resident workgroups x max_regions x
+ * 113 B); the slab is per resident workgroup, so it does not grow with the
+ * number of integrals. */
+int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes);
+
 /* Time of the last y3g_integrate's kernels in milliseconds (HIP events on
  * the library's stream) and the number of integrand evaluations they made. */
 int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
@@ -218,6 +224,54 @@ typedef enum {
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out);
+
+/* ------------------------------------------------------------------ serving
+ * Many parameter points in flight at once (samplers with many walkers or
+ * chains, several likelihood processes sharing one GPU).  The reference
+ * evaluates one sample per execute(), each a serial loop over its
+ * integrals (src/utils/CosmoSISSICUDAModule.hh:252-271, 282-335); here a
+ * request is one parameter point plus its integrals, queued FIFO and taken
+ * whole by the first free lane (an own runtime: HIP stream, tables, region
+ * slab) in batches of up to max_batch_integrals.  The lane count is capped
+ * so that the lanes' region slabs fit hbm_budget_bytes.  Results of
+ * (parameter point, integral) pairs are kept in an LRU cache of
+ * cache_entries entries.  Results are written into the caller's array
+ * before y3g_server_wait returns; the array must stay alive until then. */
+typedef struct y3g_server y3g_server;
+
+typedef struct {
+  int lanes;                  /* concurrent batches                        */
+  int max_batch_integrals;    /* integrals per batch (a larger request
+                                 runs alone)                               */
+  size_t hbm_budget_bytes;    /* bound on the lanes' region slabs          */
+  long long cache_entries;    /* LRU result cache size, 0 = off            */
+  y3g_options options;        /* integrator options of every batch         */
+  y3g_table_spec spec;        /* table grids of every parameter point      */
+} y3g_server_options;
+
+typedef struct {
+  long long requests_submitted, requests_done;
+  long long integrals_computed, cache_hits, batches, evaluations, queued;
+  long long slab_bytes_per_lane;
+  double kernel_ms;           /* summed over batches                       */
+  int lanes;                  /* lanes admitted by the HBM budget          */
+} y3g_server_stats;
+
+const char* y3g_server_last_error(void);
+void y3g_server_options_default(y3g_server_options* o);
+int y3g_server_create(int device, const y3g_server_options* opt, y3g_server** out);
+/* Queue one request; integral `point` fields are ignored (the server
+ * assigns table sets).  Cache hits are written to out at once. */
+int y3g_server_submit(y3g_server* s, const y3g_params* params,
+                      const y3g_integral* igs, size_t n, y3g_result* out,
+                      unsigned long long* ticket);
+/* Wait for a request (timeout_s < 0: no limit); latency_s = completion
+ * minus submission on the server's steady clock.  A ticket is waited once. */
+int y3g_server_wait(y3g_server* s, unsigned long long ticket, double timeout_s,
+                    double* latency_s);
+int y3g_server_stats_get(y3g_server* s, y3g_server_stats* st);
+/* Finish the queued requests, stop the lanes, free everything. */
+int y3g_server_destroy(y3g_server* s);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,19 @@ class Options(C.Structure):
                 ("err_c", C.c_double), ("err_smin", C.c_double)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("lanes", C.c_int), ("max_batch_integrals", C.c_int),
+                ("hbm_budget_bytes", C.c_size_t), ("cache_entries", C.c_longlong),
+                ("options", Options), ("spec", TableSpec)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in (
+        "requests_submitted", "requests_done", "integrals_computed", "cache_hits",
+        "batches", "evaluations", "queued", "slab_bytes_per_lane")] + [
+        ("kernel_ms", C.c_double), ("lanes", C.c_int)]
+
+
 RESULT_DTYPE = np.dtype([("value", "f8"), ("error", "f8"), ("prob", "f8"),
                          ("neval", "i8"), ("nregions", "i4"), ("status", "i4")])
 assert RESULT_DTYPE.itemsize == C.sizeof(Result)
@@ -95,6 +108,15 @@ SIGNATURES = {
                                            _PD, _PD, _PD, C.c_int, C.POINTER(Integral)]),
     "y3g_integrate": (C.c_int, [C.c_void_p, C.POINTER(Integral), C.c_size_t,
                                 C.POINTER(Options), C.c_void_p]),
+    "y3g_runtime_footprint": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(C.c_size_t)]),
+    "y3g_server_last_error": (C.c_char_p, []),
+    "y3g_server_options_default": (None, [C.POINTER(ServerOptions)]),
+    "y3g_server_create": (C.c_int, [C.c_int, C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
+    "y3g_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Integral),
+                                    C.c_size_t, C.c_void_p, C.POINTER(C.c_ulonglong)]),
+    "y3g_server_wait": (C.c_int, [C.c_void_p, C.c_ulonglong, C.c_double, _PD]),
+    "y3g_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "y3g_server_destroy": (C.c_int, [C.c_void_p]),
     "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
     "y3g_eval_functor": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _PD, C.c_int,
                                    C.c_size_t, _PD, C.c_int,
@@ -277,3 +299,65 @@ class Runtime:
                                       C.byref(aux) if aux is not None else None,
                                       out.ctypes.data_as(_PD)), "y3g_eval_functor")
         return out
+
+
+def server_options(**kw):
+    o = ServerOptions()
+    lib().y3g_server_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Server:
+    """Request server (y3g_server_*): submit(params, integrals) -> ticket,
+    wait(ticket) -> (results, latency seconds)."""
+
+    def __init__(self, device=0, opts=None):
+        self._h = C.c_void_p()
+        self._pending = {}
+        o = opts if opts is not None else server_options()
+        rc = lib().y3g_server_create(device, C.byref(o), C.byref(self._h))
+        if rc != 0:
+            raise RuntimeError(f"y3g_server_create failed ({rc}): "
+                               f"{lib().y3g_server_last_error().decode()}")
+
+    def submit(self, params, integrals):
+        n = len(integrals)
+        arr = integrals if isinstance(integrals, C.Array) else (Integral * n)(*integrals)
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        t = C.c_ulonglong()
+        rc = lib().y3g_server_submit(self._h, C.byref(params), arr, n,
+                                     out.ctypes.data_as(C.c_void_p), C.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"y3g_server_submit failed ({rc}): "
+                               f"{lib().y3g_server_last_error().decode()}")
+        self._pending[t.value] = (out, arr)   # keep both alive until wait
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        lat = C.c_double()
+        rc = lib().y3g_server_wait(self._h, ticket, timeout, C.byref(lat))
+        if rc != 0:
+            raise RuntimeError(f"y3g_server_wait failed ({rc}): "
+                               f"{lib().y3g_server_last_error().decode()}")
+        out, _ = self._pending.pop(ticket)
+        return out, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        rc = lib().y3g_server_stats_get(self._h, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("y3g_server_stats_get failed")
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().y3g_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -525,6 +525,20 @@ static void lpt_order(const y3g_integral* igs, const std::vector<int>& ids,
   });
 }
 
+// Workgroups kept resident by the persistent kernels: as many per CU as
+// the LDS (160 KB per CU) and 2048 threads per CU allow.
+static int resident_slots(const y3g_runtime* rt, const y3g_options& o, int algorithm)
+{
+  if (o.slots > 0) return o.slots;
+  int const block = o.block_threads == 256 ? 256 : 512;
+  int per_cu = 2048 / block;
+  if (algorithm == Y3G_PAGANI) {
+    size_t const smem = cubature_smem(o.precision, block);
+    per_cu = std::min(per_cu, std::max(1, (int)((160 * 1024) / (smem + 1024))));
+  }
+  return rt->n_cu * per_cu;
+}
+
 static int run_group(y3g_runtime* rt, const y3g_integral* igs,
                      const std::vector<int>& ids, const y3g_options& o,
                      int algorithm, KernArgs A)
@@ -538,19 +552,7 @@ static int run_group(y3g_runtime* rt, const y3g_integral* igs,
   A.n = (int)order.size();
   A.order = rt->d_order;
   int const block = o.block_threads == 256 ? 256 : 512;
-  int slots = o.slots;
-  if (slots <= 0) {
-    int per_cu = 1;
-    if (algorithm == Y3G_PAGANI) {
-      size_t const smem = cubature_smem(o.precision, block);
-      per_cu = std::max(1, (int)((160 * 1024) / (smem + 1024)));
-      per_cu = std::min(per_cu, 2048 / block);
-    } else {
-      per_cu = 2048 / block;
-    }
-    slots = rt->n_cu * per_cu;
-  }
-  if (o.single) slots = 1;
+  int slots = o.single ? 1 : resident_slots(rt, o, algorithm);
   slots = std::max(1, std::min<int>(slots, (int)ids.size()));
   if (algorithm == Y3G_PAGANI) {
     if (int rc = ensure_slab(rt, std::max(slots, rt->slab_slots), A.cap))
@@ -667,6 +669,18 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   HIPCHK(hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
   rt->last_ms = ms;
   rt->last_nevals = (long long)nev + extra_nev;
+  return 0;
+}
+
+int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes)
+{
+  if (!rt || !bytes) return fail(Y3G_ERR_ARG, "y3g_runtime_footprint: bad arguments");
+  y3g_options o;
+  y3g_options_default(&o);
+  if (opt) o = *opt;
+  size_t const per_region = 2 * kMaxDim * sizeof(double) + 2 * sizeof(double) + 1;
+  size_t const slots = (size_t)resident_slots(rt, o, Y3G_PAGANI);
+  *bytes = slots * (size_t)std::max(o.max_regions, 1) * per_region;
   return 0;
 }
 

@@ -1,0 +1,107 @@
+"""Synthetic workloads of the integration path.
+
+- the prior-box stream: parameter points drawn from Omega_m in [0.2, 0.4],
+  HMF amplitude in [0.7, 0.9], MOR slope in [0.7, 1.1], sigma_ln in
+  [0.1, 0.4], f_mis in [0, 0.4] (Philox key 0x59334331), each carrying 192
+  integrals -- 12 five-dimensional number counts (4 lambda_obs x 3 z_obs
+  bins) and 180 six-dimensional miscentred Delta-Sigma integrals (the 12
+  bins x 15 radii log-spaced in [0.1, 30] Mpc/h) -- with per-integral
+  eps_rel log-uniform in [1e-5, 1e-3] (Philox key 0x59334331 + 1);
+- the serving stream: requests arriving as a Poisson process from a mock
+  random-walk MCMC over the same box, 70% asking for the 12 counts only and
+  30% for counts + Delta-Sigma (192 integrals).  The chain moves between the
+  stream's first points (those with FP64 oracle values), proposing one of
+  the nearest neighbours of its current point and accepting with a
+  Metropolis rule on a smooth mock likelihood, so every served integral can
+  be checked.
+"""
+import numpy as np
+
+LO_EDGES = [20.0, 30.0, 45.0, 60.0, 300.0]
+ZO_EDGES = [0.20, 0.35, 0.50, 0.65]
+RADII = list(np.logspace(np.log10(0.1), np.log10(30.0), 15))
+SEED = 0x59334331
+BOX_LO = np.array([0.2, 0.7, 0.7, 0.1, 0.0])
+BOX_HI = np.array([0.4, 0.9, 1.1, 0.4, 0.4])
+N_COUNTS = 12
+N_PER_POINT = 192
+
+
+def prior_points(n, seed=SEED):
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    u = rng.random((n, 5))
+    return BOX_LO + u * (BOX_HI - BOX_LO)
+
+
+def eps_stream(n_points, seed=SEED):
+    """Per-integral eps_rel, log-uniform in [1e-5, 1e-3], 192 per point."""
+    rng = np.random.Generator(np.random.Philox(key=seed + 1))
+    return 10.0 ** rng.uniform(-5.0, -3.0, (n_points, N_PER_POINT))
+
+
+def point_params(y3, pt):
+    om, amp, slope, sln, fmis = pt
+    return y3.default_params(omega_m=om, hmf_amp=amp, mor_slope=slope, mor_sigma_ln=sln,
+                             f_mis=fmis)
+
+
+def point_integrals(y3, params, spec, k, eps_row, algorithm, counts_only=False):
+    """The 192 (or 12) integrals of one parameter point, in the golden
+    values' order: counts (lambda_obs bin major), then Delta-Sigma (bin
+    major, radius minor)."""
+    igs = []
+    n = 0
+    for i in range(4):
+        for j in range(3):
+            igs.append(y3.make_integral(params, spec, y3.COUNT5D, LO_EDGES[i], LO_EDGES[i + 1],
+                                        ZO_EDGES[j], ZO_EDGES[j + 1], point=k,
+                                        eps_rel=float(eps_row[n]), algorithm=algorithm))
+            n += 1
+    if counts_only:
+        return igs
+    for i in range(4):
+        for j in range(3):
+            for r in RADII:
+                igs.append(y3.make_integral(params, spec, y3.DSIGMA6D, LO_EDGES[i],
+                                            LO_EDGES[i + 1], ZO_EDGES[j], ZO_EDGES[j + 1], r,
+                                            point=k, eps_rel=float(eps_row[n]),
+                                            algorithm=algorithm))
+                n += 1
+    return igs
+
+
+def count_ok(res, ref, eps, cap=1e-4):
+    """Counted: |I - I_oracle| <= eps |I_oracle|, <= cap |I_oracle|, and the
+    reported error bounds the true error."""
+    err = np.abs(res["value"] - ref)
+    rel = err / np.abs(ref)
+    ok = (rel <= eps) & (rel <= cap) & (res["error"] >= err)
+    return ok, rel
+
+
+def mcmc_requests(points, n_requests, rate, seed=SEED + 2, frac_full=0.3, k_nn=6):
+    """Arrival times (s) and (point index, full?) of a mock random-walk MCMC.
+
+    Requests arrive as a Poisson process of the given rate.  The chain's
+    proposals walk the k_nn-nearest-neighbour graph of `points` (box
+    coordinates scaled to [0, 1]); each proposal is one request, 30% of them
+    for the full 192 integrals and the rest for the 12 counts only.
+    """
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    u = (np.asarray(points) - BOX_LO) / (BOX_HI - BOX_LO)
+    d2 = ((u[:, None, :] - u[None, :, :]) ** 2).sum(-1)
+    nn = np.argsort(d2, axis=1)[:, 1:k_nn + 1]
+    centre = np.full(5, 0.5)
+
+    def loglike(i):
+        return -0.5 * float(((u[i] - centre) ** 2).sum()) / 0.15 ** 2
+
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    cur = int(rng.integers(len(points)))
+    reqs = []
+    for _ in range(n_requests):
+        prop = int(nn[cur, rng.integers(k_nn)])
+        reqs.append((prop, bool(rng.random() < frac_full)))
+        if np.log(rng.random()) < loglike(prop) - loglike(cur):
+            cur = prop
+    return t, reqs
