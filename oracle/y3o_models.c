@@ -488,10 +488,15 @@ double
 y3o_dsigma_offset(double R, double rmis, double theta, double lnM, double rho,
                   double conc)
 {
-  double const ct = cos(theta);
-  double const rp2 = R * R + rmis * rmis + 2.0 * R * rmis * ct;
+  /* the same quantities with k = cos^2(theta/2), 1 + cos(theta) = 2k:
+   * R'^2 = (R - rmis)^2 + 4 R rmis k, R + rmis cos(theta) = (R - rmis) +
+   * 2 rmis k -- free of cancellation near R = rmis, theta = pi */
+  double const hc = cos(0.5 * theta);
+  double const k = hc * hc;
+  double const dr = R - rmis;
+  double const rp2 = dr * dr + 4.0 * R * rmis * k;
   if (rp2 <= 1e-300) return 0.0;
-  double const a = R + rmis * ct;
+  double const a = dr + 2.0 * rmis * k;
   double const c2b = 2.0 * a * a / rp2 - 1.0;
   return y3o_nfw_dsigma(sqrt(rp2), lnM, rho, conc) * c2b;
 }

@@ -31,6 +31,7 @@ constexpr double kLnPivotHmf = 32.236191301916641;  // ln 1e14
 constexpr double kLnPivotMor = 33.334803590584751;  // ln 3e14
 constexpr double kRhoCrit = 2.77533742639e+11;      // h^2 Msun / Mpc^3
 constexpr double kDvConst = 2997.92;                // dv_do_dz_t.cuh:25
+constexpr double kLn3e14Over800Pi = 25.505461977067423;  // ln(3e14 / (800 pi))
 constexpr double kDh = 2997.92458;                  // c / H0 [Mpc/h]
 constexpr double kSciConst = 6.0134795e-7;          // 4 pi G/c^2 * 1e6 pc/Mpc
 constexpr double kTMax = 7.0;
@@ -47,19 +48,29 @@ struct PointDev {
 
 // ---------------------------------------------------------------- math
 template <typename T> struct M;
+// FP32: the hardware transcendental instructions (v_exp_f32, v_log_f32,
+// v_sqrt_f32, v_cos_f32; about 1 ulp on the arguments met here) in place of
+// the correctly-rounded library sequences -- the FP32 integrand's inputs
+// already carry errors of that size.  FP64 keeps the library functions.
 template <> struct M<float> {
-  static __device__ __forceinline__ float exp(float x) { return expf(x); }
-  static __device__ __forceinline__ float log(float x) { return logf(x); }
+  static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+  static __device__ __forceinline__ float log(float x) { return __logf(x); }
   static __device__ __forceinline__ float log1p(float x) { return log1pf(x); }
-  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float erf(float x) { return erff(x); }
   static __device__ __forceinline__ float erfc(float x) { return erfcf(x); }
   static __device__ __forceinline__ float erfcx(float x) { return erfcxf(x); }
-  static __device__ __forceinline__ float cos(float x) { return cosf(x); }
+  static __device__ __forceinline__ float cos(float x) { return __cosf(x); }
   static __device__ __forceinline__ float atan(float x) { return atanf(x); }
   static __device__ __forceinline__ float cbrt(float x) { return cbrtf(x); }
   static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
   static __device__ __forceinline__ float pow(float x, float y) { return powf(x, y); }
+  // a / b as a times the hardware reciprocal (1 ulp), a / sqrt(2)
+  static __device__ __forceinline__ float div(float a, float b)
+  {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
+  static __device__ __forceinline__ float div_sqrt2(float a) { return a * 0.70710678118654752f; }
 };
 template <> struct M<double> {
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
@@ -74,6 +85,8 @@ template <> struct M<double> {
   static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
   static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
   static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double div_sqrt2(double a) { return a / 1.4142135623730951; }
 };
 
 #define Y3G_HD __device__ __forceinline__
@@ -89,8 +102,8 @@ template <typename T> struct Tabs {
 // primitives.hh:60-65
 template <typename T> Y3G_HD T gaussian(T x, T mu, T sigma)
 {
-  T const z = (x - mu) / sigma;
-  return M<T>::exp(T(-0.5) * z * z) * T(0.3989422804014327) / sigma;
+  T const z = M<T>::div(x - mu, sigma);
+  return M<T>::div(M<T>::exp(T(-0.5) * z * z) * T(0.3989422804014327), sigma);
 }
 
 // roffset_t.cuh:15-21 (Gamma k = 2 kernel of the reference).
@@ -137,7 +150,7 @@ template <typename T> Y3G_HD T half_erf_diff(T a, T b)
 template <typename T> Y3G_HD T int_zo_zt_des(T zomin, T zomax, T zt)
 {
   T const base = T(1.4142135623730951) * sigma_photoz_des<T>(zt);
-  return half_erf_diff<T>((zomin - zt) / base, (zomax - zt) / base);
+  return half_erf_diff<T>(M<T>::div(zomin - zt, base), M<T>::div(zomax - zt, base));
 }
 
 // z_obs ~ N(zt, sigz0 (1+zt)): pdf and bin integral.
@@ -148,16 +161,16 @@ template <typename T> Y3G_HD T zo_pdf(T zo, T zt, T sigz0)
 template <typename T> Y3G_HD T zo_bin(T zlo, T zhi, T zt, T sigz0)
 {
   T const base = T(1.4142135623730951) * sigz0 * (T(1) + zt);
-  return half_erf_diff<T>((zlo - zt) / base, (zhi - zt) / base);
+  return half_erf_diff<T>(M<T>::div(zlo - zt, base), M<T>::div(zhi - zt, base));
 }
 
 // Exponentially-modified Gaussian pdf (lc_lt_t.cuh:56-61 second term with
 // fmsk = 0; emg_des_t.cuh:172-193), overflow-free through erfcx.
 template <typename T> Y3G_HD T emg_pdf(T x, T mu, T sigma, T tau)
 {
-  T const B = (mu + tau * sigma * sigma - x) / (T(1.4142135623730951) * sigma);
+  T const B = M<T>::div(mu + tau * sigma * sigma - x, T(1.4142135623730951) * sigma);
   if (B >= T(0)) {
-    T const d = (x - mu) / sigma;
+    T const d = M<T>::div(x - mu, sigma);
     return T(0.5) * tau * M<T>::exp(T(-0.5) * d * d) * M<T>::erfcx(B);
   }
   T const A = tau * (mu - x) + T(0.5) * tau * tau * sigma * sigma;
@@ -167,9 +180,9 @@ template <typename T> Y3G_HD T emg_pdf(T x, T mu, T sigma, T tau)
 // EMG CDF (emg_des_t.cuh:136-168).
 template <typename T> Y3G_HD T emg_cdf(T x, T mu, T sigma, T tau)
 {
-  T const z = (x - mu) / sigma;
-  T const phi = T(0.5) * M<T>::erfc(-z / T(1.4142135623730951));
-  T const u = (tau * sigma - z) / T(1.4142135623730951);
+  T const z = M<T>::div(x - mu, sigma);
+  T const phi = T(0.5) * M<T>::erfc(M<T>::div_sqrt2(-z));
+  T const u = M<T>::div_sqrt2(tau * sigma - z);
   T tail;
   if (u >= T(0))
     tail = T(0.5) * M<T>::exp(T(-0.5) * z * z) * M<T>::erfcx(u);
@@ -183,9 +196,9 @@ template <typename T> Y3G_HD T emg_cdf(T x, T mu, T sigma, T tau)
 // 1 - F_EMG(x) = Phi(-z) + exp(A) Phi(z - tau sigma).
 template <typename T> Y3G_HD T emg_sf(T x, T mu, T sigma, T tau)
 {
-  T const z = (x - mu) / sigma;
-  T const q = T(0.5) * M<T>::erfc(z / T(1.4142135623730951));
-  T const u = (tau * sigma - z) / T(1.4142135623730951);
+  T const z = M<T>::div(x - mu, sigma);
+  T const q = T(0.5) * M<T>::erfc(M<T>::div_sqrt2(z));
+  T const u = M<T>::div_sqrt2(tau * sigma - z);
   T tail;
   if (u >= T(0))
     tail = T(0.5) * M<T>::exp(T(-0.5) * z * z) * M<T>::erfcx(u);
@@ -210,7 +223,7 @@ template <typename T> Y3G_HD T lo_lt_pdf(T lo, T lt, T fprj, T pscale)
   T const sigma = M<T>::sqrt(lt);
   T const g = gaussian<T>(lo, lt, sigma);
   if (fprj == T(0)) return g;
-  T const tau = T(1) / (pscale * lt);
+  T const tau = M<T>::div(T(1), pscale * lt);
   return (T(1) - fprj) * g + fprj * emg_pdf<T>(lo, lt, sigma, tau);
 }
 
@@ -219,9 +232,9 @@ template <typename T> Y3G_HD T lo_lt_bin(T lo_lo, T lo_hi, T lt, T fprj, T pscal
 {
   T const sigma = M<T>::sqrt(lt);
   T const base = T(1.4142135623730951) * sigma;
-  T const g = half_erf_diff<T>((lo_lo - lt) / base, (lo_hi - lt) / base);
+  T const g = half_erf_diff<T>(M<T>::div(lo_lo - lt, base), M<T>::div(lo_hi - lt, base));
   if (fprj == T(0)) return g;
-  T const tau = T(1) / (pscale * lt);
+  T const tau = M<T>::div(T(1), pscale * lt);
   T const e = emg_cdf_diff<T>(lo_lo, lo_hi, lt, sigma, tau);
   return (T(1) - fprj) * g + fprj * e;
 }
@@ -267,7 +280,7 @@ Y3G_HD T dv_do_dz(const PointDev& P, const T* da, double zt)
   T const om = T(P.p.omega_m);
   T const ez = M<T>::sqrt(om * zp * zp * zp + (T(1) - om));
   T const h = T(P.p.h);
-  return T(kDvConst) * zp * zp * d * h * d * h / ez;
+  return M<T>::div(T(kDvConst) * zp * zp * d * h * d * h, ez);
 }
 
 // NFW: F and h(x) = (1 - F) / (x^2 - 1), series in t = x^2 - 1 near x = 1.
@@ -290,12 +303,12 @@ template <typename T> Y3G_HD void nfw_F_h(T x, T& F, T& h)
   }
   if (t < T(0)) {
     T const s = M<T>::sqrt(-t);
-    F = M<T>::log((T(1) + s) / x) / s;
+    F = M<T>::div(M<T>::log(M<T>::div(T(1) + s, x)), s);
   } else {
     T const s = M<T>::sqrt(t);
-    F = M<T>::atan(s) / s;
+    F = M<T>::div(M<T>::atan(s), s);
   }
-  h = (T(1) - F) / t;
+  h = M<T>::div(T(1) - F, t);
 }
 
 // Delta-Sigma / (rho_s r_s) of NFW (Wright & Brainerd 2000).
@@ -305,11 +318,13 @@ template <typename T> Y3G_HD T nfw_g(T x)
   nfw_F_h<T>(x, F, h);
   if (x < T(0.9)) {
     T const s = M<T>::sqrt(T(1) - x * x);
-    T const q = x * x / (T(2) * (T(1) + s));
-    T const phi = -M<T>::log1p(-q) / q;
-    m = (M<T>::log(T(2) / x) / (T(1) + s) - T(0.5) * phi / (T(1) + s)) / s;
+    T const q = M<T>::div(x * x, T(2) * (T(1) + s));
+    T const phi = M<T>::div(-M<T>::log1p(-q), q);
+    m = M<T>::div(M<T>::div(M<T>::log(M<T>::div(T(2), x)), T(1) + s) -
+                      M<T>::div(T(0.5) * phi, T(1) + s),
+                  s);
   } else {
-    m = (M<T>::log(T(0.5) * x) + F) / (x * x);
+    m = M<T>::div(M<T>::log(T(0.5) * x) + F, x * x);
   }
   return T(4) * m - T(2) * h;
 }
@@ -325,12 +340,11 @@ Y3G_HD NfwHalo<T> nfw_halo(double lnM, T rho, T conc)
 {
   T const dl = T(lnM - kLnPivotHmf);
   T const c = conc > T(0) ? conc : T(5) * M<T>::exp(T(-0.1) * dl);
-  // r200 = cbrt(3 M / (800 pi rho)), M = 1e14 e^dl
-  T const r200 = M<T>::cbrt(T(3e14 / (800.0 * 3.14159265358979323846)) *
-                            M<T>::exp(dl) / rho);
-  T const rs = r200 / c;
-  T const dc = T(200.0 / 3.0) * c * c * c /
-               (M<T>::log(T(1) + c) - c / (T(1) + c));
+  // r200 = cbrt(3 M / (800 pi rho)), M = 1e14 e^dl, through logarithms
+  T const r200 = M<T>::exp((dl + T(kLn3e14Over800Pi) - M<T>::log(rho)) * T(1.0 / 3.0));
+  T const rs = M<T>::div(r200, c);
+  T const dc = M<T>::div(T(200.0 / 3.0) * c * c * c,
+                         M<T>::log(T(1) + c) - M<T>::div(c, T(1) + c));
   NfwHalo<T> out;
   out.rs = rs;
   out.amp = T(1e-12) * dc * rho * rs;
@@ -339,7 +353,7 @@ Y3G_HD NfwHalo<T> nfw_halo(double lnM, T rho, T conc)
 
 template <typename T> Y3G_HD T nfw_dsigma(T R, const NfwHalo<T>& H)
 {
-  return H.amp * nfw_g<T>(R / H.rs);
+  return H.amp * nfw_g<T>(M<T>::div(R, H.rs));
 }
 
 // Tangential Delta-Sigma about a centre offset by rmis at angle theta:
@@ -347,11 +361,16 @@ template <typename T> Y3G_HD T nfw_dsigma(T R, const NfwHalo<T>& H)
 template <typename T>
 Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 {
-  T const ct = M<T>::cos(theta);
-  T const rp2 = R * R + rmis * rmis + T(2) * R * rmis * ct;
+  // with k = cos^2(theta/2): R'^2 = (R - rmis)^2 + 4 R rmis k and
+  // R + rmis cos(theta) = (R - rmis) + 2 rmis k, free of the cancellation
+  // of the textbook form near R = rmis, theta = pi
+  T const hc = M<T>::cos(T(0.5) * theta);
+  T const k = hc * hc;
+  T const dr = R - rmis;
+  T const rp2 = dr * dr + T(4) * R * rmis * k;
   if (!(rp2 > T(1e-20))) return T(0);
-  T const a = R + rmis * ct;
-  T const c2b = T(2) * a * a / rp2 - T(1);
+  T const a = dr + T(2) * rmis * k;
+  T const c2b = M<T>::div(T(2) * a * a, rp2) - T(1);
   return nfw_dsigma<T>(M<T>::sqrt(rp2), H) * c2b;
 }
 
@@ -378,7 +397,7 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
   T const zp = T(1) + T(zt);
   T const lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
                    T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
-  T const isl = T(1) / sln;
+  T const isl = M<T>::div(T(1), sln);
   T tlo = (T(ig.lnlt_lo) - lnlbar) * isl;
   T thi = (T(ig.lnlt_hi) - lnlbar) * isl;
   tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
@@ -400,8 +419,10 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
     default: {
       T const lo = T(y[0]);
       T const s = T(ig.sigma_mis);
-      T const one_minus_u = T(1.0 - y[4]);
-      T const rmis = s * M<T>::sqrt(T(-2) * M<T>::log(one_minus_u));
+      // -2 ln(1 - u): log1p for small u, ln of the FP64 difference near 1
+      double const u = y[4];
+      T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
+      T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
       T const th = T(y[5]);
       T const rho = T(P.rho_m);
       NfwHalo<T> const H = nfw_halo<T>(lnM, rho, T(0));
