@@ -219,7 +219,9 @@ typedef enum {
   Y3G_F_NFW_DSIGMA = 10,   /* (R, lnM, rho, conc)  centred NFW Delta-Sigma  */
   Y3G_F_DSIGMA_OFFSET = 11,/* (R, rmis, theta, lnM, rho, conc) tangential    */
   Y3G_F_SCI = 12,          /* (zt) of point 0   Sigma_crit^-1              */
-  Y3G_F_INTEGRAND = 13     /* (y[ndim]) mapped integrand of integral `aux` */
+  Y3G_F_INTEGRAND = 13,    /* (y[ndim]) mapped integrand of integral `aux` */
+  Y3G_F_INTEGRAND_PAIRED = 14 /* the same through pair_factors + integrand_pair,
+                                 the cubature's factorised form       */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

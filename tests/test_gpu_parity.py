@@ -164,12 +164,14 @@ def test_mapped_integrand_matches_oracle(runtime, kind, prec, tol):
     oig = O.make_integral(op, O.spec_wide() if wide else O.spec_config0(), kind, 20., 30., 0.2, 0.35, 1.0)
     rng = np.random.default_rng(kind)
     Y = np.array([[rng.uniform(ig.lows[k], ig.highs[k]) for k in range(ig.ndim)] for _ in range(500)])
-    v = runtime.eval_functor(y3.F_INTEGRAND, Y, precision=prec, aux=ig)
     r = np.array([O.lib.y3o_integrand_mapped(C.byref(op), C.byref(ot.t), C.byref(oig),
                                              O.ptr(np.ascontiguousarray(y))) for y in Y])
     m = np.abs(r) > 1e-6 * np.max(np.abs(r))
-    assert np.max(np.abs(v[m] / r[m] - 1)) < tol
-    assert np.max(np.abs(v - r)) < 1e-4 * np.max(np.abs(r))
+    # the direct form and the cubature's (zt, lnM)-factorised form
+    for which in (y3.F_INTEGRAND, y3.F_INTEGRAND_PAIRED):
+        v = runtime.eval_functor(which, Y, precision=prec, aux=ig)
+        assert np.max(np.abs(v[m] / r[m] - 1)) < tol, which
+        assert np.max(np.abs(v - r)) < 1e-4 * np.max(np.abs(r)), which
 
 
 def _check(res, gold, eps):

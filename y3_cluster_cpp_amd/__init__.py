@@ -19,7 +19,7 @@ FP32SPLIT, FP64 = 0, 1
 
 (F_ROFFSET_GAMMA, F_LO_LC, F_ZO_ZT_DES, F_INT_ZO_ZT_DES, F_SIGMA_PHOTOZ,
  F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
- F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND) = range(14)
+ F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND, F_INTEGRAND_PAIRED) = range(15)
 
 
 class Params(C.Structure):
@@ -287,8 +287,6 @@ class Runtime:
 
     def eval_functor(self, which, x, par=(), precision=FP64, aux=None):
         x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))
-        if x.shape[0] == 1 and x.shape[1] > 1 and which not in (F_INTEGRAND,):
-            pass
         n, nin = x.shape
         par = np.ascontiguousarray(np.asarray(par, dtype=np.float64).reshape(-1))
         if par.size == 0:

@@ -176,6 +176,9 @@ __host__ __device__ inline void select_bins(const int* hc,
 }
 constexpr int kMaxNodes = gm_num_nodes(kMaxDim);  // 149
 constexpr int kKmax = 1024;
+// distinct (zt, lnM) offset pairs among a region's Genz-Malik nodes: the
+// centre, +-l2, +-l3, +-l4 on either axis alone, (+-l4, +-l4), (+-l5, +-l5)
+constexpr int kMaxPairs = 21;
 
 template <typename T, int BLOCK> struct CubShared {
   T hmf[kMaxHmf];
@@ -197,7 +200,37 @@ template <typename T, int BLOCK> struct CubShared {
   y3g_integral ig;
   PointDev P;
   int item, point, ncount;
+  // FP32 path: (zt, lnM) pair factors of the chunk's regions, the pairs'
+  // unit offsets and each node's pair
+  PairF<T> pf[sizeof(T) == 4 ? kChunk : 1][kMaxPairs];
+  double poff[kMaxPairs][2];
+  unsigned char npair[kMaxNodes];
+  int npairs;
 };
+
+// The distinct (zt, lnM) unit offsets of the current integral's nodes and
+// each node's index among them (one thread).
+template <typename T, int BLOCK>
+__device__ void pair_setup(CubShared<T, BLOCK>& S)
+{
+  int it, iz, im;
+  kind_axes<T>(S.ig.kind, it, iz, im);
+  int const d = S.ig.ndim;
+  int const nn = gm_num_nodes(d);
+  int np = 0;
+  for (int j = 0; j < nn; ++j) {
+    double const oz = gm_offset(d, j, iz), om = gm_offset(d, j, im);
+    int k = 0;
+    while (k < np && !(S.poff[k][0] == oz && S.poff[k][1] == om)) ++k;
+    if (k == np && np < kMaxPairs) {
+      S.poff[np][0] = oz;
+      S.poff[np][1] = om;
+      ++np;
+    }
+    S.npair[j] = (unsigned char)(k < kMaxPairs ? k : kMaxPairs - 1);
+  }
+  S.npairs = np;
+}
 
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
@@ -207,13 +240,35 @@ __device__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double e
   int const d = S.ig.ndim;
   int const nn = gm_num_nodes(d);
   Tabs<T> tb{S.hmf, S.da, S.sci};
-  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
-    int const r = q / nn, j = q - r * nn;
-    double y[kMaxDim];
+  if constexpr (sizeof(T) == 4) {
+    // factors of the (zt, lnM) pairs once per region, then the nodes
+    int it, iz, im;
+    kind_axes<T>(S.ig.kind, it, iz, im);
+    int const np = S.npairs;
+    for (int q = threadIdx.x; q < m * np; q += BLOCK) {
+      int const r = q / np, k = q - r * np;
+      double const zt = S.c[r][iz] + S.h[r][iz] * S.poff[k][0];
+      double const lnM = S.c[r][im] + S.h[r][im] * S.poff[k][1];
+      S.pf[r][k] = pair_factors<T>(S.ig, S.P, tb, zt, lnM);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+      int const r = q / nn, j = q - r * nn;
+      double y[kMaxDim];
 #pragma unroll
-    for (int a = 0; a < kMaxDim; ++a)
-      y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
-    S.f[q] = integrand<T>(S.ig, S.P, tb, y);
+      for (int a = 0; a < kMaxDim; ++a)
+        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
+      S.f[q] = integrand_pair<T>(S.ig, S.P, S.pf[r][S.npair[j]], y);
+    }
+  } else {
+    for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+      int const r = q / nn, j = q - r * nn;
+      double y[kMaxDim];
+#pragma unroll
+      for (int a = 0; a < kMaxDim; ++a)
+        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
+      S.f[q] = integrand<T>(S.ig, S.P, tb, y);
+    }
   }
   __syncthreads();
   int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -439,6 +439,94 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
   }
 }
 
+// The same integrand split in two: the factors that depend on (zt, lnM)
+// only, and the rest.  A Genz-Malik region's nodes take only 21 distinct
+// (zt, lnM) pairs, so the cubature evaluates pair_factors once per pair
+// and integrand_pair once per node.
+template <typename T> struct PairF {
+  T lnlbar, tlo, thi;  // MOR centre and the clipped t window
+  T zfac;              // hmf x dV/dz [x zo bin x Sigma_crit^-1 x angular norm]
+  T rs, amp;           // NFW halo (Delta-Sigma)
+  T cds;               // (1 - f_mis) x centred Delta-Sigma(R)
+  T pad;
+};
+
+template <typename T>
+Y3G_HD void kind_axes(int kind, int& it, int& iz, int& im)
+{
+  switch (kind) {
+    case Y3G_COUNT3D: it = 0, iz = 1, im = 2; break;
+    case Y3G_COUNT5D: it = 2, iz = 3, im = 4; break;
+    default: it = 1, iz = 2, im = 3; break;
+  }
+}
+
+template <typename T>
+Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                             double zt, double lnM)
+{
+  PairF<T> f;
+  T const sln = T(P.p.mor_sigma_ln);
+  T const zp = T(1) + T(zt);
+  f.lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
+             T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
+  T const isl = M<T>::div(T(1), sln);
+  T tlo = (T(ig.lnlt_lo) - f.lnlbar) * isl;
+  T thi = (T(ig.lnlt_hi) - f.lnlbar) * isl;
+  f.tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
+  f.thi = thi < T(kTMax) ? thi : T(kTMax);
+  T zf = hmf<T>(P, tb.hmf, lnM, zt) * dv_do_dz<T>(P, tb.da, zt);
+  f.rs = f.amp = f.cds = f.pad = T(0);
+  if (ig.kind == Y3G_DSIGMA6D) {
+    T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
+    // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
+    zf *= zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
+          T(ig.rmis_norm * 0.31830988618379067);
+    NfwHalo<T> const H = nfw_halo<T>(lnM, T(P.rho_m), T(0));
+    f.rs = H.rs;
+    f.amp = H.amp;
+    f.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
+  }
+  f.zfac = zf;
+  return f;
+}
+
+template <typename T>
+Y3G_HD T integrand_pair(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
+                        const double* y)
+{
+  if (!(f.thi > f.tlo)) return T(0);
+  int it, iz, im;
+  kind_axes<T>(ig.kind, it, iz, im);
+  T const sln = T(P.p.mor_sigma_ln);
+  T const tt = f.tlo + (f.thi - f.tlo) * T(y[it]);
+  T const lt = M<T>::exp(f.lnlbar + sln * tt);
+  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
+  T const common = f.zfac * phi;
+  T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
+  switch (ig.kind) {
+    case Y3G_COUNT3D:
+      return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+    case Y3G_COUNT5D: {
+      T const lo = T(y[0]), zo = T(y[1]);
+      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+             zo_pdf<T>(zo, T(y[iz]), T(P.p.sigz0));
+    }
+    default: {
+      T const lo = T(y[0]);
+      T const s = T(ig.sigma_mis);
+      double const u = y[4];
+      T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
+      T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
+      NfwHalo<T> H;
+      H.rs = f.rs;
+      H.amp = f.amp;
+      T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, T(y[5]), H);
+      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) * prof;
+    }
+  }
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

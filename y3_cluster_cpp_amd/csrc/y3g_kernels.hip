@@ -118,6 +118,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
     if (tid == 0) {
       S.ig = A.igs[iid];
       gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
+      if constexpr (sizeof(T) == 4) pair_setup(S);
     }
     __syncthreads();
     int const pt = S.ig.point;
@@ -574,6 +575,12 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     }
     case Y3G_F_SCI: r = lerp_clamp<T>(tb.sci, P.nsci, x[0], P.inv_dzsci); break;
     case Y3G_F_INTEGRAND: r = integrand<T>(*F.ig, P, tb, x); break;
+    case Y3G_F_INTEGRAND_PAIRED: {
+      int it, iz, im;
+      kind_axes<T>(F.ig->kind, it, iz, im);
+      r = integrand_pair<T>(*F.ig, P, pair_factors<T>(*F.ig, P, tb, x[iz], x[im]), x);
+      break;
+    }
   }
   F.out[i] = double(r);
 }

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
     S.ig = *A.ig;
     S.P = A.K.points[S.ig.point];
     gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
+    if constexpr (sizeof(T) == 4) pair_setup(S);
   }
   __syncthreads();
   stage_tables<T>(A.K, S.ig.point, S.hmf, S.da, S.sci);
