@@ -193,6 +193,8 @@ template <typename T, int BLOCK> struct CubShared {
   int sel[kKmax];
   double red[2 * (BLOCK / 64)];
   int ired[BLOCK / 64];
+  int ired1[BLOCK / 64];
+  int ncount1;
   int hc[64];
   unsigned long long hq[64];
   int selB, lim0, lim1;
@@ -205,6 +207,7 @@ template <typename T, int BLOCK> struct CubShared {
   PairF<T> pf[sizeof(T) == 4 ? kChunk : 1][kMaxPairs];
   double poff[kMaxPairs][2];
   unsigned char npair[kMaxNodes];
+  unsigned ncode[sizeof(T) == 4 ? kMaxNodes : 1];  // gm_code of each node
   int npairs;
 };
 
@@ -228,6 +231,7 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
       ++np;
     }
     S.npair[j] = (unsigned char)(k < kMaxPairs ? k : kMaxPairs - 1);
+    if constexpr (sizeof(T) == 4) S.ncode[j] = gm_code(d, j);
   }
   S.npairs = np;
 }
@@ -254,10 +258,11 @@ __device__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double e
     __syncthreads();
     for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
       int const r = q / nn, j = q - r * nn;
+      unsigned const code = S.ncode[j];
       double y[kMaxDim];
 #pragma unroll
       for (int a = 0; a < kMaxDim; ++a)
-        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
+        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_decode(code, a) : 0.0;
       S.f[q] = integrand_pair<T>(S.ig, S.P, S.pf[r][S.npair[j]], y);
     }
   } else {

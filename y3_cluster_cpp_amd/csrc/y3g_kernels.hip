@@ -168,13 +168,17 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
     }
     long long neval = (long long)N * nn;
     int status = 1;
+    // totals: summed once here and at the end, updated by each step's
+    // (children - parent) differences in between
     double I = 0.0, E = 0.0;
-    for (;;) {
+    {
       double a = 0.0, b = 0.0;
       for (int r = tid; r < N; r += BLOCK) a += est[r], b += err[r];
       block_sum2<BLOCK>(a, b, S.red);
       I = a;
       E = b;
+    }
+    for (;;) {
       if (E <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
         status = 0;
         break;
@@ -200,40 +204,41 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       int const B = S.selB, lim0 = S.lim0, lim1 = S.lim1;
       int const ns = lim0 + lim1;
       if (ns <= 0 || neval + 2LL * ns * nn > S.ig.max_eval) break;
-      // ordered compaction: bins < B (up to lim0), then bin B (up to lim1)
-      if (tid == 0) S.ncount = 0;
+      // ordered compaction in one pass: bins < B in index order to
+      // sel[0, lim0), bin B in index order to sel[lim0, lim0 + lim1)
+      if (tid == 0) S.ncount = 0, S.ncount1 = 0;
       __syncthreads();
-      for (int pass = 0; pass < 2; ++pass) {
-        int const limit = pass == 0 ? lim0 : lim0 + lim1;
-        for (int base = 0; base < N; base += BLOCK) {
-          int const r = base + tid;
-          bool f = false;
-          if (r < N) {
-            int const b = err_bin(err[r], E, eE);
-            f = pass == 0 ? (b < B) : (b == B);
-          }
-          unsigned long long const bal = __ballot(f);
-          int const lane = tid & 63, w = tid >> 6;
-          int const before = __popcll(bal & ((1ull << lane) - 1ull));
-          if (lane == 0) S.ired[w] = __popcll(bal);
-          __syncthreads();
-          int off = S.ncount;
-          for (int i = 0; i < w; ++i) off += S.ired[i];
-          if (f && off + before < limit) S.sel[off + before] = r;
-          __syncthreads();
-          if (tid == 0) {
-            int t = 0;
-            for (int i = 0; i < BLOCK / 64; ++i) t += S.ired[i];
-            S.ncount = min(S.ncount + t, limit);
-          }
-          __syncthreads();
-          if (S.ncount >= limit) break;
+      for (int base = 0; base < N; base += BLOCK) {
+        int const r = base + tid;
+        bool f0 = false, f1 = false;
+        if (r < N) {
+          int const b = err_bin(err[r], E, eE);
+          f0 = b < B;
+          f1 = b == B;
+        }
+        unsigned long long const bal0 = __ballot(f0), bal1 = __ballot(f1);
+        int const lane = tid & 63, w = tid >> 6;
+        unsigned long long const below = (1ull << lane) - 1ull;
+        if (lane == 0) S.ired[w] = __popcll(bal0), S.ired1[w] = __popcll(bal1);
+        __syncthreads();
+        int off0 = S.ncount, off1 = S.ncount1;
+        for (int i = 0; i < w; ++i) off0 += S.ired[i], off1 += S.ired1[i];
+        off0 += __popcll(bal0 & below);
+        off1 += __popcll(bal1 & below);
+        if (f0 && off0 < lim0) S.sel[off0] = r;
+        if (f1 && off1 < lim1) S.sel[lim0 + off1] = r;
+        __syncthreads();
+        if (tid == 0) {
+          int t0 = 0, t1 = 0;
+          for (int i = 0; i < BLOCK / 64; ++i) t0 += S.ired[i], t1 += S.ired1[i];
+          S.ncount = min(S.ncount + t0, lim0);
+          S.ncount1 = min(S.ncount1 + t1, lim1);
         }
         __syncthreads();
-        if (tid == 0) S.ncount = min(S.ncount, limit);
-        __syncthreads();
+        if (S.ncount >= lim0 && S.ncount1 >= lim1) break;
       }
       // split and evaluate, kChunk/2 pairs at a time
+      double dI = 0.0, dE = 0.0;
       for (int pb = 0; pb < ns; pb += kChunk / 2) {
         int const np = min(kChunk / 2, ns - pb);
         for (int q = tid; q < np * d; q += BLOCK) {
@@ -280,6 +285,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
               ea = eb = 0.5 * delta;
             }
           }
+          dI += S.est[2 * tid] + S.est[2 * tid + 1] - S.pest[tid];
+          dE += ea + eb - S.perr[tid];
           est[p] = S.est[2 * tid];
           err[p] = ea;
           axs[p] = (int8_t)S.ax[2 * tid];
@@ -289,8 +296,18 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
         }
         __syncthreads();
       }
+      block_sum2<BLOCK>(dI, dE, S.red);
+      I += dI;
+      E += dE;
       N += ns;
       neval += 2LL * ns * nn;
+    }
+    {
+      double a = 0.0, b = 0.0;
+      for (int r = tid; r < N; r += BLOCK) a += est[r], b += err[r];
+      block_sum2<BLOCK>(a, b, S.red);
+      I = a;
+      E = b;
     }
     if (tid == 0) {
       y3g_result res;
