@@ -236,34 +236,44 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
   S.npairs = np;
 }
 
+// FP32 node sweep of m regions for integrand kind K: factors of the
+// (zt, lnM) pairs once per region, then every node from its pair's factors.
+template <typename T, int BLOCK, int K>
+__device__ void eval_nodes(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
+{
+  constexpr int d = KindAxes<K>::nd, iz = KindAxes<K>::iz, im = KindAxes<K>::im;
+  constexpr int nn = gm_num_nodes(d);
+  int const np = S.npairs;
+  for (int q = threadIdx.x; q < m * np; q += BLOCK) {
+    int const r = q / np, k = q - r * np;
+    double const zt = S.c[r][iz] + S.h[r][iz] * S.poff[k][0];
+    double const lnM = S.c[r][im] + S.h[r][im] * S.poff[k][1];
+    S.pf[r][k] = pair_factors<T>(S.ig, S.P, tb, zt, lnM);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+    int const r = q / nn, j = q - r * nn;
+    unsigned const code = S.ncode[j];
+    double y[d];
+#pragma unroll
+    for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_decode(code, a);
+    S.f[q] = integrand_pair_k<T, K>(S.ig, S.P, S.pf[r][S.npair[j]], y);
+  }
+}
+
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
 template <typename T, int BLOCK>
-__device__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double err_smin)
+__device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double err_smin)
 {
   int const d = S.ig.ndim;
   int const nn = gm_num_nodes(d);
   Tabs<T> tb{S.hmf, S.da, S.sci};
   if constexpr (sizeof(T) == 4) {
-    // factors of the (zt, lnM) pairs once per region, then the nodes
-    int it, iz, im;
-    kind_axes<T>(S.ig.kind, it, iz, im);
-    int const np = S.npairs;
-    for (int q = threadIdx.x; q < m * np; q += BLOCK) {
-      int const r = q / np, k = q - r * np;
-      double const zt = S.c[r][iz] + S.h[r][iz] * S.poff[k][0];
-      double const lnM = S.c[r][im] + S.h[r][im] * S.poff[k][1];
-      S.pf[r][k] = pair_factors<T>(S.ig, S.P, tb, zt, lnM);
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
-      int const r = q / nn, j = q - r * nn;
-      unsigned const code = S.ncode[j];
-      double y[kMaxDim];
-#pragma unroll
-      for (int a = 0; a < kMaxDim; ++a)
-        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_decode(code, a) : 0.0;
-      S.f[q] = integrand_pair<T>(S.ig, S.P, S.pf[r][S.npair[j]], y);
+    switch (S.ig.kind) {
+      case Y3G_COUNT3D: eval_nodes<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
+      case Y3G_COUNT5D: eval_nodes<T, BLOCK, Y3G_COUNT5D>(S, m, tb); break;
+      default: eval_nodes<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   } else {
     for (int q = threadIdx.x; q < m * nn; q += BLOCK) {

@@ -491,39 +491,59 @@ Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   return f;
 }
 
-template <typename T>
-Y3G_HD T integrand_pair(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
-                        const double* y)
+// Axis positions of the integration variables per kind, at compile time
+// (node coordinates then stay in registers).
+template <int K> struct KindAxes;
+template <> struct KindAxes<Y3G_COUNT3D> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_COUNT5D> {
+  static constexpr int it = 2, iz = 3, im = 4, nd = 5;
+};
+template <> struct KindAxes<Y3G_DSIGMA6D> {
+  static constexpr int it = 1, iz = 2, im = 3, nd = 6;
+};
+
+template <typename T, int K>
+Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
+                          const double* y)
 {
   if (!(f.thi > f.tlo)) return T(0);
-  int it, iz, im;
-  kind_axes<T>(ig.kind, it, iz, im);
+  constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
   T const sln = T(P.p.mor_sigma_ln);
   T const tt = f.tlo + (f.thi - f.tlo) * T(y[it]);
   T const lt = M<T>::exp(f.lnlbar + sln * tt);
   T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
   T const common = f.zfac * phi;
   T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
+  if constexpr (K == Y3G_COUNT3D) {
+    return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+  } else if constexpr (K == Y3G_COUNT5D) {
+    T const lo = T(y[0]), zo = T(y[1]);
+    return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+           zo_pdf<T>(zo, T(y[iz]), T(P.p.sigz0));
+  } else {
+    T const lo = T(y[0]);
+    T const s = T(ig.sigma_mis);
+    double const u = y[4];
+    T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
+    T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
+    NfwHalo<T> H;
+    H.rs = f.rs;
+    H.amp = f.amp;
+    T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, T(y[5]), H);
+    return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) * prof;
+  }
+}
+
+template <typename T>
+Y3G_HD T integrand_pair(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
+                        const double* y)
+{
   switch (ig.kind) {
-    case Y3G_COUNT3D:
-      return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
-    case Y3G_COUNT5D: {
-      T const lo = T(y[0]), zo = T(y[1]);
-      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
-             zo_pdf<T>(zo, T(y[iz]), T(P.p.sigz0));
-    }
-    default: {
-      T const lo = T(y[0]);
-      T const s = T(ig.sigma_mis);
-      double const u = y[4];
-      T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
-      T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
-      NfwHalo<T> H;
-      H.rs = f.rs;
-      H.amp = f.amp;
-      T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, T(y[5]), H);
-      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) * prof;
-    }
+    case Y3G_COUNT3D: return integrand_pair_k<T, Y3G_COUNT3D>(ig, P, f, y);
+    case Y3G_COUNT5D: return integrand_pair_k<T, Y3G_COUNT5D>(ig, P, f, y);
+    default: return integrand_pair_k<T, Y3G_DSIGMA6D>(ig, P, f, y);
   }
 }
 
