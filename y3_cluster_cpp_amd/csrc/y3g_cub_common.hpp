@@ -261,6 +261,22 @@ __device__ void eval_nodes(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   }
 }
 
+// Node sweep of m regions evaluating the integrand directly at every node
+// (the FP64 path, arithmetic identical to the oracle's).
+template <typename T, int BLOCK, int K>
+__device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
+{
+  constexpr int d = KindAxes<K>::nd;
+  constexpr int nn = gm_num_nodes(d);
+  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+    int const r = q / nn, j = q - r * nn;
+    double y[d];
+#pragma unroll
+    for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_offset(d, j, a);
+    S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
+  }
+}
+
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
 template <typename T, int BLOCK>
@@ -276,13 +292,10 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       default: eval_nodes<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   } else {
-    for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
-      int const r = q / nn, j = q - r * nn;
-      double y[kMaxDim];
-#pragma unroll
-      for (int a = 0; a < kMaxDim; ++a)
-        y[a] = a < d ? S.c[r][a] + S.h[r][a] * gm_offset(d, j, a) : 0.0;
-      S.f[q] = integrand<T>(S.ig, S.P, tb, y);
+    switch (S.ig.kind) {
+      case Y3G_COUNT3D: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
+      case Y3G_COUNT5D: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D>(S, m, tb); break;
+      default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }
   __syncthreads();

@@ -382,16 +382,25 @@ Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 // through v in [0, 1]; for Delta-Sigma the offset radius is integrated in
 // its Rayleigh CDF u (P(rmis) drmis = du) and the angle on [0, pi] (the
 // integrand is even about pi).
-template <typename T>
-Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
-                   const double* y)
+// Axis positions of the integration variables per kind, at compile time
+// (node coordinates then stay in registers).
+template <int K> struct KindAxes;
+template <> struct KindAxes<Y3G_COUNT3D> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_COUNT5D> {
+  static constexpr int it = 2, iz = 3, im = 4, nd = 5;
+};
+template <> struct KindAxes<Y3G_DSIGMA6D> {
+  static constexpr int it = 1, iz = 2, im = 3, nd = 6;
+};
+
+// Integrand of kind K (axis positions known at compile time).
+template <typename T, int K>
+Y3G_HD T integrand_k(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                     const double* y)
 {
-  int it, iz, im;
-  switch (ig.kind) {
-    case Y3G_COUNT3D: it = 0, iz = 1, im = 2; break;
-    case Y3G_COUNT5D: it = 2, iz = 3, im = 4; break;
-    default: it = 1, iz = 2, im = 3; break;
-  }
+  constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz, im = KindAxes<K>::im;
   double const zt = y[iz], lnM = y[im];
   T const sln = T(P.p.mor_sigma_ln);
   T const zp = T(1) + T(zt);
@@ -408,34 +417,42 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
   T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (thi - tlo);
   T const common = hmf<T>(P, tb.hmf, lnM, zt) * phi * dv_do_dz<T>(P, tb.da, zt);
   T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
+  if constexpr (K == Y3G_COUNT3D) {
+    return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+  } else if constexpr (K == Y3G_COUNT5D) {
+    T const lo = T(y[0]), zo = T(y[1]);
+    return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+           zo_pdf<T>(zo, T(zt), T(P.p.sigz0));
+  } else {
+    T const lo = T(y[0]);
+    T const s = T(ig.sigma_mis);
+    // -2 ln(1 - u): log1p for small u, ln of the FP64 difference near 1
+    double const u = y[4];
+    T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
+    T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
+    T const th = T(y[5]);
+    T const rho = T(P.rho_m);
+    NfwHalo<T> const H = nfw_halo<T>(lnM, rho, T(0));
+    T const R = T(ig.radius);
+    T const fmis = T(P.p.f_mis);
+    T const prof = (T(1) - fmis) * nfw_dsigma<T>(R, H) +
+                   fmis * dsigma_offset<T>(R, rmis, th, H);
+    T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
+    // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
+    return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
+           zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
+           T(ig.rmis_norm * 0.31830988618379067) * prof;
+  }
+}
+
+template <typename T>
+Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                   const double* y)
+{
   switch (ig.kind) {
-    case Y3G_COUNT3D:
-      return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
-    case Y3G_COUNT5D: {
-      T const lo = T(y[0]), zo = T(y[1]);
-      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
-             zo_pdf<T>(zo, T(zt), T(P.p.sigz0));
-    }
-    default: {
-      T const lo = T(y[0]);
-      T const s = T(ig.sigma_mis);
-      // -2 ln(1 - u): log1p for small u, ln of the FP64 difference near 1
-      double const u = y[4];
-      T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
-      T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
-      T const th = T(y[5]);
-      T const rho = T(P.rho_m);
-      NfwHalo<T> const H = nfw_halo<T>(lnM, rho, T(0));
-      T const R = T(ig.radius);
-      T const fmis = T(P.p.f_mis);
-      T const prof = (T(1) - fmis) * nfw_dsigma<T>(R, H) +
-                     fmis * dsigma_offset<T>(R, rmis, th, H);
-      T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
-      // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
-      return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
-             zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
-             T(ig.rmis_norm * 0.31830988618379067) * prof;
-    }
+    case Y3G_COUNT3D: return integrand_k<T, Y3G_COUNT3D>(ig, P, tb, y);
+    case Y3G_COUNT5D: return integrand_k<T, Y3G_COUNT5D>(ig, P, tb, y);
+    default: return integrand_k<T, Y3G_DSIGMA6D>(ig, P, tb, y);
   }
 }
 
@@ -491,18 +508,6 @@ Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   return f;
 }
 
-// Axis positions of the integration variables per kind, at compile time
-// (node coordinates then stay in registers).
-template <int K> struct KindAxes;
-template <> struct KindAxes<Y3G_COUNT3D> {
-  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
-};
-template <> struct KindAxes<Y3G_COUNT5D> {
-  static constexpr int it = 2, iz = 3, im = 4, nd = 5;
-};
-template <> struct KindAxes<Y3G_DSIGMA6D> {
-  static constexpr int it = 1, iz = 2, im = 3, nd = 6;
-};
 
 template <typename T, int K>
 Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,

@@ -368,6 +368,159 @@ template <> struct Uniform<float> {
   }
 };
 
+// One integral of kind K by VEGAS + m-Cubes (the whole workgroup).
+template <typename T, int BLOCK, int K>
+__device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs& A, int iid,
+                                          unsigned long long& nev_total)
+{
+  int const tid = threadIdx.x;
+  int const nb = A.vegas_nbins;
+  constexpr int d = KindAxes<K>::nd;
+  Tabs<T> tb{S.hmf, S.da, S.sci};
+  for (int q = tid; q < d * (nb + 1); q += BLOCK) {
+    int const a = q / (nb + 1), b = q - a * (nb + 1);
+    S.xg[a][b] = double(b) / nb;
+  }
+  long long const calls = A.vegas_calls;
+  int g = (int)floor(pow(calls / 2.0, 1.0 / d));
+  if (g < 1) g = 1;
+  long long ncubes = 1;
+  for (int a = 0; a < d; ++a) ncubes *= g;
+  long long p = calls / ncubes;
+  if (p < 2) p = 2;
+  double range[d], vol = 1.0;
+  for (int a = 0; a < d; ++a)
+    range[a] = S.ig.highs[a] - S.ig.lows[a], vol *= range[a];
+  uint32_t const k0 = (uint32_t)S.ig.seed, k1 = (uint32_t)(S.ig.seed >> 32);
+  int const nblk = (Uniform<T>::words_per_dim * d + 3) / 4;
+  double wsum = 0.0, isum = 0.0, i2sum = 0.0, I = 0.0, sig = 0.0;
+  int nacc = 0, status = 1;
+  long long neval = 0;
+  if (tid == 0) S.stop = 0;
+  __syncthreads();
+  for (int it = 0; it < A.vegas_itmax; ++it) {
+    for (int q = tid; q < d * nb; q += BLOCK) S.hist[q / nb][q % nb] = 0.0;
+    __syncthreads();
+    double sI = 0.0, sV = 0.0;
+    for (long long c = tid; c < ncubes; c += BLOCK) {
+      int ck[d];
+      long long cc = c;
+      for (int a = 0; a < d; ++a) ck[a] = (int)(cc % g), cc /= g;
+      double s1 = 0.0, s2 = 0.0;
+      for (long long j = 0; j < p; ++j) {
+        uint32_t w[12];
+        for (int blk = 0; blk < nblk; ++blk) {
+          uint32_t ctr[4] = {(uint32_t)c, (uint32_t)j, (uint32_t)it,
+                             (uint32_t)blk};
+          philox(ctr, k0, k1);
+          w[4 * blk] = ctr[0], w[4 * blk + 1] = ctr[1];
+          w[4 * blk + 2] = ctr[2], w[4 * blk + 3] = ctr[3];
+        }
+        double y[d], jac = vol;
+        int bins[d];
+        for (int a = 0; a < d; ++a) {
+          double const u = Uniform<T>::get(w, a);
+          double const z = (ck[a] + u) / g * nb;
+          int b = (int)z;
+          if (b > nb - 1) b = nb - 1;
+          double const wb = S.xg[a][b + 1] - S.xg[a][b];
+          y[a] = S.ig.lows[a] + (S.xg[a][b] + (z - b) * wb) * range[a];
+          jac *= nb * wb;
+          bins[a] = b;
+        }
+        double const fx = double(integrand_k<T, K>(S.ig, S.P, tb, y)) * jac;
+        s1 += fx;
+        s2 += fx * fx;
+        for (int a = 0; a < d; ++a) atomicAdd(&S.hist[a][bins[a]], fx * fx);
+      }
+      double const m = s1 / p;
+      double const v = (s2 - s1 * m) / (double(p) * (p - 1));
+      sI += m;
+      sV += v > 0.0 ? v : 0.0;
+    }
+    block_sum2<BLOCK>(sI, sV, S.red);
+    neval += ncubes * p;
+    double const Ii = sI / ncubes;
+    double const Vi = sV / (double(ncubes) * ncubes);
+    if (it >= A.vegas_itskip) {
+      double const wi = Vi > 0.0 ? 1.0 / Vi : 1e300;
+      wsum += wi;
+      isum += wi * Ii;
+      i2sum += wi * Ii * Ii;
+      ++nacc;
+      I = isum / wsum;
+      sig = sqrt(1.0 / wsum);
+      double const dof = nacc - 1;
+      double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
+      double const sigr = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
+      if (sigr <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
+        status = 0;
+        break;
+      }
+    }
+    if (neval + ncubes * p > S.ig.max_eval) break;
+    if (it + 1 < A.vegas_itmax) {
+      // Lepage refinement, one thread per axis
+      if (tid < d) {
+        int const a = tid;
+        double* hs = S.hist[a];
+        double* dd = S.d1[a];
+        double sum = 0.0;
+        if (nb == 1) {
+          dd[0] = hs[0];
+        } else {
+          dd[0] = 0.5 * (hs[0] + hs[1]);
+          dd[nb - 1] = 0.5 * (hs[nb - 2] + hs[nb - 1]);
+          for (int b = 1; b < nb - 1; ++b)
+            dd[b] = (hs[b - 1] + hs[b] + hs[b + 1]) / 3.0;
+        }
+        for (int b = 0; b < nb; ++b) sum += dd[b];
+        if (sum > 0.0) {
+          double rs = 0.0;
+          for (int b = 0; b < nb; ++b) {
+            double const q = dd[b] / sum;
+            dd[b] = (q > 0.0 && q < 1.0) ? pow((q - 1.0) / log(q), A.vegas_alpha)
+                                         : 0.0;
+            rs += dd[b];
+          }
+          if (rs > 0.0) {
+            double const target = rs / nb;
+            double xn[kNbMax + 1];
+            int kk = -1;
+            double dr = 0.0;
+            xn[0] = 0.0;
+            for (int i = 1; i < nb; ++i) {
+              while (dr < target && kk < nb - 1) {
+                ++kk;
+                dr += dd[kk];
+              }
+              dr -= target;
+              xn[i] = S.xg[a][kk + 1] -
+                      (S.xg[a][kk + 1] - S.xg[a][kk]) * dr / dd[kk];
+            }
+            xn[nb] = 1.0;
+            for (int i = 0; i <= nb; ++i) S.xg[a][i] = xn[i];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    double const dof = nacc - 1;
+    double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
+    y3g_result res;
+    res.value = I;
+    res.error = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
+    res.prob = chi;
+    res.neval = neval;
+    res.nregions = 0;
+    res.status = status;
+    A.out[iid] = res;
+    nev_total += (unsigned long long)neval;
+  }
+}
+
 template <typename T, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
 {
@@ -393,149 +546,10 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       __syncthreads();
       if (tid == 0) S.point = pt;
     }
-    int const d = S.ig.ndim;
-    Tabs<T> tb{S.hmf, S.da, S.sci};
-    for (int q = tid; q < d * (nb + 1); q += BLOCK) {
-      int const a = q / (nb + 1), b = q - a * (nb + 1);
-      S.xg[a][b] = double(b) / nb;
-    }
-    long long const calls = A.vegas_calls;
-    int g = (int)floor(pow(calls / 2.0, 1.0 / d));
-    if (g < 1) g = 1;
-    long long ncubes = 1;
-    for (int a = 0; a < d; ++a) ncubes *= g;
-    long long p = calls / ncubes;
-    if (p < 2) p = 2;
-    double range[kMaxDim], vol = 1.0;
-    for (int a = 0; a < d; ++a)
-      range[a] = S.ig.highs[a] - S.ig.lows[a], vol *= range[a];
-    uint32_t const k0 = (uint32_t)S.ig.seed, k1 = (uint32_t)(S.ig.seed >> 32);
-    int const nblk = (Uniform<T>::words_per_dim * d + 3) / 4;
-    double wsum = 0.0, isum = 0.0, i2sum = 0.0, I = 0.0, sig = 0.0;
-    int nacc = 0, status = 1;
-    long long neval = 0;
-    if (tid == 0) S.stop = 0;
-    __syncthreads();
-    for (int it = 0; it < A.vegas_itmax; ++it) {
-      for (int q = tid; q < d * nb; q += BLOCK) S.hist[q / nb][q % nb] = 0.0;
-      __syncthreads();
-      double sI = 0.0, sV = 0.0;
-      for (long long c = tid; c < ncubes; c += BLOCK) {
-        int ck[kMaxDim];
-        long long cc = c;
-        for (int a = 0; a < d; ++a) ck[a] = (int)(cc % g), cc /= g;
-        double s1 = 0.0, s2 = 0.0;
-        for (long long j = 0; j < p; ++j) {
-          uint32_t w[12];
-          for (int blk = 0; blk < nblk; ++blk) {
-            uint32_t ctr[4] = {(uint32_t)c, (uint32_t)j, (uint32_t)it,
-                               (uint32_t)blk};
-            philox(ctr, k0, k1);
-            w[4 * blk] = ctr[0], w[4 * blk + 1] = ctr[1];
-            w[4 * blk + 2] = ctr[2], w[4 * blk + 3] = ctr[3];
-          }
-          double y[kMaxDim], jac = vol;
-          int bins[kMaxDim];
-          for (int a = 0; a < d; ++a) {
-            double const u = Uniform<T>::get(w, a);
-            double const z = (ck[a] + u) / g * nb;
-            int b = (int)z;
-            if (b > nb - 1) b = nb - 1;
-            double const wb = S.xg[a][b + 1] - S.xg[a][b];
-            y[a] = S.ig.lows[a] + (S.xg[a][b] + (z - b) * wb) * range[a];
-            jac *= nb * wb;
-            bins[a] = b;
-          }
-          double const fx = double(integrand<T>(S.ig, S.P, tb, y)) * jac;
-          s1 += fx;
-          s2 += fx * fx;
-          for (int a = 0; a < d; ++a) atomicAdd(&S.hist[a][bins[a]], fx * fx);
-        }
-        double const m = s1 / p;
-        double const v = (s2 - s1 * m) / (double(p) * (p - 1));
-        sI += m;
-        sV += v > 0.0 ? v : 0.0;
-      }
-      block_sum2<BLOCK>(sI, sV, S.red);
-      neval += ncubes * p;
-      double const Ii = sI / ncubes;
-      double const Vi = sV / (double(ncubes) * ncubes);
-      if (it >= A.vegas_itskip) {
-        double const wi = Vi > 0.0 ? 1.0 / Vi : 1e300;
-        wsum += wi;
-        isum += wi * Ii;
-        i2sum += wi * Ii * Ii;
-        ++nacc;
-        I = isum / wsum;
-        sig = sqrt(1.0 / wsum);
-        double const dof = nacc - 1;
-        double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
-        double const sigr = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
-        if (sigr <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
-          status = 0;
-          break;
-        }
-      }
-      if (neval + ncubes * p > S.ig.max_eval) break;
-      if (it + 1 < A.vegas_itmax) {
-        // Lepage refinement, one thread per axis
-        if (tid < d) {
-          int const a = tid;
-          double* hs = S.hist[a];
-          double* dd = S.d1[a];
-          double sum = 0.0;
-          if (nb == 1) {
-            dd[0] = hs[0];
-          } else {
-            dd[0] = 0.5 * (hs[0] + hs[1]);
-            dd[nb - 1] = 0.5 * (hs[nb - 2] + hs[nb - 1]);
-            for (int b = 1; b < nb - 1; ++b)
-              dd[b] = (hs[b - 1] + hs[b] + hs[b + 1]) / 3.0;
-          }
-          for (int b = 0; b < nb; ++b) sum += dd[b];
-          if (sum > 0.0) {
-            double rs = 0.0;
-            for (int b = 0; b < nb; ++b) {
-              double const q = dd[b] / sum;
-              dd[b] = (q > 0.0 && q < 1.0) ? pow((q - 1.0) / log(q), A.vegas_alpha)
-                                           : 0.0;
-              rs += dd[b];
-            }
-            if (rs > 0.0) {
-              double const target = rs / nb;
-              double xn[kNbMax + 1];
-              int kk = -1;
-              double dr = 0.0;
-              xn[0] = 0.0;
-              for (int i = 1; i < nb; ++i) {
-                while (dr < target && kk < nb - 1) {
-                  ++kk;
-                  dr += dd[kk];
-                }
-                dr -= target;
-                xn[i] = S.xg[a][kk + 1] -
-                        (S.xg[a][kk + 1] - S.xg[a][kk]) * dr / dd[kk];
-              }
-              xn[nb] = 1.0;
-              for (int i = 0; i <= nb; ++i) S.xg[a][i] = xn[i];
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (tid == 0) {
-      double const dof = nacc - 1;
-      double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
-      y3g_result res;
-      res.value = I;
-      res.error = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
-      res.prob = chi;
-      res.neval = neval;
-      res.nregions = 0;
-      res.status = status;
-      A.out[iid] = res;
-      nev_total += (unsigned long long)neval;
+    switch (S.ig.kind) {
+      case Y3G_COUNT3D: vegas_one<T, BLOCK, Y3G_COUNT3D>(S, A, iid, nev_total); break;
+      case Y3G_COUNT5D: vegas_one<T, BLOCK, Y3G_COUNT5D>(S, A, iid, nev_total); break;
+      default: vegas_one<T, BLOCK, Y3G_DSIGMA6D>(S, A, iid, nev_total); break;
     }
   }
   if (tid == 0 && nev_total) atomicAdd(A.nevals, nev_total);
