@@ -134,6 +134,11 @@ typedef struct {
                              sequence (the plain path)                   */
   double err_c, err_smin; /* region error |R7-R5| * clamp(c r, smin, 1),
                              r = |R7-R5| / |R5-R3|; c = 0 keeps |R7-R5|  */
+  double tl_c, tl_smin;   /* two-level estimate (batched cubature): the
+                             children of a split region report |R7-R5| x
+                             clamp(tl_c d / e, tl_smin, 1), d = |R7(A) +
+                             R7(B) - R7(parent)|, e = the parent's |R7-R5|;
+                             tl_c = 0 keeps |R7-R5|                     */
 } y3g_options;
 
 typedef struct y3g_runtime y3g_runtime;

@@ -55,7 +55,8 @@ class CubOpts(C.Structure):
     _fields_ = [("eps_rel", C.c_double), ("eps_abs", C.c_double),
                 ("max_eval", C.c_longlong), ("max_regions", C.c_int),
                 ("max_split_per_step", C.c_int), ("init_split", C.c_int),
-                ("err_c", C.c_double), ("err_smin", C.c_double)]
+                ("err_c", C.c_double), ("err_smin", C.c_double),
+                ("tl_c", C.c_double), ("tl_smin", C.c_double)]
 
 
 class VegasOpts(C.Structure):
@@ -222,9 +223,10 @@ def integrand(params, tables, ig, x):
 
 
 def cub_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000, max_regions=65536,
-             max_split_per_step=1024, init_split=0, err_c=0.0, err_smin=1.0):
+             max_split_per_step=1024, init_split=0, err_c=0.0, err_smin=1.0, tl_c=0.0,
+             tl_smin=1.0 / 16.0):
     return CubOpts(eps_rel, eps_abs, max_eval, max_regions, max_split_per_step, init_split,
-                   err_c, err_smin)
+                   err_c, err_smin, tl_c, tl_smin)
 
 
 def vegas_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000,

@@ -187,6 +187,7 @@ typedef struct {
   int max_split_per_step;
   int init_split;           /* pieces per axis of the initial partition */
   double err_c, err_smin;   /* region error E1 clamp(c E1/E2, smin, 1)     */
+  double tl_c, tl_smin;     /* two-level child scale clamp(c d/e, smin, 1) */
 } y3o_cub_opts;
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,

@@ -702,9 +702,20 @@ typedef struct {
 
 typedef struct {
   double lo[8], hi[8];
-  double est, err;
+  double est, err, raw; /* raw = the region's own |R7 - R5| */
   int axis;
 } region_t;
+
+/* Two-level scale of a split region's children (device: two_level_scale in
+ * y3g_cub_common.hpp): clamp(c d / e, smin, 1), d = |R7(A) + R7(B) -
+ * R7(parent)|, e = the parent's |R7 - R5|; c = 0 leaves them unscaled. */
+static double
+two_level_scale(double d, double e, double c, double smin)
+{
+  if (!(c > 0.0) || !(e > 0.0)) return 1.0;
+  double const s = c * d / e;
+  return s < smin ? smin : (s > 1.0 ? 1.0 : s);
+}
 
 static void
 eval_region(const cub_prob* P, region_t* r)
@@ -717,6 +728,7 @@ eval_region(const cub_prob* P, region_t* r)
   double r3;
   r->est = y3o_gm_rule3(P->f, P->ctx, P->d, c, hw, &r5, &r3, fourth);
   r->err = rule_error(r->est, r5, r3, P->err_c, P->err_smin);
+  r->raw = r->err;
   int ax = 0;
   double best = fourth[0];
   for (int i = 1; i < P->d; ++i)
@@ -858,6 +870,9 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
       eval_region(P, &a);
       eval_region(P, &b);
       double const delta = fabs(a.est + b.est - par.est);
+      double const sc = two_level_scale(delta, par.raw, o->tl_c, o->tl_smin);
+      a.err *= sc;
+      b.err *= sc;
       double const se = a.err + b.err;
       if (delta > se) {
         if (se > 0.0) {

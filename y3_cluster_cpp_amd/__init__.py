@@ -63,7 +63,8 @@ class Options(C.Structure):
                 ("vegas_calls", C.c_longlong), ("vegas_itmax", C.c_int),
                 ("vegas_itskip", C.c_int), ("vegas_nbins", C.c_int),
                 ("vegas_alpha", C.c_double), ("single", C.c_int),
-                ("err_c", C.c_double), ("err_smin", C.c_double)]
+                ("err_c", C.c_double), ("err_smin", C.c_double),
+                ("tl_c", C.c_double), ("tl_smin", C.c_double)]
 
 
 class ServerOptions(C.Structure):

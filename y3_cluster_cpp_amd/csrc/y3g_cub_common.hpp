@@ -189,7 +189,7 @@ template <typename T, int BLOCK> struct CubShared {
   double est[kChunk], err[kChunk];
   int ax[kChunk];
   int idx[kChunk];
-  double pest[kChunk / 2], perr[kChunk / 2];
+  double pest[kChunk / 2], perr[kChunk / 2], praw[kChunk / 2];
   int sel[kKmax];
   double red[2 * (BLOCK / 64)];
   int ired[BLOCK / 64];
@@ -275,6 +275,19 @@ __device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
     for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_offset(d, j, a);
     S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
   }
+}
+
+// Two-level scale of a split region's children: their |R7-R5| times
+// clamp(c d / e, smin, 1), where d = |R7(A) + R7(B) - R7(parent)| measures
+// the parent's degree-7 error and e is the parent's |R7-R5|, i.e. the
+// children's degree-5 differences rescaled by how much the parent's
+// overstated its degree-7 error (c = 0: no scaling).  Restated by the
+// oracle (y3o_quad.c two_level_scale).
+__host__ __device__ inline double two_level_scale(double d, double e, double c, double smin)
+{
+  if (!(c > 0.0) || !(e > 0.0)) return 1.0;
+  double const s = c * d / e;
+  return s < smin ? smin : (s > 1.0 ? 1.0 : s);
 }
 
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:

@@ -44,11 +44,13 @@ struct KernArgs {
   double* slab_hi;
   double* slab_est;
   double* slab_err;
+  double* slab_raw;  // the regions' own |R7-R5| (two-level estimate)
   int8_t* slab_ax;
   int cap;
   int kmax;
   int init_split;
   double err_c, err_smin;
+  double tl_c, tl_smin;
   // VEGAS
   long long vegas_calls;
   int vegas_itmax, vegas_itskip, vegas_nbins;

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   double* hi = A.slab_hi + (size_t)slot * kMaxDim * cap;
   double* est = A.slab_est + (size_t)slot * cap;
   double* err = A.slab_err + (size_t)slot * cap;
+  double* raw = A.slab_raw + (size_t)slot * cap;
   int8_t* axs = A.slab_ax + (size_t)slot * cap;
   if (tid == 0) S.point = -1;
   unsigned long long nev_total = 0;
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       if (tid < m) {
         est[base + tid] = S.est[tid];
         err[base + tid] = S.err[tid];
+        raw[base + tid] = S.err[tid];
         axs[base + tid] = (int8_t)S.ax[tid];
       }
       __syncthreads();
@@ -267,14 +269,17 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
           int const p = S.sel[pb + tid];
           S.pest[tid] = est[p];
           S.perr[tid] = err[p];
+          S.praw[tid] = raw[p];
         }
         __syncthreads();
         eval_chunk<T, BLOCK>(S, 2 * np, A.err_c, A.err_smin);
         if (tid < np) {
           int const p = S.sel[pb + tid];
-          double ea = S.err[2 * tid], eb = S.err[2 * tid + 1];
+          double const ra = S.err[2 * tid], rb = S.err[2 * tid + 1];
           double const delta =
             fabs(S.est[2 * tid] + S.est[2 * tid + 1] - S.pest[tid]);
+          double const sc = two_level_scale(delta, S.praw[tid], A.tl_c, A.tl_smin);
+          double ea = ra * sc, eb = rb * sc;
           double const se = ea + eb;
           if (delta > se) {
             if (se > 0.0) {
@@ -289,9 +294,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
           dE += ea + eb - S.perr[tid];
           est[p] = S.est[2 * tid];
           err[p] = ea;
+          raw[p] = ra;
           axs[p] = (int8_t)S.ax[2 * tid];
           est[N + pb + tid] = S.est[2 * tid + 1];
           err[N + pb + tid] = eb;
+          raw[N + pb + tid] = rb;
           axs[N + pb + tid] = (int8_t)S.ax[2 * tid + 1];
         }
         __syncthreads();

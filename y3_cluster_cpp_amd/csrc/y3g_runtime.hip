@@ -72,7 +72,7 @@ struct y3g_runtime {
   int hmf_stride = 0, da_stride = 0, sci_stride = 0;
   // cubature slab
   double *slab_lo = nullptr, *slab_hi = nullptr, *slab_est = nullptr,
-         *slab_err = nullptr;
+         *slab_err = nullptr, *slab_raw = nullptr;
   int8_t* slab_ax = nullptr;
   int slab_slots = 0, slab_cap = 0;
   // batch buffers
@@ -111,6 +111,8 @@ void y3g_options_default(y3g_options* o)
   o->single = 0;
   o->err_c = 0.0;
   o->err_smin = 1.0;
+  o->tl_c = 0.0;
+  o->tl_smin = 1.0 / 16.0;
 }
 
 void y3g_default_params(y3g_params* p)
@@ -319,7 +321,7 @@ int y3g_runtime_destroy(y3g_runtime* rt)
   dfree(rt->hmf64), dfree(rt->da64), dfree(rt->sci64);
   dfree(rt->hmf32), dfree(rt->da32), dfree(rt->sci32);
   dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
-    dfree(rt->slab_err), dfree(rt->slab_ax);
+    dfree(rt->slab_err), dfree(rt->slab_raw), dfree(rt->slab_ax);
   dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
   dfree(rt->d_queue), dfree(rt->d_nevals);
   dfree(rt->sg.lo), dfree(rt->sg.hi), dfree(rt->sg.est), dfree(rt->sg.err);
@@ -481,12 +483,12 @@ static int ensure_slab(y3g_runtime* rt, int slots, int cap)
 {
   if (slots <= rt->slab_slots && cap == rt->slab_cap) return 0;
   dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
-    dfree(rt->slab_err), dfree(rt->slab_ax);
+    dfree(rt->slab_err), dfree(rt->slab_raw), dfree(rt->slab_ax);
   rt->slab_slots = rt->slab_cap = 0;
   size_t const n = (size_t)slots * cap;
   if (dalloc(&rt->slab_lo, n * kMaxDim) || dalloc(&rt->slab_hi, n * kMaxDim) ||
       dalloc(&rt->slab_est, n) || dalloc(&rt->slab_err, n) ||
-      dalloc(&rt->slab_ax, n))
+      dalloc(&rt->slab_raw, n) || dalloc(&rt->slab_ax, n))
     return Y3G_ERR_MEMORY;
   rt->slab_slots = slots;
   rt->slab_cap = cap;
@@ -558,7 +560,7 @@ static int run_group(y3g_runtime* rt, const y3g_integral* igs,
     if (int rc = ensure_slab(rt, std::max(slots, rt->slab_slots), A.cap))
       return rc;
     A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
-    A.slab_est = rt->slab_est, A.slab_err = rt->slab_err;
+    A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
     A.slab_ax = rt->slab_ax;
     HIPCHK(launch_cubature(A, o.precision, block, slots, rt->stream));
   } else {
@@ -613,6 +615,8 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.init_split = o.init_split;
   A.err_c = o.err_c;
   A.err_smin = o.err_smin;
+  A.tl_c = o.tl_c;
+  A.tl_smin = o.tl_smin;
   A.vegas_calls = o.vegas_calls;
   A.vegas_itmax = o.vegas_itmax;
   A.vegas_itskip = o.vegas_itskip;
@@ -678,7 +682,7 @@ int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes
   y3g_options o;
   y3g_options_default(&o);
   if (opt) o = *opt;
-  size_t const per_region = 2 * kMaxDim * sizeof(double) + 2 * sizeof(double) + 1;
+  size_t const per_region = 2 * kMaxDim * sizeof(double) + 3 * sizeof(double) + 1;
   size_t const slots = (size_t)resident_slots(rt, o, Y3G_PAGANI);
   *bytes = slots * (size_t)std::max(o.max_regions, 1) * per_region;
   return 0;
