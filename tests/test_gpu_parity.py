@@ -284,3 +284,54 @@ def test_plain_path_device_wide(runtime, prec):
     _check(res, np.array([gold1[0, 0], gold1[3, 2], gold2[1, 1, 9]]), np.array([1e-4, 1e-3, 1e-3]))
     ms, nev = runtime.last_timing()
     assert nev == res["neval"].sum()
+
+
+def test_two_level_fp64_equals_cpu_restatement(runtime):
+    """The optional two-level estimate follows the oracle's restatement:
+    same regions, evaluations and value in FP64."""
+    p, s = fid(runtime, wide=False)
+    op = O.default_params(f_prj=0.0)
+    ot = O.OracleTables(op, O.spec_config0())
+    oig = O.make_integral(op, O.spec_config0(), O.COUNT3D)
+    for eps in (1e-4, 1e-6):
+        ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=eps)
+        g = runtime.integrate([ig], y3.options(precision=y3.FP64, tl_c=2.0, tl_smin=0.125))[0]
+        c = O.cubature(op, ot, oig, O.cub_opts(eps_rel=eps, max_regions=262144, max_eval=1 << 40,
+                                               tl_c=2.0, tl_smin=0.125))
+        assert (g["nregions"], g["neval"], g["status"]) == (c.nregions, c.neval, c.status)
+        assert abs(g["value"] / c.value - 1) < 1e-11 and abs(g["error"] / c.error - 1) < 1e-9
+
+
+def test_large_batch_independence(runtime):
+    """49,152 integrals (256 prior points) in one persistent launch: the
+    first 16 points' results are bit-identical to a 16-point launch, every
+    result is finite within the region budget, and the first 64 points'
+    reported errors bound their true errors against the golden values."""
+    from y3_cluster_cpp_amd import workloads as W
+    with open(os.path.join(GOLDEN, "golden_points.json")) as f:
+        gp = json.load(f)
+    ref = np.concatenate([np.array(gp["counts"]), np.array(gp["dsigma"])], axis=1)
+    pts = W.prior_points(256)
+    assert np.allclose(pts[:64], np.array(gp["points"]))
+    eps = np.minimum(np.vstack([W.eps_stream(256)]), 1e-4)
+    spec = y3.spec_wide()
+    params = [W.point_params(y3, pt) for pt in pts]
+    opts = y3.options(max_regions=32768)
+
+    def run(n):
+        igs = []
+        for k in range(n):
+            igs += W.point_integrals(y3, params[k], spec, k, eps[k], y3.PAGANI)
+        runtime.tables_build(params[:n], spec)
+        return runtime.integrate((y3.Integral * len(igs))(*igs), opts)
+
+    big = run(256)
+    small = run(16)
+    assert len(big) == 256 * 192
+    for f in ("value", "error", "nregions", "neval", "status"):
+        np.testing.assert_array_equal(big[f][:16 * 192], small[f], err_msg=f)
+    assert np.all(np.isfinite(big["value"])) and np.all(np.isfinite(big["error"]))
+    assert np.all(big["nregions"] <= 32768) and set(np.unique(big["status"])) <= {0, 1}
+    head = big[:64 * 192]
+    err = np.abs(head["value"] - ref.reshape(-1))
+    assert np.all(head["error"] >= err)
