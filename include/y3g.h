@@ -76,6 +76,8 @@ typedef struct {
   double f_prj, prj_scale;
   double f_mis, mis_rlambda;
   double z_source, sigz0;
+  double hmf_s, hmf_q;  /* HMF_t amplitude factor s (lnM log10 e - 13.81) + q
+                           (hmf_t.cuh:54-60); 0 and 1 for synthetic tables */
 } y3g_params;
 
 /* Uniform table grids of one parameter point. */

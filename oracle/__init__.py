@@ -19,7 +19,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "omega_m", "h", "hmf_amp", "hmf_A", "hmf_a", "hmf_c", "mor_lnl0",
         "mor_slope", "mor_zslope", "mor_sigma_ln", "f_prj", "prj_scale",
-        "f_mis", "mis_rlambda", "z_source", "sigz0")]
+        "f_mis", "mis_rlambda", "z_source", "sigz0", "hmf_s", "hmf_q")]
 
 
 class TableSpec(C.Structure):
@@ -112,7 +112,7 @@ def _load():
         "y3o_spec_wide": (None, [C.POINTER(TableSpec)]),
         "y3o_tables_build": (C.c_int, [C.POINTER(Params), C.POINTER(TableSpec), C.POINTER(Tables)]),
         "y3o_tables_free": (None, [C.POINTER(Tables)]),
-        "y3o_hmf": (_D, [C.POINTER(Tables), _D, _D]),
+        "y3o_hmf": (_D, [C.POINTER(Params), C.POINTER(Tables), _D, _D]),
         "y3o_dvdz": (_D, [C.POINTER(Params), C.POINTER(Tables), _D]),
         "y3o_sci": (_D, [C.POINTER(Tables), _D]),
         "y3o_integrand": (_D, [C.POINTER(Params), C.POINTER(Tables), C.POINTER(Integral), _PD]),

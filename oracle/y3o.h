@@ -50,6 +50,9 @@ typedef struct {
   double mis_rlambda;    /* sigma_Rayleigh / R_lambda (0.2)                 */
   double z_source;       /* source plane for Sigma_crit^-1 (1.2)            */
   double sigz0;          /* photo-z scatter sigma_z = sigz0 (1+z) (0.01)    */
+  double hmf_s, hmf_q;   /* HMF_t amplitude factor s (lnM log10 e - 13.81)
+                            + q (hmf_t.cuh:54-60); 0 and 1 for the
+                            synthetic tables                               */
 } y3o_params;
 
 /* Table geometry (uniform grids). */
@@ -144,7 +147,7 @@ void y3o_spec_wide(y3o_table_spec* s);
 int  y3o_tables_build(const y3o_params* p, const y3o_table_spec* s,
                       y3o_tables* t);
 void y3o_tables_free(y3o_tables* t);
-double y3o_hmf(const y3o_tables* t, double lnM, double zt);
+double y3o_hmf(const y3o_params* p, const y3o_tables* t, double lnM, double zt);
 double y3o_dvdz(const y3o_params* p, const y3o_tables* t, double zt);
 double y3o_sci(const y3o_tables* t, double zt);
 

@@ -304,6 +304,8 @@ y3o_default_params(y3o_params* p)
   p->mis_rlambda = 0.2;
   p->z_source = 1.2;
   p->sigz0 = 0.01;
+  p->hmf_s = 0.0;
+  p->hmf_q = 1.0;
 }
 
 /* dn/dlnM = A (sigma^-a + 1) exp(-c / sigma^2),
@@ -589,10 +591,13 @@ y3o_tables_free(y3o_tables* t)
 /* HMF_t::operator() (hmf_t.hh:45-50) with s = 0, q = 1: the clamped
  * bilinear table lookup. */
 double
-y3o_hmf(const y3o_tables* t, double lnM, double zt)
+y3o_hmf(const y3o_params* p, const y3o_tables* t, double lnM, double zt)
 {
+  /* HMF_t::operator() (hmf_t.cuh:54-60): clamped bilinear table value times
+   * s (lnM log10(e) - 13.8124426028) + q */
   return y3o_interp2d_clamp(t->lnm, t->spec.n_lnm, t->z, t->spec.n_z, t->hmf,
-                            lnM, zt);
+                            lnM, zt) *
+         (p->hmf_s * (lnM * 0.4342944819 - 13.8124426028) + p->hmf_q);
 }
 
 double
@@ -622,12 +627,12 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
   switch (ig->kind) {
     case Y3O_COUNT3D: {
       double const lt = x[0], zt = x[1], lnM = x[2];
-      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_bin(p, ig->lo_lo, ig->lo_hi, lt) * y3o_dvdz(p, t, zt);
     }
     case Y3O_COUNT5D: {
       double const lo = x[0], zo = x[1], lt = x[2], zt = x[3], lnM = x[4];
-      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_pdf(p, lo, lt) * y3o_zo_pdf(p, zo, zt) *
              y3o_dvdz(p, t, zt);
     }
@@ -641,7 +646,7 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
       double const prof =
         (1.0 - p->f_mis) * y3o_nfw_dsigma(ig->radius, lnM, rho, 0.0) +
         p->f_mis * y3o_dsigma_offset(ig->radius, rmis, th, lnM, rho, 0.0);
-      return y3o_hmf(t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_pdf(p, lo, lt) *
              y3o_zo_bin(p, ig->zo_lo, ig->zo_hi, zt) * y3o_dvdz(p, t, zt) *
              y3o_sci(t, zt) * pr / (2.0 * Y3O_PI) * prof;

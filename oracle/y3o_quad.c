@@ -502,7 +502,7 @@ y3o_golden(const y3o_params* p, const y3o_tables* t, int count_kind,
         double const lnM = mn[k];
         double const L = pp.mor_lnl0 + pp.mor_slope * (lnM - ln3e14) + lz;
         cheb_eval_all(&cj, L, Jv);
-        double const base = wz * mw[k] * y3o_hmf(t, lnM, zt) * dv;
+        double const base = wz * mw[k] * y3o_hmf(&pp, t, lnM, zt) * dv;
         for (int i = 0; i < n_lo; ++i) {
           double const bj = base * Jv[i];
           for (int j = 0; j < nzb; ++j) {

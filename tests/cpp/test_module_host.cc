@@ -356,6 +356,61 @@ namespace {
     }
   }
 
+  // The reference's table sections in a sample (mass_function, distances).
+  void
+  test_table_inputs()
+  {
+    DataBlock cfg;
+    NumberCounts5DIntegrand ig(cfg);
+    y3g_table_spec sp;
+    y3g_spec_wide(&sp);
+    ig.set_spec(sp);
+    auto sample = [](int nm, bool uniform, bool good_shape) {
+      DataBlock s;
+      s.put_val("cosmological_parameters", "omega_M", 0.3);  // = omega_m
+      s.put_val("cosmological_parameters", "omega_nu", 0.0);
+      s.put_val("cluster_abundance", "hmf_s", 0.1);
+      s.put_val("cluster_abundance", "hmf_q", 0.9);
+      std::vector<double> mh(nm), z(8), vals;
+      for (int i = 0; i < nm; ++i) {
+        double const lnm = 30.0 + 5.0 * i / (nm - 1) + (uniform || i != 3 ? 0.0 : 0.01);
+        mh[i] = std::exp(lnm) / 0.3;
+      }
+      for (int j = 0; j < 8; ++j) z[j] = 0.1 + 0.1 * j;
+      for (int j = 0; j < 8; ++j)
+        for (int i = 0; i < nm; ++i) vals.push_back(100.0 * j + i);
+      s.put_val("mass_function", "m_h", mh);
+      s.put_val("mass_function", "z", z);
+      if (good_shape)
+        s.put_val("mass_function", "dndlnmh", ndarray<double>(vals, {8, std::size_t(nm)}));
+      else
+        s.put_val("mass_function", "dndlnmh", ndarray<double>(vals, {std::size_t(nm), 8}));
+      std::vector<double> dz(101), da(101);
+      for (int i = 0; i < 101; ++i) dz[i] = 0.01 * i, da[i] = 3000.0 * dz[i];
+      s.put_val("distances", "z", dz);
+      s.put_val("distances", "d_a", da);
+      return s;
+    };
+    DataBlock s = sample(50, true, true);
+    ig.set_sample(s);
+    auto const& q = ig.spec();
+    CHECK(q.n_lnm == 50 && q.n_z == 8 && close(q.lnm_lo, 30.0, 1e-12) &&
+          close(q.lnm_hi, 35.0, 1e-12) && close(q.z_lo, 0.1) && close(q.z_hi, 0.8));
+    CHECK(q.n_da == 101 && close(q.da_zmax, 1.0));
+    CHECK(ig.hmf_table().size() == 400 && ig.hmf_table()[51] == 101.0);
+    CHECK(ig.da_table().size() == 101 && ig.da_table()[100] == 3000.0);
+    CHECK(ig.params().omega_m == 0.3 && ig.params().hmf_s == 0.1 && ig.params().hmf_q == 0.9);
+    DataBlock bad = sample(50, false, true);
+    CHECK(throws<std::runtime_error>([&] { ig.set_sample(bad); }, "not uniformly spaced"));
+    DataBlock bad2 = sample(50, true, false);
+    CHECK(throws<std::runtime_error>([&] { ig.set_sample(bad2); }, "shape"));
+    // without the sections the configured spec comes back
+    DataBlock plain;
+    plain.put_val("cosmological_parameters", "omega_m", 0.3);
+    ig.set_sample(plain);
+    CHECK(ig.spec().n_lnm == sp.n_lnm && ig.hmf_table().empty() && ig.da_table().empty());
+  }
+
   // setup / execute / cleanup through dlopen, as CosmoSIS calls them.
   void
   test_entry_points(char const* module_path, bool have_gpu)
@@ -405,6 +460,7 @@ main(int argc, char** argv)
   test_integral_list();
   test_counts_lists();
   test_put_results();
+  test_table_inputs();
   if (argc > 1) {
     y3g_runtime* rt = nullptr;
     bool const have_gpu = y3g_runtime_create(0, &rt) == Y3G_OK;

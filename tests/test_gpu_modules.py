@@ -194,3 +194,47 @@ def test_module_mcubes_and_table_refresh(runtime):
     np.testing.assert_allclose(a, direct["value"], rtol=1e-8)
     np.testing.assert_allclose(probs, direct["prob"], rtol=1e-5, atol=1e-12)
     assert np.all(probs >= 0.0)
+
+
+def test_module_reads_reference_table_sections(runtime):
+    """The reference's table inputs in the sample -- mass_function/{m_h, z,
+    dndlnmh} on the axis ln(m_h (omega_M - omega_nu)) (hmf_t.cuh:10-20, 44-52; keys
+    are case-insensitive, so omega_M is omega_m) and
+    distances/{z, d_a} (dv_do_dz_t.cuh:13-17) -- replace the device-built
+    tables: fed the device's own tables the module reproduces the synthetic
+    run, and with cluster_abundance/hmf_s, hmf_q it meets the oracle's golden
+    values of the HMF_t amplitude factor."""
+    import oracle as O
+    L = "NumberCounts5DIntegrand"
+    p, s = y3.default_params(), y3.spec_wide()
+    runtime.tables_build(p, s)
+    hmf, da, _ = runtime.tables_download(0, s)
+    lnm = np.linspace(s.lnm_lo, s.lnm_hi, s.n_lnm)
+    zt = np.linspace(s.z_lo, s.z_hi, s.n_z)
+    zda = np.linspace(0.0, s.da_zmax, s.n_da)
+    bins = [(0, 0), (1, 1), (3, 2)]
+    cfg = config(L, "pagani", 1e-3, False, count5d_volumes(bins))
+    synth = run(L, cfg, sample())
+
+    def with_tables(**ca):
+        smp = sample()
+        smp.put("cosmological_parameters", "omega_nu", 0.0)
+        smp.put("mass_function", "m_h", np.exp(lnm) / 0.3)
+        smp.put("mass_function", "z", zt)
+        smp.put("mass_function", "dndlnmh", hmf.reshape(s.n_z, s.n_lnm))
+        smp.put("distances", "z", zda)
+        smp.put("distances", "d_a", da)
+        for k, v in ca.items():
+            smp.put("cluster_abundance", k, v)
+        return smp
+
+    tab = run(L, config(L, "pagani", 1e-3, False, count5d_volumes(bins)), with_tables())
+    np.testing.assert_allclose(tab["vals"], synth["vals"], rtol=1e-9)
+    sq = run(L, config(L, "pagani", 1e-3, False, count5d_volumes(bins)),
+             with_tables(hmf_s=0.1, hmf_q=0.9))
+    op = O.default_params(hmf_s=0.1, hmf_q=0.9)
+    ot = O.OracleTables(op, O.spec_wide())
+    counts, _ = O.golden(op, ot, O.COUNT5D, LO, ZO)
+    gold = np.array([counts[i, j] for i, j in bins])
+    check(sq, gold, 1e-3)
+    assert np.all(np.abs(sq["vals"] / synth["vals"] - 1) > 1e-3)

@@ -113,7 +113,7 @@ def test_functors_match_oracle(runtime, prec, tol):
     lt = np.exp(rng.uniform(0.5, 5.5, n))
     lo = rng.uniform(20, 300, n)
     cases = [
-        (y3.F_HMF, np.c_[lnM, zt], lambda r: O.lib.y3o_hmf(C.byref(ot.t), r[0], r[1])),
+        (y3.F_HMF, np.c_[lnM, zt], lambda r: O.lib.y3o_hmf(C.byref(op), C.byref(ot.t), r[0], r[1])),
         (y3.F_MOR, np.c_[lt, lnM, zt], lambda r: O.lib.y3o_mor_lognormal(C.byref(op), *r)),
         (y3.F_DV_DO_DZ, np.c_[zt], lambda r: O.lib.y3o_dvdz(C.byref(op), C.byref(ot.t), r[0])),
         (y3.F_SCI, np.c_[zt], lambda r: O.lib.y3o_sci(C.byref(ot.t), r[0])),

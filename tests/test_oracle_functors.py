@@ -213,3 +213,20 @@ def test_dsigma_offset_limits():
     R = 1.0
     ring = L.y3o_dsigma_mis_ring(R, 1e-4, 32.5, rho, 4.0, 1e-12)
     assert abs(ring / L.y3o_nfw_dsigma(R, 32.5, rho, 4.0) - 1) < 1e-6
+
+
+def test_hmf_amplitude_factor():
+    """HMF_t's s (lnM log10 e - 13.8124426028) + q (hmf_t.cuh:54-60): at
+    log10 M = 13.8124426028 the factor is q whatever s, and it is linear in
+    lnM with slope s log10 e."""
+    p0 = O.default_params()
+    t = O.OracleTables(p0, O.spec_wide())
+    lnm_piv = 13.8124426028 / 0.4342944819
+    z = 0.4
+    base = O.lib.y3o_hmf(C.byref(p0), C.byref(t.t), lnm_piv, z)
+    p1 = O.default_params(hmf_s=0.3, hmf_q=0.8)
+    assert O.lib.y3o_hmf(C.byref(p1), C.byref(t.t), lnm_piv, z) == pytest.approx(0.8 * base, rel=1e-12)
+    lnm = lnm_piv + 1.0
+    b1 = O.lib.y3o_hmf(C.byref(p0), C.byref(t.t), lnm, z)
+    assert O.lib.y3o_hmf(C.byref(p1), C.byref(t.t), lnm, z) == pytest.approx(
+        (0.8 + 0.3 * 0.4342944819) * b1, rel=1e-12)

@@ -26,7 +26,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "omega_m", "h", "hmf_amp", "hmf_A", "hmf_a", "hmf_c", "mor_lnl0",
         "mor_slope", "mor_zslope", "mor_sigma_ln", "f_prj", "prj_scale",
-        "f_mis", "mis_rlambda", "z_source", "sigz0")]
+        "f_mis", "mis_rlambda", "z_source", "sigz0", "hmf_s", "hmf_q")]
 
 
 class TableSpec(C.Structure):

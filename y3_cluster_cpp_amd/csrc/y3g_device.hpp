@@ -251,8 +251,8 @@ Y3G_HD T lerp_clamp(const T* ys, int n, double x_off, double inv_dx)
   return ys[i] + t * (ys[i + 1] - ys[i]);
 }
 
-// HMF_t::operator() (hmf_t.cuh:54-60) with s = 0, q = 1: clamped bilinear
-// lookup of the (n_z, n_lnM) table, x-fastest.
+// HMF_t::operator() (hmf_t.cuh:54-60): clamped bilinear lookup of the
+// (n_z, n_lnM) table, x-fastest, times s (lnM log10 e - 13.81) + q.
 template <typename T>
 Y3G_HD T hmf(const PointDev& P, const T* tab, double lnM, double zt)
 {
@@ -268,7 +268,8 @@ Y3G_HD T hmf(const PointDev& P, const T* tab, double lnM, double zt)
   const T* r1 = r0 + P.nm;
   T const a = r0[0] + t * (r0[1] - r0[0]);
   T const b = r1[0] + t * (r1[1] - r1[0]);
-  return a + s * (b - a);
+  return (a + s * (b - a)) *
+         T(P.p.hmf_s * (lnM * 0.4342944819 - 13.8124426028) + P.p.hmf_q);
 }
 
 // DV_DO_DZ_t::operator() (dv_do_dz_t.cuh:19-27).

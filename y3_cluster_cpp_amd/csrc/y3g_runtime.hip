@@ -133,6 +133,8 @@ void y3g_default_params(y3g_params* p)
   p->mis_rlambda = 0.2;
   p->z_source = 1.2;
   p->sigz0 = 0.01;
+  p->hmf_s = 0.0;
+  p->hmf_q = 1.0;
 }
 
 void y3g_spec_config0(y3g_table_spec* s)

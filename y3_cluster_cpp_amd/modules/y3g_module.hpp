@@ -492,10 +492,22 @@ namespace y3g_cosmosis {
       if (timing_) record_timestamp(*timing_, "enter");
       integrand_.set_sample(sample);
       y3g_params const& p = integrand_.params();
-      if (!have_tables_ || std::memcmp(&p, &built_, sizeof(p)) != 0) {
-        if (y3g_tables_build(device_.get(), &p, &cfg_.spec, 1) != Y3G_OK)
+      y3g_table_spec const& sp = integrand_.spec();
+      if (!have_tables_ || std::memcmp(&p, &built_, sizeof(p)) != 0 ||
+          std::memcmp(&sp, &built_spec_, sizeof(sp)) != 0 ||
+          integrand_.hmf_table() != built_hmf_ || integrand_.da_table() != built_da_) {
+        if (y3g_tables_build(device_.get(), &p, &sp, 1) != Y3G_OK)
           throw std::runtime_error(std::string("table build: ") + y3g_last_error());
+        auto const& h = integrand_.hmf_table();
+        auto const& d = integrand_.da_table();
+        if ((!h.empty() || !d.empty()) &&
+            y3g_tables_upload(device_.get(), 0, h.empty() ? nullptr : h.data(),
+                              d.empty() ? nullptr : d.data(), nullptr) != Y3G_OK)
+          throw std::runtime_error(std::string("table upload: ") + y3g_last_error());
         built_ = p;
+        built_spec_ = sp;
+        built_hmf_ = h;
+        built_da_ = d;
         have_tables_ = true;
       }
       std::vector<y3g_integral> igs = build_integrals(integrand_, cfg_);
@@ -516,15 +528,26 @@ namespace y3g_cosmosis {
     DeviceBinding device_;
     I integrand_;
     y3g_params built_{};
+    y3g_table_spec built_spec_{};
+    std::vector<double> built_hmf_, built_da_;
     bool have_tables_ = false;
     std::optional<std::ofstream> timing_;
   };
 
   // ------------------------------------------------ the integrands' common part
 
-  // Per-sample physics of the synthetic integrands: omega_m (and h0) from
-  // cosmological_parameters, the rest from the optional y3g_synthetic section
-  // (defaults y3g_default_params).
+  // Per-sample inputs of the integrands.
+  //   physics   omega_m (and h0) from cosmological_parameters, the rest from
+  //             the optional y3g_synthetic section (defaults
+  //             y3g_default_params); hmf_s / hmf_q from cluster_abundance
+  //             when present (HMF_t, hmf_t.cuh:44-52);
+  //   tables    built on the device from the physics, unless the sample
+  //             carries the reference's table sections:
+  //               mass_function/{m_h, z, dndlnmh} -> the HMF table on the
+  //                 axis ln(m_h (omega_M - omega_nu)) (hmf_t.cuh:10-20, 44-52);
+  //               distances/{z, d_a} -> the D_A table (dv_do_dz_t.cuh:13-17).
+  //             Their grids must be uniform (z from 0 for distances); the
+  //             values are loaded as they are.
   class SyntheticSample {
   public:
     void
@@ -532,7 +555,7 @@ namespace y3g_cosmosis {
     {
       y3g_default_params(&p_);
       std::string const C = "cosmological_parameters", S = "y3g_synthetic";
-      p_.omega_m = view_as_double(sample, C, "omega_m");
+      p_.omega_m = view_as_double(sample, C, "omega_m");  // = omega_M (case-insensitive)
       p_.h = optional_double(sample, C, "h0", p_.h);
       struct {
         char const* key;
@@ -546,9 +569,19 @@ namespace y3g_cosmosis {
         {"f_mis", &p_.f_mis},             {"mis_rlambda", &p_.mis_rlambda},
         {"z_source", &p_.z_source},       {"sigz0", &p_.sigz0}};
       for (auto const& k : keys) *k.dst = optional_double(sample, S, k.key, *k.dst);
+      p_.hmf_s = optional_double(sample, "cluster_abundance", "hmf_s", p_.hmf_s);
+      p_.hmf_q = optional_double(sample, "cluster_abundance", "hmf_q", p_.hmf_q);
+      spec_ = base_spec_;
+      hmf_tab_.clear();
+      da_tab_.clear();
+      if (sample.has_val("mass_function", "dndlnmh")) read_hmf_table_(sample);
+      if (sample.has_val("distances", "d_a")) read_da_table_(sample);
     }
-    void set_spec(y3g_table_spec const& s) { spec_ = s; }
+    void set_spec(y3g_table_spec const& s) { base_spec_ = spec_ = s; }
     y3g_params const& params() const { return p_; }
+    y3g_table_spec const& spec() const { return spec_; }
+    std::vector<double> const& hmf_table() const { return hmf_tab_; }
+    std::vector<double> const& da_table() const { return da_tab_; }
 
   protected:
     y3g_integral
@@ -561,7 +594,61 @@ namespace y3g_cosmosis {
       return ig;
     }
     y3g_params p_{};
-    y3g_table_spec spec_{};
+    y3g_table_spec spec_{}, base_spec_{};
+    std::vector<double> hmf_tab_, da_tab_;
+
+  private:
+    static void
+    require_uniform_(std::vector<double> const& x, std::string const& what)
+    {
+      if (x.size() < 2) throw std::runtime_error(what + " needs at least two points");
+      double const step = (x.back() - x.front()) / double(x.size() - 1);
+      if (!(step > 0.0)) throw std::runtime_error(what + " must increase");
+      for (std::size_t i = 1; i < x.size(); ++i)
+        if (std::fabs((x[i] - x[i - 1]) - step) > 1e-6 * step)
+          throw std::runtime_error(what + " is not uniformly spaced; the device "
+                                          "tables need uniform grids");
+    }
+
+    void
+    read_hmf_table_(cosmosis::DataBlock& sample)
+    {
+      std::string const C = "cosmological_parameters", F = "mass_function";
+      auto const mh = get_vector_double(sample, F, "m_h");
+      auto const z = get_vector_double(sample, F, "z");
+      auto const nd = sample.view<cosmosis::ndarray<double>>(F, "dndlnmh");
+      double const om = p_.omega_m;
+      double const onu = optional_double(sample, C, "omega_nu", 0.0);
+      std::vector<double> x(mh.size());
+      for (std::size_t i = 0; i < mh.size(); ++i) x[i] = std::log(mh[i] * (om - onu));
+      require_uniform_(x, "mass_function ln(m_h (omega_M - omega_nu))");
+      require_uniform_(z, "mass_function/z");
+      if (nd.ndims() != 2 || nd.extents()[0] != z.size() || nd.extents()[1] != mh.size())
+        throw std::runtime_error("mass_function/dndlnmh must have shape (n_z, n_m)");
+      if (mh.size() * z.size() > 6400)
+        throw std::runtime_error("mass_function table larger than 6400 entries");
+      spec_.lnm_lo = x.front();
+      spec_.lnm_hi = x.back();
+      spec_.n_lnm = int(x.size());
+      spec_.z_lo = z.front();
+      spec_.z_hi = z.back();
+      spec_.n_z = int(z.size());
+      hmf_tab_.assign(nd.begin(), nd.end());
+    }
+
+    void
+    read_da_table_(cosmosis::DataBlock& sample)
+    {
+      auto const z = get_vector_double(sample, "distances", "z");
+      auto const da = get_vector_double(sample, "distances", "d_a");
+      require_uniform_(z, "distances/z");
+      if (std::fabs(z.front()) > 1e-12 || da.size() != z.size() || z.size() > 512)
+        throw std::runtime_error("distances/{z, d_a} must start at z = 0, match in "
+                                 "length and hold at most 512 points");
+      spec_.da_zmax = z.back();
+      spec_.n_da = int(z.size());
+      da_tab_ = da;
+    }
   };
 
 } // namespace y3g_cosmosis
