@@ -86,9 +86,19 @@ def sample_indices(n_points, n, seed=SEED + 3):
     return out
 
 
-def cpu_baseline(pts, eps, sample, max_regions, threads):
+def band_limit(kind_ndim, eps, bands):
+    """The per-integral call limit the tolerance bands give (None: none)."""
+    for e, r in bands or ():
+        if eps <= e:
+            nn = 1 + 4 * kind_ndim + 2 * kind_ndim * (kind_ndim - 1) + (1 << kind_ndim)
+            return 2 * int(r) * nn
+    return None
+
+
+def cpu_baseline(pts, eps, sample, max_regions, threads, bands=None):
     """The oracle's FP64 integrands under its CPU restatement of the same
-    cubature, one integral per OpenMP thread, on `sample`."""
+    cubature (same region budget and tolerance-band call limits), one
+    integral per OpenMP thread, on `sample`."""
     import oracle as O
     ks = sorted(set(k for k, _ in sample))
     pidx = {k: i for i, k in enumerate(ks)}
@@ -115,17 +125,20 @@ def cpu_baseline(pts, eps, sample, max_regions, threads):
     sa = (O.TableSpec * len(params))(*([spec] * len(params)))
     out = (O.Result * n)()
     e = np.ascontiguousarray(el)
+    lim = [band_limit(g.ndim, x, bands) for g, x in zip(igs, el)]
+    me = (C.c_longlong * n)(*[v if v is not None else 1 << 40 for v in lim])
     t0 = time.perf_counter()
-    used = O.lib.y3o_batch_integrate(pa, sa, len(params), arr, (C.c_int * n)(*pt), O.ptr(e), n,
-                                     0, C.byref(O.cub_opts(max_regions=max_regions,
-                                                           max_eval=1 << 40)),
+    used = O.lib.y3o_batch_integrate(pa, sa, len(params), arr, (C.c_int * n)(*pt), O.ptr(e),
+                                     me, n, 0,
+                                     C.byref(O.cub_opts(max_regions=max_regions,
+                                                        max_eval=1 << 40)),
                                      C.byref(O.vegas_opts()), threads, out)
     dt = time.perf_counter() - t0
     res = np.array([(r.value, r.error) for r in out], dtype=[("value", "f8"), ("error", "f8")])
     return res, used, dt
 
 
-def serving_run(y3, pts, ref, eps_all, rates, seconds, max_regions, lanes):
+def serving_run(y3, pts, ref, eps_all, rates, seconds, opts, lanes):
     """Poisson arrivals of mock-MCMC requests at each rate; latency from the
     server's own clock (submission to completion)."""
     spec = y3.spec_wide()
@@ -140,7 +153,7 @@ def serving_run(y3, pts, ref, eps_all, rates, seconds, max_regions, lanes):
         return cache[key]
 
     o = y3.server_options(lanes=lanes, cache_entries=0)
-    o.options.max_regions = max_regions
+    o.options = opts
     srv = y3.Server(0, o)
     p, arr = integrals(0, True)  # first-launch costs outside the measurement
     srv.wait(srv.submit(p, arr))
@@ -158,25 +171,27 @@ def serving_run(y3, pts, ref, eps_all, rates, seconds, max_regions, lanes):
                 time.sleep(dt)
             p, arr = integrals(k, full)
             sent.append((k, len(arr), srv.submit(p, arr)))
-        lat, n_int, n_ok = [], 0, 0
+        lat, n_int, n_ok, n_in = [], 0, 0, 0
         for k, m, t in sent:
             res, lt = srv.wait(t)
             lat.append(lt)
-            ok, _ = count_ok(res, ref[k, :m], eps_all[k, :m])
+            ok, rel = count_ok(res, ref[k, :m], eps_all[k, :m])
             n_int += m
             n_ok += int(ok.sum())
+            n_in += int((rel <= eps_all[k, :m]).sum())
         wall = time.perf_counter() - t0
         lat = np.array(lat) * 1e3
         out.append({"rate_req_per_s": rate, "requests": n, "integrals": n_int,
                     "p50_ms": float(np.percentile(lat, 50)),
                     "p99_ms": float(np.percentile(lat, 99)),
                     "integrals_per_s": n_int / wall, "counted_per_s": n_ok / wall,
-                    "counted_frac": n_ok / n_int, "wall_s": wall})
+                    "counted_frac": n_ok / n_int, "within_eps_frac": n_in / n_int,
+                    "wall_s": wall})
         print(f"serving: rate {rate}/s, {n} requests, p50 {out[-1]['p50_ms']:.1f} ms, "
               f"p99 {out[-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     st = srv.stats()
     srv.close()
-    return {"rates": out, "lanes": st["lanes"], "max_regions": max_regions, "cache": "off",
+    return {"rates": out, "lanes": st["lanes"], "max_regions": opts.max_regions, "cache": "off",
             "requests": "mock random-walk MCMC over the first 64 prior points, 30% counts + "
                         "Delta-Sigma (192 integrals), 70% counts (12), eps_rel as streamed "
                         "(log-uniform [1e-5, 1e-3]); counted with the headline rule"}
@@ -189,8 +204,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points-per-step", type=int, default=16)
     ap.add_argument("--eps-cap", type=float, default=1e-4)
-    ap.add_argument("--max-regions", type=int, default=DEFAULT_REGIONS,
-                    help="per-integral region budget of the cubature")
+    ap.add_argument("--max-regions", type=int, default=0,
+                    help="per-integral region budget of the cubature (0: the tuned "
+                         "bands when y3_cluster_cpp_amd/tuning/gfx950.json exists, "
+                         "else %d)" % DEFAULT_REGIONS)
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ignore the autotuner's file (fixed budget for every integral)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -231,7 +250,16 @@ def main():
     params_all = [point_params(y3, pt) for pt in pts]
     rt = y3.Runtime(0)
     info = rt.device_info()
-    opts = y3.options(precision=y3.FP32SPLIT, max_regions=a.max_regions)
+    from y3_cluster_cpp_amd import autotune
+    tuned = None if (a.no_tuning or a.max_regions > 0) else autotune.load()
+    if tuned:
+        topts = dict(tuned)
+        tuning = {"file": "y3_cluster_cpp_amd/tuning/gfx950.json", **topts}
+    else:
+        topts = {"max_regions": a.max_regions or DEFAULT_REGIONS}
+        tuning = None
+    opts = y3.options(precision=y3.FP32SPLIT, **topts)
+    bands = topts.get("bands")
 
     def batch(step):
         ks = range(step * B, (step + 1) * B)
@@ -288,7 +316,7 @@ def main():
         sub = (y3.Integral * len(sub_idx))(*[arr[k * N_PER_POINT + i] for k, i in sub_idx])
         sref = np.array([ref[k, i] for k, i in sub_idx])
         seps = np.array([eps[k, i] for k, i in sub_idx])
-        popts = y3.options(precision=y3.FP64, single=1, max_regions=a.max_regions)
+        popts = y3.options(precision=y3.FP64, single=1, **topts)
         tp = time.perf_counter()
         rp = rt.integrate(sub, popts)
         dtp = time.perf_counter() - tp
@@ -311,7 +339,7 @@ def main():
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         sub_idx = sample[: a.cpu_sample]
-        res, used, dt = cpu_baseline(pts, eps, sub_idx, a.max_regions, threads)
+        res, used, dt = cpu_baseline(pts, eps, sub_idx, opts.max_regions, threads, bands)
         okc, _ = count_ok(res, np.array([ref[k, i] for k, i in sub_idx]),
                           np.array([eps[k, i] for k, i in sub_idx]))
         cpu = {"value": int(okc.sum()) / dt, "unit": "integrals/s", "cores": int(used),
@@ -327,7 +355,7 @@ def main():
     serving = None
     if not a.no_serve:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
-        serving = serving_run(y3, pts, ref, eps_raw, rates, a.serve_seconds, a.max_regions,
+        serving = serving_run(y3, pts, ref, eps_raw, rates, a.serve_seconds, opts,
                               a.serve_lanes)
 
     achieved = flops / (kern_ms * 1e-3) if kern_ms > 0 else 0.0
@@ -349,7 +377,8 @@ def main():
                    "points_per_step": B, "integrals_per_step": n_int // a.steps,
                    "eps_rel": "log-uniform [1e-5,1e-3] capped at %g" % a.eps_cap,
                    "integrator": "Genz-Malik cubature, workgroup per integral",
-                   "max_regions": a.max_regions,
+                   "max_regions": opts.max_regions,
+                   "tuning": tuning,
                    "counted": counted, "integrals": n_int,
                    "converged": int((status_all == 0).sum()),
                    "max_rel_err": float(rel_all.max()),

@@ -136,6 +136,14 @@ typedef struct {
                              sequence (the plain path)                   */
   double err_c, err_smin; /* region error |R7-R5| * clamp(c r, smin, 1),
                              r = |R7-R5| / |R5-R3|; c = 0 keeps |R7-R5|  */
+  int n_bands;            /* tolerance bands (0: every integral gets
+                             max_regions): an integral whose eps_rel <=
+                             band_eps[k] (first such k, ascending edges)
+                             gets a budget of band_regions[k] regions
+                             (per-integral call limit 2 x regions x nodes;
+                             cubature); from an autotuner                */
+  double band_eps[8];
+  int band_regions[8];
   double tl_c, tl_smin;   /* two-level estimate (batched cubature): the
                              children of a split region report |R7-R5| x
                              clamp(tl_c d / e, tl_smin, 1), d = |R7(A) +

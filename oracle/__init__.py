@@ -144,7 +144,8 @@ def _load():
         "y3o_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "y3o_test_fn": (_D, [C.c_int, _PD, C.c_int]),
         "y3o_batch_integrate": (C.c_int, [C.POINTER(Params), C.POINTER(TableSpec), C.c_int,
-                                          C.POINTER(Integral), C.POINTER(C.c_int), _PD, C.c_int,
+                                          C.POINTER(Integral), C.POINTER(C.c_int), _PD,
+                                          C.POINTER(C.c_longlong), C.c_int,
                                           C.c_int, C.POINTER(CubOpts), C.POINTER(VegasOpts),
                                           C.c_int, C.POINTER(Result)]),
         "y3o_batch_golden": (C.c_int, [C.POINTER(Params), C.POINTER(TableSpec), C.c_int, C.c_int,

@@ -13,12 +13,12 @@
 #include <stdlib.h>
 
 /* Integrate n integrals; integral i uses params[point[i]] and
- * tables[point[i]].  algo 0 = cubature, 1 = VEGAS.  Returns the thread count
- * used. */
+ * tables[point[i]], eps_rel[i] and (cubature) max_eval[i] when those arrays
+ * are given.  algo 0 = cubature, 1 = VEGAS.  Returns the thread count used. */
 int
 y3o_batch_integrate(const y3o_params* params, const y3o_table_spec* specs,
                     int n_points, const y3o_integral* igs, const int* point,
-                    const double* eps_rel, int n, int algo,
+                    const double* eps_rel, const long long* max_eval, int n, int algo,
                     const y3o_cub_opts* copt, const y3o_vegas_opts* vopt,
                     int nthreads, y3o_result* out)
 {
@@ -37,6 +37,7 @@ y3o_batch_integrate(const y3o_params* params, const y3o_table_spec* specs,
       if (algo == 0) {
         y3o_cub_opts o = *copt;
         if (eps_rel) o.eps_rel = eps_rel[i];
+        if (max_eval) o.max_eval = max_eval[i];
         y3o_cubature(&params[k], &t[k], &igs[i], &o, &out[i]);
       } else {
         y3o_vegas_opts o = *vopt;

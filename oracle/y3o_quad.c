@@ -838,7 +838,10 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
       status = 0;
       break;
     }
-    int const room = (kmax < cap - n) ? kmax : cap - n;
+    /* at most kmax splits, within the capacity and the call limit */
+    long long const left = (o->max_eval - neval) / (2LL * nn);
+    int room = (kmax < cap - n) ? kmax : cap - n;
+    if (left < room) room = (int)left;
     if (room <= 0) break;
     int eE, hc[64], B, lim0, lim1;
     unsigned long long hq[64];

@@ -335,3 +335,26 @@ def test_large_batch_independence(runtime):
     head = big[:64 * 192]
     err = np.abs(head["value"] - ref.reshape(-1))
     assert np.all(head["error"] >= err)
+
+
+def test_tolerance_bands(runtime):
+    """Tolerance-band budgets: an integral gets the budget of the first band
+    whose edge is >= its eps_rel (per-integral call limit 2 x regions x
+    nodes), the same integral outside every band the whole max_regions; the
+    plain path honours the same limits; malformed bands are refused."""
+    p, s = fid(runtime)
+    igs = [y3.make_integral(p, s, y3.DSIGMA6D, LO[0], LO[1], ZO[0], ZO[1], RADII[5],
+                            eps_rel=e) for e in (1e-9, 5e-5, 5e-4)]
+    bands = [(1e-8, 8192), (1e-4, 2048)]
+    r = runtime.integrate(igs, y3.options(bands=bands, max_regions=16384))
+    nn = 149
+    assert r["neval"][0] <= 2 * 8192 * nn and r["nregions"][0] > 4096
+    assert r["neval"][1] <= 2 * 2048 * nn and r["nregions"][1] <= 2048 + 64
+    assert r["nregions"][2] > r["nregions"][1] or r["status"][2] == 0  # outside: max_regions
+    rp = runtime.integrate(igs[:2], y3.options(bands=bands, max_regions=16384, single=1,
+                                               precision=y3.FP64))
+    assert rp["neval"][0] <= 2 * 8192 * nn and rp["neval"][1] <= 2 * 2048 * nn
+    with pytest.raises(RuntimeError, match="tolerance bands"):
+        runtime.integrate(igs, y3.options(bands=[(1e-4, 2048), (1e-5, 4096)], max_regions=16384))
+    with pytest.raises(RuntimeError, match="tolerance bands"):
+        runtime.integrate(igs, y3.options(bands=[(1e-4, 32768)], max_regions=16384))

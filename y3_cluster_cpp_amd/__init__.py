@@ -64,6 +64,8 @@ class Options(C.Structure):
                 ("vegas_itskip", C.c_int), ("vegas_nbins", C.c_int),
                 ("vegas_alpha", C.c_double), ("single", C.c_int),
                 ("err_c", C.c_double), ("err_smin", C.c_double),
+                ("n_bands", C.c_int), ("band_eps", C.c_double * 8),
+                ("band_regions", C.c_int * 8),
                 ("tl_c", C.c_double), ("tl_smin", C.c_double)]
 
 
@@ -177,11 +179,20 @@ def spec_wide():
     return s
 
 
-def options(**kw):
+def options(bands=None, **kw):
+    """y3g_options with defaults; bands = [(eps_edge, regions), ...]
+    ascending in eps_edge sets the tolerance-band budgets."""
     o = Options()
     lib().y3g_options_default(C.byref(o))
     for k, v in kw.items():
         setattr(o, k, v)
+    if bands:
+        if len(bands) > 8:
+            raise ValueError("at most 8 tolerance bands")
+        o.n_bands = len(bands)
+        for i, (e, r) in enumerate(bands):
+            o.band_eps[i] = e
+            o.band_regions[i] = int(r)
     return o
 
 

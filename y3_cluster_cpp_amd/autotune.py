@@ -1,0 +1,174 @@
+"""Autotuner of the batched cubature for the device it runs on.
+
+    python -m y3_cluster_cpp_amd.autotune [--points 48:64] [--out PATH]
+
+It tunes on held-out parameter points of the prior-box stream: points 48 to
+63 by default, which the benchmark never times. Their FP64 reference
+values come from `tests/golden/golden_points.json`. It chooses:
+
+1. **Region budgets by tolerance.**
+   - Every calibration integral is run to each candidate region budget
+     (4096 ... 65536) with its tolerance out of reach, so that it uses the
+     whole budget.
+   - One budget B* serves every tolerance <= 1e-4. It is the smallest
+     budget that meets two conditions, over the calibration integrals with
+     their eps_rel capped at 1e-4:
+     - it keeps at least 97% of the integrals that the largest budget
+       gets within tolerance;
+     - every reported error bounds the true error.
+   - Looser bands (eps_rel <= 2e-4, 5e-4, 1e-3) each get the smallest
+     budget <= B* that meets the same two conditions on their own
+     integrals.
+   - This keeps accuracy. Budgets chosen for throughput alone would leave
+     many more integrals outside their tolerance.
+2. **Launch shape.** The workgroup size (256 or 512 threads) and the
+   regions split per step (256, 512 or 1024), chosen by counted integrals
+   per second under those budgets. Shapes with any error that does not
+   bound the true error are rejected.
+
+The result is a JSON file read by `bench.py` and by `load()`. Its options
+map onto `y3g_options` (`n_bands`, `band_eps`, `band_regions`,
+`block_threads`, `max_split_per_step`, `max_regions`).
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+DEFAULT_PATH = os.path.join(HERE, "tuning", "gfx950.json")
+BUDGETS = (4096, 8192, 16384, 32768, 65536)
+LOOSE_EDGES = (2e-4, 5e-4, 1e-3)
+KEEP = 0.97  # share of the largest budget's in-tolerance integrals kept
+
+
+def load(path=DEFAULT_PATH):
+    """The tuned options as y3g_options keyword arguments, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bands": [tuple(b) for b in t["bands"]], "block_threads": t["block_threads"],
+            "max_split_per_step": t["max_split_per_step"], "max_regions": t["max_regions"]}
+
+
+def _calibration(y3, first, last):
+    from y3_cluster_cpp_amd import workloads as W
+    with open(os.path.join(ROOT, "tests", "golden", "golden_points.json")) as f:
+        g = json.load(f)
+    pts = np.array(g["points"])
+    ref = np.concatenate([np.array(g["counts"]), np.array(g["dsigma"])], axis=1)
+    eps = W.eps_stream(len(pts))
+    ks = list(range(first, last))
+    spec = y3.spec_wide()
+    params = [W.point_params(y3, pts[k]) for k in ks]
+    igs = []
+    for local, k in enumerate(ks):
+        igs += W.point_integrals(y3, params[local], spec, local, eps[k], y3.PAGANI)
+    return params, spec, igs, ref[ks].reshape(-1), eps[ks].reshape(-1)
+
+
+def tune(first=48, last=64, out=DEFAULT_PATH, log=print):
+    import y3_cluster_cpp_amd as y3
+    rt = y3.Runtime(0)
+    params, spec, igs, ref, eps = _calibration(y3, first, last)
+    rt.tables_build(params, spec)
+    n = len(igs)
+    # 1. budgets: every integral run to each budget (tolerance out of reach)
+    forced = []
+    for g in igs:
+        h = y3.Integral.from_buffer_copy(g)
+        h.eps_rel = 1e-14
+        forced.append(h)
+    arr = (y3.Integral * n)(*forced)
+    rel, honest, secs = {}, {}, {}
+    rt.integrate(arr, y3.options(max_regions=BUDGETS[0]))  # first-launch costs
+    for b in BUDGETS:
+        t0 = time.perf_counter()
+        r = rt.integrate(arr, y3.options(max_regions=b))
+        secs[b] = time.perf_counter() - t0
+        err = np.abs(r["value"] - ref)
+        rel[b] = err / np.abs(ref)
+        honest[b] = r["error"] >= err
+        log(f"budget {b:6d}: {secs[b]:.3f} s, within eps {int((rel[b] <= eps).sum())}/{n}, "
+            f"dishonest {int((~honest[b]).sum())}")
+    def pick(mask, tol, limit):
+        ok = {b: ((rel[b] <= tol) & honest[b])[mask] for b in BUDGETS}
+        frac = {b: float(ok[b].mean()) for b in BUDGETS}
+        best = max(frac[b] for b in BUDGETS if b <= limit)
+        for b in BUDGETS:
+            if b <= limit and frac[b] >= KEEP * best and honest[b][mask].all():
+                return b, frac
+        return limit, frac
+
+    tight = np.ones(n, dtype=bool)
+    bstar, frac = pick(tight, np.minimum(eps, 1e-4), BUDGETS[-1])
+    bands = [(1e-4, bstar)]
+    table = [{"eps_le": 1e-4, "integrals": n, "budget": bstar,
+              "note": "all integrals, eps_rel capped at 1e-4",
+              "counted_frac": {str(b): frac[b] for b in BUDGETS}}]
+    log(f"tolerances <= 1e-4: budget {bstar}")
+    lo = 1e-4
+    for edge in LOOSE_EDGES:
+        m = (eps > lo) & (eps <= edge)
+        lo = edge
+        if not m.any():
+            continue
+        b, frac = pick(m, eps, bstar)
+        bands.append((edge, b))
+        table.append({"eps_le": edge, "integrals": int(m.sum()), "budget": b,
+                      "counted_frac": {str(k): frac[k] for k in BUDGETS}})
+        log(f"band eps <= {edge:.1e}: {int(m.sum())} integrals, budget {b}")
+    max_regions = max(b for _, b in bands)
+    # 2. launch shape under those budgets, with the integrals' own tolerances
+    arr = (y3.Integral * n)(*igs)
+    shapes = []
+    for block in (256, 512):
+        for kmax in (256, 512, 1024):
+            o = y3.options(bands=bands, max_regions=max_regions, block_threads=block,
+                           max_split_per_step=kmax)
+            rt.integrate(arr, o)
+            t0 = time.perf_counter()
+            r = rt.integrate(arr, o)
+            dt = time.perf_counter() - t0
+            err = np.abs(r["value"] - ref)
+            ok = (err <= np.minimum(eps, 1e-4) * np.abs(ref)) & (r["error"] >= err)
+            dis = int((r["error"] < err).sum())
+            shapes.append({"block_threads": block, "max_split_per_step": kmax, "seconds": dt,
+                           "counted": int(ok.sum()), "counted_per_s": float(ok.sum() / dt),
+                           "dishonest": dis})
+            log(f"block {block} kmax {kmax}: {dt:.3f} s, counted {int(ok.sum())}, "
+                f"dishonest {dis}")
+    good = [s for s in shapes if s["dishonest"] == 0] or shapes
+    best = max(good, key=lambda s: s["counted_per_s"])
+    info = rt.device_info()
+    rt.close()
+    res = {"device": info["name"], "calibration_points": [first, last],
+           "bands": [[e, b] for e, b in bands], "max_regions": max_regions,
+           "block_threads": best["block_threads"],
+           "max_split_per_step": best["max_split_per_step"],
+           "budget_sweep": {str(b): {"seconds": secs[b],
+                                     "within_eps": int((rel[b] <= eps).sum()),
+                                     "dishonest": int((~honest[b]).sum())} for b in BUDGETS},
+           "bands_detail": table, "shapes": shapes}
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"wrote {out}")
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--points", default="48:64")
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    a = ap.parse_args()
+    first, last = (int(v) for v in a.points.split(":"))
+    tune(first, last, a.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -189,7 +189,9 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       // the largest errors), error mass per bin in 2^-40 E fixed point;
       // split bins 0..B-1 and, in index order, bin B, where B covers at
       // least kAlpha of E and at least max(64, N/8) regions, at most `room`
-      int const room = min(A.kmax, cap - N);
+      // at most kmax splits, within the slab and the call limit
+      long long const left = (S.ig.max_eval - neval) / (2LL * nn);
+      int const room = (int)min((long long)min(A.kmax, cap - N), left);
       if (room <= 0) break;
       int eE;
       (void)frexp(E, &eE);

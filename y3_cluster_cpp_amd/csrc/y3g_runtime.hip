@@ -111,6 +111,8 @@ void y3g_options_default(y3g_options* o)
   o->single = 0;
   o->err_c = 0.0;
   o->err_smin = 1.0;
+  o->n_bands = 0;
+  for (int k = 0; k < 8; ++k) o->band_eps[k] = 0.0, o->band_regions[k] = 0;
   o->tl_c = 0.0;
   o->tl_smin = 1.0 / 16.0;
 }
@@ -586,6 +588,28 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
       o.max_split_per_step > 1024 || o.vegas_nbins < 1 || o.vegas_nbins > 128 ||
       o.vegas_calls < 2 || o.vegas_itmax < 1)
     return fail(Y3G_ERR_ARG, "y3g_integrate: options out of range");
+  if (o.n_bands < 0 || o.n_bands > 8)
+    return fail(Y3G_ERR_ARG, "y3g_integrate: n_bands out of range");
+  for (int k = 0; k < o.n_bands; ++k)
+    if (o.band_regions[k] < 64 || o.band_regions[k] > o.max_regions ||
+        (k > 0 && !(o.band_eps[k] > o.band_eps[k - 1])))
+      return fail(Y3G_ERR_ARG, "y3g_integrate: tolerance bands need ascending edges and "
+                               "budgets in [64, max_regions]");
+  // tolerance-band budgets become per-integral call limits
+  std::vector<y3g_integral> banded;
+  if (o.n_bands > 0) {
+    banded.assign(igs, igs + n);
+    for (auto& g : banded) {
+      if (g.algorithm == Y3G_MCUBES) continue;
+      for (int k = 0; k < o.n_bands; ++k)
+        if (g.eps_rel <= o.band_eps[k]) {
+          long long const lim = 2LL * o.band_regions[k] * gm_num_nodes(g.ndim);
+          if (lim < g.max_eval) g.max_eval = lim;
+          break;
+        }
+    }
+    igs = banded.data();
+  }
   std::vector<int> cub, veg;
   for (size_t i = 0; i < n; ++i) {
     y3g_integral const& g = igs[i];

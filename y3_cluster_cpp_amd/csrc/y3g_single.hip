@@ -289,7 +289,8 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
       status = 0;
       break;
     }
-    int const room = std::min(kmax, cap - N);
+    long long const calls_left = (ig.max_eval - neval) / (2LL * nn);
+    int const room = (int)std::min((long long)std::min(kmax, cap - N), calls_left);
     if (room <= 0) break;
     int eE;
     (void)std::frexp(E, &eE);

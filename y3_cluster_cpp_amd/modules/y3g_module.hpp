@@ -364,6 +364,21 @@ namespace y3g_cosmosis {
     m.options.vegas_calls =
       (long long)optional_double(cfg, L, "vegas_calls", double(m.options.vegas_calls));
     m.options.vegas_itmax = int(optional_double(cfg, L, "vegas_itmax", m.options.vegas_itmax));
+    // tolerance-band budgets (the autotuner's bands): ascending eps edges
+    if (cfg.has_val(L, "band_eps") || cfg.has_val(L, "band_regions")) {
+      auto const be = get_vector_double(cfg, L, "band_eps");
+      auto const br = get_vector_double(cfg, L, "band_regions");
+      if (be.size() != br.size() || be.size() > 8)
+        throw std::runtime_error(L + "/band_eps and band_regions must have the same length "
+                                     "(at most 8)");
+      m.options.n_bands = int(be.size());
+      for (std::size_t k = 0; k < be.size(); ++k) {
+        m.options.band_eps[k] = be[k];
+        m.options.band_regions[k] = int(br[k]);
+        if (m.options.band_regions[k] > m.options.max_regions)
+          m.options.max_regions = m.options.band_regions[k];
+      }
+    }
 
     y3g_spec_wide(&m.spec);
     m.spec.lnm_lo = optional_double(cfg, L, "table_lnm_low", m.spec.lnm_lo);
