@@ -133,8 +133,6 @@ def _load():
                                 C.POINTER(CubOpts), C.POINTER(Result)]),
         "y3o_cubature_fn": (None, [_FNN, C.c_void_p, C.c_int, _PD, _PD,
                                    C.POINTER(CubOpts), C.POINTER(Result)]),
-        "y3o_cubature_fn_snap": (None, [_FNN, C.c_void_p, C.c_int, _PD, _PD, _PD, _PD,
-                                        C.POINTER(CubOpts), C.POINTER(Result)]),
         "y3o_gm_rule": (_D, [_FNN, C.c_void_p, C.c_int, _PD, _PD, _PD, _PD]),
         "y3o_gm_num_nodes": (C.c_int, [C.c_int]),
         "y3o_vegas": (None, [C.POINTER(Params), C.POINTER(Tables), C.POINTER(Integral),

@@ -195,14 +195,10 @@ typedef struct {
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,
                   y3o_result* r);
-/* Generic: integrate f over [lows, highs] (no table snapping). */
+/* Generic: integrate f over [lows, highs]. */
 void y3o_cubature_fn(double (*f)(const double*, void*), void* ctx, int ndim,
                      const double* lows, const double* highs,
                      const y3o_cub_opts* o, y3o_result* r);
-void y3o_cubature_fn_snap(double (*f)(const double*, void*), void* ctx,
-                          int ndim, const double* lows, const double* highs,
-                          const double* snap_x0, const double* snap_dx,
-                          const y3o_cub_opts* o, y3o_result* r);
 /* Genz-Malik rule of degree 7 / 5 on one region; returns R7, sets *r5 and
  * the fourth differences fourth[ndim]. */
 double y3o_gm_rule(double (*f)(const double*, void*), void* ctx, int ndim,

@@ -694,9 +694,6 @@ typedef struct {
   void* ctx;
   int d;
   double lows[8], highs[8];
-  /* table axes: axis -> (x0, dx) of a uniform knot grid, or dx = 0; the
-   * initial partition cuts these axes at every knot inside the domain */
-  double snap_x0[8], snap_dx[8];
   double err_c, err_smin;
 } cub_prob;
 
@@ -916,26 +913,6 @@ y3o_cubature_fn(double (*f)(const double*, void*), void* ctx, int ndim,
   cub_run(&P, o, r);
 }
 
-/* As y3o_cubature_fn, with split positions snapped to the uniform knot
- * grids (snap_x0[i], snap_dx[i]); snap_dx[i] = 0 disables axis i. */
-void
-y3o_cubature_fn_snap(double (*f)(const double*, void*), void* ctx, int ndim,
-                     const double* lows, const double* highs,
-                     const double* snap_x0, const double* snap_dx,
-                     const y3o_cub_opts* o, y3o_result* r)
-{
-  cub_prob P;
-  memset(&P, 0, sizeof(P));
-  P.f = f;
-  P.ctx = ctx;
-  P.d = ndim;
-  for (int i = 0; i < ndim; ++i) {
-    P.lows[i] = lows[i], P.highs[i] = highs[i];
-    P.snap_x0[i] = snap_x0[i], P.snap_dx[i] = snap_dx[i];
-  }
-  cub_run(&P, o, r);
-}
-
 typedef struct {
   const y3o_params* p;
   const y3o_tables* t;
@@ -949,22 +926,10 @@ ig_eval(const double* x, void* v)
   return y3o_integrand_mapped(c->p, c->t, c->ig, x);
 }
 
-/* table axes of each integrand: (lnM axis, zt axis) */
-static void
-integrand_axes(int kind, int* ax_lnm, int* ax_zt)
-{
-  switch (kind) {
-    case Y3O_COUNT3D: *ax_lnm = 2, *ax_zt = 1; break;
-    case Y3O_COUNT5D: *ax_lnm = 4, *ax_zt = 3; break;
-    default: *ax_lnm = 3, *ax_zt = 2; break;
-  }
-}
-
 void
 y3o_cubature(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
              const y3o_cub_opts* o, y3o_result* r)
 {
-
   y3o_params pp = *p;
   if (ig->kind == Y3O_COUNT3D) pp.f_prj = 0.0;
   ig_ctx c = {&pp, t, ig};
@@ -975,13 +940,6 @@ y3o_cubature(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
   P.d = ig->ndim;
   for (int i = 0; i < ig->ndim; ++i)
     P.lows[i] = ig->lows[i], P.highs[i] = ig->highs[i];
-  int am, az;
-  integrand_axes(ig->kind, &am, &az);
-  const y3o_table_spec* s = &t->spec;
-  P.snap_x0[am] = s->lnm_lo;
-  P.snap_dx[am] = (s->lnm_hi - s->lnm_lo) / (s->n_lnm - 1);
-  P.snap_x0[az] = s->z_lo;
-  P.snap_dx[az] = (s->z_hi - s->z_lo) / (s->n_z - 1);
   cub_run(&P, o, r);
 }
 
