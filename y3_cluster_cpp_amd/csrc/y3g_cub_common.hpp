@@ -207,7 +207,8 @@ template <typename T, int BLOCK> struct CubShared {
   PairF<T> pf[sizeof(T) == 4 ? kChunk : 1][kMaxPairs];
   double poff[kMaxPairs][2];
   unsigned char npair[kMaxNodes];
-  unsigned ncode[sizeof(T) == 4 ? kMaxNodes : 1];  // gm_code of each node
+  unsigned char nclass[kMaxNodes];                    // gm_class of each node
+  float noff[sizeof(T) == 4 ? kMaxNodes : 1][kMaxDim];  // unit offsets, float
   int npairs;
 };
 
@@ -231,7 +232,9 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
       ++np;
     }
     S.npair[j] = (unsigned char)(k < kMaxPairs ? k : kMaxPairs - 1);
-    if constexpr (sizeof(T) == 4) S.ncode[j] = gm_code(d, j);
+    S.nclass[j] = (unsigned char)gm_class(d, j);
+    if constexpr (sizeof(T) == 4)
+      for (int a = 0; a < kMaxDim; ++a) S.noff[j][a] = a < d ? float(gm_offset(d, j, a)) : 0.0f;
   }
   S.npairs = np;
 }
@@ -253,11 +256,15 @@ __device__ void eval_nodes(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   __syncthreads();
   for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
     int const r = q / nn, j = q - r * nn;
-    unsigned const code = S.ncode[j];
-    double y[d];
+    // node coordinates in float, one FMA each from the region's centre and
+    // half-width (the (zt, lnM)-dependent factors come from the pair, in FP64
+    // coordinates); for Delta-Sigma also 1 - u, accurate near u = 1
+    float yf[d];
 #pragma unroll
-    for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_decode(code, a);
-    S.f[q] = integrand_pair_k<T, K>(S.ig, S.P, S.pf[r][S.npair[j]], y);
+    for (int a = 0; a < d; ++a) yf[a] = float(S.c[r][a]) + float(S.h[r][a]) * S.noff[j][a];
+    float w = 1.0f;
+    if constexpr (K == Y3G_DSIGMA6D) w = float(1.0 - S.c[r][4]) - float(S.h[r][4]) * S.noff[j][4];
+    S.f[q] = integrand_node_f<K>(S.ig, S.P, S.pf[r][S.npair[j]], yf, w);
   }
 }
 
@@ -318,7 +325,11 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     const T* fr = S.f + r * nn;
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
     for (int j = lane; j < nn; j += 64) {
-      int const g = gm_class(d, j);
+      int g;
+      if constexpr (sizeof(T) == 4)
+        g = S.nclass[j];
+      else
+        g = gm_class(d, j);
       double const fv = double(fr[j]);
       a7 += S.w7[g] * fv;
       a5 += S.w5[g] * fv;

@@ -55,7 +55,14 @@ template <typename T> struct M;
 template <> struct M<float> {
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
   static __device__ __forceinline__ float log(float x) { return __logf(x); }
-  static __device__ __forceinline__ float log1p(float x) { return log1pf(x); }
+  // log1p through the hardware log: log(1 + x) x / ((1 + x) - 1), which
+  // cancels the rounding of 1 + x (D. Goldberg, "What every computer
+  // scientist should know about floating-point arithmetic", 1991, Thm 4)
+  static __device__ __forceinline__ float log1p(float x)
+  {
+    float const p = 1.0f + x;
+    return p == 1.0f ? x : __logf(p) * x * __builtin_amdgcn_rcpf(p - 1.0f);
+  }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float erf(float x) { return erff(x); }
   static __device__ __forceinline__ float erfc(float x) { return erfcf(x); }
@@ -466,7 +473,6 @@ template <typename T> struct PairF {
   T zfac;              // hmf x dV/dz [x zo bin x Sigma_crit^-1 x angular norm]
   T rs, amp;           // NFW halo (Delta-Sigma)
   T cds;               // (1 - f_mis) x centred Delta-Sigma(R)
-  T pad;
 };
 
 template <typename T>
@@ -494,7 +500,7 @@ Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   f.tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
   f.thi = thi < T(kTMax) ? thi : T(kTMax);
   T zf = hmf<T>(P, tb.hmf, lnM, zt) * dv_do_dz<T>(P, tb.da, zt);
-  f.rs = f.amp = f.cds = f.pad = T(0);
+  f.rs = f.amp = f.cds = T(0);
   if (ig.kind == Y3G_DSIGMA6D) {
     T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
     // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
@@ -539,6 +545,40 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
     H.amp = f.amp;
     T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, T(y[5]), H);
     return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) * prof;
+  }
+}
+
+// FP32 node form of integrand_pair_k: the node's coordinates in float (yf)
+// and, for Delta-Sigma, 1 - u in float (w) next to u (yf[4]); each integrand
+// variable then costs one FMA from its region's centre and half-width.
+template <int K>
+Y3G_HD float integrand_node_f(const y3g_integral& ig, const PointDev& P,
+                              const PairF<float>& f, const float* yf, float w)
+{
+  using T = float;
+  if (!(f.thi > f.tlo)) return T(0);
+  constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
+  T const sln = T(P.p.mor_sigma_ln);
+  T const tt = f.tlo + (f.thi - f.tlo) * yf[it];
+  T const lt = M<T>::exp(f.lnlbar + sln * tt);
+  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
+  T const common = f.zfac * phi;
+  T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
+  if constexpr (K == Y3G_COUNT3D) {
+    return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+  } else if constexpr (K == Y3G_COUNT5D) {
+    return common * lo_lt_pdf<T>(yf[0], lt, fprj, pscale) *
+           zo_pdf<T>(yf[1], yf[iz], T(P.p.sigz0));
+  } else {
+    T const s = T(ig.sigma_mis);
+    T const u = yf[4];
+    T const l1mu = u < T(0.5) ? M<T>::log1p(-u) : M<T>::log(w);
+    T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
+    NfwHalo<T> H;
+    H.rs = f.rs;
+    H.amp = f.amp;
+    T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, yf[5], H);
+    return common * lo_lt_pdf<T>(yf[0], lt, fprj, pscale) * prof;
   }
 }
 
