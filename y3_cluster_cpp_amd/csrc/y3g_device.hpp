@@ -175,6 +175,17 @@ template <typename T> Y3G_HD T zo_bin(T zlo, T zhi, T zt, T sigz0)
 // fmsk = 0; emg_des_t.cuh:172-193), overflow-free through erfcx.
 template <typename T> Y3G_HD T emg_pdf(T x, T mu, T sigma, T tau)
 {
+  if constexpr (sizeof(T) == 4) {
+    // one erfcx for both signs of B (no divergent branch): with
+    // E = exp(-d^2/2) erfcx(|B|) = exp(A) erfc(|B|), the pdf is tau/2 E for
+    // B >= 0 and tau/2 (2 exp(A) - E) = tau/2 exp(A) erfc(B) for B < 0
+    T const B = M<T>::div(mu + tau * sigma * sigma - x, T(1.4142135623730951) * sigma);
+    T const d = M<T>::div(x - mu, sigma);
+    T const E = M<T>::exp(T(-0.5) * d * d) * M<T>::erfcx(B < T(0) ? -B : B);
+    if (B >= T(0)) return T(0.5) * tau * E;
+    T const A = tau * (mu - x) + T(0.5) * tau * tau * sigma * sigma;
+    return T(0.5) * tau * (T(2) * M<T>::exp(A) - E);
+  }
   T const B = M<T>::div(mu + tau * sigma * sigma - x, T(1.4142135623730951) * sigma);
   if (B >= T(0)) {
     T const d = M<T>::div(x - mu, sigma);
@@ -319,9 +330,47 @@ template <typename T> Y3G_HD void nfw_F_h(T x, T& F, T& h)
   h = M<T>::div(T(1) - F, t);
 }
 
+// FP32 form of nfw_g below with the work its branches share done once:
+// ln x, |x^2 - 1| and its square root; ln((1 + s) / x) = ln(1 + s) - ln x.
+Y3G_HD float nfw_g_f32(float x)
+{
+  using T = float;
+  T const x2 = x * x, t = x2 - T(1);
+  T const at = t < T(0) ? -t : t;
+  T const s = M<T>::sqrt(at);
+  T const lx = M<T>::log(x);
+  T F, h;
+  if (at < T(0.1)) {
+    T hs = T(0), tp = T(1);
+#pragma unroll
+    for (int k = 1; k <= 9; ++k) {
+      T const term = tp * T(1.0 / (2 * k + 1));
+      hs += (k & 1) ? term : -term;
+      tp *= t;
+    }
+    h = hs;
+    F = T(1) - t * hs;
+  } else {
+    T const is = M<T>::div(T(1), s);
+    F = (t < T(0) ? M<T>::log(T(1) + s) - lx : M<T>::atan(s)) * is;
+    h = M<T>::div(T(1) - F, t);
+  }
+  T m;
+  if (x < T(0.9)) {
+    T const i1s = M<T>::div(T(1), T(1) + s);
+    T const q = T(0.5) * x2 * i1s;
+    T const phi = M<T>::div(-M<T>::log1p(-q), q);
+    m = ((T(0.69314718055994531) - lx) * i1s - T(0.5) * phi * i1s) * M<T>::div(T(1), s);
+  } else {
+    m = M<T>::div(lx - T(0.69314718055994531) + F, x2);
+  }
+  return T(4) * m - T(2) * h;
+}
+
 // Delta-Sigma / (rho_s r_s) of NFW (Wright & Brainerd 2000).
 template <typename T> Y3G_HD T nfw_g(T x)
 {
+  if constexpr (sizeof(T) == 4) return nfw_g_f32(x);
   T F, h, m;
   nfw_F_h<T>(x, F, h);
   if (x < T(0.9)) {
