@@ -24,6 +24,33 @@ __device__ __forceinline__ int wave_sum_i(int v)
   return v;
 }
 
+// Reductions within each row of 16 lanes through DPP lane permutations
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror):
+// after the four steps every lane of a row holds the row's sum / maximum.
+// No LDS round trip; the whole wave must be active.
+template <int CTRL> __device__ __forceinline__ double dpp_row(double v)
+{
+  int const lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  int const hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row16_sum(double v)
+{
+  v += dpp_row<0xB1>(v);
+  v += dpp_row<0x4E>(v);
+  v += dpp_row<0x141>(v);
+  v += dpp_row<0x140>(v);
+  return v;
+}
+__device__ __forceinline__ double row16_max(double v)
+{
+  v = fmax(v, dpp_row<0xB1>(v));
+  v = fmax(v, dpp_row<0x4E>(v));
+  v = fmax(v, dpp_row<0x141>(v));
+  v = fmax(v, dpp_row<0x140>(v));
+  return v;
+}
+
 template <int BLOCK>
 __device__ double block_sum(double v, double* red)
 {
@@ -319,46 +346,51 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     }
   }
   __syncthreads();
-  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  constexpr int NW = BLOCK / 64;
-  for (int r = w; r < m; r += NW) {
-    const T* fr = S.f + r * nn;
+  // rule sums: a row of 16 lanes per region (four regions per wave), partial
+  // sums over nodes j = sub, sub + 16, ... reduced within the row by DPP
+  constexpr int NG = BLOCK / 16;
+  int const sub = threadIdx.x & 15;
+  for (int r0 = 0; r0 < m; r0 += NG) {
+    int const r = r0 + (threadIdx.x >> 4);
+    bool const live = r < m;
+    const T* fr = S.f + (live ? r : 0) * nn;
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
-    for (int j = lane; j < nn; j += 64) {
-      int g;
-      if constexpr (sizeof(T) == 4)
-        g = S.nclass[j];
-      else
-        g = gm_class(d, j);
-      double const fv = double(fr[j]);
-      a7 += S.w7[g] * fv;
-      a5 += S.w5[g] * fv;
-      a3 += S.w3[g] * fv;
+    if (live) {
+      for (int j = sub; j < nn; j += 16) {
+        int g;
+        if constexpr (sizeof(T) == 4)
+          g = S.nclass[j];
+        else
+          g = gm_class(d, j);
+        double const fv = double(fr[j]);
+        a7 += S.w7[g] * fv;
+        a5 += S.w5[g] * fv;
+        a3 += S.w3[g] * fv;
+      }
     }
-    a7 = wave_sum(a7);
-    a5 = wave_sum(a5);
-    a3 = wave_sum(a3);
-    // fourth differences, one lane per axis
+    a7 = row16_sum(a7);
+    a5 = row16_sum(a5);
+    a3 = row16_sum(a3);
+    // fourth differences, one lane per axis; split axis = the lowest axis
+    // holding the largest
     double four = -1.0;
-    if (lane < d) {
+    if (live && sub < d) {
       double const f0 = double(fr[0]);
-      double const f2 = double(fr[1 + 2 * lane]) + double(fr[2 + 2 * lane]);
+      double const f2 = double(fr[1 + 2 * sub]) + double(fr[2 + 2 * sub]);
       double const f3 =
-        double(fr[1 + 2 * d + 2 * lane]) + double(fr[2 + 2 * d + 2 * lane]);
+        double(fr[1 + 2 * d + 2 * sub]) + double(fr[2 + 2 * d + 2 * sub]);
       four = fabs(f2 - 2.0 * f0 - (f3 - 2.0 * f0) / 7.0);
     }
-    // argmax over lanes 0..d-1 (ties -> lowest axis)
-    int best = 0;
-    double bv = __shfl(four, 0, 64);
-    for (int a = 1; a < d; ++a) {
-      double const v = __shfl(four, a, 64);
-      if (v > bv) bv = v, best = a;
-    }
-    if (lane == 0) {
+    double const bv = row16_max(four);
+    unsigned long long const hit = __ballot(sub < d && four == bv);
+    unsigned const row = (unsigned)(hit >> (threadIdx.x & 48)) & 0xFFFFu;
+    int best = row ? __builtin_ctz(row) : 0;
+    if (live && sub == 0) {
       double vol = 1.0;
       for (int a = 0; a < d; ++a) vol *= 2.0 * S.h[r][a];
       if (!(bv > 0.0)) {
         double bw = -1.0;
+        best = 0;
         for (int a = 0; a < d; ++a) {
           double const rel = 2.0 * S.h[r][a] / (S.ig.highs[a] - S.ig.lows[a]);
           if (rel > bw) bw = rel, best = a;

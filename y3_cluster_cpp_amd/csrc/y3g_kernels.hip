@@ -538,7 +538,6 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
   int const tid = threadIdx.x;
   if (tid == 0) S.point = -1;
   unsigned long long nev_total = 0;
-  int const nb = A.vegas_nbins;
   for (;;) {
     __syncthreads();
     if (tid == 0) S.item = atomicAdd(A.queue, 1);
