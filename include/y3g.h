@@ -235,8 +235,11 @@ typedef enum {
   Y3G_F_DSIGMA_OFFSET = 11,/* (R, rmis, theta, lnM, rho, conc) tangential    */
   Y3G_F_SCI = 12,          /* (zt) of point 0   Sigma_crit^-1              */
   Y3G_F_INTEGRAND = 13,    /* (y[ndim]) mapped integrand of integral `aux` */
-  Y3G_F_INTEGRAND_PAIRED = 14 /* the same through pair_factors + integrand_pair,
+  Y3G_F_INTEGRAND_PAIRED = 14,/* the same through pair_factors + integrand_pair,
                                  the cubature's factorised form       */
+  Y3G_F_INTEGRAND_NODE = 15   /* the FP32 cubature's node form: pair factors
+                                 at (zt, lnM), the other coordinates
+                                 rounded to float (FP64: as 14)        */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

@@ -167,8 +167,9 @@ def test_mapped_integrand_matches_oracle(runtime, kind, prec, tol):
     r = np.array([O.lib.y3o_integrand_mapped(C.byref(op), C.byref(ot.t), C.byref(oig),
                                              O.ptr(np.ascontiguousarray(y))) for y in Y])
     m = np.abs(r) > 1e-6 * np.max(np.abs(r))
-    # the direct form and the cubature's (zt, lnM)-factorised form
-    for which in (y3.F_INTEGRAND, y3.F_INTEGRAND_PAIRED):
+    # the direct form, the cubature's (zt, lnM)-factorised form and its FP32
+    # node form (float coordinates)
+    for which in (y3.F_INTEGRAND, y3.F_INTEGRAND_PAIRED, y3.F_INTEGRAND_NODE):
         v = runtime.eval_functor(which, Y, precision=prec, aux=ig)
         assert np.max(np.abs(v[m] / r[m] - 1)) < tol, which
         assert np.max(np.abs(v - r)) < 1e-4 * np.max(np.abs(r)), which

@@ -266,13 +266,12 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
   S.npairs = np;
 }
 
-// FP32 node sweep of m regions for integrand kind K: factors of the
-// (zt, lnM) pairs once per region, then every node from its pair's factors.
-template <typename T, int BLOCK, int K>
-__device__ void eval_nodes(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
+// Factors of the (zt, lnM) pairs of m regions (FP32 path), one per thread.
+template <typename T, int BLOCK>
+__device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
-  constexpr int d = KindAxes<K>::nd, iz = KindAxes<K>::iz, im = KindAxes<K>::im;
-  constexpr int nn = gm_num_nodes(d);
+  int iz, im, it;
+  kind_axes<T>(S.ig.kind, it, iz, im);
   int const np = S.npairs;
   for (int q = threadIdx.x; q < m * np; q += BLOCK) {
     int const r = q / np, k = q - r * np;
@@ -280,18 +279,41 @@ __device__ void eval_nodes(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
     double const lnM = S.c[r][im] + S.h[r][im] * S.poff[k][1];
     S.pf[r][k] = pair_factors<T>(S.ig, S.P, tb, zt, lnM);
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
-    int const r = q / nn, j = q - r * nn;
-    // node coordinates in float, one FMA each from the region's centre and
-    // half-width (the (zt, lnM)-dependent factors come from the pair, in FP64
-    // coordinates); for Delta-Sigma also 1 - u, accurate near u = 1
+}
+
+// FP32 nodes of region r for integrand kind K, by the 16 lanes of a row
+// (lane sub takes nodes sub, sub + 16, ...): the values into S.f and this
+// lane's partial sums of the degree-7, -5 and -3 rules.  The region's centre
+// and half-widths stay in registers across its nodes; each node coordinate
+// is one FMA from them (the (zt, lnM)-dependent factors come from the pair,
+// in FP64 coordinates); for Delta-Sigma also 1 - u, accurate near u = 1.
+template <typename T, int BLOCK, int K>
+__device__ __forceinline__ void node_row(CubShared<T, BLOCK>& S, int r, int sub,
+                                         const NodeConstF& kc, double& a7, double& a5,
+                                         double& a3)
+{
+  constexpr int d = KindAxes<K>::nd;
+  constexpr int nn = gm_num_nodes(d);
+  float cf[d], hf[d];
+#pragma unroll
+  for (int a = 0; a < d; ++a) cf[a] = float(S.c[r][a]), hf[a] = float(S.h[r][a]);
+  float w0 = 1.0f;
+  if constexpr (K == Y3G_DSIGMA6D) w0 = float(1.0 - S.c[r][4]);
+  const PairF<T>* pfr = S.pf[r];
+  T* fr = S.f + r * nn;
+  for (int j = sub; j < nn; j += 16) {
     float yf[d];
 #pragma unroll
-    for (int a = 0; a < d; ++a) yf[a] = float(S.c[r][a]) + float(S.h[r][a]) * S.noff[j][a];
+    for (int a = 0; a < d; ++a) yf[a] = cf[a] + hf[a] * S.noff[j][a];
     float w = 1.0f;
-    if constexpr (K == Y3G_DSIGMA6D) w = float(1.0 - S.c[r][4]) - float(S.h[r][4]) * S.noff[j][4];
-    S.f[q] = integrand_node_f<K>(S.ig, S.P, S.pf[r][S.npair[j]], yf, w);
+    if constexpr (K == Y3G_DSIGMA6D) w = w0 - hf[4] * S.noff[j][4];
+    float const fv = integrand_node_f<K>(kc, pfr[S.npair[j]], yf, w);
+    fr[j] = fv;
+    int const g = S.nclass[j];
+    double const fd = fv;
+    a7 += S.w7[g] * fd;
+    a5 += S.w5[g] * fd;
+    a3 += S.w3[g] * fd;
   }
 }
 
@@ -333,11 +355,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
   int const nn = gm_num_nodes(d);
   Tabs<T> tb{S.hmf, S.da, S.sci};
   if constexpr (sizeof(T) == 4) {
-    switch (S.ig.kind) {
-      case Y3G_COUNT3D: eval_nodes<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
-      case Y3G_COUNT5D: eval_nodes<T, BLOCK, Y3G_COUNT5D>(S, m, tb); break;
-      default: eval_nodes<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
-    }
+    eval_pairs<T, BLOCK>(S, m, tb);
   } else {
     switch (S.ig.kind) {
       case Y3G_COUNT3D: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
@@ -347,7 +365,8 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
   }
   __syncthreads();
   // rule sums: a row of 16 lanes per region (four regions per wave), partial
-  // sums over nodes j = sub, sub + 16, ... reduced within the row by DPP
+  // sums over nodes j = sub, sub + 16, ... reduced within the row by DPP; in
+  // the FP32 path the row evaluates its region's nodes as it sums them
   constexpr int NG = BLOCK / 16;
   int const sub = threadIdx.x & 15;
   for (int r0 = 0; r0 < m; r0 += NG) {
@@ -355,17 +374,25 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     bool const live = r < m;
     const T* fr = S.f + (live ? r : 0) * nn;
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
-    if (live) {
-      for (int j = sub; j < nn; j += 16) {
-        int g;
-        if constexpr (sizeof(T) == 4)
-          g = S.nclass[j];
-        else
-          g = gm_class(d, j);
-        double const fv = double(fr[j]);
-        a7 += S.w7[g] * fv;
-        a5 += S.w5[g] * fv;
-        a3 += S.w3[g] * fv;
+    if constexpr (sizeof(T) == 4) {
+      if (live) {
+        NodeConstF const kc = node_const_f(S.ig, S.P);
+        switch (S.ig.kind) {
+          case Y3G_COUNT3D: node_row<T, BLOCK, Y3G_COUNT3D>(S, r, sub, kc, a7, a5, a3); break;
+          case Y3G_COUNT5D: node_row<T, BLOCK, Y3G_COUNT5D>(S, r, sub, kc, a7, a5, a3); break;
+          default: node_row<T, BLOCK, Y3G_DSIGMA6D>(S, r, sub, kc, a7, a5, a3); break;
+        }
+      }
+      __syncthreads();  // the rows' node values, for the fourth differences
+    } else {
+      if (live) {
+        for (int j = sub; j < nn; j += 16) {
+          int const g = gm_class(d, j);
+          double const fv = double(fr[j]);
+          a7 += S.w7[g] * fv;
+          a5 += S.w5[g] * fv;
+          a3 += S.w3[g] * fv;
+        }
       }
     }
     a7 = row16_sum(a7);

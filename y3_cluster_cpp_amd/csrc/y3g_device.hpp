@@ -51,17 +51,22 @@ template <typename T> struct M;
 // FP32: the hardware transcendental instructions (v_exp_f32, v_log_f32,
 // v_sqrt_f32, v_cos_f32; about 1 ulp on the arguments met here) in place of
 // the correctly-rounded library sequences -- the FP32 integrand's inputs
-// already carry errors of that size.  FP64 keeps the library functions.
+// already carry errors of that size.  The logarithm is v_log_f32 (log2)
+// times ln 2, without the denormal-input scaling of the library form (no
+// argument here is denormal).  FP64 keeps the library functions.
 template <> struct M<float> {
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
-  static __device__ __forceinline__ float log(float x) { return __logf(x); }
+  static __device__ __forceinline__ float log(float x)
+  {
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+  }
   // log1p through the hardware log: log(1 + x) x / ((1 + x) - 1), which
   // cancels the rounding of 1 + x (D. Goldberg, "What every computer
   // scientist should know about floating-point arithmetic", 1991, Thm 4)
   static __device__ __forceinline__ float log1p(float x)
   {
     float const p = 1.0f + x;
-    return p == 1.0f ? x : __logf(p) * x * __builtin_amdgcn_rcpf(p - 1.0f);
+    return p == 1.0f ? x : log(p) * x * __builtin_amdgcn_rcpf(p - 1.0f);
   }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float erf(float x) { return erff(x); }
@@ -597,37 +602,54 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
   }
 }
 
+// Per-integral constants of the FP32 node integrand, converted once per
+// chunk of regions and kept in registers across its nodes.
+struct NodeConstF {
+  float sln, fprj, pscale, sigz0, smis, fmis, radius, lo_lo, lo_hi;
+};
+Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
+{
+  NodeConstF k;
+  k.sln = float(P.p.mor_sigma_ln);
+  k.fprj = float(P.p.f_prj);
+  k.pscale = float(P.p.prj_scale);
+  k.sigz0 = float(P.p.sigz0);
+  k.smis = float(ig.sigma_mis);
+  k.fmis = float(P.p.f_mis);
+  k.radius = float(ig.radius);
+  k.lo_lo = float(ig.lo_lo);
+  k.lo_hi = float(ig.lo_hi);
+  return k;
+}
+
 // FP32 node form of integrand_pair_k: the node's coordinates in float (yf)
 // and, for Delta-Sigma, 1 - u in float (w) next to u (yf[4]); each integrand
 // variable then costs one FMA from its region's centre and half-width.
 template <int K>
-Y3G_HD float integrand_node_f(const y3g_integral& ig, const PointDev& P,
-                              const PairF<float>& f, const float* yf, float w)
+Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairF<float>& f, const float* yf,
+                              float w)
 {
   using T = float;
   if (!(f.thi > f.tlo)) return T(0);
   constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
-  T const sln = T(P.p.mor_sigma_ln);
   T const tt = f.tlo + (f.thi - f.tlo) * yf[it];
-  T const lt = M<T>::exp(f.lnlbar + sln * tt);
+  T const lt = M<T>::exp(f.lnlbar + kc.sln * tt);
   T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
   T const common = f.zfac * phi;
-  T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
   if constexpr (K == Y3G_COUNT3D) {
-    return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
+    return common * lo_lt_bin<T>(kc.lo_lo, kc.lo_hi, lt, T(0), kc.pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    return common * lo_lt_pdf<T>(yf[0], lt, fprj, pscale) *
-           zo_pdf<T>(yf[1], yf[iz], T(P.p.sigz0));
+    return common * lo_lt_pdf<T>(yf[0], lt, kc.fprj, kc.pscale) *
+           zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
   } else {
-    T const s = T(ig.sigma_mis);
     T const u = yf[4];
     T const l1mu = u < T(0.5) ? M<T>::log1p(-u) : M<T>::log(w);
-    T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
+    T const rmis = kc.smis * M<T>::sqrt(T(-2) * l1mu);
     NfwHalo<T> H;
     H.rs = f.rs;
     H.amp = f.amp;
-    T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, yf[5], H);
-    return common * lo_lt_pdf<T>(yf[0], lt, fprj, pscale) * prof;
+    T const prof = f.cds + kc.fmis * dsigma_offset<T>(kc.radius, rmis, yf[5], H);
+    return common * lo_lt_pdf<T>(yf[0], lt, kc.fprj, kc.pscale) * prof;
   }
 }
 

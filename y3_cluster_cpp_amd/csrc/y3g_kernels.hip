@@ -620,6 +620,24 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
       r = integrand_pair<T>(*F.ig, P, pair_factors<T>(*F.ig, P, tb, x[iz], x[im]), x);
       break;
     }
+    case Y3G_F_INTEGRAND_NODE: {
+      int it, iz, im;
+      kind_axes<T>(F.ig->kind, it, iz, im);
+      PairF<T> const pf = pair_factors<T>(*F.ig, P, tb, x[iz], x[im]);
+      if constexpr (sizeof(T) == 4) {
+        float yf[kMaxDim];
+        for (int a = 0; a < kMaxDim; ++a) yf[a] = a < F.ig->ndim ? float(x[a]) : 0.0f;
+        NodeConstF const kc = node_const_f(*F.ig, P);
+        switch (F.ig->kind) {
+          case Y3G_COUNT3D: r = integrand_node_f<Y3G_COUNT3D>(kc, pf, yf, 1.0f); break;
+          case Y3G_COUNT5D: r = integrand_node_f<Y3G_COUNT5D>(kc, pf, yf, 1.0f); break;
+          default: r = integrand_node_f<Y3G_DSIGMA6D>(kc, pf, yf, float(1.0 - x[4])); break;
+        }
+      } else {
+        r = integrand_pair<T>(*F.ig, P, pf, x);
+      }
+      break;
+    }
   }
   F.out[i] = double(r);
 }

@@ -726,17 +726,19 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INTEGRAND_PAIRED)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INTEGRAND_NODE)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
   if (n == 0) return 0;
   HIPCHK(hipSetDevice(rt->device));
   bool const needs_tables = which == Y3G_F_DV_DO_DZ || which == Y3G_F_HMF ||
                             which == Y3G_F_MOR || which == Y3G_F_LO_LT_PDF ||
                             which == Y3G_F_LO_LT_BIN || which == Y3G_F_SCI ||
-                            which == Y3G_F_INTEGRAND || which == Y3G_F_INTEGRAND_PAIRED;
+                            which == Y3G_F_INTEGRAND || which == Y3G_F_INTEGRAND_PAIRED ||
+                            which == Y3G_F_INTEGRAND_NODE;
   if (needs_tables && rt->n_points <= 0)
     return fail(Y3G_ERR_STATE, "y3g_eval_functor: no tables");
-  bool const is_ig = which == Y3G_F_INTEGRAND || which == Y3G_F_INTEGRAND_PAIRED;
+  bool const is_ig = which == Y3G_F_INTEGRAND || which == Y3G_F_INTEGRAND_PAIRED ||
+                     which == Y3G_F_INTEGRAND_NODE;
   if (is_ig && (!aux || aux->ndim != nin))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: integrand needs aux integral");
   int const pt = is_ig ? aux->point : 0;
