@@ -176,21 +176,58 @@ template <typename T> Y3G_HD T zo_bin(T zlo, T zhi, T zt, T sigz0)
   return half_erf_diff<T>(M<T>::div(zlo - zt, base), M<T>::div(zhi - zt, base));
 }
 
+// FP32 erfcx on x >= 0 by the form of M. M. Shepherd and J. G. Laframboise,
+// "Chebyshev approximation of (1 + 2x) exp(x^2) erfc x in 0 <= x < infinity",
+// Math. Comp. 36 (1981): (1 + 2x) erfcx(x) as a degree-10 polynomial in
+// t = (x - 2) / (x + 2), coefficients fitted by csrc/fit_erfcx.py (largest
+// relative error 2.7e-7 in FP32 on [0, 1e7]); branch-free, two reciprocals.
+__device__ __forceinline__ float erfcx_pos_f(float x)
+{
+  float const t = (x - 2.0f) * __builtin_amdgcn_rcpf(x + 2.0f);
+  float p = 0.00016080358f;
+  p = p * t - 0.00018365444f;
+  p = p * t - 0.0013957069f;
+  p = p * t + 0.0010431446f;
+  p = p * t + 0.0086891595f;
+  p = p * t - 0.007942721f;
+  p = p * t - 0.05421196f;
+  p = p * t + 0.16403584f;
+  p = p * t - 0.16603109f;
+  p = p * t - 0.09276308f;
+  p = p * t + 1.2769784f;
+  return p * __builtin_amdgcn_rcpf(1.0f + 2.0f * x);
+}
+
+// FP32 line-of-sight pdf, Gaussian (sigma^2 = lt) plus the fraction fprj in
+// the EMG tail with tau = ips / lt (ips = 1 / prj_scale): one reciprocal
+// square root gives 1 / sigma, one exponential serves both terms, and
+// tau sigma^2 = ips.  Same quantity as lo_lt_pdf below.
+__device__ __forceinline__ float lo_lt_pdf_f(float lo, float lt, float fprj, float ips)
+{
+  float const isig = __builtin_amdgcn_rsqf(lt);
+  float const z = (lo - lt) * isig;
+  float const e = __expf(-0.5f * z * z);
+  float const g = e * 0.3989422804014327f * isig;
+  if (fprj == 0.0f) return g;
+  float const B = (lt + ips - lo) * isig * 0.70710678118654752f;
+  // E = exp(-z^2/2) erfcx(|B|); the pdf is tau/2 E for B >= 0 and
+  // tau/2 (2 exp(A) - E) = tau/2 exp(A) erfc(B) for B < 0
+  float const E = e * erfcx_pos_f(fabsf(B));
+  float const tau = ips * isig * isig;
+  float emg;
+  if (B >= 0.0f) {
+    emg = 0.5f * tau * E;
+  } else {
+    float const A = tau * (lt - lo) + 0.5f * tau * ips;
+    emg = 0.5f * tau * (2.0f * __expf(A) - E);
+  }
+  return (1.0f - fprj) * g + fprj * emg;
+}
+
 // Exponentially-modified Gaussian pdf (lc_lt_t.cuh:56-61 second term with
 // fmsk = 0; emg_des_t.cuh:172-193), overflow-free through erfcx.
 template <typename T> Y3G_HD T emg_pdf(T x, T mu, T sigma, T tau)
 {
-  if constexpr (sizeof(T) == 4) {
-    // one erfcx for both signs of B (no divergent branch): with
-    // E = exp(-d^2/2) erfcx(|B|) = exp(A) erfc(|B|), the pdf is tau/2 E for
-    // B >= 0 and tau/2 (2 exp(A) - E) = tau/2 exp(A) erfc(B) for B < 0
-    T const B = M<T>::div(mu + tau * sigma * sigma - x, T(1.4142135623730951) * sigma);
-    T const d = M<T>::div(x - mu, sigma);
-    T const E = M<T>::exp(T(-0.5) * d * d) * M<T>::erfcx(B < T(0) ? -B : B);
-    if (B >= T(0)) return T(0.5) * tau * E;
-    T const A = tau * (mu - x) + T(0.5) * tau * tau * sigma * sigma;
-    return T(0.5) * tau * (T(2) * M<T>::exp(A) - E);
-  }
   T const B = M<T>::div(mu + tau * sigma * sigma - x, T(1.4142135623730951) * sigma);
   if (B >= T(0)) {
     T const d = M<T>::div(x - mu, sigma);
@@ -243,6 +280,7 @@ template <typename T> Y3G_HD T emg_cdf_diff(T lo, T hi, T mu, T sigma, T tau)
 // P(lo | lt) = (1 - f_prj) N(lt, sqrt lt) + f_prj EMG(lt, sqrt lt, 1/(s lt)).
 template <typename T> Y3G_HD T lo_lt_pdf(T lo, T lt, T fprj, T pscale)
 {
+  if constexpr (sizeof(T) == 4) return lo_lt_pdf_f(lo, lt, fprj, __builtin_amdgcn_rcpf(pscale));
   T const sigma = M<T>::sqrt(lt);
   T const g = gaussian<T>(lo, lt, sigma);
   if (fprj == T(0)) return g;
@@ -335,39 +373,46 @@ template <typename T> Y3G_HD void nfw_F_h(T x, T& F, T& h)
   h = M<T>::div(T(1) - F, t);
 }
 
-// FP32 form of nfw_g below with the work its branches share done once:
-// ln x, |x^2 - 1| and its square root; ln((1 + s) / x) = ln(1 + s) - ln x.
-Y3G_HD float nfw_g_f32(float x)
+// FP32 form of nfw_g below as a function of x^2 (no square root of x is
+// needed): ln x = ln(x^2) / 2, one reciprocal square root of |x^2 - 1|
+// gives s and 1 / s.  Below x^2 = 0.9 the logarithm of (1 + s) / 2 is
+// log1p(-q), q = x^2 / (2 (1 + s)) (as (1 + s)^2 = 2 (1 + s) - x^2), which
+// also gives the small-x form of the mass term.
+__device__ __forceinline__ float nfw_g_x2(float x2)
 {
   using T = float;
-  T const x2 = x * x, t = x2 - T(1);
-  T const at = t < T(0) ? -t : t;
-  T const s = M<T>::sqrt(at);
-  T const lx = M<T>::log(x);
-  T F, h;
-  if (at < T(0.1)) {
+  T const t = x2 - T(1);
+  T const at = fabsf(t);
+  T const lx = T(0.5) * M<T>::log(x2);
+  T F, h, m;
+  if (at < T(0.3)) {
+    // series in t (truncation below 2e-8 for |t| < 0.3)
     T hs = T(0), tp = T(1);
 #pragma unroll
-    for (int k = 1; k <= 9; ++k) {
+    for (int k = 1; k <= 12; ++k) {
       T const term = tp * T(1.0 / (2 * k + 1));
       hs += (k & 1) ? term : -term;
       tp *= t;
     }
     h = hs;
     F = T(1) - t * hs;
+    m = (lx - T(0.69314718055994531) + F) * __builtin_amdgcn_rcpf(x2);
   } else {
-    T const is = M<T>::div(T(1), s);
-    F = (t < T(0) ? M<T>::log(T(1) + s) - lx : M<T>::atan(s)) * is;
-    h = M<T>::div(T(1) - F, t);
-  }
-  T m;
-  if (x < T(0.9)) {
-    T const i1s = M<T>::div(T(1), T(1) + s);
-    T const q = T(0.5) * x2 * i1s;
-    T const phi = M<T>::div(-M<T>::log1p(-q), q);
-    m = ((T(0.69314718055994531) - lx) * i1s - T(0.5) * phi * i1s) * M<T>::div(T(1), s);
-  } else {
-    m = M<T>::div(lx - T(0.69314718055994531) + F, x2);
+    T const is = __builtin_amdgcn_rsqf(at);
+    T const s = at * is;
+    if (t < T(0)) {
+      T const i1s = __builtin_amdgcn_rcpf(T(1) + s);
+      T const q = T(0.5) * x2 * i1s;
+      T const p = T(1) - q;
+      // g = -log1p(-q) / q, Goldberg's form
+      T const g = p == T(1) ? T(1) : M<T>::log(p) * __builtin_amdgcn_rcpf(p - T(1));
+      F = (T(0.69314718055994531) - lx - q * g) * is;
+      m = ((T(0.69314718055994531) - lx) - T(0.5) * g) * i1s * is;
+    } else {
+      F = M<T>::atan(s) * is;
+      m = (lx - T(0.69314718055994531) + F) * __builtin_amdgcn_rcpf(x2);
+    }
+    h = (T(1) - F) * __builtin_amdgcn_rcpf(t);
   }
   return T(4) * m - T(2) * h;
 }
@@ -375,7 +420,7 @@ Y3G_HD float nfw_g_f32(float x)
 // Delta-Sigma / (rho_s r_s) of NFW (Wright & Brainerd 2000).
 template <typename T> Y3G_HD T nfw_g(T x)
 {
-  if constexpr (sizeof(T) == 4) return nfw_g_f32(x);
+  if constexpr (sizeof(T) == 4) return nfw_g_x2(x * x);
   T F, h, m;
   nfw_F_h<T>(x, F, h);
   if (x < T(0.9)) {
@@ -420,9 +465,25 @@ template <typename T> Y3G_HD T nfw_dsigma(T R, const NfwHalo<T>& H)
 
 // Tangential Delta-Sigma about a centre offset by rmis at angle theta:
 // R' (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:176-178), cos(2 beta).
+// FP32 form from 1 / r_s: R'^2 and x^2 = R'^2 / r_s^2 need no square root.
+__device__ __forceinline__ float dsigma_offset_f(float R, float rmis, float theta, float irs,
+                                                 float amp)
+{
+  float const hc = __cosf(0.5f * theta);
+  float const k = hc * hc;
+  float const dr = R - rmis;
+  float const rp2 = dr * dr + 4.0f * R * rmis * k;
+  if (!(rp2 > 1e-20f)) return 0.0f;
+  float const a = dr + 2.0f * rmis * k;
+  float const c2b = 2.0f * a * a * __builtin_amdgcn_rcpf(rp2) - 1.0f;
+  return amp * nfw_g_x2(rp2 * irs * irs) * c2b;
+}
+
 template <typename T>
 Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 {
+  if constexpr (sizeof(T) == 4)
+    return dsigma_offset_f(R, rmis, theta, __builtin_amdgcn_rcpf(H.rs), H.amp);
   // with k = cos^2(theta/2): R'^2 = (R - rmis)^2 + 4 R rmis k and
   // R + rmis cos(theta) = (R - rmis) + 2 rmis k, free of the cancellation
   // of the textbook form near R = rmis, theta = pi
@@ -523,10 +584,11 @@ Y3G_HD T integrand(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
 // (zt, lnM) pairs, so the cubature evaluates pair_factors once per pair
 // and integrand_pair once per node.
 template <typename T> struct PairF {
-  T lnlbar, tlo, thi;  // MOR centre and the clipped t window
-  T zfac;              // hmf x dV/dz [x zo bin x Sigma_crit^-1 x angular norm]
-  T rs, amp;           // NFW halo (Delta-Sigma)
-  T cds;               // (1 - f_mis) x centred Delta-Sigma(R)
+  T lnlbar, tlo, dt;  // MOR centre, the clipped t window [tlo, tlo + dt]
+  T zfac;             // hmf x dV/dz x dt / sqrt(2 pi)
+                      //   [x zo bin x Sigma_crit^-1 x angular norm]
+  T irs, amp;         // NFW halo (Delta-Sigma): 1 / r_s, amplitude
+  T cds;              // (1 - f_mis) x centred Delta-Sigma(R)
 };
 
 template <typename T>
@@ -551,17 +613,20 @@ Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   T const isl = M<T>::div(T(1), sln);
   T tlo = (T(ig.lnlt_lo) - f.lnlbar) * isl;
   T thi = (T(ig.lnlt_hi) - f.lnlbar) * isl;
-  f.tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
-  f.thi = thi < T(kTMax) ? thi : T(kTMax);
-  T zf = hmf<T>(P, tb.hmf, lnM, zt) * dv_do_dz<T>(P, tb.da, zt);
-  f.rs = f.amp = f.cds = T(0);
+  tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
+  thi = thi < T(kTMax) ? thi : T(kTMax);
+  f.tlo = tlo;
+  f.dt = thi - tlo;
+  T zf = hmf<T>(P, tb.hmf, lnM, zt) * dv_do_dz<T>(P, tb.da, zt) *
+         (T(0.3989422804014327) * f.dt);
+  f.irs = f.amp = f.cds = T(0);
   if (ig.kind == Y3G_DSIGMA6D) {
     T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
     // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
     zf *= zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
           T(ig.rmis_norm * 0.31830988618379067);
     NfwHalo<T> const H = nfw_halo<T>(lnM, T(P.rho_m), T(0));
-    f.rs = H.rs;
+    f.irs = M<T>::div(T(1), H.rs);
     f.amp = H.amp;
     f.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
   }
@@ -574,13 +639,12 @@ template <typename T, int K>
 Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
                           const double* y)
 {
-  if (!(f.thi > f.tlo)) return T(0);
+  if (!(f.dt > T(0))) return T(0);
   constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
   T const sln = T(P.p.mor_sigma_ln);
-  T const tt = f.tlo + (f.thi - f.tlo) * T(y[it]);
+  T const tt = f.tlo + f.dt * T(y[it]);
   T const lt = M<T>::exp(f.lnlbar + sln * tt);
-  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
-  T const common = f.zfac * phi;
+  T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
   T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
   if constexpr (K == Y3G_COUNT3D) {
     return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
@@ -595,7 +659,7 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
     T const l1mu = u < 0.5 ? M<T>::log1p(T(-u)) : M<T>::log(T(1.0 - u));
     T const rmis = s * M<T>::sqrt(T(-2) * l1mu);
     NfwHalo<T> H;
-    H.rs = f.rs;
+    H.rs = M<T>::div(T(1), f.irs);
     H.amp = f.amp;
     T const prof = f.cds + T(P.p.f_mis) * dsigma_offset<T>(T(ig.radius), rmis, T(y[5]), H);
     return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) * prof;
@@ -605,7 +669,7 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
 // Per-integral constants of the FP32 node integrand, converted once per
 // chunk of regions and kept in registers across its nodes.
 struct NodeConstF {
-  float sln, fprj, pscale, sigz0, smis, fmis, radius, lo_lo, lo_hi;
+  float sln, fprj, pscale, ips, sigz0, smis, fmis, radius, lo_lo, lo_hi;
 };
 Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
 {
@@ -613,6 +677,7 @@ Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
   k.sln = float(P.p.mor_sigma_ln);
   k.fprj = float(P.p.f_prj);
   k.pscale = float(P.p.prj_scale);
+  k.ips = float(1.0 / P.p.prj_scale);
   k.sigz0 = float(P.p.sigz0);
   k.smis = float(ig.sigma_mis);
   k.fmis = float(P.p.f_mis);
@@ -630,26 +695,26 @@ Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairF<float>& f, const
                               float w)
 {
   using T = float;
-  if (!(f.thi > f.tlo)) return T(0);
+  if (!(f.dt > T(0))) return T(0);
   constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
-  T const tt = f.tlo + (f.thi - f.tlo) * yf[it];
+  T const tt = f.tlo + f.dt * yf[it];
   T const lt = M<T>::exp(f.lnlbar + kc.sln * tt);
-  T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (f.thi - f.tlo);
-  T const common = f.zfac * phi;
+  T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
   if constexpr (K == Y3G_COUNT3D) {
     return common * lo_lt_bin<T>(kc.lo_lo, kc.lo_hi, lt, T(0), kc.pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    return common * lo_lt_pdf<T>(yf[0], lt, kc.fprj, kc.pscale) *
-           zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
+    return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
   } else {
+    // ln(1 - u): of w = 1 - u (accurate near u = 1) for u >= 1/2, else
+    // log1p(-u) in Goldberg's form; one logarithm either way
     T const u = yf[4];
-    T const l1mu = u < T(0.5) ? M<T>::log1p(-u) : M<T>::log(w);
+    bool const low = u < T(0.5);
+    T const arg = low ? T(1) - u : w;
+    T const L = M<T>::log(arg);
+    T const l1mu = !low ? L : (arg == T(1) ? -u : L * (-u) * __builtin_amdgcn_rcpf(arg - T(1)));
     T const rmis = kc.smis * M<T>::sqrt(T(-2) * l1mu);
-    NfwHalo<T> H;
-    H.rs = f.rs;
-    H.amp = f.amp;
-    T const prof = f.cds + kc.fmis * dsigma_offset<T>(kc.radius, rmis, yf[5], H);
-    return common * lo_lt_pdf<T>(yf[0], lt, kc.fprj, kc.pscale) * prof;
+    T const prof = f.cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, yf[5], f.irs, f.amp);
+    return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * prof;
   }
 }
 
