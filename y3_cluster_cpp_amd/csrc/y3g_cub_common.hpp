@@ -206,6 +206,8 @@ constexpr int kKmax = 1024;
 // distinct (zt, lnM) offset pairs among a region's Genz-Malik nodes: the
 // centre, +-l2, +-l3, +-l4 on either axis alone, (+-l4, +-l4), (+-l5, +-l5)
 constexpr int kMaxPairs = 21;
+// distinct offsets on one axis: 0, +-l2, +-l3 (= +-l4), +-l5
+constexpr int kMaxAxisOffsets = 7;
 
 template <typename T, int BLOCK> struct CubShared {
   T hmf[kMaxHmf];
@@ -233,6 +235,9 @@ template <typename T, int BLOCK> struct CubShared {
   // unit offsets and each node's pair
   PairF<T> pf[sizeof(T) == 4 ? kChunk : 1][kMaxPairs];
   double poff[kMaxPairs][2];
+  double zoffs[kMaxAxisOffsets], moffs[kMaxAxisOffsets];  // distinct zt / lnM offsets
+  unsigned char pzi[kMaxPairs], pmi[kMaxPairs];            // each pair's two
+  int nzo, nmo;
   unsigned char npair[kMaxNodes];
   unsigned char nclass[kMaxNodes];                    // gm_class of each node
   float noff[sizeof(T) == 4 ? kMaxNodes : 1][kMaxDim];  // unit offsets, float
@@ -264,20 +269,52 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
       for (int a = 0; a < kMaxDim; ++a) S.noff[j][a] = a < d ? float(gm_offset(d, j, a)) : 0.0f;
   }
   S.npairs = np;
+  // the pairs' distinct zt and lnM offsets
+  int nz = 0, nm = 0;
+  for (int k = 0; k < np; ++k) {
+    int a = 0;
+    while (a < nz && S.zoffs[a] != S.poff[k][0]) ++a;
+    if (a == nz && nz < kMaxAxisOffsets) S.zoffs[nz++] = S.poff[k][0];
+    S.pzi[k] = (unsigned char)(a < kMaxAxisOffsets ? a : kMaxAxisOffsets - 1);
+    int b = 0;
+    while (b < nm && S.moffs[b] != S.poff[k][1]) ++b;
+    if (b == nm && nm < kMaxAxisOffsets) S.moffs[nm++] = S.poff[k][1];
+    S.pmi[k] = (unsigned char)(b < kMaxAxisOffsets ? b : kMaxAxisOffsets - 1);
+  }
+  S.nzo = nz;
+  S.nmo = nm;
 }
 
-// Factors of the (zt, lnM) pairs of m regions (FP32 path), one per thread.
+// Factors of the (zt, lnM) pairs of m regions (FP32 path): the zt and lnM
+// factors of each region's 7 + 7 distinct values first (one per thread, kept
+// in the node-value array, which is free until the nodes), then their
+// combinations for the pairs.
 template <typename T, int BLOCK>
 __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
+  static_assert(sizeof(ZFac<T>) * kChunk * kMaxAxisOffsets +
+                    sizeof(MFac<T>) * kChunk * kMaxAxisOffsets <= sizeof(S.f),
+                "zt / lnM factors exceed the node-value array");
   int iz, im, it;
   kind_axes<T>(S.ig.kind, it, iz, im);
+  ZFac<T>* zf = reinterpret_cast<ZFac<T>*>(S.f);
+  MFac<T>* mf = reinterpret_cast<MFac<T>*>(zf + kChunk * kMaxAxisOffsets);
+  int const nzo = S.nzo, nmo = S.nmo, nzm = nzo + nmo;
+  for (int q = threadIdx.x; q < m * nzm; q += BLOCK) {
+    int const r = q / nzm, k = q - r * nzm;
+    if (k < nzo)
+      zf[r * kMaxAxisOffsets + k] =
+        z_factors<T>(S.ig, S.P, tb, S.c[r][iz] + S.h[r][iz] * S.zoffs[k]);
+    else
+      mf[r * kMaxAxisOffsets + k - nzo] =
+        m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k - nzo]);
+  }
+  __syncthreads();
   int const np = S.npairs;
   for (int q = threadIdx.x; q < m * np; q += BLOCK) {
     int const r = q / np, k = q - r * np;
-    double const zt = S.c[r][iz] + S.h[r][iz] * S.poff[k][0];
-    double const lnM = S.c[r][im] + S.h[r][im] * S.poff[k][1];
-    S.pf[r][k] = pair_factors<T>(S.ig, S.P, tb, zt, lnM);
+    S.pf[r][k] = pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
+                                 mf[r * kMaxAxisOffsets + S.pmi[k]]);
   }
 }
 

@@ -601,39 +601,99 @@ Y3G_HD void kind_axes(int kind, int& it, int& iz, int& im)
   }
 }
 
+// The pair factors themselves split once more: what depends on zt alone
+// (the MOR redshift term, dV/dz [x zo bin x Sigma_crit^-1], the HMF table
+// row), what depends on lnM alone (the MOR mass term, the HMF table column
+// and amplitude factor, the NFW halo and the centred profile), and their
+// combination (the bilinear HMF value, the t window).  A region's 21 pairs
+// take 7 distinct zt and 7 distinct lnM values.
+template <typename T> struct ZFac {
+  T lz, zz, sz;  // MOR z term, dV/dz [x ...], HMF row fraction
+  int jz;        // HMF row
+};
+template <typename T> struct MFac {
+  T lm, hamp, tm;    // MOR mass term, HMF amplitude factor, column fraction
+  T irs, amp, cds;   // NFW halo, centred profile (Delta-Sigma)
+  int im;            // HMF column
+};
+
 template <typename T>
-Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
-                             double zt, double lnM)
+Y3G_HD ZFac<T> z_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb, double zt)
+{
+  ZFac<T> z;
+  T const zp = T(1) + T(zt);
+  z.lz = T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
+  double v = (zt - P.z0) * P.inv_dz;
+  v = v < 0.0 ? 0.0 : (v > double(P.nz - 1) ? double(P.nz - 1) : v);
+  int j = int(v);
+  if (j > P.nz - 2) j = P.nz - 2;
+  z.jz = j;
+  z.sz = T(v - double(j));
+  T zz = dv_do_dz<T>(P, tb.da, zt);
+  if (ig.kind == Y3G_DSIGMA6D) {
+    T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
+    // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
+    zz *= zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
+          T(ig.rmis_norm * 0.31830988618379067);
+  }
+  z.zz = zz;
+  return z;
+}
+
+template <typename T>
+Y3G_HD MFac<T> m_factors(const y3g_integral& ig, const PointDev& P, double lnM)
+{
+  MFac<T> m;
+  m.lm = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor);
+  double u = (lnM - P.lnm0) * P.inv_dlnm;
+  u = u < 0.0 ? 0.0 : (u > double(P.nm - 1) ? double(P.nm - 1) : u);
+  int i = int(u);
+  if (i > P.nm - 2) i = P.nm - 2;
+  m.im = i;
+  m.tm = T(u - double(i));
+  m.hamp = T(P.p.hmf_s * (lnM * 0.4342944819 - 13.8124426028) + P.p.hmf_q);
+  m.irs = m.amp = m.cds = T(0);
+  if (ig.kind == Y3G_DSIGMA6D) {
+    NfwHalo<T> const H = nfw_halo<T>(lnM, T(P.rho_m), T(0));
+    m.irs = M<T>::div(T(1), H.rs);
+    m.amp = H.amp;
+    m.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
+  }
+  return m;
+}
+
+template <typename T>
+Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                             const ZFac<T>& z, const MFac<T>& m)
 {
   PairF<T> f;
-  T const sln = T(P.p.mor_sigma_ln);
-  T const zp = T(1) + T(zt);
-  f.lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
-             T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
-  T const isl = M<T>::div(T(1), sln);
+  f.lnlbar = m.lm + z.lz;
+  T const isl = M<T>::div(T(1), T(P.p.mor_sigma_ln));
   T tlo = (T(ig.lnlt_lo) - f.lnlbar) * isl;
   T thi = (T(ig.lnlt_hi) - f.lnlbar) * isl;
   tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
   thi = thi < T(kTMax) ? thi : T(kTMax);
   f.tlo = tlo;
   f.dt = thi - tlo;
-  T zf = hmf<T>(P, tb.hmf, lnM, zt) * dv_do_dz<T>(P, tb.da, zt) *
-         (T(0.3989422804014327) * f.dt);
-  f.irs = f.amp = f.cds = T(0);
-  if (ig.kind == Y3G_DSIGMA6D) {
-    T const sci = lerp_clamp<T>(tb.sci, P.nsci, zt, P.inv_dzsci);
-    // (1/2pi) x 2 (theta symmetry) x rmis_norm (truncated Rayleigh)
-    zf *= zo_bin<T>(T(ig.zo_lo), T(ig.zo_hi), T(zt), T(P.p.sigz0)) * sci *
-          T(ig.rmis_norm * 0.31830988618379067);
-    NfwHalo<T> const H = nfw_halo<T>(lnM, T(P.rho_m), T(0));
-    f.irs = M<T>::div(T(1), H.rs);
-    f.amp = H.amp;
-    f.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
-  }
-  f.zfac = zf;
+  // HMF: clamped bilinear lookup (hmf_t.cuh:54-60), as hmf() below
+  const T* r0 = tb.hmf + z.jz * P.nm + m.im;
+  const T* r1 = r0 + P.nm;
+  T const a = r0[0] + m.tm * (r0[1] - r0[0]);
+  T const b = r1[0] + m.tm * (r1[1] - r1[0]);
+  T const hm = (a + z.sz * (b - a)) * m.hamp;
+  f.zfac = hm * z.zz * (T(0.3989422804014327) * f.dt);
+  f.irs = m.irs;
+  f.amp = m.amp;
+  f.cds = m.cds;
   return f;
 }
 
+template <typename T>
+Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
+                             double zt, double lnM)
+{
+  return pair_combine<T>(ig, P, tb, z_factors<T>(ig, P, tb, zt), m_factors<T>(ig, P, lnM));
+}
 
 template <typename T, int K>
 Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF<T>& f,
