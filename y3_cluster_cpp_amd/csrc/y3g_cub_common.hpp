@@ -299,15 +299,19 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   kind_axes<T>(S.ig.kind, it, iz, im);
   ZFac<T>* zf = reinterpret_cast<ZFac<T>*>(S.f);
   MFac<T>* mf = reinterpret_cast<MFac<T>*>(zf + kChunk * kMaxAxisOffsets);
-  int const nzo = S.nzo, nmo = S.nmo, nzm = nzo + nmo;
-  for (int q = threadIdx.x; q < m * nzm; q += BLOCK) {
-    int const r = q / nzm, k = q - r * nzm;
-    if (k < nzo)
+  int const nzo = S.nzo, nmo = S.nmo;
+  // all zt tasks, then all lnM tasks: a wave takes one kind (but at the seam)
+  for (int q = threadIdx.x; q < m * (nzo + nmo); q += BLOCK) {
+    if (q < m * nzo) {
+      int const r = q / nzo, k = q - r * nzo;
       zf[r * kMaxAxisOffsets + k] =
         z_factors<T>(S.ig, S.P, tb, S.c[r][iz] + S.h[r][iz] * S.zoffs[k]);
-    else
-      mf[r * kMaxAxisOffsets + k - nzo] =
-        m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k - nzo]);
+    } else {
+      int const qm = q - m * nzo;
+      int const r = qm / nmo, k = qm - r * nmo;
+      mf[r * kMaxAxisOffsets + k] =
+        m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k]);
+    }
   }
   __syncthreads();
   int const np = S.npairs;
