@@ -208,12 +208,19 @@ constexpr int kKmax = 1024;
 constexpr int kMaxPairs = 21;
 // distinct offsets on one axis: 0, +-l2, +-l3 (= +-l4), +-l5
 constexpr int kMaxAxisOffsets = 7;
+// distinct (lnM, u, theta) offset triples among the nodes of a 6-D region:
+// the centre, +-l2 and +-l3 on one of the three axes, (+-l4, +-l4) on two of
+// them, (+-l5, +-l5, +-l5)
+constexpr int kMaxProf = 1 + 6 + 6 + 12 + 8;
+// nodes read by the fourth differences: the centre, +-l2 and +-l3 per axis
+constexpr int kFourNodes = 1 + 4 * kMaxDim;
 
 template <typename T, int BLOCK> struct CubShared {
   T hmf[kMaxHmf];
   T da[kMaxDa];
   T sci[kMaxSci];
-  T f[kChunk * kMaxNodes];
+  // node values: the FP32 path keeps only those the fourth differences read
+  T f[kChunk * (sizeof(T) == 4 ? kFourNodes : kMaxNodes)];
   double c[kChunk][kMaxDim], h[kChunk][kMaxDim];
   double est[kChunk], err[kChunk];
   int ax[kChunk];
@@ -231,16 +238,28 @@ template <typename T, int BLOCK> struct CubShared {
   y3g_integral ig;
   PointDev P;
   int item, point, ncount;
-  // FP32 path: (zt, lnM) pair factors of the chunk's regions, the pairs'
-  // unit offsets and each node's pair
-  PairF<T> pf[sizeof(T) == 4 ? kChunk : 1][kMaxPairs];
+  // FP32 path, per chunk: the pairs' factors and (Delta-Sigma) the profile
+  // of each (lnM, u, theta) triple of the chunk's regions; the zt and lnM
+  // factors they are made of (zmbuf: ZFac [kChunk][7], then MFac [kChunk][7])
+  static constexpr bool kF32 = sizeof(T) == 4;
+  PairC pc[kF32 ? kChunk : 1][kMaxPairs];
+  float prof[kF32 ? kChunk : 1][kMaxProf];
+  alignas(16) unsigned char
+    zmbuf[kF32 ? (sizeof(ZFac<float>) + sizeof(MFac<float>)) * kChunk * kMaxAxisOffsets : 1];
+  // FP32 path, per integral: the pairs' unit offsets, their distinct zt and
+  // lnM offsets, the profile triples' offsets, and per node its pair, class,
+  // triple and unit offsets
   double poff[kMaxPairs][2];
   double zoffs[kMaxAxisOffsets], moffs[kMaxAxisOffsets];  // distinct zt / lnM offsets
   unsigned char pzi[kMaxPairs], pmi[kMaxPairs];            // each pair's two
   int nzo, nmo;
+  float tuo[kMaxProf], tto[kMaxProf];                      // triple: u, theta offsets
+  unsigned char tmi[kMaxProf];                             //   and its lnM offset
+  int nprofs;
   unsigned char npair[kMaxNodes];
   unsigned char nclass[kMaxNodes];                    // gm_class of each node
-  float noff[sizeof(T) == 4 ? kMaxNodes : 1][kMaxDim];  // unit offsets, float
+  unsigned char nprof[kMaxNodes];                     // triple of each node
+  float noff[kF32 ? kMaxNodes : 1][kMaxDim];          // unit offsets, float
   int npairs;
 };
 
@@ -283,21 +302,37 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
   }
   S.nzo = nz;
   S.nmo = nm;
+  // Delta-Sigma: the distinct (lnM, u, theta) offset triples
+  int nt = 0;
+  if (S.ig.kind == Y3G_DSIGMA6D) {
+    for (int j = 0; j < nn; ++j) {
+      double const om = gm_offset(d, j, im), ou = gm_offset(d, j, 4), ot = gm_offset(d, j, 5);
+      int b = 0;
+      while (b < nm && S.moffs[b] != om) ++b;
+      int t = 0;
+      while (t < nt && !(S.tmi[t] == b && S.tuo[t] == float(ou) && S.tto[t] == float(ot))) ++t;
+      if (t == nt && nt < kMaxProf) {
+        S.tmi[nt] = (unsigned char)(b < kMaxAxisOffsets ? b : kMaxAxisOffsets - 1);
+        S.tuo[nt] = float(ou);
+        S.tto[nt] = float(ot);
+        ++nt;
+      }
+      S.nprof[j] = (unsigned char)(t < kMaxProf ? t : kMaxProf - 1);
+    }
+  }
+  S.nprofs = nt;
 }
 
 // Factors of the (zt, lnM) pairs of m regions (FP32 path): the zt and lnM
-// factors of each region's 7 + 7 distinct values first (one per thread, kept
-// in the node-value array, which is free until the nodes), then their
-// combinations for the pairs.
+// factors of each region's 7 + 7 distinct values first (one per thread),
+// then their combinations for the pairs and, for Delta-Sigma, the profiles of
+// the region's (lnM, u, theta) triples.
 template <typename T, int BLOCK>
 __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
-  static_assert(sizeof(ZFac<T>) * kChunk * kMaxAxisOffsets +
-                    sizeof(MFac<T>) * kChunk * kMaxAxisOffsets <= sizeof(S.f),
-                "zt / lnM factors exceed the node-value array");
   int iz, im, it;
   kind_axes<T>(S.ig.kind, it, iz, im);
-  ZFac<T>* zf = reinterpret_cast<ZFac<T>*>(S.f);
+  ZFac<T>* zf = reinterpret_cast<ZFac<T>*>(S.zmbuf);
   MFac<T>* mf = reinterpret_cast<MFac<T>*>(zf + kChunk * kMaxAxisOffsets);
   int const nzo = S.nzo, nmo = S.nmo;
   // all zt tasks, then all lnM tasks: a wave takes one kind (but at the seam)
@@ -314,20 +349,32 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
     }
   }
   __syncthreads();
-  int const np = S.npairs;
-  for (int q = threadIdx.x; q < m * np; q += BLOCK) {
-    int const r = q / np, k = q - r * np;
-    S.pf[r][k] = pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
-                                 mf[r * kMaxAxisOffsets + S.pmi[k]]);
+  int const np = S.npairs, nt = S.nprofs;
+  NodeConstF const kc = node_const_f(S.ig, S.P);
+  for (int q = threadIdx.x; q < m * (np + nt); q += BLOCK) {
+    if (q < m * np) {
+      int const r = q / np, k = q - r * np;
+      S.pc[r][k] = pair_c(pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
+                                          mf[r * kMaxAxisOffsets + S.pmi[k]]));
+    } else {
+      int const qt = q - m * np;
+      int const r = qt / nt, t = qt - r * nt;
+      MFac<T> const& g = mf[r * kMaxAxisOffsets + S.tmi[t]];
+      float const u = float(S.c[r][4]) + float(S.h[r][4]) * S.tuo[t];
+      float const w = float(1.0 - S.c[r][4]) - float(S.h[r][4]) * S.tuo[t];
+      float const th = float(S.c[r][5]) + float(S.h[r][5]) * S.tto[t];
+      S.prof[r][t] = profile_f(kc, g.irs, g.amp, g.cds, u, w, th);
+    }
   }
 }
 
 // FP32 nodes of region r for integrand kind K, by the 16 lanes of a row
-// (lane sub takes nodes sub, sub + 16, ...): the values into S.f and this
-// lane's partial sums of the degree-7, -5 and -3 rules.  The region's centre
-// and half-widths stay in registers across its nodes; each node coordinate
-// is one FMA from them (the (zt, lnM)-dependent factors come from the pair,
-// in FP64 coordinates); for Delta-Sigma also 1 - u, accurate near u = 1.
+// (lane sub takes nodes sub, sub + 16, ...): this lane's partial sums of the
+// degree-7, -5 and -3 rules, and into S.f the values the fourth differences
+// read.  The region's centre and half-widths stay in registers across its
+// nodes; each node coordinate is one FMA from them (the (zt, lnM)-dependent
+// factors come from the pair, the Delta-Sigma profile from the triple, both
+// from FP64 coordinates).
 template <typename T, int BLOCK, int K>
 __device__ __forceinline__ void node_row(CubShared<T, BLOCK>& S, int r, int sub,
                                          const NodeConstF& kc, double& a7, double& a5,
@@ -338,18 +385,17 @@ __device__ __forceinline__ void node_row(CubShared<T, BLOCK>& S, int r, int sub,
   float cf[d], hf[d];
 #pragma unroll
   for (int a = 0; a < d; ++a) cf[a] = float(S.c[r][a]), hf[a] = float(S.h[r][a]);
-  float w0 = 1.0f;
-  if constexpr (K == Y3G_DSIGMA6D) w0 = float(1.0 - S.c[r][4]);
-  const PairF<T>* pfr = S.pf[r];
-  T* fr = S.f + r * nn;
+  const PairC* pcr = S.pc[r];
+  const float* profr = S.prof[r];
+  T* fr = S.f + r * kFourNodes;
   for (int j = sub; j < nn; j += 16) {
     float yf[d];
 #pragma unroll
     for (int a = 0; a < d; ++a) yf[a] = cf[a] + hf[a] * S.noff[j][a];
-    float w = 1.0f;
-    if constexpr (K == Y3G_DSIGMA6D) w = w0 - hf[4] * S.noff[j][4];
-    float const fv = integrand_node_f<K>(kc, pfr[S.npair[j]], yf, w);
-    fr[j] = fv;
+    float prof = 1.0f;
+    if constexpr (K == Y3G_DSIGMA6D) prof = profr[S.nprof[j]];
+    float const fv = integrand_node_f<K>(kc, pcr[S.npair[j]], yf, prof);
+    if (j <= 4 * d) fr[j] = fv;
     int const g = S.nclass[j];
     double const fd = fv;
     a7 += S.w7[g] * fd;
@@ -413,7 +459,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
   for (int r0 = 0; r0 < m; r0 += NG) {
     int const r = r0 + (threadIdx.x >> 4);
     bool const live = r < m;
-    const T* fr = S.f + (live ? r : 0) * nn;
+    const T* fr = S.f + (live ? r : 0) * (sizeof(T) == 4 ? kFourNodes : nn);
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
     if constexpr (sizeof(T) == 4) {
       if (live) {

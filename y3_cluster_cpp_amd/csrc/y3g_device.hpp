@@ -750,9 +750,37 @@ Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
 // FP32 node form of integrand_pair_k: the node's coordinates in float (yf)
 // and, for Delta-Sigma, 1 - u in float (w) next to u (yf[4]); each integrand
 // variable then costs one FMA from its region's centre and half-width.
+// The FP32 cubature's per-pair factors: those the nodes read.
+struct PairC {
+  float lnlbar, tlo, dt, zfac;
+};
+Y3G_HD PairC pair_c(const PairF<float>& f)
+{
+  return PairC{f.lnlbar, f.tlo, f.dt, f.zfac};
+}
+
+// Miscentred profile of Delta-Sigma, f_mis x offset + (1 - f_mis) x centred
+// (cds), at the Rayleigh CDF value u (w = 1 - u, accurate near u = 1) and the
+// angle theta, for a halo of 1 / r_s = irs and amplitude amp.  It depends on
+// (lnM, u, theta) only: a region's 149 nodes take 33 distinct triples.
+Y3G_HD float profile_f(const NodeConstF& kc, float irs, float amp, float cds, float u, float w,
+                       float theta)
+{
+  using T = float;
+  // ln(1 - u): of w for u >= 1/2, else log1p(-u) in Goldberg's form; one
+  // logarithm either way
+  bool const low = u < T(0.5);
+  T const arg = low ? T(1) - u : w;
+  T const L = M<T>::log(arg);
+  T const l1mu = !low ? L : (arg == T(1) ? -u : L * (-u) * __builtin_amdgcn_rcpf(arg - T(1)));
+  T const rmis = kc.smis * M<T>::sqrt(T(-2) * l1mu);
+  return cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, theta, irs, amp);
+}
+
+// FP32 node form of integrand_pair_k: the node's coordinates in float (yf;
+// lo, t [, zo]), its pair's factors and, for Delta-Sigma, its profile.
 template <int K>
-Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairF<float>& f, const float* yf,
-                              float w)
+Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairC& f, const float* yf, float prof)
 {
   using T = float;
   if (!(f.dt > T(0))) return T(0);
@@ -765,15 +793,6 @@ Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairF<float>& f, const
   } else if constexpr (K == Y3G_COUNT5D) {
     return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
   } else {
-    // ln(1 - u): of w = 1 - u (accurate near u = 1) for u >= 1/2, else
-    // log1p(-u) in Goldberg's form; one logarithm either way
-    T const u = yf[4];
-    bool const low = u < T(0.5);
-    T const arg = low ? T(1) - u : w;
-    T const L = M<T>::log(arg);
-    T const l1mu = !low ? L : (arg == T(1) ? -u : L * (-u) * __builtin_amdgcn_rcpf(arg - T(1)));
-    T const rmis = kc.smis * M<T>::sqrt(T(-2) * l1mu);
-    T const prof = f.cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, yf[5], f.irs, f.amp);
     return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * prof;
   }
 }

@@ -628,10 +628,16 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
         float yf[kMaxDim];
         for (int a = 0; a < kMaxDim; ++a) yf[a] = a < F.ig->ndim ? float(x[a]) : 0.0f;
         NodeConstF const kc = node_const_f(*F.ig, P);
+        PairC const pc = pair_c(pf);
         switch (F.ig->kind) {
-          case Y3G_COUNT3D: r = integrand_node_f<Y3G_COUNT3D>(kc, pf, yf, 1.0f); break;
-          case Y3G_COUNT5D: r = integrand_node_f<Y3G_COUNT5D>(kc, pf, yf, 1.0f); break;
-          default: r = integrand_node_f<Y3G_DSIGMA6D>(kc, pf, yf, float(1.0 - x[4])); break;
+          case Y3G_COUNT3D: r = integrand_node_f<Y3G_COUNT3D>(kc, pc, yf, 1.0f); break;
+          case Y3G_COUNT5D: r = integrand_node_f<Y3G_COUNT5D>(kc, pc, yf, 1.0f); break;
+          default: {
+            float const prof =
+              profile_f(kc, pf.irs, pf.amp, pf.cds, yf[4], float(1.0 - x[4]), yf[5]);
+            r = integrand_node_f<Y3G_DSIGMA6D>(kc, pc, yf, prof);
+            break;
+          }
         }
       } else {
         r = integrand_pair<T>(*F.ig, P, pf, x);
