@@ -10,6 +10,16 @@
 
 namespace y3g {
 
+// Workgroup barrier ordering LDS only: each wave waits for its own LDS
+// operations (lgkmcnt) but not for its vector-memory ones, so that the next
+// chunk's parents fetched straight into LDS (fetch_parents) stay in flight
+// across a chunk's evaluation.  For phases that exchange data through LDS
+// alone.
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -215,6 +225,17 @@ constexpr int kMaxProf = 1 + 6 + 6 + 12 + 8;
 // nodes read by the fourth differences: the centre, +-l2 and +-l3 per axis
 constexpr int kFourNodes = 1 + 4 * kMaxDim;
 
+// One chunk's parents as fetched from the region slab, 512 dwords: lo and hi
+// [parent][axis], est, err, raw [parent], and the dword holding each
+// parent's split axis byte.
+struct ParentStage {
+  double lo[kChunk / 2][kMaxDim], hi[kChunk / 2][kMaxDim];
+  double est[kChunk / 2], err[kChunk / 2], raw[kChunk / 2];
+  unsigned axw[kChunk / 2];
+  unsigned pad[512 - 2 * (2 * (kChunk / 2) * kMaxDim + 3 * (kChunk / 2)) - kChunk / 2];
+};
+static_assert(sizeof(ParentStage) == 2048, "ParentStage is 512 dwords");
+
 template <typename T, int BLOCK> struct CubShared {
   T hmf[kMaxHmf];
   T da[kMaxDa];
@@ -348,7 +369,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
         m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k]);
     }
   }
-  __syncthreads();
+  lds_barrier();
   int const np = S.npairs, nt = S.nprofs;
   NodeConstF const kc = node_const_f(S.ig, S.P);
   for (int q = threadIdx.x; q < m * (np + nt); q += BLOCK) {
@@ -366,6 +387,59 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
       S.prof[r][t] = profile_f(kc, g.irs, g.amp, g.cds, u, w, th);
     }
   }
+}
+
+// The dwords of parents sel[pb, pb + np) in ParentStage order, 512 / BLOCK
+// per thread (dword i = k BLOCK + threadIdx.x): plain loads whose results
+// are first used when stored to the stage, so that they stay in flight across
+// whatever the caller does in between.  Lanes past np fetch parent sel[pb]
+// again, so every address lies in the slab (the split-axis array is padded by
+// 64 bytes for the dword holding its last byte).
+template <int BLOCK> struct StageWords {
+  unsigned w[512 / BLOCK];
+};
+template <int BLOCK>
+__device__ __forceinline__ StageWords<BLOCK> fetch_parents(const int* sel, int pb, int np, int cap,
+                                                          const double* lo, const double* hi,
+                                                          const double* est, const double* err,
+                                                          const double* raw, const int8_t* axs)
+{
+  constexpr int kPar = kChunk / 2, kLoW = 2 * kPar * kMaxDim;
+  StageWords<BLOCK> out;
+#pragma unroll
+  for (int k = 0; k < 512 / BLOCK; ++k) {
+    int const i = k * BLOCK + (int)threadIdx.x;
+    const unsigned* src;
+    if (i < 2 * kLoW) {
+      int const half = i & 1, jd = (i % kLoW) >> 1;
+      int const j = jd / kMaxDim, a = jd - j * kMaxDim;
+      int const p = sel[pb + (j < np ? j : 0)];
+      const double* base = i < kLoW ? lo : hi;
+      src = reinterpret_cast<const unsigned*>(base + (size_t)a * cap + p) + half;
+    } else if (i < 2 * kLoW + 6 * kPar) {
+      int const q = i - 2 * kLoW, f = q / (2 * kPar), j = (q % (2 * kPar)) >> 1;
+      int const p = sel[pb + (j < np ? j : 0)];
+      const double* base = f == 0 ? est : (f == 1 ? err : raw);
+      src = reinterpret_cast<const unsigned*>(base + p) + (q & 1);
+    } else if (i < 2 * kLoW + 7 * kPar) {
+      int const j = i - 2 * kLoW - 6 * kPar;
+      int const p = sel[pb + (j < np ? j : 0)];
+      src = reinterpret_cast<const unsigned*>(reinterpret_cast<uintptr_t>(axs + p) & ~uintptr_t(3));
+    } else {
+      src = reinterpret_cast<const unsigned*>(lo);
+    }
+    // through the global address space: a flat load would also count on
+    // lgkmcnt and hold up every LDS wait until it returns
+    out.w[k] = *(const __attribute__((address_space(1))) unsigned*)src;
+  }
+  return out;
+}
+template <int BLOCK>
+__device__ __forceinline__ void store_stage(ParentStage* st, const StageWords<BLOCK>& v)
+{
+#pragma unroll
+  for (int k = 0; k < 512 / BLOCK; ++k)
+    reinterpret_cast<unsigned*>(st)[k * BLOCK + threadIdx.x] = v.w[k];
 }
 
 // FP32 nodes of region r for integrand kind K, by the 16 lanes of a row
@@ -450,7 +524,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }
-  __syncthreads();
+  lds_barrier();
   // rule sums: a row of 16 lanes per region (four regions per wave), partial
   // sums over nodes j = sub, sub + 16, ... reduced within the row by DPP; in
   // the FP32 path the row evaluates its region's nodes as it sums them
@@ -470,7 +544,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
           default: node_row<T, BLOCK, Y3G_DSIGMA6D>(S, r, sub, kc, a7, a5, a3); break;
         }
       }
-      __syncthreads();  // the rows' node values, for the fourth differences
+      lds_barrier();  // the rows' node values, for the fourth differences
     } else {
       if (live) {
         for (int j = sub; j < nn; j += 16) {
@@ -515,7 +589,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       S.ax[r] = best;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 }  // namespace y3g

@@ -98,6 +98,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
 {
   extern __shared__ char smem_raw[];
   CubShared<T, BLOCK>& S = *reinterpret_cast<CubShared<T, BLOCK>*>(smem_raw);
+  // the parents of the chunk being split
+  __shared__ ParentStage stage;
   int const tid = threadIdx.x;
   int const slot = blockIdx.x;
   int const cap = A.cap;
@@ -242,15 +244,22 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
         if (S.ncount >= lim0 && S.ncount1 >= lim1) break;
       }
       // split and evaluate, kChunk/2 pairs at a time
+      // the parents of the next chunk are fetched into LDS while this one is
+      // evaluated
       double dI = 0.0, dE = 0.0;
+      store_stage<BLOCK>(&stage, fetch_parents<BLOCK>(S.sel, 0, min(kChunk / 2, ns), cap, lo,
+                                                      hi, est, err, raw, axs));
       for (int pb = 0; pb < ns; pb += kChunk / 2) {
         int const np = min(kChunk / 2, ns - pb);
+        bool const more = pb + kChunk / 2 < ns;
+        __syncthreads();
+        ParentStage const& st = stage;
         for (int q = tid; q < np * d; q += BLOCK) {
           int const j = q / d, a = q - j * d;
           int const p = S.sel[pb + j];
-          int const ax = axs[p];
-          double const l = lo[(size_t)a * cap + p];
-          double const h = hi[(size_t)a * cap + p];
+          int const ax = (int)(int8_t)(st.axw[j] >> (8 * (int)(reinterpret_cast<uintptr_t>(axs + p) & 3)));
+          double const l = st.lo[j][a];
+          double const h = st.hi[j][a];
           if (a == ax) {
             double const mid = 0.5 * (l + h);
             S.c[2 * j][a] = 0.5 * (l + mid);
@@ -268,13 +277,21 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
           }
         }
         if (tid < np) {
-          int const p = S.sel[pb + tid];
-          S.pest[tid] = est[p];
-          S.perr[tid] = err[p];
-          S.praw[tid] = raw[p];
+          S.pest[tid] = st.est[tid];
+          S.perr[tid] = st.err[tid];
+          S.praw[tid] = st.raw[tid];
         }
-        __syncthreads();
+        // the next chunk's parents: loaded now, stored to the stage after
+        // this chunk's evaluation
+        StageWords<BLOCK> next;
+        if (more)
+          next = fetch_parents<BLOCK>(S.sel, pb + kChunk / 2, min(kChunk / 2, ns - pb - kChunk / 2),
+                                      cap, lo, hi, est, err, raw, axs);
+        lds_barrier();
         eval_chunk<T, BLOCK>(S, 2 * np, A.err_c, A.err_smin);
+        if (more) store_stage<BLOCK>(&stage, next);
+        // results; the next chunk's first barrier orders these LDS reads
+        // before its writes
         if (tid < np) {
           int const p = S.sel[pb + tid];
           double const ra = S.err[2 * tid], rb = S.err[2 * tid + 1];
@@ -303,8 +320,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
           raw[N + pb + tid] = rb;
           axs[N + pb + tid] = (int8_t)S.ax[2 * tid + 1];
         }
-        __syncthreads();
       }
+      __syncthreads();
       block_sum2<BLOCK>(dI, dE, S.red);
       I += dI;
       E += dE;

@@ -492,7 +492,7 @@ static int ensure_slab(y3g_runtime* rt, int slots, int cap)
   size_t const n = (size_t)slots * cap;
   if (dalloc(&rt->slab_lo, n * kMaxDim) || dalloc(&rt->slab_hi, n * kMaxDim) ||
       dalloc(&rt->slab_est, n) || dalloc(&rt->slab_err, n) ||
-      dalloc(&rt->slab_raw, n) || dalloc(&rt->slab_ax, n))
+      dalloc(&rt->slab_raw, n) || dalloc(&rt->slab_ax, n + 64))
     return Y3G_ERR_MEMORY;
   rt->slab_slots = slots;
   rt->slab_cap = cap;
