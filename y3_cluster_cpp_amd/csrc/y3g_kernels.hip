@@ -243,9 +243,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
         __syncthreads();
         if (S.ncount >= lim0 && S.ncount1 >= lim1) break;
       }
-      // split and evaluate, kChunk/2 pairs at a time
-      // the parents of the next chunk are fetched into LDS while this one is
-      // evaluated
+      // split and evaluate, kChunk/2 pairs at a time; the next chunk's
+      // parents are loaded while this one is evaluated
       double dI = 0.0, dE = 0.0;
       store_stage<BLOCK>(&stage, fetch_parents<BLOCK>(S.sel, 0, min(kChunk / 2, ns), cap, lo,
                                                       hi, est, err, raw, axs));
