@@ -149,10 +149,19 @@ template <typename T> Y3G_HD T sigma_photoz_des(T zt)
   return s + c[10];
 }
 
+__device__ __forceinline__ float erfcx_pos_f(float x);
+
 // Half the difference erf(b) - erf(a), b >= a, in the cancellation-free
-// form (complementary functions on the tails).
+// form (complementary functions on the tails).  FP32: erfc(|x|) =
+// exp(-x^2) erfcx(|x|) with the polynomial erfcx below, branch-free.
 template <typename T> Y3G_HD T half_erf_diff(T a, T b)
 {
+  if constexpr (sizeof(T) == 4) {
+    float const aa = fabsf(a), ab = fabsf(b);
+    float const ea = __expf(-aa * aa) * erfcx_pos_f(aa);
+    float const eb = __expf(-ab * ab) * erfcx_pos_f(ab);
+    return a > 0.0f ? 0.5f * (ea - eb) : (b < 0.0f ? 0.5f * (eb - ea) : 0.5f * (2.0f - ea - eb));
+  }
   if (a > T(0)) return T(0.5) * (M<T>::erfc(a) - M<T>::erfc(b));
   if (b < T(0)) return T(0.5) * (M<T>::erfc(-b) - M<T>::erfc(-a));
   return T(0.5) * (M<T>::erf(b) - M<T>::erf(a));
