@@ -10,6 +10,18 @@
 
 namespace y3g {
 
+// q / n for the task loops (q < 2^20, 0 < n <= 64) by a multiply-high with
+// ceil(2^32 / n): exact there, and a few instructions in place of a
+// run-time integer division.
+__host__ __device__ inline unsigned long long div_magic(int n)
+{
+  return 0xFFFFFFFFull / (unsigned long long)n + 1ull;
+}
+__device__ __forceinline__ int div_by(int q, unsigned long long magic)
+{
+  return (int)(((unsigned long long)(unsigned)q * magic) >> 32);
+}
+
 // Workgroup barrier ordering LDS only: each wave waits for its own LDS
 // operations (lgkmcnt) but not for its vector-memory ones, so that the next
 // chunk's parents fetched straight into LDS (fetch_parents) stay in flight
@@ -224,6 +236,10 @@ constexpr int kMaxAxisOffsets = 7;
 constexpr int kMaxProf = 1 + 6 + 6 + 12 + 8;
 // nodes read by the fourth differences: the centre, +-l2 and +-l3 per axis
 constexpr int kFourNodes = 1 + 4 * kMaxDim;
+// distinct (lo, t, zt, lnM) offset quadruples among the nodes of a 5-D or 6-D
+// region: the centre, +-l2 and +-l3 on one of the four axes, (+-l4, +-l4) on
+// two of them, (+-l5, ..., +-l5)
+constexpr int kMaxQuads = 1 + 8 + 8 + 24 + 16;
 
 // One chunk's parents as fetched from the region slab, 512 dwords: lo and hi
 // [parent][axis], est, err, raw [parent], and the dword holding each
@@ -276,7 +292,12 @@ template <typename T, int BLOCK> struct CubShared {
   int nzo, nmo;
   float tuo[kMaxProf], tto[kMaxProf];                      // triple: u, theta offsets
   unsigned char tmi[kMaxProf];                             //   and its lnM offset
-  int nprofs;
+  int nprofs;            // (5-D counts: the (zo, zt) offset pairs in tuo / tto)
+  float qo0[kMaxQuads], qo1[kMaxQuads];                    // quadruple: lo, t offsets
+  unsigned char qk[kMaxQuads];                             //   and its pair
+  int nquads;
+  unsigned char nquad[kMaxNodes];                          // quadruple of each node
+  unsigned long long dv_zo, dv_mo, dv_np, dv_nt, dv_nq;    // div_magic of the counts
   unsigned char npair[kMaxNodes];
   unsigned char nclass[kMaxNodes];                    // gm_class of each node
   unsigned char nprof[kMaxNodes];                     // triple of each node
@@ -341,13 +362,54 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
       S.nprof[j] = (unsigned char)(t < kMaxProf ? t : kMaxProf - 1);
     }
   }
+  // 5-D counts: the distinct (zo, zt) offset pairs
+  if (S.ig.kind == Y3G_COUNT5D) {
+    for (int j = 0; j < nn; ++j) {
+      float const ozo = float(gm_offset(d, j, 1)), ozt = float(gm_offset(d, j, iz));
+      int t = 0;
+      while (t < nt && !(S.tuo[t] == ozo && S.tto[t] == ozt)) ++t;
+      if (t == nt && nt < kMaxProf) {
+        S.tuo[nt] = ozo;
+        S.tto[nt] = ozt;
+        ++nt;
+      }
+      S.nprof[j] = (unsigned char)(t < kMaxProf ? t : kMaxProf - 1);
+    }
+  }
   S.nprofs = nt;
+  // 5-D counts and Delta-Sigma: the distinct (lo, t, zt, lnM) quadruples,
+  // each with its (zt, lnM) pair
+  int nq = 0;
+  if (S.ig.kind != Y3G_COUNT3D) {
+    for (int j = 0; j < nn; ++j) {
+      float const o0 = float(gm_offset(d, j, 0)), ot = float(gm_offset(d, j, it));
+      int const k = S.npair[j];
+      int q = 0;
+      while (q < nq && !(S.qo0[q] == o0 && S.qo1[q] == ot && S.qk[q] == k)) ++q;
+      if (q == nq && nq < kMaxQuads) {
+        S.qo0[nq] = o0;
+        S.qo1[nq] = ot;
+        S.qk[nq] = (unsigned char)k;
+        ++nq;
+      }
+      S.nquad[j] = (unsigned char)(q < kMaxQuads ? q : kMaxQuads - 1);
+    }
+  }
+  S.nquads = nq;
+  S.dv_zo = div_magic(nz > 0 ? nz : 1);
+  S.dv_mo = div_magic(nm > 0 ? nm : 1);
+  S.dv_np = div_magic(np > 0 ? np : 1);
+  S.dv_nt = div_magic(nt > 0 ? nt : 1);
+  S.dv_nq = div_magic(nq > 0 ? nq : 1);
 }
 
-// Factors of the (zt, lnM) pairs of m regions (FP32 path): the zt and lnM
-// factors of each region's 7 + 7 distinct values first (one per thread),
-// then their combinations for the pairs and, for Delta-Sigma, the profiles of
-// the region's (lnM, u, theta) triples.
+// Factors of m regions' nodes (FP32 path), one per thread in three phases:
+// the zt and lnM factors of each region's 7 + 7 distinct values; their
+// combinations for the 21 (zt, lnM) pairs and, for Delta-Sigma, the profiles
+// of the 33 (lnM, u, theta) triples; then (5-D counts, Delta-Sigma) the
+// factor of the 57 (lo, t, zt, lnM) quadruples and, for the counts, the
+// photo-z pdf of the 17 (zo, zt) pairs.  A node's value is then a product of
+// two table entries.
 template <typename T, int BLOCK>
 __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
@@ -359,32 +421,57 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   // all zt tasks, then all lnM tasks: a wave takes one kind (but at the seam)
   for (int q = threadIdx.x; q < m * (nzo + nmo); q += BLOCK) {
     if (q < m * nzo) {
-      int const r = q / nzo, k = q - r * nzo;
+      int const r = div_by(q, S.dv_zo), k = q - r * nzo;
       zf[r * kMaxAxisOffsets + k] =
         z_factors<T>(S.ig, S.P, tb, S.c[r][iz] + S.h[r][iz] * S.zoffs[k]);
     } else {
       int const qm = q - m * nzo;
-      int const r = qm / nmo, k = qm - r * nmo;
+      int const r = div_by(qm, S.dv_mo), k = qm - r * nmo;
       mf[r * kMaxAxisOffsets + k] =
         m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k]);
     }
   }
   lds_barrier();
   int const np = S.npairs, nt = S.nprofs;
+  int const ntp = S.ig.kind == Y3G_DSIGMA6D ? nt : 0;  // profiles
   NodeConstF const kc = node_const_f(S.ig, S.P);
-  for (int q = threadIdx.x; q < m * (np + nt); q += BLOCK) {
+  for (int q = threadIdx.x; q < m * (np + ntp); q += BLOCK) {
     if (q < m * np) {
-      int const r = q / np, k = q - r * np;
+      int const r = div_by(q, S.dv_np), k = q - r * np;
       S.pc[r][k] = pair_c(pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
                                           mf[r * kMaxAxisOffsets + S.pmi[k]]));
     } else {
       int const qt = q - m * np;
-      int const r = qt / nt, t = qt - r * nt;
+      int const r = div_by(qt, S.dv_nt), t = qt - r * nt;
       MFac<T> const& g = mf[r * kMaxAxisOffsets + S.tmi[t]];
       float const u = float(S.c[r][4]) + float(S.h[r][4]) * S.tuo[t];
       float const w = float(1.0 - S.c[r][4]) - float(S.h[r][4]) * S.tuo[t];
       float const th = float(S.c[r][5]) + float(S.h[r][5]) * S.tto[t];
       S.prof[r][t] = profile_f(kc, g.irs, g.amp, g.cds, u, w, th);
+    }
+  }
+  // 5-D counts and Delta-Sigma: the (lo, t, zt, lnM) factor of each
+  // quadruple (over the zt / lnM factors, no longer needed) and, for the
+  // counts, the photo-z pdf of each (zo, zt) pair
+  if (S.ig.kind == Y3G_COUNT3D) return;
+  static_assert(sizeof(float) * kChunk * kMaxQuads <= sizeof(S.zmbuf),
+                "quadruple factors exceed their buffer");
+  lds_barrier();
+  float* Q = reinterpret_cast<float*>(S.zmbuf);
+  int const nq = S.nquads;
+  int const nz2 = S.ig.kind == Y3G_COUNT5D ? nt : 0;
+  for (int q = threadIdx.x; q < m * (nq + nz2); q += BLOCK) {
+    if (q < m * nq) {
+      int const r = div_by(q, S.dv_nq), k = q - r * nq;
+      float const lo = float(S.c[r][0]) + float(S.h[r][0]) * S.qo0[k];
+      float const yt = float(S.c[r][it]) + float(S.h[r][it]) * S.qo1[k];
+      Q[r * kMaxQuads + k] = quad_f(kc, S.pc[r][S.qk[k]], lo, yt);
+    } else {
+      int const qz = q - m * nq;
+      int const r = div_by(qz, S.dv_nt), t = qz - r * nz2;
+      float const zo = float(S.c[r][1]) + float(S.h[r][1]) * S.tuo[t];
+      float const zt = float(S.c[r][iz]) + float(S.h[r][iz]) * S.tto[t];
+      S.prof[r][t] = zo_pdf<float>(zo, zt, kc.sigz0);
     }
   }
 }
@@ -440,6 +527,30 @@ __device__ __forceinline__ void store_stage(ParentStage* st, const StageWords<BL
 #pragma unroll
   for (int k = 0; k < 512 / BLOCK; ++k)
     reinterpret_cast<unsigned*>(st)[k * BLOCK + threadIdx.x] = v.w[k];
+}
+
+// FP32 nodes of region r of a 5-D count or Delta-Sigma integral, by the 16
+// lanes of a row: each node's value is the product of its quadruple's factor
+// and its profile (Delta-Sigma) or photo-z pdf (counts); this lane's partial
+// rule sums, and into S.f the values the fourth differences read.
+template <typename T, int BLOCK, int K>
+__device__ __forceinline__ void node_row_q(CubShared<T, BLOCK>& S, int r, int sub, double& a7,
+                                           double& a5, double& a3)
+{
+  constexpr int d = KindAxes<K>::nd;
+  constexpr int nn = gm_num_nodes(d);
+  const float* Qr = reinterpret_cast<const float*>(S.zmbuf) + r * kMaxQuads;
+  const float* F2r = S.prof[r];
+  T* fr = S.f + r * kFourNodes;
+  for (int j = sub; j < nn; j += 16) {
+    float const fv = Qr[S.nquad[j]] * F2r[S.nprof[j]];
+    if (j <= 4 * d) fr[j] = fv;
+    int const g = S.nclass[j];
+    double const fd = fv;
+    a7 += S.w7[g] * fd;
+    a5 += S.w5[g] * fd;
+    a3 += S.w3[g] * fd;
+  }
 }
 
 // FP32 nodes of region r for integrand kind K, by the 16 lanes of a row
@@ -540,8 +651,8 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
         NodeConstF const kc = node_const_f(S.ig, S.P);
         switch (S.ig.kind) {
           case Y3G_COUNT3D: node_row<T, BLOCK, Y3G_COUNT3D>(S, r, sub, kc, a7, a5, a3); break;
-          case Y3G_COUNT5D: node_row<T, BLOCK, Y3G_COUNT5D>(S, r, sub, kc, a7, a5, a3); break;
-          default: node_row<T, BLOCK, Y3G_DSIGMA6D>(S, r, sub, kc, a7, a5, a3); break;
+          case Y3G_COUNT5D: node_row_q<T, BLOCK, Y3G_COUNT5D>(S, r, sub, a7, a5, a3); break;
+          default: node_row_q<T, BLOCK, Y3G_DSIGMA6D>(S, r, sub, a7, a5, a3); break;
         }
       }
       lds_barrier();  // the rows' node values, for the fourth differences

@@ -786,23 +786,38 @@ Y3G_HD float profile_f(const NodeConstF& kc, float irs, float amp, float cds, fl
   return cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, theta, irs, amp);
 }
 
+// The factor of the 5-D count and Delta-Sigma integrands on (lo, t, zt,
+// lnM): the pair's zfac exp(-t^2 / 2) times the line-of-sight pdf at lo.  A
+// region's 93 / 149 nodes take 57 distinct values of these four coordinates.
+Y3G_HD float quad_f(const NodeConstF& kc, const PairC& f, float lo, float yt)
+{
+  using T = float;
+  if (!(f.dt > T(0))) return T(0);
+  T const tt = f.tlo + f.dt * yt;
+  T const lt = M<T>::exp(f.lnlbar + kc.sln * tt);
+  T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
+  return common * lo_lt_pdf_f(lo, lt, kc.fprj, kc.ips);
+}
+
 // FP32 node form of integrand_pair_k: the node's coordinates in float (yf;
-// lo, t [, zo]), its pair's factors and, for Delta-Sigma, its profile.
+// lo, t [, zo]), its pair's factors and, for Delta-Sigma, its profile.  The
+// cubature evaluates the same products factor by factor (quad_f, profile_f,
+// zo_pdf).
 template <int K>
 Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairC& f, const float* yf, float prof)
 {
   using T = float;
-  if (!(f.dt > T(0))) return T(0);
   constexpr int it = KindAxes<K>::it, iz = KindAxes<K>::iz;
-  T const tt = f.tlo + f.dt * yf[it];
-  T const lt = M<T>::exp(f.lnlbar + kc.sln * tt);
-  T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
   if constexpr (K == Y3G_COUNT3D) {
+    if (!(f.dt > T(0))) return T(0);
+    T const tt = f.tlo + f.dt * yf[it];
+    T const lt = M<T>::exp(f.lnlbar + kc.sln * tt);
+    T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
     return common * lo_lt_bin<T>(kc.lo_lo, kc.lo_hi, lt, T(0), kc.pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
+    return quad_f(kc, f, yf[0], yf[it]) * zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
   } else {
-    return common * lo_lt_pdf_f(yf[0], lt, kc.fprj, kc.ips) * prof;
+    return quad_f(kc, f, yf[0], yf[it]) * prof;
   }
 }
 
