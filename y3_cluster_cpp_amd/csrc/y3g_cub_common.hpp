@@ -133,7 +133,8 @@ template <> struct TabSel<double> {
   static __device__ const double* sci(const KernArgs& A) { return A.sci64; }
 };
 
-// Stage the tables of point pt into LDS.
+// Stage the tables of point pt into LDS (the HMF table only when s_hmf is
+// given).
 template <typename T>
 __device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
                              T* s_sci)
@@ -141,7 +142,8 @@ __device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
   const T* g_hmf = TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
   const T* g_da = TabSel<T>::da(A) + (size_t)pt * A.da_stride;
   const T* g_sci = TabSel<T>::sci(A) + (size_t)pt * A.sci_stride;
-  for (int k = threadIdx.x; k < A.hmf_stride; k += blockDim.x) s_hmf[k] = g_hmf[k];
+  if (s_hmf)
+    for (int k = threadIdx.x; k < A.hmf_stride; k += blockDim.x) s_hmf[k] = g_hmf[k];
   for (int k = threadIdx.x; k < A.da_stride; k += blockDim.x) s_da[k] = g_da[k];
   for (int k = threadIdx.x; k < A.sci_stride; k += blockDim.x) s_sci[k] = g_sci[k];
 }
@@ -253,7 +255,12 @@ struct ParentStage {
 static_assert(sizeof(ParentStage) == 2048, "ParentStage is 512 dwords");
 
 template <typename T, int BLOCK> struct CubShared {
-  T hmf[kMaxHmf];
+  // the HMF table: in LDS for the FP64 path; the FP32 path reads it only
+  // for the pair factors, from global memory (L2), which leaves room for three
+  // workgroups per CU
+  static constexpr bool kHmfLds = sizeof(T) == 8;
+  T hmf[kHmfLds ? kMaxHmf : 1];
+  const T* hmf_tab;  // the table the evaluation reads (LDS or global)
   T da[kMaxDa];
   T sci[kMaxSci];
   // node values: the FP32 path keeps only those the fourth differences read
@@ -625,7 +632,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
 {
   int const d = S.ig.ndim;
   int const nn = gm_num_nodes(d);
-  Tabs<T> tb{S.hmf, S.da, S.sci};
+  Tabs<T> tb{S.hmf_tab, S.da, S.sci};
   if constexpr (sizeof(T) == 4) {
     eval_pairs<T, BLOCK>(S, m, tb);
   } else {

@@ -684,9 +684,11 @@ Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   thi = thi < T(kTMax) ? thi : T(kTMax);
   f.tlo = tlo;
   f.dt = thi - tlo;
-  // HMF: clamped bilinear lookup (hmf_t.cuh:54-60), as hmf() below
-  const T* r0 = tb.hmf + z.jz * P.nm + m.im;
-  const T* r1 = r0 + P.nm;
+  // HMF: clamped bilinear lookup (hmf_t.cuh:54-60), as hmf() below; the
+  // table is in global memory for every caller
+  typedef const __attribute__((address_space(1))) T* GPtr;
+  GPtr r0 = (GPtr)(tb.hmf + z.jz * P.nm + m.im);
+  GPtr r1 = r0 + P.nm;
   T const a = r0[0] + m.tm * (r0[1] - r0[0]);
   T const b = r1[0] + m.tm * (r1[1] - r1[0]);
   T const hm = (a + z.sz * (b - a)) * m.hamp;

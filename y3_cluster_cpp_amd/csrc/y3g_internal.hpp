@@ -75,6 +75,8 @@ struct FunctorArgs {
 };
 
 size_t cubature_smem(int precision, int block);
+// Workgroups of k_cubature resident per CU (registers, LDS), 0 on error.
+int cubature_blocks_per_cu(int precision, int block);
 hipError_t launch_cubature(const KernArgs& A, int precision, int block,
                            int slots, hipStream_t s);
 hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,

@@ -90,9 +90,10 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
 }
 
 #ifndef Y3G_CUB_MIN_WAVES
-#define Y3G_CUB_MIN_WAVES 4
+#define Y3G_CUB_MIN_WAVES 6
 #endif
-// FP32: at most 128 VGPRs, four waves per SIMD; FP64: the compiler's choice
+// FP32: six waves per SIMD (at most 80 VGPRs), so that three 512-thread
+// workgroups share a CU (their LDS fits too); FP64: the compiler's choice
 template <typename T, int BLOCK>
 __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature(KernArgs A)
 {
@@ -126,8 +127,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
     __syncthreads();
     int const pt = S.ig.point;
     if (pt != S.point) {
-      if (tid == 0) S.P = A.points[pt];
-      stage_tables<T>(A, pt, S.hmf, S.da, S.sci);
+      if (tid == 0) {
+        S.P = A.points[pt];
+        S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
+      }
+      stage_tables<T>(A, pt, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
       __syncthreads();
       if (tid == 0) S.point = pt;
     }
@@ -689,6 +693,23 @@ static hipError_t launch_veg(const KernArgs& A, int slots, hipStream_t s)
   hipLaunchKernelGGL((k_vegas<T, BLOCK>), dim3(slots), dim3(BLOCK), smem, s,
                      A);
   return hipGetLastError();
+}
+
+int cubature_blocks_per_cu(int precision, int block)
+{
+  int n = 0;
+  size_t const smem = cubature_smem(precision, block);
+  const void* f =
+    precision == Y3G_FP64
+      ? (block == 256 ? reinterpret_cast<const void*>(&k_cubature<double, 256>)
+                      : reinterpret_cast<const void*>(&k_cubature<double, 512>))
+      : (block == 256 ? reinterpret_cast<const void*>(&k_cubature<float, 256>)
+                      : reinterpret_cast<const void*>(&k_cubature<float, 512>));
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) !=
+        hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, smem) != hipSuccess)
+    return 0;
+  return n;
 }
 
 size_t cubature_smem(int precision, int block)

@@ -539,8 +539,8 @@ static int resident_slots(const y3g_runtime* rt, const y3g_options& o, int algor
   int const block = o.block_threads == 256 ? 256 : 512;
   int per_cu = 2048 / block;
   if (algorithm == Y3G_PAGANI) {
-    size_t const smem = cubature_smem(o.precision, block);
-    per_cu = std::min(per_cu, std::max(1, (int)((160 * 1024) / (smem + 1024))));
+    int const n = cubature_blocks_per_cu(o.precision, block);
+    per_cu = n > 0 ? n : 1;
   }
   return rt->n_cu * per_cu;
 }

@@ -50,7 +50,9 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
     if constexpr (sizeof(T) == 4) pair_setup(S);
   }
   __syncthreads();
-  stage_tables<T>(A.K, S.ig.point, S.hmf, S.da, S.sci);
+  if (tid == 0)
+    S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A.K) + (size_t)S.ig.point * A.K.hmf_stride;
+  stage_tables<T>(A.K, S.ig.point, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
   __syncthreads();
   int const d = S.ig.ndim;
   int const nchunks = (nlist + kChunk - 1) / kChunk;
