@@ -274,14 +274,14 @@ template <typename T, int BLOCK> struct CubShared {
   double red[2 * (BLOCK / 64)];
   int ired[BLOCK / 64];
   int ired1[BLOCK / 64];
-  int ncount1;
   int hc[64];
   unsigned long long hq[64];
+  int wc[BLOCK / 64][64];  // per-wave error-bin counts
   int selB, lim0, lim1;
   double w7[6], w5[6], w3[6];
   y3g_integral ig;
   PointDev P;
-  int item, point, ncount;
+  int item, point;
   // FP32 path, per chunk: the pairs' factors and (Delta-Sigma) the profile
   // of each (lnM, u, theta) triple of the chunk's regions; the zt and lnM
   // factors they are made of (zmbuf: ZFac [kChunk][7], then MFac [kChunk][7])

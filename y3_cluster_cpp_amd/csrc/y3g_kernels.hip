@@ -201,12 +201,28 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       if (room <= 0) break;
       int eE;
       (void)frexp(E, &eE);
-      if (tid < 64) S.hc[tid] = 0, S.hq[tid] = 0ull;
+      // wave w takes regions [sb, se), contiguous and in wave order, so
+      // that index order is kept without block-wide steps: per-wave bin
+      // counts (summed into the histogram), then each wave's share of the
+      // selection from their exclusive prefix
+      constexpr int NW = BLOCK / 64;
+      int const lane = tid & 63, wv = tid >> 6;
+      int const seg = (N + NW - 1) / NW;
+      int const sb = min(N, wv * seg), se = min(N, sb + seg);
+      for (int k = tid; k < NW * 64; k += BLOCK) (&S.wc[0][0])[k] = 0;
+      if (tid < 64) S.hq[tid] = 0ull;
       __syncthreads();
-      for (int r = tid; r < N; r += BLOCK) {
-        int const b = err_bin(err[r], E, eE);
-        atomicAdd(&S.hc[b], 1);
-        atomicAdd(&S.hq[b], err_q(err[r], E));
+      for (int r = sb + lane; r < se; r += 64) {
+        double const e = err[r];
+        int const b = err_bin(e, E, eE);
+        atomicAdd(&S.wc[wv][b], 1);
+        atomicAdd(&S.hq[b], err_q(e, E));
+      }
+      __syncthreads();
+      if (tid < 64) {
+        int c = 0;
+        for (int v = 0; v < NW; ++v) c += S.wc[v][tid];
+        S.hc[tid] = c;
       }
       __syncthreads();
       if (tid == 0) select_bins(S.hc, S.hq, N, room, S.selB, S.lim0, S.lim1);
@@ -214,39 +230,36 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       int const B = S.selB, lim0 = S.lim0, lim1 = S.lim1;
       int const ns = lim0 + lim1;
       if (ns <= 0 || neval + 2LL * ns * nn > S.ig.max_eval) break;
-      // ordered compaction in one pass: bins < B in index order to
-      // sel[0, lim0), bin B in index order to sel[lim0, lim0 + lim1)
-      if (tid == 0) S.ncount = 0, S.ncount1 = 0;
-      __syncthreads();
-      for (int base = 0; base < N; base += BLOCK) {
-        int const r = base + tid;
-        bool f0 = false, f1 = false;
-        if (r < N) {
-          int const b = err_bin(err[r], E, eE);
-          f0 = b < B;
-          f1 = b == B;
-        }
-        unsigned long long const bal0 = __ballot(f0), bal1 = __ballot(f1);
-        int const lane = tid & 63, w = tid >> 6;
-        unsigned long long const below = (1ull << lane) - 1ull;
-        if (lane == 0) S.ired[w] = __popcll(bal0), S.ired1[w] = __popcll(bal1);
-        __syncthreads();
-        int off0 = S.ncount, off1 = S.ncount1;
-        for (int i = 0; i < w; ++i) off0 += S.ired[i], off1 += S.ired1[i];
-        off0 += __popcll(bal0 & below);
-        off1 += __popcll(bal1 & below);
-        if (f0 && off0 < lim0) S.sel[off0] = r;
-        if (f1 && off1 < lim1) S.sel[lim0 + off1] = r;
-        __syncthreads();
-        if (tid == 0) {
-          int t0 = 0, t1 = 0;
-          for (int i = 0; i < BLOCK / 64; ++i) t0 += S.ired[i], t1 += S.ired1[i];
-          S.ncount = min(S.ncount + t0, lim0);
-          S.ncount1 = min(S.ncount1 + t1, lim1);
-        }
-        __syncthreads();
-        if (S.ncount >= lim0 && S.ncount1 >= lim1) break;
+      // ordered compaction: bins < B in index order to sel[0, lim0), bin B
+      // in index order to sel[lim0, lim0 + lim1)
+      {
+        int c0 = 0;
+        for (int b = lane; b < B; b += 64) c0 += S.wc[wv][b];
+        c0 = wave_sum_i(c0);
+        if (lane == 0) S.ired[wv] = c0, S.ired1[wv] = S.wc[wv][B];
       }
+      __syncthreads();
+      {
+        int off0 = 0, off1 = 0;
+        for (int v = 0; v < wv; ++v) off0 += S.ired[v], off1 += S.ired1[v];
+        unsigned long long const below = (1ull << lane) - 1ull;
+        for (int base = sb; base < se && (off0 < lim0 || off1 < lim1); base += 64) {
+          int const r = base + lane;
+          bool f0 = false, f1 = false;
+          if (r < se) {
+            int const b = err_bin(err[r], E, eE);
+            f0 = b < B;
+            f1 = b == B;
+          }
+          unsigned long long const bal0 = __ballot(f0), bal1 = __ballot(f1);
+          int const o0 = off0 + __popcll(bal0 & below), o1 = off1 + __popcll(bal1 & below);
+          if (f0 && o0 < lim0) S.sel[o0] = r;
+          if (f1 && o1 < lim1) S.sel[lim0 + o1] = r;
+          off0 += __popcll(bal0);
+          off1 += __popcll(bal1);
+        }
+      }
+      __syncthreads();
       // split and evaluate, kChunk/2 pairs at a time; the next chunk's
       // parents are loaded while this one is evaluated
       double dI = 0.0, dE = 0.0;
