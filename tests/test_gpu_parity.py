@@ -153,6 +153,20 @@ def test_nfw_functors_match_oracle(runtime, prec, tol):
     assert np.max(np.abs(v - r) / np.maximum(np.abs(r), 1e-3 * np.abs(scale))) < 10 * tol
 
 
+def test_fp32_nfw_table_wide_range(runtime):
+    """The FP32 NFW shape (knot table) over x = R / r_s from ~1e-6 to ~5e4,
+    against the oracle's closed form: within 5e-6 relative (table 7e-7, the
+    halo amplitude's FP32 arithmetic the rest)."""
+    n = 2000
+    rho = 0.3 * 2.77533742639e11
+    R = np.geomspace(1e-6, 1e4, n)
+    lnM = np.full(n, 32.2)
+    v = runtime.eval_functor(y3.F_NFW_DSIGMA, np.c_[R, lnM, np.full(n, rho), np.zeros(n)],
+                             precision=y3.FP32SPLIT)
+    r = np.array([O.lib.y3o_nfw_dsigma(a, b, rho, 0.0) for a, b in zip(R, lnM)])
+    assert np.max(np.abs(v / r - 1)) < 5e-6
+
+
 @pytest.mark.parametrize("kind", [y3.COUNT3D, y3.COUNT5D, y3.DSIGMA6D])
 @pytest.mark.parametrize("prec,tol", [(y3.FP64, 1e-12), (y3.FP32SPLIT, 5e-5)])
 def test_mapped_integrand_matches_oracle(runtime, kind, prec, tol):
