@@ -53,12 +53,12 @@ FLOPS_PER_EVAL = {0: 55, 1: 80, 2: 165}
 SFU_PER_EVAL = {0: 7, 1: 9, 2: 24}
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # HBM bytes per integrand evaluation of k_cubature<float,512>: rocprofv3
-# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 256 MiB copy read
-# by the same counters) over the evaluations of the same launch, 16 points
-# (profiles/k_cubature_prior_stream_pmc.md: 29,429,698 KiB fetched,
-# 17,732,993 KiB written, 2.916e10 evaluations).  Region-slab traffic; the
-# integrand reads only LDS-resident tables.
-HBM_BYTES_PER_EVAL = (29429697.8 * 1.99974 + 17732993.4) * 1024 / 2.9159861584e10
+# FETCH_SIZE x 1.99974 + WRITE_SIZE x 1.000 (factors from a 256 MiB copy read
+# by the same counters) over the evaluations of the same launch, 16 points:
+# 54,249,404 KiB fetched, 25,815,057 KiB written, 2.9160e10 evaluations.
+# Region-slab passes, register spills of the step loop and the HMF table
+# reads of the pair factors; the node integrand reads only LDS.
+HBM_BYTES_PER_EVAL = (54249404.3 * 1.99974 + 25815057.3) * 1024 / 2.9159544810e10
 DEFAULT_REGIONS = 32768
 
 
