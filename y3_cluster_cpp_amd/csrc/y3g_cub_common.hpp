@@ -304,6 +304,7 @@ template <typename T, int BLOCK> struct CubShared {
   unsigned char qk[kMaxQuads];                             //   and its pair
   int nquads;
   unsigned char nquad[kMaxNodes];                          // quadruple of each node
+  unsigned short nqp[kMaxNodes];                           // quadruple | second << 8
   unsigned long long dv_zo, dv_mo, dv_np, dv_nt, dv_nq;    // div_magic of the counts
   unsigned char npair[kMaxNodes];
   unsigned char nclass[kMaxNodes];                    // gm_class of each node
@@ -403,6 +404,7 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
     }
   }
   S.nquads = nq;
+  for (int j = 0; j < nn; ++j) S.nqp[j] = (unsigned short)(S.nquad[j] | (S.nprof[j] << 8));
   S.dv_zo = div_magic(nz > 0 ? nz : 1);
   S.dv_mo = div_magic(nm > 0 ? nm : 1);
   S.dv_np = div_magic(np > 0 ? np : 1);
@@ -544,20 +546,41 @@ template <typename T, int BLOCK, int K>
 __device__ __forceinline__ void node_row_q(CubShared<T, BLOCK>& S, int r, int sub, double& a7,
                                            double& a5, double& a3)
 {
+  // the nodes class by class (centre, +-l2 and +-l3 per axis, the l4 pairs,
+  // the l5 corners: consecutive node indices), so that each loop has its
+  // class's rule weights as compile-time constants and skips the zero ones
   constexpr int d = KindAxes<K>::nd;
-  constexpr int nn = gm_num_nodes(d);
+  constexpr int c4 = 2 * d * (d - 1), c5 = 1 << d;
+  constexpr GmWeights W = gm_weights_of(d);
   const float* Qr = reinterpret_cast<const float*>(S.zmbuf) + r * kMaxQuads;
   const float* F2r = S.prof[r];
   T* fr = S.f + r * kFourNodes;
-  for (int j = sub; j < nn; j += 16) {
-    float const fv = Qr[S.nquad[j]] * F2r[S.nprof[j]];
-    if (j <= 4 * d) fr[j] = fv;
-    int const g = S.nclass[j];
+  auto value = [&](int j) {
+    unsigned const qp = S.nqp[j];
+    return Qr[qp & 0xFFu] * F2r[qp >> 8];
+  };
+  if (sub == 0) {
+    float const fv = value(0);
+    fr[0] = fv;
     double const fd = fv;
-    a7 += S.w7[g] * fd;
-    a5 += S.w5[g] * fd;
-    a3 += S.w3[g] * fd;
+    a7 += W.w7[1] * fd;
+    a5 += W.w5[1] * fd;
+    a3 += W.w3[1] * fd;
   }
+  if (sub < 2 * d) {
+    float const fv2 = value(1 + sub), fv3 = value(1 + 2 * d + sub);
+    fr[1 + sub] = fv2;
+    fr[1 + 2 * d + sub] = fv3;
+    double const f2 = fv2, f3 = fv3;
+    a7 += W.w7[2] * f2 + W.w7[3] * f3;
+    a5 += W.w5[2] * f2 + W.w5[3] * f3;
+    a3 += W.w3[3] * f3;
+  }
+  double s4 = 0.0, s5 = 0.0;
+  for (int k = sub; k < c4; k += 16) s4 += double(value(1 + 4 * d + k));
+  for (int k = sub; k < c5; k += 16) s5 += double(value(1 + 4 * d + c4 + k));
+  a7 += W.w7[4] * s4 + W.w7[5] * s5;
+  a5 += W.w5[4] * s4;
 }
 
 // FP32 nodes of region r for integrand kind K, by the 16 lanes of a row

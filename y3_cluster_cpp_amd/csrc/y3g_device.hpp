@@ -882,24 +882,30 @@ Y3G_HD double gm_decode(unsigned code, int a)
 // Weights per generator class (1..5) of the degree-7 rule, its embedded
 // degree-5 rule, and a degree-3 rule on the centre and the +-l3 axis nodes
 // (w3 = 1 / (6 l3^2) on those, 1 - 2d w3 at the centre).
+struct GmWeights {
+  double w7[6], w5[6], w3[6];
+};
+__host__ __device__ constexpr GmWeights gm_weights_of(int d)
+{
+  GmWeights w{};
+  double const dd = d;
+  w.w3[3] = 10.0 / 54.0;
+  w.w3[1] = 1.0 - 2.0 * dd * w.w3[3];
+  w.w7[1] = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
+  w.w7[2] = 980.0 / 6561.0;
+  w.w7[3] = (1820.0 - 400.0 * dd) / 19683.0;
+  w.w7[4] = 200.0 / 19683.0;
+  w.w7[5] = 6859.0 / 19683.0 / double(1 << d);
+  w.w5[1] = (729.0 - 950.0 * dd + 50.0 * dd * dd) / 729.0;
+  w.w5[2] = 245.0 / 486.0;
+  w.w5[3] = (265.0 - 100.0 * dd) / 1458.0;
+  w.w5[4] = 25.0 / 729.0;
+  return w;
+}
 __host__ __device__ inline void gm_weights(int d, double* w7, double* w5, double* w3)
 {
-  double const dd = d;
-  w7[0] = 0.0;
-  w5[0] = 0.0;
-  for (int g = 0; g < 6; ++g) w3[g] = 0.0;
-  w3[3] = 10.0 / 54.0;
-  w3[1] = 1.0 - 2.0 * dd * w3[3];
-  w7[1] = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
-  w7[2] = 980.0 / 6561.0;
-  w7[3] = (1820.0 - 400.0 * dd) / 19683.0;
-  w7[4] = 200.0 / 19683.0;
-  w7[5] = 6859.0 / 19683.0 / double(1 << d);
-  w5[1] = (729.0 - 950.0 * dd + 50.0 * dd * dd) / 729.0;
-  w5[2] = 245.0 / 486.0;
-  w5[3] = (265.0 - 100.0 * dd) / 1458.0;
-  w5[4] = 25.0 / 729.0;
-  w5[5] = 0.0;
+  GmWeights const w = gm_weights_of(d);
+  for (int g = 0; g < 6; ++g) w7[g] = w.w7[g], w5[g] = w.w5[g], w3[g] = w.w3[g];
 }
 
 // --------------------------------------------------------- Philox4x32-10
