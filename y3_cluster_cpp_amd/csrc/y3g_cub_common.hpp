@@ -282,6 +282,7 @@ template <typename T, int BLOCK> struct CubShared {
   y3g_integral ig;
   PointDev P;
   int item, point;
+  int setup_kind;  // kind x 16 + ndim the node maps (pair_setup) hold, -1 none
   // FP32 path, per chunk: the pairs' factors and (Delta-Sigma) the profile
   // of each (lnM, u, theta) triple of the chunk's regions; the zt and lnM
   // factors they are made of (zmbuf: ZFac [kChunk][7], then MFac [kChunk][7])

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   double* err = A.slab_err + (size_t)slot * cap;
   double* raw = A.slab_raw + (size_t)slot * cap;
   int8_t* axs = A.slab_ax + (size_t)slot * cap;
-  if (tid == 0) S.point = -1;
+  if (tid == 0) S.point = -1, S.setup_kind = -1;
   unsigned long long nev_total = 0;
   for (;;) {
     __syncthreads();
@@ -122,7 +122,12 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
     if (tid == 0) {
       S.ig = A.igs[iid];
       gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
-      if constexpr (sizeof(T) == 4) pair_setup(S);
+      // the node maps depend on the kind and dimension only; the queue's
+      // longest-first order keeps a workgroup on one kind for long stretches
+      if constexpr (sizeof(T) == 4) {
+        int const key = S.ig.kind * 16 + S.ig.ndim;
+        if (S.setup_kind != key) pair_setup(S), S.setup_kind = key;
+      }
     }
     __syncthreads();
     int const pt = S.ig.point;
