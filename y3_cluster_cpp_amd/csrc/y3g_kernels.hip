@@ -111,7 +111,6 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   double* raw = A.slab_raw + (size_t)slot * cap;
   int8_t* axs = A.slab_ax + (size_t)slot * cap;
   if (tid == 0) S.point = -1, S.setup_kind = -1;
-  unsigned long long nev_total = 0;
   for (;;) {
     __syncthreads();
     if (tid == 0) S.item = atomicAdd(A.queue, 1);
@@ -365,10 +364,9 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       res.nregions = N;
       res.status = status;
       A.out[iid] = res;
-      nev_total += (unsigned long long)neval;
+      atomicAdd(A.nevals, (unsigned long long)neval);
     }
   }
-  if (tid == 0 && nev_total) atomicAdd(A.nevals, nev_total);
 }
 
 // ------------------------------------------------------------------ VEGAS
