@@ -306,6 +306,7 @@ template <typename T, int BLOCK> struct CubShared {
   int nquads;
   unsigned char nquad[kMaxNodes];                          // quadruple of each node
   unsigned short nqp[kMaxNodes];                           // quadruple | second << 8
+  NfwKnot nfwtab[kF32 ? kNfwN : 1];                        // NFW knot table, LDS copy
   unsigned long long dv_zo, dv_mo, dv_np, dv_nt, dv_nq;    // div_magic of the counts
   unsigned char npair[kMaxNodes];
   unsigned char nclass[kMaxNodes];                    // gm_class of each node
@@ -438,13 +439,13 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
       int const qm = q - m * nzo;
       int const r = div_by(qm, S.dv_mo), k = qm - r * nmo;
       mf[r * kMaxAxisOffsets + k] =
-        m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k]);
+        m_factors<T>(S.ig, S.P, S.c[r][im] + S.h[r][im] * S.moffs[k], S.nfwtab);
     }
   }
   lds_barrier();
   int const np = S.npairs, nt = S.nprofs;
   int const ntp = S.ig.kind == Y3G_DSIGMA6D ? nt : 0;  // profiles
-  NodeConstF const kc = node_const_f(S.ig, S.P);
+  NodeConstF const kc = node_const_f(S.ig, S.P, S.nfwtab);
   for (int q = threadIdx.x; q < m * (np + ntp); q += BLOCK) {
     if (q < m * np) {
       int const r = div_by(q, S.dv_np), k = q - r * np;
@@ -679,7 +680,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
     if constexpr (sizeof(T) == 4) {
       if (live) {
-        NodeConstF const kc = node_const_f(S.ig, S.P);
+        NodeConstF const kc = node_const_f(S.ig, S.P, S.nfwtab);
         switch (S.ig.kind) {
           case Y3G_COUNT3D: node_row<T, BLOCK, Y3G_COUNT3D>(S, r, sub, kc, a7, a5, a3); break;
           case Y3G_COUNT5D: node_row_q<T, BLOCK, Y3G_COUNT5D>(S, r, sub, a7, a5, a3); break;

@@ -388,13 +388,13 @@ template <typename T> Y3G_HD void nfw_F_h(T x, T& F, T& h)
 // y3g_nfw_table.inc (fit_nfw_table.py): cubic Hermite interpolation of
 // v = log2(g (1 + x^2)) in s = log2(x^2), clamped to the table's ends
 // (x from 1e-6 to 6.5e4); relative error below 7e-7, no branches.
-__device__ __forceinline__ float nfw_g_tab(float x2)
+__device__ __forceinline__ float nfw_g_tab(float x2, const NfwKnot* tab)
 {
   float u = (__builtin_amdgcn_logf(x2) - kNfwS0) * kNfwInvStep;
   u = fminf(fmaxf(u, 0.0f), float(kNfwN - 1) - 1e-3f);
   int const i = int(u);
   float const t = u - float(i), t2 = t * t, t3 = t2 * t;
-  NfwKnot const a = kNfwTab[i], b = kNfwTab[i + 1];
+  NfwKnot const a = tab[i], b = tab[i + 1];
   float const v = (2.0f * t3 - 3.0f * t2 + 1.0f) * a.v + (t3 - 2.0f * t2 + t) * a.d +
                   (3.0f * t2 - 2.0f * t3) * b.v + (t3 - t2) * b.d;
   return __builtin_amdgcn_exp2f(v) * __builtin_amdgcn_rcpf(1.0f + x2);
@@ -403,7 +403,7 @@ __device__ __forceinline__ float nfw_g_tab(float x2)
 // Delta-Sigma / (rho_s r_s) of NFW (Wright & Brainerd 2000).
 template <typename T> Y3G_HD T nfw_g(T x)
 {
-  if constexpr (sizeof(T) == 4) return nfw_g_tab(x * x);
+  if constexpr (sizeof(T) == 4) return nfw_g_tab(x * x, kNfwTab);
   T F, h, m;
   nfw_F_h<T>(x, F, h);
   if (x < T(0.9)) {
@@ -450,7 +450,7 @@ template <typename T> Y3G_HD T nfw_dsigma(T R, const NfwHalo<T>& H)
 // R' (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:176-178), cos(2 beta).
 // FP32 form from 1 / r_s: R'^2 and x^2 = R'^2 / r_s^2 need no square root.
 __device__ __forceinline__ float dsigma_offset_f(float R, float rmis, float theta, float irs,
-                                                 float amp)
+                                                 float amp, const NfwKnot* tab)
 {
   float const hc = __cosf(0.5f * theta);
   float const k = hc * hc;
@@ -459,14 +459,14 @@ __device__ __forceinline__ float dsigma_offset_f(float R, float rmis, float thet
   if (!(rp2 > 1e-20f)) return 0.0f;
   float const a = dr + 2.0f * rmis * k;
   float const c2b = 2.0f * a * a * __builtin_amdgcn_rcpf(rp2) - 1.0f;
-  return amp * nfw_g_tab(rp2 * irs * irs) * c2b;
+  return amp * nfw_g_tab(rp2 * irs * irs, tab) * c2b;
 }
 
 template <typename T>
 Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 {
   if constexpr (sizeof(T) == 4)
-    return dsigma_offset_f(R, rmis, theta, __builtin_amdgcn_rcpf(H.rs), H.amp);
+    return dsigma_offset_f(R, rmis, theta, __builtin_amdgcn_rcpf(H.rs), H.amp, kNfwTab);
   // with k = cos^2(theta/2): R'^2 = (R - rmis)^2 + 4 R rmis k and
   // R + rmis cos(theta) = (R - rmis) + 2 rmis k, free of the cancellation
   // of the textbook form near R = rmis, theta = pi
@@ -624,7 +624,8 @@ Y3G_HD ZFac<T> z_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb, 
 }
 
 template <typename T>
-Y3G_HD MFac<T> m_factors(const y3g_integral& ig, const PointDev& P, double lnM)
+Y3G_HD MFac<T> m_factors(const y3g_integral& ig, const PointDev& P, double lnM,
+                         const NfwKnot* nfw)
 {
   MFac<T> m;
   m.lm = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor);
@@ -640,7 +641,12 @@ Y3G_HD MFac<T> m_factors(const y3g_integral& ig, const PointDev& P, double lnM)
     NfwHalo<T> const H = nfw_halo<T>(lnM, T(P.rho_m), T(0));
     m.irs = M<T>::div(T(1), H.rs);
     m.amp = H.amp;
-    m.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
+    if constexpr (sizeof(T) == 4) {
+      float const x = float(ig.radius) * m.irs;
+      m.cds = (1.0f - float(P.p.f_mis)) * (H.amp * nfw_g_tab(x * x, nfw));
+    } else {
+      m.cds = (T(1) - T(P.p.f_mis)) * nfw_dsigma<T>(T(ig.radius), H);
+    }
   }
   return m;
 }
@@ -677,7 +683,8 @@ template <typename T>
 Y3G_HD PairF<T> pair_factors(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
                              double zt, double lnM)
 {
-  return pair_combine<T>(ig, P, tb, z_factors<T>(ig, P, tb, zt), m_factors<T>(ig, P, lnM));
+  return pair_combine<T>(ig, P, tb, z_factors<T>(ig, P, tb, zt),
+                         m_factors<T>(ig, P, lnM, kNfwTab));
 }
 
 template <typename T, int K>
@@ -715,8 +722,9 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
 // chunk of regions and kept in registers across its nodes.
 struct NodeConstF {
   float sln, fprj, pscale, ips, sigz0, smis, fmis, radius, lo_lo, lo_hi;
+  const NfwKnot* nfw;  // the NFW knot table (LDS copy in the cubature)
 };
-Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
+Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P, const NfwKnot* nfw)
 {
   NodeConstF k;
   k.sln = float(P.p.mor_sigma_ln);
@@ -729,6 +737,7 @@ Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P)
   k.radius = float(ig.radius);
   k.lo_lo = float(ig.lo_lo);
   k.lo_hi = float(ig.lo_hi);
+  k.nfw = nfw;
   return k;
 }
 
@@ -759,7 +768,7 @@ Y3G_HD float profile_f(const NodeConstF& kc, float irs, float amp, float cds, fl
   T const L = M<T>::log(arg);
   T const l1mu = !low ? L : (arg == T(1) ? -u : L * (-u) * __builtin_amdgcn_rcpf(arg - T(1)));
   T const rmis = kc.smis * M<T>::sqrt(T(-2) * l1mu);
-  return cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, theta, irs, amp);
+  return cds + kc.fmis * dsigma_offset_f(kc.radius, rmis, theta, irs, amp, kc.nfw);
 }
 
 // The factor of the 5-D count and Delta-Sigma integrands on (lo, t, zt,

@@ -111,6 +111,8 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   double* raw = A.slab_raw + (size_t)slot * cap;
   int8_t* axs = A.slab_ax + (size_t)slot * cap;
   if (tid == 0) S.point = -1, S.setup_kind = -1;
+  if constexpr (sizeof(T) == 4)
+    for (int k = tid; k < kNfwN; k += BLOCK) S.nfwtab[k] = kNfwTab[k];
   for (;;) {
     __syncthreads();
     if (tid == 0) S.item = atomicAdd(A.queue, 1);
@@ -663,7 +665,7 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
       if constexpr (sizeof(T) == 4) {
         float yf[kMaxDim];
         for (int a = 0; a < kMaxDim; ++a) yf[a] = a < F.ig->ndim ? float(x[a]) : 0.0f;
-        NodeConstF const kc = node_const_f(*F.ig, P);
+        NodeConstF const kc = node_const_f(*F.ig, P, kNfwTab);
         PairC const pc = pair_c(pf);
         switch (F.ig->kind) {
           case Y3G_COUNT3D: r = integrand_node_f<Y3G_COUNT3D>(kc, pc, yf, 1.0f); break;

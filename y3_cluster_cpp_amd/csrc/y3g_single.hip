@@ -52,6 +52,8 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
   __syncthreads();
   if (tid == 0)
     S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A.K) + (size_t)S.ig.point * A.K.hmf_stride;
+  if constexpr (sizeof(T) == 4)
+    for (int k = tid; k < kNfwN; k += BLOCK) S.nfwtab[k] = kNfwTab[k];
   stage_tables<T>(A.K, S.ig.point, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
   __syncthreads();
   int const d = S.ig.ndim;
