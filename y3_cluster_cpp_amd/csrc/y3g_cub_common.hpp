@@ -687,7 +687,11 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
           default: node_row_q<T, BLOCK, Y3G_DSIGMA6D>(S, r, sub, a7, a5, a3); break;
         }
       }
-      lds_barrier();  // the rows' node values, for the fourth differences
+      // the fourth differences read node values written by lanes of the
+      // same row, hence of the same wave: a wave-level fence suffices
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
       if (live) {
         for (int j = sub; j < nn; j += 16) {
