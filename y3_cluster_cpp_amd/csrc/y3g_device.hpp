@@ -866,28 +866,6 @@ Y3G_HD double gm_offset(int d, int j, int a)
   return ((j >> a) & 1) ? -kL5 : kL5;
 }
 
-// Node j's unit offsets packed 3 bits per axis: bit 2 the sign, bits 0-1
-// the magnitude (0, l2, l3 = l4, l5); gm_decode inverts it.
-Y3G_HD unsigned gm_code(int d, int j)
-{
-  unsigned code = 0;
-  for (int a = 0; a < d; ++a) {
-    double const o = gm_offset(d, j, a);
-    double const m = o < 0.0 ? -o : o;
-    unsigned const c = m == 0.0 ? 0u : m == kL2 ? 1u : m == kL3 ? 2u : 3u;
-    code |= (c | (o < 0.0 ? 4u : 0u)) << (3 * a);
-  }
-  return code;
-}
-
-Y3G_HD double gm_decode(unsigned code, int a)
-{
-  unsigned const c = (code >> (3 * a)) & 7u;
-  unsigned const g = c & 3u;
-  double const m = g == 0u ? 0.0 : g == 1u ? kL2 : g == 2u ? kL3 : kL5;
-  return (c & 4u) ? -m : m;
-}
-
 // Weights per generator class (1..5) of the degree-7 rule, its embedded
 // degree-5 rule, and a degree-3 rule on the centre and the +-l3 axis nodes
 // (w3 = 1 / (6 l3^2) on those, 1 - 2d w3 at the centre).
