@@ -210,13 +210,38 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
 
 /* HBM bytes of the cubature region slab a batched y3g_integrate with these
  * options allocates on this runtime (resident workgroups x max_regions x
- * 113 B); the slab is per resident workgroup, so it does not grow with the
+ * 121 B); the slab is per resident workgroup, so it does not grow with the
  * number of integrals. */
 int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes);
 
 /* Time of the last y3g_integrate's kernels in milliseconds (HIP events on
  * the library's stream) and the number of integrand evaluations they made. */
 int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
+
+/* ---- Fixed Gauss-Legendre number counts with a selection function --------
+ * The reference's NumCountsSelGL (src/models/n_operator_sel_gl_t.hh:171-201
+ * over SelGLCore::build_weights, :88-118): per richness / redshift bin b,
+ *   N_b = sum_k w_k sum_q w_q dV/dz(z_q) Omega(z_q) hmf(lnM_k, z_q) S_b(lnM_k, z_q)
+ * on n_lnm x n_z Gauss-Legendre nodes over [lnm_lo, lnm_hi] x [zt_lo, zt_hi];
+ * Omega(z) is the reference's effective survey area (omega_z_des.hh:15-33),
+ * S_b the selection function (sel_function_t.hh:17-60), hmf and dV/dz the
+ * point's tables (y3g_tables_build / y3g_tables_upload). */
+typedef struct {
+  int n_lnm, n_z;            /* Gauss-Legendre nodes (reference default 96, 64) */
+  double lnm_lo, lnm_hi;     /* mass window (lnm_low, lnm_high) */
+  double zt_lo, zt_hi;       /* redshift window (zt_low, zt_high) */
+} y3g_selgl_spec;
+#define Y3G_SELGL_MAX_NODES 1024   /* n_lnm, n_z each at most this */
+/* The selection function of parameter point `point`: sel_function/lnM
+ * [n_lnm], z [n_z] (both increasing) and S_stack [n_bins][n_z][n_lnm]
+ * (sel_function_t.hh:44-60).  Replaces any earlier upload for the point. */
+int y3g_selfn_upload(y3g_runtime* rt, int point, const double* lnm, int n_lnm,
+                     const double* z, int n_z, const double* s_stack, int n_bins);
+/* N[point * max_bins + b] for points [0, n_points) and each point's bins
+ * b < its n_bins (entries beyond a point's bins are set to 0).  Every point
+ * needs tables and a selection function.  FP64 throughout. */
+int y3g_selgl_counts(y3g_runtime* rt, const y3g_selgl_spec* spec, int n_points,
+                     int max_bins, double* out);
 
 /* Evaluate a device functor on n inputs (tests of the model functors).
  * which: see y3g_functor; in[n * nin], out[n]. */

@@ -227,6 +227,17 @@ void y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int ndim,
 /* Philox4x32-10 (Salmon et al. 2011). */
 void y3o_philox(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Fixed Gauss-Legendre number counts with a selection function
+ * (n_operator_sel_gl_t.hh NumCountsSelGL; y3o_selgl.c). */
+double y3o_omega_z_des(double z);
+double y3o_sel_function(const double* lnm, int nm, const double* z, int nz,
+                        const double* plane, double lnM, double zt);
+void   y3o_gl_nodes_ab(int n, double a, double b, double* x, double* w);
+double y3o_selgl_counts(const y3o_params* p, const y3o_tables* t,
+                        const double* lnm, int nm, const double* z, int nz,
+                        const double* plane, int n_lnm, int n_z, double lnm_lo,
+                        double lnm_hi, double zt_lo, double zt_hi);
+
 /* Test integrands of the reference's integrator known-answer tests. */
 double y3o_test_fn(int which, const double* x, int ndim);
 

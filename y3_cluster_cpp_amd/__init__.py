@@ -70,6 +70,16 @@ class Options(C.Structure):
                 ("tl_c", C.c_double), ("tl_smin", C.c_double)]
 
 
+class SelGLSpec(C.Structure):
+    """y3g_selgl_spec: fixed Gauss-Legendre node counts and windows."""
+    _fields_ = [("n_lnm", C.c_int), ("n_z", C.c_int), ("lnm_lo", C.c_double),
+                ("lnm_hi", C.c_double), ("zt_lo", C.c_double), ("zt_hi", C.c_double)]
+
+
+def selgl_spec(lnm_lo, lnm_hi, zt_lo, zt_hi, n_lnm=96, n_z=64):
+    return SelGLSpec(n_lnm, n_z, lnm_lo, lnm_hi, zt_lo, zt_hi)
+
+
 class ServerOptions(C.Structure):
     _fields_ = [("lanes", C.c_int), ("max_batch_integrals", C.c_int),
                 ("hbm_budget_bytes", C.c_size_t), ("cache_entries", C.c_longlong),
@@ -122,6 +132,9 @@ SIGNATURES = {
     "y3g_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "y3g_server_destroy": (C.c_int, [C.c_void_p]),
     "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
+    "y3g_selfn_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD,
+                                   C.c_int]),
+    "y3g_selgl_counts": (C.c_int, [C.c_void_p, C.POINTER(SelGLSpec), C.c_int, C.c_int, _PD]),
     "y3g_eval_functor": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _PD, C.c_int,
                                    C.c_size_t, _PD, C.c_int,
                                    C.POINTER(Integral), _PD]),
@@ -290,6 +303,25 @@ class Runtime:
         o = opts if opts is not None else options()
         _check(lib().y3g_integrate(self._h, arr, n, C.byref(o),
                                    out.ctypes.data_as(C.c_void_p)), "y3g_integrate")
+        return out
+
+    def selfn_upload(self, point, lnm, z, s_stack):
+        """Selection function of a point: axes lnM [n_lnm], z [n_z] and
+        S_stack [n_bins, n_z, n_lnm] (y3g_selfn_upload)."""
+        lnm = np.ascontiguousarray(lnm, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        st = np.ascontiguousarray(s_stack, dtype=np.float64)
+        if st.ndim != 3 or st.shape[1] != z.size or st.shape[2] != lnm.size:
+            raise ValueError("S_stack must have shape (n_bins, n_z, n_lnm)")
+        _check(lib().y3g_selfn_upload(self._h, point, lnm.ctypes.data_as(_PD), lnm.size,
+                                      z.ctypes.data_as(_PD), z.size, st.ctypes.data_as(_PD),
+                                      st.shape[0]), "y3g_selfn_upload")
+
+    def selgl_counts(self, spec, n_points, max_bins):
+        """Fixed-GL number counts N[point, bin] (y3g_selgl_counts)."""
+        out = np.zeros((n_points, max_bins))
+        _check(lib().y3g_selgl_counts(self._h, C.byref(spec), n_points, max_bins,
+                                      out.ctypes.data_as(_PD)), "y3g_selgl_counts")
         return out
 
     def last_timing(self):

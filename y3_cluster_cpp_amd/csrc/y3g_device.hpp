@@ -817,6 +817,57 @@ Y3G_HD T integrand_pair(const y3g_integral& ig, const PointDev& P, const PairF<T
   }
 }
 
+// ------------------------------------------- fixed-GL counts with selection
+// Effective survey area Omega(z) in rad^2 (omega_z_des.hh:15-33): three
+// degree-5 polynomials, coefficients leading power first; the middle piece
+// in z - 0.6.
+Y3G_HD double omega_z_des(double z)
+{
+  double c[6];
+  double x = z;
+  if (z < 0.504) {
+    double const a[6] = {0.0, 0.0, 0.0, -0.00262353, 0.01940118, 0.45133063};
+    for (int i = 0; i < 6; ++i) c[i] = a[i];
+  } else if (z < 0.7) {
+    double const a[6] = {1.33647377e+4, 1.35291046e+3, -1.26204891e+2,
+                         -2.83454918e+1, -2.26465905, 3.84958753e-1};
+    for (int i = 0; i < 6; ++i) c[i] = a[i];
+    x = z - 0.6;
+  } else {
+    double const a[6] = {0.0, 0.0, -1.88101967, 4.8071839, -4.11424324, 1.18196785};
+    for (int i = 0; i < 6; ++i) c[i] = a[i];
+  }
+  double out = 0.0;
+  for (int i = 0; i < 6; ++i) out = c[i] + x * out;
+  return out;
+}
+
+// Largest i in [0, n - 2] with xs[i] <= x (xs increasing, x inside).
+Y3G_HD int bracket(const double* xs, int n, double x)
+{
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int const mid = (lo + hi) >> 1;
+    if (xs[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Selection function S(lnM, z) of one bin (sel_function_t.hh:28-35): zero
+// outside the axes, else bilinear in the (n_z, n_lnm) plane, lnM fastest.
+Y3G_HD double sel_function(const double* lnm, int nm, const double* z, int nz,
+                           const double* plane, double lnM, double zt)
+{
+  if (lnM < lnm[0] || lnM > lnm[nm - 1] || zt < z[0] || zt > z[nz - 1]) return 0.0;
+  int const i = bracket(lnm, nm, lnM), j = bracket(z, nz, zt);
+  double const t = (lnM - lnm[i]) / (lnm[i + 1] - lnm[i]);
+  double const u = (zt - z[j]) / (z[j + 1] - z[j]);
+  const double* r0 = plane + j * nm + i;
+  const double* r1 = r0 + nm;
+  return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r0[1] + (1.0 - t) * u * r1[0] +
+         t * u * r1[1];
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

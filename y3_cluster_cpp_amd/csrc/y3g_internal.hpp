@@ -74,6 +74,27 @@ struct FunctorArgs {
   double* out;
 };
 
+// Fixed-GL counts with a selection function: per point, its selection
+// function (axes and planes) at offsets into one FP64 buffer.
+struct SelPoint {
+  long long off_lnm, off_z, off_s;  // offsets into SelArgs::data
+  int n_lnm, n_z, n_bins;
+};
+struct SelGLArgs {
+  const PointDev* points;
+  const double* hmf64;
+  int hmf_stride;
+  const double* da64;
+  int da_stride;
+  const SelPoint* sel;
+  const double* data;
+  const double* nodes;  // lnM nodes [n_lnm], weights [n_lnm], z nodes [n_z], weights [n_z]
+  int n_lnm, n_z;
+  int max_bins;
+  double* out;          // [n_points][max_bins]
+};
+hipError_t launch_selgl(const SelGLArgs& A, int n_points, hipStream_t s);
+
 size_t cubature_smem(int precision, int block);
 // Workgroups of k_cubature resident per CU (registers, LDS), 0 on error.
 int cubature_blocks_per_cu(int precision, int block);

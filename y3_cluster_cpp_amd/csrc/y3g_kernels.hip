@@ -686,6 +686,53 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
   F.out[i] = double(r);
 }
 
+// ------------------------------------------- fixed-GL counts with selection
+// One workgroup per (point, bin): the z-only factors w_q dV/dz Omega of the
+// n_z redshift nodes into LDS, then the n_lnm x n_z terms
+// w_k zfac_q hmf(lnM_k, z_q) S_b(lnM_k, z_q), summed in FP64
+// (n_operator_sel_gl_t.hh:88-118).
+constexpr int kSelBlock = 256;
+__global__ __launch_bounds__(kSelBlock) void k_selgl(SelGLArgs A)
+{
+  __shared__ double zfac[Y3G_SELGL_MAX_NODES];
+  __shared__ double red[2 * (kSelBlock / 64)];
+  int const pt = blockIdx.x / A.max_bins, b = blockIdx.x - pt * A.max_bins;
+  SelPoint const sp = A.sel[pt];
+  if (b >= sp.n_bins) {
+    if (threadIdx.x == 0) A.out[blockIdx.x] = 0.0;
+    return;
+  }
+  PointDev const& P = A.points[pt];
+  const double* htab = A.hmf64 + (size_t)pt * A.hmf_stride;
+  const double* da = A.da64 + (size_t)pt * A.da_stride;
+  const double* xm = A.nodes;
+  const double* wm = xm + A.n_lnm;
+  const double* xz = wm + A.n_lnm;
+  const double* wz = xz + A.n_z;
+  for (int q = threadIdx.x; q < A.n_z; q += kSelBlock)
+    zfac[q] = wz[q] * dv_do_dz<double>(P, da, xz[q]) * omega_z_des(xz[q]);
+  __syncthreads();
+  const double* lnm = A.data + sp.off_lnm;
+  const double* zs = A.data + sp.off_z;
+  const double* plane = A.data + sp.off_s + (size_t)b * sp.n_z * sp.n_lnm;
+  double acc = 0.0;
+  int const nt = A.n_lnm * A.n_z;
+  for (int t = threadIdx.x; t < nt; t += kSelBlock) {
+    int const k = t / A.n_z, q = t - k * A.n_z;
+    double const lnM = xm[k], z = xz[q];
+    acc += wm[k] * zfac[q] * hmf<double>(P, htab, lnM, z) *
+           sel_function(lnm, sp.n_lnm, zs, sp.n_z, plane, lnM, z);
+  }
+  acc = block_sum<kSelBlock>(acc, red);
+  if (threadIdx.x == 0) A.out[blockIdx.x] = acc;
+}
+
+hipError_t launch_selgl(const SelGLArgs& A, int n_points, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_selgl, dim3(n_points * A.max_bins), dim3(kSelBlock), 0, s, A);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------- launchers
 template <typename T, int BLOCK>
 static hipError_t launch_cub(const KernArgs& A, int slots, hipStream_t s)

@@ -52,6 +52,45 @@ template <typename T> void dfree(T*& p)
   p = nullptr;
 }
 
+// n Gauss-Legendre nodes and weights on [a, b]: Newton iteration on the
+// Legendre polynomial P_n from the Chebyshev-like first guess
+// cos(pi (i + 3/4) / (n + 1/2)), weights 2 / ((1 - x^2) P_n'(x)^2).
+void gl_nodes(int n, double a, double b, double* x, double* w)
+{
+  double const c = 0.5 * (a + b), h = 0.5 * (b - a);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double t = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        double const p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p1 = t, p0 = 1.0;
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      double const dt = p1 / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-16) break;
+    }
+    {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        double const p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p1 = t, p0 = 1.0;
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+    }
+    double const wi = 2.0 / ((1.0 - t * t) * dp * dp);
+    x[i] = c - h * t;
+    x[n - 1 - i] = c + h * t;
+    w[i] = w[n - 1 - i] = h * wi;
+  }
+}
+
 }  // namespace
 
 struct y3g_runtime {
@@ -88,6 +127,19 @@ struct y3g_runtime {
   SingleArgs sg{};
   int sg_cap = 0;
   y3g_integral* d_one = nullptr;
+  // fixed-GL counts: selection functions per point (host copies, uploaded
+  // as one buffer when changed), nodes and results
+  struct HostSel {
+    std::vector<double> lnm, z, s;
+    int n_bins = 0;
+  };
+  std::vector<HostSel> sel;
+  bool sel_dirty = true;
+  double* d_seldata = nullptr;
+  SelPoint* d_selpts = nullptr;
+  double* d_selnodes = nullptr;
+  double* d_selout = nullptr;
+  size_t selout_cap = 0;
 };
 
 extern "C" {
@@ -332,6 +384,7 @@ int y3g_runtime_destroy(y3g_runtime* rt)
   dfree(rt->sg.ax), dfree(rt->sg.list), dfree(rt->sg.sel), dfree(rt->sg.pest);
   dfree(rt->sg.perr), dfree(rt->sg.partial), dfree(rt->sg.IE), dfree(rt->sg.hc);
   dfree(rt->sg.hq), dfree(rt->sg.blockcnt), dfree(rt->sg.blockoff), dfree(rt->d_one);
+  dfree(rt->d_seldata), dfree(rt->d_selpts), dfree(rt->d_selnodes), dfree(rt->d_selout);
   if (rt->ev0) (void)hipEventDestroy(rt->ev0);
   if (rt->ev1) (void)hipEventDestroy(rt->ev1);
   if (rt->stream) (void)hipStreamDestroy(rt->stream);
@@ -787,6 +840,108 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   cleanup();
   if (e != hipSuccess)
     return fail(Y3G_ERR_HIP, std::string("y3g_eval_functor: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int y3g_selfn_upload(y3g_runtime* rt, int point, const double* lnm, int n_lnm,
+                     const double* z, int n_z, const double* s_stack, int n_bins)
+{
+  if (!rt || point < 0 || !lnm || !z || !s_stack || n_lnm < 2 || n_z < 2 || n_bins < 1)
+    return fail(Y3G_ERR_ARG, "y3g_selfn_upload: bad arguments");
+  for (int i = 1; i < n_lnm; ++i)
+    if (!(lnm[i] > lnm[i - 1])) return fail(Y3G_ERR_ARG, "y3g_selfn_upload: lnM axis not increasing");
+  for (int i = 1; i < n_z; ++i)
+    if (!(z[i] > z[i - 1])) return fail(Y3G_ERR_ARG, "y3g_selfn_upload: z axis not increasing");
+  if ((size_t)point >= rt->sel.size()) rt->sel.resize((size_t)point + 1);
+  auto& h = rt->sel[point];
+  h.lnm.assign(lnm, lnm + n_lnm);
+  h.z.assign(z, z + n_z);
+  h.s.assign(s_stack, s_stack + (size_t)n_bins * n_z * n_lnm);
+  h.n_bins = n_bins;
+  rt->sel_dirty = true;
+  return 0;
+}
+
+int y3g_selgl_counts(y3g_runtime* rt, const y3g_selgl_spec* spec, int n_points,
+                     int max_bins, double* out)
+{
+  if (!rt || !spec || !out || n_points < 1 || max_bins < 1)
+    return fail(Y3G_ERR_ARG, "y3g_selgl_counts: bad arguments");
+  if (spec->n_lnm < 1 || spec->n_lnm > Y3G_SELGL_MAX_NODES || spec->n_z < 1 ||
+      spec->n_z > Y3G_SELGL_MAX_NODES || !(spec->lnm_hi > spec->lnm_lo) ||
+      !(spec->zt_hi > spec->zt_lo))
+    return fail(Y3G_ERR_ARG, "y3g_selgl_counts: bad node counts or windows");
+  if (n_points > rt->n_points)
+    return fail(Y3G_ERR_STATE, "y3g_selgl_counts: points without tables");
+  if ((size_t)n_points > rt->sel.size())
+    return fail(Y3G_ERR_STATE, "y3g_selgl_counts: points without a selection function");
+  for (int p = 0; p < n_points; ++p)
+    if (rt->sel[p].n_bins < 1)
+      return fail(Y3G_ERR_STATE, "y3g_selgl_counts: points without a selection function");
+  HIPCHK(hipSetDevice(rt->device));
+  if (rt->sel_dirty) {
+    std::vector<double> data;
+    std::vector<SelPoint> pts(rt->sel.size());
+    for (size_t p = 0; p < rt->sel.size(); ++p) {
+      auto const& h = rt->sel[p];
+      SelPoint& sp = pts[p];
+      sp.n_lnm = int(h.lnm.size());
+      sp.n_z = int(h.z.size());
+      sp.n_bins = h.n_bins;
+      sp.off_lnm = (long long)data.size();
+      data.insert(data.end(), h.lnm.begin(), h.lnm.end());
+      sp.off_z = (long long)data.size();
+      data.insert(data.end(), h.z.begin(), h.z.end());
+      sp.off_s = (long long)data.size();
+      data.insert(data.end(), h.s.begin(), h.s.end());
+    }
+    dfree(rt->d_seldata);
+    dfree(rt->d_selpts);
+    if (int rc = dalloc(&rt->d_seldata, data.size())) return rc;
+    if (int rc = dalloc(&rt->d_selpts, pts.size())) return rc;
+    HIPCHK(hipMemcpy(rt->d_seldata, data.data(), sizeof(double) * data.size(),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(rt->d_selpts, pts.data(), sizeof(SelPoint) * pts.size(),
+                     hipMemcpyHostToDevice));
+    rt->sel_dirty = false;
+  }
+  std::vector<double> nodes(2 * (size_t)(spec->n_lnm + spec->n_z));
+  gl_nodes(spec->n_lnm, spec->lnm_lo, spec->lnm_hi, nodes.data(), nodes.data() + spec->n_lnm);
+  gl_nodes(spec->n_z, spec->zt_lo, spec->zt_hi, nodes.data() + 2 * spec->n_lnm,
+           nodes.data() + 2 * spec->n_lnm + spec->n_z);
+  dfree(rt->d_selnodes);
+  if (int rc = dalloc(&rt->d_selnodes, nodes.size())) return rc;
+  size_t const nout = (size_t)n_points * max_bins;
+  if (nout > rt->selout_cap) {
+    dfree(rt->d_selout);
+    if (int rc = dalloc(&rt->d_selout, nout)) return rc;
+    rt->selout_cap = nout;
+  }
+  HIPCHK(hipMemcpyAsync(rt->d_selnodes, nodes.data(), sizeof(double) * nodes.size(),
+                        hipMemcpyHostToDevice, rt->stream));
+  SelGLArgs A{};
+  A.points = rt->points;
+  A.hmf64 = rt->hmf64;
+  A.hmf_stride = rt->hmf_stride;
+  A.da64 = rt->da64;
+  A.da_stride = rt->da_stride;
+  A.sel = rt->d_selpts;
+  A.data = rt->d_seldata;
+  A.nodes = rt->d_selnodes;
+  A.n_lnm = spec->n_lnm;
+  A.n_z = spec->n_z;
+  A.max_bins = max_bins;
+  A.out = rt->d_selout;
+  HIPCHK(hipEventRecord(rt->ev0, rt->stream));
+  HIPCHK(launch_selgl(A, n_points, rt->stream));
+  HIPCHK(hipEventRecord(rt->ev1, rt->stream));
+  HIPCHK(hipMemcpyAsync(out, rt->d_selout, sizeof(double) * nout, hipMemcpyDeviceToHost,
+                        rt->stream));
+  HIPCHK(hipStreamSynchronize(rt->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
+  rt->last_ms = ms;
+  rt->last_nevals = (long long)nout * spec->n_lnm * spec->n_z;
   return 0;
 }
 
