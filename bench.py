@@ -43,8 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from y3_cluster_cpp_amd.workloads import (  # noqa: E402
-    LO_EDGES, N_COUNTS, N_PER_POINT, RADII, SEED, ZO_EDGES, count_ok, eps_stream,
-    mcmc_requests, point_integrals, point_params)
+    LO_EDGES, N_COUNTS, N_PER_POINT, RADII, SEED, SEL_BINS, SEL_GL, ZO_EDGES, count_ok,
+    eps_stream, mcmc_requests, point_integrals, point_params, prior_points, selection_stack)
 
 # FP32 operations per integrand evaluation of the device integrands
 # (+ - * / = 1, FMA = 2) and transcendental (SFU) calls, counted from
@@ -60,6 +60,10 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # reads of the pair factors; the node integrand reads only LDS.
 HBM_BYTES_PER_EVAL = (54249404.3 * 1.99974 + 25815057.3) * 1024 / 2.9159544810e10
 DEFAULT_REGIONS = 32768
+# FP64 operations per term of k_selgl (HMF bilinear 10, selection bilinear
+# 13, the product and the sum 4), from csrc/y3g_kernels.hip.
+SELGL_FLOPS_PER_TERM = 27
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 vector, FLOP/s
 
 
 def load_golden():
@@ -197,6 +201,95 @@ def serving_run(y3, pts, ref, eps_all, rates, seconds, opts, lanes):
                         "(log-uniform [1e-5, 1e-3]); counted with the headline rule"}
 
 
+def selgl_run(y3, rt, spec, n_points, reps, cpu_seconds, with_cpu):
+    """Fixed Gauss-Legendre number counts with a selection function (the
+    NumCountsSel module's device work): the first n_points parameter points
+    of the prior stream, each with its own tables and a 12-bin selection
+    function on the reference's default grids, 96 x 64 GL nodes; one launch
+    computes every (point, bin).  Also timed: one point per call, the
+    module's shape (one sample per execute).  Parity: every bin of the first
+    points against the oracle (FP64 both sides)."""
+    import oracle as O
+    pts = prior_points(n_points)
+    rng = np.random.default_rng(SEED + 5)
+    params = [point_params(y3, pt) for pt in pts[:n_points]]
+    rt.tables_build(params, spec)
+    stacks = [selection_stack(rng) for _ in range(n_points)]
+    for k, (lnm, z, st) in enumerate(stacks):
+        rt.selfn_upload(k, lnm, z, st)
+    gl = y3.selgl_spec(*SEL_GL)
+    rt.selgl_counts(gl, n_points, SEL_BINS)  # untimed: uploads
+    kms, walls = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        got = rt.selgl_counts(gl, n_points, SEL_BINS)
+        walls.append(time.perf_counter() - t0)
+        kms.append(rt.last_timing()[0])
+    kms, walls = np.array(kms), np.array(walls)
+    one_k, one_w = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rt.selgl_counts(gl, 1, SEL_BINS)
+        one_w.append(time.perf_counter() - t0)
+        one_k.append(rt.last_timing()[0])
+
+    otabs = {}
+
+    def oracle_bin(k, b):
+        if k not in otabs:
+            pt = pts[k]
+            op = O.default_params(omega_m=pt[0], hmf_amp=pt[1], mor_slope=pt[2],
+                                  mor_sigma_ln=pt[3], f_mis=pt[4])
+            otabs[k] = (op, O.OracleTables(op, O.spec_wide()),
+                        [np.ascontiguousarray(p.reshape(-1)) for p in stacks[k][2]])
+        op, ot, planes = otabs[k]
+        lnm, z, _ = stacks[k]
+        return O.lib.y3o_selgl_counts(C.byref(op), C.byref(ot.t), O.ptr(lnm), len(lnm),
+                                      O.ptr(z), len(z), O.ptr(planes[b]), *SEL_GL[4:],
+                                      *SEL_GL[:4])
+
+    check = min(n_points, 4)
+    rel = max(abs(got[k, b] / oracle_bin(k, b) - 1) for k in range(check)
+              for b in range(SEL_BINS))
+    terms = n_points * SEL_BINS * SEL_GL[4] * SEL_GL[5]
+    k_avg = float(kms.mean())
+    achieved = terms * SELGL_FLOPS_PER_TERM / (k_avg * 1e-3)
+    cpu = None
+    if with_cpu:
+        # the oracle over (point, bin) in order, one thread, ~cpu_seconds;
+        # its tables are built before the clock starts
+        ncpu = min(n_points, 16)
+        for k in range(ncpu):
+            oracle_bin(k, 0)
+        n_done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            k, b = divmod(n_done % (ncpu * SEL_BINS), SEL_BINS)
+            oracle_bin(k, b)
+            n_done += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": n_done / dt, "unit": "bin counts/s", "cores": 1, "kind": "port",
+               "sample": f"{n_done} (point, bin) counts of the same workload by the oracle "
+                         f"(tables prebuilt), {dt:.1f} s"}
+    return {
+        "workload": f"{n_points} parameter points x {SEL_BINS} bins, selection grids "
+                    f"40 x 20, {SEL_GL[4]} x {SEL_GL[5]} GL nodes, FP64",
+        "value": n_points * SEL_BINS / (k_avg * 1e-3), "unit": "bin counts/s",
+        "kernel_ms": k_avg, "kernel_ms_p99": float(np.percentile(kms, 99)),
+        "call_ms_p50": float(np.median(walls) * 1e3),
+        "call_ms_p99": float(np.percentile(walls, 99) * 1e3),
+        "samples_per_s_per_call": n_points / float(np.median(walls)),
+        "one_point": {"kernel_ms": float(np.mean(one_k)),
+                      "call_ms_p50": float(np.median(one_w) * 1e3),
+                      "call_ms_p99": float(np.percentile(one_w, 99) * 1e3)},
+        "reps": reps, "max_rel_vs_oracle": float(rel), "checked_bins": check * SEL_BINS,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "kernel": "k_selgl", "terms_per_launch": terms,
+                     "flops_per_term": SELGL_FLOPS_PER_TERM},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +311,12 @@ def main():
     ap.add_argument("--serve-rates", default="4,12,24")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--no-selgl", action="store_true")
+    ap.add_argument("--selgl-only", action="store_true",
+                    help="run the fixed-GL selection counts leg alone and print its object")
+    ap.add_argument("--selgl-points", type=int, default=1024)
+    ap.add_argument("--selgl-reps", type=int, default=200)
+    ap.add_argument("--selgl-cpu-seconds", type=float, default=5.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (calibrates FETCH_SIZE / "
                          "WRITE_SIZE counters) and exit")
@@ -240,13 +339,19 @@ def main():
 
     import y3_cluster_cpp_amd as y3
     pts, ref = load_golden()
+    spec = y3.spec_wide()
+    if a.selgl_only:
+        rt = y3.Runtime(0)
+        print(json.dumps(selgl_run(y3, rt, spec, a.selgl_points, a.selgl_reps,
+                                   a.selgl_cpu_seconds, not a.no_cpu)))
+        rt.close()
+        return
     B = a.points_per_step
     need = B * max(a.steps, 1)
     if need > len(pts):
         raise SystemExit(f"{need} points needed, {len(pts)} have oracle values")
     eps_raw = eps_stream(len(pts))
     eps = np.minimum(eps_raw, a.eps_cap)
-    spec = y3.spec_wide()
     params_all = [point_params(y3, pt) for pt in pts]
     rt = y3.Runtime(0)
     info = rt.device_info()
@@ -358,6 +463,13 @@ def main():
         serving = serving_run(y3, pts, ref, eps_raw, rates, a.serve_seconds, opts,
                               a.serve_lanes)
 
+    selgl = None
+    if not a.no_selgl:
+        selgl = selgl_run(y3, rt, spec, a.selgl_points, a.selgl_reps,
+                          a.selgl_cpu_seconds, not a.no_cpu)
+        print(f"fixed-GL counts: {selgl['value']:.3g} bin counts/s", file=sys.stderr,
+              flush=True)
+
     achieved = flops / (kern_ms * 1e-3) if kern_ms > 0 else 0.0
     out = {
         "metric": "integrals/sec at rel err <=1e-4 vs FP64 oracle; p50/p99 request latency",
@@ -393,6 +505,7 @@ def main():
         "speedup_vs_plain_fp64": (value / plain["counted_per_s"])
         if plain and plain["counted_per_s"] > 0 else None,
         "serving": serving,
+        "fixed_gl_counts": selgl,
     }
     print(json.dumps(out))
 

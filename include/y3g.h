@@ -25,6 +25,10 @@
  *   y3g_result              numint::integration_result fields used at
  *                           CosmoSISSICUDAModule.hh:299-309
  *                           (estimate, errorest, chi_sq, neval, nregions, status)
+ *   y3g_selfn_upload        SelFunction_t's sample reads, src/models/sel_function_t.hh:28-80
+ *   y3g_selgl_counts        NumCountsSelGL::set_sample -> SelGLCore::build_weights,
+ *                           src/models/n_operator_sel_gl_t.hh:88-150, 184-192
+ *                           (all bins of all points in one launch)
  *   y3g_eval_functor        the host-side functor calls of the reference's
  *                           Catch2 tests (e.g. test/roffset_t.test.cc:40-46),
  *                           here evaluated by the device functors

@@ -108,6 +108,13 @@ def test_module_setup_without_device_returns_null():
         cfg.put(L, var + "_high", [float(hi)])
     assert not m.setup(cfg)
     assert m.execute(cs.DataBlock()) != 0
+    ev = cs.Module("NumCountsSel")
+    cfg = cs.DataBlock()
+    for k, v in (("zt_low", 0.1), ("zt_high", 0.8), ("lnm_low", 30.0), ("lnm_high", 35.0)):
+        cfg.put("NumCountsSel", k, v)
+    cfg.put("NumCountsSel", "bin_index", [0.0, 1.0])
+    assert not ev.setup(cfg)
+    assert ev.execute(cs.DataBlock()) != 0
 
 
 def test_datablock_library_exports_c_api():

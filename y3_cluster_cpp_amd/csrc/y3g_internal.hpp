@@ -91,6 +91,7 @@ struct SelGLArgs {
   const double* nodes;  // lnM nodes [n_lnm], weights [n_lnm], z nodes [n_z], weights [n_z]
   int n_lnm, n_z;
   int max_bins;
+  int max_axes;         // largest n_lnm + n_z of the launch's selection functions
   double* out;          // [n_points][max_bins]
 };
 hipError_t launch_selgl(const SelGLArgs& A, int n_points, hipStream_t s);

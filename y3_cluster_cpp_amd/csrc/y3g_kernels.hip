@@ -687,15 +687,47 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
 }
 
 // ------------------------------------------- fixed-GL counts with selection
-// One workgroup per (point, bin): the z-only factors w_q dV/dz Omega of the
-// n_z redshift nodes into LDS, then the n_lnm x n_z terms
-// w_k zfac_q hmf(lnM_k, z_q) S_b(lnM_k, z_q), summed in FP64
-// (n_operator_sel_gl_t.hh:88-118).
+// One workgroup per (point, bin), summing the n_lnm x n_z terms
+// w_k zfac_q hmf(lnM_k, z_q) S_b(lnM_k, z_q) in FP64
+// (n_operator_sel_gl_t.hh:88-118).  Everything that depends on one node
+// coordinate only is computed once per workgroup into LDS: per mass node its
+// weight, HMF cell / fraction / amplitude factor and selection-axis cell /
+// fraction; per redshift node w_q dV/dz Omega (zfac) and its two cells.  A
+// term is then two bilinear lookups (HMF table, selection plane: 8 loads,
+// both L2-resident) and a few FMAs, with the arithmetic of hmf() and
+// sel_function() kept in the same order as the direct per-node evaluation.
+struct SelNodeM {
+  double w, ht, hfac, st;  // GL weight; HMF lnM fraction, amplitude; sel fraction
+  int hi, si;              // HMF lnM cell; sel lnM cell (-1: outside the axis)
+};
+struct SelNodeZ {
+  double zf, hs, su;       // w_q dV/dz Omega; HMF z fraction; sel z fraction
+  int hj, sj;              // HMF z cell; sel z cell (-1: outside the axis)
+};
 constexpr int kSelBlock = 256;
+// The point's selection axes are copied to LDS (one round trip, before the
+// node descriptors, whose bracket searches then read LDS) when the launch's
+// longest axes fit this many doubles (the reference's default grids take
+// 60); longer ones are searched in global memory.  The bin's plane stays in
+// global memory: its lookups gather across lanes (a wave's lanes share the
+// mass node and step through the redshift cells), which the L1 cache serves
+// without the bank conflicts the same gathers meet in LDS.
+constexpr int kSelAxesMax = 1024;
+
+static size_t selgl_smem(int n_lnm, int n_z, int stage_doubles)
+{
+  return sizeof(SelNodeM) * (size_t)n_lnm + sizeof(SelNodeZ) * (size_t)n_z +
+         sizeof(double) * (size_t)stage_doubles;
+}
+
+template <bool kStage>
 __global__ __launch_bounds__(kSelBlock) void k_selgl(SelGLArgs A)
 {
-  __shared__ double zfac[Y3G_SELGL_MAX_NODES];
+  extern __shared__ __align__(16) unsigned char sel_smem[];
   __shared__ double red[2 * (kSelBlock / 64)];
+  SelNodeM* nm = reinterpret_cast<SelNodeM*>(sel_smem);
+  SelNodeZ* nz = reinterpret_cast<SelNodeZ*>(nm + A.n_lnm);
+  double* stage = reinterpret_cast<double*>(nz + A.n_z);
   int const pt = blockIdx.x / A.max_bins, b = blockIdx.x - pt * A.max_bins;
   SelPoint const sp = A.sel[pt];
   if (b >= sp.n_bins) {
@@ -709,19 +741,89 @@ __global__ __launch_bounds__(kSelBlock) void k_selgl(SelGLArgs A)
   const double* wm = xm + A.n_lnm;
   const double* xz = wm + A.n_lnm;
   const double* wz = xz + A.n_z;
-  for (int q = threadIdx.x; q < A.n_z; q += kSelBlock)
-    zfac[q] = wz[q] * dv_do_dz<double>(P, da, xz[q]) * omega_z_des(xz[q]);
-  __syncthreads();
+  int const npl = sp.n_z * sp.n_lnm;
   const double* lnm = A.data + sp.off_lnm;
   const double* zs = A.data + sp.off_z;
-  const double* plane = A.data + sp.off_s + (size_t)b * sp.n_z * sp.n_lnm;
+  const double* plane = A.data + sp.off_s + (size_t)b * npl;
+  if constexpr (kStage) {
+    // [lnM axis | z axis], adjacent in global memory too
+    const double* ax = lnm;
+    int const nax = sp.n_lnm + sp.n_z;
+    for (int i = threadIdx.x; i < nax; i += kSelBlock) stage[i] = ax[i];
+    lnm = stage;
+    zs = lnm + sp.n_lnm;
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < A.n_lnm; k += kSelBlock) {
+    double const lnM = xm[k];
+    SelNodeM d;
+    d.w = wm[k];
+    double u = (lnM - P.lnm0) * P.inv_dlnm;  // hmf(): lnM axis
+    u = u < 0.0 ? 0.0 : (u > double(P.nm - 1) ? double(P.nm - 1) : u);
+    int i = int(u);
+    if (i > P.nm - 2) i = P.nm - 2;
+    d.hi = i;
+    d.ht = u - double(i);
+    d.hfac = P.p.hmf_s * (lnM * 0.4342944819 - 13.8124426028) + P.p.hmf_q;
+    d.si = -1;                               // sel_function(): lnM axis
+    d.st = 0.0;
+    if (lnM >= lnm[0] && lnM <= lnm[sp.n_lnm - 1]) {
+      int const c = bracket(lnm, sp.n_lnm, lnM);
+      d.si = c;
+      d.st = (lnM - lnm[c]) / (lnm[c + 1] - lnm[c]);
+    }
+    nm[k] = d;
+  }
+  for (int q = threadIdx.x; q < A.n_z; q += kSelBlock) {
+    double const z = xz[q];
+    SelNodeZ d;
+    d.zf = wz[q] * dv_do_dz<double>(P, da, z) * omega_z_des(z);
+    double v = (z - P.z0) * P.inv_dz;        // hmf(): z axis
+    v = v < 0.0 ? 0.0 : (v > double(P.nz - 1) ? double(P.nz - 1) : v);
+    int j = int(v);
+    if (j > P.nz - 2) j = P.nz - 2;
+    d.hj = j;
+    d.hs = v - double(j);
+    d.sj = -1;                               // sel_function(): z axis
+    d.su = 0.0;
+    if (z >= zs[0] && z <= zs[sp.n_z - 1]) {
+      int const c = bracket(zs, sp.n_z, z);
+      d.sj = c;
+      d.su = (z - zs[c]) / (zs[c + 1] - zs[c]);
+    }
+    nz[q] = d;
+  }
+  __syncthreads();
+
+  // term t = q * n_lnm + k, stepping by the block size without a division.
+  // Mass fastest: a wave's lanes then read one HMF row and one plane row
+  // (both lnM-fastest) at neighbouring cells, few cache lines per gather.
   double acc = 0.0;
   int const nt = A.n_lnm * A.n_z;
+  int const dq = kSelBlock / A.n_lnm, dk = kSelBlock - dq * A.n_lnm;
+  int q = threadIdx.x / A.n_lnm, k = threadIdx.x - q * A.n_lnm;
   for (int t = threadIdx.x; t < nt; t += kSelBlock) {
-    int const k = t / A.n_z, q = t - k * A.n_z;
-    double const lnM = xm[k], z = xz[q];
-    acc += wm[k] * zfac[q] * hmf<double>(P, htab, lnM, z) *
-           sel_function(lnm, sp.n_lnm, zs, sp.n_z, plane, lnM, z);
+    SelNodeM const m = nm[k];
+    SelNodeZ const z = nz[q];
+    if (m.si >= 0 && z.sj >= 0) {            // S_b = 0 outside its axes
+      const double* h0 = htab + z.hj * P.nm + m.hi;
+      const double* h1 = h0 + P.nm;
+      double const ha = h0[0] + m.ht * (h0[1] - h0[0]);
+      double const hb = h1[0] + m.ht * (h1[1] - h1[0]);
+      double const hm = (ha + z.hs * (hb - ha)) * m.hfac;
+      const double* r0 = plane + z.sj * sp.n_lnm + m.si;
+      const double* r1 = r0 + sp.n_lnm;
+      double const t1 = m.st, u1 = z.su;
+      double const sv = (1.0 - t1) * (1.0 - u1) * r0[0] + t1 * (1.0 - u1) * r0[1] +
+                        (1.0 - t1) * u1 * r1[0] + t1 * u1 * r1[1];
+      acc += m.w * z.zf * hm * sv;
+    }
+    q += dq;
+    k += dk;
+    if (k >= A.n_lnm) {
+      k -= A.n_lnm;
+      ++q;
+    }
   }
   acc = block_sum<kSelBlock>(acc, red);
   if (threadIdx.x == 0) A.out[blockIdx.x] = acc;
@@ -729,7 +831,17 @@ __global__ __launch_bounds__(kSelBlock) void k_selgl(SelGLArgs A)
 
 hipError_t launch_selgl(const SelGLArgs& A, int n_points, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_selgl, dim3(n_points * A.max_bins), dim3(kSelBlock), 0, s, A);
+  bool const stage = A.max_axes <= kSelAxesMax;
+  size_t const smem = selgl_smem(A.n_lnm, A.n_z, stage ? A.max_axes : 0);
+  const void* fn = stage ? reinterpret_cast<const void*>(&k_selgl<true>)
+                         : reinterpret_cast<const void*>(&k_selgl<false>);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  if (stage)
+    hipLaunchKernelGGL(k_selgl<true>, dim3(n_points * A.max_bins), dim3(kSelBlock), smem, s, A);
+  else
+    hipLaunchKernelGGL(k_selgl<false>, dim3(n_points * A.max_bins), dim3(kSelBlock), smem, s,
+                       A);
   return hipGetLastError();
 }
 

@@ -138,6 +138,9 @@ struct y3g_runtime {
   double* d_seldata = nullptr;
   SelPoint* d_selpts = nullptr;
   double* d_selnodes = nullptr;
+  size_t selnodes_cap = 0, seldata_cap = 0, selpts_cap = 0;
+  y3g_selgl_spec selnodes_spec{};  // the spec whose nodes d_selnodes holds
+  bool selnodes_valid = false;
   double* d_selout = nullptr;
   size_t selout_cap = 0;
 };
@@ -895,30 +898,49 @@ int y3g_selgl_counts(y3g_runtime* rt, const y3g_selgl_spec* spec, int n_points,
       sp.off_s = (long long)data.size();
       data.insert(data.end(), h.s.begin(), h.s.end());
     }
-    dfree(rt->d_seldata);
-    dfree(rt->d_selpts);
-    if (int rc = dalloc(&rt->d_seldata, data.size())) return rc;
-    if (int rc = dalloc(&rt->d_selpts, pts.size())) return rc;
+    if (data.size() > rt->seldata_cap) {  // grow only: a sampler re-uploads every sample
+      dfree(rt->d_seldata);
+      rt->seldata_cap = 0;
+      if (int rc = dalloc(&rt->d_seldata, data.size())) return rc;
+      rt->seldata_cap = data.size();
+    }
+    if (pts.size() > rt->selpts_cap) {
+      dfree(rt->d_selpts);
+      rt->selpts_cap = 0;
+      if (int rc = dalloc(&rt->d_selpts, pts.size())) return rc;
+      rt->selpts_cap = pts.size();
+    }
     HIPCHK(hipMemcpy(rt->d_seldata, data.data(), sizeof(double) * data.size(),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(rt->d_selpts, pts.data(), sizeof(SelPoint) * pts.size(),
                      hipMemcpyHostToDevice));
     rt->sel_dirty = false;
   }
-  std::vector<double> nodes(2 * (size_t)(spec->n_lnm + spec->n_z));
-  gl_nodes(spec->n_lnm, spec->lnm_lo, spec->lnm_hi, nodes.data(), nodes.data() + spec->n_lnm);
-  gl_nodes(spec->n_z, spec->zt_lo, spec->zt_hi, nodes.data() + 2 * spec->n_lnm,
-           nodes.data() + 2 * spec->n_lnm + spec->n_z);
-  dfree(rt->d_selnodes);
-  if (int rc = dalloc(&rt->d_selnodes, nodes.size())) return rc;
+  // GL nodes and weights: computed and uploaded when the spec changes
+  if (!rt->selnodes_valid || std::memcmp(&rt->selnodes_spec, spec, sizeof(*spec)) != 0) {
+    std::vector<double> nodes(2 * (size_t)(spec->n_lnm + spec->n_z));
+    gl_nodes(spec->n_lnm, spec->lnm_lo, spec->lnm_hi, nodes.data(),
+             nodes.data() + spec->n_lnm);
+    gl_nodes(spec->n_z, spec->zt_lo, spec->zt_hi, nodes.data() + 2 * spec->n_lnm,
+             nodes.data() + 2 * spec->n_lnm + spec->n_z);
+    rt->selnodes_valid = false;
+    if (nodes.size() > rt->selnodes_cap) {
+      dfree(rt->d_selnodes);
+      rt->selnodes_cap = 0;
+      if (int rc = dalloc(&rt->d_selnodes, nodes.size())) return rc;
+      rt->selnodes_cap = nodes.size();
+    }
+    HIPCHK(hipMemcpy(rt->d_selnodes, nodes.data(), sizeof(double) * nodes.size(),
+                     hipMemcpyHostToDevice));
+    rt->selnodes_spec = *spec;
+    rt->selnodes_valid = true;
+  }
   size_t const nout = (size_t)n_points * max_bins;
   if (nout > rt->selout_cap) {
     dfree(rt->d_selout);
     if (int rc = dalloc(&rt->d_selout, nout)) return rc;
     rt->selout_cap = nout;
   }
-  HIPCHK(hipMemcpyAsync(rt->d_selnodes, nodes.data(), sizeof(double) * nodes.size(),
-                        hipMemcpyHostToDevice, rt->stream));
   SelGLArgs A{};
   A.points = rt->points;
   A.hmf64 = rt->hmf64;
@@ -931,6 +953,11 @@ int y3g_selgl_counts(y3g_runtime* rt, const y3g_selgl_spec* spec, int n_points,
   A.n_lnm = spec->n_lnm;
   A.n_z = spec->n_z;
   A.max_bins = max_bins;
+  A.max_axes = 0;
+  for (int p = 0; p < n_points; ++p) {
+    size_t const nax = rt->sel[p].lnm.size() + rt->sel[p].z.size();
+    A.max_axes = (int)std::min<size_t>(std::max<size_t>(A.max_axes, nax), 1u << 30);
+  }
   A.out = rt->d_selout;
   HIPCHK(hipEventRecord(rt->ev0, rt->stream));
   HIPCHK(launch_selgl(A, n_points, rt->stream));

@@ -478,6 +478,44 @@ namespace y3g_cosmosis {
 
   // ------------------------------------------------------------- the module
 
+  // The device tables of parameter point 0: built from the sample's physics
+  // and overwritten by the sample's own mass_function / distances tables
+  // when it carries them; rebuilt only when any of those inputs changed.
+  class TableCache {
+  public:
+    template <class S>
+    void
+    ensure(y3g_runtime* rt, S const& smp)
+    {
+      y3g_params const& p = smp.params();
+      y3g_table_spec const& sp = smp.spec();
+      if (have_ && std::memcmp(&p, &built_, sizeof(p)) == 0 &&
+          std::memcmp(&sp, &built_spec_, sizeof(sp)) == 0 &&
+          smp.hmf_table() == built_hmf_ && smp.da_table() == built_da_)
+        return;
+      have_ = false;
+      if (y3g_tables_build(rt, &p, &sp, 1) != Y3G_OK)
+        throw std::runtime_error(std::string("table build: ") + y3g_last_error());
+      auto const& h = smp.hmf_table();
+      auto const& d = smp.da_table();
+      if ((!h.empty() || !d.empty()) &&
+          y3g_tables_upload(rt, 0, h.empty() ? nullptr : h.data(),
+                            d.empty() ? nullptr : d.data(), nullptr) != Y3G_OK)
+        throw std::runtime_error(std::string("table upload: ") + y3g_last_error());
+      built_ = p;
+      built_spec_ = sp;
+      built_hmf_ = h;
+      built_da_ = d;
+      have_ = true;
+    }
+
+  private:
+    y3g_params built_{};
+    y3g_table_spec built_spec_{};
+    std::vector<double> built_hmf_, built_da_;
+    bool have_ = false;
+  };
+
   template <class I>
   class HIPIntegrationModule {
   public:
@@ -506,25 +544,7 @@ namespace y3g_cosmosis {
     {
       if (timing_) record_timestamp(*timing_, "enter");
       integrand_.set_sample(sample);
-      y3g_params const& p = integrand_.params();
-      y3g_table_spec const& sp = integrand_.spec();
-      if (!have_tables_ || std::memcmp(&p, &built_, sizeof(p)) != 0 ||
-          std::memcmp(&sp, &built_spec_, sizeof(sp)) != 0 ||
-          integrand_.hmf_table() != built_hmf_ || integrand_.da_table() != built_da_) {
-        if (y3g_tables_build(device_.get(), &p, &sp, 1) != Y3G_OK)
-          throw std::runtime_error(std::string("table build: ") + y3g_last_error());
-        auto const& h = integrand_.hmf_table();
-        auto const& d = integrand_.da_table();
-        if ((!h.empty() || !d.empty()) &&
-            y3g_tables_upload(device_.get(), 0, h.empty() ? nullptr : h.data(),
-                              d.empty() ? nullptr : d.data(), nullptr) != Y3G_OK)
-          throw std::runtime_error(std::string("table upload: ") + y3g_last_error());
-        built_ = p;
-        built_spec_ = sp;
-        built_hmf_ = h;
-        built_da_ = d;
-        have_tables_ = true;
-      }
+      tables_.ensure(device_.get(), integrand_);
       std::vector<y3g_integral> igs = build_integrals(integrand_, cfg_);
       std::vector<y3g_result> res(igs.size());
       if (!igs.empty() &&
@@ -542,10 +562,7 @@ namespace y3g_cosmosis {
     rank_info rank_;
     DeviceBinding device_;
     I integrand_;
-    y3g_params built_{};
-    y3g_table_spec built_spec_{};
-    std::vector<double> built_hmf_, built_da_;
-    bool have_tables_ = false;
+    TableCache tables_;
     std::optional<std::ofstream> timing_;
   };
 
@@ -666,13 +683,66 @@ namespace y3g_cosmosis {
     }
   };
 
+  // ------------------------------------------------------ evaluator modules
+
+  // A module whose class does a fixed quadrature of its own per sample (on
+  // the device, in set_sample) and is then queried per grid point; each of
+  // its K outputs is one 1-D array "vals" in its own section (reference
+  // CosmoSISScalarEvaluatorModule, src/utils/CosmoSISScalarEvaluatorModule.hh
+  // :9-26, 111-136).  The class is constructed from the config block and the
+  // device runtime of this process.
+  template <class I>
+  class HIPEvaluatorModule {
+  public:
+    explicit HIPEvaluatorModule(cosmosis::DataBlock& cfg)
+      : rank_(get_rank_info()),
+        device_(rank_.local_rank),
+        integrand_(cfg, device_.get()),
+        grid_(I::make_grid_points(cfg))
+    {}
+
+    void
+    execute(cosmosis::DataBlock& sample)
+    {
+      integrand_.set_sample(sample);
+      constexpr std::size_t K = I::n_outputs;
+      std::size_t const n = grid_.size();
+      std::array<std::vector<double>, K> buf;
+      for (auto& b : buf) b.assign(n, 0.0);
+      for (std::size_t i = 0; i != n; ++i) {
+        auto const v = integrand_.evaluate(grid_.points[i]);
+        for (std::size_t k = 0; k != K; ++k) buf[k][i] = v[k];
+      }
+      auto const sections = I::output_sections();
+      for (std::size_t k = 0; k != K; ++k)
+        if (sample.put_val(sections[k], "vals", cosmosis::ndarray<double>(buf[k], {n})) !=
+            DBS_SUCCESS)
+          throw std::runtime_error(std::string(sections[k]) +
+                                   "/vals could not be put into the sample");
+    }
+
+  private:
+    rank_info rank_;
+    DeviceBinding device_;
+    I integrand_;
+    typename I::grid_t grid_;
+  };
+
 } // namespace y3g_cosmosis
 
 // The three CosmoSIS entry points for module class `klass` (reference:
-// src/utils/cuda_module_macros.cuh:20-45), with exceptions stopped at the
-// C boundary.
+// src/utils/cuda_module_macros.cuh:20-45 for the integration modules,
+// src/utils/module_macros.hh:76-99 for the evaluators), with exceptions
+// stopped at the C boundary.
 #define DEFINE_COSMOSIS_HIP_INTEGRATION_MODULE(klass)                          \
   using y3g_module_type = y3g_cosmosis::HIPIntegrationModule<klass>;           \
+  Y3G_COSMOSIS_ENTRY_POINTS(klass)
+
+#define DEFINE_COSMOSIS_HIP_EVALUATOR_MODULE(klass)                            \
+  using y3g_module_type = y3g_cosmosis::HIPEvaluatorModule<klass>;             \
+  Y3G_COSMOSIS_ENTRY_POINTS(klass)
+
+#define Y3G_COSMOSIS_ENTRY_POINTS(klass)                                       \
   extern "C" {                                                                 \
   void*                                                                        \
   setup(cosmosis::DataBlock* cfg)                                              \

@@ -105,3 +105,26 @@ def mcmc_requests(points, n_requests, rate, seed=SEED + 2, frac_full=0.3, k_nn=6
         if np.log(rng.random()) < loglike(prop) - loglike(cur):
             cur = prop
     return t, reqs
+
+
+# Fixed-GL number counts with a selection function: the reference's default
+# grids (sel_function.py: n_lnm 40, n_z_shared 20, 12 bins) and GL nodes
+# (n_operator_sel_gl_t.hh: n_lnm 96, n_z 64).
+SEL_BINS, SEL_NLNM, SEL_NZ = 12, 40, 20
+SEL_GL = (30.5, 35.0, 0.15, 0.75, 96, 64)  # lnm_lo, lnm_hi, zt_lo, zt_hi, n_lnm, n_z
+
+
+def selection_stack(rng, n_bins=SEL_BINS, nm=SEL_NLNM, nz=SEL_NZ):
+    """A smooth synthetic sel_function/{lnM, z, S_stack}: lnM linear in
+    [30, 35.5], z linear in [0.1, 0.8], S_stack [n_bins, n_z, n_lnm] with
+    richness bins (b % 4) as sigmoids in lnM and redshift bins (b // 4) as
+    Gaussian windows in z, the shape of S_ij = S_i(lnM, z) K_j(z)."""
+    lnm = np.linspace(30.0, 35.5, nm)
+    z = np.linspace(0.1, 0.8, nz)
+    M, Z = np.meshgrid(lnm, z)
+    out = np.empty((n_bins, nz, nm))
+    for b in range(n_bins):
+        m0 = 31.5 + 0.5 * (b % 4) + rng.uniform(-0.1, 0.1)
+        zc = 0.25 + 0.15 * (b // 4)
+        out[b] = (1 / (1 + np.exp(-(M - m0) / 0.3))) * np.exp(-0.5 * ((Z - zc) / 0.08) ** 2)
+    return lnm, z, out
