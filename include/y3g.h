@@ -30,8 +30,10 @@
  *                           src/models/n_operator_sel_gl_t.hh:88-150, 184-192
  *                           (all bins of all points in one launch)
  *   y3g_eval_functor        the host-side functor calls of the reference's
- *                           Catch2 tests (e.g. test/roffset_t.test.cc:40-46),
- *                           here evaluated by the device functors
+ *                           Catch2 tests (e.g. test/roffset_t.test.cc:40-46,
+ *                           test/int_lc_lt_des_t_test.test.cu for
+ *                           INT_LC_LT_DES_t2), here evaluated by the device
+ *                           functors
  */
 #ifndef Y3G_H
 #define Y3G_H
@@ -266,9 +268,12 @@ typedef enum {
   Y3G_F_INTEGRAND = 13,    /* (y[ndim]) mapped integrand of integral `aux` */
   Y3G_F_INTEGRAND_PAIRED = 14,/* the same through pair_factors + integrand_pair,
                                  the cubature's factorised form       */
-  Y3G_F_INTEGRAND_NODE = 15   /* the FP32 cubature's node form: pair factors
+  Y3G_F_INTEGRAND_NODE = 15,  /* the FP32 cubature's node form: pair factors
                                  at (zt, lnM), the other coordinates
                                  rounded to float (FP64: as 14)        */
+  Y3G_F_INT_LC_LT_DES = 16    /* (lc, lt, zt); par = {n_lt, n_zt, lt knots,
+                                 zt knots, 4 tables of n_lt x n_zt, lt
+                                 fastest}           INT_LC_LT_DES_t2, FP64 */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

@@ -21,6 +21,11 @@
  *                                                      cluster_toolkit values of
  *                                                      nfw_dsigma_mis.test.cu:32-49 (5e-3)
  *   y3o_comoving_distance   ez.hh / dv_do_dz fixture   data/d_a_test.txt (1e-4)
+ *   y3o_int_lc_lt_des       int_lc_lt_des_t2.cuh:48-63 the 720 expected values of
+ *                                                      int_lc_lt_des_t_test.test.cu
+ *                                                      (tables 0, 1, 3 at knots, 1e-5);
+ *                                                      exact on bilinear tables (all
+ *                                                      four, between knots); bin edges
  */
 #include "y3o.h"
 
@@ -169,6 +174,29 @@ y3o_interp2d_clamp(const double* xs, int nx, const double* ys, int ny,
   double const yc = y < ys[0] ? ys[0] : (y > ys[ny - 1] ? ys[ny - 1] : y);
   y3o_interp2d(xs, nx, ys, ny, zs, xc, yc, &v);
   return v;
+}
+
+/* int_lc_lt_des_t2.cuh:48-63: the DES richness transfer.  The lambda_c bin
+ * [20, 30), [30, 45), [45, 60), else, picks one of four Interp2D tables over
+ * (lt knots, zt knots) with the values in data-file order (x-fastest,
+ * interp_2d.cc:115-127).  The fourth is read with clamp(); the first three
+ * with operator(), which is evaluated here clamped as well (the reference's
+ * device operator() belongs to the un-vendored gpuintegration library).
+ * par = {n_lt, n_zt, lt[n_lt], zt[n_zt], tab0, tab1, tab2, tab3}. */
+double
+y3o_int_lc_lt_des(const double* par, double lc, double lt, double zt)
+{
+  int const nl = (int)par[0], nz = (int)par[1];
+  const double* xl = par + 2;
+  const double* xz = xl + nl;
+  int b = 3;
+  if (lc >= 20.0 && lc < 30.0)
+    b = 0;
+  else if (lc >= 30.0 && lc < 45.0)
+    b = 1;
+  else if (lc >= 45.0 && lc < 60.0)
+    b = 2;
+  return y3o_interp2d_clamp(xl, nl, xz, nz, xz + nz + (size_t)b * nl * nz, lt, zt);
 }
 
 /* dv_do_dz_t.hh:23-31 (D_A in Mpc, result (Mpc/h)^3 / sr). */

@@ -1,0 +1,47 @@
+"""GPU: the device DES richness transfer (y3g_eval_functor
+Y3G_F_INT_LC_LT_DES, INT_LC_LT_DES_t2) against the oracle on the
+reference's tables, inside and outside the knots and on every lambda_c bin,
+and the reference's own expected values; malformed tables are errors."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import y3_cluster_cpp_amd as y3
+from conftest import REF_DATA
+from test_oracle_des import lclt_par
+
+pytestmark = pytest.mark.gpu
+
+
+def test_int_lc_lt_des_device_matches_oracle(runtime):
+    par, lt, zt = lclt_par()
+    rng = np.random.default_rng(12)
+    n = 4000
+    x = np.column_stack([rng.uniform(5.0, 120.0, n), rng.uniform(-5.0, 220.0, n),
+                         rng.uniform(-0.1, 0.9, n)])
+    x[:40, 0] = [20, 30, 45, 60, 19.999, 29.999, 44.999, 59.999] * 5   # bin edges
+    got = runtime.eval_functor(y3.F_INT_LC_LT_DES, x, par)
+    ref = np.array([O.lib.y3o_int_lc_lt_des(O.ptr(par), *row) for row in x])
+    assert np.all(np.abs(got - ref) <= 1e-14 + 1e-13 * np.abs(ref)), np.abs(got - ref).max()
+    with open(os.path.join(REF_DATA, "int_lc_lt_des_answers.json")) as fh:
+        a = json.load(fh)
+    pts = np.array([(lc, l, z) for lc in a["lc"] for z in a["zt"] for l in a["lt"]])
+    got = runtime.eval_functor(y3.F_INT_LC_LT_DES, pts, par)
+    exp = np.array(a["answers"])
+    assert np.all(np.abs(got - exp) <= 1e-5 + 1e-3 * np.abs(exp))
+
+
+def test_int_lc_lt_des_rejects_bad_tables(runtime):
+    par, _, _ = lclt_par()
+    x = np.array([[25.0, 10.0, 0.3]])
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_INT_LC_LT_DES, x, par[:-1])        # one value short
+    bad = par.copy()
+    bad[3] = bad[2]                                                  # lt knots not increasing
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_INT_LC_LT_DES, x, bad)
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_INT_LC_LT_DES, x[:, :2], par)      # two inputs

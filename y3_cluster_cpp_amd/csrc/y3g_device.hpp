@@ -868,6 +868,38 @@ Y3G_HD double sel_function(const double* lnm, int nm, const double* z, int nz,
          t * u * r1[1];
 }
 
+// INT_LC_LT_DES_t2::operator() (int_lc_lt_des_t2.cuh:48-63): the DES
+// richness transfer P(lambda_c | lambda_true, z_true) from one of four
+// (lt, zt) tables, picked by the lambda_c bin [20, 30), [30, 45), [45, 60),
+// and the fourth table otherwise (lambda_c < 20 included).  The tables are
+// the reference's data files read in file order into Interp2D(xs = lt knots,
+// ys = zt knots, zs), lt fastest: value(lt_i, zt_j) = tab[j n_lt + i];
+// bilinear between knots.  Packed as
+//   par = {n_lt, n_zt, lt[n_lt], zt[n_zt], tab0, tab1, tab2, tab3}.
+// Outside the knots every table is clamped to its edge (the reference clamps
+// the fourth; the first three go through the device Interp2D's operator() of
+// the un-vendored gpuintegration library).
+Y3G_HD double int_lc_lt_des(const double* par, double lc, double lt, double zt)
+{
+  int const nl = int(par[0]), nz = int(par[1]);
+  const double* xl = par + 2;
+  const double* xz = xl + nl;
+  int const b = (lc >= 20.0 && lc < 30.0)   ? 0
+                : (lc >= 30.0 && lc < 45.0) ? 1
+                : (lc >= 45.0 && lc < 60.0) ? 2
+                                            : 3;
+  const double* tab = xz + nz + (size_t)b * nl * nz;
+  double const x = lt < xl[0] ? xl[0] : (lt > xl[nl - 1] ? xl[nl - 1] : lt);
+  double const y = zt < xz[0] ? xz[0] : (zt > xz[nz - 1] ? xz[nz - 1] : zt);
+  int const i = bracket(xl, nl, x), j = bracket(xz, nz, y);
+  double const t = (x - xl[i]) / (xl[i + 1] - xl[i]);
+  double const u = (y - xz[j]) / (xz[j + 1] - xz[j]);
+  const double* r0 = tab + j * nl + i;
+  const double* r1 = r0 + nl;
+  return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r0[1] + (1.0 - t) * u * r1[0] +
+         t * u * r1[1];
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

@@ -782,8 +782,23 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INTEGRAND_NODE)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INT_LC_LT_DES)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_INT_LC_LT_DES) {
+    // par = {n_lt, n_zt, lt knots, zt knots, four n_lt x n_zt tables}
+    if (nin != 3 || !par || npar < 2)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: INT_LC_LT_DES takes (lc, lt, zt) and tables");
+    double const a = par[0], b = par[1];
+    if (!(a >= 2.0 && b >= 2.0 && a <= 4096.0 && b <= 4096.0) || a != double(int(a)) ||
+        b != double(int(b)) || (long long)npar != 2 + (long long)(a + b) + 4LL * (long long)(a * b))
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: INT_LC_LT_DES table sizes");
+    for (int k = 1; k < int(a); ++k)
+      if (!(par[2 + k] > par[1 + k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: INT_LC_LT_DES lt knots not increasing");
+    for (int k = 1; k < int(b); ++k)
+      if (!(par[2 + int(a) + k] > par[1 + int(a) + k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: INT_LC_LT_DES zt knots not increasing");
+  }
   if (n == 0) return 0;
   HIPCHK(hipSetDevice(rt->device));
   bool const needs_tables = which == Y3G_F_DV_DO_DZ || which == Y3G_F_HMF ||
