@@ -271,9 +271,11 @@ typedef enum {
   Y3G_F_INTEGRAND_NODE = 15,  /* the FP32 cubature's node form: pair factors
                                  at (zt, lnM), the other coordinates
                                  rounded to float (FP64: as 14)        */
-  Y3G_F_INT_LC_LT_DES = 16    /* (lc, lt, zt); par = {n_lt, n_zt, lt knots,
+  Y3G_F_INT_LC_LT_DES = 16,   /* (lc, lt, zt); par = {n_lt, n_zt, lt knots,
                                  zt knots, 4 tables of n_lt x n_zt, lt
                                  fastest}           INT_LC_LT_DES_t2, FP64 */
+  Y3G_F_MOR_DES = 17          /* (lt, lnM, zt); par = {A, B, alpha, sigma_intr,
+                                 epsilon, z_pivot}  MOR_DES_t, FP64        */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

@@ -26,6 +26,10 @@
  *                                                      (tables 0, 1, 3 at knots, 1e-5);
  *                                                      exact on bilinear tables (all
  *                                                      four, between knots); bin edges
+ *   y3o_mor_des             mor_des_t.hh:262-290       table entries read by hand from
+ *                                                      the header; bilinear cell centre;
+ *                                                      normalisation and mean of the
+ *                                                      skew normal (closed forms)
  */
 #include "y3o.h"
 
@@ -197,6 +201,55 @@ y3o_int_lc_lt_des(const double* par, double lc, double lt, double zt)
   else if (lc >= 45.0 && lc < 60.0)
     b = 2;
   return y3o_interp2d_clamp(xl, nl, xz, nz, xz + nz + (size_t)b * nl * nz, lt, zt);
+}
+
+/* The skew-normal scatter tables of MOR_DES_t (mor_des_t.hh:16-230), rows
+ * = sigma_intr knots, columns = lambda_sat knots, as generated from the
+ * reference header; the reference's Interp2D(matrix) stores them
+ * column-major for GSL (interp_2d.hh:129-139), done here once. */
+#define Y3G_MOR_DES_TABLE(name, n) static const double name[n]
+#include "../y3_cluster_cpp_amd/csrc/y3g_mor_des_tables.inc"
+#undef Y3G_MOR_DES_TABLE
+
+static double mor_des_sig_cm[14 * 42], mor_des_skw_cm[14 * 42];
+static int mor_des_ready = 0;
+
+static void
+mor_des_init(void)
+{
+  if (mor_des_ready) return;
+  for (int i = 0; i < 14; ++i)
+    for (int j = 0; j < 42; ++j) {
+      mor_des_sig_cm[i + j * 14] = kMorDesSigma[i * 42 + j];
+      mor_des_skw_cm[i + j * 14] = kMorDesSkew[i * 42 + j];
+    }
+  mor_des_ready = 1;
+}
+
+/* mor_des_t.hh:276-288: sigma and skewness, Interp2D::clamp over
+ * (sigma_intr, lambda). */
+void
+y3o_mor_des_sigma_skew(double sigma_intr, double lam, double* sigma, double* skw)
+{
+  mor_des_init();
+  *sigma = y3o_interp2d_clamp(kMorDesSigintr, 14, kMorDesLsat, 42, mor_des_sig_cm, sigma_intr,
+                              lam);
+  *skw = y3o_interp2d_clamp(kMorDesSigintr, 14, kMorDesLsat, 42, mor_des_skw_cm, sigma_intr,
+                            lam);
+}
+
+/* mor_des_t.hh:262-290 (operator()): q = {A, B, alpha, sigma_intr, epsilon,
+ * z_pivot}.  std::max(1.0, v) keeps 1.0 when v is NaN (a mass below A). */
+double
+y3o_mor_des(const double* q, double lt, double lnM, double zt)
+{
+  double const v = pow((exp(lnM) - q[0]) / (q[1] - q[0]), q[2]) *
+                   pow((1.0 + zt) / (1.0 + q[5]), q[4]);
+  double const lam = (1.0 < v) ? v : 1.0;
+  double sigma, skw;
+  y3o_mor_des_sigma_skew(q[3], lam, &sigma, &skw);
+  double const x = lt - lam;
+  return y3o_gaussian(x, 0.0, sigma) * erfc(-1.0 * skw * x / (sqrt(2.0) * sigma));
 }
 
 /* dv_do_dz_t.hh:23-31 (D_A in Mpc, result (Mpc/h)^3 / sr). */

@@ -45,3 +45,20 @@ def test_int_lc_lt_des_rejects_bad_tables(runtime):
         runtime.eval_functor(y3.F_INT_LC_LT_DES, x, bad)
     with pytest.raises(RuntimeError):
         runtime.eval_functor(y3.F_INT_LC_LT_DES, x[:, :2], par)      # two inputs
+
+
+def test_mor_des_device_matches_oracle(runtime):
+    """MOR_DES_t on the device against the oracle: masses below M_min
+    (lambda = 1), lambdas beyond the tables' knots, two parameter sets."""
+    rng = np.random.default_rng(31)
+    n = 3000
+    x = np.column_stack([rng.uniform(0.0, 400.0, n), rng.uniform(np.log(1e10), np.log(5e15), n),
+                         rng.uniform(0.05, 0.9, n)])
+    for q in (np.array([10 ** 11.2, 10 ** 12.4, 0.75, 0.2, 0.1, 0.3]),
+              np.array([10 ** 11.5, 10 ** 12.9, 0.9, 1.5, -0.2, 0.45])):
+        got = runtime.eval_functor(y3.F_MOR_DES, x, q)
+        ref = np.array([O.lib.y3o_mor_des(O.ptr(q), *row) for row in x])
+        assert np.all(np.abs(got - ref) <= 1e-290 + 1e-12 * np.abs(ref)), \
+            np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_MOR_DES, x[:4], q[:5])

@@ -64,3 +64,60 @@ def test_int_lc_lt_des_bilinear_bins_and_clamp():
         x, y = lt[-1], zt[0]
         assert f(par, lc, 1e4, -1.0) == f(par, lc, x, y)
         assert abs(f(par, lc, 1e4, -1.0) - (c + a * x + d * y + e * x * y)) < 1e-12
+
+
+# ---------------------------------------------------------------- MOR_DES_t
+
+# cluster_abundance/{mor_A, mor_B, mor_alpha, mor_sigma, mor_epsilon,
+# z_mor_pivot}: a DES-like point (A = M_min, B = M_1 in M_sun/h)
+MOR_Q = np.array([10 ** 11.2, 10 ** 12.4, 0.75, 0.2, 0.1, 0.3])
+
+
+def mor_sigma_skew(si, lam):
+    s, k = np.zeros(1), np.zeros(1)
+    O.lib.y3o_mor_des_sigma_skew(si, lam, O.ptr(s), O.ptr(k))
+    return s[0], k[0]
+
+
+def test_mor_des_tables_read_by_hand():
+    """Entries of sig_skewnorml_flat and skews at knots, read by hand from
+    mor_des_t.cuh (row = sigma_intr knot, column = lambda_sat knot), and the
+    bilinear value at a cell centre (the mean of its corners)."""
+    cases = [(0.05, 0.5, 1.211319, 9.120214), (0.05, 244.0, 19.830625, 0.053913),
+             (2.0, 244.0, 488.220129, 0.002402), (0.5, 0.7, 1.282591, 2.759347),
+             (0.1, 14.0, 4.061805, 0.250297), (1.3, 24.0, 31.606019, 0.039318),
+             (1.7, 24.0, 41.117558, 0.030344)]
+    for si, lam, sig, skw in cases:
+        s, k = mor_sigma_skew(si, lam)
+        assert abs(s - sig) < 1e-12 and abs(k - skw) < 1e-12, (si, lam, s, k)
+    s, _ = mor_sigma_skew(0.075, 0.55)
+    assert abs(s - (1.211319 + 1.307980 + 1.211227 + 1.309620) / 4) < 1e-12
+    # clamped beyond the knots
+    assert mor_sigma_skew(5.0, 1e4) == mor_sigma_skew(2.0, 244.0)
+
+
+def _mor(lt, lnM, zt, q=MOR_Q):
+    return O.lib.y3o_mor_des(O.ptr(np.ascontiguousarray(q)), lt, lnM, zt)
+
+
+def _lam(lnM, zt, q=MOR_Q):
+    v = ((np.exp(lnM) - q[0]) / (q[1] - q[0])) ** q[2] * ((1 + zt) / (1 + q[5])) ** q[4]
+    return max(1.0, v)
+
+
+def test_mor_des_skew_normal_moments():
+    """N(x; 0, s) erfc(-a x / (sqrt2 s)) is the skew-normal density: unit
+    mass over lt, mean lambda + s d sqrt(2/pi) with d = a / sqrt(1 + a^2)."""
+    from scipy import integrate
+    for lnM, zt in ((np.log(3e13), 0.3), (np.log(2e14), 0.55), (np.log(1e15), 0.7),
+                    (np.log(1e11), 0.4)):   # M < A: lambda = 1
+        lam = _lam(lnM, zt) if np.exp(lnM) > MOR_Q[0] else 1.0
+        s, a = mor_sigma_skew(MOR_Q[3], lam)
+        lo, hi = lam - 40 * s, lam + 40 * s
+        mass = integrate.quad(lambda t: _mor(t, lnM, zt), lo, hi, points=[lam], limit=200,
+                              epsabs=0, epsrel=1e-11)[0]
+        mean = integrate.quad(lambda t: t * _mor(t, lnM, zt), lo, hi, points=[lam], limit=200,
+                              epsabs=0, epsrel=1e-11)[0]
+        d = a / np.sqrt(1 + a * a)
+        assert abs(mass - 1) < 1e-9, (lnM, zt, mass)
+        assert abs(mean - (lam + s * d * np.sqrt(2 / np.pi))) < 1e-8 * max(1.0, lam), (lnM, mean)

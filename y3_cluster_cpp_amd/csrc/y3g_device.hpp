@@ -900,6 +900,47 @@ Y3G_HD double int_lc_lt_des(const double* par, double lc, double lt, double zt)
          t * u * r1[1];
 }
 
+// MOR_DES_t::operator() (mor_des_t.cuh:276-305): the DES skew-normal
+// mass-observable relation P(lambda_true | lnM, zt).
+//   lambda(M, z) = max(1, ((M - A) / (B - A))^alpha ((1 + z) / (1 + z_p))^eps)
+// (a mass below A gives a NaN base and lambda = 1, as std::max(1.0, NaN)
+// does); sigma and skewness are clamped bilinear lookups in the
+// (sigma_intr, lambda) tables; with x = lt - lambda the value is
+//   N(x; 0, sigma) erfc(-skw x / (sqrt(2) sigma)).
+// q = {A, B, alpha, sigma_intr, epsilon, z_pivot} (cluster_abundance/mor_A,
+// mor_B, mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot).
+#define Y3G_MOR_DES_TABLE(name, n) static __constant__ double name[n]
+#include "y3g_mor_des_tables.inc"
+#undef Y3G_MOR_DES_TABLE
+
+__device__ inline double mor_des_lookup(const double* tab, double si, double lam)
+{
+  double const x = si < kMorDesSigintr[0] ? kMorDesSigintr[0]
+                   : (si > kMorDesSigintr[13] ? kMorDesSigintr[13] : si);
+  double const y = lam < kMorDesLsat[0] ? kMorDesLsat[0]
+                   : (lam > kMorDesLsat[41] ? kMorDesLsat[41] : lam);
+  int const i = bracket(kMorDesSigintr, 14, x), j = bracket(kMorDesLsat, 42, y);
+  double const t = (x - kMorDesSigintr[i]) / (kMorDesSigintr[i + 1] - kMorDesSigintr[i]);
+  double const u = (y - kMorDesLsat[j]) / (kMorDesLsat[j + 1] - kMorDesLsat[j]);
+  const double* r0 = tab + i * 42 + j;  // row = sigma_intr knot
+  const double* r1 = r0 + 42;
+  return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r1[0] + (1.0 - t) * u * r0[1] +
+         t * u * r1[1];
+}
+
+__device__ inline double mor_des(const double* q, double lt, double lnM, double zt)
+{
+  double const v = pow((exp(lnM) - q[0]) / (q[1] - q[0]), q[2]) *
+                   pow((1.0 + zt) / (1.0 + q[5]), q[4]);
+  double const lam = 1.0 < v ? v : 1.0;
+  double const sigma = mor_des_lookup(kMorDesSigma, q[3], lam);
+  double const skw = mor_des_lookup(kMorDesSkew, q[3], lam);
+  double const x = lt - lam;
+  double const z = x / sigma;
+  return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
+         erfc(-skw * x / (1.4142135623730951 * sigma));
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

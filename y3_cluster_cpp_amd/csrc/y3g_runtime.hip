@@ -782,8 +782,11 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_INT_LC_LT_DES)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_MOR_DES)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_MOR_DES && (nin != 3 || !par || npar != 6))
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: MOR_DES takes (lt, lnM, zt) and "
+                             "{A, B, alpha, sigma_intr, epsilon, z_pivot}");
   if (which == Y3G_F_INT_LC_LT_DES) {
     // par = {n_lt, n_zt, lt knots, zt knots, four n_lt x n_zt tables}
     if (nin != 3 || !par || npar < 2)
