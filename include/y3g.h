@@ -274,8 +274,12 @@ typedef enum {
   Y3G_F_INT_LC_LT_DES = 16,   /* (lc, lt, zt); par = {n_lt, n_zt, lt knots,
                                  zt knots, 4 tables of n_lt x n_zt, lt
                                  fastest}           INT_LC_LT_DES_t2, FP64 */
-  Y3G_F_MOR_DES = 17          /* (lt, lnM, zt); par = {A, B, alpha, sigma_intr,
+  Y3G_F_MOR_DES = 17,         /* (lt, lnM, zt); par = {A, B, alpha, sigma_intr,
                                  epsilon, z_pivot}  MOR_DES_t, FP64        */
+  Y3G_F_EMG_DES_PDF = 18,     /* (lob, ltr, z); par = {n, z[n], a_mu, b_mu,
+                                 a_sig, b_sig, a_tau, b_tau, a_fprj, b_fprj
+                                 (n each)}          EMG_DES_t(), FP64      */
+  Y3G_F_EMG_DES_CDF = 19      /* the same arguments  EMG_DES_t::cdf, FP64  */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

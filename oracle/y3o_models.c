@@ -30,6 +30,9 @@
  *                                                      the header; bilinear cell centre;
  *                                                      normalisation and mean of the
  *                                                      skew normal (closed forms)
+ *   y3o_emg_des_*           emg_des_t.cuh:37-178       cdf' == pdf, unit mass, Gaussian
+ *                                                      and EMG moments, curves linear
+ *                                                      in z between knots, clamps
  */
 #include "y3o.h"
 
@@ -250,6 +253,79 @@ y3o_mor_des(const double* q, double lt, double lnM, double zt)
   y3o_mor_des_sigma_skew(q[3], lam, &sigma, &skw);
   double const x = lt - lam;
   return y3o_gaussian(x, 0.0, sigma) * erfc(-1.0 * skw * x / (sqrt(2.0) * sigma));
+}
+
+/* EMG_DES_t::get_params (emg_des_t.cuh:95-117): eight curves in z, each an
+ * Interp1D::clamp over plob_ltr_params/z; par = {n, z[n], a_mu[n], b_mu[n],
+ * a_sig[n], b_sig[n], a_tau[n], b_tau[n], a_fprj[n], b_fprj[n]}.
+ * out = {mu, sigma, tau, fprj}. */
+void
+y3o_emg_des_params(const double* par, double ltr, double z, double* out)
+{
+  int const n = (int)par[0];
+  const double* zs = par + 1;
+  double c[8];
+  for (int k = 0; k < 8; ++k) c[k] = y3o_interp1d_clamp(zs, zs + (size_t)n * (k + 1), n, z);
+  double const l = fmax(ltr, 0.5);
+  double mu = c[0] + c[1] * l;
+  double sigma = c[3] * pow(l, c[2]);
+  double tau = c[5] / pow(l, c[4]);
+  double const den = pow(1.0 + exp(-l), c[6]);
+  double fprj = c[7] / fmax(den, 1e-300);
+  out[0] = mu;
+  out[1] = fmax(sigma, 1e-8);
+  out[2] = fmax(tau, 1e-8);
+  out[3] = fmax(0.0, fmin(1.0, fprj));
+}
+
+/* erfcx_impl (emg_des_t.cuh:37-51): the reference's own scaled erfc, exact
+ * form below |x| = 4 and a three-term asymptotic series above. */
+static double
+emg_des_erfcx(double x)
+{
+  double const ax = fabs(x);
+  if (ax < 4.0) return exp(x * x) * erfc(x);
+  double const x2 = ax * ax;
+  double r = (1.0 / sqrt(3.14159265358979323846)) / ax * (1.0 - 0.5 / x2 + 0.75 / (x2 * x2));
+  if (x < 0.0) r = 2.0 * exp(x2) - r;
+  return r;
+}
+
+/* EMG_DES_t::cdf (emg_des_t.cuh:128-156), Eq. 30 in CDF form:
+ * Phi(z) - fprj exp(-tau (x - mu) + tau^2 sigma^2 / 2) Phi(z - tau sigma). */
+double
+y3o_emg_des_cdf(const double* par, double lob, double ltr, double z)
+{
+  if (!isfinite(lob)) return lob > 0.0 ? 1.0 : 0.0;
+  double p[4];
+  y3o_emg_des_params(par, ltr, z, p);
+  double const mu = p[0], sigma = p[1], tau = p[2], fprj = p[3];
+  double const zs = (lob - mu) / sigma;
+  double const phi = 0.5 * (1.0 + erf(zs / Y3O_SQRT2));
+  double const u = (tau * sigma - zs) / Y3O_SQRT2;
+  double const base = 0.5 * emg_des_erfcx(fabs(u)) * exp(-0.5 * zs * zs);
+  double tail = base;
+  if (u < 0.0) {
+    double A = -tau * (lob - mu) + 0.5 * tau * tau * sigma * sigma;
+    A = fmax(-700.0, fmin(700.0, A));
+    tail = exp(A) - base;
+  }
+  double const r = phi - fprj * tail;
+  return fmax(0.0, fmin(1.0, r));
+}
+
+/* EMG_DES_t::operator() (emg_des_t.cuh:160-178). */
+double
+y3o_emg_des_pdf(const double* par, double lob, double ltr, double z)
+{
+  double p[4];
+  y3o_emg_des_params(par, ltr, z, p);
+  double const mu = p[0], sigma = p[1], tau = p[2], fprj = p[3];
+  double const ea = 0.5 * tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lob);
+  double const eb = (mu + tau * sigma * sigma - lob) / (Y3O_SQRT2 * sigma);
+  double emg = 0.0;
+  if (ea > -700.0 && ea < 700.0) emg = 0.5 * tau * exp(ea) * erfc(eb);
+  return (1.0 - fprj) * y3o_gaussian(lob, mu, sigma) + fprj * emg;
 }
 
 /* dv_do_dz_t.hh:23-31 (D_A in Mpc, result (Mpc/h)^3 / sr). */

@@ -62,3 +62,25 @@ def test_mor_des_device_matches_oracle(runtime):
             np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))
     with pytest.raises(RuntimeError):
         runtime.eval_functor(y3.F_MOR_DES, x[:4], q[:5])
+
+
+def test_emg_des_device_matches_oracle(runtime):
+    """EMG_DES_t pdf and cdf on the device against the oracle, across both
+    branches of the reference's erfcx (|u| below and above 4), z beyond the
+    curves' knots, ltr below 0.5, and +-inf for the cdf."""
+    from test_oracle_des import emg_par
+    par = emg_par()
+    rng = np.random.default_rng(41)
+    n = 3000
+    ltr = rng.uniform(0.1, 150.0, n)
+    x = np.column_stack([ltr * rng.uniform(0.0, 2.5, n), ltr, rng.uniform(0.0, 1.0, n)])
+    x[:2, 0] = [np.inf, -np.inf]
+    for which, fn in ((y3.F_EMG_DES_PDF, O.lib.y3o_emg_des_pdf),
+                      (y3.F_EMG_DES_CDF, O.lib.y3o_emg_des_cdf)):
+        rows = x if which == y3.F_EMG_DES_CDF else x[2:]
+        got = runtime.eval_functor(which, rows, par)
+        ref = np.array([fn(O.ptr(par), *row) for row in rows])
+        assert np.all(np.abs(got - ref) <= 1e-290 + 1e-12 * np.abs(ref)), \
+            (which, np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-290)))
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_EMG_DES_CDF, x[:4], par[:-1])

@@ -121,3 +121,66 @@ def test_mor_des_skew_normal_moments():
         d = a / np.sqrt(1 + a * a)
         assert abs(mass - 1) < 1e-9, (lnM, zt, mass)
         assert abs(mean - (lam + s * d * np.sqrt(2 / np.pi))) < 1e-8 * max(1.0, lam), (lnM, mean)
+
+
+# ---------------------------------------------------------------- EMG_DES_t
+
+EMG_Z = np.array([0.1, 0.3, 0.5, 0.8])
+
+
+def emg_par(curves=None):
+    """par = {n, z, a_mu, b_mu, a_sig, b_sig, a_tau, b_tau, a_fprj, b_fprj}:
+    DES-like curves (mu ~ l, sigma ~ l^0.5, tau ~ 1/l, fprj ~ 0.2) that vary
+    linearly in z."""
+    if curves is None:
+        t = (EMG_Z - 0.1) / 0.7
+        curves = [0.5 + 0.5 * t, 0.95 + 0.1 * t, 0.5 + 0.05 * t, 1.0 + 0.2 * t,
+                  0.9 + 0.1 * t, 2.0 - 0.5 * t, 1.0 + 0 * t, 0.2 + 0.1 * t]
+    return np.ascontiguousarray(np.concatenate([[len(EMG_Z)], EMG_Z] + list(curves)))
+
+
+def emg_params(par, ltr, z):
+    out = np.zeros(4)
+    O.lib.y3o_emg_des_params(O.ptr(par), ltr, z, O.ptr(out))
+    return out
+
+
+def test_emg_des_curves_linear_between_knots_and_clamped():
+    par = emg_par()
+    for z in (0.1, 0.2, 0.45, 0.8):
+        t = (z - 0.1) / 0.7
+        mu, s, tau, f = emg_params(par, 10.0, z)
+        assert abs(mu - ((0.5 + 0.5 * t) + (0.95 + 0.1 * t) * 10.0)) < 1e-12
+        assert abs(s - (1.0 + 0.2 * t) * 10.0 ** (0.5 + 0.05 * t)) < 1e-12
+        assert abs(tau - (2.0 - 0.5 * t) / 10.0 ** (0.9 + 0.1 * t)) < 1e-12
+        assert abs(f - (0.2 + 0.1 * t) / (1 + np.exp(-10.0))) < 1e-12
+    assert np.all(emg_params(par, 10.0, 1.5) == emg_params(par, 10.0, 0.8))
+    assert np.all(emg_params(par, 0.1, 0.3) == emg_params(par, 0.5, 0.3))   # ltr >= 0.5
+
+
+def test_emg_des_pdf_cdf_consistent_and_moments():
+    """cdf' = pdf; unit mass; with the projection fraction forced to 0 and
+    to 1 (b_fprj = 0 / 1e9, clamped to 1) the mean and variance are the
+    Gaussian's and the EMG's (mu + 1/tau, sigma^2 + 1/tau^2)."""
+    from scipy import integrate
+    par = emg_par()
+    for ltr, z in ((5.0, 0.2), (25.0, 0.45), (80.0, 0.7)):
+        mu, s, tau, f = emg_params(par, ltr, z)
+        pdf = lambda x: O.lib.y3o_emg_des_pdf(O.ptr(par), x, ltr, z)
+        cdf = lambda x: O.lib.y3o_emg_des_cdf(O.ptr(par), x, ltr, z)
+        lo, hi = mu - 30 * s, mu + 30 * s + 60 / tau
+        assert abs(integrate.quad(pdf, lo, hi, points=[mu], limit=400, epsrel=1e-12)[0] - 1) < 1e-8
+        for x in np.linspace(mu - 3 * s, mu + 3 * s + 2 / tau, 9):
+            h = 1e-4 * s
+            d = (cdf(x + h) - cdf(x - h)) / (2 * h)
+            assert abs(d - pdf(x)) < 2e-4 * pdf(x) + 1e-9, (ltr, z, x, d, pdf(x))
+        assert cdf(-np.inf) == 0.0 and cdf(np.inf) == 1.0
+        for bf, m1, v in ((0.0, mu, s * s), (1e9, mu + 1 / tau, s * s + 1 / tau ** 2)):
+            curves = list(par[1 + len(EMG_Z):].reshape(8, -1))
+            curves[7] = np.full(len(EMG_Z), bf)
+            pq = emg_par(curves)
+            g = lambda x, k: x ** k * O.lib.y3o_emg_des_pdf(O.ptr(pq), x, ltr, z)
+            mean = integrate.quad(g, lo, hi, args=(1,), points=[mu], limit=400, epsrel=1e-12)[0]
+            m2 = integrate.quad(g, lo, hi, args=(2,), points=[mu], limit=400, epsrel=1e-12)[0]
+            assert abs(mean - m1) < 1e-7 * max(1, abs(m1)), (bf, mean, m1)
+            assert abs(m2 - mean ** 2 - v) < 1e-6 * v, (bf, m2 - mean ** 2, v)

@@ -941,6 +941,91 @@ __device__ inline double mor_des(const double* q, double lt, double lnM, double 
          erfc(-skw * x / (1.4142135623730951 * sigma));
 }
 
+// EMG_DES_t (emg_des_t.cuh:59-206): the DES observed-richness distribution
+// P(lob | ltr, z), a Gaussian plus an exponentially modified Gaussian
+// (projection) tail.  Its eight parameter curves are linear interpolations
+// in z, clamped at the ends (quad::Interp1D::clamp over plob_ltr_params/z),
+// packed as
+//   par = {n, z[n], a_mu[n], b_mu[n], a_sig[n], b_sig[n], a_tau[n],
+//          b_tau[n], a_fprj[n], b_fprj[n]}.
+__device__ inline double emg_des_curve(const double* zs, const double* v, int n, double z)
+{
+  double const x = z < zs[0] ? zs[0] : (z > zs[n - 1] ? zs[n - 1] : z);
+  int const i = bracket(zs, n, x);
+  return v[i] + (x - zs[i]) * (v[i + 1] - v[i]) / (zs[i + 1] - zs[i]);
+}
+
+// get_params (:95-117): mu = a_mu + b_mu l, sigma = b_sig l^a_sig,
+// tau = b_tau / l^a_tau, fprj = b_fprj / (1 + e^-l)^a_fprj, l = max(ltr, 0.5),
+// then sigma, tau >= 1e-8 and fprj in [0, 1].
+__device__ inline void emg_des_params(const double* par, double ltr, double z, double& mu,
+                                      double& sigma, double& tau, double& fprj)
+{
+  int const n = int(par[0]);
+  const double* zs = par + 1;
+  double c[8];
+  for (int k = 0; k < 8; ++k) c[k] = emg_des_curve(zs, zs + n * (k + 1), n, z);
+  double const l = fmax(ltr, 0.5);
+  mu = c[0] + c[1] * l;
+  sigma = c[3] * pow(l, c[2]);
+  tau = c[5] / pow(l, c[4]);
+  double const denom = pow(1.0 + exp(-l), c[6]);
+  fprj = c[7] / fmax(denom, 1e-300);
+  sigma = fmax(sigma, 1e-8);
+  tau = fmax(tau, 1e-8);
+  fprj = fmax(0.0, fmin(1.0, fprj));
+}
+
+// The reference's scaled complementary error function (:37-51): exp(x^2)
+// erfc(x) below |x| = 4, a three-term asymptotic series above (reflected
+// for x < 0).
+__device__ inline double emg_des_erfcx(double x)
+{
+  double const ax = fabs(x);
+  if (ax < 4.0) return exp(x * x) * erfc(x);
+  double const x2 = ax * ax;
+  double r = (1.0 / 1.7724538509055159) / ax * (1.0 - 0.5 / x2 + 0.75 / (x2 * x2));
+  if (x < 0.0) r = 2.0 * exp(x2) - r;
+  return r;
+}
+
+// cdf() (:128-156): Phi((x - mu)/sigma) - fprj [tail], the tail through
+// the erfcx form, clamped to [0, 1]; +-inf give 1 / 0.
+__device__ inline double emg_des_cdf(const double* par, double lob, double ltr, double z)
+{
+  if (!isfinite(lob)) return lob > 0.0 ? 1.0 : 0.0;
+  double mu, sigma, tau, fprj;
+  emg_des_params(par, ltr, z, mu, sigma, tau, fprj);
+  double const zs = (lob - mu) / sigma;
+  double const gcdf = 0.5 * (1.0 + erf(zs * 0.7071067811865475));
+  double const u = (tau * sigma - zs) * 0.7071067811865475;
+  double const base = 0.5 * emg_des_erfcx(fabs(u)) * exp(-0.5 * zs * zs);
+  double tail;
+  if (u < 0.0) {
+    double A = -tau * (lob - mu) + 0.5 * tau * tau * sigma * sigma;
+    A = fmax(-700.0, fmin(700.0, A));
+    tail = exp(A) - base;
+  } else {
+    tail = base;
+  }
+  double const r = gcdf - fprj * tail;
+  return fmax(0.0, fmin(1.0, r));
+}
+
+// operator() (:160-178): (1 - fprj) N(lob; mu, sigma) + fprj EMG(lob).
+__device__ inline double emg_des_pdf(const double* par, double lob, double ltr, double z)
+{
+  double mu, sigma, tau, fprj;
+  emg_des_params(par, ltr, z, mu, sigma, tau, fprj);
+  double const zz = (lob - mu) / sigma;
+  double const g = exp(-zz * zz / 2.0) * 0.3989422804014327 / sigma;
+  double const ea = 0.5 * tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lob);
+  double const eb = (mu + tau * sigma * sigma - lob) / (1.4142135623730951 * sigma);
+  double emg = 0.0;
+  if (ea > -700.0 && ea < 700.0) emg = 0.5 * tau * exp(ea) * erfc(eb);
+  return (1.0 - fprj) * g + fprj * emg;
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

@@ -782,8 +782,20 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_MOR_DES)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_EMG_DES_CDF)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_EMG_DES_PDF || which == Y3G_F_EMG_DES_CDF) {
+    // par = {n, z[n], eight parameter curves of n values}
+    if (nin != 3 || !par || npar < 1)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: EMG_DES takes (lob, ltr, z) and curves");
+    double const a = par[0];
+    if (!(a >= 2.0 && a <= 65536.0) || a != double(int(a)) ||
+        (long long)npar != 1 + 9LL * (long long)a)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: EMG_DES curve sizes");
+    for (int k = 1; k < int(a); ++k)
+      if (!(par[1 + k] > par[k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: EMG_DES z knots not increasing");
+  }
   if (which == Y3G_F_MOR_DES && (nin != 3 || !par || npar != 6))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: MOR_DES takes (lt, lnM, zt) and "
                              "{A, B, alpha, sigma_intr, epsilon, z_pivot}");
