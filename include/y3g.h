@@ -279,7 +279,10 @@ typedef enum {
   Y3G_F_EMG_DES_PDF = 18,     /* (lob, ltr, z); par = {n, z[n], a_mu, b_mu,
                                  a_sig, b_sig, a_tau, b_tau, a_fprj, b_fprj
                                  (n each)}          EMG_DES_t(), FP64      */
-  Y3G_F_EMG_DES_CDF = 19      /* the same arguments  EMG_DES_t::cdf, FP64  */
+  Y3G_F_EMG_DES_CDF = 19,     /* the same arguments  EMG_DES_t::cdf, FP64  */
+  Y3G_F_SIG_MAX = 20          /* (r, lnM, zt); par = {n_r, n_m, n_z, r, lnM, z,
+                                 sigma_1 [n_m][n_r], sigma_2 [n_z][n_r],
+                                 bias [n_m][n_z]}   SIG_MAX, FP64          */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

@@ -130,6 +130,7 @@ def _load():
         "y3o_emg_des_params": (None, [_PD, _D, _D, _PD]),
         "y3o_emg_des_cdf": (_D, [_PD, _D, _D, _D]),
         "y3o_emg_des_pdf": (_D, [_PD, _D, _D, _D]),
+        "y3o_sig_max": (_D, [_PD, _D, _D, _D]),
         "y3o_sel_function": (_D, [_PD, C.c_int, _PD, C.c_int, _PD, _D, _D]),
         "y3o_gl_nodes_ab": (None, [C.c_int, _D, _D, _PD, _PD]),
         "y3o_selgl_counts": (_D, [C.POINTER(Params), C.POINTER(Tables), _PD, C.c_int, _PD,

@@ -117,6 +117,7 @@ double y3o_mor_des(const double* q, double lt, double lnM, double zt);
 void   y3o_emg_des_params(const double* par, double ltr, double z, double* out);
 double y3o_emg_des_cdf(const double* par, double lob, double ltr, double z);
 double y3o_emg_des_pdf(const double* par, double lob, double ltr, double z);
+double y3o_sig_max(const double* par, double r, double lnM, double zt);
 double y3o_dv_do_dz(double zt, const double* da_z, const double* da, int n,
                     double om, double ol, double ok, double h);
 double y3o_lc_lt_eq15(double lc, double lt, double mu, double sigma,

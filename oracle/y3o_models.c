@@ -33,6 +33,8 @@
  *   y3o_emg_des_*           emg_des_t.cuh:37-178       cdf' == pdf, unit mass, Gaussian
  *                                                      and EMG moments, curves linear
  *                                                      in z between knots, clamps
+ *   y3o_sig_max             sig_max.cuh:60-68          exact on bilinear tables in each
+ *                                                      of the two branches; clamps
  */
 #include "y3o.h"
 
@@ -326,6 +328,26 @@ y3o_emg_des_pdf(const double* par, double lob, double ltr, double z)
   double emg = 0.0;
   if (ea > -700.0 && ea < 700.0) emg = 0.5 * tau * exp(ea) * erfc(eb);
   return (1.0 - fprj) * y3o_gaussian(lob, mu, sigma) + fprj * emg;
+}
+
+/* SIG_MAX::operator() (sig_max.cuh:60-68): max(sigma_1(r, lnM),
+ * bias(zt, lnM) sigma_2(r, zt)), each an Interp2D::clamp over the sample's
+ * tables (make_interp_2d.cuh:26-40: axes and the ndarray in its own order,
+ * x-fastest).  par = {n_r, n_m, n_z, r, lnM, z, sigma_1, sigma_2, bias}. */
+double
+y3o_sig_max(const double* par, double r, double lnM, double zt)
+{
+  int const nr = (int)par[0], nm = (int)par[1], nz = (int)par[2];
+  const double* xr = par + 3;
+  const double* xm = xr + nr;
+  const double* xz = xm + nm;
+  const double* s1 = xz + nz;
+  const double* s2 = s1 + (size_t)nm * nr;
+  const double* bi = s2 + (size_t)nz * nr;
+  double const sig_1 = y3o_interp2d_clamp(xr, nr, xm, nm, s1, r, lnM);
+  double const sig_2 = y3o_interp2d_clamp(xz, nz, xm, nm, bi, zt, lnM) *
+                       y3o_interp2d_clamp(xr, nr, xz, nz, s2, r, zt);
+  return sig_1 < sig_2 ? sig_2 : sig_1;
 }
 
 /* dv_do_dz_t.hh:23-31 (D_A in Mpc, result (Mpc/h)^3 / sr). */

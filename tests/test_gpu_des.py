@@ -84,3 +84,21 @@ def test_emg_des_device_matches_oracle(runtime):
             (which, np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-290)))
     with pytest.raises(RuntimeError):
         runtime.eval_functor(y3.F_EMG_DES_CDF, x[:4], par[:-1])
+
+
+def test_sig_max_device_matches_oracle(runtime):
+    from test_oracle_des import sig_par
+    par = sig_par()
+    rng = np.random.default_rng(51)
+    n = 3000
+    x = np.column_stack([rng.uniform(0.01, 50.0, n), rng.uniform(29.0, 37.0, n),
+                         rng.uniform(0.0, 1.2, n)])
+    got = runtime.eval_functor(y3.F_SIG_MAX, x, par)
+    ref = np.array([O.lib.y3o_sig_max(O.ptr(par), *row) for row in x])
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(1.0, np.abs(ref)))
+    bad = par.copy()
+    bad[4] = bad[3]                   # r axis not increasing
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_SIG_MAX, x[:4], bad)
+    with pytest.raises(RuntimeError):
+        runtime.eval_functor(y3.F_SIG_MAX, x[:4], par[:-1])

@@ -184,3 +184,40 @@ def test_emg_des_pdf_cdf_consistent_and_moments():
             m2 = integrate.quad(g, lo, hi, args=(2,), points=[mu], limit=400, epsrel=1e-12)[0]
             assert abs(mean - m1) < 1e-7 * max(1, abs(m1)), (bf, mean, m1)
             assert abs(m2 - mean ** 2 - v) < 1e-6 * v, (bf, m2 - mean ** 2, v)
+
+
+# ------------------------------------------------------------------ SIG_MAX
+
+SIG_R = np.array([0.1, 0.3, 1.0, 3.0, 10.0, 30.0])
+SIG_M = np.array([30.0, 31.5, 33.0, 34.5, 36.0])
+SIG_Z = np.array([0.1, 0.4, 0.7, 1.0])
+S1 = (2.0, 0.5, -0.05, 0.01)      # sigma_1 = c + a r + d lnM + e r lnM
+S2 = (1.0, 0.8, 2.0, -0.2)        # sigma_2 = c + a r + d z + e r z
+BI = (-0.4, 0.5, 0.02, 0.001)     # bias    = c + a z + d lnM + e z lnM
+
+
+def sig_par():
+    """Bilinear tables, each x-fastest: sigma_1 over (r, lnM), sigma_2 over
+    (r, z), bias over (z, lnM)."""
+    def tab(xs, ys, c):
+        X, Y = np.meshgrid(xs, ys)   # [ny, nx], x fastest
+        return c[0] + c[1] * X + c[2] * Y + c[3] * X * Y
+    body = [tab(SIG_R, SIG_M, S1), tab(SIG_R, SIG_Z, S2), tab(SIG_Z, SIG_M, BI)]
+    return np.ascontiguousarray(np.concatenate(
+        [[len(SIG_R), len(SIG_M), len(SIG_Z)], SIG_R, SIG_M, SIG_Z] + [b.reshape(-1) for b in body]))
+
+
+def test_sig_max_bilinear_branches_and_clamp():
+    par = sig_par()
+    f = lambda r, m, z: O.lib.y3o_sig_max(O.ptr(par), r, m, z)
+    bil = lambda c, x, y: c[0] + c[1] * x + c[2] * y + c[3] * x * y
+    rng = np.random.default_rng(6)
+    wins = [0, 0]
+    for _ in range(400):
+        r, m, z = rng.uniform(0.1, 30.0), rng.uniform(30.0, 36.0), rng.uniform(0.1, 1.0)
+        a, b = bil(S1, r, m), bil(BI, z, m) * bil(S2, r, z)
+        wins[int(b > a)] += 1
+        assert abs(f(r, m, z) - max(a, b)) < 1e-11 * max(1.0, abs(a), abs(b))
+    assert min(wins) > 20, wins   # both branches exercised
+    assert f(100.0, 40.0, 2.0) == f(30.0, 36.0, 1.0)
+    assert f(0.01, 20.0, 0.0) == f(0.1, 30.0, 0.1)

@@ -20,7 +20,8 @@ FP32SPLIT, FP64 = 0, 1
 (F_ROFFSET_GAMMA, F_LO_LC, F_ZO_ZT_DES, F_INT_ZO_ZT_DES, F_SIGMA_PHOTOZ,
  F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
  F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND, F_INTEGRAND_PAIRED,
- F_INTEGRAND_NODE, F_INT_LC_LT_DES, F_MOR_DES, F_EMG_DES_PDF, F_EMG_DES_CDF) = range(20)
+ F_INTEGRAND_NODE, F_INT_LC_LT_DES, F_MOR_DES, F_EMG_DES_PDF, F_EMG_DES_CDF,
+ F_SIG_MAX) = range(21)
 
 
 class Params(C.Structure):

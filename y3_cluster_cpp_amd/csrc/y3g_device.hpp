@@ -1026,6 +1026,47 @@ __device__ inline double emg_des_pdf(const double* par, double lob, double ltr, 
   return (1.0 - fprj) * g + fprj * emg;
 }
 
+// Clamped bilinear lookup in a table over (xs[nx], ys[ny]) stored x-fastest,
+// value(x_i, y_j) = z[j nx + i]: the convention of the reference's Interp2D
+// for a flattened table (interp_2d.cc:115-127; the DES tables' expected
+// values confirm it for the device class).
+__device__ inline double interp2d_clamp(const double* xs, int nx, const double* ys, int ny,
+                                        const double* z, double x, double y)
+{
+  double const xc = x < xs[0] ? xs[0] : (x > xs[nx - 1] ? xs[nx - 1] : x);
+  double const yc = y < ys[0] ? ys[0] : (y > ys[ny - 1] ? ys[ny - 1] : y);
+  int const i = bracket(xs, nx, xc), j = bracket(ys, ny, yc);
+  double const t = (xc - xs[i]) / (xs[i + 1] - xs[i]);
+  double const u = (yc - ys[j]) / (ys[j + 1] - ys[j]);
+  const double* r0 = z + j * nx + i;
+  const double* r1 = r0 + nx;
+  return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r0[1] + (1.0 - t) * u * r1[0] +
+         t * u * r1[1];
+}
+
+// SIG_MAX::operator() (sig_max.cuh:60-68): the halo Delta-Sigma as the larger
+// of the one-halo term sigma_1(r, lnM) and bias(zt, lnM) sigma_2(r, zt), each
+// a clamped bilinear lookup in the sample's tables (deltasigma/{r_sigma_
+// deltasigma, lnM, sigma_1, sigma_2, bias}, matter_power_lin/z; make_interp_
+// 2d.cuh:26-40).  Packed as
+//   par = {n_r, n_m, n_z, r[n_r], lnM[n_m], z[n_z],
+//          sigma_1[n_m][n_r], sigma_2[n_z][n_r], bias[n_m][n_z]}
+// (each table x-fastest in its (x, y) = (r, lnM), (r, z), (z, lnM)).
+__device__ inline double sig_max(const double* par, double r, double lnM, double zt)
+{
+  int const nr = int(par[0]), nm = int(par[1]), nz = int(par[2]);
+  const double* xr = par + 3;
+  const double* xm = xr + nr;
+  const double* xz = xm + nm;
+  const double* s1 = xz + nz;
+  const double* s2 = s1 + (size_t)nm * nr;
+  const double* bi = s2 + (size_t)nz * nr;
+  double const a = interp2d_clamp(xr, nr, xm, nm, s1, r, lnM);
+  double const b = interp2d_clamp(xz, nz, xm, nm, bi, zt, lnM) *
+                   interp2d_clamp(xr, nr, xz, nz, s2, r, zt);
+  return a < b ? b : a;  // std::max(sig_1, sig_2)
+}
+
 // ------------------------------------------------ Genz-Malik rule nodes
 constexpr double kL2 = 0.35856858280031809;  // sqrt(9/70)
 constexpr double kL3 = 0.94868329805051380;  // sqrt(9/10)

@@ -628,6 +628,7 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     case Y3G_F_MOR_DES: r = T(mor_des(q, x[0], x[1], x[2])); break;
     case Y3G_F_EMG_DES_PDF: r = T(emg_des_pdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_EMG_DES_CDF: r = T(emg_des_cdf(q, x[0], x[1], x[2])); break;
+    case Y3G_F_SIG_MAX: r = T(sig_max(q, x[0], x[1], x[2])); break;
     case Y3G_F_DV_DO_DZ: r = dv_do_dz<T>(P, tb.da, x[0]); break;
     case Y3G_F_HMF: r = hmf<T>(P, tb.hmf, x[0], x[1]); break;
     case Y3G_F_MOR: {

@@ -782,8 +782,25 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_EMG_DES_CDF)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_SIG_MAX)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_SIG_MAX) {
+    // par = {n_r, n_m, n_z, r, lnM, z, sigma_1, sigma_2, bias}
+    if (nin != 3 || !par || npar < 3)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: SIG_MAX takes (r, lnM, zt) and tables");
+    double const a = par[0], b = par[1], c = par[2];
+    bool ok = a >= 2.0 && b >= 2.0 && c >= 2.0 && a <= 4096.0 && b <= 4096.0 && c <= 4096.0 &&
+              a == double(int(a)) && b == double(int(b)) && c == double(int(c));
+    if (ok) {
+      long long const nr = (long long)a, nm = (long long)b, nz = (long long)c;
+      ok = (long long)npar == 3 + nr + nm + nz + nm * nr + nz * nr + nm * nz;
+      long long const n[3] = {nr, nm, nz};
+      long long off = 3;
+      for (int ax = 0; ok && ax < 3; off += n[ax], ++ax)
+        for (long long k = 1; ok && k < n[ax]; ++k) ok = par[off + k] > par[off + k - 1];
+    }
+    if (!ok) return fail(Y3G_ERR_ARG, "y3g_eval_functor: SIG_MAX table sizes or axes");
+  }
   if (which == Y3G_F_EMG_DES_PDF || which == Y3G_F_EMG_DES_CDF) {
     // par = {n, z[n], eight parameter curves of n values}
     if (nin != 3 || !par || npar < 1)
