@@ -58,7 +58,11 @@ typedef enum {
 typedef enum {
   Y3G_COUNT3D = 0,  /* (t, zt, lnM), lambda_obs bin analytic (erf)        */
   Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM)                               */
-  Y3G_DSIGMA6D = 2  /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
+  Y3G_DSIGMA6D = 2, /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
+  Y3G_DSIGMA6D_DES = 3 /* (lo, lc, zt, lnM, rmis, theta) in the volume's own
+                          units: the reference's real-data integrand of
+                          SigmaMiscentY1MortCUDAScalarIntegrand with the
+                          point's DES data (y3g_des_upload); FP64 cubature */
 } y3g_kind;
 
 /* Integrators: the names of gpu_integrator.cuh:54-65. */
@@ -223,6 +227,21 @@ int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes
 /* Time of the last y3g_integrate's kernels in milliseconds (HIP events on
  * the library's stream) and the number of integrand evaluations they made. */
 int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
+
+/* ---- Real-data DES inputs of the Delta-Sigma integrand --------------------
+ * The DES data of parameter point `point` for Y3G_DSIGMA6D_DES integrals
+ * (the reference's SigmaMiscentY1MortCUDAScalarIntegrand::set_sample,
+ * sigma_miscent_y1_mort_cuda_scalarintegrand.cu:140-150):
+ *   des = {roffset_tau,                                   ROFFSET_t
+ *          LO_LC_alpha, LO_LC_a, LO_LC_b, LO_LC_R_lambda,  LO_LC_t
+ *          mor_A, mor_B, mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot,
+ *                                                         MOR_DES_t
+ *          n_r, n_m, n_z, r[n_r], lnM[n_m], z[n_z],
+ *          sigma_1[n_m][n_r], sigma_2[n_z][n_r], bias[n_m][n_z]}   SIG_MAX
+ * (the SIG_MAX tables x-fastest, as Interp2D reads the sample's ndarrays).
+ * The point's HMF and D_A tables are its y3g_tables_build / _upload ones.
+ * Replaces any earlier upload for the point. */
+int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des);
 
 /* ---- Fixed Gauss-Legendre number counts with a selection function --------
  * The reference's NumCountsSelGL (src/models/n_operator_sel_gl_t.hh:171-201

@@ -116,6 +116,8 @@ def _load():
         "y3o_dvdz": (_D, [C.POINTER(Params), C.POINTER(Tables), _D]),
         "y3o_sci": (_D, [C.POINTER(Tables), _D]),
         "y3o_integrand": (_D, [C.POINTER(Params), C.POINTER(Tables), C.POINTER(Integral), _PD]),
+        "y3o_integrand_des": (_D, [C.POINTER(Params), C.POINTER(Tables), _PD,
+                                   C.POINTER(Integral), _PD]),
         "y3o_integrand_mapped": (_D, [C.POINTER(Params), C.POINTER(Tables), C.POINTER(Integral), _PD]),
         "y3o_integrand_mapped_batch": (None, [C.POINTER(Params), C.POINTER(Tables),
                                               C.POINTER(Integral), _PD, C.c_long, _PD]),
@@ -143,6 +145,8 @@ def _load():
         "y3o_golden_set_orders": (None, [C.c_int, C.c_int]),
         "y3o_cubature": (None, [C.POINTER(Params), C.POINTER(Tables), C.POINTER(Integral),
                                 C.POINTER(CubOpts), C.POINTER(Result)]),
+        "y3o_cubature_des": (None, [C.POINTER(Params), C.POINTER(Tables), _PD,
+                                    C.POINTER(Integral), C.POINTER(CubOpts), C.POINTER(Result)]),
         "y3o_cubature_fn": (None, [_FNN, C.c_void_p, C.c_int, _PD, _PD,
                                    C.POINTER(CubOpts), C.POINTER(Result)]),
         "y3o_gm_rule": (_D, [_FNN, C.c_void_p, C.c_int, _PD, _PD, _PD, _PD]),

@@ -202,6 +202,12 @@ typedef struct {
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,
                   y3o_result* r);
+/* The real-data Delta-Sigma integrand (Y3G_DSIGMA6D_DES) and its cubature
+ * (y3o_models.c). */
+double y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
+                         const y3o_integral* ig, const double* x);
+void   y3o_cubature_des(const y3o_params* p, const y3o_tables* t, const double* des,
+                        const y3o_integral* ig, const y3o_cub_opts* o, y3o_result* r);
 /* Generic: integrate f over [lows, highs]. */
 void y3o_cubature_fn(double (*f)(const double*, void*), void* ctx, int ndim,
                      const double* lows, const double* highs,

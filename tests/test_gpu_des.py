@@ -102,3 +102,66 @@ def test_sig_max_device_matches_oracle(runtime):
         runtime.eval_functor(y3.F_SIG_MAX, x[:4], bad)
     with pytest.raises(RuntimeError):
         runtime.eval_functor(y3.F_SIG_MAX, x[:4], par[:-1])
+
+
+# ------------------------------------------- the real-data Delta-Sigma integrand
+
+def _des_setup(runtime):
+    import ctypes as C
+    from test_oracle_des import DES_BOX, des_data, des_integral
+    p = y3.default_params()
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    des = des_data()
+    runtime.des_upload(0, des)
+    lows = [b[0] for b in DES_BOX]
+    highs = [b[1] for b in DES_BOX]
+    ig = y3.make_integral_volume(p, spec, y3.DSIGMA6D_DES, lows, highs, (0.3, 0.45, 1.0),
+                                 eps_rel=1e-3)
+    op = O.default_params()
+    ot = O.OracleTables(op, O.spec_wide())
+    oig = des_integral(DES_BOX, zo=(0.3, 0.45), radius=1.0)
+    return C, ig, des, op, ot, oig
+
+
+def test_des_integrand_device_matches_oracle(runtime):
+    C, ig, des, op, ot, oig = _des_setup(runtime)
+    rng = np.random.default_rng(61)
+    n = 2000
+    from test_oracle_des import DES_BOX
+    x = np.column_stack([rng.uniform(l, h, n) for l, h in DES_BOX])
+    got = runtime.eval_functor(y3.F_INTEGRAND, x, aux=ig)
+    ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t), O.ptr(des),
+                                            C.byref(oig), O.ptr(np.ascontiguousarray(row)))
+                    for row in x])
+    scale = np.abs(ref).max()
+    assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * scale), \
+        np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))
+    with pytest.raises(RuntimeError):       # FP32 has no real-data path
+        runtime.eval_functor(y3.F_INTEGRAND, x[:4], aux=ig, precision=y3.FP32SPLIT)
+
+
+def test_des_cubature_matches_oracle(runtime):
+    """The batched FP64 cubature of the real-data integral against the
+    oracle's cubature of the same integrand: within their errors; a
+    non-FP64 or VEGAS request and a point without DES data are errors."""
+    C, ig, des, op, ot, oig = _des_setup(runtime)
+    r = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=16384))
+    o = O.Result()
+    O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(des), C.byref(oig),
+                           C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(o))
+    v, e = r["value"][0], r["error"][0]
+    assert abs(v - o.value) <= 2 * (e + o.error), (v, e, o.value, o.error)
+    assert abs(v / o.value - 1) < 3e-3, (v, o.value)
+    with pytest.raises(RuntimeError):
+        runtime.integrate([ig], y3.options(precision=y3.FP32SPLIT))
+    ig2 = y3.Integral.from_buffer_copy(ig)
+    ig2.algorithm = y3.MCUBES
+    with pytest.raises(RuntimeError):
+        runtime.integrate([ig2], y3.options(precision=y3.FP64))
+    ig3 = y3.Integral.from_buffer_copy(ig)
+    runtime.tables_build([y3.default_params(), y3.default_params(omega_m=0.28)], y3.spec_wide())
+    ig3.point = 1
+    with pytest.raises(RuntimeError):
+        runtime.integrate([ig3], y3.options(precision=y3.FP64))

@@ -4,6 +4,7 @@
 compared there with epsilon 1e-3 and margin 1e-5) and against what the
 mathematics fixes: exactness on bilinear tables between knots, the lambda_c
 bin edges, and clamping outside the knots."""
+import ctypes as C
 import json
 import os
 
@@ -221,3 +222,102 @@ def test_sig_max_bilinear_branches_and_clamp():
     assert min(wins) > 20, wins   # both branches exercised
     assert f(100.0, 40.0, 2.0) == f(30.0, 36.0, 1.0)
     assert f(0.01, 20.0, 0.0) == f(0.1, 30.0, 0.1)
+
+
+# ------------------------------------------- the real-data Delta-Sigma integrand
+
+def des_data(sig_const=None):
+    """{roffset_tau, LO_LC alpha a b R_lambda, MOR_DES A B alpha sigma eps z_p,
+    SIG_MAX}: with sig_const the SIG_MAX tables are constant (sigma_1 =
+    sig_const, sigma_2 = 0), else the bilinear tables of sig_par()."""
+    head = [0.2, 3.0, 0.3, 1.0, 0.9, 1e11, 10 ** 12.4, 0.75, 0.2, 0.1, 0.3]
+    sp = sig_par()
+    if sig_const is not None:
+        nr, nm, nz = len(SIG_R), len(SIG_M), len(SIG_Z)
+        o = 3 + nr + nm + nz
+        sp = sp.copy()
+        sp[o:o + nm * nr] = sig_const
+        sp[o + nm * nr:o + nm * nr + nz * nr] = 0.0
+    return np.ascontiguousarray(np.concatenate([head, sp]))
+
+
+def des_integral(box, zo=(0.3, 0.45), radius=1.0):
+    ig = O.Integral()
+    ig.kind, ig.ndim = 3, 6
+    ig.zo_lo, ig.zo_hi, ig.radius = zo[0], zo[1], radius
+    for a, (l, h) in enumerate(box):
+        ig.lows[a], ig.highs[a] = l, h
+    return ig
+
+
+DES_BOX = [(0.0, 600.0), (5.0, 150.0), (0.25, 0.5), (np.log(1.3e14), np.log(4e14)),
+           (0.2, 1.5), (0.0, 2 * np.pi)]
+
+
+def test_des_integrand_reduces_to_the_count_integral():
+    """With a constant SIG_MAX the 6-D real-data integral over a box that
+    holds the lo mass factorises: lo_lc integrates to 1 over lo, the angle
+    gives 2 pi / (2 x 3.1415926535897), the Gamma(2) offset integrates in
+    closed form over [r1, r2], and what is left is
+    int dzt dlnM dV/dz hmf Omega int_zo_zt P_MOR(lc box | lnM, zt) sigma,
+    here by Gauss-Legendre products of the separately pinned functors.  The
+    oracle's cubature of the 6-D integrand must agree within its error."""
+    p = O.default_params()
+    t = O.OracleTables(p, O.spec_wide())
+    des = des_data(sig_const=2.5)
+    ig = des_integral(DES_BOX)
+    r = O.Result()
+    O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
+                           C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(r))
+    tau, (r1, r2) = des[0], DES_BOX[4]
+    rfrac = (1 + r1 / tau) * np.exp(-r1 / tau) - (1 + r2 / tau) * np.exp(-r2 / tau)
+    q = np.ascontiguousarray(des[5:11])
+
+    def gl(n, a, b):
+        x, w = np.polynomial.legendre.leggauss(n)
+        return 0.5 * (b - a) * x + 0.5 * (a + b), 0.5 * (b - a) * w
+
+    xz, wz = gl(24, *DES_BOX[2])
+    xm, wm = gl(24, *DES_BOX[3])
+    xl, wl = gl(400, *DES_BOX[1])
+    exp = 0.0
+    for z, a in zip(xz, wz):
+        zf = (O.lib.y3o_dvdz(C.byref(p), C.byref(t.t), z) * O.lib.y3o_omega_z_des(z) *
+              O.lib.y3o_int_zo_zt_des(0.3, 0.45, z))
+        for m, b in zip(xm, wm):
+            mfrac = sum(w * O.lib.y3o_mor_des(O.ptr(q), lc, m, z) for lc, w in zip(xl, wl))
+            exp += a * b * zf * O.lib.y3o_hmf(C.byref(p), C.byref(t.t), m, z) * mfrac
+    exp *= 2.5 * rfrac * 2 * np.pi / (2 * 3.1415926535897)
+    assert abs(r.value - exp) <= max(2 * r.error, 1e-3 * exp), (r.value, exp, r.error, r.status)
+
+
+def test_des_integrand_offset_radius_geometry():
+    """The offset radius R' of the ring at angle theta: with sigma_1 linear
+    in r (sigma_2 = 0) the angular integral is 2 pi (c + d lnM) + (a + e lnM)
+    4 (R + rmis) E(k), k^2 = 4 R rmis / (R + rmis)^2 (E the complete elliptic
+    integral of the second kind), scaled by the integrand at theta = pi,
+    where R' = |R - rmis|."""
+    from scipy import integrate, special
+    p = O.default_params()
+    t = O.OracleTables(p, O.spec_wide())
+    des = des_data()
+    nr, nm, nz = len(SIG_R), len(SIG_M), len(SIG_Z)
+    o = 11 + 3 + nr + nm + nz
+    des[o + nm * nr:o + nm * nr + nz * nr] = 0.0          # sigma_2 = 0
+    for R, rm, lnM in ((1.0, 0.5, 33.0), (3.0, 0.8, 32.2), (0.7, 0.6, 34.1)):
+        ig = des_integral(DES_BOX, radius=R)
+        x = np.array([45.0, 40.0, 0.33, lnM, rm, 0.0])
+
+        def f(th):
+            x[5] = th
+            return O.lib.y3o_integrand_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
+                                           O.ptr(x))
+
+        c, a, d, e = S1
+        s1 = lambda r: c + a * r + d * lnM + e * r * lnM
+        K = f(np.pi) / s1(abs(R - rm))
+        k2 = 4 * R * rm / (R + rm) ** 2
+        exp = K * (2 * np.pi * (c + d * lnM) + (a + e * lnM) * 4 * (R + rm) * special.ellipe(k2))
+        got = integrate.quad(f, 0, 2 * np.pi, epsabs=0, epsrel=1e-12, limit=200)[0]
+        assert abs(got / exp - 1) < 1e-10, (R, rm, got, exp)

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liby3g.so")
 
-COUNT3D, COUNT5D, DSIGMA6D = 0, 1, 2
+COUNT3D, COUNT5D, DSIGMA6D, DSIGMA6D_DES = 0, 1, 2, 3
 PAGANI, MCUBES = 0, 1
 FP32SPLIT, FP64 = 0, 1
 
@@ -136,6 +136,7 @@ SIGNATURES = {
     "y3g_selfn_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD,
                                    C.c_int]),
     "y3g_selgl_counts": (C.c_int, [C.c_void_p, C.POINTER(SelGLSpec), C.c_int, C.c_int, _PD]),
+    "y3g_des_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int]),
     "y3g_eval_functor": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _PD, C.c_int,
                                    C.c_size_t, _PD, C.c_int,
                                    C.POINTER(Integral), _PD]),
@@ -305,6 +306,14 @@ class Runtime:
         _check(lib().y3g_integrate(self._h, arr, n, C.byref(o),
                                    out.ctypes.data_as(C.c_void_p)), "y3g_integrate")
         return out
+
+    def des_upload(self, point, des):
+        """DES data of a point for DSIGMA6D_DES integrals (y3g_des_upload):
+        {roffset_tau, LO_LC alpha a b R_lambda, MOR_DES A B alpha sigma eps
+        z_pivot, SIG_MAX n_r n_m n_z r lnM z sigma_1 sigma_2 bias}."""
+        d = np.ascontiguousarray(des, dtype=np.float64).reshape(-1)
+        _check(lib().y3g_des_upload(self._h, point, d.ctypes.data_as(_PD), d.size),
+               "y3g_des_upload")
 
     def selfn_upload(self, point, lnm, z, s_stack):
         """Selection function of a point: axes lnM [n_lnm], z [n_z] and

@@ -261,6 +261,7 @@ template <typename T, int BLOCK> struct CubShared {
   static constexpr bool kHmfLds = sizeof(T) == 8;
   T hmf[kHmfLds ? kMaxHmf : 1];
   const T* hmf_tab;  // the table the evaluation reads (LDS or global)
+  const double* des;  // the point's DES data (Y3G_DSIGMA6D_DES), global
   T da[kMaxDa];
   T sci[kMaxSci];
   // node values: the FP32 path keeps only those the fourth differences read
@@ -633,7 +634,10 @@ __device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
     double y[d];
 #pragma unroll
     for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_offset(d, j, a);
-    S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
+    if constexpr (K == Y3G_DSIGMA6D_DES)
+      S.f[q] = integrand_des(S.ig, S.P, tb, S.des, y);
+    else
+      S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
   }
 }
 
@@ -664,6 +668,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     switch (S.ig.kind) {
       case Y3G_COUNT3D: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
       case Y3G_COUNT5D: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D>(S, m, tb); break;
+      case Y3G_DSIGMA6D_DES: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D_DES>(S, m, tb); break;
       default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }

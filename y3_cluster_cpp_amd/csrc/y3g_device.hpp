@@ -500,6 +500,9 @@ template <> struct KindAxes<Y3G_COUNT5D> {
 template <> struct KindAxes<Y3G_DSIGMA6D> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 6;
 };
+template <> struct KindAxes<Y3G_DSIGMA6D_DES> {
+  static constexpr int it = 1, iz = 2, im = 3, nd = 6;
+};
 
 // Integrand of kind K (axis positions known at compile time).
 template <typename T, int K>
@@ -1065,6 +1068,37 @@ __device__ inline double sig_max(const double* par, double r, double lnM, double
   double const b = interp2d_clamp(xz, nz, xm, nm, bi, zt, lnM) *
                    interp2d_clamp(xr, nr, xz, nz, s2, r, zt);
   return a < b ? b : a;  // std::max(sig_1, sig_2)
+}
+
+// The point's DES data (y3g_des_upload): {roffset_tau, LO_LC alpha, a, b,
+// R_lambda, MOR_DES A, B, alpha, sigma_intr, epsilon, z_pivot, SIG_MAX
+// tables as packed for sig_max()}.
+constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
+
+// SigmaMiscentY1MortCUDAScalarIntegrand::operator()
+// (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182), FP64, in the
+// volume's own variables (lo, lc, zt, lnM, rmis, theta):
+//   roffset(rmis) lo_lc(lo, lc, rmis) mor_des(lc, lnM, zt) dV/dz hmf Omega(zt)
+//   / 2 / 3.1415926535897 (the reference's constant)
+//   x sig_max(R', lnM, zt) int_zo_zt(zo_lo, zo_hi, zt),
+//   R' = sqrt(R^2 + rmis^2 + 2 rmis R cos theta).
+__device__ inline double integrand_des(const y3g_integral& ig, const PointDev& P,
+                                       Tabs<double> tb, const double* des, const double* y)
+{
+  double const lo = y[0], lc = y[1], zt = y[2], lnM = y[3], rmis = y[4], th = y[5];
+  double const roffset_v = roffset_gamma<double>(rmis, des[0]);
+  double const lo_lc_v = lo_lc<double>(lo, lc, rmis, des[kDesLoLc], des[kDesLoLc + 1],
+                                       des[kDesLoLc + 2], des[kDesLoLc + 3]);
+  double const mor_v = mor_des(des + kDesMor, lc, lnM, zt);
+  double const dv = dv_do_dz<double>(P, tb.da, zt);
+  double const hm = hmf<double>(P, tb.hmf, lnM, zt);
+  double const om = omega_z_des(zt);
+  double const common = roffset_v * lo_lc_v * mor_v * dv * hm * om / 2.0 / 3.1415926535897;
+  double const R = ig.radius;
+  double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(th));
+  double const sig = sig_max(des + kDesSig, rs, lnM, zt);
+  double const izo = int_zo_zt_des<double>(ig.zo_lo, ig.zo_hi, zt);
+  return sig * izo * common;
 }
 
 // ------------------------------------------------ Genz-Malik rule nodes

@@ -39,6 +39,9 @@ struct KernArgs {
   const float* sci32;
   const double* sci64;
   int sci_stride;
+  // DES data of the points (Y3G_DSIGMA6D_DES), point p at des + des_off[p]
+  const double* des;
+  const long long* des_off;
   // cubature region slab: per slot lo/hi [kMaxDim][cap], est/err/axis [cap]
   double* slab_lo;
   double* slab_hi;
@@ -70,6 +73,7 @@ struct FunctorArgs {
   const void* hmf;
   const void* da;
   const void* sci;
+  const double* des;       // the point's DES data (Y3G_DSIGMA6D_DES integrands)
   const y3g_integral* ig;
   double* out;
 };

@@ -136,6 +136,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       if (tid == 0) {
         S.P = A.points[pt];
         S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
+        S.des = A.des ? A.des + A.des_off[pt] : nullptr;
       }
       stage_tables<T>(A, pt, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
       __syncthreads();
@@ -656,7 +657,15 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
       break;
     }
     case Y3G_F_SCI: r = lerp_clamp<T>(tb.sci, P.nsci, x[0], P.inv_dzsci); break;
-    case Y3G_F_INTEGRAND: r = integrand<T>(*F.ig, P, tb, x); break;
+    case Y3G_F_INTEGRAND:
+      if constexpr (sizeof(T) == 8) {
+        if (F.ig->kind == Y3G_DSIGMA6D_DES) {
+          r = integrand_des(*F.ig, P, tb, F.des, x);
+          break;
+        }
+      }
+      r = integrand<T>(*F.ig, P, tb, x);
+      break;
     case Y3G_F_INTEGRAND_PAIRED: {
       int it, iz, im;
       kind_axes<T>(F.ig->kind, it, iz, im);

@@ -143,7 +143,66 @@ struct y3g_runtime {
   bool selnodes_valid = false;
   double* d_selout = nullptr;
   size_t selout_cap = 0;
+  // DES data per point (Y3G_DSIGMA6D_DES), uploaded as one buffer when changed
+  std::vector<std::vector<double>> des;
+  bool des_dirty = true;
+  double* d_des = nullptr;
+  long long* d_des_off = nullptr;
+  size_t des_cap = 0, des_off_cap = 0;
 };
+
+// SIG_MAX tables packed as sig_max() reads them: sizes, increasing axes.
+static bool sigmax_ok(const double* par, long long npar)
+{
+  if (npar < 3) return false;
+  double const a = par[0], b = par[1], c = par[2];
+  if (!(a >= 2.0 && b >= 2.0 && c >= 2.0 && a <= 4096.0 && b <= 4096.0 && c <= 4096.0) ||
+      a != double(int(a)) || b != double(int(b)) || c != double(int(c)))
+    return false;
+  long long const nr = (long long)a, nm = (long long)b, nz = (long long)c;
+  if (npar != 3 + nr + nm + nz + nm * nr + nz * nr + nm * nz) return false;
+  long long const n[3] = {nr, nm, nz};
+  long long off = 3;
+  for (int ax = 0; ax < 3; off += n[ax], ++ax)
+    for (long long k = 1; k < n[ax]; ++k)
+      if (!(par[off + k] > par[off + k - 1])) return false;
+  return true;
+}
+
+// The points' DES data on the device (one buffer, offsets per point).
+static int ensure_des(y3g_runtime* rt)
+{
+  if (!rt->des_dirty) return 0;
+  std::vector<double> data;
+  std::vector<long long> off(std::max<size_t>(rt->des.size(), 1), 0);
+  for (size_t p = 0; p < rt->des.size(); ++p) {
+    off[p] = (long long)data.size();
+    data.insert(data.end(), rt->des[p].begin(), rt->des[p].end());
+  }
+  if (data.empty()) data.push_back(0.0);
+  if (data.size() > rt->des_cap) {
+    dfree(rt->d_des);
+    rt->des_cap = 0;
+    if (int rc = dalloc(&rt->d_des, data.size())) return rc;
+    rt->des_cap = data.size();
+  }
+  if (off.size() > rt->des_off_cap) {
+    dfree(rt->d_des_off);
+    rt->des_off_cap = 0;
+    if (int rc = dalloc(&rt->d_des_off, off.size())) return rc;
+    rt->des_off_cap = off.size();
+  }
+  HIPCHK(hipMemcpy(rt->d_des, data.data(), sizeof(double) * data.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rt->d_des_off, off.data(), sizeof(long long) * off.size(),
+                   hipMemcpyHostToDevice));
+  rt->des_dirty = false;
+  return 0;
+}
+
+static bool has_des(const y3g_runtime* rt, int point)
+{
+  return point >= 0 && (size_t)point < rt->des.size() && !rt->des[point].empty();
+}
 
 extern "C" {
 
@@ -271,6 +330,37 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
 {
   if (!p || !s || !lows || !highs || !ig)
     return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: null argument");
+  if (kind == Y3G_DSIGMA6D_DES) {
+    // the reference's volume as it is (lo, lc, zt, lnm, rmis, theta) and its
+    // grid point (zo_low, zo_high, radius), no change of variables
+    if (n_grid != 3 || !grid)
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: kind 3 takes 3 grid values, got " +
+                                   std::to_string(n_grid));
+    for (int i = 0; i < 6; ++i)
+      if (!(lows[i] <= highs[i]) || !std::isfinite(lows[i]) || !std::isfinite(highs[i]))
+        return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: axis " + std::to_string(i) +
+                                     " has low > high or a non-finite bound");
+    if (!(lows[1] > 0.0))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: lc range must be > 0");
+    if (!(lows[2] >= 0.0) || !(highs[2] <= s->da_zmax))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: zt range outside the distance "
+                               "table [0, " + std::to_string(s->da_zmax) + "]");
+    if (!(grid[2] > 0.0) || !(grid[0] <= grid[1]))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: grid needs zo_low <= zo_high "
+                               "and radius > 0");
+    std::memset(ig, 0, sizeof(*ig));
+    ig->kind = kind;
+    ig->ndim = 6;
+    ig->zo_lo = grid[0];
+    ig->zo_hi = grid[1];
+    ig->radius = grid[2];
+    for (int i = 0; i < 6; ++i) ig->lows[i] = lows[i], ig->highs[i] = highs[i];
+    ig->eps_rel = 1e-4;
+    ig->eps_abs = 1e-300;
+    ig->max_eval = 1LL << 40;
+    ig->seed = 0x59334331ull;
+    return 0;
+  }
   int const nd = kind == Y3G_COUNT3D ? 3 : kind == Y3G_COUNT5D ? 5
                : kind == Y3G_DSIGMA6D ? 6 : 0;
   int const ng = kind == Y3G_COUNT3D ? 2 : kind == Y3G_COUNT5D ? 0 : 3;
@@ -388,6 +478,7 @@ int y3g_runtime_destroy(y3g_runtime* rt)
   dfree(rt->sg.perr), dfree(rt->sg.partial), dfree(rt->sg.IE), dfree(rt->sg.hc);
   dfree(rt->sg.hq), dfree(rt->sg.blockcnt), dfree(rt->sg.blockoff), dfree(rt->d_one);
   dfree(rt->d_seldata), dfree(rt->d_selpts), dfree(rt->d_selnodes), dfree(rt->d_selout);
+  dfree(rt->d_des), dfree(rt->d_des_off);
   if (rt->ev0) (void)hipEventDestroy(rt->ev0);
   if (rt->ev1) (void)hipEventDestroy(rt->ev1);
   if (rt->stream) (void)hipStreamDestroy(rt->stream);
@@ -667,13 +758,24 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     igs = banded.data();
   }
   std::vector<int> cub, veg;
+  bool any_des = false;
   for (size_t i = 0; i < n; ++i) {
     y3g_integral const& g = igs[i];
     if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
-        g.point >= rt->n_points || g.kind < 0 || g.kind > 2 ||
+        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_DSIGMA6D_DES ||
         !(g.eps_rel >= 0) || g.max_eval < 1)
       return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                  " is malformed");
+    if (g.kind == Y3G_DSIGMA6D_DES) {
+      if (g.ndim != 6 || o.precision != Y3G_FP64 || g.algorithm != Y3G_PAGANI || o.single)
+        return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
+                                   ": the real-data Delta-Sigma integrand runs on the FP64 "
+                                   "batched cubature only (precision fp64, pagani)");
+      if (!has_des(rt, g.point))
+        return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
+                                     ": its point has no DES data (y3g_des_upload)");
+      any_des = true;
+    }
     for (int a = 0; a < g.ndim; ++a)
       if (!(g.highs[a] > g.lows[a]))
         return fail(Y3G_ERR_ARG, "y3g_integrate: empty volume in integral " +
@@ -681,6 +783,8 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     (g.algorithm == Y3G_MCUBES ? veg : cub).push_back((int)i);
   }
   HIPCHK(hipSetDevice(rt->device));
+  if (any_des)
+    if (int rc = ensure_des(rt)) return rc;
   if (int rc = ensure_batch(rt, n)) return rc;
   HIPCHK(hipMemcpyAsync(rt->d_igs, igs, sizeof(y3g_integral) * n,
                         hipMemcpyHostToDevice, rt->stream));
@@ -692,6 +796,8 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
   A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
   A.sci32 = rt->sci32, A.sci64 = rt->sci64, A.sci_stride = rt->sci_stride;
+  A.des = any_des ? rt->d_des : nullptr;
+  A.des_off = any_des ? rt->d_des_off : nullptr;
   A.cap = o.max_regions;
   A.kmax = o.max_split_per_step;
   A.init_split = o.init_split;
@@ -786,20 +892,10 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
   if (which == Y3G_F_SIG_MAX) {
     // par = {n_r, n_m, n_z, r, lnM, z, sigma_1, sigma_2, bias}
-    if (nin != 3 || !par || npar < 3)
+    if (nin != 3 || !par)
       return fail(Y3G_ERR_ARG, "y3g_eval_functor: SIG_MAX takes (r, lnM, zt) and tables");
-    double const a = par[0], b = par[1], c = par[2];
-    bool ok = a >= 2.0 && b >= 2.0 && c >= 2.0 && a <= 4096.0 && b <= 4096.0 && c <= 4096.0 &&
-              a == double(int(a)) && b == double(int(b)) && c == double(int(c));
-    if (ok) {
-      long long const nr = (long long)a, nm = (long long)b, nz = (long long)c;
-      ok = (long long)npar == 3 + nr + nm + nz + nm * nr + nz * nr + nm * nz;
-      long long const n[3] = {nr, nm, nz};
-      long long off = 3;
-      for (int ax = 0; ok && ax < 3; off += n[ax], ++ax)
-        for (long long k = 1; ok && k < n[ax]; ++k) ok = par[off + k] > par[off + k - 1];
-    }
-    if (!ok) return fail(Y3G_ERR_ARG, "y3g_eval_functor: SIG_MAX table sizes or axes");
+    if (!sigmax_ok(par, npar))
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: SIG_MAX table sizes or axes");
   }
   if (which == Y3G_F_EMG_DES_PDF || which == Y3G_F_EMG_DES_CDF) {
     // par = {n, z[n], eight parameter curves of n values}
@@ -847,6 +943,15 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   int const pt = is_ig ? aux->point : 0;
   if (needs_tables && (pt < 0 || pt >= rt->n_points))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad point");
+  bool const des_ig = is_ig && aux->kind == Y3G_DSIGMA6D_DES;
+  if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !has_des(rt, pt)))
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: the real-data Delta-Sigma integrand is "
+                             "Y3G_F_INTEGRAND in FP64 with the point's DES data");
+  if (des_ig)
+    if (int rc = ensure_des(rt)) return rc;
+  long long des_off = 0;
+  if (des_ig) HIPCHK(hipMemcpy(&des_off, rt->d_des_off + pt, sizeof(des_off),
+                               hipMemcpyDeviceToHost));
   double *d_in = nullptr, *d_out = nullptr, *d_par = nullptr;
   y3g_integral* d_ig = nullptr;
   int rc = 0;
@@ -881,6 +986,7 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   } else {
     F.hmf = F.da = F.sci = nullptr;
   }
+  F.des = des_ig ? rt->d_des + des_off : nullptr;
   F.ig = d_ig;
   F.out = d_out;
   hipError_t e = launch_functor(F, precision, rt->stream);
@@ -909,6 +1015,21 @@ int y3g_selfn_upload(y3g_runtime* rt, int point, const double* lnm, int n_lnm,
   h.s.assign(s_stack, s_stack + (size_t)n_bins * n_z * n_lnm);
   h.n_bins = n_bins;
   rt->sel_dirty = true;
+  return 0;
+}
+
+int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des)
+{
+  if (!rt || point < 0 || !des || n_des < kDesSig + 3)
+    return fail(Y3G_ERR_ARG, "y3g_des_upload: bad arguments");
+  for (int k = 0; k < kDesSig; ++k)
+    if (!std::isfinite(des[k]))
+      return fail(Y3G_ERR_ARG, "y3g_des_upload: non-finite parameter " + std::to_string(k));
+  if (!sigmax_ok(des + kDesSig, n_des - kDesSig))
+    return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX table sizes or axes");
+  if ((size_t)point >= rt->des.size()) rt->des.resize((size_t)point + 1);
+  rt->des[point].assign(des, des + n_des);
+  rt->des_dirty = true;
   return 0;
 }
 
