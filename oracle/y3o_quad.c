@@ -1027,7 +1027,12 @@ y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
   double* hist = (double*)malloc(sizeof(double) * d * nb);
   for (int k = 0; k < d; ++k)
     for (int b = 0; b <= nb; ++b) xg[k * (nb + 1) + b] = (double)b / nb;
-  long long const calls = o->calls_per_iter;
+  /* calls per iteration, fewer when the call limit could not hold all
+   * iterations (as the device kernel) */
+  long long calls = o->calls_per_iter;
+  long long const per_it = o->max_eval / (o->itmax > 0 ? o->itmax : 1);
+  if (per_it < calls) calls = per_it;
+  if (calls < 2) calls = 2;
   int g = (int)floor(pow(calls / 2.0, 1.0 / d));
   if (g < 1) g = 1;
   long long ncubes = 1;

@@ -142,9 +142,9 @@ def test_des_integrand_device_matches_oracle(runtime):
 
 
 def test_des_cubature_matches_oracle(runtime):
-    """The batched FP64 cubature of the real-data integral against the
-    oracle's cubature of the same integrand: within their errors; a
-    non-FP64 or VEGAS request and a point without DES data are errors."""
+    """The batched FP64 cubature and m-Cubes of the real-data integral
+    against the oracle's cubature of the same integrand, within their
+    errors; an FP32 request and a point without DES data are errors."""
     C, ig, des, op, ot, oig = _des_setup(runtime)
     r = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=16384))
     o = O.Result()
@@ -156,12 +156,99 @@ def test_des_cubature_matches_oracle(runtime):
     assert abs(v / o.value - 1) < 3e-3, (v, o.value)
     with pytest.raises(RuntimeError):
         runtime.integrate([ig], y3.options(precision=y3.FP32SPLIT))
+    # m-Cubes (the reference's demo configuration runs this module with it)
     ig2 = y3.Integral.from_buffer_copy(ig)
     ig2.algorithm = y3.MCUBES
-    with pytest.raises(RuntimeError):
-        runtime.integrate([ig2], y3.options(precision=y3.FP64))
+    ig2.max_eval = 20_000_000
+    rv = runtime.integrate([ig2], y3.options(precision=y3.FP64, vegas_calls=2_000_000))
+    v2, e2 = rv["value"][0], rv["error"][0]
+    assert e2 > 0 and abs(v2 - o.value) <= 4 * e2 + 2 * o.error, (v2, e2, o.value, o.error)
     ig3 = y3.Integral.from_buffer_copy(ig)
     runtime.tables_build([y3.default_params(), y3.default_params(omega_m=0.28)], y3.spec_wide())
     ig3.point = 1
     with pytest.raises(RuntimeError):
         runtime.integrate([ig3], y3.options(precision=y3.FP64))
+
+
+def test_des_module_end_to_end(runtime):
+    """The reference's real-data Delta-Sigma module label through the
+    CosmoSIS entry points, its keys read from the config and the sample:
+    the result equals the library's on the same descriptor and the oracle's
+    cubature within the errors."""
+    import ctypes as C
+    from y3_cluster_cpp_amd import cosmosis as cs
+    from test_oracle_des import DES_BOX, SIG_M, SIG_R, SIG_Z, des_data, des_integral
+    L = "SigmaMiscentY1MortCUDAScalarIntegrand"
+    cfg = cs.DataBlock()
+    for k, v in (("algorithm", "pagani"), ("eps_rel", 1e-3), ("eps_abs", 0.0),
+                 ("max_eval", 100_000_000), ("use_cartesian_product", False),
+                 ("max_regions", 16384)):
+        cfg.put(L, k, v)
+    for name, (lo, hi) in zip(("lo", "lc", "zt", "lnm", "rmis", "theta"), DES_BOX):
+        cfg.put(L, name + "_low", [float(lo)])
+        cfg.put(L, name + "_high", [float(hi)])
+    cfg.put(L, "zo_low", [0.3])
+    cfg.put(L, "zo_high", [0.45])
+    cfg.put(L, "radii", [1.0])
+    des = des_data()
+    nr, nm, nz = len(SIG_R), len(SIG_M), len(SIG_Z)
+    o = 11 + 3 + nr + nm + nz
+    def make_sample():
+        smp = cs.DataBlock()
+        smp.put("cosmological_parameters", "omega_m", 0.3)
+        for k, v in zip(("roffset_tau", "LO_LC_alpha", "LO_LC_a", "LO_LC_b", "LO_LC_R_lambda",
+                         "mor_A", "mor_B", "mor_alpha", "mor_sigma", "mor_epsilon",
+                         "z_mor_pivot"), des[:11]):
+            smp.put("cluster_abundance", k, float(v))
+        smp.put("deltasigma", "r_sigma_deltasigma", SIG_R)
+        smp.put("deltasigma", "lnM", SIG_M)
+        smp.put("matter_power_lin", "z", SIG_Z)
+        smp.put("deltasigma", "sigma_1", des[o:o + nm * nr].reshape(nm, nr))
+        smp.put("deltasigma", "sigma_2", des[o + nm * nr:o + nm * nr + nz * nr].reshape(nz, nr))
+        smp.put("deltasigma", "bias", des[o + nm * nr + nz * nr:].reshape(nm, nz))
+        return smp
+
+    smp = make_sample()
+    m = cs.Module(L)
+    assert m.setup(cfg), "setup returned NULL"
+    assert m.execute(smp) == 0
+    vals, errs = smp.get_array(L, "vals"), smp.get_array(L, "errors")
+    assert m.cleanup() == 0
+    # the reference demo's settings for this module: algorithm = mcubes,
+    # max_eval = 1e5 (y1_rerun/demo_sigma_mort_cuda.ini:139-142)
+    cfg_mc = cs.DataBlock()
+    for k, v in (("algorithm", "mcubes"), ("eps_rel", 1e-3), ("eps_abs", 0.0),
+                 ("max_eval", 100_000), ("use_cartesian_product", False)):
+        cfg_mc.put(L, k, v)
+    for name, (lo, hi) in zip(("lo", "lc", "zt", "lnm", "rmis", "theta"), DES_BOX):
+        cfg_mc.put(L, name + "_low", [float(lo)])
+        cfg_mc.put(L, name + "_high", [float(hi)])
+    cfg_mc.put(L, "zo_low", [0.3])
+    cfg_mc.put(L, "zo_high", [0.45])
+    cfg_mc.put(L, "radii", [1.0])
+    mc = cs.Module(L)
+    assert mc.setup(cfg_mc), "setup returned NULL"
+    smp2 = make_sample()
+    assert mc.execute(smp2) == 0
+    vmc, emc = smp2.get_array(L, "vals"), smp2.get_array(L, "errors")
+    assert mc.cleanup() == 0
+    # the library on the same descriptor
+    p = y3.default_params(omega_m=0.3)
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, des)
+    ig = y3.make_integral_volume(p, spec, y3.DSIGMA6D_DES, [b[0] for b in DES_BOX],
+                                 [b[1] for b in DES_BOX], (0.3, 0.45, 1.0), eps_rel=1e-3)
+    ig.eps_abs, ig.max_eval = 0.0, 1 << 62
+    r = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=16384))
+    assert vals.shape == (1,) and abs(vals[0] / r["value"][0] - 1) < 1e-12, (vals, r["value"])
+    op = O.default_params(omega_m=0.3)
+    ot = O.OracleTables(op, O.spec_wide())
+    oo = O.Result()
+    O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(des),
+                           C.byref(des_integral(DES_BOX)),
+                           C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(oo))
+    assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)
+    assert emc[0] > 0 and abs(vmc[0] - oo.value) <= 5 * emc[0] + 2 * oo.error, \
+        (vmc[0], emc[0], oo.value)

@@ -396,6 +396,7 @@ template <typename T, int BLOCK> struct VegShared {
   double red[2 * (BLOCK / 64)];
   y3g_integral ig;
   PointDev P;
+  const double* des;  // the point's DES data (Y3G_DSIGMA6D_DES)
   int item, point, stop;
 };
 
@@ -429,7 +430,12 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
     int const a = q / (nb + 1), b = q - a * (nb + 1);
     S.xg[a][b] = double(b) / nb;
   }
-  long long const calls = A.vegas_calls;
+  // calls per iteration, fewer when the integral's call limit could not
+  // hold all iterations (a small max_eval still runs every iteration)
+  long long calls = A.vegas_calls;
+  long long const per_it = S.ig.max_eval / (A.vegas_itmax > 0 ? A.vegas_itmax : 1);
+  if (per_it < calls) calls = per_it;
+  if (calls < 2) calls = 2;
   int g = (int)floor(pow(calls / 2.0, 1.0 / d));
   if (g < 1) g = 1;
   long long ncubes = 1;
@@ -476,7 +482,11 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
           jac *= nb * wb;
           bins[a] = b;
         }
-        double const fx = double(integrand_k<T, K>(S.ig, S.P, tb, y)) * jac;
+        double fx;
+        if constexpr (K == Y3G_DSIGMA6D_DES)
+          fx = integrand_des(S.ig, S.P, tb, S.des, y) * jac;
+        else
+          fx = double(integrand_k<T, K>(S.ig, S.P, tb, y)) * jac;
         s1 += fx;
         s2 += fx * fx;
         for (int a = 0; a < d; ++a) atomicAdd(&S.hist[a][bins[a]], fx * fx);
@@ -588,7 +598,10 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
     __syncthreads();
     int const pt = S.ig.point;
     if (pt != S.point) {
-      if (tid == 0) S.P = A.points[pt];
+      if (tid == 0) {
+        S.P = A.points[pt];
+        S.des = A.des ? A.des + A.des_off[pt] : nullptr;
+      }
       stage_tables<T>(A, pt, S.hmf, S.da, S.sci);
       __syncthreads();
       if (tid == 0) S.point = pt;
@@ -596,6 +609,9 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
     switch (S.ig.kind) {
       case Y3G_COUNT3D: vegas_one<T, BLOCK, Y3G_COUNT3D>(S, A, iid, nev_total); break;
       case Y3G_COUNT5D: vegas_one<T, BLOCK, Y3G_COUNT5D>(S, A, iid, nev_total); break;
+      case Y3G_DSIGMA6D_DES:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_DSIGMA6D_DES>(S, A, iid, nev_total);
+        break;
       default: vegas_one<T, BLOCK, Y3G_DSIGMA6D>(S, A, iid, nev_total); break;
     }
   }

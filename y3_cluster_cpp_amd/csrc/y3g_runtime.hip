@@ -767,10 +767,10 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
       return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                  " is malformed");
     if (g.kind == Y3G_DSIGMA6D_DES) {
-      if (g.ndim != 6 || o.precision != Y3G_FP64 || g.algorithm != Y3G_PAGANI || o.single)
+      if (g.ndim != 6 || o.precision != Y3G_FP64 || o.single)
         return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                    ": the real-data Delta-Sigma integrand runs on the FP64 "
-                                   "batched cubature only (precision fp64, pagani)");
+                                   "batched integrators only (precision fp64)");
       if (!has_des(rt, g.point))
         return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
                                      ": its point has no DES data (y3g_des_upload)");

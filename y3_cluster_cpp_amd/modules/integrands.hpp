@@ -150,4 +150,97 @@ private:
   grid_point_t gp_{};
 };
 
+// SigmaMiscentY1MortCUDAScalarIntegrand -- CosmoSIS module: the reference's
+// real-data miscentred Delta-Sigma module under its own label, keys and
+// variables (src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_
+// scalarintegrand.cu: functors :33-41, set_sample :140-150, operator()
+// :161-182, volumes :190-203, grid :205-215).  The integrand is the device
+// kind Y3G_DSIGMA6D_DES (FP64 batched cubature, pagani only), with
+//   ROFFSET_t     cluster_abundance/roffset_tau
+//   LO_LC_t       cluster_abundance/LO_LC_alpha, LO_LC_a, LO_LC_b, LO_LC_R_lambda
+//   MOR_DES_t     cluster_abundance/mor_A, mor_B, mor_alpha, mor_sigma,
+//                 mor_epsilon, z_mor_pivot
+//   SIG_MAX       deltasigma/{r_sigma_deltasigma, lnM, sigma_1, sigma_2, bias},
+//                 matter_power_lin/z
+//   HMF_t, DV_DO_DZ_t  the sample's mass_function / distances tables when
+//                 present (else the synthetic tables), OMEGA_Z_DES, INT_ZO_ZT_DES_t
+//
+// Config (<label> = SigmaMiscentY1MortCUDAScalarIntegrand):
+//   algorithm (pagani), eps_rel, eps_abs, max_eval, use_cartesian_product
+//   lo_*, lc_*, zt_*, lnm_*, rmis_*, theta_* (_low / _high) -- volumes
+//   zo_low, zo_high, radii                                  -- grid
+
+class SigmaMiscentY1MortCUDAScalarIntegrand : public y3g_cosmosis::SyntheticSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<3>;
+  using grid_point_t = grid_t::value_type;
+  static constexpr bool fp64_only = true;
+
+  explicit SigmaMiscentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    using y3g_cosmosis::view_as_double;
+    using doubles = std::vector<double>;
+    std::string const A = "cluster_abundance";
+    des_.clear();
+    for (char const* k : {"roffset_tau", "LO_LC_alpha", "LO_LC_a", "LO_LC_b", "LO_LC_R_lambda",
+                          "mor_A", "mor_B", "mor_alpha", "mor_sigma", "mor_epsilon",
+                          "z_mor_pivot"})
+      des_.push_back(view_as_double(sample, A, k));
+    auto const r = sample.view<doubles>("deltasigma", "r_sigma_deltasigma");
+    auto const m = sample.view<doubles>("deltasigma", "lnM");
+    auto const z = sample.view<doubles>("matter_power_lin", "z");
+    auto const s1 = sample.view<cosmosis::ndarray<double>>("deltasigma", "sigma_1");
+    auto const s2 = sample.view<cosmosis::ndarray<double>>("deltasigma", "sigma_2");
+    auto const bi = sample.view<cosmosis::ndarray<double>>("deltasigma", "bias");
+    if (s1.size() != r.size() * m.size() || s2.size() != r.size() * z.size() ||
+        bi.size() != z.size() * m.size())
+      throw std::runtime_error("deltasigma/sigma_1, sigma_2, bias must hold n_r n_lnM, "
+                               "n_r n_z and n_z n_lnM values");
+    des_.push_back(double(r.size()));
+    des_.push_back(double(m.size()));
+    des_.push_back(double(z.size()));
+    des_.insert(des_.end(), r.begin(), r.end());
+    des_.insert(des_.end(), m.begin(), m.end());
+    des_.insert(des_.end(), z.begin(), z.end());
+    des_.insert(des_.end(), s1.begin(), s1.end());
+    des_.insert(des_.end(), s2.begin(), s2.end());
+    des_.insert(des_.end(), bi.begin(), bi.end());
+  }
+
+  void
+  upload(y3g_runtime* rt) const
+  {
+    if (y3g_des_upload(rt, 0, des_.data(), int(des_.size())) != Y3G_OK)
+      throw std::runtime_error(std::string("DES data upload: ") + y3g_last_error());
+  }
+
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_DSIGMA6D_DES, v, gp_.data(), 3);
+  }
+  static char const* module_label() { return "SigmaMiscentY1MortCUDAScalarIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lo", "lc", "zt", "lnm", "rmis", "theta"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<3>(
+      cfg, module_label(), {"zo_low", "zo_high", "radii"});
+  }
+
+private:
+  grid_point_t gp_{};
+  std::vector<double> des_;
+};
+
 #endif

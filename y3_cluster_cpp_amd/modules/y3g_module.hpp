@@ -39,9 +39,26 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace y3g_cosmosis {
+
+  // Optional parts of a module's integrand class: upload(y3g_runtime*) for
+  // per-sample device data beyond the tables, and a static fp64_only for
+  // integrands that have no FP32 path.
+  template <class I, class = void>
+  struct has_device_upload : std::false_type {};
+  template <class I>
+  struct has_device_upload<
+    I, std::void_t<decltype(std::declval<I&>().upload(std::declval<y3g_runtime*>()))>>
+    : std::true_type {};
+  template <class I, class = void>
+  struct is_fp64_only : std::false_type {};
+  template <class I>
+  struct is_fp64_only<I, std::void_t<decltype(I::fp64_only)>>
+    : std::bool_constant<I::fp64_only> {};
 
   // ---------------------------------------------------------- config readers
 
@@ -351,14 +368,16 @@ namespace y3g_cosmosis {
 
     // device-side settings (defaults from the library)
     y3g_options_default(&m.options);
-    std::string prec = "fp32split";
+    std::string prec = is_fp64_only<I>::value ? "fp64" : "fp32split";
     cfg.get_val(L, "precision", prec);
     if (prec == "fp64")
       m.options.precision = Y3G_FP64;
-    else if (prec == "fp32split")
+    else if (prec == "fp32split" && !is_fp64_only<I>::value)
       m.options.precision = Y3G_FP32SPLIT;
     else
-      throw std::runtime_error(L + "/precision must be fp32split or fp64, not " + prec);
+      throw std::runtime_error(L + "/precision must be " +
+                               (is_fp64_only<I>::value ? "fp64" : "fp32split or fp64") +
+                               ", not " + prec);
     m.options.max_regions =
       int(optional_double(cfg, L, "max_regions", m.options.max_regions));
     m.options.vegas_calls =
@@ -545,6 +564,7 @@ namespace y3g_cosmosis {
       if (timing_) record_timestamp(*timing_, "enter");
       integrand_.set_sample(sample);
       tables_.ensure(device_.get(), integrand_);
+      if constexpr (has_device_upload<I>::value) integrand_.upload(device_.get());
       std::vector<y3g_integral> igs = build_integrals(integrand_, cfg_);
       std::vector<y3g_result> res(igs.size());
       if (!igs.empty() &&
