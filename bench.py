@@ -44,7 +44,8 @@ sys.path.insert(0, ROOT)
 
 from y3_cluster_cpp_amd.workloads import (  # noqa: E402
     LO_EDGES, N_COUNTS, N_PER_POINT, RADII, SEED, SEL_BINS, SEL_GL, ZO_EDGES, count_ok,
-    eps_stream, mcmc_requests, point_integrals, point_params, prior_points, selection_stack)
+    des_synthetic, des_volumes, eps_stream, mcmc_requests, point_integrals, point_params,
+    prior_points, selection_stack)
 
 # FP32 operations per integrand evaluation of the device integrands
 # (+ - * / = 1, FMA = 2) and transcendental (SFU) calls, counted from
@@ -290,6 +291,72 @@ def selgl_run(y3, rt, spec, n_points, reps, cpu_seconds, with_cpu):
     }
 
 
+def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals):
+    """The real-data Delta-Sigma integrand (the SigmaMiscentY1MortCUDAScalar-
+    Integrand module's device work): n_points parameter points of the prior
+    stream, each with the synthetic DES inputs and the reference demo's 180
+    integrals (4 x 3 bins x 15 radii), FP64 batched cubature at eps_rel
+    `eps` and `regions` regions per integral.  Parity and CPU baseline: the
+    oracle's cubature of the first `cpu_integrals` integrals."""
+    import oracle as O
+    pts = prior_points(n_points)
+    params = [point_params(y3, pt) for pt in pts]
+    rt.tables_build(params, spec)
+    des = des_synthetic()
+    for k in range(n_points):
+        rt.des_upload(k, des)
+    vols = des_volumes()
+    igs = []
+    for k in range(n_points):
+        for lows, highs, grid in vols:
+            ig = y3.make_integral_volume(params[k], spec, y3.DSIGMA6D_DES, lows, highs, grid,
+                                         eps_rel=eps)
+            ig.point = k
+            igs.append(ig)
+    arr = (y3.Integral * len(igs))(*igs)
+    opts = y3.options(precision=y3.FP64, max_regions=regions)
+    rt.integrate(arr, opts)  # untimed: uploads
+    t0 = time.perf_counter()
+    res = rt.integrate(arr, opts)
+    wall = time.perf_counter() - t0
+    ms, nev = rt.last_timing()
+    out = {"workload": f"{n_points} parameter points x 180 integrals (4 lambda_obs x 3 z_obs "
+                       f"bins x 15 radii), synthetic DES inputs, eps_rel {eps}, "
+                       f"{regions} regions per integral, FP64",
+           "value": len(igs) / wall, "unit": "integrals/s", "kernel_ms": ms,
+           "evals_per_s": nev / (ms * 1e-3) if ms > 0 else 0.0,
+           "converged": int((res["status"] == 0).sum()), "integrals": len(igs)}
+    # the oracle on the first integrals (point 0): parity within the errors
+    pt = pts[0]
+    op = O.default_params(omega_m=pt[0], hmf_amp=pt[1], mor_slope=pt[2], mor_sigma_ln=pt[3],
+                          f_mis=pt[4])
+    ot = O.OracleTables(op, O.spec_wide())
+    dd = np.ascontiguousarray(des)
+    worst, t_cpu = 0.0, 0.0
+    for i in range(cpu_integrals):
+        lows, highs, (zlo, zhi, R) = vols[i]
+        oi = O.Integral()
+        oi.kind, oi.ndim, oi.zo_lo, oi.zo_hi, oi.radius = 3, 6, zlo, zhi, R
+        for a in range(6):
+            oi.lows[a], oi.highs[a] = lows[a], highs[a]
+        r = O.Result()
+        t1 = time.perf_counter()
+        O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(dd), C.byref(oi),
+                               C.byref(O.cub_opts(eps_rel=eps, max_regions=regions,
+                                                  max_eval=1 << 40)), C.byref(r))
+        t_cpu += time.perf_counter() - t1
+        worst = max(worst, abs(res["value"][i] - r.value) / (res["error"][i] + r.error))
+    out["max_diff_over_errors_vs_oracle"] = worst
+    out["checked"] = cpu_integrals
+    if with_cpu:
+        out["cpu_baseline"] = {"value": cpu_integrals / t_cpu, "unit": "integrals/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the first {cpu_integrals} integrals by the oracle's "
+                                         f"cubature (same eps and region budget), "
+                                         f"{t_cpu:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +384,12 @@ def main():
     ap.add_argument("--selgl-points", type=int, default=1024)
     ap.add_argument("--selgl-reps", type=int, default=200)
     ap.add_argument("--selgl-cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--no-des", action="store_true")
+    ap.add_argument("--des-only", action="store_true",
+                    help="run the real-data Delta-Sigma leg alone and print its object")
+    ap.add_argument("--des-points", type=int, default=4)
+    ap.add_argument("--des-regions", type=int, default=16384)
+    ap.add_argument("--des-cpu-integrals", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (calibrates FETCH_SIZE / "
                          "WRITE_SIZE counters) and exit")
@@ -340,6 +413,12 @@ def main():
     import y3_cluster_cpp_amd as y3
     pts, ref = load_golden()
     spec = y3.spec_wide()
+    if a.des_only:
+        rt = y3.Runtime(0)
+        print(json.dumps(des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
+                                 a.des_cpu_integrals)))
+        rt.close()
+        return
     if a.selgl_only:
         rt = y3.Runtime(0)
         print(json.dumps(selgl_run(y3, rt, spec, a.selgl_points, a.selgl_reps,
@@ -470,6 +549,13 @@ def main():
         print(f"fixed-GL counts: {selgl['value']:.3g} bin counts/s", file=sys.stderr,
               flush=True)
 
+    des_leg = None
+    if not a.no_des:
+        des_leg = des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
+                          a.des_cpu_integrals)
+        print(f"real-data Delta-Sigma: {des_leg['value']:.3g} integrals/s", file=sys.stderr,
+              flush=True)
+
     achieved = flops / (kern_ms * 1e-3) if kern_ms > 0 else 0.0
     out = {
         "metric": "integrals/sec at rel err <=1e-4 vs FP64 oracle; p50/p99 request latency",
@@ -506,6 +592,7 @@ def main():
         if plain and plain["counted_per_s"] > 0 else None,
         "serving": serving,
         "fixed_gl_counts": selgl,
+        "des_dsigma": des_leg,
     }
     print(json.dumps(out))
 

@@ -25,6 +25,10 @@
  *   y3g_result              numint::integration_result fields used at
  *                           CosmoSISSICUDAModule.hh:299-309
  *                           (estimate, errorest, chi_sq, neval, nregions, status)
+ *   y3g_des_upload          the DES functors' sample reads in SigmaMiscentY1MortCUDAScalar-
+ *                           Integrand::set_sample (sigma_miscent_y1_mort_cuda_scalarintegrand.cu
+ *                           :140-150; roffset_t.cuh:9-10, lo_lc_t.cuh:13-18, mor_des_t.cuh
+ *                           :264-273, sig_max.cuh:38-57)
  *   y3g_selfn_upload        SelFunction_t's sample reads, src/models/sel_function_t.hh:28-80
  *   y3g_selgl_counts        NumCountsSelGL::set_sample -> SelGLCore::build_weights,
  *                           src/models/n_operator_sel_gl_t.hh:88-150, 184-192

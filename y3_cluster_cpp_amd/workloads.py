@@ -128,3 +128,47 @@ def selection_stack(rng, n_bins=SEL_BINS, nm=SEL_NLNM, nz=SEL_NZ):
         zc = 0.25 + 0.15 * (b // 4)
         out[b] = (1 / (1 + np.exp(-(M - m0) / 0.3))) * np.exp(-0.5 * ((Z - zc) / 0.08) ** 2)
     return lnm, z, out
+
+
+# The real-data Delta-Sigma integrand (SigmaMiscentY1MortCUDAScalarIntegrand):
+# synthetic DES inputs of the reference's shapes.
+DES_HEAD = (0.15,                                   # roffset_tau (Mpc/h)
+            1.2, 0.1, 3.0, 0.9,                     # LO_LC alpha, a, b, R_lambda
+            10 ** 11.2, 10 ** 12.4, 0.75, 0.2, 0.1, 0.3)  # MOR_DES A, B, alpha, sigma, eps, z_p
+
+
+def des_synthetic():
+    """{roffset_tau, LO_LC, MOR_DES, SIG_MAX} for y3g_des_upload: SIG_MAX
+    tables on 30 log-spaced radii in [0.05, 40] Mpc/h, 13 lnM in [30, 36],
+    10 z in [0.1, 1.0]: a one-halo term A(M) r^-0.9 exp(-r / 4 r_200) and a
+    two-halo term 0.08 r^-0.7 with bias (1 + (M / 1e14)^0.5)(1 + z)^0.5, the
+    reference's SIG_MAX inputs in shape (deltasigma/sigma_1 [n_lnM][n_r],
+    sigma_2 [n_z][n_r], bias [n_lnM][n_z])."""
+    r = np.logspace(np.log10(0.05), np.log10(40.0), 30)
+    m = np.linspace(30.0, 36.0, 13)
+    z = np.linspace(0.1, 1.0, 10)
+    M = np.exp(m)[:, None]
+    r200 = 1.6 * (M / 1e15) ** (1 / 3)
+    s1 = 60.0 * (M / 1e14) ** 0.6 * r[None, :] ** -0.9 * np.exp(-r[None, :] / (4 * r200))
+    s2 = np.broadcast_to(0.08 * r[None, :] ** -0.7, (len(z), len(r)))
+    bias = (1 + (np.exp(m)[:, None] / 1e14) ** 0.5) * (1 + z[None, :]) ** 0.5
+    return np.concatenate([list(DES_HEAD), [len(r), len(m), len(z)], r, m, z,
+                           s1.reshape(-1), np.ascontiguousarray(s2).reshape(-1),
+                           bias.reshape(-1)])
+
+
+def des_volumes():
+    """The reference demo's binning for the real-data Delta-Sigma module:
+    4 lambda_obs x 3 z_obs bins x 15 radii log-spaced in [0.1, 30] Mpc/h
+    (180 integrals), each over lc in [1, 300], zt within 0.1 of its z_obs
+    bin, lnM in [29.93, 35.69], rmis in [0, 1.5] Mpc/h, theta in [0, 2 pi].
+    Returns [(lows, highs, (zo_low, zo_high, radius))]."""
+    out = []
+    for a in range(4):
+        for b in range(3):
+            zlo, zhi = ZO_EDGES[b], ZO_EDGES[b + 1]
+            lows = [LO_EDGES[a], 1.0, zlo - 0.1, 29.9336, 0.0, 0.0]
+            highs = [LO_EDGES[a + 1], 300.0, zhi + 0.1, 35.6901, 1.5, 2 * np.pi]
+            for R in RADII:
+                out.append((lows, highs, (zlo, zhi, R)))
+    return out
