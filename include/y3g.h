@@ -63,10 +63,12 @@ typedef enum {
   Y3G_COUNT3D = 0,  /* (t, zt, lnM), lambda_obs bin analytic (erf)        */
   Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM)                               */
   Y3G_DSIGMA6D = 2, /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
-  Y3G_DSIGMA6D_DES = 3 /* (lo, lc, zt, lnM, rmis, theta) in the volume's own
-                          units: the reference's real-data integrand of
-                          SigmaMiscentY1MortCUDAScalarIntegrand with the
-                          point's DES data (y3g_des_upload); FP64 cubature */
+  /* The reference's real-data (DES) integrands, in the volume's own units,
+   * with the point's DES data (y3g_des_upload); FP64 cubature and m-Cubes: */
+  Y3G_DSIGMA6D_DES = 3, /* (lo, lc, zt, lnM, rmis, theta)  SigmaMiscentY1Mort… */
+  Y3G_DSIGMA3D_DES = 4, /* (lo, zt, lnM)                   SigmaCentY1Mort…    */
+  Y3G_COUNT3D_DES = 5,  /* (lo, zt, lnM)                   NCCentY1Mort…       */
+  Y3G_COUNT5D_DES = 6   /* (lo, lc, zt, lnM, rmis)         NCMiscentY1Mort…    */
 } y3g_kind;
 
 /* Integrators: the names of gpu_integrator.cuh:54-65. */

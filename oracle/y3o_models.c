@@ -950,26 +950,56 @@ y3o_integrand_mapped_batch(const y3o_params* p, const y3o_tables* t,
     out[i] = y3o_integrand_mapped(p, t, ig, y + i * ig->ndim);
 }
 
-/* SigmaMiscentY1MortCUDAScalarIntegrand::operator()
- * (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182), the real-data
- * Delta-Sigma integrand over (lo, lc, zt, lnM, rmis, theta) in the volume's
- * own units.  des = {roffset_tau, LO_LC alpha, a, b, R_lambda, MOR_DES
- * A, B, alpha, sigma_intr, epsilon, z_pivot, SIG_MAX tables}; zo_lo, zo_hi
- * and the radius come from ig; HMF and dV/dz from the point's tables. */
+/* The reference's real-data integrands in the volume's own variables,
+ * kinds as the device's (3..6):
+ *   3 SigmaMiscentY1MortCUDAScalarIntegrand::operator() (lo, lc, zt, lnM, rmis,
+ *     theta), sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182;
+ *   4 SigmaCentY1MortCUDAScalarIntegrand::operator() (lo, zt, lnM),
+ *     sigma_cent_y1_mort_cuda_scalarintegrand.cu:155-166;
+ *   5 NCCentY1MortCUDAScalarIntegrand::operator() (lo, zt, lnM),
+ *     NC_cent_y1_mort_cuda_scalarintegrand.cu:165-175;
+ *   6 NCMiscentY1MortCUDAScalarIntegrand::operator() (lo, lc, zt, lnM, rmis),
+ *     NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164.
+ * des = {roffset_tau, LO_LC alpha, a, b, R_lambda, MOR_DES A, B, alpha,
+ * sigma_intr, epsilon, z_pivot, SIG_MAX tables}; zo_lo, zo_hi and the radius
+ * come from ig; HMF and dV/dz from the point's tables. */
 double
 y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
                   const y3o_integral* ig, const double* x)
 {
-  double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4], th = x[5];
-  double const roffset_v = y3o_roffset_gamma(rmis, des[0]);
-  double const lo_lc_v = y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]);
-  double const mor_v = y3o_mor_des(des + 5, lc, lnM, zt);
-  double const common = roffset_v * lo_lc_v * mor_v * y3o_dvdz(p, t, zt) *
-                        y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt) / 2.0 / 3.1415926535897;
-  double const R = ig->radius;
-  double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(th));
-  return y3o_sig_max(des + 11, rs, lnM, zt) * y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) *
-         common;
+  switch (ig->kind) {
+    case 3: {
+      double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4], th = x[5];
+      double const roffset_v = y3o_roffset_gamma(rmis, des[0]);
+      double const lo_lc_v = y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]);
+      double const mor_v = y3o_mor_des(des + 5, lc, lnM, zt);
+      double const common = roffset_v * lo_lc_v * mor_v * y3o_dvdz(p, t, zt) *
+                            y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt) / 2.0 /
+                            3.1415926535897;
+      double const R = ig->radius;
+      double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(th));
+      return y3o_sig_max(des + 11, rs, lnM, zt) *
+             y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) * common;
+    }
+    case 4:
+    case 5: {
+      double const lo = x[0], zt = x[1], lnM = x[2];
+      double const common = y3o_mor_des(des + 5, lo, lnM, zt) * y3o_dvdz(p, t, zt) *
+                            y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt);
+      double const izo = y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt);
+      if (ig->kind == 4) return y3o_sig_max(des + 11, ig->radius, lnM, zt) * izo * common;
+      return izo * common;
+    }
+    case 6: {
+      double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4];
+      double const common = y3o_roffset_gamma(rmis, des[0]) *
+                            y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]) *
+                            y3o_mor_des(des + 5, lc, lnM, zt) * y3o_dvdz(p, t, zt) *
+                            y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt);
+      return y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) * common;
+    }
+  }
+  return 0.0;
 }
 
 typedef struct {
@@ -992,5 +1022,5 @@ y3o_cubature_des(const y3o_params* p, const y3o_tables* t, const double* des,
                  const y3o_integral* ig, const y3o_cub_opts* o, y3o_result* r)
 {
   des_ctx c = {p, t, des, ig};
-  y3o_cubature_fn(des_fn, &c, 6, ig->lows, ig->highs, o, r);
+  y3o_cubature_fn(des_fn, &c, ig->ndim, ig->lows, ig->highs, o, r);
 }

@@ -252,3 +252,92 @@ def test_des_module_end_to_end(runtime):
     assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)
     assert emc[0] > 0 and abs(vmc[0] - oo.value) <= 5 * emc[0] + 2 * oo.error, \
         (vmc[0], emc[0], oo.value)
+
+
+DES_MODULES = [  # label, kind, volume names, box, grid point
+    ("SigmaCentY1MortCUDAScalarIntegrand", 4, ("lo", "zt", "lnm"), (1, 2, 3), (0.3, 0.45, 1.0)),
+    ("NCCentY1MortCUDAScalarIntegrand", 5, ("lo", "zt", "lnm"), (1, 2, 3), (0.3, 0.45)),
+    ("NCMiscentY1MortCUDAScalarIntegrand", 6, ("lo", "lc", "zt", "lnm", "rmis"),
+     (0, 1, 2, 3, 4), (0.3, 0.45)),
+]
+
+
+@pytest.mark.parametrize("label,kind,names,axes,grid", DES_MODULES)
+def test_des_other_modules(runtime, label, kind, names, axes, grid):
+    """The other real-data modules (centred Delta-Sigma, centred and
+    miscentred counts): the device integrand against the oracle at random
+    points, and the module end to end against the library on the same
+    descriptor and the oracle's cubature, within the errors."""
+    import ctypes as C
+    from y3_cluster_cpp_amd import cosmosis as cs
+    from test_oracle_des import DES_BOX, SIG_M, SIG_R, SIG_Z, des_data, des_integral
+    box = [DES_BOX[a] for a in axes]
+    des = des_data()
+    nr, nm, nz = len(SIG_R), len(SIG_M), len(SIG_Z)
+    o = 11 + 3 + nr + nm + nz
+    sigma = kind == 4
+    miscent = kind == 6
+    up = des if sigma else des[:11]
+    if not miscent:
+        up = up.copy()
+        up[:5] = 0.0
+    p = y3.default_params(omega_m=0.3)
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, up)
+    ig = y3.make_integral_volume(p, spec, kind, [b[0] for b in box], [b[1] for b in box], grid,
+                                 eps_rel=1e-3)
+    op = O.default_params(omega_m=0.3)
+    ot = O.OracleTables(op, O.spec_wide())
+    oig = des_integral(box, zo=grid[:2], radius=grid[2] if sigma else 1.0)
+    oig.kind, oig.ndim = kind, len(box)
+    rng = np.random.default_rng(70 + kind)
+    x = np.column_stack([rng.uniform(l, h, 1500) for l, h in box])
+    got = runtime.eval_functor(y3.F_INTEGRAND, x, aux=ig)
+    ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t),
+                                            O.ptr(np.ascontiguousarray(up)), C.byref(oig),
+                                            O.ptr(np.ascontiguousarray(row))) for row in x])
+    assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * np.abs(ref).max())
+
+    cfg = cs.DataBlock()
+    for k, v in (("algorithm", "pagani"), ("eps_rel", 1e-3), ("eps_abs", 0.0),
+                 ("max_eval", 100_000_000), ("use_cartesian_product", False),
+                 ("max_regions", 16384)):
+        cfg.put(label, k, v)
+    for name, (lo, hi) in zip(names, box):
+        cfg.put(label, name + "_low", [float(lo)])
+        cfg.put(label, name + "_high", [float(hi)])
+    for name, v in zip(("zo_low", "zo_high", "radii"), grid):
+        cfg.put(label, name, [float(v)])
+    smp = cs.DataBlock()
+    smp.put("cosmological_parameters", "omega_m", 0.3)
+    keys = ["mor_A", "mor_B", "mor_alpha", "mor_sigma", "mor_epsilon", "z_mor_pivot"]
+    for k, v in zip(keys, des[5:11]):
+        smp.put("cluster_abundance", k, float(v))
+    if miscent:
+        for k, v in zip(("roffset_tau", "LO_LC_alpha", "LO_LC_a", "LO_LC_b", "LO_LC_R_lambda"),
+                        des[:5]):
+            smp.put("cluster_abundance", k, float(v))
+    if sigma:
+        smp.put("deltasigma", "r_sigma_deltasigma", SIG_R)
+        smp.put("deltasigma", "lnM", SIG_M)
+        smp.put("matter_power_lin", "z", SIG_Z)
+        smp.put("deltasigma", "sigma_1", des[o:o + nm * nr].reshape(nm, nr))
+        smp.put("deltasigma", "sigma_2", des[o + nm * nr:o + nm * nr + nz * nr].reshape(nz, nr))
+        smp.put("deltasigma", "bias", des[o + nm * nr + nz * nr:].reshape(nm, nz))
+    m = cs.Module(label)
+    assert m.setup(cfg), "setup returned NULL"
+    assert m.execute(smp) == 0
+    vals, errs = smp.get_array(label, "vals"), smp.get_array(label, "errors")
+    assert m.cleanup() == 0
+    ig.eps_abs, ig.max_eval = 0.0, 1 << 62
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, up)
+    r = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=16384))
+    assert abs(vals[0] / r["value"][0] - 1) < 1e-12, (vals, r["value"])
+    oo = O.Result()
+    O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(np.ascontiguousarray(up)),
+                           C.byref(oig), C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
+                                                            max_eval=2_000_000_000)),
+                           C.byref(oo))
+    assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)

@@ -9,6 +9,7 @@ import json
 import os
 
 import numpy as np
+import pytest
 
 import oracle as O
 from conftest import REF_DATA
@@ -321,3 +322,46 @@ def test_des_integrand_offset_radius_geometry():
         exp = K * (2 * np.pi * (c + d * lnM) + (a + e * lnM) * 4 * (R + rm) * special.ellipe(k2))
         got = integrate.quad(f, 0, 2 * np.pi, epsabs=0, epsrel=1e-12, limit=200)[0]
         assert abs(got / exp - 1) < 1e-10, (R, rm, got, exp)
+
+
+@pytest.mark.parametrize("kind", [4, 5, 6])
+def test_des_cent_and_count_integrands_reduce(kind):
+    """The other real-data integrands reduce the same way: the centred
+    Delta-Sigma (4) and counts (5) over (lo, zt, lnM) to the 2-D count
+    integral with the MOR mass inside the lo box (times the constant
+    SIG_MAX for 4), the miscentred counts (6) over (lo, lc, zt, lnM, rmis)
+    to it times the offset's closed-form mass (lo_lc integrating to 1)."""
+    p = O.default_params()
+    t = O.OracleTables(p, O.spec_wide())
+    des = des_data(sig_const=2.5)
+    box = {4: [DES_BOX[1], DES_BOX[2], DES_BOX[3]], 5: [DES_BOX[1], DES_BOX[2], DES_BOX[3]],
+           6: DES_BOX[:5]}[kind]
+    ig = des_integral(box, radius=1.0)
+    ig.kind, ig.ndim = kind, len(box)
+    r = O.Result()
+    O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
+                           C.byref(O.cub_opts(eps_rel=1e-5, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(r))
+    q = np.ascontiguousarray(des[5:11])
+    lam_box = DES_BOX[1]
+
+    def gl(n, a, b):
+        x, w = np.polynomial.legendre.leggauss(n)
+        return 0.5 * (b - a) * x + 0.5 * (a + b), 0.5 * (b - a) * w
+
+    xz, wz = gl(24, *DES_BOX[2])
+    xm, wm = gl(24, *DES_BOX[3])
+    xl, wl = gl(400, *lam_box)
+    exp = 0.0
+    for z, a in zip(xz, wz):
+        zf = (O.lib.y3o_dvdz(C.byref(p), C.byref(t.t), z) * O.lib.y3o_omega_z_des(z) *
+              O.lib.y3o_int_zo_zt_des(0.3, 0.45, z))
+        for m, b in zip(xm, wm):
+            mfrac = sum(w * O.lib.y3o_mor_des(O.ptr(q), lc, m, z) for lc, w in zip(xl, wl))
+            exp += a * b * zf * O.lib.y3o_hmf(C.byref(p), C.byref(t.t), m, z) * mfrac
+    if kind == 4:
+        exp *= 2.5
+    if kind == 6:
+        tau, (r1, r2) = des[0], DES_BOX[4]
+        exp *= (1 + r1 / tau) * np.exp(-r1 / tau) - (1 + r2 / tau) * np.exp(-r2 / tau)
+    assert abs(r.value - exp) <= max(2 * r.error, 2e-4 * exp), (kind, r.value, exp, r.error)

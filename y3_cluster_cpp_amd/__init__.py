@@ -13,7 +13,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liby3g.so")
 
-COUNT3D, COUNT5D, DSIGMA6D, DSIGMA6D_DES = 0, 1, 2, 3
+(COUNT3D, COUNT5D, DSIGMA6D, DSIGMA6D_DES, DSIGMA3D_DES, COUNT3D_DES,
+ COUNT5D_DES) = range(7)
 PAGANI, MCUBES = 0, 1
 FP32SPLIT, FP64 = 0, 1
 

@@ -17,7 +17,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 MODULE_DIR = os.path.join(_HERE, "modules")
 DATABLOCK_LIB = os.path.join(MODULE_DIR, "libdatablock.so")
 MODULES = ("NumberCounts3DIntegrand", "NumberCounts5DIntegrand", "SigmaMiscent6DIntegrand",
-           "NumCountsSel", "SigmaMiscentY1MortCUDAScalarIntegrand")
+           "NumCountsSel", "SigmaMiscentY1MortCUDAScalarIntegrand",
+           "SigmaCentY1MortCUDAScalarIntegrand", "NCCentY1MortCUDAScalarIntegrand",
+           "NCMiscentY1MortCUDAScalarIntegrand")
 
 DBS_SUCCESS = 0
 

@@ -634,8 +634,8 @@ __device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
     double y[d];
 #pragma unroll
     for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_offset(d, j, a);
-    if constexpr (K == Y3G_DSIGMA6D_DES)
-      S.f[q] = integrand_des(S.ig, S.P, tb, S.des, y);
+    if constexpr (is_des_kind(K))
+      S.f[q] = integrand_des_k<K>(S.ig, S.P, tb, S.des, y);
     else
       S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
   }
@@ -669,6 +669,9 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       case Y3G_COUNT3D: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;
       case Y3G_COUNT5D: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D>(S, m, tb); break;
       case Y3G_DSIGMA6D_DES: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D_DES>(S, m, tb); break;
+      case Y3G_DSIGMA3D_DES: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA3D_DES>(S, m, tb); break;
+      case Y3G_COUNT3D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D_DES>(S, m, tb); break;
+      case Y3G_COUNT5D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D_DES>(S, m, tb); break;
       default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }

@@ -503,6 +503,16 @@ template <> struct KindAxes<Y3G_DSIGMA6D> {
 template <> struct KindAxes<Y3G_DSIGMA6D_DES> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 6;
 };
+template <> struct KindAxes<Y3G_DSIGMA3D_DES> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_COUNT3D_DES> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_COUNT5D_DES> {
+  static constexpr int it = 1, iz = 2, im = 3, nd = 5;
+};
+constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_COUNT5D_DES; }
 
 // Integrand of kind K (axis positions known at compile time).
 template <typename T, int K>
@@ -1075,30 +1085,58 @@ __device__ inline double sig_max(const double* par, double r, double lnM, double
 // tables as packed for sig_max()}.
 constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 
-// SigmaMiscentY1MortCUDAScalarIntegrand::operator()
-// (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182), FP64, in the
-// volume's own variables (lo, lc, zt, lnM, rmis, theta):
-//   roffset(rmis) lo_lc(lo, lc, rmis) mor_des(lc, lnM, zt) dV/dz hmf Omega(zt)
-//   / 2 / 3.1415926535897 (the reference's constant)
-//   x sig_max(R', lnM, zt) int_zo_zt(zo_lo, zo_hi, zt),
-//   R' = sqrt(R^2 + rmis^2 + 2 rmis R cos theta).
+// The reference's real-data integrands, FP64, in the volume's own variables
+// (each the operator() of its module):
+//   DSIGMA6D_DES  (lo, lc, zt, lnM, rmis, theta)
+//     roffset(rmis) lo_lc(lo, lc, rmis) mor_des(lc, lnM, zt) dV/dz hmf Omega
+//     / 2 / 3.1415926535897 (the reference's constant) x sig_max(R', lnM, zt)
+//     int_zo_zt(zo_lo, zo_hi, zt), R' = sqrt(R^2 + rmis^2 + 2 rmis R cos theta)
+//     (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182);
+//   DSIGMA3D_DES  (lo, zt, lnM): mor_des(lo, lnM, zt) dV/dz hmf Omega
+//     sig_max(R, lnM, zt) int_zo_zt (sigma_cent_y1_mort_cuda_scalarintegrand.cu
+//     :155-166);
+//   COUNT3D_DES   (lo, zt, lnM): mor_des(lo, lnM, zt) dV/dz hmf Omega int_zo_zt
+//     (NC_cent_y1_mort_cuda_scalarintegrand.cu:165-175);
+//   COUNT5D_DES   (lo, lc, zt, lnM, rmis): roffset lo_lc mor_des(lc, ...) dV/dz
+//     hmf Omega int_zo_zt (NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164).
+template <int K>
+__device__ inline double integrand_des_k(const y3g_integral& ig, const PointDev& P,
+                                         Tabs<double> tb, const double* des, const double* y)
+{
+  constexpr int iz = KindAxes<K>::iz, im = KindAxes<K>::im;
+  double const zt = y[iz], lnM = y[im];
+  constexpr bool miscent = K == Y3G_DSIGMA6D_DES || K == Y3G_COUNT5D_DES;
+  double const lam = miscent ? y[1] : y[0];  // the MOR_DES richness argument
+  double common = mor_des(des + kDesMor, lam, lnM, zt) * dv_do_dz<double>(P, tb.da, zt) *
+                  hmf<double>(P, tb.hmf, lnM, zt) * omega_z_des(zt);
+  if constexpr (miscent) {
+    double const rmis = y[4];
+    common = roffset_gamma<double>(rmis, des[0]) *
+             lo_lc<double>(y[0], y[1], rmis, des[kDesLoLc], des[kDesLoLc + 1],
+                           des[kDesLoLc + 2], des[kDesLoLc + 3]) *
+             common;
+  }
+  double const izo = int_zo_zt_des<double>(ig.zo_lo, ig.zo_hi, zt);
+  if constexpr (K == Y3G_DSIGMA6D_DES) {
+    double const R = ig.radius, rmis = y[4];
+    double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(y[5]));
+    return sig_max(des + kDesSig, rs, lnM, zt) * izo * (common / 2.0 / 3.1415926535897);
+  } else if constexpr (K == Y3G_DSIGMA3D_DES) {
+    return sig_max(des + kDesSig, ig.radius, lnM, zt) * izo * common;
+  } else {
+    return izo * common;
+  }
+}
+
 __device__ inline double integrand_des(const y3g_integral& ig, const PointDev& P,
                                        Tabs<double> tb, const double* des, const double* y)
 {
-  double const lo = y[0], lc = y[1], zt = y[2], lnM = y[3], rmis = y[4], th = y[5];
-  double const roffset_v = roffset_gamma<double>(rmis, des[0]);
-  double const lo_lc_v = lo_lc<double>(lo, lc, rmis, des[kDesLoLc], des[kDesLoLc + 1],
-                                       des[kDesLoLc + 2], des[kDesLoLc + 3]);
-  double const mor_v = mor_des(des + kDesMor, lc, lnM, zt);
-  double const dv = dv_do_dz<double>(P, tb.da, zt);
-  double const hm = hmf<double>(P, tb.hmf, lnM, zt);
-  double const om = omega_z_des(zt);
-  double const common = roffset_v * lo_lc_v * mor_v * dv * hm * om / 2.0 / 3.1415926535897;
-  double const R = ig.radius;
-  double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(th));
-  double const sig = sig_max(des + kDesSig, rs, lnM, zt);
-  double const izo = int_zo_zt_des<double>(ig.zo_lo, ig.zo_hi, zt);
-  return sig * izo * common;
+  switch (ig.kind) {
+    case Y3G_DSIGMA6D_DES: return integrand_des_k<Y3G_DSIGMA6D_DES>(ig, P, tb, des, y);
+    case Y3G_DSIGMA3D_DES: return integrand_des_k<Y3G_DSIGMA3D_DES>(ig, P, tb, des, y);
+    case Y3G_COUNT3D_DES: return integrand_des_k<Y3G_COUNT3D_DES>(ig, P, tb, des, y);
+    default: return integrand_des_k<Y3G_COUNT5D_DES>(ig, P, tb, des, y);
+  }
 }
 
 // ------------------------------------------------ Genz-Malik rule nodes

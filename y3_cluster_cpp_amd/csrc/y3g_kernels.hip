@@ -483,8 +483,8 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
           bins[a] = b;
         }
         double fx;
-        if constexpr (K == Y3G_DSIGMA6D_DES)
-          fx = integrand_des(S.ig, S.P, tb, S.des, y) * jac;
+        if constexpr (is_des_kind(K))
+          fx = integrand_des_k<K>(S.ig, S.P, tb, S.des, y) * jac;
         else
           fx = double(integrand_k<T, K>(S.ig, S.P, tb, y)) * jac;
         s1 += fx;
@@ -612,6 +612,15 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       case Y3G_DSIGMA6D_DES:
         if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_DSIGMA6D_DES>(S, A, iid, nev_total);
         break;
+      case Y3G_DSIGMA3D_DES:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_DSIGMA3D_DES>(S, A, iid, nev_total);
+        break;
+      case Y3G_COUNT3D_DES:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_COUNT3D_DES>(S, A, iid, nev_total);
+        break;
+      case Y3G_COUNT5D_DES:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_COUNT5D_DES>(S, A, iid, nev_total);
+        break;
       default: vegas_one<T, BLOCK, Y3G_DSIGMA6D>(S, A, iid, nev_total); break;
     }
   }
@@ -675,7 +684,7 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     case Y3G_F_SCI: r = lerp_clamp<T>(tb.sci, P.nsci, x[0], P.inv_dzsci); break;
     case Y3G_F_INTEGRAND:
       if constexpr (sizeof(T) == 8) {
-        if (F.ig->kind == Y3G_DSIGMA6D_DES) {
+        if (is_des_kind(F.ig->kind)) {
           r = integrand_des(*F.ig, P, tb, F.des, x);
           break;
         }

@@ -199,6 +199,11 @@ static int ensure_des(y3g_runtime* rt)
   return 0;
 }
 
+static int des_ndim(int kind)
+{
+  return kind == Y3G_DSIGMA6D_DES ? 6 : kind == Y3G_COUNT5D_DES ? 5 : 3;
+}
+
 static bool has_des(const y3g_runtime* rt, int point)
 {
   return point >= 0 && (size_t)point < rt->des.size() && !rt->des[point].empty();
@@ -330,31 +335,37 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
 {
   if (!p || !s || !lows || !highs || !ig)
     return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: null argument");
-  if (kind == Y3G_DSIGMA6D_DES) {
-    // the reference's volume as it is (lo, lc, zt, lnm, rmis, theta) and its
-    // grid point (zo_low, zo_high, radius), no change of variables
-    if (n_grid != 3 || !grid)
-      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: kind 3 takes 3 grid values, got " +
+  if (is_des_kind(kind)) {
+    // the reference's volume as it is and its grid point: (zo_low, zo_high,
+    // radius) for Delta-Sigma, (zo_low, zo_high) for the counts
+    int const nd = des_ndim(kind);
+    bool const sig = kind == Y3G_DSIGMA6D_DES || kind == Y3G_DSIGMA3D_DES;
+    int const ng = sig ? 3 : 2;
+    if (n_grid != ng || !grid)
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: kind " + std::to_string(kind) +
+                                   " takes " + std::to_string(ng) + " grid values, got " +
                                    std::to_string(n_grid));
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < nd; ++i)
       if (!(lows[i] <= highs[i]) || !std::isfinite(lows[i]) || !std::isfinite(highs[i]))
         return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: axis " + std::to_string(i) +
                                      " has low > high or a non-finite bound");
-    if (!(lows[1] > 0.0))
+    bool const miscent = kind == Y3G_DSIGMA6D_DES || kind == Y3G_COUNT5D_DES;
+    int const iz = miscent ? 2 : 1;
+    if (miscent && !(lows[1] > 0.0))
       return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: lc range must be > 0");
-    if (!(lows[2] >= 0.0) || !(highs[2] <= s->da_zmax))
+    if (!(lows[iz] >= 0.0) || !(highs[iz] <= s->da_zmax))
       return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: zt range outside the distance "
                                "table [0, " + std::to_string(s->da_zmax) + "]");
-    if (!(grid[2] > 0.0) || !(grid[0] <= grid[1]))
+    if (!(grid[0] <= grid[1]) || (sig && !(grid[2] > 0.0)))
       return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: grid needs zo_low <= zo_high "
-                               "and radius > 0");
+                               "(and radius > 0)");
     std::memset(ig, 0, sizeof(*ig));
     ig->kind = kind;
-    ig->ndim = 6;
+    ig->ndim = nd;
     ig->zo_lo = grid[0];
     ig->zo_hi = grid[1];
-    ig->radius = grid[2];
-    for (int i = 0; i < 6; ++i) ig->lows[i] = lows[i], ig->highs[i] = highs[i];
+    ig->radius = sig ? grid[2] : 0.0;
+    for (int i = 0; i < nd; ++i) ig->lows[i] = lows[i], ig->highs[i] = highs[i];
     ig->eps_rel = 1e-4;
     ig->eps_abs = 1e-300;
     ig->max_eval = 1LL << 40;
@@ -762,18 +773,22 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   for (size_t i = 0; i < n; ++i) {
     y3g_integral const& g = igs[i];
     if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
-        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_DSIGMA6D_DES ||
+        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_COUNT5D_DES ||
         !(g.eps_rel >= 0) || g.max_eval < 1)
       return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                  " is malformed");
-    if (g.kind == Y3G_DSIGMA6D_DES) {
-      if (g.ndim != 6 || o.precision != Y3G_FP64 || o.single)
+    if (is_des_kind(g.kind)) {
+      if (g.ndim != des_ndim(g.kind) || o.precision != Y3G_FP64 || o.single)
         return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
-                                   ": the real-data Delta-Sigma integrand runs on the FP64 "
-                                   "batched integrators only (precision fp64)");
+                                   ": the real-data integrands run on the FP64 batched "
+                                   "integrators only (precision fp64)");
       if (!has_des(rt, g.point))
         return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
                                      ": its point has no DES data (y3g_des_upload)");
+      if ((g.kind == Y3G_DSIGMA6D_DES || g.kind == Y3G_DSIGMA3D_DES) &&
+          rt->des[g.point].size() <= (size_t)kDesSig)
+        return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
+                                     ": its point's DES data has no SIG_MAX tables");
       any_des = true;
     }
     for (int a = 0; a < g.ndim; ++a)
@@ -943,8 +958,10 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   int const pt = is_ig ? aux->point : 0;
   if (needs_tables && (pt < 0 || pt >= rt->n_points))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad point");
-  bool const des_ig = is_ig && aux->kind == Y3G_DSIGMA6D_DES;
-  if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !has_des(rt, pt)))
+  bool const des_ig = is_ig && is_des_kind(aux->kind);
+  if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !has_des(rt, pt) ||
+                 ((aux->kind == Y3G_DSIGMA6D_DES || aux->kind == Y3G_DSIGMA3D_DES) &&
+                  rt->des[pt].size() <= (size_t)kDesSig)))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: the real-data Delta-Sigma integrand is "
                              "Y3G_F_INTEGRAND in FP64 with the point's DES data");
   if (des_ig)
@@ -1020,12 +1037,12 @@ int y3g_selfn_upload(y3g_runtime* rt, int point, const double* lnm, int n_lnm,
 
 int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des)
 {
-  if (!rt || point < 0 || !des || n_des < kDesSig + 3)
+  if (!rt || point < 0 || !des || n_des < kDesSig)
     return fail(Y3G_ERR_ARG, "y3g_des_upload: bad arguments");
   for (int k = 0; k < kDesSig; ++k)
     if (!std::isfinite(des[k]))
       return fail(Y3G_ERR_ARG, "y3g_des_upload: non-finite parameter " + std::to_string(k));
-  if (!sigmax_ok(des + kDesSig, n_des - kDesSig))
+  if (n_des > kDesSig && !sigmax_ok(des + kDesSig, n_des - kDesSig))
     return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX table sizes or axes");
   if ((size_t)point >= rt->des.size()) rt->des.resize((size_t)point + 1);
   rt->des[point].assign(des, des + n_des);

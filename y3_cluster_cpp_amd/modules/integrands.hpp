@@ -150,47 +150,52 @@ private:
   grid_point_t gp_{};
 };
 
-// SigmaMiscentY1MortCUDAScalarIntegrand -- CosmoSIS module: the reference's
-// real-data miscentred Delta-Sigma module under its own label, keys and
-// variables (src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_
-// scalarintegrand.cu: functors :33-41, set_sample :140-150, operator()
-// :161-182, volumes :190-203, grid :205-215).  The integrand is the device
-// kind Y3G_DSIGMA6D_DES (FP64 batched cubature, pagani only), with
-//   ROFFSET_t     cluster_abundance/roffset_tau
-//   LO_LC_t       cluster_abundance/LO_LC_alpha, LO_LC_a, LO_LC_b, LO_LC_R_lambda
+// ----------------------------------------------------- real-data (DES) modules
+//
+// The reference's Y1 real-data modules under their own labels, keys and
+// variables; their integrands are the device kinds Y3G_*_DES (FP64 batched
+// cubature or m-Cubes).  Sample inputs, read as the reference's functors
+// read them in set_sample:
+//   ROFFSET_t     cluster_abundance/roffset_tau               (miscentred)
+//   LO_LC_t       cluster_abundance/LO_LC_alpha, LO_LC_a, LO_LC_b,
+//                 LO_LC_R_lambda                              (miscentred)
 //   MOR_DES_t     cluster_abundance/mor_A, mor_B, mor_alpha, mor_sigma,
 //                 mor_epsilon, z_mor_pivot
-//   SIG_MAX       deltasigma/{r_sigma_deltasigma, lnM, sigma_1, sigma_2, bias},
-//                 matter_power_lin/z
+//   SIG_MAX       deltasigma/{r_sigma_deltasigma, lnM, sigma_1, sigma_2,
+//                 bias}, matter_power_lin/z                   (Delta-Sigma)
 //   HMF_t, DV_DO_DZ_t  the sample's mass_function / distances tables when
-//                 present (else the synthetic tables), OMEGA_Z_DES, INT_ZO_ZT_DES_t
-//
-// Config (<label> = SigmaMiscentY1MortCUDAScalarIntegrand):
-//   algorithm (pagani), eps_rel, eps_abs, max_eval, use_cartesian_product
-//   lo_*, lc_*, zt_*, lnm_*, rmis_*, theta_* (_low / _high) -- volumes
-//   zo_low, zo_high, radii                                  -- grid
+//                 present (else the synthetic tables); OMEGA_Z_DES and
+//                 INT_ZO_ZT_DES_t have no inputs.
+// Config (<label> = the module's): algorithm, eps_rel, eps_abs, max_eval,
+// use_cartesian_product, the volumes and the grid below; precision fp64.
 
-class SigmaMiscentY1MortCUDAScalarIntegrand : public y3g_cosmosis::SyntheticSample {
+class DesSample : public y3g_cosmosis::SyntheticSample {
 public:
-  using grid_t = y3g_cosmosis::grid_t<3>;
-  using grid_point_t = grid_t::value_type;
   static constexpr bool fp64_only = true;
 
-  explicit SigmaMiscentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
-  void set_grid_point(grid_point_t pt) { gp_ = pt; }
-
   void
-  set_sample(cosmosis::DataBlock& sample)
+  upload(y3g_runtime* rt) const
   {
-    SyntheticSample::set_sample(sample);
+    if (y3g_des_upload(rt, 0, des_.data(), int(des_.size())) != Y3G_OK)
+      throw std::runtime_error(std::string("DES data upload: ") + y3g_last_error());
+  }
+
+protected:
+  // The {roffset, LO_LC, MOR_DES[, SIG_MAX]} block of y3g_des_upload; the
+  // centred modules, which read no offset keys, carry zeros there.
+  void
+  read_des(cosmosis::DataBlock& sample, bool miscent, bool sigma)
+  {
     using y3g_cosmosis::view_as_double;
     using doubles = std::vector<double>;
     std::string const A = "cluster_abundance";
     des_.clear();
-    for (char const* k : {"roffset_tau", "LO_LC_alpha", "LO_LC_a", "LO_LC_b", "LO_LC_R_lambda",
-                          "mor_A", "mor_B", "mor_alpha", "mor_sigma", "mor_epsilon",
+    for (char const* k : {"roffset_tau", "LO_LC_alpha", "LO_LC_a", "LO_LC_b", "LO_LC_R_lambda"})
+      des_.push_back(miscent ? view_as_double(sample, A, k) : 0.0);
+    for (char const* k : {"mor_A", "mor_B", "mor_alpha", "mor_sigma", "mor_epsilon",
                           "z_mor_pivot"})
       des_.push_back(view_as_double(sample, A, k));
+    if (!sigma) return;
     auto const r = sample.view<doubles>("deltasigma", "r_sigma_deltasigma");
     auto const m = sample.view<doubles>("deltasigma", "lnM");
     auto const z = sample.view<doubles>("matter_power_lin", "z");
@@ -212,13 +217,25 @@ public:
     des_.insert(des_.end(), bi.begin(), bi.end());
   }
 
-  void
-  upload(y3g_runtime* rt) const
-  {
-    if (y3g_des_upload(rt, 0, des_.data(), int(des_.size())) != Y3G_OK)
-      throw std::runtime_error(std::string("DES data upload: ") + y3g_last_error());
-  }
+  std::vector<double> des_;
+};
 
+// SigmaMiscentY1MortCUDAScalarIntegrand -- miscentred Delta-Sigma over
+// (lo, lc, zt, lnm, rmis, theta), grid (zo_low, zo_high, radii)
+// (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182, 190-215).
+class SigmaMiscentY1MortCUDAScalarIntegrand : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<3>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit SigmaMiscentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    read_des(sample, true, true);
+  }
   y3g_integral
   describe(y3g_cosmosis::volume_t const& v) const
   {
@@ -240,7 +257,123 @@ public:
 
 private:
   grid_point_t gp_{};
-  std::vector<double> des_;
+};
+
+// SigmaCentY1MortCUDAScalarIntegrand -- centred Delta-Sigma over
+// (lo, zt, lnm), grid (zo_low, zo_high, radii)
+// (sigma_cent_y1_mort_cuda_scalarintegrand.cu:155-166, 177-195).
+class SigmaCentY1MortCUDAScalarIntegrand : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<3>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit SigmaCentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    read_des(sample, false, true);
+  }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_DSIGMA3D_DES, v, gp_.data(), 3);
+  }
+  static char const* module_label() { return "SigmaCentY1MortCUDAScalarIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(cfg, module_label(),
+                                                                  {"lo", "zt", "lnm"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<3>(
+      cfg, module_label(), {"zo_low", "zo_high", "radii"});
+  }
+
+private:
+  grid_point_t gp_{};
+};
+
+// NCCentY1MortCUDAScalarIntegrand -- centred counts over (lo, zt, lnm),
+// grid (zo_low, zo_high) (NC_cent_y1_mort_cuda_scalarintegrand.cu:165-175,
+// 192-227).
+class NCCentY1MortCUDAScalarIntegrand : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<2>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit NCCentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    read_des(sample, false, false);
+  }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_COUNT3D_DES, v, gp_.data(), 2);
+  }
+  static char const* module_label() { return "NCCentY1MortCUDAScalarIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(cfg, module_label(),
+                                                                  {"lo", "zt", "lnm"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<2>(cfg, module_label(),
+                                                             {"zo_low", "zo_high"});
+  }
+
+private:
+  grid_point_t gp_{};
+};
+
+// NCMiscentY1MortCUDAScalarIntegrand -- miscentred counts over
+// (lo, lc, zt, lnm, rmis), grid (zo_low, zo_high)
+// (NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164, 172-194).
+class NCMiscentY1MortCUDAScalarIntegrand : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<2>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit NCMiscentY1MortCUDAScalarIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    read_des(sample, true, false);
+  }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_COUNT5D_DES, v, gp_.data(), 2);
+  }
+  static char const* module_label() { return "NCMiscentY1MortCUDAScalarIntegrand"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lo", "lc", "zt", "lnm", "rmis"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<2>(cfg, module_label(),
+                                                             {"zo_low", "zo_high"});
+  }
+
+private:
+  grid_point_t gp_{};
 };
 
 #endif
