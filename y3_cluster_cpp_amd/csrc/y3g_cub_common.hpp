@@ -255,10 +255,11 @@ struct ParentStage {
 static_assert(sizeof(ParentStage) == 2048, "ParentStage is 512 dwords");
 
 template <typename T, int BLOCK> struct CubShared {
-  // the HMF table: in LDS for the FP64 path; the FP32 path reads it only
-  // for the pair factors, from global memory (L2), which leaves room for three
-  // workgroups per CU
-  static constexpr bool kHmfLds = sizeof(T) == 8;
+  // the HMF table stays in global memory (L2) for both precisions: the FP32
+  // path reads it only for the pair factors, and in LDS it would take 51 KB
+  // (FP64) of the workgroup, one workgroup per CU instead of two (FP64) or
+  // three (FP32)
+  static constexpr bool kHmfLds = false;
   T hmf[kHmfLds ? kMaxHmf : 1];
   const T* hmf_tab;  // the table the evaluation reads (LDS or global)
   const double* des;  // the point's DES data (Y3G_DSIGMA6D_DES), global
