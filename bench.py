@@ -291,7 +291,7 @@ def selgl_run(y3, rt, spec, n_points, reps, cpu_seconds, with_cpu):
     }
 
 
-def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals):
+def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots=0):
     """The real-data Delta-Sigma integrand (the SigmaMiscentY1MortCUDAScalar-
     Integrand module's device work): n_points parameter points of the prior
     stream, each with the synthetic DES inputs and the reference demo's 180
@@ -314,7 +314,7 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals):
             ig.point = k
             igs.append(ig)
     arr = (y3.Integral * len(igs))(*igs)
-    opts = y3.options(precision=y3.FP64, max_regions=regions)
+    opts = y3.options(precision=y3.FP64, max_regions=regions, slots=slots)
     rt.integrate(arr, opts)  # untimed: uploads
     t0 = time.perf_counter()
     res = rt.integrate(arr, opts)
@@ -390,6 +390,8 @@ def main():
     ap.add_argument("--des-points", type=int, default=4)
     ap.add_argument("--des-regions", type=int, default=16384)
     ap.add_argument("--des-cpu-integrals", type=int, default=2)
+    ap.add_argument("--des-slots", type=int, default=0,
+                    help="workgroups of the real-data launch (0: as many as fit)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy (calibrates FETCH_SIZE / "
                          "WRITE_SIZE counters) and exit")
@@ -416,7 +418,7 @@ def main():
     if a.des_only:
         rt = y3.Runtime(0)
         print(json.dumps(des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
-                                 a.des_cpu_integrals)))
+                                 a.des_cpu_integrals, a.des_slots)))
         rt.close()
         return
     if a.selgl_only:
