@@ -68,7 +68,9 @@ typedef enum {
   Y3G_DSIGMA6D_DES = 3, /* (lo, lc, zt, lnM, rmis, theta)  SigmaMiscentY1Mort… */
   Y3G_DSIGMA3D_DES = 4, /* (lo, zt, lnM)                   SigmaCentY1Mort…    */
   Y3G_COUNT3D_DES = 5,  /* (lo, zt, lnM)                   NCCentY1Mort…       */
-  Y3G_COUNT5D_DES = 6   /* (lo, lc, zt, lnM, rmis)         NCMiscentY1Mort…    */
+  Y3G_COUNT5D_DES = 6,  /* (lo, lc, zt, lnM, rmis)         NCMiscentY1Mort…    */
+  Y3G_COUNT4D_CARD = 7  /* (ln lo, ln lt, zt, lnM)         numberCountsCard:
+                           MOR_DES_LOG_t and INT_LC_LT_DES_t2 tables       */
 } y3g_kind;
 
 /* Integrators: the names of gpu_integrator.cuh:54-65. */
@@ -244,6 +246,10 @@ int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
  *                                                         MOR_DES_t
  *          n_r, n_m, n_z, r[n_r], lnM[n_m], z[n_z],
  *          sigma_1[n_m][n_r], sigma_2[n_z][n_r], bias[n_m][n_z]}   SIG_MAX
+ * or, for Y3G_COUNT4D_CARD, MOR_DES_LOG_t's {mor_logMmin, mor_logRatio,
+ * mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot} in the MOR_DES places and
+ * INT_LC_LT_DES_t2's tables packed as for Y3G_F_INT_LC_LT_DES in place of
+ * SIG_MAX.  The counts without SIG_MAX may stop after the 11 parameters.
  * (the SIG_MAX tables x-fastest, as Interp2D reads the sample's ndarrays).
  * The point's HMF and D_A tables are its y3g_tables_build / _upload ones.
  * Replaces any earlier upload for the point. */
@@ -305,9 +311,12 @@ typedef enum {
                                  a_sig, b_sig, a_tau, b_tau, a_fprj, b_fprj
                                  (n each)}          EMG_DES_t(), FP64      */
   Y3G_F_EMG_DES_CDF = 19,     /* the same arguments  EMG_DES_t::cdf, FP64  */
-  Y3G_F_SIG_MAX = 20          /* (r, lnM, zt); par = {n_r, n_m, n_z, r, lnM, z,
+  Y3G_F_SIG_MAX = 20,         /* (r, lnM, zt); par = {n_r, n_m, n_z, r, lnM, z,
                                  sigma_1 [n_m][n_r], sigma_2 [n_z][n_r],
                                  bias [n_m][n_z]}   SIG_MAX, FP64          */
+  Y3G_F_MOR_DES_LOG = 21      /* (lt, lnM, zt); par = {log10 Mmin, log10 ratio,
+                                 alpha, sigma_intr, epsilon, z_pivot}
+                                                    MOR_DES_LOG_t, FP64    */
 } y3g_functor;
 int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,

@@ -129,6 +129,7 @@ def _load():
         "y3o_int_lc_lt_des": (_D, [_PD, _D, _D, _D]),
         "y3o_mor_des_sigma_skew": (None, [_D, _D, _PD, _PD]),
         "y3o_mor_des": (_D, [_PD, _D, _D, _D]),
+        "y3o_mor_des_log": (_D, [_PD, _D, _D, _D]),
         "y3o_emg_des_params": (None, [_PD, _D, _D, _PD]),
         "y3o_emg_des_cdf": (_D, [_PD, _D, _D, _D]),
         "y3o_emg_des_pdf": (_D, [_PD, _D, _D, _D]),

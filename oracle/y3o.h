@@ -114,6 +114,7 @@ double y3o_interp2d_clamp(const double* xs, int nx, const double* ys, int ny,
 double y3o_int_lc_lt_des(const double* par, double lc, double lt, double zt);
 void   y3o_mor_des_sigma_skew(double sigma_intr, double lam, double* sigma, double* skw);
 double y3o_mor_des(const double* q, double lt, double lnM, double zt);
+double y3o_mor_des_log(const double* q, double lt, double lnM, double zt);
 void   y3o_emg_des_params(const double* par, double ltr, double z, double* out);
 double y3o_emg_des_cdf(const double* par, double lob, double ltr, double z);
 double y3o_emg_des_pdf(const double* par, double lob, double ltr, double z);

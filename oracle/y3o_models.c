@@ -243,6 +243,24 @@ y3o_mor_des_sigma_skew(double sigma_intr, double lam, double* sigma, double* skw
                             lam);
 }
 
+/* MOR_DES_LOG_t::operator() (mor_des_log_t.cuh:284-316): q = {log10 M_min,
+ * log10 (M_1 / M_min), alpha, sigma_intr, epsilon, z_pivot}; lambda = 1 below
+ * M_min, std::max(0.0, lambda). */
+double
+y3o_mor_des_log(const double* q, double lt, double lnM, double zt)
+{
+  double const mmin = pow(10.0, q[0]);
+  double const m1 = pow(10.0, q[1]) * mmin;
+  double const v = pow((exp(lnM) - mmin) / (m1 - mmin), q[2]) *
+                   pow((1.0 + zt) / (1.0 + q[5]), q[4]);
+  double const v1 = exp(lnM) < mmin ? 1.0 : v;
+  double const lam = (0.0 < v1) ? v1 : 0.0;
+  double sigma, skw;
+  y3o_mor_des_sigma_skew(q[3], lam, &sigma, &skw);
+  double const x = lt - lam;
+  return y3o_gaussian(x, 0.0, sigma) * erfc(-1.0 * skw * x / (sqrt(2.0) * sigma));
+}
+
 /* mor_des_t.hh:262-290 (operator()): q = {A, B, alpha, sigma_intr, epsilon,
  * z_pivot}.  std::max(1.0, v) keeps 1.0 when v is NaN (a mass below A). */
 double
@@ -989,6 +1007,14 @@ y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
       double const izo = y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt);
       if (ig->kind == 4) return y3o_sig_max(des + 11, ig->radius, lnM, zt) * izo * common;
       return izo * common;
+    }
+    case 7: { /* numberCountsCard (gt_card_gpu/numberCounts.cu:160-174) */
+      double const zt = x[2], lnM = x[3];
+      double const lt = exp(x[1]), lo = exp(x[0]);
+      double const common = y3o_omega_z_des(zt) * y3o_dvdz(p, t, zt) * y3o_hmf(p, t, lnM, zt) *
+                            y3o_mor_des_log(des + 5, lt, lnM, zt);
+      return y3o_int_lc_lt_des(des + 11, lo, lt, zt) *
+             y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) * common;
     }
     case 6: {
       double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4];

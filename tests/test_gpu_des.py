@@ -341,3 +341,73 @@ def test_des_other_modules(runtime, label, kind, names, axes, grid):
                                                             max_eval=2_000_000_000)),
                            C.byref(oo))
     assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)
+
+
+def test_card_module_and_integrand(runtime, monkeypatch):
+    """numberCountsCard: MOR_DES_LOG_t and the card integrand on the device
+    against the oracle, and the module (reading the reference's transfer
+    tables from the working directory, as the reference does) end to end
+    against the library and the oracle's cubature."""
+    import ctypes as C
+    from y3_cluster_cpp_amd import cosmosis as cs
+    from test_oracle_des import DES_BOX, MOR_Q, card_data, des_integral, lclt_par
+    ql = np.array([np.log10(MOR_Q[0]), np.log10(MOR_Q[1] / MOR_Q[0])] + list(MOR_Q[2:]))
+    rng = np.random.default_rng(81)
+    x = np.column_stack([rng.uniform(0.0, 300.0, 2000), rng.uniform(np.log(1e10), np.log(5e15), 2000),
+                         rng.uniform(0.05, 0.9, 2000)])
+    got = runtime.eval_functor(y3.F_MOR_DES_LOG, x, ql)
+    ref = np.array([O.lib.y3o_mor_des_log(O.ptr(ql), *row) for row in x])
+    assert np.all(np.abs(got - ref) <= 1e-290 + 1e-12 * np.abs(ref))
+
+    lclt, _, _ = lclt_par()             # the reference's tables
+    des = np.ascontiguousarray(np.concatenate([card_data()[:11], lclt]))
+    box = [(np.log(20.0), np.log(60.0)), (np.log(5.0), np.log(150.0)), DES_BOX[2], DES_BOX[3]]
+    p = y3.default_params(omega_m=0.3)
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, des)
+    ig = y3.make_integral_volume(p, spec, y3.COUNT4D_CARD, [b[0] for b in box],
+                                 [b[1] for b in box], (0.3, 0.45), eps_rel=1e-3)
+    op = O.default_params(omega_m=0.3)
+    ot = O.OracleTables(op, O.spec_wide())
+    oig = des_integral(box)
+    oig.kind, oig.ndim = 7, 4
+    xs = np.column_stack([rng.uniform(l, h, 1500) for l, h in box])
+    got = runtime.eval_functor(y3.F_INTEGRAND, xs, aux=ig)
+    ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t), O.ptr(des), C.byref(oig),
+                                            O.ptr(np.ascontiguousarray(row))) for row in xs])
+    assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref) + 1e-14 * np.abs(ref).max())
+
+    L = "numberCountsCard"
+    cfg = cs.DataBlock()
+    for k, v in (("algorithm", "pagani"), ("eps_rel", 1e-3), ("eps_abs", 0.0),
+                 ("max_eval", 100_000_000), ("use_cartesian_product", False),
+                 ("max_regions", 16384)):
+        cfg.put(L, k, v)
+    for name, (lo, hi) in zip(("lnLo", "lnLt", "zt", "lnm"), box):
+        cfg.put(L, name + "_low", [float(lo)])
+        cfg.put(L, name + "_high", [float(hi)])
+    cfg.put(L, "zo_low", [0.3])
+    cfg.put(L, "zo_high", [0.45])
+    smp = cs.DataBlock()
+    smp.put("cosmological_parameters", "omega_m", 0.3)
+    for k, v in zip(("mor_logMmin", "mor_logRatio", "mor_alpha", "mor_sigma", "mor_epsilon",
+                     "z_mor_pivot"), ql):
+        smp.put("cluster_abundance", k, float(v))
+    monkeypatch.chdir(REF_DATA)
+    m = cs.Module(L)
+    assert m.setup(cfg), "setup returned NULL"
+    assert m.execute(smp) == 0
+    vals, errs = smp.get_array(L, "vals"), smp.get_array(L, "errors")
+    assert m.cleanup() == 0
+    des2 = np.ascontiguousarray(np.concatenate([[0.0] * 5, ql, lclt]))
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, des2)
+    ig.eps_abs, ig.max_eval = 0.0, 1 << 62
+    r = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=16384))
+    assert abs(vals[0] / r["value"][0] - 1) < 1e-12, (vals, r["value"])
+    oo = O.Result()
+    O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(des2), C.byref(oig),
+                           C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(oo))
+    assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)

@@ -365,3 +365,76 @@ def test_des_cent_and_count_integrands_reduce(kind):
         tau, (r1, r2) = des[0], DES_BOX[4]
         exp *= (1 + r1 / tau) * np.exp(-r1 / tau) - (1 + r2 / tau) * np.exp(-r2 / tau)
     assert abs(r.value - exp) <= max(2 * r.error, 2e-4 * exp), (kind, r.value, exp, r.error)
+
+
+def test_mor_des_log_matches_mor_des_and_clamps():
+    """MOR_DES_LOG_t with (log10 A, log10 (B/A)) is MOR_DES_t with (A, B)
+    wherever M >= A and lambda >= 1; below M_min its lambda is 1 (the skew
+    normal's mean there is 1 + sigma d sqrt(2/pi))."""
+    from scipy import integrate
+    qa = np.ascontiguousarray(MOR_Q)
+    ql = np.ascontiguousarray([np.log10(MOR_Q[0]), np.log10(MOR_Q[1] / MOR_Q[0])] +
+                              list(MOR_Q[2:]))
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        lnM, zt = rng.uniform(np.log(3e13), np.log(2e15)), rng.uniform(0.1, 0.8)
+        lt = rng.uniform(0.0, 300.0)
+        a = O.lib.y3o_mor_des(O.ptr(qa), lt, lnM, zt)
+        b = O.lib.y3o_mor_des_log(O.ptr(ql), lt, lnM, zt)
+        assert abs(a - b) <= 1e-12 * abs(a) + 1e-300, (lnM, zt, lt, a, b)
+    lnM, zt = np.log(1e11), 0.4   # below M_min
+    s, k = mor_sigma_skew(MOR_Q[3], 1.0)
+    f = lambda t: O.lib.y3o_mor_des_log(O.ptr(ql), t, lnM, zt)
+    mass = integrate.quad(f, 1 - 40 * s, 1 + 40 * s, points=[1.0], epsrel=1e-11, limit=200)[0]
+    mean = integrate.quad(lambda t: t * f(t), 1 - 40 * s, 1 + 40 * s, points=[1.0],
+                          epsrel=1e-11, limit=200)[0]
+    assert abs(mass - 1) < 1e-9
+    assert abs(mean - (1 + s * k / np.sqrt(1 + k * k) * np.sqrt(2 / np.pi))) < 1e-8
+
+
+def card_data(lclt_const=0.7):
+    """DES data of the numberCountsCard integrand: MOR_DES_LOG parameters in
+    the MOR places and INT_LC_LT tables (constant lclt_const, on the
+    reference's knots) in place of SIG_MAX."""
+    head = [0.0] * 5 + [11.2, 1.2, 0.75, 0.2, 0.1, 0.3]
+    _, lt, zt = lclt_par()
+    L, Z = np.meshgrid(lt, zt)
+    par, _, _ = lclt_par([np.full_like(L, lclt_const)] * 4)
+    return np.ascontiguousarray(np.concatenate([head, par]))
+
+
+def test_card_integrand_reduces():
+    """numberCountsCard over (ln lo, ln lt, zt, lnM) with constant transfer
+    tables c: the ln lo integral is c times the box width, the ln lt one is
+    int mor_log(lt) / lt dlt (no Jacobian is added: the variables are
+    logarithms), leaving the 2-D count integral."""
+    p = O.default_params()
+    t = O.OracleTables(p, O.spec_wide())
+    des = card_data(0.7)
+    box = [(np.log(22.0), np.log(28.0)), (np.log(5.0), np.log(150.0)), DES_BOX[2], DES_BOX[3]]
+    ig = des_integral(box)
+    ig.kind, ig.ndim = 7, 4
+    r = O.Result()
+    O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
+                           C.byref(O.cub_opts(eps_rel=1e-6, max_regions=40_000,
+                                              max_eval=2_000_000_000)), C.byref(r))
+    q = np.ascontiguousarray(des[5:11])
+
+    def gl(n, a, b):
+        x, w = np.polynomial.legendre.leggauss(n)
+        return 0.5 * (b - a) * x + 0.5 * (a + b), 0.5 * (b - a) * w
+
+    xz, wz = gl(24, *DES_BOX[2])
+    xm, wm = gl(24, *DES_BOX[3])
+    xl, wl = gl(400, *box[1])   # in ln lt
+    exp = 0.0
+    for z, a in zip(xz, wz):
+        zf = (O.lib.y3o_dvdz(C.byref(p), C.byref(t.t), z) * O.lib.y3o_omega_z_des(z) *
+              O.lib.y3o_int_zo_zt_des(0.3, 0.45, z))
+        for m, b in zip(xm, wm):
+            mm = sum(w * O.lib.y3o_mor_des_log(O.ptr(q), np.exp(u), m, z) for u, w in zip(xl, wl))
+            exp += a * b * zf * O.lib.y3o_hmf(C.byref(p), C.byref(t.t), m, z) * mm
+    exp *= 0.7 * (box[0][1] - box[0][0])
+    # the 24 x 24 Gauss-Legendre reference is good to ~1e-4 over the
+    # piecewise-bilinear HMF table
+    assert abs(r.value - exp) <= max(2 * r.error, 1e-4 * exp), (r.value, exp, r.error)

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liby3g.so")
 
 (COUNT3D, COUNT5D, DSIGMA6D, DSIGMA6D_DES, DSIGMA3D_DES, COUNT3D_DES,
- COUNT5D_DES) = range(7)
+ COUNT5D_DES, COUNT4D_CARD) = range(8)
 PAGANI, MCUBES = 0, 1
 FP32SPLIT, FP64 = 0, 1
 
@@ -22,7 +22,7 @@ FP32SPLIT, FP64 = 0, 1
  F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
  F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND, F_INTEGRAND_PAIRED,
  F_INTEGRAND_NODE, F_INT_LC_LT_DES, F_MOR_DES, F_EMG_DES_PDF, F_EMG_DES_CDF,
- F_SIG_MAX) = range(21)
+ F_SIG_MAX, F_MOR_DES_LOG) = range(22)
 
 
 class Params(C.Structure):

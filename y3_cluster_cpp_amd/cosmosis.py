@@ -19,7 +19,7 @@ DATABLOCK_LIB = os.path.join(MODULE_DIR, "libdatablock.so")
 MODULES = ("NumberCounts3DIntegrand", "NumberCounts5DIntegrand", "SigmaMiscent6DIntegrand",
            "NumCountsSel", "SigmaMiscentY1MortCUDAScalarIntegrand",
            "SigmaCentY1MortCUDAScalarIntegrand", "NCCentY1MortCUDAScalarIntegrand",
-           "NCMiscentY1MortCUDAScalarIntegrand")
+           "NCMiscentY1MortCUDAScalarIntegrand", "numberCountsCard")
 
 DBS_SUCCESS = 0
 

@@ -673,6 +673,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       case Y3G_DSIGMA3D_DES: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA3D_DES>(S, m, tb); break;
       case Y3G_COUNT3D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D_DES>(S, m, tb); break;
       case Y3G_COUNT5D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D_DES>(S, m, tb); break;
+      case Y3G_COUNT4D_CARD: eval_nodes_direct<T, BLOCK, Y3G_COUNT4D_CARD>(S, m, tb); break;
       default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }

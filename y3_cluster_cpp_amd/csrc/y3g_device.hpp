@@ -512,7 +512,10 @@ template <> struct KindAxes<Y3G_COUNT3D_DES> {
 template <> struct KindAxes<Y3G_COUNT5D_DES> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 5;
 };
-constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_COUNT5D_DES; }
+template <> struct KindAxes<Y3G_COUNT4D_CARD> {
+  static constexpr int it = 1, iz = 2, im = 3, nd = 4;
+};
+constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_COUNT4D_CARD; }
 
 // Integrand of kind K (axis positions known at compile time).
 template <typename T, int K>
@@ -941,6 +944,26 @@ __device__ inline double mor_des_lookup(const double* tab, double si, double lam
          t * u * r1[1];
 }
 
+// MOR_DES_LOG_t::operator() (mor_des_log_t.cuh:284-316): the same skew
+// normal with M_min = 10^q0 and M_1 = 10^q1 M_min, lambda = 1 below M_min,
+// and max(0, lambda) in place of max(1, lambda).
+// q = {log10 M_min, log10 (M_1 / M_min), alpha, sigma_intr, epsilon, z_pivot}.
+__device__ inline double mor_des_log(const double* q, double lt, double lnM, double zt)
+{
+  double const mmin = pow(10.0, q[0]);
+  double const m1 = pow(10.0, q[1]) * mmin;
+  double const v = pow((exp(lnM) - mmin) / (m1 - mmin), q[2]) *
+                   pow((1.0 + zt) / (1.0 + q[5]), q[4]);
+  double const v1 = exp(lnM) < mmin ? 1.0 : v;
+  double const lam = 0.0 < v1 ? v1 : 0.0;  // std::max(0.0, v1)
+  double const sigma = mor_des_lookup(kMorDesSigma, q[3], lam);
+  double const skw = mor_des_lookup(kMorDesSkew, q[3], lam);
+  double const x = lt - lam;
+  double const z = x / sigma;
+  return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
+         erfc(-skw * x / (1.4142135623730951 * sigma));
+}
+
 __device__ inline double mor_des(const double* q, double lt, double lnM, double zt)
 {
   double const v = pow((exp(lnM) - q[0]) / (q[1] - q[0]), q[2]) *
@@ -1099,11 +1122,23 @@ constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 //     (NC_cent_y1_mort_cuda_scalarintegrand.cu:165-175);
 //   COUNT5D_DES   (lo, lc, zt, lnM, rmis): roffset lo_lc mor_des(lc, ...) dV/dz
 //     hmf Omega int_zo_zt (NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164).
+//   COUNT4D_CARD  (ln lo, ln lt, zt, lnM): int_lc_lt(lo, lt, zt) int_zo_zt
+//     Omega dV/dz hmf mor_des_log(lt, lnM, zt), lo = e^(ln lo), lt = e^(ln lt)
+//     (gt_card_gpu/numberCounts.cu:160-174).
 template <int K>
 __device__ inline double integrand_des_k(const y3g_integral& ig, const PointDev& P,
                                          Tabs<double> tb, const double* des, const double* y)
 {
   constexpr int iz = KindAxes<K>::iz, im = KindAxes<K>::im;
+  if constexpr (K == Y3G_COUNT4D_CARD) {
+    double const zt = y[iz], lnM = y[im];
+    double const lt = exp(y[1]), lo = exp(y[0]);
+    double const mor_v = mor_des_log(des + kDesMor, lt, lnM, zt);
+    double const common = omega_z_des(zt) * dv_do_dz<double>(P, tb.da, zt) *
+                          hmf<double>(P, tb.hmf, lnM, zt) * mor_v;
+    return int_lc_lt_des(des + kDesSig, lo, lt, zt) *
+           int_zo_zt_des<double>(ig.zo_lo, ig.zo_hi, zt) * common;
+  }
   double const zt = y[iz], lnM = y[im];
   constexpr bool miscent = K == Y3G_DSIGMA6D_DES || K == Y3G_COUNT5D_DES;
   double const lam = miscent ? y[1] : y[0];  // the MOR_DES richness argument
@@ -1135,6 +1170,7 @@ __device__ inline double integrand_des(const y3g_integral& ig, const PointDev& P
     case Y3G_DSIGMA6D_DES: return integrand_des_k<Y3G_DSIGMA6D_DES>(ig, P, tb, des, y);
     case Y3G_DSIGMA3D_DES: return integrand_des_k<Y3G_DSIGMA3D_DES>(ig, P, tb, des, y);
     case Y3G_COUNT3D_DES: return integrand_des_k<Y3G_COUNT3D_DES>(ig, P, tb, des, y);
+    case Y3G_COUNT4D_CARD: return integrand_des_k<Y3G_COUNT4D_CARD>(ig, P, tb, des, y);
     default: return integrand_des_k<Y3G_COUNT5D_DES>(ig, P, tb, des, y);
   }
 }

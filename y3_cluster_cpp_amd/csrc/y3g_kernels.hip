@@ -621,6 +621,9 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       case Y3G_COUNT5D_DES:
         if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_COUNT5D_DES>(S, A, iid, nev_total);
         break;
+      case Y3G_COUNT4D_CARD:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_COUNT4D_CARD>(S, A, iid, nev_total);
+        break;
       default: vegas_one<T, BLOCK, Y3G_DSIGMA6D>(S, A, iid, nev_total); break;
     }
   }
@@ -652,6 +655,7 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     case Y3G_F_SIGMA_PHOTOZ: r = sigma_photoz_des<T>(T(x[0])); break;
     case Y3G_F_INT_LC_LT_DES: r = T(int_lc_lt_des(q, x[0], x[1], x[2])); break;
     case Y3G_F_MOR_DES: r = T(mor_des(q, x[0], x[1], x[2])); break;
+    case Y3G_F_MOR_DES_LOG: r = T(mor_des_log(q, x[0], x[1], x[2])); break;
     case Y3G_F_EMG_DES_PDF: r = T(emg_des_pdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_EMG_DES_CDF: r = T(emg_des_cdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_SIG_MAX: r = T(sig_max(q, x[0], x[1], x[2])); break;

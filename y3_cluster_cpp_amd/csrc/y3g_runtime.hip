@@ -201,7 +201,23 @@ static int ensure_des(y3g_runtime* rt)
 
 static int des_ndim(int kind)
 {
-  return kind == Y3G_DSIGMA6D_DES ? 6 : kind == Y3G_COUNT5D_DES ? 5 : 3;
+  return kind == Y3G_DSIGMA6D_DES ? 6 : kind == Y3G_COUNT5D_DES ? 5
+       : kind == Y3G_COUNT4D_CARD ? 4 : 3;
+}
+
+// INT_LC_LT_DES_t2 tables packed as int_lc_lt_des() reads them.
+static bool lclt_ok(const double* par, long long npar)
+{
+  if (npar < 2) return false;
+  double const a = par[0], b = par[1];
+  if (!(a >= 2.0 && b >= 2.0 && a <= 4096.0 && b <= 4096.0) || a != double(int(a)) ||
+      b != double(int(b)) || npar != 2 + (long long)(a + b) + 4LL * (long long)(a * b))
+    return false;
+  for (int k = 1; k < int(a); ++k)
+    if (!(par[2 + k] > par[1 + k])) return false;
+  for (int k = 1; k < int(b); ++k)
+    if (!(par[2 + int(a) + k] > par[1 + int(a) + k])) return false;
+  return true;
 }
 
 static bool has_des(const y3g_runtime* rt, int point)
@@ -350,7 +366,7 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
         return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: axis " + std::to_string(i) +
                                      " has low > high or a non-finite bound");
     bool const miscent = kind == Y3G_DSIGMA6D_DES || kind == Y3G_COUNT5D_DES;
-    int const iz = miscent ? 2 : 1;
+    int const iz = (miscent || kind == Y3G_COUNT4D_CARD) ? 2 : 1;
     if (miscent && !(lows[1] > 0.0))
       return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: lc range must be > 0");
     if (!(lows[iz] >= 0.0) || !(highs[iz] <= s->da_zmax))
@@ -773,7 +789,7 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   for (size_t i = 0; i < n; ++i) {
     y3g_integral const& g = igs[i];
     if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
-        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_COUNT5D_DES ||
+        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_COUNT4D_CARD ||
         !(g.eps_rel >= 0) || g.max_eval < 1)
       return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                  " is malformed");
@@ -785,10 +801,15 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
       if (!has_des(rt, g.point))
         return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
                                      ": its point has no DES data (y3g_des_upload)");
+      std::vector<double> const& dd = rt->des[g.point];
+      long long const nt = (long long)dd.size() - kDesSig;
       if ((g.kind == Y3G_DSIGMA6D_DES || g.kind == Y3G_DSIGMA3D_DES) &&
-          rt->des[g.point].size() <= (size_t)kDesSig)
+          !(nt > 0 && sigmax_ok(dd.data() + kDesSig, nt)))
         return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
                                      ": its point's DES data has no SIG_MAX tables");
+      if (g.kind == Y3G_COUNT4D_CARD && !(nt > 0 && lclt_ok(dd.data() + kDesSig, nt)))
+        return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
+                                     ": its point's DES data has no INT_LC_LT_DES tables");
       any_des = true;
     }
     for (int a = 0; a < g.ndim; ++a)
@@ -903,8 +924,11 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_SIG_MAX)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_MOR_DES_LOG)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_MOR_DES_LOG && (nin != 3 || !par || npar != 6))
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: MOR_DES_LOG takes (lt, lnM, zt) and "
+                             "{log10 Mmin, log10 ratio, alpha, sigma_intr, epsilon, z_pivot}");
   if (which == Y3G_F_SIG_MAX) {
     // par = {n_r, n_m, n_z, r, lnM, z, sigma_1, sigma_2, bias}
     if (nin != 3 || !par)
@@ -959,9 +983,16 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   if (needs_tables && (pt < 0 || pt >= rt->n_points))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad point");
   bool const des_ig = is_ig && is_des_kind(aux->kind);
-  if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !has_des(rt, pt) ||
-                 ((aux->kind == Y3G_DSIGMA6D_DES || aux->kind == Y3G_DSIGMA3D_DES) &&
-                  rt->des[pt].size() <= (size_t)kDesSig)))
+  bool des_tab_ok = des_ig && has_des(rt, pt);
+  if (des_tab_ok) {
+    std::vector<double> const& dd = rt->des[pt];
+    long long const nt = (long long)dd.size() - kDesSig;
+    if (aux->kind == Y3G_DSIGMA6D_DES || aux->kind == Y3G_DSIGMA3D_DES)
+      des_tab_ok = nt > 0 && sigmax_ok(dd.data() + kDesSig, nt);
+    else if (aux->kind == Y3G_COUNT4D_CARD)
+      des_tab_ok = nt > 0 && lclt_ok(dd.data() + kDesSig, nt);
+  }
+  if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !des_tab_ok))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: the real-data Delta-Sigma integrand is "
                              "Y3G_F_INTEGRAND in FP64 with the point's DES data");
   if (des_ig)
@@ -1042,8 +1073,9 @@ int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des)
   for (int k = 0; k < kDesSig; ++k)
     if (!std::isfinite(des[k]))
       return fail(Y3G_ERR_ARG, "y3g_des_upload: non-finite parameter " + std::to_string(k));
-  if (n_des > kDesSig && !sigmax_ok(des + kDesSig, n_des - kDesSig))
-    return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX table sizes or axes");
+  if (n_des > kDesSig && !sigmax_ok(des + kDesSig, n_des - kDesSig) &&
+      !lclt_ok(des + kDesSig, n_des - kDesSig))
+    return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX or INT_LC_LT_DES table sizes or axes");
   if ((size_t)point >= rt->des.size()) rt->des.resize((size_t)point + 1);
   rt->des[point].assign(des, des + n_des);
   rt->des_dirty = true;

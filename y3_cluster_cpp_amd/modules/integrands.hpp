@@ -376,4 +376,96 @@ private:
   grid_point_t gp_{};
 };
 
+// numberCountsCard -- counts over (lnLo, lnLt, zt, lnm), grid (zo_low,
+// zo_high) (gt_card_gpu/numberCounts.cu:126-235): the observed-richness
+// transfer INT_LC_LT_DES_t2 from the reference's table files, read from the
+// working directory as the reference reads them (int_lc_lt_des_t_lt_bins.txt,
+// int_lc_lt_des_t_zt_bins.txt, int_lc_lt_des_t_lbd_bin0..3.txt; values in
+// file order), and MOR_DES_LOG_t from cluster_abundance/mor_logMmin,
+// mor_logRatio, mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot.  Config also
+// takes do_cartesian_product_of_bins (bool, default false), as the
+// reference's constructor does.
+class numberCountsCard : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<2>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit numberCountsCard(cosmosis::DataBlock& cfg)
+  {
+    // an absent key takes the default (the reference insists on success; a
+    // DataBlock may report the default as DBS_USED_DEFAULT)
+    auto const rc =
+      cfg.get_val(module_label(), "do_cartesian_product_of_bins", false, bins_cartesian_);
+    if (rc != DBS_SUCCESS && rc != DBS_USED_DEFAULT)
+      throw std::runtime_error("numberCountsCard failed to find do_cartesian_product_of_bins\n");
+  }
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    SyntheticSample::set_sample(sample);
+    using y3g_cosmosis::view_as_double;
+    std::string const A = "cluster_abundance";
+    des_.assign(5, 0.0);
+    for (char const* k : {"mor_logMmin", "mor_logRatio", "mor_alpha", "mor_sigma", "mor_epsilon",
+                          "z_mor_pivot"})
+      des_.push_back(view_as_double(sample, A, k));
+    if (lclt_.empty()) lclt_ = read_lclt_tables_();
+    des_.insert(des_.end(), lclt_.begin(), lclt_.end());
+  }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(Y3G_COUNT4D_CARD, v, gp_.data(), 2);
+  }
+  static char const* module_label() { return "numberCountsCard"; }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_integration_volumes_wall_of_numbers(
+      cfg, module_label(), {"lnLo", "lnLt", "zt", "lnm"});
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    return y3g_cosmosis::make_grid_points_wall_of_numbers<2>(cfg, module_label(),
+                                                             {"zo_low", "zo_high"});
+  }
+
+private:
+  static std::vector<double>
+  read_numbers_(std::string const& name)
+  {
+    std::ifstream in(name);
+    if (!in) throw std::runtime_error("numberCountsCard: cannot open " + name);
+    std::vector<double> v;
+    double x;
+    while (in >> x) v.push_back(x);
+    return v;
+  }
+  // {n_lt, n_zt, lt knots, zt knots, four tables} as y3g_des_upload takes them
+  static std::vector<double>
+  read_lclt_tables_()
+  {
+    auto const lt = read_numbers_("int_lc_lt_des_t_lt_bins.txt");
+    auto const zt = read_numbers_("int_lc_lt_des_t_zt_bins.txt");
+    std::vector<double> out{double(lt.size()), double(zt.size())};
+    out.insert(out.end(), lt.begin(), lt.end());
+    out.insert(out.end(), zt.begin(), zt.end());
+    for (int b = 0; b < 4; ++b) {
+      auto const t = read_numbers_("int_lc_lt_des_t_lbd_bin" + std::to_string(b) + ".txt");
+      if (t.size() != lt.size() * zt.size())
+        throw std::runtime_error("numberCountsCard: int_lc_lt_des_t_lbd_bin" +
+                                 std::to_string(b) + ".txt holds " + std::to_string(t.size()) +
+                                 " values, not n_lt n_zt");
+      out.insert(out.end(), t.begin(), t.end());
+    }
+    return out;
+  }
+
+  grid_point_t gp_{};
+  bool bins_cartesian_ = false;
+  std::vector<double> lclt_;
+};
+
 #endif
