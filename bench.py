@@ -56,10 +56,10 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # HBM bytes per integrand evaluation of k_cubature<float,512>: rocprofv3
 # FETCH_SIZE x 1.99974 + WRITE_SIZE x 1.000 (factors from a 256 MiB copy read
 # by the same counters) over the evaluations of the same launch, 16 points:
-# 54,249,404 KiB fetched, 25,815,057 KiB written, 2.9160e10 evaluations.
+# 90,862,665 KiB fetched, 24,643,682 KiB written, 2.9160e10 evaluations.
 # Region-slab passes, register spills of the step loop and the HMF table
 # reads of the pair factors; the node integrand reads only LDS.
-HBM_BYTES_PER_EVAL = (54249404.3 * 1.99974 + 25815057.3) * 1024 / 2.9159544810e10
+HBM_BYTES_PER_EVAL = (90862665.1 * 1.99974 + 24643681.7) * 1024 / 2.9159544810e10
 DEFAULT_REGIONS = 32768
 # FP64 operations per term of k_selgl (HMF bilinear 10, selection bilinear
 # 13, the product and the sum 4), from csrc/y3g_kernels.hip.
