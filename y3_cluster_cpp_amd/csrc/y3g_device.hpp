@@ -929,19 +929,36 @@ Y3G_HD double int_lc_lt_des(const double* par, double lc, double lt, double zt)
 #include "y3g_mor_des_tables.inc"
 #undef Y3G_MOR_DES_TABLE
 
-__device__ inline double mor_des_lookup(const double* tab, double si, double lam)
+// Bilinear weights of (sigma_intr, lambda) in the knot grid, clamped: one
+// search serves both tables.
+struct MorDesCell {
+  int i, j;
+  double t, u;
+};
+__device__ inline MorDesCell mor_des_cell(double si, double lam)
 {
   double const x = si < kMorDesSigintr[0] ? kMorDesSigintr[0]
                    : (si > kMorDesSigintr[13] ? kMorDesSigintr[13] : si);
   double const y = lam < kMorDesLsat[0] ? kMorDesLsat[0]
                    : (lam > kMorDesLsat[41] ? kMorDesLsat[41] : lam);
-  int const i = bracket(kMorDesSigintr, 14, x), j = bracket(kMorDesLsat, 42, y);
-  double const t = (x - kMorDesSigintr[i]) / (kMorDesSigintr[i + 1] - kMorDesSigintr[i]);
-  double const u = (y - kMorDesLsat[j]) / (kMorDesLsat[j + 1] - kMorDesLsat[j]);
-  const double* r0 = tab + i * 42 + j;  // row = sigma_intr knot
+  MorDesCell c;
+  c.i = bracket(kMorDesSigintr, 14, x);
+  c.j = bracket(kMorDesLsat, 42, y);
+  c.t = (x - kMorDesSigintr[c.i]) / (kMorDesSigintr[c.i + 1] - kMorDesSigintr[c.i]);
+  c.u = (y - kMorDesLsat[c.j]) / (kMorDesLsat[c.j + 1] - kMorDesLsat[c.j]);
+  return c;
+}
+__device__ inline double mor_des_at(const double* tab, MorDesCell const& c)
+{
+  const double* r0 = tab + c.i * 42 + c.j;  // row = sigma_intr knot
   const double* r1 = r0 + 42;
+  double const t = c.t, u = c.u;
   return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r1[0] + (1.0 - t) * u * r0[1] +
          t * u * r1[1];
+}
+__device__ inline double mor_des_lookup(const double* tab, double si, double lam)
+{
+  return mor_des_at(tab, mor_des_cell(si, lam));
 }
 
 // MOR_DES_LOG_t::operator() (mor_des_log_t.cuh:284-316): the same skew
@@ -956,8 +973,9 @@ __device__ inline double mor_des_log(const double* q, double lt, double lnM, dou
                    pow((1.0 + zt) / (1.0 + q[5]), q[4]);
   double const v1 = exp(lnM) < mmin ? 1.0 : v;
   double const lam = 0.0 < v1 ? v1 : 0.0;  // std::max(0.0, v1)
-  double const sigma = mor_des_lookup(kMorDesSigma, q[3], lam);
-  double const skw = mor_des_lookup(kMorDesSkew, q[3], lam);
+  MorDesCell const cell = mor_des_cell(q[3], lam);
+  double const sigma = mor_des_at(kMorDesSigma, cell);
+  double const skw = mor_des_at(kMorDesSkew, cell);
   double const x = lt - lam;
   double const z = x / sigma;
   return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
@@ -969,8 +987,9 @@ __device__ inline double mor_des(const double* q, double lt, double lnM, double 
   double const v = pow((exp(lnM) - q[0]) / (q[1] - q[0]), q[2]) *
                    pow((1.0 + zt) / (1.0 + q[5]), q[4]);
   double const lam = 1.0 < v ? v : 1.0;
-  double const sigma = mor_des_lookup(kMorDesSigma, q[3], lam);
-  double const skw = mor_des_lookup(kMorDesSkew, q[3], lam);
+  MorDesCell const cell = mor_des_cell(q[3], lam);
+  double const sigma = mor_des_at(kMorDesSigma, cell);
+  double const skw = mor_des_at(kMorDesSkew, cell);
   double const x = lt - lam;
   double const z = x / sigma;
   return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
@@ -1097,9 +1116,22 @@ __device__ inline double sig_max(const double* par, double r, double lnM, double
   const double* s1 = xz + nz;
   const double* s2 = s1 + (size_t)nm * nr;
   const double* bi = s2 + (size_t)nz * nr;
-  double const a = interp2d_clamp(xr, nr, xm, nm, s1, r, lnM);
-  double const b = interp2d_clamp(xz, nz, xm, nm, bi, zt, lnM) *
-                   interp2d_clamp(xr, nr, xz, nz, s2, r, zt);
+  // one clamped search per axis, shared by the tables over it
+  double const rc = r < xr[0] ? xr[0] : (r > xr[nr - 1] ? xr[nr - 1] : r);
+  double const mc = lnM < xm[0] ? xm[0] : (lnM > xm[nm - 1] ? xm[nm - 1] : lnM);
+  double const zc = zt < xz[0] ? xz[0] : (zt > xz[nz - 1] ? xz[nz - 1] : zt);
+  int const ir = bracket(xr, nr, rc), im = bracket(xm, nm, mc), iz = bracket(xz, nz, zc);
+  double const tr = (rc - xr[ir]) / (xr[ir + 1] - xr[ir]);
+  double const tm = (mc - xm[im]) / (xm[im + 1] - xm[im]);
+  double const tz = (zc - xz[iz]) / (xz[iz + 1] - xz[iz]);
+  auto bil = [](const double* z, int nx, int i, int j, double t, double u) {
+    const double* r0 = z + j * nx + i;
+    const double* r1 = r0 + nx;
+    return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r0[1] + (1.0 - t) * u * r1[0] +
+           t * u * r1[1];
+  };
+  double const a = bil(s1, nr, ir, im, tr, tm);
+  double const b = bil(bi, nz, iz, im, tz, tm) * bil(s2, nr, ir, iz, tr, tz);
   return a < b ? b : a;  // std::max(sig_1, sig_2)
 }
 
