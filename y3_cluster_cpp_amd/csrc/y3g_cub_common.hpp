@@ -254,6 +254,10 @@ struct ParentStage {
 };
 static_assert(sizeof(ParentStage) == 2048, "ParentStage is 512 dwords");
 
+// DES data of a point staged in the FP64 workgroup's LDS up to this many
+// doubles (the synthetic SIG_MAX tables of the bench take 887)
+constexpr int kDesLds = 2048;
+
 template <typename T, int BLOCK> struct CubShared {
   // the HMF table stays in global memory (L2) for both precisions: the FP32
   // path reads it only for the pair factors, and in LDS it would take 51 KB
@@ -262,7 +266,9 @@ template <typename T, int BLOCK> struct CubShared {
   static constexpr bool kHmfLds = false;
   T hmf[kHmfLds ? kMaxHmf : 1];
   const T* hmf_tab;  // the table the evaluation reads (LDS or global)
-  const double* des;  // the point's DES data (Y3G_DSIGMA6D_DES), global
+  const double* des;  // the point's DES data (real-data kinds): desl or global
+  // the point's DES data staged in LDS when it fits (FP64 only)
+  double desl[sizeof(T) == 8 ? kDesLds : 1];
   T da[kMaxDa];
   T sci[kMaxSci];
   // node values: the FP32 path keeps only those the fourth differences read

@@ -136,7 +136,16 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       if (tid == 0) {
         S.P = A.points[pt];
         S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
-        S.des = A.des ? A.des + A.des_off[pt] : nullptr;
+      }
+      if (A.des) {
+        long long const o0 = A.des_off[pt], nd = A.des_off[pt + 1] - o0;
+        bool const lds = sizeof(T) == 8 && nd <= kDesLds;
+        if constexpr (sizeof(T) == 8)
+          if (lds)
+            for (long long k = tid; k < nd; k += BLOCK) S.desl[k] = A.des[o0 + k];
+        if (tid == 0) S.des = lds ? S.desl : A.des + o0;
+      } else if (tid == 0) {
+        S.des = nullptr;
       }
       stage_tables<T>(A, pt, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
       __syncthreads();

@@ -173,12 +173,17 @@ static bool sigmax_ok(const double* par, long long npar)
 static int ensure_des(y3g_runtime* rt)
 {
   if (!rt->des_dirty) return 0;
+  // offsets for every point with tables (the kernels read des_off[pt] for
+  // any point of a launch) plus one past the end: point p's data is
+  // [off[p], off[p + 1]), empty for points without DES data
   std::vector<double> data;
-  std::vector<long long> off(std::max<size_t>(rt->des.size(), 1), 0);
-  for (size_t p = 0; p < rt->des.size(); ++p) {
+  size_t const np = std::max<size_t>(rt->des.size(), (size_t)std::max(rt->n_points, 0));
+  std::vector<long long> off(np + 1, 0);
+  for (size_t p = 0; p < np; ++p) {
     off[p] = (long long)data.size();
-    data.insert(data.end(), rt->des[p].begin(), rt->des[p].end());
+    if (p < rt->des.size()) data.insert(data.end(), rt->des[p].begin(), rt->des[p].end());
   }
+  off[np] = (long long)data.size();
   if (data.empty()) data.push_back(0.0);
   if (data.size() > rt->des_cap) {
     dfree(rt->d_des);
@@ -577,6 +582,7 @@ static int alloc_tables(y3g_runtime* rt, const y3g_table_spec* spec, int n)
       dalloc(&rt->sci32, (size_t)n * rt->sci_stride))
     return Y3G_ERR_MEMORY;
   rt->n_points = n;
+  rt->des_dirty = true;  // the DES offsets cover every point with tables
   return 0;
 }
 
