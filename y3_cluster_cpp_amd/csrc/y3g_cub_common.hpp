@@ -267,8 +267,10 @@ template <typename T, int BLOCK> struct CubShared {
   T hmf[kHmfLds ? kMaxHmf : 1];
   const T* hmf_tab;  // the table the evaluation reads (LDS or global)
   const double* des;  // the point's DES data (real-data kinds): desl or global
-  // the point's DES data staged in LDS when it fits (FP64 only)
+  // the point's DES data staged in LDS when it fits, and the MOR_DES tables
+  // (FP64 only)
   double desl[sizeof(T) == 8 ? kDesLds : 1];
+  double mort[sizeof(T) == 8 ? kMorDesWords : 1];
   T da[kMaxDa];
   T sci[kMaxSci];
   // node values: the FP32 path keeps only those the fourth differences read
@@ -642,7 +644,7 @@ __device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 #pragma unroll
     for (int a = 0; a < d; ++a) y[a] = S.c[r][a] + S.h[r][a] * gm_offset(d, j, a);
     if constexpr (is_des_kind(K))
-      S.f[q] = integrand_des_k<K>(S.ig, S.P, tb, S.des, y);
+      S.f[q] = integrand_des_k<K>(S.ig, S.P, tb, S.des, y, mor_des_tabs_at(S.mort));
     else
       S.f[q] = integrand_k<T, K>(S.ig, S.P, tb, y);
   }

@@ -929,23 +929,40 @@ Y3G_HD double int_lc_lt_des(const double* par, double lc, double lt, double zt)
 #include "y3g_mor_des_tables.inc"
 #undef Y3G_MOR_DES_TABLE
 
+// The MOR_DES knot axes and tables, wherever they are held (constant memory
+// by default, a workgroup's LDS copy in the FP64 cubature).
+struct MorDesTabs {
+  const double* si;  // sigma_intr knots [14]
+  const double* ls;  // lambda_sat knots [42]
+  const double* sg;  // sigma table [14][42]
+  const double* sk;  // skewness table [14][42]
+};
+constexpr int kMorDesWords = 14 + 42 + 2 * 14 * 42;
+__device__ inline MorDesTabs mor_des_const_tabs()
+{
+  return {kMorDesSigintr, kMorDesLsat, kMorDesSigma, kMorDesSkew};
+}
+// A copy laid out [si | ls | sg | sk] in `buf`.
+__device__ inline MorDesTabs mor_des_tabs_at(const double* buf)
+{
+  return {buf, buf + 14, buf + 56, buf + 56 + 588};
+}
+
 // Bilinear weights of (sigma_intr, lambda) in the knot grid, clamped: one
 // search serves both tables.
 struct MorDesCell {
   int i, j;
   double t, u;
 };
-__device__ inline MorDesCell mor_des_cell(double si, double lam)
+__device__ inline MorDesCell mor_des_cell(MorDesTabs const& m, double si, double lam)
 {
-  double const x = si < kMorDesSigintr[0] ? kMorDesSigintr[0]
-                   : (si > kMorDesSigintr[13] ? kMorDesSigintr[13] : si);
-  double const y = lam < kMorDesLsat[0] ? kMorDesLsat[0]
-                   : (lam > kMorDesLsat[41] ? kMorDesLsat[41] : lam);
+  double const x = si < m.si[0] ? m.si[0] : (si > m.si[13] ? m.si[13] : si);
+  double const y = lam < m.ls[0] ? m.ls[0] : (lam > m.ls[41] ? m.ls[41] : lam);
   MorDesCell c;
-  c.i = bracket(kMorDesSigintr, 14, x);
-  c.j = bracket(kMorDesLsat, 42, y);
-  c.t = (x - kMorDesSigintr[c.i]) / (kMorDesSigintr[c.i + 1] - kMorDesSigintr[c.i]);
-  c.u = (y - kMorDesLsat[c.j]) / (kMorDesLsat[c.j + 1] - kMorDesLsat[c.j]);
+  c.i = bracket(m.si, 14, x);
+  c.j = bracket(m.ls, 42, y);
+  c.t = (x - m.si[c.i]) / (m.si[c.i + 1] - m.si[c.i]);
+  c.u = (y - m.ls[c.j]) / (m.ls[c.j + 1] - m.ls[c.j]);
   return c;
 }
 __device__ inline double mor_des_at(const double* tab, MorDesCell const& c)
@@ -956,16 +973,13 @@ __device__ inline double mor_des_at(const double* tab, MorDesCell const& c)
   return (1.0 - t) * (1.0 - u) * r0[0] + t * (1.0 - u) * r1[0] + (1.0 - t) * u * r0[1] +
          t * u * r1[1];
 }
-__device__ inline double mor_des_lookup(const double* tab, double si, double lam)
-{
-  return mor_des_at(tab, mor_des_cell(si, lam));
-}
 
 // MOR_DES_LOG_t::operator() (mor_des_log_t.cuh:284-316): the same skew
 // normal with M_min = 10^q0 and M_1 = 10^q1 M_min, lambda = 1 below M_min,
 // and max(0, lambda) in place of max(1, lambda).
 // q = {log10 M_min, log10 (M_1 / M_min), alpha, sigma_intr, epsilon, z_pivot}.
-__device__ inline double mor_des_log(const double* q, double lt, double lnM, double zt)
+__device__ inline double mor_des_log(const double* q, double lt, double lnM, double zt,
+                                     MorDesTabs const& m = mor_des_const_tabs())
 {
   double const mmin = pow(10.0, q[0]);
   double const m1 = pow(10.0, q[1]) * mmin;
@@ -973,23 +987,24 @@ __device__ inline double mor_des_log(const double* q, double lt, double lnM, dou
                    pow((1.0 + zt) / (1.0 + q[5]), q[4]);
   double const v1 = exp(lnM) < mmin ? 1.0 : v;
   double const lam = 0.0 < v1 ? v1 : 0.0;  // std::max(0.0, v1)
-  MorDesCell const cell = mor_des_cell(q[3], lam);
-  double const sigma = mor_des_at(kMorDesSigma, cell);
-  double const skw = mor_des_at(kMorDesSkew, cell);
+  MorDesCell const cell = mor_des_cell(m, q[3], lam);
+  double const sigma = mor_des_at(m.sg, cell);
+  double const skw = mor_des_at(m.sk, cell);
   double const x = lt - lam;
   double const z = x / sigma;
   return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
          erfc(-skw * x / (1.4142135623730951 * sigma));
 }
 
-__device__ inline double mor_des(const double* q, double lt, double lnM, double zt)
+__device__ inline double mor_des(const double* q, double lt, double lnM, double zt,
+                                 MorDesTabs const& m = mor_des_const_tabs())
 {
   double const v = pow((exp(lnM) - q[0]) / (q[1] - q[0]), q[2]) *
                    pow((1.0 + zt) / (1.0 + q[5]), q[4]);
   double const lam = 1.0 < v ? v : 1.0;
-  MorDesCell const cell = mor_des_cell(q[3], lam);
-  double const sigma = mor_des_at(kMorDesSigma, cell);
-  double const skw = mor_des_at(kMorDesSkew, cell);
+  MorDesCell const cell = mor_des_cell(m, q[3], lam);
+  double const sigma = mor_des_at(m.sg, cell);
+  double const skw = mor_des_at(m.sk, cell);
   double const x = lt - lam;
   double const z = x / sigma;
   return exp(-z * z / 2.0) * 0.3989422804014327 / sigma *
@@ -1159,13 +1174,14 @@ constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 //     (gt_card_gpu/numberCounts.cu:160-174).
 template <int K>
 __device__ inline double integrand_des_k(const y3g_integral& ig, const PointDev& P,
-                                         Tabs<double> tb, const double* des, const double* y)
+                                         Tabs<double> tb, const double* des, const double* y,
+                                         MorDesTabs const& mt = mor_des_const_tabs())
 {
   constexpr int iz = KindAxes<K>::iz, im = KindAxes<K>::im;
   if constexpr (K == Y3G_COUNT4D_CARD) {
     double const zt = y[iz], lnM = y[im];
     double const lt = exp(y[1]), lo = exp(y[0]);
-    double const mor_v = mor_des_log(des + kDesMor, lt, lnM, zt);
+    double const mor_v = mor_des_log(des + kDesMor, lt, lnM, zt, mt);
     double const common = omega_z_des(zt) * dv_do_dz<double>(P, tb.da, zt) *
                           hmf<double>(P, tb.hmf, lnM, zt) * mor_v;
     return int_lc_lt_des(des + kDesSig, lo, lt, zt) *
@@ -1174,7 +1190,7 @@ __device__ inline double integrand_des_k(const y3g_integral& ig, const PointDev&
   double const zt = y[iz], lnM = y[im];
   constexpr bool miscent = K == Y3G_DSIGMA6D_DES || K == Y3G_COUNT5D_DES;
   double const lam = miscent ? y[1] : y[0];  // the MOR_DES richness argument
-  double common = mor_des(des + kDesMor, lam, lnM, zt) * dv_do_dz<double>(P, tb.da, zt) *
+  double common = mor_des(des + kDesMor, lam, lnM, zt, mt) * dv_do_dz<double>(P, tb.da, zt) *
                   hmf<double>(P, tb.hmf, lnM, zt) * omega_z_des(zt);
   if constexpr (miscent) {
     double const rmis = y[4];

@@ -113,6 +113,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   if (tid == 0) S.point = -1, S.setup_kind = -1;
   if constexpr (sizeof(T) == 4)
     for (int k = tid; k < kNfwN; k += BLOCK) S.nfwtab[k] = kNfwTab[k];
+  if constexpr (sizeof(T) == 8) {
+    MorDesTabs const c = mor_des_const_tabs();
+    for (int k = tid; k < kMorDesWords; k += BLOCK)
+      S.mort[k] = k < 14 ? c.si[k] : k < 56 ? c.ls[k - 14] : k < 644 ? c.sg[k - 56] : c.sk[k - 644];
+  }
   for (;;) {
     __syncthreads();
     if (tid == 0) S.item = atomicAdd(A.queue, 1);
