@@ -115,6 +115,22 @@ def test_module_setup_without_device_returns_null():
     cfg.put("NumCountsSel", "bin_index", [0.0, 1.0])
     assert not ev.setup(cfg)
     assert ev.execute(cs.DataBlock()) != 0
+    # a real-data module: same behaviour, and fp32split is refused before
+    # any device is touched (these integrands have no FP32 path)
+    L = "SigmaCentY1MortCUDAScalarIntegrand"
+    rd = cs.Module(L)
+    cfg = cs.DataBlock()
+    for k, v in [("algorithm", "pagani"), ("eps_rel", 1e-3), ("eps_abs", 0.0),
+                 ("max_eval", 1e8), ("use_cartesian_product", False)]:
+        cfg.put(L, k, v)
+    for var, lo, hi in [("lo", 20, 30), ("zt", 0.1, 0.45), ("lnm", 29.9, 35.6)]:
+        cfg.put(L, var + "_low", [float(lo)])
+        cfg.put(L, var + "_high", [float(hi)])
+    for var, v in [("zo_low", 0.2), ("zo_high", 0.35), ("radii", 1.0)]:
+        cfg.put(L, var, [v])
+    assert not rd.setup(cfg)
+    cfg.put(L, "precision", "fp32split")
+    assert not rd.setup(cfg)
 
 
 def test_datablock_library_exports_c_api():
