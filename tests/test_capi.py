@@ -54,7 +54,7 @@ def test_params_and_specs_match_oracle():
 def test_runtime_create_without_device_fails_cleanly():
     h = C.c_void_p()
     rc = y3.lib().y3g_runtime_create(0, C.byref(h))
-    if rc == 0:  # a GPU is present (GPU box): nothing to check here
+    if rc == 0:  # a GPU is present: nothing to check here
         y3.lib().y3g_runtime_destroy(h)
         return
     assert rc == 4 and "device" in y3.last_error()
