@@ -1172,8 +1172,10 @@ constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 //   COUNT4D_CARD  (ln lo, ln lt, zt, lnM): int_lc_lt(lo, lt, zt) int_zo_zt
 //     Omega dV/dz hmf mor_des_log(lt, lnM, zt), lo = e^(ln lo), lt = e^(ln lt)
 //     (gt_card_gpu/numberCounts.cu:160-174).
+// Not inlined: inlined into the cubature's node loop it raised the loop's
+// spills (237 VGPRs spilled against 161) and ran at half the speed.
 template <int K>
-__device__ inline double integrand_des_k(const y3g_integral& ig, const PointDev& P,
+__device__ __noinline__ double integrand_des_k(const y3g_integral& ig, const PointDev& P,
                                          Tabs<double> tb, const double* des, const double* y,
                                          MorDesTabs const& mt = mor_des_const_tabs())
 {
