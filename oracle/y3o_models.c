@@ -35,6 +35,17 @@
  *                                                      in z between knots, clamps
  *   y3o_sig_max             sig_max.cuh:60-68          exact on bilinear tables in each
  *                                                      of the two branches; clamps
+ *   y3o_hmf_formula         hmf_t.hh:34-43 (table)     hand values from M and z in powers
+ *                                                      of ten; the table at its knots
+ *   y3o_mor_lognormal       mor_t.hh:29-39 (slot)      unit mass, <ln lambda> = ln lbar
+ *                                                      and Var = sigma_ln^2 by quadrature
+ *   y3o_nfw_conc            nfw_dsigma_mis.hh:88-93    c(1e13, 1e14, 3.7e14, 1e15) by hand
+ *   y3o_zo_pdf / y3o_zo_bin zo_zt_des_t.hh:16-21 form  mean, width 0.01(1+z) by quadrature;
+ *                                                      bin = normal CDF difference
+ *   y3o_sci (table)         buildSigmaCritInv.py:52-73 angular-diameter form with the
+ *                                                      reference's G and c, distances by
+ *                                                      adaptive quadrature; zero behind
+ *                                                      the sources
  */
 #include "y3o.h"
 
@@ -48,7 +59,11 @@
 #define Y3O_RHOC 2.77533742639e+11 /* h^2 Msun / Mpc^3, nfw_dsigma_mis.hh:26 */
 #define Y3O_DH 2997.92458          /* c / H0 in Mpc/h                     */
 #define Y3O_DV_CONST 2997.92       /* dv_do_dz_t.hh:29                    */
-#define Y3O_SCI_CONST 6.0134795e-7 /* 4 pi G / c^2 [pc/Msun] * 1e6 pc/Mpc */
+/* 4 pi G / c^2 in pc^2/Msun per Mpc of distance, from G = 4.51710305e-48
+ * Mpc^3/Msun/s^2 and c = 9.71561189e-15 Mpc/s
+ * (src/modules/sigma_park_y1/buildSigmaCritInv.py:56-57) */
+#define Y3O_SCI_CONST                                                          \
+  (4.0 * Y3O_PI * 4.51710305e-48 / (9.71561189e-15 * 9.71561189e-15) * 1e12)
 #define Y3O_LN_MPIVOT_HMF 32.236191301916641 /* ln 1e14 */
 #define Y3O_LN_MPIVOT_MOR 33.334803590584751 /* ln 3e14 */
 

@@ -230,3 +230,128 @@ def test_hmf_amplitude_factor():
     b1 = O.lib.y3o_hmf(C.byref(p0), C.byref(t.t), lnm, z)
     assert O.lib.y3o_hmf(C.byref(p1), C.byref(t.t), lnm, z) == pytest.approx(
         (0.8 + 0.3 * 0.4342944819) * b1, rel=1e-12)
+
+
+# ---- pins of the synthetic-config functors: each value is rebuilt here from
+# ---- the configuration's own definition with independent arithmetic
+
+def test_hmf_formula_hand_values():
+    """dn/dlnM = A (sigma^-a + 1) exp(-c / sigma^2) with
+    sigma = s_8 (M / 1e14)^-0.25 / (1 + z)^0.8, A = 0.2, a = 1.5, c = 1.2,
+    evaluated from M and z directly (powers of ten, no logarithms)."""
+    def hand(M, z, s8):
+        sig = s8 * (M / 1e14) ** -0.25 / (1 + z) ** 0.8
+        return 0.2 * (sig ** -1.5 + 1) * math.exp(-1.2 / sig ** 2)
+    # at M = 1e14, z = 0 sigma is the amplitude itself: 0.2 (0.8^-1.5 + 1) e^{-1.875}
+    p = O.default_params()
+    assert L.y3o_hmf_formula(C.byref(p), math.log(1e14), 0.0) == pytest.approx(
+        0.2 * (0.8 ** -1.5 + 1) * math.exp(-1.2 / 0.64), rel=1e-14)
+    for M, z, s8 in [(1e13, 0.2, 0.8), (10 ** 15.5, 0.35, 0.8), (3e14, 0.5, 0.7),
+                     (1e15, 0.75, 0.9)]:
+        p = O.default_params(hmf_amp=s8)
+        assert L.y3o_hmf_formula(C.byref(p), math.log(M), z) == pytest.approx(hand(M, z, s8),
+                                                                              rel=1e-13)
+    # the table the integrand interpolates holds the formula at its knots
+    p = O.default_params(hmf_amp=0.75)
+    t = O.OracleTables(p, O.spec_wide())
+    for i, j in [(0, 0), (57, 11), (199, 31)]:
+        M, z = math.exp(t.lnm[i]), t.z[j]
+        assert t.hmf[j * len(t.lnm) + i] == pytest.approx(hand(M, z, 0.75), rel=1e-13)
+
+
+def test_mor_lognormal_moments():
+    """<ln lambda> = 3.2 + slope ln(M / 3e14) + 0.5 ln((1+z)/1.35) and
+    Var(ln lambda) = sigma_ln^2, both by quadrature of the pdf in lambda."""
+    for slope, sln, M, z in [(0.9, 0.25, 3e14, 0.35), (0.7, 0.1, 1e14, 0.2),
+                             (1.1, 0.4, 2e15, 0.6), (0.9, 0.25, 5e13, 0.0)]:
+        p = O.default_params(mor_slope=slope, mor_sigma_ln=sln)
+        lnM = math.log(M)
+        expect = 3.2 + slope * math.log(M / 3e14) + 0.5 * math.log((1 + z) / 1.35)
+        f = lambda lt: L.y3o_mor_lognormal(C.byref(p), lt, lnM, z)
+        c = math.exp(expect)
+        pts = [c * math.exp(k * sln) for k in (-3, 0, 3)]
+        m0 = integrate.quad(f, 0, np.inf, epsabs=0, epsrel=1e-12, limit=400, points=None)[0]
+        lo, hi = c * math.exp(-14 * sln), c * math.exp(14 * sln)
+        m1 = integrate.quad(lambda x: math.log(x) * f(x), lo, hi, epsabs=0, epsrel=1e-12,
+                            limit=400, points=pts)[0]
+        m2 = integrate.quad(lambda x: (math.log(x) - expect) ** 2 * f(x), lo, hi, epsabs=0,
+                            epsrel=1e-12, limit=400, points=pts)[0]
+        assert m0 == pytest.approx(1.0, abs=1e-9)
+        assert m1 == pytest.approx(expect, abs=1e-9)
+        assert m2 == pytest.approx(sln * sln, rel=1e-8)
+
+
+def test_nfw_concentration_values():
+    """c(M) = 5 (M / 1e14)^-0.1."""
+    assert L.y3o_nfw_conc(math.log(1e14)) == pytest.approx(5.0, rel=1e-15)
+    assert L.y3o_nfw_conc(math.log(1e15)) == pytest.approx(5.0 * 10 ** -0.1, rel=1e-14)
+    assert L.y3o_nfw_conc(math.log(1e13)) == pytest.approx(5.0 * 10 ** 0.1, rel=1e-14)
+    assert L.y3o_nfw_conc(math.log(3.7e14)) == pytest.approx(5.0 * 3.7 ** -0.1, rel=1e-14)
+
+
+def test_zobs_pdf_width_and_bin():
+    """z_obs ~ N(z_true, 0.01 (1 + z_true)): unit mass, mean z_true and
+    standard deviation 0.01 (1 + z_true) by quadrature; the bin form is the
+    normal CDF difference."""
+    from scipy.stats import norm
+    p = O.default_params()
+    for zt in (0.2, 0.35, 0.6, 0.9):
+        s = 0.01 * (1 + zt)
+        f = lambda zo: L.y3o_zo_pdf(C.byref(p), zo, zt)
+        a, b = zt - 12 * s, zt + 12 * s
+        m0 = integrate.quad(f, a, b, epsabs=0, epsrel=1e-13, limit=200)[0]
+        m1 = integrate.quad(lambda x: x * f(x), a, b, epsabs=0, epsrel=1e-13, limit=200)[0]
+        m2 = integrate.quad(lambda x: (x - zt) ** 2 * f(x), a, b, epsabs=0, epsrel=1e-13,
+                            limit=200)[0]
+        assert m0 == pytest.approx(1.0, abs=1e-12)
+        assert m1 == pytest.approx(zt, abs=1e-12)
+        assert math.sqrt(m2) == pytest.approx(s, rel=1e-10)
+        for lo, hi in [(0.2, 0.35), (0.35, 0.5), (0.5, 0.65)]:
+            if lo >= zt:  # both ends above the mean: survival functions keep the tail
+                ref = norm.sf(lo, zt, s) - norm.sf(hi, zt, s)
+            else:
+                ref = norm.cdf(hi, zt, s) - norm.cdf(lo, zt, s)
+            assert L.y3o_zo_bin(C.byref(p), lo, hi, zt) == pytest.approx(ref, rel=1e-11,
+                                                                          abs=1e-300)
+
+
+def _chi_quad(om, z):
+    """Flat LCDM comoving distance in Mpc/h by adaptive quadrature."""
+    return 2997.92458 * integrate.quad(lambda x: 1 / math.sqrt(om * (1 + x) ** 3 + 1 - om),
+                                       0, z, epsabs=0, epsrel=1e-13)[0]
+
+
+def test_sigma_crit_inverse_table():
+    """Sigma_crit^-1(z_l) from angular-diameter distances, the reference's
+    form (src/modules/sigma_park_y1/buildSigmaCritInv.py:52-73): 4 pi G / c^2
+    from its G = 4.51710305e-48 Mpc^3/Msun/s^2 and c = 9.71561189e-15 Mpc/s,
+    D_ls = D_s - (1 + z_l)/(1 + z_s) D_l, Sigma_crit^-1 = 4 pi G/c^2 D_l D_ls / D_s;
+    the table holds it in comoving units (the Delta-Sigma profile is comoving),
+    i.e. times (1 + z_l)^2, in pc^2 / Msun with distances in Mpc/h; zero for
+    z_l >= z_s.  D_A = chi / (1 + z) with chi by adaptive quadrature."""
+    from scipy import constants as K
+    G, c = 4.51710305e-48, 9.71561189e-15
+    four_pi_g_c2 = 4 * math.pi * G / c ** 2 * 1e12  # Mpc/Msun -> pc^2/Msun per Mpc
+    # the reference's constants agree with the IAU nominal solar GM (2015 B3)
+    iau = 4 * math.pi * 1.3271244e20 / K.c ** 2 / K.parsec * 1e6  # pc/Msun * pc/Mpc
+    assert four_pi_g_c2 == pytest.approx(iau, rel=2e-5)
+    for om, zs in [(0.3, 1.2), (0.22, 1.2), (0.38, 0.6)]:
+        p = O.default_params(omega_m=om, z_source=zs)
+        t = O.OracleTables(p, O.spec_wide())
+        ds = _chi_quad(om, zs) / (1 + zs)
+        for i in (0, 1, 17, 64, 100, 127):
+            zl = t.sci_z[i]
+            dl = _chi_quad(om, zl) / (1 + zl)
+            dls = ds - (1 + zl) / (1 + zs) * dl
+            ref = four_pi_g_c2 * dl * dls / ds * (1 + zl) ** 2 if zl < zs else 0.0
+            assert t.sci[i] == pytest.approx(ref, rel=1e-7, abs=1e-300), (om, zs, zl)
+        # the lookup is the linear interpolation of the table, clamped
+        zmid = 0.5 * (t.sci_z[40] + t.sci_z[41])
+        assert L.y3o_sci(C.byref(t.t), zmid) == pytest.approx(0.5 * (t.sci[40] + t.sci[41]),
+                                                              rel=1e-14)
+        assert L.y3o_sci(C.byref(t.t), 5.0) == t.sci[-1]
+    # sources in front of every lens knot above z_s: exactly zero
+    p = O.default_params(z_source=0.5)
+    t = O.OracleTables(p, O.spec_wide())
+    assert np.all(t.sci[t.sci_z >= 0.5] == 0.0)
+    assert np.all(t.sci[(t.sci_z > 0) & (t.sci_z < 0.5)] > 0.0)

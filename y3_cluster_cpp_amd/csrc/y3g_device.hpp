@@ -33,7 +33,11 @@ constexpr double kRhoCrit = 2.77533742639e+11;      // h^2 Msun / Mpc^3
 constexpr double kDvConst = 2997.92;                // dv_do_dz_t.cuh:25
 constexpr double kLn3e14Over800Pi = 25.505461977067423;  // ln(3e14 / (800 pi))
 constexpr double kDh = 2997.92458;                  // c / H0 [Mpc/h]
-constexpr double kSciConst = 6.0134795e-7;          // 4 pi G/c^2 * 1e6 pc/Mpc
+// 4 pi G / c^2 in pc^2/Msun per Mpc of distance, from G = 4.51710305e-48
+// Mpc^3/Msun/s^2 and c = 9.71561189e-15 Mpc/s
+// (src/modules/sigma_park_y1/buildSigmaCritInv.py:56-57)
+constexpr double kSciConst = 4.0 * 3.14159265358979323846 * 4.51710305e-48 /
+                             (9.71561189e-15 * 9.71561189e-15) * 1e12;
 constexpr double kTMax = 7.0;
 
 // Per-parameter-point constants (host-prepared).
