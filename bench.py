@@ -407,6 +407,8 @@ def main():
                          "else %d)" % DEFAULT_REGIONS)
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore the autotuner's file (fixed budget for every integral)")
+    ap.add_argument("--dump", default="",
+                    help="write the checked integrals' results to this .npz")
     ap.add_argument("--plan-only", action="store_true",
                     help="check the golden subset, plan the steps and build every "
                          "descriptor, print the plan and exit (no device work)")
@@ -517,6 +519,7 @@ def main():
 
     kern_ms, flops, nev_total = [], 0.0, 0
     ok_all, rel_all, st_all, n_int = [], [], [], 0
+    dump = {f: [] for f in ("point", "value", "error", "status", "neval", "nregions")}
     status_hist = np.zeros(2, dtype=np.int64)
     t0 = time.perf_counter()
     for s_i, sl in enumerate(timed):
@@ -538,6 +541,10 @@ def main():
             ok_all.append(ok)
             rel_all.append(rel)
             st_all.append(rr["status"])
+            if a.dump:
+                dump["point"].append(np.full(N_PER_POINT, k))
+                for f in ("value", "error", "status", "neval", "nregions"):
+                    dump[f].append(rr[f])
         print(f"step {s_i + 1}/{a.steps}: {len(arr)} integrals, kernel {ms:.1f} ms",
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
@@ -548,6 +555,11 @@ def main():
     counted = int(ok_all.sum())
     frac = counted / n_chk
     value = frac * n_int / elapsed
+    if a.dump:
+        d = {f: np.concatenate(v) for f, v in dump.items()}
+        d["ref"] = np.concatenate([gold[int(k)] for k in d["point"][::N_PER_POINT]])
+        d["eps"] = np.concatenate([eps[int(k)] for k in d["point"][::N_PER_POINT]])
+        np.savez(a.dump, **d)
     print(f"timed steps: {a.steps} steps, {n_int} integrals in {elapsed:.2f} s; "
           f"counted {counted}/{n_chk} checked", file=sys.stderr, flush=True)
 
