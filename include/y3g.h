@@ -226,6 +226,24 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
 int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
                   const y3g_options* opt, y3g_result* out);
 
+/* y3g_integrate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, validation, tolerance bands and results as
+ * y3g_integrate (host descriptors in, host results out); the call's device
+ * memory -- the integral and result staging, the work queue, the evaluation
+ * counter and the cubature region slab -- is a 64 MiB array that each device
+ * holds for the life of the process, cleared where it must start at zero
+ * by a fill kernel.  Limits: the staging of n integrals plus one
+ * workgroup's region slab (max_regions x 121 B) must fit in the 64 MiB; the
+ * cubature keeps as many workgroups resident as the rest of the array holds
+ * slabs for (fewer than y3g_integrate at large max_regions).  Rejected with
+ * Y3G_ERR_ARG: inputs over those limits, the plain path (single), the
+ * real-data kinds.  Every call on one device shares that device's array:
+ * calls that overlap on one device (two threads, two runtimes) race. */
+int y3g_integrate_noalloc(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                          const y3g_options* opt, y3g_result* out);
+
 /* HBM bytes of the cubature region slab a batched y3g_integrate with these
  * options allocates on this runtime (resident workgroups x max_regions x
  * 121 B); the slab is per resident workgroup, so it does not grow with the

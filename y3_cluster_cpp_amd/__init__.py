@@ -124,6 +124,8 @@ SIGNATURES = {
                                            _PD, _PD, _PD, C.c_int, C.POINTER(Integral)]),
     "y3g_integrate": (C.c_int, [C.c_void_p, C.POINTER(Integral), C.c_size_t,
                                 C.POINTER(Options), C.c_void_p]),
+    "y3g_integrate_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Integral), C.c_size_t,
+                                        C.POINTER(Options), C.c_void_p]),
     "y3g_runtime_footprint": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(C.c_size_t)]),
     "y3g_server_last_error": (C.c_char_p, []),
     "y3g_server_options_default": (None, [C.POINTER(ServerOptions)]),
@@ -306,6 +308,18 @@ class Runtime:
         o = opts if opts is not None else options()
         _check(lib().y3g_integrate(self._h, arr, n, C.byref(o),
                                    out.ctypes.data_as(C.c_void_p)), "y3g_integrate")
+        return out
+
+    def integrate_noalloc(self, integrals, opts=None):
+        """y3g_integrate_noalloc: as integrate(), with the call's device memory
+        in the library's fixed scratch array (no device allocation)."""
+        n = len(integrals)
+        arr = integrals if isinstance(integrals, C.Array) else (Integral * n)(*integrals)
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        o = opts if opts is not None else options()
+        _check(lib().y3g_integrate_noalloc(self._h, arr, n, C.byref(o),
+                                           out.ctypes.data_as(C.c_void_p)),
+               "y3g_integrate_noalloc")
         return out
 
     def des_upload(self, point, des):

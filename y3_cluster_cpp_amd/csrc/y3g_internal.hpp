@@ -108,6 +108,7 @@ hipError_t launch_cubature(const KernArgs& A, int precision, int block,
 hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
                         hipStream_t s);
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s);
+hipError_t launch_zero_words(void* p, size_t n, hipStream_t s);
 hipError_t launch_functor(const FunctorArgs& F, int precision, hipStream_t s);
 
 }  // namespace y3g

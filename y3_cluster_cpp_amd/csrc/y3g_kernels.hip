@@ -975,6 +975,22 @@ hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
                       : launch_veg<float, 512>(A, slots, s);
 }
 
+// Zero n 32-bit words (counters and queues of memory a caller may capture
+// into a graph: a kernel, not a memset node).
+__global__ void k_zero_words(unsigned* p, size_t n)
+{
+  size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(void* p, size_t n, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  unsigned const nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_words, dim3(nb), dim3(256), 0, s, static_cast<unsigned*>(p), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s)
 {
   hipLaunchKernelGGL(k_tables_build, dim3(n_points), dim3(256), 0, s, A);

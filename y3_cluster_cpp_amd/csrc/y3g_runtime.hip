@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -93,6 +94,40 @@ void gl_nodes(int n, double a, double b, double* x, double* w)
 
 }  // namespace
 
+// Scratch memory of y3g_integrate_noalloc: the integral / result staging,
+// the work queue, the evaluation counter and the cubature region slab of a
+// call live here instead of in buffers the runtime allocates.  One array per
+// device (each device holds its own copy, at its own address, until the
+// process ends).  Host code never uses the name `scratch` as an address: it
+// uses the per-device address from hipGetSymbolAddress (scratch_address).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on `device` (current), looked up once per
+// device.
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_addr;
+
+int scratch_address(int device, unsigned char** out)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(Y3G_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                 hipGetErrorString(e));
+    g_scratch_addr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_addr[device];
+  return 0;
+}
+
+}  // namespace
+
 struct y3g_runtime {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -121,6 +156,7 @@ struct y3g_runtime {
   size_t batch_cap = 0;
   int* d_queue = nullptr;
   unsigned long long* d_nevals = nullptr;
+  unsigned char* scratch = nullptr;  // this device's scratch (y3g_integrate_noalloc)
   double last_ms = 0.0;
   long long last_nevals = 0;
   // plain path workspace
@@ -489,6 +525,11 @@ int y3g_runtime_create(int device, y3g_runtime** out)
     y3g_runtime_destroy(rt);
     return fail(Y3G_ERR_HIP, "runtime resources");
   }
+  if (int rc = scratch_address(device, &rt->scratch)) {
+    std::string const msg = g_err;
+    y3g_runtime_destroy(rt);
+    return fail(rc, msg);
+  }
   *out = rt;
   return 0;
 }
@@ -725,27 +766,109 @@ static int resident_slots(const y3g_runtime* rt, const y3g_options& o, int algor
   return rt->n_cu * per_cu;
 }
 
+// Device buffers of one y3g_integrate call: the runtime's own (allocated
+// and grown on demand), or carved out of the device's scratch array
+// (y3g_integrate_noalloc).
+struct CallBufs {
+  y3g_integral* igs = nullptr;
+  y3g_result* res = nullptr;
+  int* order = nullptr;
+  int* queue = nullptr;
+  unsigned long long* nevals = nullptr;
+  double *lo = nullptr, *hi = nullptr, *est = nullptr, *err = nullptr, *raw = nullptr;
+  int8_t* ax = nullptr;
+  int slab_slots = 0;  // workgroups the scratch slab holds at max_regions
+  bool noalloc = false;
+};
+
+// Zero n words of a call's counters: a fill kernel in the no-allocation
+// path (whose callers may capture it), a memset otherwise.
+static int zero_words(y3g_runtime* rt, const CallBufs& B, void* p, size_t n)
+{
+  if (B.noalloc) {
+    HIPCHK(launch_zero_words(p, n, rt->stream));
+  } else {
+    HIPCHK(hipMemsetAsync(p, 0, 4 * n, rt->stream));
+  }
+  return 0;
+}
+
+// Lay out n integrals, their results and order, the counters and (cubature)
+// a region slab of max_regions per workgroup in the scratch array: 256-byte
+// aligned pieces, as many slab workgroups as the rest holds.
+static int carve_scratch(y3g_runtime* rt, size_t n, int cap, bool slab, CallBufs& B)
+{
+  unsigned char* const base = rt->scratch;
+  size_t off = 256;  // queue (int) at 0, evaluation counter (u64) at 8
+  auto take = [&](size_t bytes) {
+    size_t const at = off;
+    off += (bytes + 255) & ~size_t(255);
+    return base + at;
+  };
+  B.noalloc = true;
+  B.queue = reinterpret_cast<int*>(base);
+  B.nevals = reinterpret_cast<unsigned long long*>(base + 8);
+  B.order = reinterpret_cast<int*>(take(n * sizeof(int)));
+  B.igs = reinterpret_cast<y3g_integral*>(take(n * sizeof(y3g_integral)));
+  B.res = reinterpret_cast<y3g_result*>(take(n * sizeof(y3g_result)));
+  std::string const need = std::to_string(n) + " integrals at max_regions " +
+                           std::to_string(cap);
+  if (off > kScratchBytes)
+    return fail(Y3G_ERR_ARG, "y3g_integrate_noalloc: " + need + " need more than its " +
+                               std::to_string(kScratchBytes) + " bytes of scratch");
+  if (!slab) return 0;
+  // per workgroup: lo, hi [6][cap], est, err, raw [cap] (doubles), axis [cap]
+  size_t const per_slot = (size_t)cap * (2 * kMaxDim * sizeof(double) + 3 * sizeof(double) + 1);
+  size_t const fixed = 6 * 256 + 64;
+  size_t const room = kScratchBytes - off;
+  if (room <= fixed || (room - fixed) / per_slot < 1)
+    return fail(Y3G_ERR_ARG, "y3g_integrate_noalloc: " + need + " need more than its " +
+                               std::to_string(kScratchBytes) + " bytes of scratch (one "
+                               "workgroup's region slab is " + std::to_string(per_slot) +
+                               " bytes)");
+  size_t const slots = std::min<size_t>((room - fixed) / per_slot, 1 << 20);
+  size_t const nr = slots * (size_t)cap;
+  B.lo = reinterpret_cast<double*>(take(nr * kMaxDim * sizeof(double)));
+  B.hi = reinterpret_cast<double*>(take(nr * kMaxDim * sizeof(double)));
+  B.est = reinterpret_cast<double*>(take(nr * sizeof(double)));
+  B.err = reinterpret_cast<double*>(take(nr * sizeof(double)));
+  B.raw = reinterpret_cast<double*>(take(nr * sizeof(double)));
+  B.ax = reinterpret_cast<int8_t*>(take(nr + 64));
+  if (off > kScratchBytes)  // cannot happen: slots were sized to fit
+    return fail(Y3G_ERR_STATE, "y3g_integrate_noalloc: scratch layout");
+  B.slab_slots = (int)slots;
+  return 0;
+}
+
 static int run_group(y3g_runtime* rt, const y3g_integral* igs,
                      const std::vector<int>& ids, const y3g_options& o,
-                     int algorithm, KernArgs A)
+                     int algorithm, KernArgs A, const CallBufs& B)
 {
   if (ids.empty()) return 0;
   std::vector<int> order;
   lpt_order(igs, ids, order);
-  HIPCHK(hipMemcpyAsync(rt->d_order, order.data(), sizeof(int) * order.size(),
+  HIPCHK(hipMemcpyAsync(B.order, order.data(), sizeof(int) * order.size(),
                         hipMemcpyHostToDevice, rt->stream));
-  HIPCHK(hipMemsetAsync(rt->d_queue, 0, sizeof(int), rt->stream));
+  if (int rc = zero_words(rt, B, B.queue, 1)) return rc;
   A.n = (int)order.size();
-  A.order = rt->d_order;
+  A.order = B.order;
+  A.queue = B.queue;
   int const block = o.block_threads == 256 ? 256 : 512;
   int slots = o.single ? 1 : resident_slots(rt, o, algorithm);
   slots = std::max(1, std::min<int>(slots, (int)ids.size()));
   if (algorithm == Y3G_PAGANI) {
-    if (int rc = ensure_slab(rt, std::max(slots, rt->slab_slots), A.cap))
-      return rc;
-    A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
-    A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
-    A.slab_ax = rt->slab_ax;
+    if (B.noalloc) {
+      slots = std::min(slots, B.slab_slots);
+      A.slab_lo = B.lo, A.slab_hi = B.hi;
+      A.slab_est = B.est, A.slab_err = B.err, A.slab_raw = B.raw;
+      A.slab_ax = B.ax;
+    } else {
+      if (int rc = ensure_slab(rt, std::max(slots, rt->slab_slots), A.cap))
+        return rc;
+      A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
+      A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
+      A.slab_ax = rt->slab_ax;
+    }
     HIPCHK(launch_cubature(A, o.precision, block, slots, rt->stream));
   } else {
     HIPCHK(launch_vegas(A, o.precision, block, slots, rt->stream));
@@ -753,28 +876,33 @@ static int run_group(y3g_runtime* rt, const y3g_integral* igs,
   return 0;
 }
 
-int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
-                  const y3g_options* opt, y3g_result* out)
+static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                          const y3g_options* opt, y3g_result* out, bool noalloc)
 {
+  char const* const who = noalloc ? "y3g_integrate_noalloc" : "y3g_integrate";
   if (!rt || (!igs && n) || (!out && n))
-    return fail(Y3G_ERR_ARG, "y3g_integrate: bad arguments");
+    return fail(Y3G_ERR_ARG, std::string(who) + ": bad arguments");
   y3g_options o;
   y3g_options_default(&o);
   if (opt) o = *opt;
   if (n == 0) return 0;
   if (rt->n_points <= 0)
-    return fail(Y3G_ERR_STATE, "y3g_integrate: no tables (call y3g_tables_build)");
+    return fail(Y3G_ERR_STATE, std::string(who) + ": no tables (call y3g_tables_build)");
   if (o.max_regions < 64 || o.max_split_per_step < 1 ||
       o.max_split_per_step > 1024 || o.vegas_nbins < 1 || o.vegas_nbins > 128 ||
       o.vegas_calls < 2 || o.vegas_itmax < 1)
-    return fail(Y3G_ERR_ARG, "y3g_integrate: options out of range");
+    return fail(Y3G_ERR_ARG, std::string(who) + ": options out of range");
   if (o.n_bands < 0 || o.n_bands > 8)
-    return fail(Y3G_ERR_ARG, "y3g_integrate: n_bands out of range");
+    return fail(Y3G_ERR_ARG, std::string(who) + ": n_bands out of range");
   for (int k = 0; k < o.n_bands; ++k)
     if (o.band_regions[k] < 64 || o.band_regions[k] > o.max_regions ||
         (k > 0 && !(o.band_eps[k] > o.band_eps[k - 1])))
-      return fail(Y3G_ERR_ARG, "y3g_integrate: tolerance bands need ascending edges and "
-                               "budgets in [64, max_regions]");
+      return fail(Y3G_ERR_ARG, std::string(who) +
+                                 ": tolerance bands need ascending edges and "
+                                 "budgets in [64, max_regions]");
+  if (noalloc && o.single)
+    return fail(Y3G_ERR_ARG, "y3g_integrate_noalloc: the plain path (single) allocates its "
+                             "own workspace; use y3g_integrate");
   // tolerance-band budgets become per-integral call limits
   std::vector<y3g_integral> banded;
   if (o.n_bands > 0) {
@@ -797,9 +925,13 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
         g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_COUNT4D_CARD ||
         !(g.eps_rel >= 0) || g.max_eval < 1)
-      return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
+      return fail(Y3G_ERR_ARG, std::string(who) + ": integral " + std::to_string(i) +
                                  " is malformed");
     if (is_des_kind(g.kind)) {
+      if (noalloc)
+        return fail(Y3G_ERR_ARG, "y3g_integrate_noalloc: integral " + std::to_string(i) +
+                                   ": the real-data kinds upload their DES data on demand; "
+                                   "use y3g_integrate");
       if (g.ndim != des_ndim(g.kind) || o.precision != Y3G_FP64 || o.single)
         return fail(Y3G_ERR_ARG, "y3g_integrate: integral " + std::to_string(i) +
                                    ": the real-data integrands run on the FP64 batched "
@@ -820,20 +952,27 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     }
     for (int a = 0; a < g.ndim; ++a)
       if (!(g.highs[a] > g.lows[a]))
-        return fail(Y3G_ERR_ARG, "y3g_integrate: empty volume in integral " +
+        return fail(Y3G_ERR_ARG, std::string(who) + ": empty volume in integral " +
                                    std::to_string(i));
     (g.algorithm == Y3G_MCUBES ? veg : cub).push_back((int)i);
   }
   HIPCHK(hipSetDevice(rt->device));
-  if (any_des)
-    if (int rc = ensure_des(rt)) return rc;
-  if (int rc = ensure_batch(rt, n)) return rc;
-  HIPCHK(hipMemcpyAsync(rt->d_igs, igs, sizeof(y3g_integral) * n,
+  CallBufs B;
+  if (noalloc) {
+    if (int rc = carve_scratch(rt, n, o.max_regions, !cub.empty(), B)) return rc;
+  } else {
+    if (any_des)
+      if (int rc = ensure_des(rt)) return rc;
+    if (int rc = ensure_batch(rt, n)) return rc;
+    B.igs = rt->d_igs, B.res = rt->d_res, B.order = rt->d_order;
+    B.queue = rt->d_queue, B.nevals = rt->d_nevals;
+  }
+  HIPCHK(hipMemcpyAsync(B.igs, igs, sizeof(y3g_integral) * n,
                         hipMemcpyHostToDevice, rt->stream));
-  HIPCHK(hipMemsetAsync(rt->d_nevals, 0, sizeof(unsigned long long), rt->stream));
+  if (int rc = zero_words(rt, B, B.nevals, 2)) return rc;
   KernArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.igs = rt->d_igs;
+  A.igs = B.igs;
   A.points = rt->points;
   A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
   A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
@@ -852,9 +991,9 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.vegas_itskip = o.vegas_itskip;
   A.vegas_nbins = o.vegas_nbins;
   A.vegas_alpha = o.vegas_alpha;
-  A.queue = rt->d_queue;
-  A.out = rt->d_res;
-  A.nevals = rt->d_nevals;
+  A.queue = B.queue;
+  A.out = B.res;
+  A.nevals = B.nevals;
   HIPCHK(hipEventRecord(rt->ev0, rt->stream));
   long long extra_nev = 0;
   if (o.single) {
@@ -867,7 +1006,7 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     for (size_t i = 0; i < n; ++i) {
       if (igs[i].algorithm == Y3G_MCUBES) {
         std::vector<int> one{(int)i};
-        if (int rc = run_group(rt, igs, one, o, Y3G_MCUBES, A)) return rc;
+        if (int rc = run_group(rt, igs, one, o, Y3G_MCUBES, A, B)) return rc;
         continue;
       }
       SingleArgs sa = rt->sg;
@@ -883,20 +1022,20 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
     // the VEGAS results were written on the device; cubature ones on the host
     HIPCHK(hipStreamSynchronize(rt->stream));
     std::vector<y3g_result> dres(n);
-    HIPCHK(hipMemcpy(dres.data(), rt->d_res, sizeof(y3g_result) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dres.data(), B.res, sizeof(y3g_result) * n, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i)
       if (igs[i].algorithm != Y3G_MCUBES) dres[i] = host[i];
-    HIPCHK(hipMemcpy(rt->d_res, dres.data(), sizeof(y3g_result) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B.res, dres.data(), sizeof(y3g_result) * n, hipMemcpyHostToDevice));
     extra_nev = nev_single;
   } else {
-    if (int rc = run_group(rt, igs, cub, o, Y3G_PAGANI, A)) return rc;
-    if (int rc = run_group(rt, igs, veg, o, Y3G_MCUBES, A)) return rc;
+    if (int rc = run_group(rt, igs, cub, o, Y3G_PAGANI, A, B)) return rc;
+    if (int rc = run_group(rt, igs, veg, o, Y3G_MCUBES, A, B)) return rc;
   }
   HIPCHK(hipEventRecord(rt->ev1, rt->stream));
-  HIPCHK(hipMemcpyAsync(out, rt->d_res, sizeof(y3g_result) * n,
+  HIPCHK(hipMemcpyAsync(out, B.res, sizeof(y3g_result) * n,
                         hipMemcpyDeviceToHost, rt->stream));
   unsigned long long nev = 0;
-  HIPCHK(hipMemcpyAsync(&nev, rt->d_nevals, sizeof(nev), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(&nev, B.nevals, sizeof(nev), hipMemcpyDeviceToHost,
                         rt->stream));
   HIPCHK(hipStreamSynchronize(rt->stream));
   float ms = 0.f;
@@ -904,6 +1043,18 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   rt->last_ms = ms;
   rt->last_nevals = (long long)nev + extra_nev;
   return 0;
+}
+
+int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                  const y3g_options* opt, y3g_result* out)
+{
+  return integrate_call(rt, igs, n, opt, out, false);
+}
+
+int y3g_integrate_noalloc(y3g_runtime* rt, const y3g_integral* igs, size_t n,
+                          const y3g_options* opt, y3g_result* out)
+{
+  return integrate_call(rt, igs, n, opt, out, true);
 }
 
 int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes)
