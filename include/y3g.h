@@ -33,6 +33,9 @@
  *   y3g_selgl_counts        NumCountsSelGL::set_sample -> SelGLCore::build_weights,
  *                           src/models/n_operator_sel_gl_t.hh:88-150, 184-192
  *                           (all bins of all points in one launch)
+ *   y3g_nfw_mis_table_*     NFW_DSIGMA_MIS's table (nfw_dsigma_mis.cuh:29-48,
+ *                           62-99: read_vector of the three nfw_off_center
+ *                           files into gpu_support::Interp2D)
  *   y3g_eval_functor        the host-side functor calls of the reference's
  *                           Catch2 tests (e.g. test/roffset_t.test.cc:40-46,
  *                           test/int_lc_lt_des_t_test.test.cu for
@@ -273,6 +276,27 @@ int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
  * Replaces any earlier upload for the point. */
 int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des);
 
+/* ---- Tabulated miscentred NFW Delta-Sigma -------------------------------
+ * The table NFW_DSIGMA_MIS reads (src/models/nfw_dsigma_mis.cuh:29-48,
+ * 104-123): T(ln x, ln xmis), the log of the gamma-kernel miscentred
+ * Delta-Sigma in units of 2 r_s delta_c rho, in the reference's on-disk
+ * layout -- read_vector() of nfw_off_center/table_1000_1e-03_5e+03_
+ * gamma_logx.txt (ln x, nx values), ..._gamma_logxmis.txt (ln xmis, nm
+ * values) and ..._log_deltasigma_gamma.txt (nm x nx values, x fastest).
+ *   y3g_nfw_mis_table_build    builds it on the device (1000 x 250 knots,
+ *                              the reference's axes) from the closed-form
+ *                              NFW profile, averaged over the offset ring
+ *                              and the gamma offset kernel (roffset_t)
+ *   y3g_nfw_mis_table_upload   takes the three files' contents instead
+ *   y3g_nfw_mis_table_shape / _download  return them (to write the files)
+ * Y3G_F_NFW_DSIGMA_MIS evaluates the functor over the runtime's table. */
+int y3g_nfw_mis_table_build(y3g_runtime* rt);
+int y3g_nfw_mis_table_upload(y3g_runtime* rt, const double* logx, int nx,
+                             const double* logxmis, int nm, const double* log_dsigma);
+int y3g_nfw_mis_table_shape(y3g_runtime* rt, int* nx, int* nm);
+int y3g_nfw_mis_table_download(y3g_runtime* rt, double* logx, double* logxmis,
+                               double* log_dsigma);
+
 /* ---- Fixed Gauss-Legendre number counts with a selection function --------
  * The reference's NumCountsSelGL (src/models/n_operator_sel_gl_t.hh:171-201
  * over SelGLCore::build_weights, :88-118): per richness / redshift bin b,
@@ -332,6 +356,9 @@ typedef enum {
   Y3G_F_SIG_MAX = 20,         /* (r, lnM, zt); par = {n_r, n_m, n_z, r, lnM, z,
                                  sigma_1 [n_m][n_r], sigma_2 [n_z][n_r],
                                  bias [n_m][n_z]}   SIG_MAX, FP64          */
+  Y3G_F_NFW_DSIGMA_MIS = 22,  /* (r, rmis, lnM); par = {c, rho}: NFW_DSIGMA_MIS
+                                 over the runtime's table (y3g_nfw_mis_table_*),
+                                 FP64 */
   Y3G_F_MOR_DES_LOG = 21      /* (lt, lnM, zt); par = {log10 Mmin, log10 ratio,
                                  alpha, sigma_intr, epsilon, z_pivot}
                                                     MOR_DES_LOG_t, FP64    */

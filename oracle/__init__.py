@@ -107,6 +107,13 @@ def _load():
         "y3o_dsigma_mis_ring": (_D, [_D] * 6),
         "y3o_dsigma_mis_gamma": (_D, [_D] * 6),
         "y3o_dsigma_mis_rayleigh": (_D, [_D] * 7),
+        "y3o_nfw_ring_levels": (C.c_int, [_D, _D]),
+        "y3o_nfw_ring": (_D, [_D, _D]),
+        "y3o_nfw_mis_unit": (_D, [_D, _D]),
+        "y3o_nfw_mis_unit_abs": (_D, [_D, _D]),
+        "y3o_nfw_mis_axes": (None, [_PD, _PD]),
+        "y3o_nfw_mis_entry": (_D, [_D, _D]),
+        "y3o_nfw_dsigma_mis_lookup": (_D, [_PD, C.c_int, _PD, C.c_int, _PD] + [_D] * 5),
         "y3o_default_params": (None, [C.POINTER(Params)]),
         "y3o_spec_config0": (None, [C.POINTER(TableSpec)]),
         "y3o_spec_wide": (None, [C.POINTER(TableSpec)]),

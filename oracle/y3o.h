@@ -255,6 +255,21 @@ double y3o_selgl_counts(const y3o_params* p, const y3o_tables* t,
 /* Test integrands of the reference's integrator known-answer tests. */
 double y3o_test_fn(int which, const double* x, int ndim);
 
+/* Tabulated miscentred NFW Delta-Sigma, NFW_DSIGMA_MIS (y3o_nfwtab.c). */
+#define Y3O_NT_NX 1000
+#define Y3O_NT_NM 250
+#define Y3O_NT_MAXPAN 128
+int    y3o_nfw_ring_levels(double x, double s);
+double y3o_nfw_ring(double x, double s);
+int    y3o_nfw_mis_panels(double x, double m, double* lo, double* hi);
+double y3o_nfw_mis_unit(double x, double m);
+double y3o_nfw_mis_unit_abs(double x, double m);
+void   y3o_nfw_mis_axes(double* logx, double* logxmis);
+double y3o_nfw_mis_entry(double logx, double logxmis);
+double y3o_nfw_dsigma_mis_lookup(const double* logx, int nx, const double* logxmis, int nm,
+                                 const double* T, double r, double rmis, double lnM,
+                                 double c, double rho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,10 @@ script; nothing else of the reference is copied.
     data/int_lc_lt_des_t_*.txt  test/int_lc_lt_des_t_test.test.cu     all (the
                                 (tables of INT_LC_LT_DES_t2)          lt / zt knots
                                                                       and 4 tables)
+    data/nfw_off_center/table_1000_1e-03_5e+03_gamma_logx.txt,
+    ..._gamma_logxmis.txt       the axes NFW_DSIGMA_MIS reads          all
+                                (src/models/nfw_dsigma_mis.cuh:29-41,
+                                test/nfw_dsigma_mis.test.cu)
 
 and the expected values of test/int_lc_lt_des_t_test.test.cu (its `lc`,
 `lt`, `zt` lists and its 720-entry `answers` array, in the test's order
@@ -34,7 +38,9 @@ PLAN = [("test_Rmis.txt", 2, 1), ("test_dv_do_dz.txt", 0, 1),
         ("test_lolc_t.txt", 2, 50),
         ("int_lc_lt_des_t_lt_bins.txt", 0, 1), ("int_lc_lt_des_t_zt_bins.txt", 0, 1),
         ("int_lc_lt_des_t_lbd_bin0.txt", 0, 1), ("int_lc_lt_des_t_lbd_bin1.txt", 0, 1),
-        ("int_lc_lt_des_t_lbd_bin2.txt", 0, 1), ("int_lc_lt_des_t_lbd_bin3.txt", 0, 1)]
+        ("int_lc_lt_des_t_lbd_bin2.txt", 0, 1), ("int_lc_lt_des_t_lbd_bin3.txt", 0, 1),
+        ("nfw_off_center/table_1000_1e-03_5e+03_gamma_logx.txt", 0, 1),
+        ("nfw_off_center/table_1000_1e-03_5e+03_gamma_logxmis.txt", 0, 1)]
 LCLT_TEST = "/root/reference/test/int_lc_lt_des_t_test.test.cu"
 
 
@@ -62,7 +68,7 @@ def main():
             lines = f.read().splitlines()
         head, body = lines[:nheader], [l for l in lines[nheader:] if l.strip()]
         keep = body[::stride]
-        with open(os.path.join(DST, name), "w") as f:
+        with open(os.path.join(DST, os.path.basename(name)), "w") as f:
             f.write("\n".join(head + keep) + "\n")
         print(f"{name}: {len(keep)} of {len(body)} rows")
     lclt_answers()

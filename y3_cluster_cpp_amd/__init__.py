@@ -22,7 +22,7 @@ FP32SPLIT, FP64 = 0, 1
  F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
  F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND, F_INTEGRAND_PAIRED,
  F_INTEGRAND_NODE, F_INT_LC_LT_DES, F_MOR_DES, F_EMG_DES_PDF, F_EMG_DES_CDF,
- F_SIG_MAX, F_MOR_DES_LOG) = range(22)
+ F_SIG_MAX, F_MOR_DES_LOG, F_NFW_DSIGMA_MIS) = range(23)
 
 
 class Params(C.Structure):
@@ -140,6 +140,10 @@ SIGNATURES = {
                                    C.c_int]),
     "y3g_selgl_counts": (C.c_int, [C.c_void_p, C.POINTER(SelGLSpec), C.c_int, C.c_int, _PD]),
     "y3g_des_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int]),
+    "y3g_nfw_mis_table_build": (C.c_int, [C.c_void_p]),
+    "y3g_nfw_mis_table_upload": (C.c_int, [C.c_void_p, _PD, C.c_int, _PD, C.c_int, _PD]),
+    "y3g_nfw_mis_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "y3g_nfw_mis_table_download": (C.c_int, [C.c_void_p, _PD, _PD, _PD]),
     "y3g_eval_functor": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _PD, C.c_int,
                                    C.c_size_t, _PD, C.c_int,
                                    C.POINTER(Integral), _PD]),
@@ -329,6 +333,36 @@ class Runtime:
         d = np.ascontiguousarray(des, dtype=np.float64).reshape(-1)
         _check(lib().y3g_des_upload(self._h, point, d.ctypes.data_as(_PD), d.size),
                "y3g_des_upload")
+
+    def nfw_mis_table_build(self):
+        """Build NFW_DSIGMA_MIS's table on the device (y3g_nfw_mis_table_build)."""
+        _check(lib().y3g_nfw_mis_table_build(self._h), "y3g_nfw_mis_table_build")
+
+    def nfw_mis_table_upload(self, logx, logxmis, log_dsigma):
+        """The reference's three nfw_off_center files' contents
+        (y3g_nfw_mis_table_upload); log_dsigma [len(logxmis), len(logx)]."""
+        a = np.ascontiguousarray(logx, dtype=np.float64)
+        b = np.ascontiguousarray(logxmis, dtype=np.float64)
+        t = np.ascontiguousarray(log_dsigma, dtype=np.float64).reshape(-1)
+        if t.size != a.size * b.size:
+            raise ValueError("log_dsigma must hold len(logxmis) x len(logx) values")
+        _check(lib().y3g_nfw_mis_table_upload(self._h, a.ctypes.data_as(_PD), a.size,
+                                              b.ctypes.data_as(_PD), b.size,
+                                              t.ctypes.data_as(_PD)),
+               "y3g_nfw_mis_table_upload")
+
+    def nfw_mis_table_download(self):
+        """(ln x, ln xmis, T [n_xmis, n_x]) of the runtime's table."""
+        nx, nm = C.c_int(), C.c_int()
+        _check(lib().y3g_nfw_mis_table_shape(self._h, C.byref(nx), C.byref(nm)),
+               "y3g_nfw_mis_table_shape")
+        a, b = np.zeros(nx.value), np.zeros(nm.value)
+        t = np.zeros((nm.value, nx.value))
+        _check(lib().y3g_nfw_mis_table_download(self._h, a.ctypes.data_as(_PD),
+                                                b.ctypes.data_as(_PD),
+                                                t.ctypes.data_as(_PD)),
+               "y3g_nfw_mis_table_download")
+        return a, b, t
 
     def selfn_upload(self, point, lnm, z, s_stack):
         """Selection function of a point: axes lnM [n_lnm], z [n_z] and

@@ -1118,6 +1118,121 @@ __device__ inline double interp2d_clamp(const double* xs, int nx, const double* 
          t * u * r1[1];
 }
 
+// ------------------------------------- tabulated miscentred NFW profile
+// NFW_DSIGMA_MIS (nfw_dsigma_mis.cuh:51-131) reads T(ln x, ln xmis), the log
+// of the gamma-kernel miscentred Delta-Sigma in units of 2 rho_s r_s, on
+// 1000 x 250 log-uniform knots (x = r / r_s over [1e-3, 5e3], xmis = rmis /
+// r_s over [1e-2, 5e2]; the reference's data/nfw_off_center/table_1000_
+// 1e-03_5e+03_gamma_{logx,logxmis}.txt axes).  The table itself is not in the
+// reference tree; it is built here from the closed-form profile:
+//   T = ln(G / 2),  G(x, m) = int_0^60 u e^-u Ring(x, m u) du,
+//   Ring(x, s) = (1/pi) int_0^pi g(X) cos 2 beta dphi,
+//   X^2 = (x - s)^2 + 4 x s sin^2(phi/2),  cos beta = (x - s cos phi) / X,
+// both integrals by 8-point Gauss-Legendre on graded panels (phi toward 0,
+// where the offset centre comes closest; u toward u* = x / m from both
+// sides, pieces at most 2 wide).
+constexpr int kNfwMisNx = 1000, kNfwMisNm = 250;
+constexpr int kNfwMisMaxPanels = 128;
+
+__host__ __device__ inline double gl8_node(int k)
+{
+  constexpr double x[4] = {0.1834346424956498, 0.5255324099163290, 0.7966664774136267,
+                           0.9602898564975363};
+  return k < 4 ? -x[3 - k] : x[k - 4];
+}
+__host__ __device__ inline double gl8_weight(int k)
+{
+  constexpr double w[4] = {0.3626837833783620, 0.3137066458778873, 0.2223810344533745,
+                           0.1012285362903763};
+  return k < 4 ? w[3 - k] : w[k - 4];
+}
+
+// Panel levels of Ring(x, s): the first panel [0, pi 2^-L] is at most 1/16
+// of the width phi_c = |x - s| / sqrt(x s) of the feature near phi = 0.
+__host__ __device__ inline int nfw_ring_levels(double x, double s)
+{
+  double const pc = fabs(x - s) / sqrt(x * s);
+  int L = 1;
+  if (pc < 3.14159265358979323846) {
+    double const q = log2(3.14159265358979323846 / fmax(pc, 1e-300));
+    L = q > 36.0 ? 40 : (int)ceil(q) + 4;
+  }
+  return L > 40 ? 40 : L;
+}
+
+__host__ __device__ inline double nfw_ring(double x, double s)
+{
+  int const L = nfw_ring_levels(x, s);
+  double acc = 0.0, a = 0.0;
+  for (int j = 0; j <= L; ++j) {
+    double const b = 3.14159265358979323846 * ldexp(1.0, j - L);
+    double const c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double ps = 0.0;
+    for (int k = 0; k < 8; ++k) {
+      double const phi = c + h * gl8_node(k);
+      double const sn = sin(0.5 * phi);
+      double const d = x - s;
+      double const X2 = d * d + 4.0 * x * s * sn * sn;
+      double const cb = d + 2.0 * s * sn * sn;  // x - s cos(phi)
+      ps += gl8_weight(k) * nfw_g<double>(sqrt(X2)) * (2.0 * cb * cb / X2 - 1.0);
+    }
+    acc += h * ps;
+    a = b;
+  }
+  return acc / 3.14159265358979323846;
+}
+
+// The outer panels of G(x, m) on u in [0, 60] (at most kNfwMisMaxPanels).
+__host__ __device__ inline int nfw_mis_panels(double x, double m, double* lo, double* hi)
+{
+  double const U = 60.0, us = x / m;
+  double br[64];
+  int nb = 0;
+  br[nb++] = 0.0;
+  if (us > 0.0 && us < U) {
+    for (int k = 0; k <= 24; ++k) {
+      double const p = us - us * ldexp(1.0, -k);
+      if (p > br[nb - 1]) br[nb++] = p;
+    }
+    br[nb++] = us;
+    for (int k = 24; k >= 0; --k) {
+      double const p = us + (U - us) * ldexp(1.0, -k);
+      if (p > br[nb - 1] && p <= U) br[nb++] = p;
+    }
+    if (br[nb - 1] < U) br[nb++] = U;
+  } else {
+    br[nb++] = U;
+  }
+  int n = 0;
+  for (int i = 0; i + 1 < nb; ++i) {
+    double const a = br[i], b = br[i + 1];
+    int const pieces = (int)ceil((b - a) / 2.0);
+    int const np = pieces < 1 ? 1 : pieces;
+    for (int q = 0; q < np && n < kNfwMisMaxPanels; ++q) {
+      lo[n] = a + (b - a) * q / np;
+      hi[n] = q == np - 1 ? b : a + (b - a) * (q + 1) / np;
+      ++n;
+    }
+  }
+  return n;
+}
+
+// NFW_DSIGMA_MIS::operator()(r, rmis, lnM) (nfw_dsigma_mis.cuh:109-123) over
+// a table tab = {ln x [nx], ln xmis [nm], T [nm][nx]} (x fastest):
+// 2 r_s delta_c rho exp(T.clamp(ln(r / r_s), ln(rmis / r_s))) 1e-12 with
+// delta_c = (200 c^3 / 3) / (ln(1 + c) - c / (1 + c)) and
+// r_s = cbrt(3 M / (800 pi rho)) / c.
+__device__ inline double nfw_dsigma_mis_tab(const double* tab, int nx, int nm, double r,
+                                            double rmis, double lnM, double c, double rho)
+{
+  double const delta_c = (200.0 * c * c * c / 3.0) / (log(1.0 + c) - c / (1.0 + c));
+  double const r_200 = cbrt(3.0 * exp(lnM) / (800.0 * 3.14159265358979323846 * rho));
+  double const r_s = r_200 / c;
+  double const lu = interp2d_clamp(tab, nx, tab + nx, nm, tab + nx + nm, log(r / r_s),
+                                   log(rmis / r_s));
+  return 2.0 * r_s * delta_c * rho * exp(lu) * 1e-12;
+}
+
 // SIG_MAX::operator() (sig_max.cuh:60-68): the halo Delta-Sigma as the larger
 // of the one-halo term sigma_1(r, lnM) and bias(zt, lnM) sigma_2(r, zt), each
 // a clamped bilinear lookup in the sample's tables (deltasigma/{r_sigma_

@@ -75,6 +75,8 @@ struct FunctorArgs {
   const void* sci;
   const double* des;       // the point's DES data (Y3G_DSIGMA6D_DES integrands)
   const y3g_integral* ig;
+  const double* nfwmis;    // NFW_DSIGMA_MIS table {ln x, ln xmis, T}
+  int nfw_nx, nfw_nm;
   double* out;
 };
 
@@ -109,6 +111,7 @@ hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
                         hipStream_t s);
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s);
 hipError_t launch_zero_words(void* p, size_t n, hipStream_t s);
+hipError_t launch_nfw_mis_table(double* tab, int nx, int nm, hipStream_t s);
 hipError_t launch_functor(const FunctorArgs& F, int precision, hipStream_t s);
 
 }  // namespace y3g

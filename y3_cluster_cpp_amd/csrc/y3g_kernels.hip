@@ -673,6 +673,9 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     case Y3G_F_EMG_DES_PDF: r = T(emg_des_pdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_EMG_DES_CDF: r = T(emg_des_cdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_SIG_MAX: r = T(sig_max(q, x[0], x[1], x[2])); break;
+    case Y3G_F_NFW_DSIGMA_MIS:
+      r = T(nfw_dsigma_mis_tab(F.nfwmis, F.nfw_nx, F.nfw_nm, x[0], x[1], x[2], q[0], q[1]));
+      break;
     case Y3G_F_DV_DO_DZ: r = dv_do_dz<T>(P, tb.da, x[0]); break;
     case Y3G_F_HMF: r = hmf<T>(P, tb.hmf, x[0], x[1]); break;
     case Y3G_F_MOR: {
@@ -988,6 +991,41 @@ hipError_t launch_zero_words(void* p, size_t n, hipStream_t s)
   if (n == 0) return hipSuccess;
   unsigned const nb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_zero_words, dim3(nb), dim3(256), 0, s, static_cast<unsigned*>(p), n);
+  return hipGetLastError();
+}
+
+// NFW_DSIGMA_MIS table, one 256-lane workgroup per entry (xmis_j, x_i):
+// the lanes share the outer Gauss-Legendre nodes of G(x_i, xmis_j) (panels
+// from lane 0 through LDS), each node a ring average; FP64 block sum, then
+// T = ln(G / 2).  tab = {ln x [nx], ln xmis [nm], T [nm][nx]}, axes filled.
+__global__ __launch_bounds__(256) void k_nfw_mis_table(double* tab, int nx, int nm)
+{
+  __shared__ double plo[kNfwMisMaxPanels], phi[kNfwMisMaxPanels];
+  __shared__ double red[4];
+  __shared__ int npan;
+  int const e = blockIdx.x;
+  int const i = e % nx, j = e / nx;
+  double const x = exp(tab[i]), m = exp(tab[nx + j]);
+  if (threadIdx.x == 0) npan = nfw_mis_panels(x, m, plo, phi);
+  __syncthreads();
+  int const nn = 8 * npan;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < nn; q += 256) {
+    int const p = q >> 3, k = q & 7;
+    double const c = 0.5 * (plo[p] + phi[p]), h = 0.5 * (phi[p] - plo[p]);
+    double const u = c + h * gl8_node(k);
+    acc += h * gl8_weight(k) * u * exp(-u) * nfw_ring(x, m * u);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    tab[nx + nm + (size_t)j * nx + i] = log(0.5 * (red[0] + red[1] + red[2] + red[3]));
+}
+
+hipError_t launch_nfw_mis_table(double* tab, int nx, int nm, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_nfw_mis_table, dim3((unsigned)(nx * nm)), dim3(256), 0, s, tab, nx, nm);
   return hipGetLastError();
 }
 

@@ -185,6 +185,9 @@ struct y3g_runtime {
   double* d_des = nullptr;
   long long* d_des_off = nullptr;
   size_t des_cap = 0, des_off_cap = 0;
+  // NFW_DSIGMA_MIS table {ln x [nx], ln xmis [nm], T [nm][nx]}
+  double* d_nfwmis = nullptr;
+  int nfw_nx = 0, nfw_nm = 0;
 };
 
 // SIG_MAX tables packed as sig_max() reads them: sizes, increasing axes.
@@ -545,6 +548,7 @@ int y3g_runtime_destroy(y3g_runtime* rt)
   dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
     dfree(rt->slab_err), dfree(rt->slab_raw), dfree(rt->slab_ax);
   dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
+  dfree(rt->d_nfwmis);
   dfree(rt->d_queue), dfree(rt->d_nevals);
   dfree(rt->sg.lo), dfree(rt->sg.hi), dfree(rt->sg.est), dfree(rt->sg.err);
   dfree(rt->sg.ax), dfree(rt->sg.list), dfree(rt->sg.sel), dfree(rt->sg.pest);
@@ -1081,8 +1085,16 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_MOR_DES_LOG)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_NFW_DSIGMA_MIS)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_NFW_DSIGMA_MIS) {
+    if (nin != 3 || !par || npar != 2 || precision != Y3G_FP64)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: NFW_DSIGMA_MIS takes (r, rmis, lnM), "
+                               "{c, rho} and FP64");
+    if (!rt->d_nfwmis)
+      return fail(Y3G_ERR_STATE, "y3g_eval_functor: NFW_DSIGMA_MIS needs its table "
+                                 "(y3g_nfw_mis_table_build / _upload)");
+  }
   if (which == Y3G_F_MOR_DES_LOG && (nin != 3 || !par || npar != 6))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: MOR_DES_LOG takes (lt, lnM, zt) and "
                              "{log10 Mmin, log10 ratio, alpha, sigma_intr, epsilon, z_pivot}");
@@ -1193,6 +1205,9 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
   }
   F.des = des_ig ? rt->d_des + des_off : nullptr;
   F.ig = d_ig;
+  F.nfwmis = rt->d_nfwmis;
+  F.nfw_nx = rt->nfw_nx;
+  F.nfw_nm = rt->nfw_nm;
   F.out = d_out;
   hipError_t e = launch_functor(F, precision, rt->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
@@ -1347,3 +1362,82 @@ int y3g_selgl_counts(y3g_runtime* rt, const y3g_selgl_spec* spec, int n_points,
 }
 
 }  // extern "C"
+
+// ------------------------------------------ NFW_DSIGMA_MIS table
+static int nfw_mis_alloc(y3g_runtime* rt, int nx, int nm)
+{
+  if (rt->d_nfwmis && rt->nfw_nx == nx && rt->nfw_nm == nm) return 0;
+  dfree(rt->d_nfwmis);
+  rt->nfw_nx = rt->nfw_nm = 0;
+  if (int rc = dalloc(&rt->d_nfwmis, (size_t)nx + nm + (size_t)nx * nm)) return rc;
+  rt->nfw_nx = nx;
+  rt->nfw_nm = nm;
+  return 0;
+}
+
+int y3g_nfw_mis_table_build(y3g_runtime* rt)
+{
+  if (!rt) return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_build: null runtime");
+  HIPCHK(hipSetDevice(rt->device));
+  int const nx = kNfwMisNx, nm = kNfwMisNm;
+  // the reference's axes: ln x over [ln 1e-3, ln 5e3], ln xmis over
+  // [ln 1e-2, ln 5e2], uniform
+  std::vector<double> ax((size_t)nx + nm);
+  double const a = std::log(1e-3), b = std::log(5e3), c = std::log(1e-2), d = std::log(5e2);
+  for (int i = 0; i < nx; ++i) ax[i] = a + i * ((b - a) / (nx - 1));
+  for (int j = 0; j < nm; ++j) ax[nx + j] = c + j * ((d - c) / (nm - 1));
+  if (int rc = nfw_mis_alloc(rt, nx, nm)) return rc;
+  HIPCHK(hipMemcpyAsync(rt->d_nfwmis, ax.data(), sizeof(double) * ax.size(),
+                        hipMemcpyHostToDevice, rt->stream));
+  HIPCHK(launch_nfw_mis_table(rt->d_nfwmis, nx, nm, rt->stream));
+  HIPCHK(hipStreamSynchronize(rt->stream));
+  return 0;
+}
+
+int y3g_nfw_mis_table_upload(y3g_runtime* rt, const double* logx, int nx,
+                             const double* logxmis, int nm, const double* log_dsigma)
+{
+  if (!rt || !logx || !logxmis || !log_dsigma || nx < 2 || nm < 2 || nx > (1 << 16) ||
+      nm > (1 << 16))
+    return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_upload: bad arguments");
+  for (int i = 1; i < nx; ++i)
+    if (!(logx[i] > logx[i - 1]))
+      return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_upload: ln x not increasing");
+  for (int j = 1; j < nm; ++j)
+    if (!(logxmis[j] > logxmis[j - 1]))
+      return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_upload: ln xmis not increasing");
+  for (size_t k = 0; k < (size_t)nx * nm; ++k)
+    if (!std::isfinite(log_dsigma[k]))
+      return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_upload: table value not finite");
+  HIPCHK(hipSetDevice(rt->device));
+  if (int rc = nfw_mis_alloc(rt, nx, nm)) return rc;
+  HIPCHK(hipMemcpy(rt->d_nfwmis, logx, sizeof(double) * nx, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rt->d_nfwmis + nx, logxmis, sizeof(double) * nm, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rt->d_nfwmis + nx + nm, log_dsigma, sizeof(double) * nx * nm,
+                   hipMemcpyHostToDevice));
+  return 0;
+}
+
+int y3g_nfw_mis_table_shape(y3g_runtime* rt, int* nx, int* nm)
+{
+  if (!rt || !nx || !nm) return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_shape: bad arguments");
+  *nx = rt->nfw_nx;
+  *nm = rt->nfw_nm;
+  return 0;
+}
+
+int y3g_nfw_mis_table_download(y3g_runtime* rt, double* logx, double* logxmis,
+                               double* log_dsigma)
+{
+  if (!rt) return fail(Y3G_ERR_ARG, "y3g_nfw_mis_table_download: null runtime");
+  if (!rt->d_nfwmis) return fail(Y3G_ERR_STATE, "y3g_nfw_mis_table_download: no table");
+  HIPCHK(hipSetDevice(rt->device));
+  int const nx = rt->nfw_nx, nm = rt->nfw_nm;
+  if (logx) HIPCHK(hipMemcpy(logx, rt->d_nfwmis, sizeof(double) * nx, hipMemcpyDeviceToHost));
+  if (logxmis)
+    HIPCHK(hipMemcpy(logxmis, rt->d_nfwmis + nx, sizeof(double) * nm, hipMemcpyDeviceToHost));
+  if (log_dsigma)
+    HIPCHK(hipMemcpy(log_dsigma, rt->d_nfwmis + nx + nm, sizeof(double) * nx * nm,
+                     hipMemcpyDeviceToHost));
+  return 0;
+}
