@@ -359,6 +359,8 @@ typedef enum {
   Y3G_F_NFW_DSIGMA_MIS = 22,  /* (r, rmis, lnM); par = {c, rho}: NFW_DSIGMA_MIS
                                  over the runtime's table (y3g_nfw_mis_table_*),
                                  FP64 */
+  Y3G_F_LC_LT = 23,           /* (lc, lt, zt)   LC_LT_t over its fitted tables
+                                 (lc_lt_t.cuh:36-68, lc_lt_t.cc:10-214) */
   Y3G_F_MOR_DES_LOG = 21      /* (lt, lnM, zt); par = {log10 Mmin, log10 ratio,
                                  alpha, sigma_intr, epsilon, z_pivot}
                                                     MOR_DES_LOG_t, FP64    */

@@ -107,6 +107,8 @@ def _load():
         "y3o_dsigma_mis_ring": (_D, [_D] * 6),
         "y3o_dsigma_mis_gamma": (_D, [_D] * 6),
         "y3o_dsigma_mis_rayleigh": (_D, [_D] * 7),
+        "y3o_lc_lt": (_D, [_D, _D, _D]),
+        "y3o_lc_lt_params": (None, [_D, _D, _PD]),
         "y3o_nfw_ring_levels": (C.c_int, [_D, _D]),
         "y3o_nfw_ring": (_D, [_D, _D]),
         "y3o_nfw_mis_unit": (_D, [_D, _D]),

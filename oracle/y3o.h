@@ -255,6 +255,10 @@ double y3o_selgl_counts(const y3o_params* p, const y3o_tables* t,
 /* Test integrands of the reference's integrator known-answer tests. */
 double y3o_test_fn(int which, const double* x, int ndim);
 
+/* LC_LT_t over the reference's fitted tables (y3o_models.c). */
+double y3o_lc_lt(double lc, double lt, double zt);
+void   y3o_lc_lt_params(double lt, double zt, double* out5);
+
 /* Tabulated miscentred NFW Delta-Sigma, NFW_DSIGMA_MIS (y3o_nfwtab.c). */
 #define Y3O_NT_NX 1000
 #define Y3O_NT_NM 250

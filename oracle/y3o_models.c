@@ -42,6 +42,10 @@
  *   y3o_nfw_conc            nfw_dsigma_mis.hh:88-93    c(1e13, 1e14, 3.7e14, 1e15) by hand
  *   y3o_zo_pdf / y3o_zo_bin zo_zt_des_t.hh:16-21 form  mean, width 0.01(1+z) by quadrature;
  *                                                      bin = normal CDF difference
+ *   y3o_lc_lt               lc_lt_t.hh:31-63, .cc      data/test_lc_lt_t_SDSS.txt (1000 rows,
+ *                                                      1e-6, lc_lt_t.test.cc:11-51); unit
+ *                                                      mass (lc_lt_probability_unity); table
+ *                                                      entries at knots read by hand
  *   y3o_sci (table)         buildSigmaCritInv.py:52-73 angular-diameter form with the
  *                                                      reference's G and c, distances by
  *                                                      adaptive quadrature; zero behind
@@ -410,6 +414,33 @@ y3o_lc_lt_eq15(double lc, double lt, double mu, double sigma, double tau,
          0.5 * fmsk / lt * (erfc((lc - mu) / r2s) - erfc((lc + lt - mu) / r2s)) -
          0.5 * fmsk * fprj / lt *
            (exp(-tau * lt) * exptau * erfc((mts - (lc + lt)) / r2s));
+}
+
+/* LC_LT_t::operator()(lc, lt, zt) (lc_lt_t.hh:31-63 / lc_lt_t.cuh:36-68):
+ * eq. 15 with tau, mu, sigma, fmsk, fprj clamped bilinear in (lt, zt) over
+ * the reference's fitted tables (lc_lt_t.cc:10-214: 22 lt knots and 5 zt
+ * knots, each bin list without its upper limit; tables lt fastest). */
+#define Y3G_LC_LT_TABLE(name, n) static const double name[n]
+#include "../y3_cluster_cpp_amd/csrc/y3g_lc_lt_tables.inc"
+#undef Y3G_LC_LT_TABLE
+
+double
+y3o_lc_lt(double lc, double lt, double zt)
+{
+  double const tau = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtTau, lt, zt);
+  double const mu = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtMu, lt, zt);
+  double const sigma = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtSigma, lt, zt);
+  double const fmsk = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtFmsk, lt, zt);
+  double const fprj = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtFprj, lt, zt);
+  return y3o_lc_lt_eq15(lc, lt, mu, sigma, tau, fmsk, fprj);
+}
+
+/* The five parameters of LC_LT_t at (lt, zt): tau, mu, sigma, fmsk, fprj. */
+void
+y3o_lc_lt_params(double lt, double zt, double* out5)
+{
+  const double* t[5] = {kLcLtTau, kLcLtMu, kLcLtSigma, kLcLtFmsk, kLcLtFprj};
+  for (int k = 0; k < 5; ++k) out5[k] = y3o_interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, t[k], lt, zt);
 }
 
 /* ---------------- erfcx (FP64) ------------------------------------------ */

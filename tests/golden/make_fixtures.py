@@ -16,6 +16,7 @@ script; nothing else of the reference is copied.
     data/int_lc_lt_des_t_*.txt  test/int_lc_lt_des_t_test.test.cu     all (the
                                 (tables of INT_LC_LT_DES_t2)          lt / zt knots
                                                                       and 4 tables)
+    data/test_lc_lt_t_SDSS.txt  test/lc_lt_t.test.cc:11-51            all
     data/nfw_off_center/table_1000_1e-03_5e+03_gamma_logx.txt,
     ..._gamma_logxmis.txt       the axes NFW_DSIGMA_MIS reads          all
                                 (src/models/nfw_dsigma_mis.cuh:29-41,
@@ -39,6 +40,7 @@ PLAN = [("test_Rmis.txt", 2, 1), ("test_dv_do_dz.txt", 0, 1),
         ("int_lc_lt_des_t_lt_bins.txt", 0, 1), ("int_lc_lt_des_t_zt_bins.txt", 0, 1),
         ("int_lc_lt_des_t_lbd_bin0.txt", 0, 1), ("int_lc_lt_des_t_lbd_bin1.txt", 0, 1),
         ("int_lc_lt_des_t_lbd_bin2.txt", 0, 1), ("int_lc_lt_des_t_lbd_bin3.txt", 0, 1),
+        ("test_lc_lt_t_SDSS.txt", 0, 1),
         ("nfw_off_center/table_1000_1e-03_5e+03_gamma_logx.txt", 0, 1),
         ("nfw_off_center/table_1000_1e-03_5e+03_gamma_logxmis.txt", 0, 1)]
 LCLT_TEST = "/root/reference/test/int_lc_lt_des_t_test.test.cu"

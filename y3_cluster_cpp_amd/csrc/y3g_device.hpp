@@ -1118,6 +1118,37 @@ __device__ inline double interp2d_clamp(const double* xs, int nx, const double* 
          t * u * r1[1];
 }
 
+// ------------------------------------------------ tabulated LC_LT_t
+// LC_LT_t::operator()(lc, lt, zt) (lc_lt_t.cuh:36-68): Costanzi et al. 2018
+// eq. 15 with its five parameters clamped bilinear in (lt, zt) over the
+// reference's fitted tables (lc_lt_t.cc:10-214; y3g_lc_lt_tables.inc):
+//   (1 - fmsk)(1 - fprj) G(lc; mu, sigma)
+//   + 1/2 ((1 - fmsk) fprj tau + fmsk fprj / lt) e^{tau(2 mu + tau sigma^2 - 2 lc)/2}
+//       erfc((mu + tau sigma^2 - lc) / (sqrt2 sigma))
+//   + 1/2 fmsk / lt (erfc((lc - mu) / (sqrt2 sigma)) - erfc((lc + lt - mu) / (sqrt2 sigma)))
+//   - 1/2 fmsk fprj / lt e^{-tau lt} e^{tau(...)} erfc((mu + tau sigma^2 - lc - lt) / (sqrt2 sigma))
+#define Y3G_LC_LT_TABLE(name, n) static __constant__ double name[n]
+#include "y3g_lc_lt_tables.inc"
+#undef Y3G_LC_LT_TABLE
+
+__device__ inline double lc_lt_tab(double lc, double lt, double zt)
+{
+  double const tau = interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtTau, lt, zt);
+  double const mu = interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtMu, lt, zt);
+  double const sigma = interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtSigma, lt, zt);
+  double const fmsk = interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtFmsk, lt, zt);
+  double const fprj = interp2d_clamp(kLcLtLt, 22, kLcLtZt, 5, kLcLtFprj, lt, zt);
+  double const exptau = exp(tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lc) / 2.0);
+  double const r2s = 1.4142135623730951 * sigma;
+  double const mts = mu + tau * sigma * sigma;
+  double const z = (lc - mu) / sigma;
+  double const gauss = exp(-0.5 * z * z) * 0.3989422804014327 / sigma;
+  return (1.0 - fmsk) * (1.0 - fprj) * gauss +
+         0.5 * ((1.0 - fmsk) * fprj * tau + fmsk * fprj / lt) * exptau * erfc((mts - lc) / r2s) +
+         0.5 * fmsk / lt * (erfc((lc - mu) / r2s) - erfc((lc + lt - mu) / r2s)) -
+         0.5 * fmsk * fprj / lt * (exp(-tau * lt) * exptau * erfc((mts - (lc + lt)) / r2s));
+}
+
 // ------------------------------------- tabulated miscentred NFW profile
 // NFW_DSIGMA_MIS (nfw_dsigma_mis.cuh:51-131) reads T(ln x, ln xmis), the log
 // of the gamma-kernel miscentred Delta-Sigma in units of 2 rho_s r_s, on

@@ -1085,8 +1085,10 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_NFW_DSIGMA_MIS)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_LC_LT)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_LC_LT && nin != 3)
+    return fail(Y3G_ERR_ARG, "y3g_eval_functor: LC_LT takes (lc, lt, zt)");
   if (which == Y3G_F_NFW_DSIGMA_MIS) {
     if (nin != 3 || !par || npar != 2 || precision != Y3G_FP64)
       return fail(Y3G_ERR_ARG, "y3g_eval_functor: NFW_DSIGMA_MIS takes (r, rmis, lnM), "
