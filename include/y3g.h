@@ -359,6 +359,12 @@ typedef enum {
   Y3G_F_NFW_DSIGMA_MIS = 22,  /* (r, rmis, lnM); par = {c, rho}: NFW_DSIGMA_MIS
                                  over the runtime's table (y3g_nfw_mis_table_*),
                                  FP64 */
+  Y3G_F_SCI_AVERAGE = 24,     /* (zl); par = {n, zlense[n], sci_average[n]}:
+                                 the sample's average_sigma_crit_inv table,
+                                 Interp1D clamp (lensing_weights.hh:55-60, 308) */
+  Y3G_F_SCI_RZ = 25,          /* (r, zt); par = {n_r, n_z, r[n_r], z[n_z],
+                                 sigma_crit_inv[n_z][n_r]}: Interp2D clamp(r, z)
+                                 (gamma_1h_nfw.cuh:81-88, 120) */
   Y3G_F_LC_LT = 23,           /* (lc, lt, zt)   LC_LT_t over its fitted tables
                                  (lc_lt_t.cuh:36-68, lc_lt_t.cc:10-214) */
   Y3G_F_MOR_DES_LOG = 21      /* (lt, lnM, zt); par = {log10 Mmin, log10 ratio,

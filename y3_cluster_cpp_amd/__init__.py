@@ -22,7 +22,8 @@ FP32SPLIT, FP64 = 0, 1
  F_DV_DO_DZ, F_HMF, F_MOR, F_LO_LT_PDF, F_LO_LT_BIN, F_NFW_DSIGMA,
  F_DSIGMA_OFFSET, F_SCI, F_INTEGRAND, F_INTEGRAND_PAIRED,
  F_INTEGRAND_NODE, F_INT_LC_LT_DES, F_MOR_DES, F_EMG_DES_PDF, F_EMG_DES_CDF,
- F_SIG_MAX, F_MOR_DES_LOG, F_NFW_DSIGMA_MIS, F_LC_LT) = range(24)
+ F_SIG_MAX, F_MOR_DES_LOG, F_NFW_DSIGMA_MIS, F_LC_LT, F_SCI_AVERAGE,
+ F_SCI_RZ) = range(26)
 
 
 class Params(C.Structure):

@@ -674,6 +674,16 @@ __global__ __launch_bounds__(256) void k_eval_functor(FunctorArgs F)
     case Y3G_F_EMG_DES_CDF: r = T(emg_des_cdf(q, x[0], x[1], x[2])); break;
     case Y3G_F_SIG_MAX: r = T(sig_max(q, x[0], x[1], x[2])); break;
     case Y3G_F_LC_LT: r = T(lc_lt_tab(x[0], x[1], x[2])); break;
+    case Y3G_F_SCI_AVERAGE: {
+      int const n = int(q[0]);
+      r = T(emg_des_curve(q + 1, q + 1 + n, n, x[0]));
+      break;
+    }
+    case Y3G_F_SCI_RZ: {
+      int const nr = int(q[0]), nz = int(q[1]);
+      r = T(interp2d_clamp(q + 2, nr, q + 2 + nr, nz, q + 2 + nr + nz, x[0], x[1]));
+      break;
+    }
     case Y3G_F_NFW_DSIGMA_MIS:
       r = T(nfw_dsigma_mis_tab(F.nfwmis, F.nfw_nx, F.nfw_nm, x[0], x[1], x[2], q[0], q[1]));
       break;

@@ -1085,8 +1085,34 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
                      const double* in, int nin, size_t n, const double* par,
                      int npar, const y3g_integral* aux, double* out)
 {
-  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_LC_LT)
+  if (!rt || !in || !out || nin < 1 || which < 0 || which > Y3G_F_SCI_RZ)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: bad arguments");
+  if (which == Y3G_F_SCI_AVERAGE) {
+    // par = {n, z[n], sci[n]}, z increasing
+    double const a = par && npar >= 1 ? par[0] : 0.0;
+    if (nin != 1 || !(a >= 2.0 && a <= 65536.0) || a != double(int(a)) ||
+        (long long)npar != 1 + 2LL * (long long)a)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: SCI_AVERAGE takes (zl) and "
+                               "{n, zlense[n], sci_average[n]}");
+    for (int k = 1; k < int(a); ++k)
+      if (!(par[1 + k] > par[k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: SCI_AVERAGE z knots not increasing");
+  }
+  if (which == Y3G_F_SCI_RZ) {
+    // par = {n_r, n_z, r[n_r], z[n_z], table[n_z][n_r]}
+    double const a = par && npar >= 2 ? par[0] : 0.0, b = par && npar >= 2 ? par[1] : 0.0;
+    if (nin != 2 || !(a >= 2.0 && b >= 2.0 && a <= 65536.0 && b <= 65536.0) ||
+        a != double(int(a)) || b != double(int(b)) ||
+        (long long)npar != 2 + (long long)(a + b) + (long long)a * (long long)b)
+      return fail(Y3G_ERR_ARG, "y3g_eval_functor: SCI_RZ takes (r, zt) and "
+                               "{n_r, n_z, r, z, sigma_crit_inv[n_z][n_r]}");
+    for (int k = 1; k < int(a); ++k)
+      if (!(par[2 + k] > par[1 + k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: SCI_RZ r knots not increasing");
+    for (int k = 1; k < int(b); ++k)
+      if (!(par[2 + int(a) + k] > par[1 + int(a) + k]))
+        return fail(Y3G_ERR_ARG, "y3g_eval_functor: SCI_RZ z knots not increasing");
+  }
   if (which == Y3G_F_LC_LT && nin != 3)
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: LC_LT takes (lc, lt, zt)");
   if (which == Y3G_F_NFW_DSIGMA_MIS) {
