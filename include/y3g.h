@@ -253,6 +253,13 @@ int y3g_integrate_noalloc(y3g_runtime* rt, const y3g_integral* igs, size_t n,
  * number of integrals. */
 int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes);
 
+/* Device memory as the reference's MemTrackSentry logs it
+ * (src/utils/mem_tracking_sentry.hh:37-54, cudaMemGetInfo): the device's free
+ * and total bytes, and the bytes this runtime holds (tables, region slab,
+ * batch and plain-path buffers, DES, selection and NFW tables). */
+int y3g_runtime_memory(y3g_runtime* rt, size_t* free_bytes, size_t* total_bytes,
+                       size_t* runtime_bytes);
+
 /* Time of the last y3g_integrate's kernels in milliseconds (HIP events on
  * the library's stream) and the number of integrand evaluations they made. */
 int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
@@ -273,7 +280,10 @@ int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
  * SIG_MAX.  The counts without SIG_MAX may stop after the 11 parameters.
  * (the SIG_MAX tables x-fastest, as Interp2D reads the sample's ndarrays).
  * The point's HMF and D_A tables are its y3g_tables_build / _upload ones.
- * Replaces any earlier upload for the point. */
+ * Replaces any earlier upload for the point.  y3g_tables_build (every point)
+ * and y3g_tables_upload (its point) discard DES data uploaded before them,
+ * which belonged to the previous sample: upload it after the tables, and a
+ * real-data integral whose point has none fails with Y3G_ERR_STATE. */
 int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des);
 
 /* ---- Tabulated miscentred NFW Delta-Sigma -------------------------------

@@ -411,3 +411,73 @@ def test_card_module_and_integrand(runtime, monkeypatch):
                            C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
                                               max_eval=2_000_000_000)), C.byref(oo))
     assert abs(vals[0] - oo.value) <= 2 * (errs[0] + oo.error), (vals[0], errs[0], oo.value)
+
+
+def test_des_data_discarded_by_table_rebuild(runtime):
+    """DES data belongs to one sample: rebuilding the tables (same count and
+    spec) or uploading a point's tables discards it, so a real-data
+    integral can never read the previous sample's data."""
+    from y3_cluster_cpp_amd.workloads import des_synthetic, des_volumes
+    p = y3.default_params()
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, des_synthetic())
+    lows, highs, grid = des_volumes()[0]
+    ig = y3.make_integral_volume(p, spec, y3.DSIGMA6D_DES, lows, highs, grid, eps_rel=1e-2)
+    o = y3.options(precision=y3.FP64, max_regions=256)
+    assert np.isfinite(runtime.integrate([ig], o)["value"][0])
+    runtime.tables_build([p], spec)
+    with pytest.raises(RuntimeError, match="no DES data"):
+        runtime.integrate([ig], o)
+    runtime.des_upload(0, des_synthetic())
+    runtime.integrate([ig], o)
+    hmf, da, sci = runtime.tables_download(0, spec)
+    runtime.tables_upload(0, hmf=hmf)
+    with pytest.raises(RuntimeError, match="no DES data"):
+        runtime.integrate([ig], o)
+
+
+def test_des_cubature_real_table_sizes(runtime):
+    """SIG_MAX tables at the reference's real sizes (1000 radii x 300 masses,
+    data/deltasigma_r_perp.txt, deltasigma_m_h.txt; 10 redshifts): far past
+    the workgroup's LDS copy (2048 doubles), so the kernel reads them from
+    global memory.  The tables hold the same bilinear functions as the small
+    ones (bilinear interpolation reproduces them), so the integral is the
+    small tables' one; the device FP64 cubature follows the oracle's step for
+    step at the same budget."""
+    import ctypes as C
+    from test_oracle_des import BI, DES_BOX, S1, S2, des_data, des_integral
+    # the small tables' ranges (so that both clamp alike), 1000 x 300 x 10 knots
+    r = np.logspace(np.log10(0.1), np.log10(30.0), 1000)
+    m = np.linspace(30.0, 36.0, 300)
+    z = np.linspace(0.1, 1.0, 10)
+
+    def tab(xs, ys, c):
+        X, Y = np.meshgrid(xs, ys)
+        return (c[0] + c[1] * X + c[2] * Y + c[3] * X * Y).reshape(-1)
+    head = des_data()[:11]
+    big = np.ascontiguousarray(np.concatenate(
+        [head, [len(r), len(m), len(z)], r, m, z, tab(r, m, S1), tab(r, z, S2), tab(z, m, BI)]))
+    assert big.size > 300_000
+    p = y3.default_params()
+    spec = y3.spec_wide()
+    runtime.tables_build([p], spec)
+    runtime.des_upload(0, big)
+    lows = [b[0] for b in DES_BOX]
+    highs = [b[1] for b in DES_BOX]
+    ig = y3.make_integral_volume(p, spec, y3.DSIGMA6D_DES, lows, highs, (0.3, 0.45, 1.0),
+                                 eps_rel=1e-9)
+    g = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=2048))[0]
+    op = O.default_params()
+    ot = O.OracleTables(op, O.spec_wide())
+    oig = des_integral(DES_BOX, zo=(0.3, 0.45), radius=1.0)
+    o = O.Result()
+    O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(big), C.byref(oig),
+                           C.byref(O.cub_opts(eps_rel=1e-9, max_regions=2048,
+                                              max_eval=1 << 40)), C.byref(o))
+    assert (g["nregions"], g["neval"]) == (o.nregions, o.neval)
+    assert abs(g["value"] / o.value - 1) < 1e-10, (g["value"], o.value)
+    # the small tables' integral, same budget
+    runtime.des_upload(0, des_data())
+    s = runtime.integrate([ig], y3.options(precision=y3.FP64, max_regions=2048))[0]
+    assert abs(g["value"] / s["value"] - 1) < 1e-9, (g["value"], s["value"])

@@ -238,3 +238,35 @@ def test_module_reads_reference_table_sections(runtime):
     gold = np.array([counts[i, j] for i, j in bins])
     check(sq, gold, 1e-3)
     assert np.all(np.abs(sq["vals"] / synth["vals"] - 1) > 1e-3)
+
+
+def test_memory_track_file(runtime, tmp_path):
+    """memory_track_file (CosmoSISSICUDAModule.hh:214-224): the log is
+    appended to, header first, then MemTrackSentry-format rows
+    (mem_tracking_sentry.hh:37-54) with the device's free bytes and the
+    module runtime's bytes around each execute."""
+    L = "NumberCounts5DIntegrand"
+    path = tmp_path / "mem.csv"
+    path.write_text("earlier content\n")
+    bins = [(0, 0), (1, 1)]
+    cfg = config(L, "pagani", 1e-3, False, count5d_volumes(bins),
+                 extra=[("memory_track_file", str(path))])
+    m = cs.Module(L)
+    assert m.setup(cfg)
+    assert m.execute(sample()) == 0
+    assert m.execute(sample(omega_m=0.31)) == 0
+    assert m.cleanup() == 0
+    lines = path.read_text().splitlines()
+    assert lines[0] == "earlier content"
+    assert lines[1] == ("free_mem, module_mem, module_label, track_label, loc, volume_id, "
+                        "grid_point_id")
+    rows = [l.split(",") for l in lines[2:]]
+    assert [(r[3], r[4]) for r in rows] == [("setup", "end"), ("execute", "start"),
+                                             ("execute", "end"), ("execute", "start"),
+                                             ("execute", "end")]
+    _, total, _ = runtime.memory()
+    for r in rows:
+        assert len(r) == 7 and r[2] == L and r[5] == "-1" and r[6] == "-1"
+        assert 0 < int(r[0]) <= total
+    # after an execute the runtime holds its tables and region slab
+    assert int(rows[2][1]) > 0 and int(rows[2][1]) >= int(rows[1][1])

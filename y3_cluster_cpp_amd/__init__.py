@@ -137,6 +137,8 @@ SIGNATURES = {
     "y3g_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "y3g_server_destroy": (C.c_int, [C.c_void_p]),
     "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
+    "y3g_runtime_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                     C.POINTER(C.c_size_t)]),
     "y3g_selfn_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD,
                                    C.c_int]),
     "y3g_selgl_counts": (C.c_int, [C.c_void_p, C.POINTER(SelGLSpec), C.c_int, C.c_int, _PD]),
@@ -383,6 +385,13 @@ class Runtime:
         _check(lib().y3g_selgl_counts(self._h, C.byref(spec), n_points, max_bins,
                                       out.ctypes.data_as(_PD)), "y3g_selgl_counts")
         return out
+
+    def memory(self):
+        """(free, total, runtime) device bytes (y3g_runtime_memory)."""
+        f, t, r = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _check(lib().y3g_runtime_memory(self._h, C.byref(f), C.byref(t), C.byref(r)),
+               "y3g_runtime_memory")
+        return f.value, t.value, r.value
 
     def last_timing(self):
         ms = C.c_double()

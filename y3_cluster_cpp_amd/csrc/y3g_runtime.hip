@@ -654,6 +654,9 @@ int y3g_tables_build(y3g_runtime* rt, const y3g_params* params,
   A.sci64 = rt->sci64, A.sci32 = rt->sci32, A.sci_stride = rt->sci_stride;
   HIPCHK(launch_tables(A, n_points, rt->stream));
   HIPCHK(hipStreamSynchronize(rt->stream));
+  // the points are new samples: DES data uploaded for the old ones is stale
+  rt->des.clear();
+  rt->des_dirty = true;
   return 0;
 }
 
@@ -675,6 +678,11 @@ int y3g_tables_upload(y3g_runtime* rt, int point, const double* hmf,
   if (int rc = up(hmf, rt->hmf64, rt->hmf32, rt->hmf_stride)) return rc;
   if (int rc = up(da, rt->da64, rt->da32, rt->da_stride)) return rc;
   if (int rc = up(sci, rt->sci64, rt->sci32, rt->sci_stride)) return rc;
+  // a new sample's tables: the point's earlier DES data is stale
+  if ((size_t)point < rt->des.size() && !rt->des[point].empty()) {
+    rt->des[point].clear();
+    rt->des_dirty = true;
+  }
   return 0;
 }
 
@@ -1070,6 +1078,29 @@ int y3g_runtime_footprint(y3g_runtime* rt, const y3g_options* opt, size_t* bytes
   size_t const per_region = 2 * kMaxDim * sizeof(double) + 3 * sizeof(double) + 1;
   size_t const slots = (size_t)resident_slots(rt, o, Y3G_PAGANI);
   *bytes = slots * (size_t)std::max(o.max_regions, 1) * per_region;
+  return 0;
+}
+
+int y3g_runtime_memory(y3g_runtime* rt, size_t* free_bytes, size_t* total_bytes,
+                       size_t* runtime_bytes)
+{
+  if (!rt) return fail(Y3G_ERR_ARG, "y3g_runtime_memory: null runtime");
+  HIPCHK(hipSetDevice(rt->device));
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = fr;
+  if (total_bytes) *total_bytes = tot;
+  if (runtime_bytes) {
+    size_t const np = (size_t)std::max(rt->n_points, 0);
+    size_t b = np * (sizeof(PointDev) + (size_t)rt->hmf_stride * 12 +
+                     (size_t)rt->da_stride * 12 + (size_t)rt->sci_stride * 12);
+    b += (size_t)rt->slab_slots * rt->slab_cap * (2 * kMaxDim * 8 + 3 * 8 + 1);
+    b += rt->batch_cap * (sizeof(y3g_integral) + sizeof(y3g_result) + sizeof(int));
+    b += (size_t)rt->sg_cap * (2 * kMaxDim * 8 + 6 * 8);
+    b += (rt->des_cap + rt->seldata_cap + rt->selnodes_cap + rt->selout_cap) * 8;
+    b += (size_t)rt->nfw_nx * rt->nfw_nm * 8;
+    *runtime_bytes = b;
+  }
   return 0;
 }
 

@@ -312,6 +312,7 @@ namespace y3g_cosmosis {
     double eps_rel = 0.0, eps_abs = 0.0, max_eval = 0.0;
     bool cartesian = false;
     bool do_timing = false;
+    std::string memory_track_file;  // empty: no memory log
     std::vector<volume_t> volumes;
     typename I::grid_t grid;
     y3g_options options{};
@@ -365,6 +366,8 @@ namespace y3g_cosmosis {
         "number of gridpoints.\n");
     bool timing = false;
     if (cfg.get_val(L, "do_timing", timing) == DBS_SUCCESS) m.do_timing = timing;
+    std::string mem;
+    if (cfg.get_val(L, "memory_track_file", mem) == DBS_SUCCESS) m.memory_track_file = mem;
 
     // device-side settings (defaults from the library)
     y3g_options_default(&m.options);
@@ -551,6 +554,15 @@ namespace y3g_cosmosis {
         timing_.emplace(fn.str());
         record_timestamp(*timing_, "constructed");
       }
+      // memory log (CosmoSISSICUDAModule.hh:214-224): appended to, header
+      // first; rows in MemTrackSentry's format (mem_tracking_sentry.hh:37-54)
+      if (!cfg_.memory_track_file.empty()) {
+        mem_track_.emplace(cfg_.memory_track_file, std::ios_base::app);
+        *mem_track_ << "free_mem, module_mem, module_label, track_label, loc, "
+                       "volume_id, grid_point_id"
+                    << std::endl;
+        track_memory("setup", "end");
+      }
     }
 
     ~HIPIntegrationModule()
@@ -562,6 +574,7 @@ namespace y3g_cosmosis {
     execute(cosmosis::DataBlock& sample)
     {
       if (timing_) record_timestamp(*timing_, "enter");
+      track_memory("execute", "start");
       integrand_.set_sample(sample);
       tables_.ensure(device_.get(), integrand_);
       if constexpr (has_device_upload<I>::value) integrand_.upload(device_.get());
@@ -572,6 +585,7 @@ namespace y3g_cosmosis {
                         res.data()) != Y3G_OK)
         throw std::runtime_error(std::string("integrate: ") + y3g_last_error());
       put_results(sample, cfg_, res);
+      track_memory("execute", "end");
       if (timing_) record_timestamp(*timing_, "exit");
     }
 
@@ -584,6 +598,21 @@ namespace y3g_cosmosis {
     I integrand_;
     TableCache tables_;
     std::optional<std::ofstream> timing_;
+    std::optional<std::ofstream> mem_track_;
+
+    // One row of the memory log: the device's free bytes, the bytes this
+    // module's runtime holds, labels, and volume / grid point ids (-1: the
+    // whole submission).
+    void
+    track_memory(char const* label, char const* loc)
+    {
+      if (!mem_track_) return;
+      size_t fr = 0, tot = 0, mine = 0;
+      if (y3g_runtime_memory(device_.get(), &fr, &tot, &mine) != Y3G_OK)
+        std::cerr << "y3g_runtime_memory: " << y3g_last_error() << '\n';
+      *mem_track_ << fr << "," << mine << "," << I::module_label() << "," << label << ","
+                  << loc << "," << -1 << "," << -1 << std::endl;
+    }
   };
 
   // ------------------------------------------------ the integrands' common part
