@@ -902,7 +902,7 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
  * variable is integrated in its standardised form
  *   t = (ln lt - ln lbar(lnM, zt)) / sigma_ln,  lt = lbar e^{sigma_ln t},
  *   MOR(lt) dlt = phi(t) dt,
- * with t running over its window [max(-7, t(lt_lo)), min(7, t(lt_hi))]
+ * with t running over its window [t(lt_lo), t(lt_hi)] clipped smoothly to [-7, 7]
  * through v in [0, 1] (dt = (t_hi - t_lo) dv), so the integral is the same
  * one over lt in [1, 300] (the Gaussian mass beyond |t| = 7 is 2.6e-12).  For the
  * Delta-Sigma integrand the offset radius is integrated in its Rayleigh CDF
@@ -910,6 +910,7 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
  * (the integrand is even about theta = pi).  Axis order follows the
  * physical integrands with lt replaced by t, rmis by u. */
 #define Y3O_T_MAX 7.0
+#define Y3O_T_SOFT 4.0
 
 double
 y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
@@ -938,9 +939,14 @@ y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
   double const s = p->mor_sigma_ln;
   double const lnlbar = p->mor_lnl0 + p->mor_slope * (lnM - Y3O_LN_MPIVOT_MOR) +
                         p->mor_zslope * log((1.0 + zt) / 1.35);
-  /* t window: [-T, T] clipped to the lambda_true range */
-  double const tlo = fmax(-Y3O_T_MAX, (log(ig->lt_lo) - lnlbar) / s);
-  double const thi = fmin(Y3O_T_MAX, (log(ig->lt_hi) - lnlbar) / s);
+  /* t window: [t(lt_lo), t(lt_hi)] clipped to [-T, T] by a smoothed clip
+   * (softplus of sharpness Y3O_T_SOFT): within ln 2 / k of the hard clip and
+   * never wider, so < 1e-11 of the Gaussian mass is dropped, and free of the
+   * kink a hard max / min puts along a0(lnM, zt) = -T, b0(lnM, zt) = T */
+  double const a0 = (log(ig->lt_lo) - lnlbar) / s, b0 = (log(ig->lt_hi) - lnlbar) / s;
+  double const ua = Y3O_T_SOFT * (-Y3O_T_MAX - a0), ub = Y3O_T_SOFT * (b0 - Y3O_T_MAX);
+  double const tlo = a0 + (ua > 30.0 ? ua : log1p(exp(ua))) * (1.0 / Y3O_T_SOFT);
+  double const thi = b0 - (ub > 30.0 ? ub : log1p(exp(ub))) * (1.0 / Y3O_T_SOFT);
   if (!(thi > tlo) || jac == 0.0) return 0.0;
   double const tt = tlo + (thi - tlo) * v;
   double const lt = exp(lnlbar + s * tt);

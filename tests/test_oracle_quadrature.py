@@ -198,8 +198,16 @@ def test_mapped_integrand_equals_physical():
         for _ in range(20):
             y = np.array([rng.uniform(ig.lows[k], ig.highs[k]) for k in range(ig.ndim)])
             lnlbar = p.mor_slope * (y[im] - L3) + p.mor_lnl0 + p.mor_zslope * math.log((1 + y[iz]) / 1.35)
-            tlo = max(-7.0, (math.log(1.0) - lnlbar) / p.mor_sigma_ln)
-            thi = min(7.0, (math.log(300.0) - lnlbar) / p.mor_sigma_ln)
+            a0 = (math.log(1.0) - lnlbar) / p.mor_sigma_ln
+            b0 = (math.log(300.0) - lnlbar) / p.mor_sigma_ln
+            # the window's smoothed clip: softplus of sharpness 4, never
+            # wider than the hard clip [max(-7, a0), min(7, b0)] and within
+            # ln 2 / 4 of it
+            sp = lambda u: u if u > 30 else math.log1p(math.exp(u))
+            tlo = a0 + sp(4 * (-7.0 - a0)) / 4
+            thi = b0 - sp(4 * (b0 - 7.0)) / 4
+            assert max(-7.0, a0) <= tlo <= max(-7.0, a0) + math.log(2) / 4 + 1e-12
+            assert min(7.0, b0) - math.log(2) / 4 - 1e-12 <= thi <= min(7.0, b0)
             tt = tlo + (thi - tlo) * y[iv]
             lt = math.exp(lnlbar + p.mor_sigma_ln * tt)
             x = y.copy()

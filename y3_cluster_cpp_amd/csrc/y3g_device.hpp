@@ -39,6 +39,7 @@ constexpr double kDh = 2997.92458;                  // c / H0 [Mpc/h]
 constexpr double kSciConst = 4.0 * 3.14159265358979323846 * 4.51710305e-48 /
                              (9.71561189e-15 * 9.71561189e-15) * 1e12;
 constexpr double kTMax = 7.0;
+constexpr double kTSoft = 4.0;  // sharpness of the t window's smoothed clip
 
 // Per-parameter-point constants (host-prepared).
 struct PointDev {
@@ -488,7 +489,7 @@ Y3G_HD T dsigma_offset(T R, T rmis, T theta, const NfwHalo<T>& H)
 // Mapped integrand: y are the integration variables of the box.  The MOR
 // variable is integrated in its standardised form
 //   t = (ln lt - ln lbar(lnM, zt)) / sigma_ln,  MOR(lt) dlt = phi(t) dt,
-// with t running over its window [max(-7, t(lt_lo)), min(7, t(lt_hi))]
+// with t running over its window [t(lt_lo), t(lt_hi)] clipped smoothly to [-7, 7]
 // through v in [0, 1]; for Delta-Sigma the offset radius is integrated in
 // its Rayleigh CDF u (P(rmis) drmis = du) and the angle on [0, pi] (the
 // integrand is even about pi).
@@ -521,6 +522,21 @@ template <> struct KindAxes<Y3G_COUNT4D_CARD> {
 };
 constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_COUNT4D_CARD; }
 
+// The MOR variable's window: [a0, b0] = [t(lt_lo), t(lt_hi)] clipped to
+// [-T, T] by a smoothed clip, tlo = a0 + softplus(k (-T - a0)) / k and
+// thi = b0 - softplus(k (b0 - T)) / k (k = kTSoft).  tlo lies within
+// ln 2 / k of max(a0, -T) and never below it (likewise thi), so the window
+// drops < 1e-11 of the Gaussian mass, while a hard max / min would give the
+// integrand a kink along the curves a0(lnM, zt) = -T and b0(lnM, zt) = T.
+template <typename T> Y3G_HD void t_window(T a0, T b0, T& tlo, T& thi)
+{
+  T const ua = T(kTSoft) * (T(-kTMax) - a0), ub = T(kTSoft) * (b0 - T(kTMax));
+  T const sa = ua > T(30) ? ua : M<T>::log1p(M<T>::exp(ua));
+  T const sb = ub > T(30) ? ub : M<T>::log1p(M<T>::exp(ub));
+  tlo = a0 + sa * T(1.0 / kTSoft);
+  thi = b0 - sb * T(1.0 / kTSoft);
+}
+
 // Integrand of kind K (axis positions known at compile time).
 template <typename T, int K>
 Y3G_HD T integrand_k(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
@@ -533,10 +549,8 @@ Y3G_HD T integrand_k(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
   T const lnlbar = T(P.p.mor_lnl0) + T(P.p.mor_slope) * T(lnM - kLnPivotMor) +
                    T(P.p.mor_zslope) * M<T>::log(zp * T(1.0 / 1.35));
   T const isl = M<T>::div(T(1), sln);
-  T tlo = (T(ig.lnlt_lo) - lnlbar) * isl;
-  T thi = (T(ig.lnlt_hi) - lnlbar) * isl;
-  tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
-  thi = thi < T(kTMax) ? thi : T(kTMax);
+  T tlo, thi;
+  t_window<T>((T(ig.lnlt_lo) - lnlbar) * isl, (T(ig.lnlt_hi) - lnlbar) * isl, tlo, thi);
   if (!(thi > tlo)) return T(0);
   T const tt = tlo + (thi - tlo) * T(y[it]);
   T const lt = M<T>::exp(lnlbar + sln * tt);
@@ -678,10 +692,8 @@ Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   PairF<T> f;
   f.lnlbar = m.lm + z.lz;
   T const isl = M<T>::div(T(1), T(P.p.mor_sigma_ln));
-  T tlo = (T(ig.lnlt_lo) - f.lnlbar) * isl;
-  T thi = (T(ig.lnlt_hi) - f.lnlbar) * isl;
-  tlo = tlo > T(-kTMax) ? tlo : T(-kTMax);
-  thi = thi < T(kTMax) ? thi : T(kTMax);
+  T tlo, thi;
+  t_window<T>((T(ig.lnlt_lo) - f.lnlbar) * isl, (T(ig.lnlt_hi) - f.lnlbar) * isl, tlo, thi);
   f.tlo = tlo;
   f.dt = thi - tlo;
   // HMF: clamped bilinear lookup (hmf_t.cuh:54-60), as hmf() below; the
