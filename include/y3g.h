@@ -72,8 +72,15 @@ typedef enum {
   Y3G_DSIGMA3D_DES = 4, /* (lo, zt, lnM)                   SigmaCentY1Mort…    */
   Y3G_COUNT3D_DES = 5,  /* (lo, zt, lnM)                   NCCentY1Mort…       */
   Y3G_COUNT5D_DES = 6,  /* (lo, lc, zt, lnM, rmis)         NCMiscentY1Mort…    */
-  Y3G_COUNT4D_CARD = 7  /* (ln lo, ln lt, zt, lnM)         numberCountsCard:
+  Y3G_COUNT4D_CARD = 7, /* (ln lo, ln lt, zt, lnM)         numberCountsCard:
                            MOR_DES_LOG_t and INT_LC_LT_DES_t2 tables       */
+  /* The Costanzi-2026 P[X] operator integrands P_operator_gpu<W>
+   * (p_operator_gpu_t.cuh:118-299), (lt, zt, lnM) with an inner fixed
+   * Gauss-Legendre angle integral, grid (zo_low, zo_high, lambda_bin): */
+  Y3G_POP_P1 = 8,       /* F = 1                              bSelMargP1GPU  */
+  Y3G_POP_I1 = 9,       /* F = b(M, z) xi_NL sigmoid(theta)   bSelMargI1GPU  */
+  Y3G_POP_I2 = 10,      /* F = b(M, z) xi_NL                  bSelMargI2GPU  */
+  Y3G_POP_J = 11        /* F = b(M, z) xi_NL (1 - sigmoid)    bSelMargJGPU   */
 } y3g_kind;
 
 /* Integrators: the names of gpu_integrator.cuh:54-65. */
@@ -220,6 +227,11 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
  * rmis in Mpc/h (the offset pdf is integrated over [rmis_low, rmis_high]),
  * theta in radians inside [0, 2 pi] (with the 1/2pi of the angular average,
  * src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_scalarintegrand.cu:174-175).
+ * The real-data kinds keep the volume as it is, grid (zo_low, zo_high
+ * [, radius]); the P[X] operator kinds Y3G_POP_* take (lt, zt, lnm) and grid
+ * (zo_low, zo_high, lambda_bin), lambda_bin in 0..3 (the DES bins [20, 30),
+ * [30, 45), [45, 60), [60, 200] of p_operator_gpu_t.cuh:55-60), and keep the
+ * bin in lo_lo and its centre, the integrand's upper lt limit, in lo_hi.
  * Host only; no device is needed. */
 int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
                              int kind, const double* lows, const double* highs,
@@ -278,6 +290,15 @@ int y3g_last_timing(y3g_runtime* rt, double* kernel_ms, long long* nevals);
  * mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot} in the MOR_DES places and
  * INT_LC_LT_DES_t2's tables packed as for Y3G_F_INT_LC_LT_DES in place of
  * SIG_MAX.  The counts without SIG_MAX may stop after the 11 parameters.
+ * For the P[X] operator kinds (Y3G_POP_*), P_operator_gpu::set_sample's
+ * inputs (p_operator_gpu_t.cuh:150-163):
+ *   des = {h0, 0, 0, 0, 0,                                cosmological_parameters/h0
+ *          log10_Mmin, log10_M1, alpha, sigma_intr, epsilon, z_pivot,
+ *                                                         MOR_SHIFTED_POISSON_t
+ *          n_chi, n_bm, n_bz, n_xr, n_xz,
+ *          chi_z[n_chi], chi_dc[n_chi],                   distances/{z, d_c}
+ *          bias_lnM[n_bm], bias_z[n_bz], bias[n_bz][n_bm], haloModel/{lnM, z, bias}
+ *          xi_r[n_xr], xi_z[n_xz], xi[n_xz][n_xr]}        xi_nl/{r, z, xi_nl}
  * (the SIG_MAX tables x-fastest, as Interp2D reads the sample's ndarrays).
  * The point's HMF and D_A tables are its y3g_tables_build / _upload ones.
  * Replaces any earlier upload for the point.  y3g_tables_build (every point)

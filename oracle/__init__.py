@@ -108,6 +108,14 @@ def _load():
         "y3o_dsigma_mis_gamma": (_D, [_D] * 6),
         "y3o_dsigma_mis_rayleigh": (_D, [_D] * 7),
         "y3o_lc_lt": (_D, [_D, _D, _D]),
+        "y3o_sigma_photoz_table": (_D, [_D]),
+        "y3o_mor_shifted_poisson": (_D, [_PD, _D, _D, _D]),
+        "y3o_pop_area_overlap": (_D, [_D, _D, _D]),
+        "y3o_pop_theta_excl": (_D, [_D, _D, _D]),
+        "y3o_pop_theta_term": (_D, [C.c_int, _D, _D, _D, _D, _D, _D, _PD, C.c_int, _PD,
+                                    C.c_int, _PD]),
+        "y3o_integrand_pop": (_D, [C.POINTER(Params), C.POINTER(Tables), _PD,
+                                   C.POINTER(Integral), _PD]),
         "y3o_lc_lt_params": (None, [_D, _D, _PD]),
         "y3o_nfw_ring_levels": (C.c_int, [_D, _D]),
         "y3o_nfw_ring": (_D, [_D, _D]),

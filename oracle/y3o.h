@@ -255,6 +255,18 @@ double y3o_selgl_counts(const y3o_params* p, const y3o_tables* t,
 /* Test integrands of the reference's integrator known-answer tests. */
 double y3o_test_fn(int which, const double* x, int ndim);
 
+/* The Costanzi-2026 P[X] operator integrands (y3o_pop.c); kinds 8..11 of
+ * y3o_integrand_des / y3o_cubature_des. */
+double y3o_sigma_photoz_table(double z);
+double y3o_mor_shifted_poisson(const double* q, double ltr, double lnM, double z);
+double y3o_pop_area_overlap(double th, double lob, double lt);
+double y3o_pop_theta_excl(double chi_z, double chi_o, double R_excl);
+double y3o_pop_theta_term(int kind, double th, double theta_lob, double theta_lt,
+                          double chi_z, double chi_o, double zob, const double* xr,
+                          int nxr, const double* xz, int nxz, const double* xt);
+double y3o_integrand_pop(const y3o_params* p, const y3o_tables* t, const double* des,
+                         const y3o_integral* ig, const double* x);
+
 /* LC_LT_t over the reference's fitted tables (y3o_models.c). */
 double y3o_lc_lt(double lc, double lt, double zt);
 void   y3o_lc_lt_params(double lt, double zt, double* out5);

@@ -1068,6 +1068,11 @@ y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
       return y3o_int_lc_lt_des(des + 11, lo, lt, zt) *
              y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) * common;
     }
+    case 8:
+    case 9:
+    case 10:
+    case 11: /* the P[X] operator integrands (y3o_pop.c) */
+      return y3o_integrand_pop(p, t, des, ig, x);
     case 6: {
       double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4];
       double const common = y3o_roffset_gamma(rmis, des[0]) *

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liby3g.so")
 
 (COUNT3D, COUNT5D, DSIGMA6D, DSIGMA6D_DES, DSIGMA3D_DES, COUNT3D_DES,
- COUNT5D_DES, COUNT4D_CARD) = range(8)
+ COUNT5D_DES, COUNT4D_CARD, POP_P1, POP_I1, POP_I2, POP_J) = range(12)
 PAGANI, MCUBES = 0, 1
 FP32SPLIT, FP64 = 0, 1
 

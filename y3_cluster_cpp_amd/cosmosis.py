@@ -19,7 +19,8 @@ DATABLOCK_LIB = os.path.join(MODULE_DIR, "libdatablock.so")
 MODULES = ("NumberCounts3DIntegrand", "NumberCounts5DIntegrand", "SigmaMiscent6DIntegrand",
            "NumCountsSel", "SigmaMiscentY1MortCUDAScalarIntegrand",
            "SigmaCentY1MortCUDAScalarIntegrand", "NCCentY1MortCUDAScalarIntegrand",
-           "NCMiscentY1MortCUDAScalarIntegrand", "numberCountsCard")
+           "NCMiscentY1MortCUDAScalarIntegrand", "numberCountsCard", "bSelMargP1GPU",
+           "bSelMargI1GPU", "bSelMargI2GPU", "bSelMargJGPU")
 
 DBS_SUCCESS = 0
 

@@ -682,6 +682,10 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
       case Y3G_COUNT3D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D_DES>(S, m, tb); break;
       case Y3G_COUNT5D_DES: eval_nodes_direct<T, BLOCK, Y3G_COUNT5D_DES>(S, m, tb); break;
       case Y3G_COUNT4D_CARD: eval_nodes_direct<T, BLOCK, Y3G_COUNT4D_CARD>(S, m, tb); break;
+      case Y3G_POP_P1: eval_nodes_direct<T, BLOCK, Y3G_POP_P1>(S, m, tb); break;
+      case Y3G_POP_I1: eval_nodes_direct<T, BLOCK, Y3G_POP_I1>(S, m, tb); break;
+      case Y3G_POP_I2: eval_nodes_direct<T, BLOCK, Y3G_POP_I2>(S, m, tb); break;
+      case Y3G_POP_J: eval_nodes_direct<T, BLOCK, Y3G_POP_J>(S, m, tb); break;
       default: eval_nodes_direct<T, BLOCK, Y3G_DSIGMA6D>(S, m, tb); break;
     }
   }

@@ -520,7 +520,27 @@ template <> struct KindAxes<Y3G_COUNT5D_DES> {
 template <> struct KindAxes<Y3G_COUNT4D_CARD> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 4;
 };
-constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_COUNT4D_CARD; }
+template <> struct KindAxes<Y3G_POP_P1> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_POP_I1> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_POP_I2> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+template <> struct KindAxes<Y3G_POP_J> {
+  static constexpr int it = 0, iz = 1, im = 2, nd = 3;
+};
+// The FP64-only kinds that read per-point data (y3g_des_upload): the
+// real-data DES integrands and the P[X] operator integrands.
+constexpr bool is_des_kind(int k) { return k >= Y3G_DSIGMA6D_DES && k <= Y3G_POP_J; }
+constexpr bool is_pop_kind(int k) { return k >= Y3G_POP_P1 && k <= Y3G_POP_J; }
+// DES Y3 richness bin centres of the P[X] operator (p_operator_gpu_t.cuh:55-60)
+__host__ __device__ inline double pop_lob_center(int bin)
+{
+  return bin == 1 ? 37.5 : bin == 2 ? 52.5 : bin == 3 ? 130.0 : 25.0;
+}
 
 // The MOR variable's window: [a0, b0] = [t(lt_lo), t(lt_hi)] clipped to
 // [-T, T] by a smoothed clip, tlo = a0 + softplus(k (-T - a0)) / k and
@@ -1334,6 +1354,152 @@ constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 //   COUNT4D_CARD  (ln lo, ln lt, zt, lnM): int_lc_lt(lo, lt, zt) int_zo_zt
 //     Omega dV/dz hmf mor_des_log(lt, lnM, zt), lo = e^(ln lo), lt = e^(ln lt)
 //     (gt_card_gpu/numberCounts.cu:160-174).
+// ------------------------------------------ Costanzi-2026 P[X] operator
+// P_operator_gpu<W>::operator()(lt, zt, lnM) (p_operator_gpu_t.cuh:172-255)
+// and its pieces, FP64.  The point's data (y3g_des_upload) from kDesSig on:
+//   {n_chi, n_bm, n_bz, n_xr, n_xz, chi_z, chi_dc, bias_lnM, bias_z,
+//    bias[n_bz][n_bm], xi_r, xi_z, xi[n_xz][n_xr]},
+// h0 in des[0] and MOR_SHIFTED_POISSON_t's {log10_Mmin, log10_M1, alpha,
+// sigma_intr, epsilon, z_pivot} at kDesMor.
+#define Y3G_PHOTOZ_TABLE(name, n) static __constant__ double name[n]
+#include "y3g_photoz_table.inc"
+#undef Y3G_PHOTOZ_TABLE
+
+// SIGMA_PHOTOZ_TABLE_t (sigma_photoz_table_t.cuh:15-75): linear in a
+// 120-point uniform table over z in [0.1, 0.9], clamped.
+__device__ inline double sigma_photoz_table(double z)
+{
+  if (z <= 0.1) return kPhotozSigma[0];
+  if (z >= 0.9) return kPhotozSigma[119];
+  double const dz = (0.9 - 0.1) / 119;
+  double const u = (z - 0.1) / dz;
+  int const i = int(u);
+  int const lo = i < 0 ? 0 : (i >= 119 ? 118 : i);
+  double const f = u - lo;
+  return kPhotozSigma[lo] + f * (kPhotozSigma[lo + 1] - kPhotozSigma[lo]);
+}
+
+// MOR_SHIFTED_POISSON_t::operator()(ltr, lnM, z)
+// (mor_shifted_poisson_t.cuh:107-138): l_sat = ((M - Mmin) / (M1 - Mmin))^alpha
+// ((1 + z) / (1 + z_p))^eps, mi = (l_sat sigma)^2, lam = l_sat + mi,
+// pdf = exp(-lam + (ltr + mi - 1) ln lam - lgamma(ltr + mi)); 0 below Mmin.
+__host__ __device__ inline double mor_shifted_poisson(const double* q, double ltr, double lnM,
+                                                      double z)
+{
+  double const Mmin = pow(10.0, q[0]), M1 = pow(10.0, q[1]);
+  double const dM = exp(lnM) - Mmin, dM1 = M1 - Mmin;
+  if (dM <= 0.0 || dM1 <= 0.0) return 0.0;
+  double const l_sat = pow(dM / dM1, q[2]) * pow((1.0 + z) / (1.0 + q[5]), q[4]);
+  double const mi = (l_sat * q[3]) * (l_sat * q[3]);
+  double const lam = fmax(l_sat + mi, 1.0e-300);
+  double const x = ltr + mi;
+  if (ltr < 0.0 || x <= 0.0) return 0.0;
+  return exp(-lam + (x - 1.0) * log(lam) - lgamma(x));
+}
+
+// Two-disk overlap fraction f_A(theta; theta_lob, theta_lt)
+// (p_operator_gpu_t.cuh:63-92): the area of the disk of radius theta_lt,
+// centred theta away, that lies inside the disk of radius theta_lob, over
+// the first disk's area.
+__host__ __device__ inline double pop_area_overlap(double th, double lob, double lt)
+{
+  if (th > lob + lt) return 0.0;
+  double const diff = fabs(lob - lt);
+  if (th <= diff) {
+    if (lt <= lob) return 1.0;
+    double const r = lob / lt;
+    return r * r;
+  }
+  double c1 = (th * th + lt * lt - lob * lob) / (2.0 * th * lt);
+  double c2 = (th * th + lob * lob - lt * lt) / (2.0 * th * lob);
+  c1 = fmax(-1.0, fmin(1.0, c1));
+  c2 = fmax(-1.0, fmin(1.0, c2));
+  double const sq = fmax((-th + lt + lob) * (th + lt - lob) * (th - lt + lob) * (th + lt + lob),
+                         0.0);
+  double const A = lt * lt * acos(c1) + lob * lob * acos(c2) - 0.5 * sqrt(sq);
+  return A / (3.14159265358979323846 * lt * lt);
+}
+
+// Exclusion angle at z (p_operator_gpu_t.cuh:95-104): the law of cosines
+// for R_excl between comoving distances chi_z and chi_o.
+__host__ __device__ inline double pop_theta_excl(double chi_z, double chi_o, double R_excl)
+{
+  double const den = 2.0 * chi_z * chi_o;
+  if (den < 1e-30) return 0.0;
+  double c = (chi_z * chi_z + chi_o * chi_o - R_excl * R_excl) / den;
+  c = fmax(-1.0, fmin(1.0, c));
+  if (c >= 1.0 - 1e-12) return 1e-6;
+  return acos(c);
+}
+
+template <int K>
+__device__ inline double integrand_pop_k(const y3g_integral& ig, const PointDev& P,
+                                         Tabs<double> tb, const double* des, const double* y)
+{
+  double const lt = y[0], zt = y[1], lnM = y[2];
+  double const lobc = ig.lo_hi;  // the bin's centre, the upper lt limit
+  if (lt <= 0.0 || lt > lobc) return 0.0;
+  double const zob = 0.5 * (ig.zo_lo + ig.zo_hi);
+  double const sz = sigma_photoz_table(zt);
+  double const u = (zt - zob) / sz;
+  if (fabs(u) >= 1.0) return 0.0;
+  double const wz = 1.0 - u * u;
+  double const common = hmf<double>(P, tb.hmf, lnM, zt) * dv_do_dz<double>(P, tb.da, zt) *
+                        mor_shifted_poisson(des + kDesMor, lt, lnM, zt);
+  if (common <= 0.0) return 0.0;
+  const double* q = des + kDesSig;
+  int const nc = int(q[0]), nbm = int(q[1]), nbz = int(q[2]), nxr = int(q[3]), nxz = int(q[4]);
+  const double* chz = q + 5;
+  const double* chd = chz + nc;
+  const double* bm = chd + nc;
+  const double* bz = bm + nbm;
+  const double* bt = bz + nbz;
+  const double* xr = bt + nbm * nbz;
+  const double* xz = xr + nxr;
+  const double* xt = xz + nxz;
+  double const h0 = des[0];
+  double const lob = lobc;
+  double const chi_o = emg_des_curve(chz, chd, nc, zob) * h0;
+  double const chi_z = emg_des_curve(chz, chd, nc, zt) * h0;
+  double const rl_lob = pow(lob / 100.0, 0.2);
+  double const theta_lob = rl_lob * (1.0 + zob) / chi_o;
+  double const R_excl = rl_lob * (1.0 + zob);
+  double const th_excl = pop_theta_excl(chi_z, chi_o, R_excl);
+  double const th_hi = 2.0 * theta_lob;
+  if (th_excl >= th_hi) return 0.0;
+  double const theta_lt = pow(lt / 100.0, 0.2) * (1.0 + zt) / chi_z;
+  double const k_sig = 2.5 / theta_lob;
+  double const th0 = 0.5 * theta_lob;
+  constexpr double gx[5] = {-0.9061798459386640, -0.5384693101056831, 0.0,
+                            0.5384693101056831, 0.9061798459386640};
+  constexpr double gw[5] = {0.2369268850561891, 0.4786286704993665, 0.5688888888888889,
+                            0.4786286704993665, 0.2369268850561891};
+  double const th_mid = fmax(th_excl, fmin(theta_lob, th_hi));
+  double th_sum = 0.0;
+  for (int panel = 0; panel < 2; ++panel) {
+    double const a = panel == 0 ? th_excl : th_mid, b = panel == 0 ? th_mid : th_hi;
+    if (!(b > a + 1e-10)) continue;
+    double const mid = 0.5 * (a + b), hl = 0.5 * (b - a);
+    for (int it = 0; it < 5; ++it) {
+      double const th = mid + hl * gx[it];
+      double const fA = pop_area_overlap(th, theta_lob, theta_lt);
+      if (fA <= 0.0) continue;
+      double const wth = hl * gw[it] * 2.0 * 3.14159265358979323846 * sin(th);
+      double F = 1.0;
+      if constexpr (K != Y3G_POP_P1) {
+        double const d3 =
+          sqrt(fmax(chi_z * chi_z + chi_o * chi_o - 2.0 * chi_z * chi_o * cos(th), 0.0));
+        double const xi = interp2d_clamp(xr, nxr, xz, nxz, xt, d3, zob);
+        double const sg = 1.0 / (1.0 + exp(-k_sig * (th - th0)));
+        F = K == Y3G_POP_I1 ? xi * sg : K == Y3G_POP_I2 ? xi : xi * (1.0 - sg);
+      }
+      th_sum += wth * fA * F;
+    }
+  }
+  double const b_mz = K == Y3G_POP_P1 ? 1.0 : interp2d_clamp(bm, nbm, bz, nbz, bt, lnM, zt);
+  return wz * lt * common * b_mz * th_sum;
+}
+
 // Not inlined: inlined into the cubature's node loop it raised the loop's
 // spills (237 VGPRs spilled against 161) and ran at half the speed.
 template <int K>
@@ -1341,6 +1507,7 @@ __device__ __noinline__ double integrand_des_k(const y3g_integral& ig, const Poi
                                          Tabs<double> tb, const double* des, const double* y,
                                          MorDesTabs const& mt = mor_des_const_tabs())
 {
+  if constexpr (is_pop_kind(K)) return integrand_pop_k<K>(ig, P, tb, des, y);
   constexpr int iz = KindAxes<K>::iz, im = KindAxes<K>::im;
   if constexpr (K == Y3G_COUNT4D_CARD) {
     double const zt = y[iz], lnM = y[im];
@@ -1383,6 +1550,10 @@ __device__ inline double integrand_des(const y3g_integral& ig, const PointDev& P
     case Y3G_DSIGMA3D_DES: return integrand_des_k<Y3G_DSIGMA3D_DES>(ig, P, tb, des, y);
     case Y3G_COUNT3D_DES: return integrand_des_k<Y3G_COUNT3D_DES>(ig, P, tb, des, y);
     case Y3G_COUNT4D_CARD: return integrand_des_k<Y3G_COUNT4D_CARD>(ig, P, tb, des, y);
+    case Y3G_POP_P1: return integrand_des_k<Y3G_POP_P1>(ig, P, tb, des, y);
+    case Y3G_POP_I1: return integrand_des_k<Y3G_POP_I1>(ig, P, tb, des, y);
+    case Y3G_POP_I2: return integrand_des_k<Y3G_POP_I2>(ig, P, tb, des, y);
+    case Y3G_POP_J: return integrand_des_k<Y3G_POP_J>(ig, P, tb, des, y);
     default: return integrand_des_k<Y3G_COUNT5D_DES>(ig, P, tb, des, y);
   }
 }

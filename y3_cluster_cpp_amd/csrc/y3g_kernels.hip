@@ -638,6 +638,18 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       case Y3G_COUNT4D_CARD:
         if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_COUNT4D_CARD>(S, A, iid, nev_total);
         break;
+      case Y3G_POP_P1:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_POP_P1>(S, A, iid, nev_total);
+        break;
+      case Y3G_POP_I1:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_POP_I1>(S, A, iid, nev_total);
+        break;
+      case Y3G_POP_I2:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_POP_I2>(S, A, iid, nev_total);
+        break;
+      case Y3G_POP_J:
+        if constexpr (sizeof(T) == 8) vegas_one<T, BLOCK, Y3G_POP_J>(S, A, iid, nev_total);
+        break;
       default: vegas_one<T, BLOCK, Y3G_DSIGMA6D>(S, A, iid, nev_total); break;
     }
   }

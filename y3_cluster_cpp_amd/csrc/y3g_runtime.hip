@@ -249,6 +249,34 @@ static int des_ndim(int kind)
        : kind == Y3G_COUNT4D_CARD ? 4 : 3;
 }
 
+// The P[X] operator's point data packed as integrand_pop_k() reads them:
+// {n_chi, n_bm, n_bz, n_xr, n_xz, chi_z, chi_dc, bias_lnM, bias_z, bias,
+// xi_r, xi_z, xi}, each axis increasing.
+static bool pop_ok(const double* par, long long npar)
+{
+  if (npar < 5) return false;
+  long long n[5];
+  for (int k = 0; k < 5; ++k) {
+    if (!(par[k] >= 2.0 && par[k] <= 65536.0) || par[k] != double((long long)par[k]))
+      return false;
+    n[k] = (long long)par[k];
+  }
+  long long const need = 5 + 2 * n[0] + n[1] + n[2] + n[1] * n[2] + n[3] + n[4] + n[3] * n[4];
+  if (npar != need) return false;
+  auto inc = [&](long long off, long long m) {
+    for (long long k = 1; k < m; ++k)
+      if (!(par[off + k] > par[off + k - 1])) return false;
+    return true;
+  };
+  long long o = 5;
+  if (!inc(o, n[0])) return false;                       // chi_z
+  o += 2 * n[0];
+  if (!inc(o, n[1]) || !inc(o + n[1], n[2])) return false;  // bias lnM, z
+  o += n[1] + n[2] + n[1] * n[2];
+  if (!inc(o, n[3]) || !inc(o + n[3], n[4])) return false;  // xi r, z
+  return true;
+}
+
 // INT_LC_LT_DES_t2 tables packed as int_lc_lt_des() reads them.
 static bool lclt_ok(const double* par, long long npar)
 {
@@ -395,6 +423,37 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
 {
   if (!p || !s || !lows || !highs || !ig)
     return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: null argument");
+  if (is_pop_kind(kind)) {
+    // P_operator_gpu: volume (lt, zt, lnm), grid (zo_low, zo_high,
+    // lambda_bin) (p_operator_gpu_t.cuh:172-179, 278-321)
+    if (n_grid != 3 || !grid)
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: the P[X] kinds take (zo_low, "
+                               "zo_high, lambda_bin)");
+    for (int i = 0; i < 3; ++i)
+      if (!(lows[i] <= highs[i]) || !std::isfinite(lows[i]) || !std::isfinite(highs[i]))
+        return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: axis " + std::to_string(i) +
+                                     " has low > high or a non-finite bound");
+    if (!(lows[1] >= 0.0) || !(highs[1] <= s->da_zmax))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: zt range outside the distance "
+                               "table [0, " + std::to_string(s->da_zmax) + "]");
+    if (!(grid[0] <= grid[1]) || !(grid[2] >= 0.0 && grid[2] <= 3.0) ||
+        grid[2] != double(int(grid[2])))
+      return fail(Y3G_ERR_ARG, "y3g_make_integral_volume: grid needs zo_low <= zo_high and "
+                               "lambda_bin in 0..3");
+    std::memset(ig, 0, sizeof(*ig));
+    ig->kind = kind;
+    ig->ndim = 3;
+    ig->zo_lo = grid[0];
+    ig->zo_hi = grid[1];
+    ig->lo_lo = grid[2];
+    ig->lo_hi = pop_lob_center(int(grid[2]));
+    for (int i = 0; i < 3; ++i) ig->lows[i] = lows[i], ig->highs[i] = highs[i];
+    ig->eps_rel = 1e-4;
+    ig->eps_abs = 1e-300;
+    ig->max_eval = 1LL << 40;
+    ig->seed = 0x59334331ull;
+    return 0;
+  }
   if (is_des_kind(kind)) {
     // the reference's volume as it is and its grid point: (zo_low, zo_high,
     // radius) for Delta-Sigma, (zo_low, zo_high) for the counts
@@ -935,7 +994,7 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   for (size_t i = 0; i < n; ++i) {
     y3g_integral const& g = igs[i];
     if (g.ndim < 2 || g.ndim > kMaxDim || g.point < 0 ||
-        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_COUNT4D_CARD ||
+        g.point >= rt->n_points || g.kind < 0 || g.kind > Y3G_POP_J ||
         !(g.eps_rel >= 0) || g.max_eval < 1)
       return fail(Y3G_ERR_ARG, std::string(who) + ": integral " + std::to_string(i) +
                                  " is malformed");
@@ -960,6 +1019,9 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
       if (g.kind == Y3G_COUNT4D_CARD && !(nt > 0 && lclt_ok(dd.data() + kDesSig, nt)))
         return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
                                      ": its point's DES data has no INT_LC_LT_DES tables");
+      if (is_pop_kind(g.kind) && !(nt > 0 && pop_ok(dd.data() + kDesSig, nt)))
+        return fail(Y3G_ERR_STATE, "y3g_integrate: integral " + std::to_string(i) +
+                                     ": its point's data has no P[X] operator tables");
       any_des = true;
     }
     for (int a = 0; a < g.ndim; ++a)
@@ -1219,6 +1281,8 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
       des_tab_ok = nt > 0 && sigmax_ok(dd.data() + kDesSig, nt);
     else if (aux->kind == Y3G_COUNT4D_CARD)
       des_tab_ok = nt > 0 && lclt_ok(dd.data() + kDesSig, nt);
+    else if (is_pop_kind(aux->kind))
+      des_tab_ok = nt > 0 && pop_ok(dd.data() + kDesSig, nt);
   }
   if (des_ig && (which != Y3G_F_INTEGRAND || precision != Y3G_FP64 || !des_tab_ok))
     return fail(Y3G_ERR_ARG, "y3g_eval_functor: the real-data Delta-Sigma integrand is "
@@ -1305,8 +1369,9 @@ int y3g_des_upload(y3g_runtime* rt, int point, const double* des, int n_des)
     if (!std::isfinite(des[k]))
       return fail(Y3G_ERR_ARG, "y3g_des_upload: non-finite parameter " + std::to_string(k));
   if (n_des > kDesSig && !sigmax_ok(des + kDesSig, n_des - kDesSig) &&
-      !lclt_ok(des + kDesSig, n_des - kDesSig))
-    return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX or INT_LC_LT_DES table sizes or axes");
+      !lclt_ok(des + kDesSig, n_des - kDesSig) && !pop_ok(des + kDesSig, n_des - kDesSig))
+    return fail(Y3G_ERR_ARG, "y3g_des_upload: SIG_MAX, INT_LC_LT_DES or P[X] table sizes "
+                             "or axes");
   if ((size_t)point >= rt->des.size()) rt->des.resize((size_t)point + 1);
   rt->des[point].assign(des, des + n_des);
   rt->des_dirty = true;

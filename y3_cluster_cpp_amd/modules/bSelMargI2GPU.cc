@@ -1,0 +1,6 @@
+// bSelMargI2GPU module library: the CosmoSIS entry points setup / execute /
+// cleanup for the Costanzi-2026 P[X] operator integrand with the I2 weight
+// (integrands.hpp; p_operator_gpu_t.cuh:327-366).
+#include "integrands.hpp"
+
+DEFINE_COSMOSIS_HIP_INTEGRATION_MODULE(bSelMargI2GPU)

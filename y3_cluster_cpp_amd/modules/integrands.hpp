@@ -468,4 +468,136 @@ private:
   std::vector<double> lclt_;
 };
 
+// ---------------------------------------------- Costanzi-2026 P[X] modules
+//
+// bSelMargP1GPU, bSelMargI1GPU, bSelMargI2GPU, bSelMargJGPU: the P[X]
+// operator integrands P_operator_gpu<W> (src/models/p_operator_gpu_t.cuh:
+// 118-299, weights :327-366) over (lt, zt, lnm), grid (zo_low, zo_high,
+// lambda_bin); their device integrands are Y3G_POP_* (FP64).  Sample inputs,
+// read as P_operator_gpu::set_sample reads them (:150-163):
+//   HMF_t, DV_DO_DZ_t      the sample's mass_function / distances tables
+//   MOR_SHIFTED_POISSON_t  cluster_mor/{log10_Mmin, log10_M1, alpha,
+//                          sigma_lambda, epsilon, z_pivot}, else
+//                          cluster_abundance/{mor_logMmin, mor_logRatio,
+//                          mor_alpha, mor_sigma, mor_epsilon, z_mor_pivot}
+//                          (mor_shifted_poisson_t.cuh:76-101)
+//   haloModel/{lnM, z, bias}, xi_nl/{r, z, xi_nl}, distances/{z, d_c},
+//   cosmological_parameters/h0
+// Config: algorithm, eps_rel, eps_abs, max_eval, use_cartesian_product;
+// the volume as scalars lt_low, lt_high, zt_low, zt_high, lnm_low, lnm_high
+// (:262-276); the grid zo_low, zo_high, lambda_bin (:280-299).
+struct PopP1 {
+  static constexpr int kind = Y3G_POP_P1;
+  static char const* label() { return "bSelMargP1GPU"; }
+};
+struct PopI1 {
+  static constexpr int kind = Y3G_POP_I1;
+  static char const* label() { return "bSelMargI1GPU"; }
+};
+struct PopI2 {
+  static constexpr int kind = Y3G_POP_I2;
+  static char const* label() { return "bSelMargI2GPU"; }
+};
+struct PopJ {
+  static constexpr int kind = Y3G_POP_J;
+  static char const* label() { return "bSelMargJGPU"; }
+};
+
+template <class W>
+class PopIntegrand : public DesSample {
+public:
+  using grid_t = y3g_cosmosis::grid_t<3>;
+  using grid_point_t = grid_t::value_type;
+
+  explicit PopIntegrand(cosmosis::DataBlock&) {}
+  void set_grid_point(grid_point_t pt) { gp_ = pt; }
+  void
+  set_sample(cosmosis::DataBlock& sample)
+  {
+    using y3g_cosmosis::view_as_double;
+    using doubles = std::vector<double>;
+    SyntheticSample::set_sample(sample);
+    des_.assign(11, 0.0);
+    des_[0] = view_as_double(sample, "cosmological_parameters", "h0");
+    if (sample.has_val("cluster_mor", "log10_Mmin")) {
+      int k = 5;
+      for (char const* n : {"log10_Mmin", "log10_M1", "alpha", "sigma_lambda", "epsilon",
+                            "z_pivot"})
+        des_[k++] = view_as_double(sample, "cluster_mor", n);
+    } else {
+      std::string const A = "cluster_abundance";
+      double const lmin = view_as_double(sample, A, "mor_logMmin");
+      des_[5] = lmin;
+      des_[6] = lmin + view_as_double(sample, A, "mor_logRatio");
+      des_[7] = view_as_double(sample, A, "mor_alpha");
+      des_[8] = view_as_double(sample, A, "mor_sigma");
+      des_[9] = view_as_double(sample, A, "mor_epsilon");
+      des_[10] = view_as_double(sample, A, "z_mor_pivot");
+    }
+    auto const cz = sample.view<doubles>("distances", "z");
+    auto const cd = sample.view<doubles>("distances", "d_c");
+    auto const bm = sample.view<doubles>("haloModel", "lnM");
+    auto const bz = sample.view<doubles>("haloModel", "z");
+    auto const bt = sample.view<cosmosis::ndarray<double>>("haloModel", "bias");
+    auto const xr = sample.view<doubles>("xi_nl", "r");
+    auto const xz = sample.view<doubles>("xi_nl", "z");
+    auto const xt = sample.view<cosmosis::ndarray<double>>("xi_nl", "xi_nl");
+    if (cd.size() != cz.size() || bt.size() != bm.size() * bz.size() ||
+        xt.size() != xr.size() * xz.size())
+      throw std::runtime_error(std::string(W::label()) +
+                               ": distances/d_c, haloModel/bias and xi_nl/xi_nl must hold "
+                               "n_z, n_z n_lnM and n_z n_r values");
+    for (double n : {double(cz.size()), double(bm.size()), double(bz.size()),
+                     double(xr.size()), double(xz.size())})
+      des_.push_back(n);
+    des_.insert(des_.end(), cz.begin(), cz.end());
+    des_.insert(des_.end(), cd.begin(), cd.end());
+    des_.insert(des_.end(), bm.begin(), bm.end());
+    des_.insert(des_.end(), bz.begin(), bz.end());
+    des_.insert(des_.end(), bt.begin(), bt.end());
+    des_.insert(des_.end(), xr.begin(), xr.end());
+    des_.insert(des_.end(), xz.begin(), xz.end());
+    des_.insert(des_.end(), xt.begin(), xt.end());
+  }
+  y3g_integral
+  describe(y3g_cosmosis::volume_t const& v) const
+  {
+    return describe_kind(W::kind, v, gp_.data(), 3);
+  }
+  static char const* module_label() { return W::label(); }
+  static std::vector<y3g_cosmosis::volume_t>
+  make_integration_volumes(cosmosis::DataBlock& cfg)
+  {
+    char const* L = module_label();
+    y3g_cosmosis::volume_t v;
+    v.lows = {cfg.view<double>(L, "lt_low"), cfg.view<double>(L, "zt_low"),
+              cfg.view<double>(L, "lnm_low")};
+    v.highs = {cfg.view<double>(L, "lt_high"), cfg.view<double>(L, "zt_high"),
+               cfg.view<double>(L, "lnm_high")};
+    return {v};
+  }
+  static grid_t
+  make_grid_points(cosmosis::DataBlock& cfg)
+  {
+    char const* L = module_label();
+    auto const a = y3g_cosmosis::get_vector_double(cfg, L, "zo_low");
+    auto const b = y3g_cosmosis::get_vector_double(cfg, L, "zo_high");
+    auto const c = y3g_cosmosis::get_vector_double(cfg, L, "lambda_bin");
+    if (b.size() != a.size() || c.size() != a.size())
+      throw std::runtime_error(std::string(L) + ": grid arrays must have same length");
+    grid_t g;
+    g.names = {"zo_low", "zo_high", "lambda_bin"};
+    for (std::size_t i = 0; i < a.size(); ++i) g.points.push_back({a[i], b[i], c[i]});
+    return g;
+  }
+
+private:
+  grid_point_t gp_{};
+};
+
+using bSelMargP1GPU = PopIntegrand<PopP1>;
+using bSelMargI1GPU = PopIntegrand<PopI1>;
+using bSelMargI2GPU = PopIntegrand<PopI2>;
+using bSelMargJGPU = PopIntegrand<PopJ>;
+
 #endif
