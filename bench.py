@@ -178,9 +178,10 @@ def cpu_baseline(pts, eps, sample, max_regions, threads, bands=None):
     return res, used, dt
 
 
-def serving_run(y3, pts, ref, eps_all, rates, seconds, opts, lanes):
+def serving_run(y3, pts, ref, eps_all, rates, seconds, opts):
     """Poisson arrivals of mock-MCMC requests at each rate; latency from the
-    server's own clock (submission to completion)."""
+    server's own clock (submission to completion).  The server runs one
+    persistent streaming launch that requests join as they arrive."""
     spec = y3.spec_wide()
     cache = {}
 
@@ -192,7 +193,7 @@ def serving_run(y3, pts, ref, eps_all, rates, seconds, opts, lanes):
             cache[key] = (p, (y3.Integral * len(igs))(*igs))
         return cache[key]
 
-    o = y3.server_options(lanes=lanes, cache_entries=0)
+    o = y3.server_options(cache_entries=0)
     o.options = opts
     srv = y3.Server(0, o)
     p, arr = integrals(0, True)  # first-launch costs outside the measurement
@@ -231,7 +232,9 @@ def serving_run(y3, pts, ref, eps_all, rates, seconds, opts, lanes):
               f"p99 {out[-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     st = srv.stats()
     srv.close()
-    return {"rates": out, "lanes": st["lanes"], "max_regions": opts.max_regions, "cache": "off",
+    return {"rates": out, "launches": st["launches"], "workgroups": st["workgroups"],
+            "table_slots": st["table_slots"], "table_builds": st["table_builds"],
+            "table_reuses": st["table_reuses"], "max_regions": opts.max_regions, "cache": "off",
             "requests": "mock random-walk MCMC over the first 64 prior points, 30% counts + "
                         "Delta-Sigma (192 integrals), 70% counts (12), eps_rel as streamed "
                         "(log-uniform [1e-5, 1e-3]); counted with the headline rule"}
@@ -420,7 +423,6 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--serve-rates", default="4,12,24")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
-    ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--no-selgl", action="store_true")
     ap.add_argument("--selgl-only", action="store_true",
                     help="run the fixed-GL selection counts leg alone and print its object")
@@ -619,8 +621,7 @@ def main():
     serving = None
     if not a.no_serve:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
-        serving = serving_run(y3, pts64, ref64, eps_raw[:len(pts64)], rates, a.serve_seconds, opts,
-                              a.serve_lanes)
+        serving = serving_run(y3, pts64, ref64, eps_raw[:len(pts64)], rates, a.serve_seconds, opts)
 
     selgl = None
     if not a.no_selgl:

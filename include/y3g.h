@@ -146,8 +146,11 @@ typedef struct {
 typedef struct {
   double value, error, prob;
   long long neval;
-  int nregions, status;   /* status 0 = converged, 1 = limit reached     */
+  int nregions, status;   /* status 0 = converged, 1 = limit reached,
+                             Y3G_STATUS_NO_TABLES: not run (serving: the
+                             integral's tables were not ready in time)  */
 } y3g_result;
+#define Y3G_STATUS_NO_TABLES 3
 
 /* Integrator tuning (defaults from y3g_options_default). */
 typedef struct {
@@ -411,38 +414,56 @@ int y3g_eval_functor(y3g_runtime* rt, int which, int precision,
  * chains, several likelihood processes sharing one GPU).  The reference
  * evaluates one sample per execute(), each a serial loop over its
  * integrals (src/utils/CosmoSISSICUDAModule.hh:252-271, 282-335); here a
- * request is one parameter point plus its integrals, queued FIFO and taken
- * whole by the first free lane (an own runtime: HIP stream, tables, region
- * slab) in batches of up to max_batch_integrals.  The lane count is capped
- * so that the lanes' region slabs fit hbm_budget_bytes.  Results of
- * (parameter point, integral) pairs are kept in an LRU cache of
- * cache_entries entries.  Results are written into the caller's array
- * before y3g_server_wait returns; the array must stay alive until then. */
+ * request is one parameter point plus its integrals, and the server runs
+ * ONE persistent cubature launch that takes integrals from a ring in
+ * pinned host memory as requests arrive (continuous batching: a request
+ * joins the launch that is already running, and its integrals start as
+ * soon as a workgroup is free).
+ *
+ * Tables: each parameter point's tables live in a table slot on the device.
+ * Slots are kept resident under an LRU, so a repeated cosmology reuses its
+ * tables; a new one takes a free or least-recently-used idle slot, and its
+ * table build runs inside the same launch, ahead of its integrals.
+ * Admission: the region slabs of the resident workgroups and table_slots
+ * table sets must fit hbm_budget_bytes (fewer slots are kept if they do not);
+ * a request is admitted, in arrival order, when a slot and ring_integrals
+ * room are free.  A launch with nothing to do ends after idle_ms and the
+ * next request starts another.  Results of (parameter point, integral)
+ * pairs are kept in an LRU cache of cache_entries entries.  The integrator
+ * is the cubature (Y3G_PAGANI); results are written into the caller's array
+ * before y3g_server_wait returns, and that array must stay alive until
+ * then. */
 typedef struct y3g_server y3g_server;
 
 typedef struct {
-  int lanes;                  /* concurrent batches                        */
-  int max_batch_integrals;    /* integrals per batch (a larger request
-                                 runs alone)                               */
-  size_t hbm_budget_bytes;    /* bound on the lanes' region slabs          */
-  long long cache_entries;    /* LRU result cache size, 0 = off            */
-  y3g_options options;        /* integrator options of every batch         */
+  int ring_integrals;         /* integrals (and table builds) in flight     */
+  int table_slots;            /* resident per-cosmology table sets          */
+  size_t hbm_budget_bytes;    /* bound on region slabs + table sets         */
+  long long cache_entries;    /* LRU result cache size, 0 = off             */
+  double idle_ms;             /* an idle launch ends after this long        */
+  double table_wait_ms;       /* bound on an integral's wait for its tables */
+  y3g_options options;        /* integrator options (precision fp32split or
+                                 fp64; block_threads is 512)               */
   y3g_table_spec spec;        /* table grids of every parameter point      */
 } y3g_server_options;
 
 typedef struct {
   long long requests_submitted, requests_done;
-  long long integrals_computed, cache_hits, batches, evaluations, queued;
-  long long slab_bytes_per_lane;
-  double kernel_ms;           /* summed over batches                       */
-  int lanes;                  /* lanes admitted by the HBM budget          */
+  long long integrals_computed, cache_hits, evaluations, queued;
+  long long launches;         /* persistent launches started               */
+  long long table_builds, table_reuses;
+  long long slab_bytes, table_bytes;  /* device memory of the session      */
+  int table_slots;            /* table slots admitted by the HBM budget    */
+  int workgroups;             /* resident workgroups of the launch         */
 } y3g_server_stats;
 
 const char* y3g_server_last_error(void);
 void y3g_server_options_default(y3g_server_options* o);
 int y3g_server_create(int device, const y3g_server_options* opt, y3g_server** out);
 /* Queue one request; integral `point` fields are ignored (the server
- * assigns table sets).  Cache hits are written to out at once. */
+ * assigns table slots).  Cache hits are written to out at once.  Only
+ * Y3G_PAGANI integrals of the synthetic kinds (0-2) are served; a request
+ * larger than the ring is refused. */
 int y3g_server_submit(y3g_server* s, const y3g_params* params,
                       const y3g_integral* igs, size_t n, y3g_result* out,
                       unsigned long long* ticket);
@@ -451,7 +472,7 @@ int y3g_server_submit(y3g_server* s, const y3g_params* params,
 int y3g_server_wait(y3g_server* s, unsigned long long ticket, double timeout_s,
                     double* latency_s);
 int y3g_server_stats_get(y3g_server* s, y3g_server_stats* st);
-/* Finish the queued requests, stop the lanes, free everything. */
+/* Finish the queued requests, end the launch, free everything. */
 int y3g_server_destroy(y3g_server* s);
 
 #ifdef __cplusplus

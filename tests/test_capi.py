@@ -60,3 +60,22 @@ def test_runtime_create_without_device_fails_cleanly():
     assert rc == 4 and "device" in y3.last_error()
     # null runtime is an argument error, not a crash
     assert y3.lib().y3g_integrate(None, None, 0, None, None) == 1
+
+
+def test_server_options_layout():
+    """The ctypes mirror of y3g_server_options reads the C defaults field by
+    field (a layout slip shifts every field after it)."""
+    o = y3.server_options()
+    assert (o.ring_integrals, o.table_slots) == (65536, 256)
+    assert o.hbm_budget_bytes == 64 << 30 and o.cache_entries == 1 << 20
+    assert (o.idle_ms, o.table_wait_ms) == (50.0, 2000.0)
+    d = y3.options()
+    for n, _ in y3.Options._fields_:
+        a, b = getattr(o.options, n), getattr(d, n)
+        if hasattr(a, "__len__"):
+            assert list(a) == list(b), n
+        else:
+            assert a == b, n
+    w = y3.spec_wide()
+    for n, _ in y3.TableSpec._fields_:
+        assert getattr(o.spec, n) == getattr(w, n), n

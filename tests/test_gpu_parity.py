@@ -235,23 +235,38 @@ def test_config1_all_bins(runtime, prec):
     _check(res, np.array(G["config1"]["values"]).reshape(-1), 1e-3)
 
 
+def _check_within(res, gold, eps):
+    """Every integral within eps of the golden value with an honest error;
+    status 0 (converged) or 1 (region budget reached)."""
+    gold = np.asarray(gold)
+    rel = np.abs(res["value"] - gold) / np.abs(gold)
+    assert set(np.unique(res["status"])) <= {0, 1}, res["status"]
+    assert np.all(rel <= eps), (rel.max(), int(np.argmax(rel)))
+    assert np.all(res["error"] >= np.abs(res["value"] - gold)), "error estimate below true error"
+
+
 def test_config1_tight(runtime):
+    """Config 1 at its own tolerance: all 12 five-dimensional bins at 1e-4
+    (FP32 integrand, FP64 sums, the 32768-region budget)."""
     p, s = fid(runtime)
-    gold = np.array(G["config1"]["values"])
+    gold = np.array(G["config1"]["values"]).reshape(-1)
     igs = [y3.make_integral(p, s, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-4)
-           for (i, j) in [(0, 0), (1, 1), (2, 2)]]
-    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT))
-    _check(res, np.array([gold[0, 0], gold[1, 1], gold[2, 2]]), 1e-4)
+           for i in range(4) for j in range(3)]
+    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT, max_regions=32768))
+    _check_within(res, gold, 1e-4)
 
 
-def test_config2_subset(runtime):
+def test_config2_all_integrals(runtime):
+    """Config 2 at its own tolerance: all 180 six-dimensional Delta-Sigma
+    integrals (12 bins x 15 radii) at 1e-3, one launch."""
     p, s = fid(runtime)
-    gold = np.array(G["config2"]["values"])
-    sel = [(0, 0, 3), (1, 1, 7), (3, 2, 12), (2, 0, 0)]
+    gold = np.array(G["config2"]["values"]).reshape(-1)
     igs = [y3.make_integral(p, s, y3.DSIGMA6D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], RADII[k],
-                            eps_rel=1e-3) for (i, j, k) in sel]
-    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT))
-    _check(res, np.array([gold[i, j, k] for (i, j, k) in sel]), 1e-3)
+                            eps_rel=1e-3) for i in range(4) for j in range(3) for k in range(15)]
+    res = runtime.integrate(igs, y3.options(precision=y3.FP32SPLIT, max_regions=32768))
+    _check_within(res, gold, 1e-3)
+    # most converge outright at this tolerance; those are within eps too
+    assert np.mean(res["status"] == 0) > 0.5, np.mean(res["status"] == 0)
 
 
 def test_results_independent_of_batch(runtime):
@@ -350,6 +365,16 @@ def test_large_batch_independence(runtime):
     head = big[:64 * 192]
     err = np.abs(head["value"] - ref.reshape(-1))
     assert np.all(head["error"] >= err)
+    # tolerance on the stream: every converged integral (status 0) is within
+    # its eps_rel; of all, at least 85% are within eps and counted (the
+    # accuracy at the 32768-region budget is limited by the bilinear HMF
+    # table's kinks: median relative error ~1e-5 against eps down to 1e-5)
+    rel = err / np.abs(ref.reshape(-1))
+    e = eps[:64].reshape(-1)
+    conv = head["status"] == 0
+    assert np.all(rel[conv] <= e[conv]), rel[conv].max()
+    counted = (rel <= e) & (head["error"] >= err)
+    assert counted.mean() >= 0.85, counted.mean()
 
 
 def test_tolerance_bands(runtime):

@@ -1,14 +1,20 @@
-"""GPU: the request server (y3g_server_* of include/y3g.h).
+"""GPU: the request server (y3g_server_* of include/y3g.h): one persistent
+streaming launch that requests join as they arrive.
 
 - a request's results are bit-identical whether it is served alone or in
   a mix of other requests (different parameter points and tolerances), and
   equal to the library's batched call on the same descriptors;
-- the LRU cache answers a repeated request without the device;
-- admission: a budget below one lane's region slab is refused, a budget
-  for one lane admits one;
+- a request submitted while another is on the GPU joins the running launch
+  and finishes first, with the results it has alone;
+- table slots stay resident under an LRU: a repeated parameter point reuses
+  its tables, a new one evicts the least recently used idle slot;
+- the LRU result cache answers a repeated request without the device;
+- admission: a budget below the region slabs and one table set is refused,
+  a budget for one table set admits one slot; a request larger than the
+  ring is refused;
 - a burst of mock-MCMC requests completes, every result checked against
-  the FP64 oracle's golden values within its eps_rel (capped at 1e-4 for
-  the count) with a reported error that bounds the true error.
+  the FP64 oracle's golden values within its eps_rel with a reported error
+  that bounds the true error.
 """
 import json
 import os
@@ -42,7 +48,7 @@ def opts(**kw):
 
 
 def test_alone_vs_mix_bit_identical(runtime):
-    srv = y3.Server(0, opts(lanes=2, cache_entries=0, max_batch_integrals=256))
+    srv = y3.Server(0, opts(cache_entries=0))
     p, igs = request(3, counts_only=False)
     sub = igs[:12] + igs[12::20]           # 12 counts + 9 Delta-Sigma
     t = srv.submit(p, sub)
@@ -61,37 +67,84 @@ def test_alone_vs_mix_bit_identical(runtime):
     mixed = outs["A"]
     for f in ("value", "error", "nregions", "neval", "status"):
         np.testing.assert_array_equal(mixed[f], alone[f], err_msg=f)
-    assert st["batches"] >= 2 and st["requests_done"] == 6
+    assert st["requests_done"] == 6 and st["launches"] >= 1
+    assert st["table_reuses"] >= 1  # point 3's tables served both of its requests
     # the same descriptors through the library's batched call
     runtime.tables_build(p, y3.spec_wide())
     direct = runtime.integrate(sub, y3.options(max_regions=8192))
-    np.testing.assert_array_equal(direct["value"], alone["value"])
+    for f in ("value", "error", "nregions", "neval", "status"):
+        np.testing.assert_array_equal(direct[f], alone[f], err_msg=f)
+
+
+def test_request_joins_running_launch():
+    """A: 192 integrals run to their region budget (eps 1e-9).  B, submitted
+    while A runs: 12 counts at another parameter point.  B finishes while A
+    is still on the GPU, in the same launch, with its results alone."""
+    import time
+    o = opts(cache_entries=0, idle_ms=2000.0)
+    o.options.max_regions = 32768
+    srv = y3.Server(0, o)
+    pa = W.point_params(y3, PTS[2])
+    ia = W.point_integrals(y3, pa, y3.spec_wide(), 0, np.full(192, 1e-9), y3.PAGANI)
+    pb, ib = request(4, counts_only=True)
+    ta = srv.submit(pa, ia)
+    time.sleep(0.005)
+    tb = srv.submit(pb, ib)
+    mixed, _ = srv.wait(tb)
+    with pytest.raises(RuntimeError, match="timed out"):
+        srv.wait(ta, timeout=0.0)       # A is still running
+    outa, _ = srv.wait(ta)
+    st = srv.stats()
+    srv.close()
+    assert st["launches"] == 1 and st["table_builds"] == 2, st
+    assert np.all(outa["nregions"] > 0) and np.all(np.isfinite(outa["value"]))
+    srv = y3.Server(0, o)
+    alone, _ = srv.wait(srv.submit(pb, ib))
+    srv.close()
+    for f in ("value", "error", "nregions", "neval", "status"):
+        np.testing.assert_array_equal(mixed[f], alone[f], err_msg=f)
+
+
+def test_table_slots_lru():
+    srv = y3.Server(0, opts(cache_entries=0, table_slots=2))
+    assert srv.stats()["table_slots"] == 2
+    outs = []
+    for k in (1, 6, 1, 8, 1):
+        p, igs = request(k, counts_only=True)
+        outs.append(srv.wait(srv.submit(p, igs))[0])
+    st = srv.stats()
+    srv.close()
+    # 1 build, 6 build, 1 reuse, 8 build (evicts 6), 1 reuse
+    assert (st["table_builds"], st["table_reuses"]) == (3, 2), st
+    for f in ("value", "error", "nregions"):
+        np.testing.assert_array_equal(outs[0][f], outs[2][f], err_msg=f)
+        np.testing.assert_array_equal(outs[0][f], outs[4][f], err_msg=f)
+    assert np.all(outs[3]["status"] != y3.STATUS_NO_TABLES)
 
 
 def test_cache_hits_and_admission():
-    srv = y3.Server(0, opts(lanes=1, cache_entries=1000))
+    srv = y3.Server(0, opts(cache_entries=1000))
     p, igs = request(1, counts_only=True)
     a, _ = srv.wait(srv.submit(p, igs))
     b, lat = srv.wait(srv.submit(p, igs))
     st = srv.stats()
     srv.close()
     np.testing.assert_array_equal(a, b)
-    assert st["cache_hits"] == 12 and st["integrals_computed"] == 12 and st["batches"] == 1
+    assert st["cache_hits"] == 12 and st["integrals_computed"] == 12
     assert lat < 0.05
-    o = opts(lanes=4, hbm_budget_bytes=1 << 20)
+    slab, per_slot = st["slab_bytes"], st["table_bytes"] // st["table_slots"]
+    assert slab > 0 and per_slot > 0
     with pytest.raises(RuntimeError, match="over the budget"):
-        y3.Server(0, o)
-    probe = y3.Server(0, opts(lanes=1))
-    per_lane = probe.stats()["slab_bytes_per_lane"]
-    probe.close()
-    assert per_lane > 0
-    srv = y3.Server(0, opts(lanes=4, hbm_budget_bytes=int(per_lane * 1.5)))
-    assert srv.stats()["lanes"] == 1
+        y3.Server(0, opts(hbm_budget_bytes=slab + per_slot // 2))
+    srv = y3.Server(0, opts(hbm_budget_bytes=slab + per_slot + per_slot // 2))
+    assert srv.stats()["table_slots"] == 1
+    with pytest.raises(RuntimeError, match="larger than the ring"):
+        srv.submit(p, igs * 6000)
     srv.close()
 
 
 def test_mcmc_burst_against_golden():
-    srv = y3.Server(0, opts(lanes=2, cache_entries=0))
+    srv = y3.Server(0, opts(cache_entries=0))
     t_arr, reqs = W.mcmc_requests(PTS, 24, rate=1000.0)
     sent = []
     for k, full in reqs:
@@ -105,6 +158,7 @@ def test_mcmc_burst_against_golden():
         ref = REF[k, :n]
         eps = np.minimum(EPS[k, :n], 1e-3)
         err = np.abs(out["value"] - ref)
+        assert np.all(out["status"] != y3.STATUS_NO_TABLES)
         assert np.all(out["error"] >= err), (k, np.where(out["error"] < err))
         conv = out["status"] == 0    # converged ones meet their tolerance
         assert np.all(err[conv] <= eps[conv] * np.abs(ref[conv]))

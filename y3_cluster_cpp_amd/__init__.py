@@ -84,16 +84,20 @@ def selgl_spec(lnm_lo, lnm_hi, zt_lo, zt_hi, n_lnm=96, n_z=64):
 
 
 class ServerOptions(C.Structure):
-    _fields_ = [("lanes", C.c_int), ("max_batch_integrals", C.c_int),
+    _fields_ = [("ring_integrals", C.c_int), ("table_slots", C.c_int),
                 ("hbm_budget_bytes", C.c_size_t), ("cache_entries", C.c_longlong),
+                ("idle_ms", C.c_double), ("table_wait_ms", C.c_double),
                 ("options", Options), ("spec", TableSpec)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in (
         "requests_submitted", "requests_done", "integrals_computed", "cache_hits",
-        "batches", "evaluations", "queued", "slab_bytes_per_lane")] + [
-        ("kernel_ms", C.c_double), ("lanes", C.c_int)]
+        "evaluations", "queued", "launches", "table_builds", "table_reuses",
+        "slab_bytes", "table_bytes")] + [("table_slots", C.c_int), ("workgroups", C.c_int)]
+
+
+STATUS_NO_TABLES = 3  # a served integral whose tables were not ready in time
 
 
 RESULT_DTYPE = np.dtype([("value", "f8"), ("error", "f8"), ("prob", "f8"),
@@ -423,7 +427,8 @@ def server_options(**kw):
 
 class Server:
     """Request server (y3g_server_*): submit(params, integrals) -> ticket,
-    wait(ticket) -> (results, latency seconds)."""
+    wait(ticket) -> (results, latency seconds).  One persistent streaming
+    launch serves every request; requests join it as they arrive."""
 
     def __init__(self, device=0, opts=None):
         self._h = C.c_void_p()

@@ -292,6 +292,10 @@ template <typename T, int BLOCK> struct CubShared {
   y3g_integral ig;
   PointDev P;
   int item, point;
+  unsigned pgen;           // streamed: the table generation of `point`
+  long long sitem;         // streamed: the claimed ring entry (-1: none)
+  int job, sslot;          // streamed: the entry's job and table slot
+  unsigned sgen;           //   and the generation it needs or builds
   int setup_kind;  // kind x 16 + ndim the node maps (pair_setup) hold, -1 none
   // FP32 path, per chunk: the pairs' factors and (Delta-Sigma) the profile
   // of each (lnM, u, theta) triple of the chunk's regions; the zt and lnM

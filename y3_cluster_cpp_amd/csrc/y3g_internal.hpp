@@ -63,6 +63,44 @@ struct KernArgs {
   unsigned long long* nevals;
 };
 
+// Continuous batching (k_cubature_stream).  Ring entries and the control
+// block live in coherent pinned host memory; StreamDev in device memory.
+constexpr int kStreamJobIntegral = 0;
+constexpr int kStreamJobTables = 1;
+struct StreamItem {
+  int job;           // kStreamJobIntegral or kStreamJobTables
+  int slot;          // table slot (the integral's point)
+  unsigned gen;      // the slot's table generation needed / built
+  int pad;
+  y3g_integral ig;   // integrals only
+};
+static_assert(sizeof(StreamItem) % 8 == 0, "ring entries are read as 64-bit words");
+static_assert(sizeof(PointDev) % 8 == 0, "PointDev is read as 64-bit words");
+struct StreamCtl {
+  unsigned long long tail;  // entries published by the host
+  unsigned close;           // host: exit once every published entry is claimed
+  unsigned pad;
+};
+struct StreamDev {
+  unsigned long long head;  // entries claimed
+  unsigned long long tail;  // the device's copy of StreamCtl::tail
+  unsigned close;           // the device's copy of StreamCtl::close
+  unsigned lock;            // held by the workgroup refreshing the copies
+};
+struct StreamArgs {
+  const StreamItem* ring;       // [ring_cap], host
+  y3g_result* res;              // [ring_cap], host
+  unsigned long long* seq;      // [ring_cap], host: entry + 1 once res is written
+  const StreamCtl* ctl;         // host
+  const PointDev* hpoints;      // [table slots], host: what a table build reads
+  StreamDev* dev;               // device
+  unsigned* ready;              // [table slots], device: generation of the tables
+  PointDev* points;             // [table slots], device: the kernels' PointDev
+  TableArgs tab;                // the slots' tables (written by table builds)
+  unsigned ring_cap;
+  long long idle_ticks, wait_ticks;  // wall-clock ticks
+};
+
 struct FunctorArgs {
   int which;
   const double* in;
@@ -109,6 +147,10 @@ hipError_t launch_cubature(const KernArgs& A, int precision, int block,
                            int slots, hipStream_t s);
 hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,
                         hipStream_t s);
+// The streaming cubature (512-thread workgroups) and its residency per CU.
+hipError_t launch_cubature_stream(const KernArgs& A, const StreamArgs& Z, int precision,
+                                  int slots, hipStream_t s);
+int cubature_stream_blocks_per_cu(int precision);
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s);
 hipError_t launch_zero_words(void* p, size_t n, hipStream_t s);
 hipError_t launch_nfw_mis_table(double* tab, int nx, int nm, hipStream_t s);

@@ -48,16 +48,16 @@ __device__ __forceinline__ double lin_knot(double a, double b, int n, int i)
   return a + double(i) * ((b - a) / double(n - 1));
 }
 
-// One workgroup per parameter point: HMF (n_z x n_lnM), D_A (n_da),
-// Sigma_crit^-1 (n_sci), FP64 and FP32 copies.
-__global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
+// The tables of one parameter point, by the nt threads of one workgroup:
+// HMF (n_z x n_lnM), D_A (n_da), Sigma_crit^-1 (n_sci), FP64 and FP32
+// copies.
+__device__ __attribute__((noinline)) void build_point_tables(const TableArgs& A, int pt,
+                                                             const PointDev& P, int tid, int nt)
 {
-  int const pt = blockIdx.x;
-  PointDev const P = A.points[pt];
   y3g_table_spec const& s = A.spec;
   double* hmf64 = A.hmf64 + (size_t)pt * A.hmf_stride;
   float* hmf32 = A.hmf32 + (size_t)pt * A.hmf_stride;
-  for (int k = threadIdx.x; k < s.n_lnm * s.n_z; k += blockDim.x) {
+  for (int k = tid; k < s.n_lnm * s.n_z; k += nt) {
     int const i = k % s.n_lnm, j = k / s.n_lnm;
     double const lnM = lin_knot(s.lnm_lo, s.lnm_hi, s.n_lnm, i);
     double const z = lin_knot(s.z_lo, s.z_hi, s.n_z, j);
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
   }
   double* da64 = A.da64 + (size_t)pt * A.da_stride;
   float* da32 = A.da32 + (size_t)pt * A.da_stride;
-  for (int i = threadIdx.x; i < s.n_da; i += blockDim.x) {
+  for (int i = tid; i < s.n_da; i += nt) {
     double const z = lin_knot(0.0, s.da_zmax, s.n_da, i);
     double const v = comoving_distance(P.p.omega_m, z) / P.p.h / (1.0 + z);
     da64[i] = v;
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
   double const chis = comoving_distance(P.p.omega_m, P.p.z_source);
   double* sci64 = A.sci64 + (size_t)pt * A.sci_stride;
   float* sci32 = A.sci32 + (size_t)pt * A.sci_stride;
-  for (int i = threadIdx.x; i < s.n_sci; i += blockDim.x) {
+  for (int i = tid; i < s.n_sci; i += nt) {
     double const z = lin_knot(0.0, s.sci_zmax, s.n_sci, i);
     double const chil = comoving_distance(P.p.omega_m, z);
     double const v =
@@ -89,13 +89,108 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
   }
 }
 
+// One workgroup per parameter point.
+__global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
+{
+  int const pt = blockIdx.x;
+  build_point_tables(A, pt, A.points[pt], threadIdx.x, blockDim.x);
+}
+
+// ------------------------------------------------- continuous batching
+//
+// The streaming form of the cubature (k_cubature_stream) takes its work from
+// a ring in coherent pinned host memory instead of a fixed list: the host
+// writes ring entries, then publishes them by raising StreamCtl::tail.  A
+// workgroup claims an entry only once it is published (compare-and-swap on
+// the device's head counter), so no claim is ever held on an entry that
+// does not exist yet.  One idle workgroup at a time (a try-lock) copies the
+// host's tail and close flag into device memory, where the others read
+// them.  Every wait is bounded: a workgroup that finds no work for
+// idle_ticks of the constant-rate wall clock exits, and the host relaunches
+// when it publishes more; an integral whose tables are not ready within
+// wait_ticks returns status Y3G_STATUS_NO_TABLES.
+//
+// Entries are integrals or table builds.  A table build fills one table
+// slot from the PointDev the host wrote for it and then releases the slot's
+// generation number; integrals on that slot wait (acquire) for their
+// generation.  Builds are claimed before the integrals that follow them in
+// the ring, by workgroups that are running, so the wait always ends.
+
+// Relaxed system-scope loads of pinned host memory, one 64-bit word at a
+// time (never through the scalar cache).
+__device__ __forceinline__ void load_host_words(void* dst, const void* src, int words)
+{
+  auto* d = reinterpret_cast<unsigned long long*>(dst);
+  auto* s = reinterpret_cast<const unsigned long long*>(src);
+  for (int k = 0; k < words; ++k)
+    d[k] = __hip_atomic_load(const_cast<unsigned long long*>(s + k), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The next published entry, or -1 (closed and drained, or idle too long).
+__device__ long long stream_claim(const StreamArgs& Z)
+{
+  StreamDev* D = Z.dev;
+  long long const t0 = wall_clock64();
+  for (;;) {
+    unsigned long long h = __hip_atomic_load(&D->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long const t =
+      __hip_atomic_load(&D->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (h < t) {
+      if (__hip_atomic_compare_exchange_strong(&D->head, &h, h + 1, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return (long long)h;
+      continue;
+    }
+    if (__hip_atomic_load(&D->close, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return -1;
+    unsigned z = 0u;
+    if (__hip_atomic_compare_exchange_strong(&D->lock, &z, 1u, __ATOMIC_ACQUIRE,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      // close first: once it reads 1 the tail read after it is the final one
+      unsigned const cl =
+        __hip_atomic_load(&Z.ctl->close, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+      unsigned long long const th =
+        __hip_atomic_load(&Z.ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_fetch_max(&D->tail, th, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (cl) __hip_atomic_store(&D->close, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&D->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (th > h) continue;
+    }
+    if (wall_clock64() - t0 > Z.idle_ticks) return -1;
+    __builtin_amdgcn_s_sleep(16);
+  }
+}
+
+// Wait (bounded) until the slot's tables are those of generation gen.
+__device__ bool stream_wait_tables(const StreamArgs& Z, int slot, unsigned gen)
+{
+  long long const t0 = wall_clock64();
+  while (__hip_atomic_load(Z.ready + slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    if (wall_clock64() - t0 > Z.wait_ticks) return false;
+    __builtin_amdgcn_s_sleep(16);
+  }
+  return true;
+}
+
+// Write an entry's result, then release its sequence number (entry + 1).
+__device__ void stream_publish(const StreamArgs& Z, long long item, const y3g_result& r)
+{
+  size_t const k = (size_t)(item % (long long)Z.ring_cap);
+  Z.res[k] = r;
+  __hip_atomic_store(Z.seq + k, (unsigned long long)item + 1ull, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 #ifndef Y3G_CUB_MIN_WAVES
 #define Y3G_CUB_MIN_WAVES 6
 #endif
 // FP32: six waves per SIMD (at most 80 VGPRs), so that three 512-thread
 // workgroups share a CU (their LDS fits too); FP64: the compiler's choice
-template <typename T, int BLOCK>
-__global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature(KernArgs A)
+// The cubature's work loop: integrals from the device queue (k_cubature) or
+// from the host ring (STREAM, k_cubature_stream); the region algorithm of
+// an integral is the same code in both.
+template <typename T, int BLOCK, bool STREAM>
+__device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamArgs& Z)
 {
   extern __shared__ char smem_raw[];
   CubShared<T, BLOCK>& S = *reinterpret_cast<CubShared<T, BLOCK>*>(smem_raw);
@@ -110,7 +205,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   double* err = A.slab_err + (size_t)slot * cap;
   double* raw = A.slab_raw + (size_t)slot * cap;
   int8_t* axs = A.slab_ax + (size_t)slot * cap;
-  if (tid == 0) S.point = -1, S.setup_kind = -1;
+  if (tid == 0) S.point = -1, S.pgen = 0u, S.setup_kind = -1;
   if constexpr (sizeof(T) == 4)
     for (int k = tid; k < kNfwN; k += BLOCK) S.nfwtab[k] = kNfwTab[k];
   if constexpr (sizeof(T) == 8) {
@@ -120,13 +215,60 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
   }
   for (;;) {
     __syncthreads();
-    if (tid == 0) S.item = atomicAdd(A.queue, 1);
-    __syncthreads();
-    int const item = S.item;
-    if (item >= A.n) break;
-    int const iid = A.order ? A.order[item] : item;
+    int iid = 0;
+    if constexpr (STREAM) {
+      if (tid == 0) {
+        long long const it = stream_claim(Z);
+        S.sitem = it;
+        if (it >= 0) {
+          StreamItem const* e = Z.ring + (size_t)(it % (long long)Z.ring_cap);
+          StreamItem h;
+          load_host_words(&h, e, sizeof(StreamItem) / 8);
+          S.job = h.job;
+          S.sslot = h.slot;
+          S.sgen = h.gen;
+          S.ig = h.ig;
+          S.ig.point = h.slot;
+        }
+      }
+      __syncthreads();
+      if (S.sitem < 0) break;
+      if (S.job == kStreamJobTables) {
+        // build the slot's tables, then release its generation
+        if (tid == 0) load_host_words(&S.P, Z.hpoints + S.sslot, sizeof(PointDev) / 8);
+        __syncthreads();
+        build_point_tables(Z.tab, S.sslot, S.P, tid, BLOCK);
+        if (tid == 0) Z.points[S.sslot] = S.P;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+          __hip_atomic_store(Z.ready + S.sslot, S.sgen, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          y3g_result r{};
+          stream_publish(Z, S.sitem, r);
+          S.point = -1;  // S.P now holds this slot's constants: reload
+        }
+        continue;
+      }
+      if (tid == 0) S.job = stream_wait_tables(Z, S.sslot, S.sgen) ? 0 : -1;
+      __syncthreads();
+      if (S.job < 0) {
+        if (tid == 0) {
+          y3g_result r{};
+          r.status = Y3G_STATUS_NO_TABLES;
+          stream_publish(Z, S.sitem, r);
+        }
+        continue;
+      }
+    } else {
+      if (tid == 0) S.item = atomicAdd(A.queue, 1);
+      __syncthreads();
+      int const item = S.item;
+      if (item >= A.n) break;
+      iid = A.order ? A.order[item] : item;
+    }
     if (tid == 0) {
-      S.ig = A.igs[iid];
+      if constexpr (!STREAM) S.ig = A.igs[iid];
       gm_weights(S.ig.ndim, S.w7, S.w5, S.w3);
       // the node maps depend on the kind and dimension only; the queue's
       // longest-first order keeps a workgroup on one kind for long stretches
@@ -137,7 +279,9 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
     }
     __syncthreads();
     int const pt = S.ig.point;
-    if (pt != S.point) {
+    // a streamed slot's tables change with its generation
+    unsigned const pgen = STREAM ? S.sgen : 0u;
+    if (pt != S.point || pgen != S.pgen) {
       if (tid == 0) {
         S.P = A.points[pt];
         S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
@@ -154,7 +298,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       }
       stage_tables<T>(A, pt, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
       __syncthreads();
-      if (tid == 0) S.point = pt;
+      if (tid == 0) S.point = pt, S.pgen = pgen;
     }
     int const d = S.ig.ndim;
     int const nn = gm_num_nodes(d);
@@ -380,10 +524,26 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void
       res.neval = neval;
       res.nregions = N;
       res.status = status;
-      A.out[iid] = res;
+      if constexpr (STREAM)
+        stream_publish(Z, S.sitem, res);
+      else
+        A.out[iid] = res;
       atomicAdd(A.nevals, (unsigned long long)neval);
     }
   }
+}
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature(KernArgs A)
+{
+  cubature_items<T, BLOCK, false>(A, StreamArgs{});
+}
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature_stream(
+  KernArgs A, StreamArgs Z)
+{
+  cubature_items<T, BLOCK, true>(A, Z);
 }
 
 // ------------------------------------------------------------------ VEGAS
@@ -989,6 +1149,40 @@ hipError_t launch_cubature(const KernArgs& A, int precision, int block,
                         : launch_cub<double, 512>(A, slots, s);
   return block == 256 ? launch_cub<float, 256>(A, slots, s)
                       : launch_cub<float, 512>(A, slots, s);
+}
+
+template <typename T>
+static hipError_t launch_cub_stream(const KernArgs& A, const StreamArgs& Z, int slots,
+                                    hipStream_t s)
+{
+  size_t const smem = sizeof(CubShared<T, 512>);
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void*>(&k_cubature_stream<T, 512>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_cubature_stream<T, 512>), dim3(slots), dim3(512), smem, s, A, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_cubature_stream(const KernArgs& A, const StreamArgs& Z, int precision,
+                                  int slots, hipStream_t s)
+{
+  return precision == Y3G_FP64 ? launch_cub_stream<double>(A, Z, slots, s)
+                               : launch_cub_stream<float>(A, Z, slots, s);
+}
+
+int cubature_stream_blocks_per_cu(int precision)
+{
+  int n = 0;
+  size_t const smem = cubature_smem(precision, 512);
+  const void* f = precision == Y3G_FP64
+                    ? reinterpret_cast<const void*>(&k_cubature_stream<double, 512>)
+                    : reinterpret_cast<const void*>(&k_cubature_stream<float, 512>);
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) !=
+        hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 512, smem) != hipSuccess)
+    return 0;
+  return n;
 }
 
 hipError_t launch_vegas(const KernArgs& A, int precision, int block, int slots,

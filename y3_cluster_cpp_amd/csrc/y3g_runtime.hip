@@ -18,6 +18,7 @@
 #include "y3g.h"
 #include "y3g_internal.hpp"
 #include "y3g_single.hpp"
+#include "y3g_stream.hpp"
 
 using namespace y3g;
 
@@ -1565,3 +1566,302 @@ int y3g_nfw_mis_table_download(y3g_runtime* rt, double* logx, double* logxmis,
                      hipMemcpyDeviceToHost));
   return 0;
 }
+
+// ------------------------------------------------------- streaming session
+//
+// The continuous-batching session of y3g_stream.hpp: ring, results and
+// control block in coherent pinned host memory (the launch reads and writes
+// them over the bus), the head counter, table generations and the slots'
+// tables in device memory.
+
+struct y3g_stream {
+  y3g_runtime* rt = nullptr;
+  y3g::StreamConfig cfg{};
+  int workgroups = 0;
+  size_t slab_bytes = 0, slot_bytes = 0;
+  // pinned host memory
+  StreamItem* ring = nullptr;
+  y3g_result* res = nullptr;
+  unsigned long long* seq = nullptr;
+  StreamCtl* ctl = nullptr;
+  PointDev* hpoints = nullptr;
+  // device memory
+  StreamDev* dev = nullptr;
+  unsigned* ready = nullptr;
+  // host state
+  unsigned long long tail = 0;        // entries published
+  unsigned long long low = 0;         // every entry below is finished and reported
+  unsigned long long head_known = 0;  // entries claimed when the last launch ended
+  std::vector<unsigned char> kind;    // per ring entry: 0 free, 1 integral, 2 table build
+  std::vector<unsigned> gen;          // per slot: the last generation published
+  std::vector<int> tables_pending;    // per slot: table builds not finished
+  bool running = false;
+  long long launches = 0;
+  hipEvent_t done = nullptr;
+  KernArgs A{};
+  StreamArgs Z{};
+};
+
+namespace y3g {
+
+static int stream_shape(y3g_runtime* rt, const StreamConfig& c, int* wg, size_t* slab,
+                        size_t* slot)
+{
+  if (c.options.precision != Y3G_FP32SPLIT && c.options.precision != Y3G_FP64)
+    return fail(Y3G_ERR_ARG, "stream: precision must be fp32split or fp64");
+  if (c.options.max_regions < 64 || c.options.max_split_per_step < 1 ||
+      c.options.max_split_per_step > 1024)
+    return fail(Y3G_ERR_ARG, "stream: options out of range");
+  if (int rc = check_spec(&c.spec)) return rc;
+  HIPCHK(hipSetDevice(rt->device));
+  int const per_cu = cubature_stream_blocks_per_cu(c.options.precision);
+  if (per_cu < 1) return fail(Y3G_ERR_HIP, "stream: occupancy query failed");
+  *wg = c.options.slots > 0 ? c.options.slots : rt->n_cu * per_cu;
+  *slab = (size_t)*wg * c.options.max_regions * (2 * kMaxDim * 8 + 3 * 8 + 1);
+  size_t const nh = (size_t)c.spec.n_lnm * c.spec.n_z;
+  *slot = sizeof(PointDev) + nh * 12 + (size_t)c.spec.n_da * 12 + (size_t)c.spec.n_sci * 12 + 4;
+  return 0;
+}
+
+int stream_footprint(y3g_runtime* rt, const StreamConfig& c, size_t* slab_bytes,
+                     size_t* slot_bytes, int* workgroups)
+{
+  if (!rt || !slab_bytes || !slot_bytes || !workgroups)
+    return fail(Y3G_ERR_ARG, "stream_footprint: bad arguments");
+  return stream_shape(rt, c, workgroups, slab_bytes, slot_bytes);
+}
+
+static void stream_free(y3g_stream* s)
+{
+  if (s->ring) (void)hipHostFree(s->ring);
+  if (s->res) (void)hipHostFree(s->res);
+  if (s->seq) (void)hipHostFree(s->seq);
+  if (s->ctl) (void)hipHostFree(s->ctl);
+  if (s->hpoints) (void)hipHostFree(s->hpoints);
+  dfree(s->dev);
+  dfree(s->ready);
+  if (s->done) (void)hipEventDestroy(s->done);
+  delete s;
+}
+
+template <typename T> static hipError_t halloc(T** p, size_t n)
+{
+  // coherent: the launch polls and writes this memory while it runs
+  hipError_t const e = hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n,
+                                     hipHostMallocCoherent | hipHostMallocMapped);
+  if (e == hipSuccess) std::memset(*p, 0, sizeof(T) * n);
+  return e;
+}
+
+int stream_open(y3g_runtime* rt, const StreamConfig& c, y3g_stream** out)
+{
+  if (!rt || !out) return fail(Y3G_ERR_ARG, "stream_open: bad arguments");
+  *out = nullptr;
+  if (c.ring_items < 2 || c.table_slots < 1 || !(c.idle_ms > 0) || !(c.table_wait_ms > 0))
+    return fail(Y3G_ERR_ARG, "stream_open: ring_items >= 2, table_slots >= 1 and positive "
+                             "idle / wait bounds are needed");
+  auto* s = new y3g_stream;
+  s->rt = rt;
+  s->cfg = c;
+  auto bail = [&](int rc) {
+    stream_free(s);
+    return rc;
+  };
+  if (int rc = stream_shape(rt, c, &s->workgroups, &s->slab_bytes, &s->slot_bytes))
+    return bail(rc);
+  if (int rc = alloc_tables(rt, &c.spec, c.table_slots)) return bail(rc);
+  if (int rc = ensure_slab(rt, s->workgroups, c.options.max_regions)) return bail(rc);
+  size_t const n = (size_t)c.ring_items;
+  if (halloc(&s->ring, n) != hipSuccess || halloc(&s->res, n) != hipSuccess ||
+      halloc(&s->seq, n) != hipSuccess || halloc(&s->ctl, 1) != hipSuccess ||
+      halloc(&s->hpoints, (size_t)c.table_slots) != hipSuccess)
+    return bail(fail(Y3G_ERR_MEMORY, "stream_open: pinned host memory"));
+  if (dalloc(&s->dev, 1) || dalloc(&s->ready, (size_t)c.table_slots))
+    return bail(Y3G_ERR_MEMORY);
+  if (hipMemset(s->dev, 0, sizeof(StreamDev)) != hipSuccess ||
+      hipMemset(s->ready, 0, sizeof(unsigned) * c.table_slots) != hipSuccess ||
+      hipEventCreateWithFlags(&s->done, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "stream_open: device set-up"));
+  if (hipDeviceSynchronize() != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "stream_open: device set-up"));
+  s->kind.assign(n, 0);
+  s->gen.assign(c.table_slots, 0u);
+  s->tables_pending.assign(c.table_slots, 0);
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, rt->device) != hipSuccess ||
+      khz <= 0)
+    return bail(fail(Y3G_ERR_HIP, "stream_open: wall-clock rate"));
+  KernArgs& A = s->A;
+  std::memset(&A, 0, sizeof(A));
+  A.points = rt->points;
+  A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
+  A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
+  A.sci32 = rt->sci32, A.sci64 = rt->sci64, A.sci_stride = rt->sci_stride;
+  A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
+  A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
+  A.slab_ax = rt->slab_ax;
+  A.cap = c.options.max_regions;
+  A.kmax = c.options.max_split_per_step;
+  A.init_split = c.options.init_split;
+  A.err_c = c.options.err_c, A.err_smin = c.options.err_smin;
+  A.tl_c = c.options.tl_c, A.tl_smin = c.options.tl_smin;
+  A.nevals = rt->d_nevals;
+  StreamArgs& Z = s->Z;
+  Z.ring = s->ring;
+  Z.res = s->res;
+  Z.seq = s->seq;
+  Z.ctl = s->ctl;
+  Z.hpoints = s->hpoints;
+  Z.dev = s->dev;
+  Z.ready = s->ready;
+  Z.points = rt->points;
+  Z.tab.points = rt->points;
+  Z.tab.spec = c.spec;
+  Z.tab.hmf64 = rt->hmf64, Z.tab.hmf32 = rt->hmf32, Z.tab.hmf_stride = rt->hmf_stride;
+  Z.tab.da64 = rt->da64, Z.tab.da32 = rt->da32, Z.tab.da_stride = rt->da_stride;
+  Z.tab.sci64 = rt->sci64, Z.tab.sci32 = rt->sci32, Z.tab.sci_stride = rt->sci_stride;
+  Z.ring_cap = (unsigned)c.ring_items;
+  Z.idle_ticks = (long long)(c.idle_ms * khz);
+  Z.wait_ticks = (long long)(c.table_wait_ms * khz);
+  if (hipMemset(rt->d_nevals, 0, 8) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "stream_open: device set-up"));
+  *out = s;
+  return 0;
+}
+
+size_t stream_room(const y3g_stream* s)
+{
+  return (size_t)s->cfg.ring_items - (size_t)(s->tail - s->low);
+}
+
+int stream_tables_pending(const y3g_stream* s, int slot)
+{
+  return slot >= 0 && slot < (int)s->tables_pending.size() ? s->tables_pending[slot] : 0;
+}
+
+// Make entries [tail, tail + n) visible to the launch.
+static void stream_publish_tail(y3g_stream* s, unsigned long long n)
+{
+  s->tail += n;
+  __atomic_store_n(&s->ctl->tail, s->tail, __ATOMIC_RELEASE);
+}
+
+int stream_tables(y3g_stream* s, int slot, const y3g_params& p, unsigned* gen)
+{
+  if (!s || !gen || slot < 0 || slot >= s->cfg.table_slots)
+    return fail(Y3G_ERR_ARG, "stream_tables: bad arguments");
+  if (stream_room(s) < 1) return fail(Y3G_ERR_STATE, "stream_tables: the ring is full");
+  if (s->tables_pending[slot])
+    return fail(Y3G_ERR_STATE, "stream_tables: the slot's last build has not finished");
+  s->hpoints[slot] = make_point(p, s->cfg.spec);
+  unsigned const g = ++s->gen[slot];
+  size_t const k = (size_t)(s->tail % (unsigned long long)s->cfg.ring_items);
+  StreamItem e;
+  std::memset(&e, 0, sizeof(e));
+  e.job = kStreamJobTables;
+  e.slot = slot;
+  e.gen = g;
+  s->ring[k] = e;
+  s->kind[k] = 2;
+  s->tables_pending[slot] += 1;
+  stream_publish_tail(s, 1);
+  *gen = g;
+  return 0;
+}
+
+int stream_push(y3g_stream* s, int slot, unsigned gen, const y3g_integral* igs, size_t n,
+                unsigned long long* first)
+{
+  if (!s || (!igs && n) || !first || slot < 0 || slot >= s->cfg.table_slots)
+    return fail(Y3G_ERR_ARG, "stream_push: bad arguments");
+  if (stream_room(s) < n) return fail(Y3G_ERR_STATE, "stream_push: the ring is full");
+  for (size_t i = 0; i < n; ++i) {
+    y3g_integral const& g = igs[i];
+    if (g.ndim < 2 || g.ndim > kMaxDim || g.kind < 0 || g.kind > Y3G_DSIGMA6D ||
+        g.algorithm != Y3G_PAGANI || !(g.eps_rel >= 0) || g.max_eval < 1)
+      return fail(Y3G_ERR_ARG, "stream_push: integral " + std::to_string(i) +
+                                 " is not a cubature integral of kinds 0-2");
+    for (int a = 0; a < g.ndim; ++a)
+      if (!(g.highs[a] > g.lows[a]))
+        return fail(Y3G_ERR_ARG, "stream_push: empty volume in integral " + std::to_string(i));
+  }
+  *first = s->tail;
+  for (size_t i = 0; i < n; ++i) {
+    size_t const k = (size_t)((s->tail + i) % (unsigned long long)s->cfg.ring_items);
+    StreamItem e;
+    std::memset(&e, 0, sizeof(e));
+    e.job = kStreamJobIntegral;
+    e.slot = slot;
+    e.gen = gen;
+    e.ig = igs[i];
+    e.ig.point = slot;
+    s->ring[k] = e;
+    s->kind[k] = 1;
+  }
+  stream_publish_tail(s, n);
+  return 0;
+}
+
+size_t stream_collect(y3g_stream* s, void (*f)(void*, unsigned long long, const y3g_result&),
+                      void* ctx)
+{
+  size_t got = 0;
+  unsigned long long const cap = (unsigned long long)s->cfg.ring_items;
+  for (unsigned long long i = s->low; i < s->tail; ++i) {
+    size_t const k = (size_t)(i % cap);
+    if (!s->kind[k]) continue;  // reported already
+    if (__atomic_load_n(&s->seq[k], __ATOMIC_ACQUIRE) != i + 1) continue;
+    y3g_result const r = s->res[k];
+    if (s->kind[k] == 2) {
+      s->tables_pending[s->ring[k].slot] -= 1;
+    } else {
+      f(ctx, i, r);
+      ++got;
+    }
+    s->kind[k] = 0;
+  }
+  while (s->low < s->tail && !s->kind[(size_t)(s->low % cap)]) ++s->low;
+  return got;
+}
+
+int stream_kick(y3g_stream* s)
+{
+  if (s->running) {
+    hipError_t const q = hipEventQuery(s->done);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) return fail(Y3G_ERR_HIP, std::string("stream launch: ") +
+                                                    hipGetErrorString(q));
+    s->running = false;
+    HIPCHK(hipMemcpy(&s->head_known, &s->dev->head, sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  }
+  if (s->tail <= s->head_known) return 0;
+  HIPCHK(hipSetDevice(s->rt->device));
+  HIPCHK(launch_cubature_stream(s->A, s->Z, s->cfg.options.precision, s->workgroups,
+                                s->rt->stream));
+  HIPCHK(hipEventRecord(s->done, s->rt->stream));
+  s->running = true;
+  s->launches += 1;
+  return 0;
+}
+
+long long stream_launches(const y3g_stream* s) { return s->launches; }
+
+int stream_close(y3g_stream* s, long long* nevals)
+{
+  if (!s) return 0;
+  __atomic_store_n(&s->ctl->close, 1u, __ATOMIC_RELEASE);
+  int rc = 0;
+  if (s->running && hipStreamSynchronize(s->rt->stream) != hipSuccess)
+    rc = fail(Y3G_ERR_HIP, "stream_close: the launch failed");
+  s->running = false;
+  unsigned long long nev = 0;
+  if (hipMemcpy(&nev, s->rt->d_nevals, sizeof(nev), hipMemcpyDeviceToHost) != hipSuccess &&
+      !rc)
+    rc = fail(Y3G_ERR_HIP, "stream_close: counters");
+  if (nevals) *nevals = (long long)nev;
+  stream_free(s);
+  return rc;
+}
+
+}  // namespace y3g
