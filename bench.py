@@ -421,7 +421,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=48)
     ap.add_argument("--cpu-sample", type=int, default=64)
-    ap.add_argument("--serve-rates", default="4,12,24")
+    ap.add_argument("--serve-rates", default="4,12,24,48,96")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--no-selgl", action="store_true")
     ap.add_argument("--selgl-only", action="store_true",

@@ -81,11 +81,14 @@ struct StreamCtl {
   unsigned close;           // host: exit once every published entry is claimed
   unsigned pad;
 };
-struct StreamDev {
-  unsigned long long head;  // entries claimed
-  unsigned long long tail;  // the device's copy of StreamCtl::tail
-  unsigned close;           // the device's copy of StreamCtl::close
-  unsigned lock;            // held by the workgroup refreshing the copies
+struct StreamDev {  // one 128-byte line per word that workgroups contend for
+  alignas(128) unsigned long long head;  // entries claimed
+  alignas(128) unsigned long long tail;  // the device's copy of StreamCtl::tail
+  alignas(128) unsigned close;           // the device's copy of StreamCtl::close
+  alignas(128) unsigned lock;            // held by the workgroup refreshing the copies
+  alignas(128) unsigned busy;            // workgroups claiming or working on an entry
+  alignas(128) long long last;           // wall clock of the last claim or finish
+  alignas(128) unsigned quit;            // set once the device has been idle: all exit
 };
 struct StreamArgs {
   const StreamItem* ring;       // [ring_cap], host

@@ -51,8 +51,8 @@ __device__ __forceinline__ double lin_knot(double a, double b, int n, int i)
 // The tables of one parameter point, by the nt threads of one workgroup:
 // HMF (n_z x n_lnM), D_A (n_da), Sigma_crit^-1 (n_sci), FP64 and FP32
 // copies.
-__device__ __attribute__((noinline)) void build_point_tables(const TableArgs& A, int pt,
-                                                             const PointDev& P, int tid, int nt)
+__device__ void build_point_tables(const TableArgs& A, int pt, const PointDev& P, int tid,
+                                   int nt)
 {
   y3g_table_spec const& s = A.spec;
   double* hmf64 = A.hmf64 + (size_t)pt * A.hmf_stride;
@@ -105,26 +105,54 @@ __global__ __launch_bounds__(256) void k_tables_build(TableArgs A)
 // the device's head counter), so no claim is ever held on an entry that
 // does not exist yet.  One idle workgroup at a time (a try-lock) copies the
 // host's tail and close flag into device memory, where the others read
-// them.  Every wait is bounded: a workgroup that finds no work for
-// idle_ticks of the constant-rate wall clock exits, and the host relaunches
-// when it publishes more; an integral whose tables are not ready within
-// wait_ticks returns status Y3G_STATUS_NO_TABLES.
+// them.  Every wait is bounded.  The launch ends as a whole: once no
+// workgroup has claimed or finished an entry for idle_ticks of the
+// constant-rate wall clock and none is busy, the first to notice sets
+// StreamDev::quit and every workgroup exits (a workgroup leaving on its own
+// would shrink the launch under light load); the host relaunches when it
+// publishes more.  An integral whose tables are not ready within wait_ticks
+// returns status Y3G_STATUS_NO_TABLES.
+//
+// The polling uses relaxed atomics of the right scope only (device memory
+// at agent scope, host memory at system scope): each such access is
+// coherent by itself, and none of them writes back or invalidates the L2,
+// which an acquire or release would do on every poll -- under the
+// workgroups that are integrating.  Ordering rests on dependencies: a
+// workgroup reads a ring entry (system-scope loads, which bypass the
+// caches) only after it has read a tail beyond it, and the host wrote the
+// entry before raising the tail.
 //
 // Entries are integrals or table builds.  A table build fills one table
-// slot from the PointDev the host wrote for it and then releases the slot's
-// generation number; integrals on that slot wait (acquire) for their
-// generation.  Builds are claimed before the integrals that follow them in
+// slot from the PointDev the host wrote for it, writes it back
+// (__threadfence) and then stores the slot's generation number; an integral
+// on that slot waits for its generation, then acquires once (agent scope)
+// before it reads the tables -- once per slot and generation per
+// workgroup.  Builds are claimed before the integrals that follow them in
 // the ring, by workgroups that are running, so the wait always ends.
 
-// Relaxed system-scope loads of pinned host memory, one 64-bit word at a
-// time (never through the scalar cache).
+template <typename W>
+__device__ __forceinline__ W ld_agent(const W* p)
+{
+  return __hip_atomic_load(const_cast<W*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename W>
+__device__ __forceinline__ W ld_system(const W* p)
+{
+  return __hip_atomic_load(const_cast<W*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Pinned host memory to dst, one 64-bit system-scope load at a time.
 __device__ __forceinline__ void load_host_words(void* dst, const void* src, int words)
 {
   auto* d = reinterpret_cast<unsigned long long*>(dst);
   auto* s = reinterpret_cast<const unsigned long long*>(src);
-  for (int k = 0; k < words; ++k)
-    d[k] = __hip_atomic_load(const_cast<unsigned long long*>(s + k), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_SYSTEM);
+  for (int k = 0; k < words; ++k) d[k] = ld_system(s + k);
+}
+
+// Sleep about nap x 8128 cycles.
+__device__ __forceinline__ void nap_for(int nap)
+{
+  for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(127);
 }
 
 // The next published entry, or -1 (closed and drained, or idle too long).
@@ -132,52 +160,84 @@ __device__ long long stream_claim(const StreamArgs& Z)
 {
   StreamDev* D = Z.dev;
   long long const t0 = wall_clock64();
+  int nap = 1;
   for (;;) {
-    unsigned long long h = __hip_atomic_load(&D->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long const t =
-      __hip_atomic_load(&D->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (ld_agent(&D->quit)) return -1;
+    unsigned long long h = ld_agent(&D->head);
+    unsigned long long const t = ld_agent(&D->tail);
     if (h < t) {
+      // busy from before the claim, so that no workgroup sees an idle device
+      // while a claim is under way
+      __hip_atomic_fetch_add(&D->busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (__hip_atomic_compare_exchange_strong(&D->head, &h, h + 1, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_fetch_max(&D->last, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         return (long long)h;
+      }
+      __hip_atomic_fetch_sub(&D->busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       continue;
     }
-    if (__hip_atomic_load(&D->close, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return -1;
+    if (ld_agent(&D->close)) return -1;
     unsigned z = 0u;
-    if (__hip_atomic_compare_exchange_strong(&D->lock, &z, 1u, __ATOMIC_ACQUIRE,
+    if (__hip_atomic_compare_exchange_strong(&D->lock, &z, 1u, __ATOMIC_RELAXED,
                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-      // close first: once it reads 1 the tail read after it is the final one
-      unsigned const cl =
-        __hip_atomic_load(&Z.ctl->close, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-      unsigned long long const th =
-        __hip_atomic_load(&Z.ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_fetch_max(&D->tail, th, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      if (cl) __hip_atomic_store(&D->close, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&D->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      if (th > h) continue;
+      // close first: the host raises it after its last tail
+      unsigned const cl = ld_system(&Z.ctl->close);
+      unsigned long long const th = ld_system(&Z.ctl->tail);
+      __hip_atomic_fetch_max(&D->tail, th, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cl) __hip_atomic_store(&D->close, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&D->lock, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (th > h) {
+        nap = 1;
+        continue;
+      }
     }
-    if (wall_clock64() - t0 > Z.idle_ticks) return -1;
-    __builtin_amdgcn_s_sleep(16);
+    long long const now = wall_clock64();
+    long long const last = ld_agent(&D->last);
+    if (ld_agent(&D->busy) == 0u && now - (last > t0 ? last : t0) > Z.idle_ticks) {
+      __hip_atomic_store(&D->quit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return -1;
+    }
+    nap_for(nap);
+    nap = nap < 8 ? 2 * nap : 8;
   }
 }
 
-// Wait (bounded) until the slot's tables are those of generation gen.
+// An entry is finished: the device was active now, one fewer busy.
+__device__ void stream_done(const StreamArgs& Z)
+{
+  __hip_atomic_fetch_max(&Z.dev->last, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_sub(&Z.dev->busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait (bounded) until the slot's tables are those of generation gen, then
+// acquire them.
 __device__ bool stream_wait_tables(const StreamArgs& Z, int slot, unsigned gen)
 {
   long long const t0 = wall_clock64();
-  while (__hip_atomic_load(Z.ready + slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+  while (ld_agent(Z.ready + slot) != gen) {
     if (wall_clock64() - t0 > Z.wait_ticks) return false;
-    __builtin_amdgcn_s_sleep(16);
+    __builtin_amdgcn_s_sleep(64);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
-// Write an entry's result, then release its sequence number (entry + 1).
+// Write an entry's result (system-scope stores to pinned host memory, past
+// the caches), wait until they are done, then store its sequence number
+// (entry + 1), which the host polls.
 __device__ void stream_publish(const StreamArgs& Z, long long item, const y3g_result& r)
 {
   size_t const k = (size_t)(item % (long long)Z.ring_cap);
-  Z.res[k] = r;
-  __hip_atomic_store(Z.seq + k, (unsigned long long)item + 1ull, __ATOMIC_RELEASE,
+  static_assert(sizeof(y3g_result) % 8 == 0, "results are written as 64-bit words");
+  auto const* w = reinterpret_cast<const unsigned long long*>(&r);
+  auto* d = reinterpret_cast<unsigned long long*>(Z.res + k);
+  for (int i = 0; i < (int)(sizeof(y3g_result) / 8); ++i)
+    __hip_atomic_store(d + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __builtin_amdgcn_s_waitcnt(0);  // every vector memory operation of the wave done
+  __hip_atomic_store(Z.seq + k, (unsigned long long)item + 1ull, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
@@ -242,21 +302,28 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
         __threadfence();
         __syncthreads();
         if (tid == 0) {
-          __hip_atomic_store(Z.ready + S.sslot, S.sgen, __ATOMIC_RELEASE,
+          __hip_atomic_store(Z.ready + S.sslot, S.sgen, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
           y3g_result r{};
           stream_publish(Z, S.sitem, r);
+          stream_done(Z);
           S.point = -1;  // S.P now holds this slot's constants: reload
         }
         continue;
       }
-      if (tid == 0) S.job = stream_wait_tables(Z, S.sslot, S.sgen) ? 0 : -1;
+      // tables this workgroup has staged already are ready
+      if (tid == 0)
+        S.job = (S.point == S.sslot && S.pgen == S.sgen) ||
+                    stream_wait_tables(Z, S.sslot, S.sgen)
+                  ? 0
+                  : -1;
       __syncthreads();
       if (S.job < 0) {
         if (tid == 0) {
           y3g_result r{};
           r.status = Y3G_STATUS_NO_TABLES;
           stream_publish(Z, S.sitem, r);
+          stream_done(Z);
         }
         continue;
       }
@@ -524,10 +591,12 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       res.neval = neval;
       res.nregions = N;
       res.status = status;
-      if constexpr (STREAM)
+      if constexpr (STREAM) {
         stream_publish(Z, S.sitem, res);
-      else
+        stream_done(Z);
+      } else {
         A.out[iid] = res;
+      }
       atomicAdd(A.nevals, (unsigned long long)neval);
     }
   }

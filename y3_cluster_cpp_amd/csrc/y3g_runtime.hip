@@ -1837,6 +1837,9 @@ int stream_kick(y3g_stream* s)
   }
   if (s->tail <= s->head_known) return 0;
   HIPCHK(hipSetDevice(s->rt->device));
+  // a new launch starts with no busy workgroup and without the quit flag
+  HIPCHK(hipMemsetAsync(&s->dev->busy, 0, sizeof(unsigned), s->rt->stream));
+  HIPCHK(hipMemsetAsync(&s->dev->quit, 0, sizeof(unsigned), s->rt->stream));
   HIPCHK(launch_cubature_stream(s->A, s->Z, s->cfg.options.precision, s->workgroups,
                                 s->rt->stream));
   HIPCHK(hipEventRecord(s->done, s->rt->stream));
