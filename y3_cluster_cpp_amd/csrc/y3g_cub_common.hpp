@@ -32,6 +32,18 @@ __device__ __forceinline__ void lds_barrier()
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// threadIdx.x behind an empty asm: the per-thread values derived from it
+// (LDS offsets, lane masks, pointer choices) are recomputed where they are
+// used, a few VALU operations, instead of being hoisted out of the loops,
+// which at the FP32 kernel's 80 VGPRs spills them to scratch and reloads
+// them from memory at every use.
+__device__ __forceinline__ int tid_here()
+{
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -520,7 +532,7 @@ __device__ __forceinline__ StageWords<BLOCK> fetch_parents(const int* sel, int p
   StageWords<BLOCK> out;
 #pragma unroll
   for (int k = 0; k < 512 / BLOCK; ++k) {
-    int const i = k * BLOCK + (int)threadIdx.x;
+    int const i = k * BLOCK + tid_here();
     const unsigned* src;
     if (i < 2 * kLoW) {
       int const half = i & 1, jd = (i % kLoW) >> 1;
@@ -698,9 +710,10 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
   // sums over nodes j = sub, sub + 16, ... reduced within the row by DPP; in
   // the FP32 path the row evaluates its region's nodes as it sums them
   constexpr int NG = BLOCK / 16;
-  int const sub = threadIdx.x & 15;
   for (int r0 = 0; r0 < m; r0 += NG) {
-    int const r = r0 + (threadIdx.x >> 4);
+    int const tt = tid_here();
+    int const sub = tt & 15;
+    int const r = r0 + (tt >> 4);
     bool const live = r < m;
     const T* fr = S.f + (live ? r : 0) * (sizeof(T) == 4 ? kFourNodes : nn);
     double a7 = 0.0, a5 = 0.0, a3 = 0.0;
@@ -744,7 +757,7 @@ __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double
     }
     double const bv = row16_max(four);
     unsigned long long const hit = __ballot(sub < d && four == bv);
-    unsigned const row = (unsigned)(hit >> (threadIdx.x & 48)) & 0xFFFFu;
+    unsigned const row = (unsigned)(hit >> (tt & 48)) & 0xFFFFu;
     int best = row ? __builtin_ctz(row) : 0;
     if (live && sub == 0) {
       double vol = 1.0;

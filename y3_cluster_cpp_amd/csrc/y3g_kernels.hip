@@ -474,7 +474,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       {
         int off0 = 0, off1 = 0;
         for (int v = 0; v < wv; ++v) off0 += S.ired[v], off1 += S.ired1[v];
-        unsigned long long const below = (1ull << lane) - 1ull;
+        unsigned long long const below = (1ull << (tid_here() & 63)) - 1ull;
         for (int base = sb; base < se && (off0 < lim0 || off1 < lim1); base += 64) {
           int const r = base + lane;
           bool f0 = false, f1 = false;
