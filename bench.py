@@ -373,10 +373,12 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
     worst, t_cpu = 0.0, 0.0
     for i in range(cpu_integrals):
         lows, highs, (zlo, zhi, R) = vols[i]
+        # as the device integrates it: lambda_obs in closed form over lo's range
         oi = O.Integral()
-        oi.kind, oi.ndim, oi.zo_lo, oi.zo_hi, oi.radius = 3, 6, zlo, zhi, R
-        for a in range(6):
-            oi.lows[a], oi.highs[a] = lows[a], highs[a]
+        oi.kind, oi.ndim, oi.zo_lo, oi.zo_hi, oi.radius = 3, 5, zlo, zhi, R
+        oi.lo_lo, oi.lo_hi = lows[0], highs[0]
+        for a in range(5):
+            oi.lows[a], oi.highs[a] = lows[a + 1], highs[a + 1]
         r = O.Result()
         t1 = time.perf_counter()
         O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(dd), C.byref(oi),

@@ -67,11 +67,13 @@ typedef enum {
   Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM)                               */
   Y3G_DSIGMA6D = 2, /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
   /* The reference's real-data (DES) integrands, in the volume's own units,
-   * with the point's DES data (y3g_des_upload); FP64 cubature and m-Cubes: */
-  Y3G_DSIGMA6D_DES = 3, /* (lo, lc, zt, lnM, rmis, theta)  SigmaMiscentY1Mort… */
+   * with the point's DES data (y3g_des_upload); FP64 cubature and m-Cubes.
+   * The miscentred two integrate lambda_obs in closed form over the
+   * descriptor's [lo_lo, lo_hi] (the reference volume's lo range): */
+  Y3G_DSIGMA6D_DES = 3, /* (lc, zt, lnM, rmis, theta)      SigmaMiscentY1Mort… */
   Y3G_DSIGMA3D_DES = 4, /* (lo, zt, lnM)                   SigmaCentY1Mort…    */
   Y3G_COUNT3D_DES = 5,  /* (lo, zt, lnM)                   NCCentY1Mort…       */
-  Y3G_COUNT5D_DES = 6,  /* (lo, lc, zt, lnM, rmis)         NCMiscentY1Mort…    */
+  Y3G_COUNT5D_DES = 6,  /* (lc, zt, lnM, rmis)             NCMiscentY1Mort…    */
   Y3G_COUNT4D_CARD = 7, /* (ln lo, ln lt, zt, lnM)         numberCountsCard:
                            MOR_DES_LOG_t and INT_LC_LT_DES_t2 tables       */
   /* The Costanzi-2026 P[X] operator integrands P_operator_gpu<W>
@@ -230,8 +232,11 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
  * rmis in Mpc/h (the offset pdf is integrated over [rmis_low, rmis_high]),
  * theta in radians inside [0, 2 pi] (with the 1/2pi of the angular average,
  * src/modules/sigma_mort_y1/sigma_miscent_y1_mort_cuda_scalarintegrand.cu:174-175).
- * The real-data kinds keep the volume as it is, grid (zo_low, zo_high
- * [, radius]); the P[X] operator kinds Y3G_POP_* take (lt, zt, lnm) and grid
+ * The real-data kinds take the reference module's volume, (lo, lc, zt, lnm,
+ * rmis, theta), (lo, zt, lnm), (lo, zt, lnm), (lo, lc, zt, lnm, rmis) and
+ * (ln lo, ln lt, zt, lnm), grid (zo_low, zo_high [, radius]); the
+ * miscentred two put its lo range in lo_lo / lo_hi and keep the other axes
+ * (lambda_obs in closed form).  The P[X] operator kinds Y3G_POP_* take (lt, zt, lnm) and grid
  * (zo_low, zo_high, lambda_bin), lambda_bin in 0..3 (the DES bins [20, 30),
  * [30, 45), [45, 60), [60, 200] of p_operator_gpu_t.cuh:55-60), and keep the
  * bin in lo_lo and its centre, the integrand's upper lt limit, in lo_hi.

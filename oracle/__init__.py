@@ -80,6 +80,7 @@ def _load():
         "y3o_gaussian": (_D, [_D, _D, _D]),
         "y3o_roffset_gamma": (_D, [_D, _D]),
         "y3o_lo_lc": (_D, [_D] * 7),
+        "y3o_lo_lc_bin": (_D, [_D] * 8),
         "y3o_sigma_photoz_des": (_D, [_D]),
         "y3o_zo_zt_des": (_D, [_D, _D]),
         "y3o_int_zo_zt_des": (_D, [_D, _D, _D]),

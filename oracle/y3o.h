@@ -100,6 +100,8 @@ double y3o_gaussian(double x, double mu, double sigma);
 double y3o_roffset_gamma(double x, double tau);
 double y3o_lo_lc(double lo, double lc, double R_mis, double alpha, double a,
                  double b, double R_lambda);
+double y3o_lo_lc_bin(double lo_lo, double lo_hi, double lc, double R_mis, double alpha,
+                     double a, double b, double R_lambda);
 double y3o_sigma_photoz_des(double zt);
 double y3o_zo_zt_des(double zo, double zt);
 double y3o_int_zo_zt_des(double zomin, double zomax, double zt);

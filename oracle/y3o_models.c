@@ -100,6 +100,24 @@ y3o_lo_lc(double lo, double lc, double R_mis, double alpha, double a, double b,
   return y3o_gaussian(y, mu_y, sigma_y) / lc;
 }
 
+static double half_erf_diff(double a, double b);
+
+/* lo_lc integrated over lo in [lo_lo, lo_hi]: y = lo / lc ~ N(mu_y, sigma_y)
+ * with dy = dlo / lc, so the integral is Phi((lo_hi/lc - mu_y)/sigma_y) -
+ * Phi((lo_lo/lc - mu_y)/sigma_y); sigma_y = 0 leaves the indicator of
+ * lo_lo <= mu_y lc < lo_hi. */
+double
+y3o_lo_lc_bin(double lo_lo, double lo_hi, double lc, double R_mis, double alpha, double a,
+              double b, double R_lambda)
+{
+  double const x = R_mis / R_lambda;
+  double const mu_y = exp(-x * x / alpha / alpha);
+  double const sigma_y = a * atan(b * x);
+  if (!(sigma_y > 0.0)) return (lo_lo <= mu_y * lc && mu_y * lc < lo_hi) ? 1.0 : 0.0;
+  double const s = Y3O_SQRT2 * sigma_y;
+  return half_erf_diff((lo_lo / lc - mu_y) / s, (lo_hi / lc - mu_y) / s);
+}
+
 /* sigma_photoz_des.hh: degree-10 Horner polynomial in (z - 0.4) on the
  * clamped range [0.15, 0.7].  The coefficients are the reference's fit. */
 double
@@ -1020,8 +1038,11 @@ y3o_integrand_mapped_batch(const y3o_params* p, const y3o_tables* t,
     out[i] = y3o_integrand_mapped(p, t, ig, y + i * ig->ndim);
 }
 
-/* The reference's real-data integrands in the volume's own variables,
- * kinds as the device's (3..6):
+/* The reference's real-data integrands, kinds as the device's (3..6).  For
+ * the miscentred kinds 3 and 6, ndim selects the variables: the reference's
+ * own (6 / 5, with lo) or, as the device integrates them, lambda_obs in
+ * closed form over [ig->lo_lo, ig->lo_hi] (y3o_lo_lc_bin; 5 / 4, without
+ * lo):
  *   3 SigmaMiscentY1MortCUDAScalarIntegrand::operator() (lo, lc, zt, lnM, rmis,
  *     theta), sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182;
  *   4 SigmaCentY1MortCUDAScalarIntegrand::operator() (lo, zt, lnM),
@@ -1039,9 +1060,14 @@ y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
 {
   switch (ig->kind) {
     case 3: {
-      double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4], th = x[5];
+      int const red = ig->ndim == 5;  /* lambda_obs in closed form */
+      double const lo = red ? 0.0 : x[0];
+      x += red ? 0 : 1;
+      double const lc = x[0], zt = x[1], lnM = x[2], rmis = x[3], th = x[4];
       double const roffset_v = y3o_roffset_gamma(rmis, des[0]);
-      double const lo_lc_v = y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]);
+      double const lo_lc_v =
+        red ? y3o_lo_lc_bin(ig->lo_lo, ig->lo_hi, lc, rmis, des[1], des[2], des[3], des[4])
+            : y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]);
       double const mor_v = y3o_mor_des(des + 5, lc, lnM, zt);
       double const common = roffset_v * lo_lc_v * mor_v * y3o_dvdz(p, t, zt) *
                             y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt) / 2.0 /
@@ -1074,9 +1100,14 @@ y3o_integrand_des(const y3o_params* p, const y3o_tables* t, const double* des,
     case 11: /* the P[X] operator integrands (y3o_pop.c) */
       return y3o_integrand_pop(p, t, des, ig, x);
     case 6: {
-      double const lo = x[0], lc = x[1], zt = x[2], lnM = x[3], rmis = x[4];
-      double const common = y3o_roffset_gamma(rmis, des[0]) *
-                            y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]) *
+      int const red = ig->ndim == 4;  /* lambda_obs in closed form */
+      double const lo = red ? 0.0 : x[0];
+      x += red ? 0 : 1;
+      double const lc = x[0], zt = x[1], lnM = x[2], rmis = x[3];
+      double const lo_lc_v =
+        red ? y3o_lo_lc_bin(ig->lo_lo, ig->lo_hi, lc, rmis, des[1], des[2], des[3], des[4])
+            : y3o_lo_lc(lo, lc, rmis, des[1], des[2], des[3], des[4]);
+      double const common = y3o_roffset_gamma(rmis, des[0]) * lo_lc_v *
                             y3o_mor_des(des + 5, lc, lnM, zt) * y3o_dvdz(p, t, zt) *
                             y3o_hmf(p, t, lnM, zt) * y3o_omega_z_des(zt);
       return y3o_int_zo_zt_des(ig->zo_lo, ig->zo_hi, zt) * common;

@@ -128,8 +128,8 @@ def test_des_integrand_device_matches_oracle(runtime):
     C, ig, des, op, ot, oig = _des_setup(runtime)
     rng = np.random.default_rng(61)
     n = 2000
-    from test_oracle_des import DES_BOX
-    x = np.column_stack([rng.uniform(l, h, n) for l, h in DES_BOX])
+    from test_oracle_des import DES_BOX, des_axes
+    x = np.column_stack([rng.uniform(l, h, n) for l, h in des_axes(DES_BOX)])
     got = runtime.eval_functor(y3.F_INTEGRAND, x, aux=ig)
     ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t), O.ptr(des),
                                             C.byref(oig), O.ptr(np.ascontiguousarray(row)))
@@ -270,7 +270,7 @@ def test_des_other_modules(runtime, label, kind, names, axes, grid):
     descriptor and the oracle's cubature, within the errors."""
     import ctypes as C
     from y3_cluster_cpp_amd import cosmosis as cs
-    from test_oracle_des import DES_BOX, SIG_M, SIG_R, SIG_Z, des_data, des_integral
+    from test_oracle_des import DES_BOX, SIG_M, SIG_R, SIG_Z, des_axes, des_data, des_integral
     box = [DES_BOX[a] for a in axes]
     des = des_data()
     nr, nm, nz = len(SIG_R), len(SIG_M), len(SIG_Z)
@@ -289,10 +289,9 @@ def test_des_other_modules(runtime, label, kind, names, axes, grid):
                                  eps_rel=1e-3)
     op = O.default_params(omega_m=0.3)
     ot = O.OracleTables(op, O.spec_wide())
-    oig = des_integral(box, zo=grid[:2], radius=grid[2] if sigma else 1.0)
-    oig.kind, oig.ndim = kind, len(box)
+    oig = des_integral(box, zo=grid[:2], radius=grid[2] if sigma else 1.0, kind=kind)
     rng = np.random.default_rng(70 + kind)
-    x = np.column_stack([rng.uniform(l, h, 1500) for l, h in box])
+    x = np.column_stack([rng.uniform(l, h, 1500) for l, h in des_axes(box, kind)])
     got = runtime.eval_functor(y3.F_INTEGRAND, x, aux=ig)
     ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t),
                                             O.ptr(np.ascontiguousarray(up)), C.byref(oig),
@@ -370,8 +369,7 @@ def test_card_module_and_integrand(runtime, monkeypatch):
                                  [b[1] for b in box], (0.3, 0.45), eps_rel=1e-3)
     op = O.default_params(omega_m=0.3)
     ot = O.OracleTables(op, O.spec_wide())
-    oig = des_integral(box)
-    oig.kind, oig.ndim = 7, 4
+    oig = des_integral(box, kind=7)
     xs = np.column_stack([rng.uniform(l, h, 1500) for l, h in box])
     got = runtime.eval_functor(y3.F_INTEGRAND, xs, aux=ig)
     ref = np.array([O.lib.y3o_integrand_des(C.byref(op), C.byref(ot.t), O.ptr(des), C.byref(oig),

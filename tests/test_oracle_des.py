@@ -242,11 +242,23 @@ def des_data(sig_const=None):
     return np.ascontiguousarray(np.concatenate([head, sp]))
 
 
-def des_integral(box, zo=(0.3, 0.45), radius=1.0):
+def des_axes(box, kind=3, reduced=True):
+    """The integrated axes of a real-data integral over the reference volume
+    `box`: the miscentred kinds (3, 6) as the device integrates them
+    (reduced: lambda_obs in closed form, lo dropped) or all of them."""
+    return box[1:] if reduced and kind in (3, 6) else box
+
+
+def des_integral(box, zo=(0.3, 0.45), radius=1.0, kind=3, reduced=True):
+    """An oracle descriptor over the reference volume `box`; reduced
+    miscentred kinds keep lo's range in lo_lo / lo_hi."""
     ig = O.Integral()
-    ig.kind, ig.ndim = 3, 6
+    axes = des_axes(box, kind, reduced)
+    ig.kind, ig.ndim = kind, len(axes)
+    if len(axes) < len(box):
+        ig.lo_lo, ig.lo_hi = box[0]
     ig.zo_lo, ig.zo_hi, ig.radius = zo[0], zo[1], radius
-    for a, (l, h) in enumerate(box):
+    for a, (l, h) in enumerate(axes):
         ig.lows[a], ig.highs[a] = l, h
     return ig
 
@@ -255,18 +267,20 @@ DES_BOX = [(0.0, 600.0), (5.0, 150.0), (0.25, 0.5), (np.log(1.3e14), np.log(4e14
            (0.2, 1.5), (0.0, 2 * np.pi)]
 
 
-def test_des_integrand_reduces_to_the_count_integral():
-    """With a constant SIG_MAX the 6-D real-data integral over a box that
-    holds the lo mass factorises: lo_lc integrates to 1 over lo, the angle
-    gives 2 pi / (2 x 3.1415926535897), the Gamma(2) offset integrates in
-    closed form over [r1, r2], and what is left is
+@pytest.mark.parametrize("reduced", [False, True])
+def test_des_integrand_reduces_to_the_count_integral(reduced):
+    """With a constant SIG_MAX the real-data integral over a box that holds
+    the lo mass factorises: lo_lc integrates to 1 over lo, the angle gives
+    2 pi / (2 x 3.1415926535897), the Gamma(2) offset integrates in closed
+    form over [r1, r2], and what is left is
     int dzt dlnM dV/dz hmf Omega int_zo_zt P_MOR(lc box | lnM, zt) sigma,
     here by Gauss-Legendre products of the separately pinned functors.  The
-    oracle's cubature of the 6-D integrand must agree within its error."""
+    oracle's cubature must agree within its error, of the reference's 6-D
+    integrand and of the 5-D one with lambda_obs in closed form."""
     p = O.default_params()
     t = O.OracleTables(p, O.spec_wide())
     des = des_data(sig_const=2.5)
-    ig = des_integral(DES_BOX)
+    ig = des_integral(DES_BOX, reduced=reduced)
     r = O.Result()
     O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
                            C.byref(O.cub_opts(eps_rel=1e-4, max_regions=40_000,
@@ -307,7 +321,7 @@ def test_des_integrand_offset_radius_geometry():
     o = 11 + 3 + nr + nm + nz
     des[o + nm * nr:o + nm * nr + nz * nr] = 0.0          # sigma_2 = 0
     for R, rm, lnM in ((1.0, 0.5, 33.0), (3.0, 0.8, 32.2), (0.7, 0.6, 34.1)):
-        ig = des_integral(DES_BOX, radius=R)
+        ig = des_integral(DES_BOX, radius=R, reduced=False)
         x = np.array([45.0, 40.0, 0.33, lnM, rm, 0.0])
 
         def f(th):
@@ -324,6 +338,63 @@ def test_des_integrand_offset_radius_geometry():
         assert abs(got / exp - 1) < 1e-10, (R, rm, got, exp)
 
 
+def test_lo_lc_bin_is_the_lo_integral_of_lo_lc():
+    """lo_lc integrated over a lambda_obs bin in closed form against
+    adaptive quadrature of lo_lc itself, on both sides of the ridge
+    lo = mu lc and at small offsets (narrow ridge); a bin holding the whole
+    Gaussian has unit mass; R = 0 is the indicator of mu lc in the bin."""
+    from scipy import integrate
+    par = (3.0, 0.3, 1.0, 0.9)                      # alpha, a, b, R_lambda
+    rng = np.random.default_rng(17)
+    for _ in range(60):
+        lc = rng.uniform(5.0, 150.0)
+        R = 10 ** rng.uniform(-3.0, 0.3)
+        lo_lo = rng.uniform(0.5, 1.2) * lc
+        lo_hi = lo_lo + rng.uniform(0.02, 0.5) * lc
+        x = R / par[3]
+        mu, sg = np.exp(-x * x / par[0] ** 2), par[1] * np.arctan(par[2] * x)
+        pts = [p_ for p_ in (mu * lc - 3 * sg * lc, mu * lc, mu * lc + 3 * sg * lc)
+               if lo_lo < p_ < lo_hi]
+        ref = integrate.quad(lambda lo: O.lib.y3o_lo_lc(lo, lc, R, *par), lo_lo, lo_hi,
+                             points=pts or None, epsabs=1e-300, epsrel=1e-12, limit=400)[0]
+        got = O.lib.y3o_lo_lc_bin(lo_lo, lo_hi, lc, R, *par)
+        assert abs(got - ref) <= 1e-10 * max(ref, 1e-12) + 1e-15, (lc, R, lo_lo, lo_hi, got, ref)
+    from scipy.stats import norm
+    x = 0.5 / par[3]
+    mu, sg = np.exp(-x * x / par[0] ** 2), par[1] * np.arctan(par[2] * x)
+    assert O.lib.y3o_lo_lc_bin(0.0, 1e4, 40.0, 0.5, *par) == pytest.approx(
+        norm.sf(0.0, mu, sg) - norm.sf(1e4 / 40.0, mu, sg), abs=1e-15)   # 1 - 7.8e-11
+    assert O.lib.y3o_lo_lc_bin(35.0, 45.0, 40.0, 0.0, *par) == 1.0
+    assert O.lib.y3o_lo_lc_bin(20.0, 30.0, 40.0, 0.0, *par) == 0.0
+
+
+@pytest.mark.parametrize("kind", [3, 6])
+def test_reduced_integrand_is_the_lo_integral(kind):
+    """The miscentred real-data integrands as the device integrates them
+    (lambda_obs in closed form, descriptor without lo) equal the reference
+    integrand (with lo) integrated over the lo range by adaptive quadrature,
+    at random points of the other axes."""
+    from scipy import integrate
+    p = O.default_params()
+    t = O.OracleTables(p, O.spec_wide())
+    des = des_data()
+    box = [(20.0, 30.0)] + (DES_BOX[1:] if kind == 3 else DES_BOX[1:5])
+    red = des_integral(box, radius=1.0, kind=kind)
+    full = des_integral(box, radius=1.0, kind=kind, reduced=False)
+    rng = np.random.default_rng(40 + kind)
+    for _ in range(25):
+        y = np.array([rng.uniform(l, h) for l, h in box[1:]])
+        got = O.lib.y3o_integrand_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(red),
+                                      O.ptr(np.ascontiguousarray(y)))
+
+        def f(lo):
+            x = np.ascontiguousarray(np.concatenate([[lo], y]))
+            return O.lib.y3o_integrand_des(C.byref(p), C.byref(t.t), O.ptr(des),
+                                           C.byref(full), O.ptr(x))
+        ref = integrate.quad(f, *box[0], epsabs=1e-300, epsrel=1e-12, limit=400)[0]
+        assert abs(got - ref) <= 1e-9 * abs(ref) + 1e-300, (y, got, ref)
+
+
 @pytest.mark.parametrize("kind", [4, 5, 6])
 def test_des_cent_and_count_integrands_reduce(kind):
     """The other real-data integrands reduce the same way: the centred
@@ -336,8 +407,7 @@ def test_des_cent_and_count_integrands_reduce(kind):
     des = des_data(sig_const=2.5)
     box = {4: [DES_BOX[1], DES_BOX[2], DES_BOX[3]], 5: [DES_BOX[1], DES_BOX[2], DES_BOX[3]],
            6: DES_BOX[:5]}[kind]
-    ig = des_integral(box, radius=1.0)
-    ig.kind, ig.ndim = kind, len(box)
+    ig = des_integral(box, radius=1.0, kind=kind, reduced=False)
     r = O.Result()
     O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
                            C.byref(O.cub_opts(eps_rel=1e-5, max_regions=40_000,
@@ -412,8 +482,7 @@ def test_card_integrand_reduces():
     t = O.OracleTables(p, O.spec_wide())
     des = card_data(0.7)
     box = [(np.log(22.0), np.log(28.0)), (np.log(5.0), np.log(150.0)), DES_BOX[2], DES_BOX[3]]
-    ig = des_integral(box)
-    ig.kind, ig.ndim = 7, 4
+    ig = des_integral(box, kind=7)
     r = O.Result()
     O.lib.y3o_cubature_des(C.byref(p), C.byref(t.t), O.ptr(des), C.byref(ig),
                            C.byref(O.cub_opts(eps_rel=1e-6, max_regions=40_000,

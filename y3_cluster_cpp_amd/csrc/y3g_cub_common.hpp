@@ -456,7 +456,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   MFac<T>* mf = reinterpret_cast<MFac<T>*>(zf + kChunk * kMaxAxisOffsets);
   int const nzo = S.nzo, nmo = S.nmo;
   // all zt tasks, then all lnM tasks: a wave takes one kind (but at the seam)
-  for (int q = threadIdx.x; q < m * (nzo + nmo); q += BLOCK) {
+  for (int q = tid_here(); q < m * (nzo + nmo); q += BLOCK) {
     if (q < m * nzo) {
       int const r = div_by(q, S.dv_zo), k = q - r * nzo;
       zf[r * kMaxAxisOffsets + k] =
@@ -472,7 +472,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   int const np = S.npairs, nt = S.nprofs;
   int const ntp = S.ig.kind == Y3G_DSIGMA6D ? nt : 0;  // profiles
   NodeConstF const kc = node_const_f(S.ig, S.P, S.nfwtab);
-  for (int q = threadIdx.x; q < m * (np + ntp); q += BLOCK) {
+  for (int q = tid_here(); q < m * (np + ntp); q += BLOCK) {
     if (q < m * np) {
       int const r = div_by(q, S.dv_np), k = q - r * np;
       S.pc[r][k] = pair_c(pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
@@ -497,7 +497,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   float* Q = reinterpret_cast<float*>(S.zmbuf);
   int const nq = S.nquads;
   int const nz2 = S.ig.kind == Y3G_COUNT5D ? nt : 0;
-  for (int q = threadIdx.x; q < m * (nq + nz2); q += BLOCK) {
+  for (int q = tid_here(); q < m * (nq + nz2); q += BLOCK) {
     if (q < m * nq) {
       int const r = div_by(q, S.dv_nq), k = q - r * nq;
       float const lo = float(S.c[r][0]) + float(S.h[r][0]) * S.qo0[k];
@@ -654,7 +654,7 @@ __device__ void eval_nodes_direct(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
   constexpr int d = KindAxes<K>::nd;
   constexpr int nn = gm_num_nodes(d);
-  for (int q = threadIdx.x; q < m * nn; q += BLOCK) {
+  for (int q = tid_here(); q < m * nn; q += BLOCK) {
     int const r = q / nn, j = q - r * nn;
     double y[d];
 #pragma unroll

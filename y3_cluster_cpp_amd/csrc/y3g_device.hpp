@@ -172,6 +172,21 @@ template <typename T> Y3G_HD T half_erf_diff(T a, T b)
   return T(0.5) * (M<T>::erf(b) - M<T>::erf(a));
 }
 
+// lo_lc integrated over a lambda_obs bin [lo_lo, lo_hi] in closed form: with
+// y = lo / lc ~ N(mu_y, sigma_y) and dy = dlo / lc,
+//   int lo_lc dlo = Phi((lo_hi / lc - mu_y) / sigma_y) - Phi((lo_lo / lc - mu_y) / sigma_y);
+// sigma_y = 0 (R = 0) leaves the indicator of lo_lo <= mu_y lc < lo_hi.
+template <typename T>
+Y3G_HD T lo_lc_bin(T lo_lo, T lo_hi, T lc, T R, T alpha, T a, T b, T R_lambda)
+{
+  T const x = R / R_lambda;
+  T const mu_y = M<T>::exp(-x * x / alpha / alpha);
+  T const sigma_y = a * M<T>::atan(b * x);
+  if (!(sigma_y > T(0))) return (lo_lo <= mu_y * lc && mu_y * lc < lo_hi) ? T(1) : T(0);
+  T const s = T(1.4142135623730951) * sigma_y;
+  return half_erf_diff<T>((lo_lo / lc - mu_y) / s, (lo_hi / lc - mu_y) / s);
+}
+
 // int_zo_zt_des_t.cuh:11-17 with the DES photo-z polynomial.
 template <typename T> Y3G_HD T int_zo_zt_des(T zomin, T zomax, T zt)
 {
@@ -505,8 +520,10 @@ template <> struct KindAxes<Y3G_COUNT5D> {
 template <> struct KindAxes<Y3G_DSIGMA6D> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 6;
 };
+// the miscentred real-data kinds integrate lambda_obs in closed form
+// (lo_lc_bin): (lc, zt, lnM, rmis, theta) and (lc, zt, lnM, rmis)
 template <> struct KindAxes<Y3G_DSIGMA6D_DES> {
-  static constexpr int it = 1, iz = 2, im = 3, nd = 6;
+  static constexpr int it = 0, iz = 1, im = 2, nd = 5;
 };
 template <> struct KindAxes<Y3G_DSIGMA3D_DES> {
   static constexpr int it = 0, iz = 1, im = 2, nd = 3;
@@ -515,7 +532,7 @@ template <> struct KindAxes<Y3G_COUNT3D_DES> {
   static constexpr int it = 0, iz = 1, im = 2, nd = 3;
 };
 template <> struct KindAxes<Y3G_COUNT5D_DES> {
-  static constexpr int it = 1, iz = 2, im = 3, nd = 5;
+  static constexpr int it = 0, iz = 1, im = 2, nd = 4;
 };
 template <> struct KindAxes<Y3G_COUNT4D_CARD> {
   static constexpr int it = 1, iz = 2, im = 3, nd = 4;
@@ -1337,20 +1354,27 @@ __device__ inline double sig_max(const double* par, double r, double lnM, double
 // tables as packed for sig_max()}.
 constexpr int kDesLoLc = 1, kDesMor = 5, kDesSig = 11;
 
-// The reference's real-data integrands, FP64, in the volume's own variables
-// (each the operator() of its module):
-//   DSIGMA6D_DES  (lo, lc, zt, lnM, rmis, theta)
-//     roffset(rmis) lo_lc(lo, lc, rmis) mor_des(lc, lnM, zt) dV/dz hmf Omega
+// The reference's real-data integrands, FP64 (each the operator() of its
+// module), in the volume's own variables except that the two miscentred ones
+// take the lambda_obs integral in closed form (lo_lc_bin over the volume's
+// lo range, kept in ig.lo_lo / ig.lo_hi): lo enters them only through
+// lo_lc, a Gaussian in lo / lc whose width vanishes as rmis -> 0, so left
+// to the cubature it is a ridge along lo = mu lc that no axis-aligned split
+// resolves.
+//   DSIGMA6D_DES  (lc, zt, lnM, rmis, theta)
+//     roffset(rmis) lo_lc_bin(lc, rmis) mor_des(lc, lnM, zt) dV/dz hmf Omega
 //     / 2 / 3.1415926535897 (the reference's constant) x sig_max(R', lnM, zt)
 //     int_zo_zt(zo_lo, zo_hi, zt), R' = sqrt(R^2 + rmis^2 + 2 rmis R cos theta)
-//     (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182);
+//     (sigma_miscent_y1_mort_cuda_scalarintegrand.cu:161-182, its lo axis
+//     integrated);
 //   DSIGMA3D_DES  (lo, zt, lnM): mor_des(lo, lnM, zt) dV/dz hmf Omega
 //     sig_max(R, lnM, zt) int_zo_zt (sigma_cent_y1_mort_cuda_scalarintegrand.cu
 //     :155-166);
 //   COUNT3D_DES   (lo, zt, lnM): mor_des(lo, lnM, zt) dV/dz hmf Omega int_zo_zt
 //     (NC_cent_y1_mort_cuda_scalarintegrand.cu:165-175);
-//   COUNT5D_DES   (lo, lc, zt, lnM, rmis): roffset lo_lc mor_des(lc, ...) dV/dz
-//     hmf Omega int_zo_zt (NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164).
+//   COUNT5D_DES   (lc, zt, lnM, rmis): roffset lo_lc_bin mor_des(lc, ...) dV/dz
+//     hmf Omega int_zo_zt (NC_miscent_y1_mort_cuda_scalarintegrand.cu:153-164,
+//     its lo axis integrated).
 //   COUNT4D_CARD  (ln lo, ln lt, zt, lnM): int_lc_lt(lo, lt, zt) int_zo_zt
 //     Omega dV/dz hmf mor_des_log(lt, lnM, zt), lo = e^(ln lo), lt = e^(ln lt)
 //     (gt_card_gpu/numberCounts.cu:160-174).
@@ -1520,20 +1544,20 @@ __device__ __noinline__ double integrand_des_k(const y3g_integral& ig, const Poi
   }
   double const zt = y[iz], lnM = y[im];
   constexpr bool miscent = K == Y3G_DSIGMA6D_DES || K == Y3G_COUNT5D_DES;
-  double const lam = miscent ? y[1] : y[0];  // the MOR_DES richness argument
+  double const lam = y[0];  // the MOR_DES richness argument: lc, or lo (centred)
   double common = mor_des(des + kDesMor, lam, lnM, zt, mt) * dv_do_dz<double>(P, tb.da, zt) *
                   hmf<double>(P, tb.hmf, lnM, zt) * omega_z_des(zt);
   if constexpr (miscent) {
-    double const rmis = y[4];
+    double const rmis = y[3];
     common = roffset_gamma<double>(rmis, des[0]) *
-             lo_lc<double>(y[0], y[1], rmis, des[kDesLoLc], des[kDesLoLc + 1],
-                           des[kDesLoLc + 2], des[kDesLoLc + 3]) *
+             lo_lc_bin<double>(ig.lo_lo, ig.lo_hi, y[0], rmis, des[kDesLoLc],
+                               des[kDesLoLc + 1], des[kDesLoLc + 2], des[kDesLoLc + 3]) *
              common;
   }
   double const izo = int_zo_zt_des<double>(ig.zo_lo, ig.zo_hi, zt);
   if constexpr (K == Y3G_DSIGMA6D_DES) {
-    double const R = ig.radius, rmis = y[4];
-    double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(y[5]));
+    double const R = ig.radius, rmis = y[3];
+    double const rs = sqrt(R * R + rmis * rmis + 2 * rmis * R * cos(y[4]));
     return sig_max(des + kDesSig, rs, lnM, zt) * izo * (common / 2.0 / 3.1415926535897);
   } else if constexpr (K == Y3G_DSIGMA3D_DES) {
     return sig_max(des + kDesSig, ig.radius, lnM, zt) * izo * common;

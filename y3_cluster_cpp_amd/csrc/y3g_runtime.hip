@@ -244,10 +244,16 @@ static int ensure_des(y3g_runtime* rt)
   return 0;
 }
 
+// Dimensions of a real-data integral: the descriptor's (the miscentred kinds
+// integrate lambda_obs in closed form) and the reference volume's.
 static int des_ndim(int kind)
 {
-  return kind == Y3G_DSIGMA6D_DES ? 6 : kind == Y3G_COUNT5D_DES ? 5
+  return kind == Y3G_DSIGMA6D_DES ? 5 : kind == Y3G_COUNT5D_DES ? 4
        : kind == Y3G_COUNT4D_CARD ? 4 : 3;
+}
+static int des_volume_ndim(int kind)
+{
+  return kind == Y3G_DSIGMA6D_DES ? 6 : kind == Y3G_COUNT5D_DES ? 5 : des_ndim(kind);
 }
 
 // The P[X] operator's point data packed as integrand_pop_k() reads them:
@@ -456,9 +462,10 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
     return 0;
   }
   if (is_des_kind(kind)) {
-    // the reference's volume as it is and its grid point: (zo_low, zo_high,
-    // radius) for Delta-Sigma, (zo_low, zo_high) for the counts
-    int const nd = des_ndim(kind);
+    // the reference's volume and its grid point: (zo_low, zo_high, radius)
+    // for Delta-Sigma, (zo_low, zo_high) for the counts; the miscentred
+    // kinds' lo axis becomes the closed-form lambda_obs bin (lo_lo, lo_hi)
+    int const nd = des_volume_ndim(kind);
     bool const sig = kind == Y3G_DSIGMA6D_DES || kind == Y3G_DSIGMA3D_DES;
     int const ng = sig ? 3 : 2;
     if (n_grid != ng || !grid)
@@ -481,11 +488,13 @@ int y3g_make_integral_volume(const y3g_params* p, const y3g_table_spec* s,
                                "(and radius > 0)");
     std::memset(ig, 0, sizeof(*ig));
     ig->kind = kind;
-    ig->ndim = nd;
+    ig->ndim = des_ndim(kind);
     ig->zo_lo = grid[0];
     ig->zo_hi = grid[1];
     ig->radius = sig ? grid[2] : 0.0;
-    for (int i = 0; i < nd; ++i) ig->lows[i] = lows[i], ig->highs[i] = highs[i];
+    int const skip = miscent ? 1 : 0;
+    if (miscent) ig->lo_lo = lows[0], ig->lo_hi = highs[0];
+    for (int i = skip; i < nd; ++i) ig->lows[i - skip] = lows[i], ig->highs[i - skip] = highs[i];
     ig->eps_rel = 1e-4;
     ig->eps_abs = 1e-300;
     ig->max_eval = 1LL << 40;
