@@ -72,6 +72,14 @@ TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_cubature<float,5
                   "evaluations")
 DEFAULT_REGIONS = 32768
 STREAM_POINTS = 4096
+# Algorithmic FP64 operations per evaluation of the real-data miscentred
+# Delta-Sigma integrand (lambda_obs in closed form), tallied as for the FP32
+# ones: roffset 3, the lambda_obs bin 17, MOR_DES 40 (two bilinear table
+# reads, the skew normal), dV/dz 17, HMF 14, Omega(z) 10, the z_obs bin 26,
+# R' and SIG_MAX 40 (three bilinear reads, the max), products 10; and 13
+# transcendental calls (exp x3, atan, erf / erfc x4, sqrt x2, cos, ...).
+DES_FLOPS_PER_EVAL = 177
+DES_SFU_PER_EVAL = 13
 # FP64 operations per term of k_selgl (HMF bilinear 10, selection bilinear
 # 13, the product and the sum 4), from csrc/y3g_kernels.hip.
 SELGL_FLOPS_PER_TERM = 27
@@ -364,6 +372,13 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
            "value": len(igs) / wall, "unit": "integrals/s", "kernel_ms": ms,
            "evals_per_s": nev / (ms * 1e-3) if ms > 0 else 0.0,
            "converged": int((res["status"] == 0).sum()), "integrals": len(igs)}
+    if ms > 0:
+        ach = nev * DES_FLOPS_PER_EVAL / (ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP64 / 1e12,
+                           "unit": "TFLOP/s", "frac": ach / (PEAK_FP64 / 1e12),
+                           "traffic": None, "kernel": "k_cubature<double,512>",
+                           "flops_per_eval": DES_FLOPS_PER_EVAL,
+                           "sfu_per_eval": DES_SFU_PER_EVAL}
     # the oracle on the first integrals (point 0): parity within the errors
     pt = pts[0]
     op = O.default_params(omega_m=pt[0], hmf_amp=pt[1], mor_slope=pt[2], mor_sigma_ln=pt[3],
