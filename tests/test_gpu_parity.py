@@ -398,3 +398,30 @@ def test_tolerance_bands(runtime):
         runtime.integrate(igs, y3.options(bands=[(1e-4, 2048), (1e-5, 4096)], max_regions=16384))
     with pytest.raises(RuntimeError, match="tolerance bands"):
         runtime.integrate(igs, y3.options(bands=[(1e-4, 32768)], max_regions=16384))
+
+
+def test_vegas_dsigma6d_and_counts(runtime):
+    """m-Cubes on the 6-D miscentred Delta-Sigma kind (one radius of each of
+    config 2's 12 bins) and config 1's 12 counts, FP32 integrand, eps 1e-3:
+    every reported error (3 sigma, inflated by chi2/dof) bounds the true
+    error against the golden values, converged results meet their
+    tolerance, and a second run repeats the values to the rounding of the
+    FP64 histogram atomics."""
+    p, s = fid(runtime)
+    g1 = np.array(G["config1"]["values"]).reshape(-1)
+    g2 = np.array(G["config2"]["values"])
+    igs = [y3.make_integral(p, s, y3.COUNT5D, LO[i], LO[i + 1], ZO[j], ZO[j + 1], eps_rel=1e-3,
+                            algorithm=y3.MCUBES) for i in range(4) for j in range(3)]
+    igs += [y3.make_integral(p, s, y3.DSIGMA6D, LO[i], LO[i + 1], ZO[j], ZO[j + 1],
+                             RADII[(3 * i + j) % 15], eps_rel=1e-3, algorithm=y3.MCUBES)
+            for i in range(4) for j in range(3)]
+    gold = np.concatenate([g1, [g2[i, j, (3 * i + j) % 15] for i in range(4) for j in range(3)]])
+    o = y3.options(precision=y3.FP32SPLIT, vegas_calls=2_000_000, vegas_itmax=8)
+    res = runtime.integrate(igs, o)
+    err = np.abs(res["value"] - gold)
+    assert np.all(res["error"] >= err), np.where(res["error"] < err)
+    conv = res["status"] == 0
+    assert np.all(err[conv] <= 1e-3 * np.abs(gold[conv])), (err / np.abs(gold))[conv].max()
+    assert np.all(err <= 1e-2 * np.abs(gold)), (err / np.abs(gold)).max()
+    again = runtime.integrate(igs, o)
+    np.testing.assert_allclose(again["value"], res["value"], rtol=1e-8)
