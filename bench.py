@@ -386,7 +386,7 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
     ot = O.OracleTables(op, O.spec_wide())
     dd = np.ascontiguousarray(des)
 
-    def one(i):
+    def one(i, tabs):
         lows, highs, (zlo, zhi, R) = vols[i]
         # as the device integrates it: lambda_obs in closed form over lo's range
         oi = O.Integral()
@@ -395,7 +395,7 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
         for a in range(5):
             oi.lows[a], oi.highs[a] = lows[a + 1], highs[a + 1]
         r = O.Result()
-        O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(dd), C.byref(oi),
+        O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(tabs), C.byref(oi),
                                C.byref(O.cub_opts(eps_rel=eps, max_regions=regions,
                                                   max_eval=1 << 40)), C.byref(r))
         return r.value, r.error
@@ -406,11 +406,33 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
     threads = max(1, min(16, os.cpu_count() or 1, cpu_integrals))
     t1 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        got = list(ex.map(one, range(cpu_integrals)))
+        got = list(ex.map(lambda i: one(i, dd), range(cpu_integrals)))
     t_cpu = time.perf_counter() - t1
     worst = max(abs(res["value"][i] - v) / (res["error"][i] + e) for i, (v, e) in enumerate(got))
     out["max_diff_over_errors_vs_oracle"] = worst
     out["checked"] = cpu_integrals
+    # the same integrals over SIG_MAX tables at the reference's real sizes
+    # (1000 radii x 300 masses x 10 redshifts, 0.6 M doubles per point): past
+    # the workgroup's LDS copy, so the kernel reads them from global memory
+    big = des_synthetic(1000, 300, 10)
+    rt.tables_build(params, spec)
+    for k in range(n_points):
+        rt.des_upload(k, big)
+    rt.integrate(arr, opts)  # untimed: uploads
+    t0 = time.perf_counter()
+    rb = rt.integrate(arr, opts)
+    wb = time.perf_counter() - t0
+    msb, nevb = rt.last_timing()
+    out["real_tables"] = {"sig_max_tables": "1000 radii x 300 lnM x 10 z",
+                          "value": len(igs) / wb, "unit": "integrals/s", "kernel_ms": msb,
+                          "evals_per_s": nevb / (msb * 1e-3) if msb > 0 else 0.0,
+                          "converged": int((rb["status"] == 0).sum())}
+    # parity on the first integrals over the same big tables
+    bb = np.ascontiguousarray(big)
+    with ThreadPoolExecutor(threads) as ex:
+        gotb = list(ex.map(lambda i: one(i, bb), range(min(4, cpu_integrals))))
+    out["real_tables"]["max_diff_over_errors_vs_oracle"] = max(
+        abs(rb["value"][i] - v) / (rb["error"][i] + e) for i, (v, e) in enumerate(gotb))
     if with_cpu:
         out["cpu_baseline"] = {"value": cpu_integrals / t_cpu, "unit": "integrals/s",
                                "cores": threads, "kind": "port",

@@ -137,16 +137,18 @@ DES_HEAD = (0.15,                                   # roffset_tau (Mpc/h)
             10 ** 11.2, 10 ** 12.4, 0.75, 0.2, 0.1, 0.3)  # MOR_DES A, B, alpha, sigma, eps, z_p
 
 
-def des_synthetic():
+def des_synthetic(n_r=30, n_m=13, n_z=10):
     """{roffset_tau, LO_LC, MOR_DES, SIG_MAX} for y3g_des_upload: SIG_MAX
-    tables on 30 log-spaced radii in [0.05, 40] Mpc/h, 13 lnM in [30, 36],
-    10 z in [0.1, 1.0]: a one-halo term A(M) r^-0.9 exp(-r / 4 r_200) and a
+    tables on n_r log-spaced radii in [0.05, 40] Mpc/h, n_m lnM in [30, 36],
+    n_z z in [0.1, 1.0]: a one-halo term A(M) r^-0.9 exp(-r / 4 r_200) and a
     two-halo term 0.08 r^-0.7 with bias (1 + (M / 1e14)^0.5)(1 + z)^0.5, the
     reference's SIG_MAX inputs in shape (deltasigma/sigma_1 [n_lnM][n_r],
-    sigma_2 [n_z][n_r], bias [n_lnM][n_z])."""
-    r = np.logspace(np.log10(0.05), np.log10(40.0), 30)
-    m = np.linspace(30.0, 36.0, 13)
-    z = np.linspace(0.1, 1.0, 10)
+    sigma_2 [n_z][n_r], bias [n_lnM][n_z]).  The reference's real tables
+    are 1000 radii x 300 masses (data/deltasigma_r_perp.txt,
+    deltasigma_m_h.txt)."""
+    r = np.logspace(np.log10(0.05), np.log10(40.0), n_r)
+    m = np.linspace(30.0, 36.0, n_m)
+    z = np.linspace(0.1, 1.0, n_z)
     M = np.exp(m)[:, None]
     r200 = 1.6 * (M / 1e15) ** (1 / 3)
     s1 = 60.0 * (M / 1e14) ** 0.6 * r[None, :] ** -0.9 * np.exp(-r[None, :] / (4 * r200))
