@@ -464,6 +464,8 @@ def main():
                     help="check the golden subset, plan the steps and build every "
                          "descriptor, print the plan and exit (no device work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="run the steps by eager calls instead of HIP graph replays")
     ap.add_argument("--no-as-specified", action="store_true",
                     help="skip the run at the stream's own (uncapped) eps_rel")
     ap.add_argument("--no-plain", action="store_true")
@@ -563,10 +565,28 @@ def main():
         return
     rt = y3.Runtime(0)
     info = rt.device_info()
+    # each step is one HIP graph replay (tables, copies, the persistent
+    # launch), one graph per step shape, captured before the clock starts
+    graphs = {}
+    if not a.no_graph:
+        for params, arr in batches:
+            key = (len(params), len(arr))
+            if key not in graphs:
+                graphs[key] = y3.Graph(rt, spec, key[0], key[1], opts)
+
+    def run_step(params, arr):
+        if a.no_graph:
+            rt.tables_build(params, spec)
+            return rt.integrate(arr, opts)
+        g = graphs[(len(params), len(arr))]
+        pa = (y3.Params * len(params))(*params)
+        C.memmove(g.params, pa, C.sizeof(pa))
+        C.memmove(g.integrals, arr, C.sizeof(arr))
+        g.run()
+        return g.results()
+
     for w in range(len(warm)):  # untimed: slice 0
-        params, arr = batches[0]
-        rt.tables_build(params, spec)
-        rt.integrate(arr, opts)
+        run_step(*batches[0])
         print(f"warm-up step {w + 1}/{len(warm)} done", file=sys.stderr, flush=True)
 
     kern_ms, flops, nev_total = [], 0.0, 0
@@ -576,8 +596,7 @@ def main():
     t0 = time.perf_counter()
     for s_i, sl in enumerate(timed):
         params, arr = batches[s_i]
-        rt.tables_build(params, spec)
-        r = rt.integrate(arr, opts)
+        r = run_step(params, arr)
         ms, nev = rt.last_timing()
         kern_ms.append(ms)
         nev_total += nev
@@ -627,8 +646,7 @@ def main():
             for local, k in enumerate(sl):
                 for j in range(N_PER_POINT):
                     arr_s[local * N_PER_POINT + j].eps_rel = float(eps_raw[k][j])
-            rt.tables_build(params, spec)
-            r = rt.integrate(arr_s, opts)
+            r = run_step(params, arr_s)
             n_s += len(arr_s)
             for k in checks[s_i]:
                 loc = k - int(sl[0])
@@ -739,6 +757,9 @@ def main():
                    "eps_rel": "log-uniform [1e-5,1e-3] capped at %g" % a.eps_cap,
                    "integrator": "Genz-Malik cubature, one persistent launch per step, "
                                  "workgroup per integral from a device queue",
+                   "launch": "eager calls" if a.no_graph else
+                             "one HIP graph replay per step (table build, copies, "
+                             "the persistent launch)",
                    "max_regions": opts.max_regions,
                    "tuning": tuning,
                    "checked": n_chk, "checked_points": len(checked),

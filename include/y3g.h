@@ -267,6 +267,31 @@ int y3g_integrate(y3g_runtime* rt, const y3g_integral* igs, size_t n,
 int y3g_integrate_noalloc(y3g_runtime* rt, const y3g_integral* igs, size_t n,
                           const y3g_options* opt, y3g_result* out);
 
+/* One batched step as a HIP graph: the device tables of n_points parameter
+ * points, then n integrals over them in one persistent cubature launch,
+ * captured once over buffers the graph owns (tables, descriptors, results,
+ * counters; the region slab is the runtime's) and replayed.  No counterpart
+ * in the reference: it is for samplers that repeat a step of a fixed shape,
+ * the reference's per-sample loop (CosmoSISSICUDAModule.hh:282-335) with
+ * its shape frozen.  Per replay the caller writes the parameter points and
+ * the integrals (point = 0 .. n_points - 1; cubature, kinds 0-2) into
+ * y3g_graph_params / y3g_graph_integrals, calls y3g_graph_run (synchronous;
+ * the host computes the points' constants, the tolerance bands and the
+ * longest-first order, then launches the graph) and reads
+ * y3g_graph_results.  Results equal y3g_tables_build + y3g_integrate on the
+ * same inputs bit for bit; y3g_last_timing reports the replay (table build,
+ * copies and launch).  Destroy a graph before its runtime.  A call that reallocates the runtime's region
+ * slab (a larger max_regions or more workgroups) invalidates the graph:
+ * y3g_graph_run then fails with Y3G_ERR_STATE. */
+typedef struct y3g_graph y3g_graph;
+int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, size_t n,
+                     const y3g_options* opt, y3g_graph** out);
+y3g_params* y3g_graph_params(y3g_graph* g);
+y3g_integral* y3g_graph_integrals(y3g_graph* g);
+const y3g_result* y3g_graph_results(const y3g_graph* g);
+int y3g_graph_run(y3g_graph* g);
+int y3g_graph_destroy(y3g_graph* g);
+
 /* HBM bytes of the cubature region slab a batched y3g_integrate with these
  * options allocates on this runtime (resident workgroups x max_regions x
  * 121 B); the slab is per resident workgroup, so it does not grow with the

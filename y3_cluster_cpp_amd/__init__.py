@@ -141,6 +141,13 @@ SIGNATURES = {
     "y3g_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "y3g_server_destroy": (C.c_int, [C.c_void_p]),
     "y3g_last_timing": (C.c_int, [C.c_void_p, _PD, C.POINTER(C.c_longlong)]),
+    "y3g_graph_create": (C.c_int, [C.c_void_p, C.POINTER(TableSpec), C.c_int, C.c_size_t,
+                                   C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "y3g_graph_params": (C.c_void_p, [C.c_void_p]),
+    "y3g_graph_integrals": (C.c_void_p, [C.c_void_p]),
+    "y3g_graph_results": (C.c_void_p, [C.c_void_p]),
+    "y3g_graph_run": (C.c_int, [C.c_void_p]),
+    "y3g_graph_destroy": (C.c_int, [C.c_void_p]),
     "y3g_runtime_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                      C.POINTER(C.c_size_t)]),
     "y3g_selfn_upload": (C.c_int, [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD,
@@ -264,9 +271,15 @@ class Runtime:
 
     def __init__(self, device=0):
         self._h = C.c_void_p()
+        self._graphs = []  # weak references: graphs are destroyed before the runtime
         _check(lib().y3g_runtime_create(device, C.byref(self._h)), "y3g_runtime_create")
 
     def close(self):
+        for ref in self._graphs:
+            g = ref()
+            if g is not None:
+                g.close()
+        self._graphs = []
         if self._h:
             lib().y3g_runtime_destroy(self._h)
             self._h = C.c_void_p()
@@ -415,6 +428,43 @@ class Runtime:
                                       C.byref(aux) if aux is not None else None,
                                       out.ctypes.data_as(_PD)), "y3g_eval_functor")
         return out
+
+
+class Graph:
+    """One batched step captured as a HIP graph (y3g_graph_*): write
+    .params[k] and .integrals[i] (point = 0 .. n_points - 1), run(), read
+    results() (a copy).  The runtime must stay open while the graph lives."""
+
+    def __init__(self, runtime, spec, n_points, n, opts=None):
+        self._h = C.c_void_p()
+        self._rt = runtime
+        o = opts if opts is not None else options()
+        _check(lib().y3g_graph_create(runtime._h, C.byref(spec), n_points, n, C.byref(o),
+                                      C.byref(self._h)), "y3g_graph_create")
+        import weakref
+        runtime._graphs.append(weakref.ref(self))
+        self.n_points, self.n = n_points, n
+        self.params = (Params * n_points).from_address(lib().y3g_graph_params(self._h))
+        self.integrals = (Integral * n).from_address(lib().y3g_graph_integrals(self._h))
+
+    def run(self):
+        _check(lib().y3g_graph_run(self._h), "y3g_graph_run")
+
+    def results(self):
+        addr = lib().y3g_graph_results(self._h)
+        buf = (C.c_char * (self.n * RESULT_DTYPE.itemsize)).from_address(addr)
+        return np.frombuffer(bytes(buf), dtype=RESULT_DTYPE)
+
+    def close(self):
+        if self._h:
+            lib().y3g_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def server_options(**kw):

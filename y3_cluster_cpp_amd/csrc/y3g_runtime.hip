@@ -150,6 +150,7 @@ struct y3g_runtime {
          *slab_err = nullptr, *slab_raw = nullptr;
   int8_t* slab_ax = nullptr;
   int slab_slots = 0, slab_cap = 0;
+  unsigned slab_gen = 0;  // bumped when the slab is reallocated (graphs hold its address)
   // batch buffers
   y3g_integral* d_igs = nullptr;
   y3g_result* d_res = nullptr;
@@ -798,6 +799,7 @@ static int ensure_slab(y3g_runtime* rt, int slots, int cap)
     return Y3G_ERR_MEMORY;
   rt->slab_slots = slots;
   rt->slab_cap = cap;
+  rt->slab_gen += 1;
   return 0;
 }
 
@@ -1877,3 +1879,226 @@ int stream_close(y3g_stream* s, long long* nevals)
 }
 
 }  // namespace y3g
+
+// ------------------------------------------------------------- HIP graphs
+//
+// y3g_graph_*: one batched step captured once on the runtime's stream --
+// the points' constants in (pinned), k_tables_build, the descriptors and the
+// longest-first order in (pinned), the queue and counters cleared by a fill
+// kernel (no memset node), k_cubature, results and the evaluation count out
+// (pinned) -- and replayed by hipGraphLaunch between two event records on
+// the stream (the step's time: the launch and its copies and table build).
+
+struct y3g_graph {
+  y3g_runtime* rt = nullptr;
+  y3g_table_spec spec{};
+  y3g_options opt{};
+  int n_points = 0;
+  size_t n = 0;
+  int slots = 0;
+  unsigned slab_gen = 0;
+  // the caller's inputs (host) and the replay's pinned staging
+  std::vector<y3g_params> params;
+  std::vector<y3g_integral> igs;
+  PointDev* h_points = nullptr;
+  y3g_integral* h_igs = nullptr;
+  int* h_order = nullptr;
+  y3g_result* h_res = nullptr;
+  unsigned long long* h_nev = nullptr;
+  // device buffers
+  PointDev* d_points = nullptr;
+  double *hmf64 = nullptr, *da64 = nullptr, *sci64 = nullptr;
+  float *hmf32 = nullptr, *da32 = nullptr, *sci32 = nullptr;
+  y3g_integral* d_igs = nullptr;
+  y3g_result* d_res = nullptr;
+  int* d_order = nullptr;
+  unsigned* d_counters = nullptr;  // queue (int), pad, evaluation count (u64)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void graph_free(y3g_graph* g)
+{
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->h_points) (void)hipHostFree(g->h_points);
+  if (g->h_igs) (void)hipHostFree(g->h_igs);
+  if (g->h_order) (void)hipHostFree(g->h_order);
+  if (g->h_res) (void)hipHostFree(g->h_res);
+  if (g->h_nev) (void)hipHostFree(g->h_nev);
+  dfree(g->d_points);
+  dfree(g->hmf64), dfree(g->da64), dfree(g->sci64);
+  dfree(g->hmf32), dfree(g->da32), dfree(g->sci32);
+  dfree(g->d_igs), dfree(g->d_res), dfree(g->d_order), dfree(g->d_counters);
+  delete g;
+}
+
+extern "C" {
+
+int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, size_t n,
+                     const y3g_options* opt, y3g_graph** out)
+{
+  if (!rt || !spec || !out || n_points <= 0 || n == 0 || n > (size_t)(1 << 30))
+    return fail(Y3G_ERR_ARG, "y3g_graph_create: bad arguments");
+  *out = nullptr;
+  if (int rc = check_spec(spec)) return rc;
+  y3g_options o;
+  y3g_options_default(&o);
+  if (opt) o = *opt;
+  if (o.single || o.max_regions < 64 || o.max_split_per_step < 1 ||
+      o.max_split_per_step > 1024 || o.n_bands < 0 || o.n_bands > 8 ||
+      (o.precision != Y3G_FP32SPLIT && o.precision != Y3G_FP64))
+    return fail(Y3G_ERR_ARG, "y3g_graph_create: options out of range (batched cubature, "
+                             "fp32split or fp64)");
+  for (int k = 0; k < o.n_bands; ++k)
+    if (o.band_regions[k] < 64 || o.band_regions[k] > o.max_regions ||
+        (k > 0 && !(o.band_eps[k] > o.band_eps[k - 1])))
+      return fail(Y3G_ERR_ARG, "y3g_graph_create: tolerance bands need ascending edges and "
+                               "budgets in [64, max_regions]");
+  HIPCHK(hipSetDevice(rt->device));
+  auto* g = new y3g_graph;
+  auto bail = [&](int rc) {
+    graph_free(g);
+    return rc;
+  };
+  g->rt = rt;
+  g->spec = *spec;
+  g->opt = o;
+  g->n_points = n_points;
+  g->n = n;
+  g->params.assign(n_points, y3g_params{});
+  for (auto& p : g->params) y3g_default_params(&p);
+  g->igs.assign(n, y3g_integral{});
+  int const block = o.block_threads == 256 ? 256 : 512;
+  g->slots = std::max(1, std::min<int>(resident_slots(rt, o, Y3G_PAGANI), (int)n));
+  if (int rc = ensure_slab(rt, std::max(g->slots, rt->slab_slots), o.max_regions))
+    return bail(rc);
+  g->slab_gen = rt->slab_gen;
+  size_t const hs = (size_t)spec->n_lnm * spec->n_z;
+  auto pin = [](auto** p, size_t cnt) {
+    return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(**p) * cnt, hipHostMallocDefault);
+  };
+  if (pin(&g->h_points, n_points) != hipSuccess || pin(&g->h_igs, n) != hipSuccess ||
+      pin(&g->h_order, n) != hipSuccess || pin(&g->h_res, n) != hipSuccess ||
+      pin(&g->h_nev, 1) != hipSuccess)
+    return bail(fail(Y3G_ERR_MEMORY, "y3g_graph_create: pinned host memory"));
+  if (dalloc(&g->d_points, n_points) || dalloc(&g->hmf64, n_points * hs) ||
+      dalloc(&g->hmf32, n_points * hs) || dalloc(&g->da64, (size_t)n_points * spec->n_da) ||
+      dalloc(&g->da32, (size_t)n_points * spec->n_da) ||
+      dalloc(&g->sci64, (size_t)n_points * spec->n_sci) ||
+      dalloc(&g->sci32, (size_t)n_points * spec->n_sci) || dalloc(&g->d_igs, n) ||
+      dalloc(&g->d_res, n) || dalloc(&g->d_order, n) || dalloc(&g->d_counters, 4))
+    return bail(Y3G_ERR_MEMORY);
+  TableArgs T;
+  T.points = g->d_points;
+  T.spec = *spec;
+  T.hmf64 = g->hmf64, T.hmf32 = g->hmf32, T.hmf_stride = (int)hs;
+  T.da64 = g->da64, T.da32 = g->da32, T.da_stride = spec->n_da;
+  T.sci64 = g->sci64, T.sci32 = g->sci32, T.sci_stride = spec->n_sci;
+  KernArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.igs = g->d_igs;
+  A.n = (int)n;
+  A.order = g->d_order;
+  A.points = g->d_points;
+  A.hmf32 = g->hmf32, A.hmf64 = g->hmf64, A.hmf_stride = (int)hs;
+  A.da32 = g->da32, A.da64 = g->da64, A.da_stride = spec->n_da;
+  A.sci32 = g->sci32, A.sci64 = g->sci64, A.sci_stride = spec->n_sci;
+  A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
+  A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
+  A.slab_ax = rt->slab_ax;
+  A.cap = o.max_regions;
+  A.kmax = o.max_split_per_step;
+  A.init_split = o.init_split;
+  A.err_c = o.err_c, A.err_smin = o.err_smin;
+  A.tl_c = o.tl_c, A.tl_smin = o.tl_smin;
+  A.queue = reinterpret_cast<int*>(g->d_counters);
+  A.nevals = reinterpret_cast<unsigned long long*>(g->d_counters + 2);
+  A.out = g->d_res;
+  hipStream_t const st = rt->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "y3g_graph_create: stream capture"));
+  bool ok = hipMemcpyAsync(g->d_points, g->h_points, sizeof(PointDev) * n_points,
+                           hipMemcpyHostToDevice, st) == hipSuccess &&
+            launch_tables(T, n_points, st) == hipSuccess &&
+            hipMemcpyAsync(g->d_igs, g->h_igs, sizeof(y3g_integral) * n, hipMemcpyHostToDevice,
+                           st) == hipSuccess &&
+            hipMemcpyAsync(g->d_order, g->h_order, sizeof(int) * n, hipMemcpyHostToDevice,
+                           st) == hipSuccess &&
+            launch_zero_words(g->d_counters, 4, st) == hipSuccess &&
+            launch_cubature(A, o.precision, block, g->slots, st) == hipSuccess &&
+            hipMemcpyAsync(g->h_res, g->d_res, sizeof(y3g_result) * n, hipMemcpyDeviceToHost,
+                           st) == hipSuccess &&
+            hipMemcpyAsync(g->h_nev, A.nevals, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st) == hipSuccess;
+  hipError_t const ec = hipStreamEndCapture(st, &g->graph);
+  if (!ok || ec != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "y3g_graph_create: capture failed"));
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return bail(fail(Y3G_ERR_HIP, "y3g_graph_create: instantiation failed"));
+  *out = g;
+  return 0;
+}
+
+y3g_params* y3g_graph_params(y3g_graph* g) { return g ? g->params.data() : nullptr; }
+y3g_integral* y3g_graph_integrals(y3g_graph* g) { return g ? g->igs.data() : nullptr; }
+const y3g_result* y3g_graph_results(const y3g_graph* g) { return g ? g->h_res : nullptr; }
+
+int y3g_graph_run(y3g_graph* g)
+{
+  if (!g) return fail(Y3G_ERR_ARG, "y3g_graph_run: null graph");
+  y3g_runtime* rt = g->rt;
+  if (rt->slab_gen != g->slab_gen)
+    return fail(Y3G_ERR_STATE, "y3g_graph_run: the runtime's region slab was reallocated "
+                               "since the graph was captured");
+  y3g_options const& o = g->opt;
+  for (size_t i = 0; i < g->n; ++i) {
+    y3g_integral const& ig = g->igs[i];
+    if (ig.ndim < 2 || ig.ndim > kMaxDim || ig.kind < 0 || ig.kind > Y3G_DSIGMA6D ||
+        ig.point < 0 || ig.point >= g->n_points || ig.algorithm != Y3G_PAGANI ||
+        !(ig.eps_rel >= 0) || ig.max_eval < 1)
+      return fail(Y3G_ERR_ARG, "y3g_graph_run: integral " + std::to_string(i) +
+                                 " is malformed (cubature, kinds 0-2, point in range)");
+    for (int a = 0; a < ig.ndim; ++a)
+      if (!(ig.highs[a] > ig.lows[a]))
+        return fail(Y3G_ERR_ARG, "y3g_graph_run: empty volume in integral " +
+                                   std::to_string(i));
+  }
+  for (int k = 0; k < g->n_points; ++k) g->h_points[k] = make_point(g->params[k], g->spec);
+  std::vector<int> ids(g->n);
+  std::iota(ids.begin(), ids.end(), 0);
+  for (size_t i = 0; i < g->n; ++i) {
+    y3g_integral v = g->igs[i];
+    // the tolerance bands' region budgets, as y3g_integrate applies them
+    for (int k = 0; k < o.n_bands; ++k)
+      if (v.eps_rel <= o.band_eps[k]) {
+        long long const lim = 2LL * o.band_regions[k] * gm_num_nodes(v.ndim);
+        if (lim < v.max_eval) v.max_eval = lim;
+        break;
+      }
+    g->h_igs[i] = v;
+  }
+  std::vector<int> order;
+  lpt_order(g->h_igs, ids, order);
+  std::memcpy(g->h_order, order.data(), sizeof(int) * g->n);
+  HIPCHK(hipSetDevice(rt->device));
+  HIPCHK(hipEventRecord(rt->ev0, rt->stream));
+  HIPCHK(hipGraphLaunch(g->exec, rt->stream));
+  HIPCHK(hipEventRecord(rt->ev1, rt->stream));
+  HIPCHK(hipStreamSynchronize(rt->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
+  rt->last_ms = ms;
+  rt->last_nevals = (long long)*g->h_nev;
+  return 0;
+}
+
+int y3g_graph_destroy(y3g_graph* g)
+{
+  if (!g) return 0;
+  (void)hipStreamSynchronize(g->rt->stream);
+  graph_free(g);
+  return 0;
+}
+
+}  // extern "C"
