@@ -64,7 +64,8 @@ typedef enum {
 /* Integrand kinds of the hot path. */
 typedef enum {
   Y3G_COUNT3D = 0,  /* (t, zt, lnM), lambda_obs bin analytic (erf)        */
-  Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM)                               */
+  Y3G_COUNT5D = 1,  /* (lo, zo, t, zt, lnM), z_obs as its standardised
+                       photo-z variable over the bin (v in [0, 1])        */
   Y3G_DSIGMA6D = 2, /* (lo, t, zt, lnM, rmis, theta), z_obs bin analytic   */
   /* The reference's real-data (DES) integrands, in the volume's own units,
    * with the point's DES data (y3g_des_upload); FP64 cubature and m-Cubes.
@@ -182,6 +183,31 @@ typedef struct {
                              clamp(tl_c d / e, tl_smin, 1), d = |R7(A) +
                              R7(B) - R7(parent)|, e = the parent's |R7-R5|;
                              tl_c = 0 keeps |R7-R5|                     */
+  int hmf_smooth;         /* 1 (default): the synthetic integrands
+                             (COUNT3D, COUNT5D, DSIGMA6D) read the HMF
+                             table through its hat-smoothed cubic spline,
+                             whose integral against the integrands' other
+                             factors equals the bilinear table's to
+                             ~1e-7 relative and on which the cubature
+                             converges without the knot-line aliasing of
+                             the bilinear form; 0: the bilinear lookup of
+                             hmf_t.cuh:54-60 at every node              */
+  int init_split_lnm;     /* initial pieces along the lnM axis of the
+                             synthetic kinds (default 4; 0: init_split):
+                             the MOR and lambda_obs kernels confine the
+                             integrands to lnM bands 0.1 - 0.4 wide, which
+                             a region spanning half the mass range can
+                             hide between its nodes                     */
+  double err_scale;       /* safety factor on the cubature's summed error
+                             of the integrals that read the smoothed HMF:
+                             it stops when err_scale x E + the smoothing
+                             bound meets the tolerance and reports that
+                             sum (default 1.5).  The bilinear table's
+                             kinks used to inflate |R7 - R5| by ~20x;
+                             the smooth integrand's estimate is sharper
+                             and needs this margin to stay above the
+                             true error (0 of 12,288 below it, against 2
+                             at 1.0)                                     */
 } y3g_options;
 
 typedef struct y3g_runtime y3g_runtime;

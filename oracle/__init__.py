@@ -32,7 +32,7 @@ class TableSpec(C.Structure):
 
 class Tables(C.Structure):
     _fields_ = [("spec", TableSpec)] + [(n, C.POINTER(C.c_double)) for n in (
-        "lnm", "z", "hmf", "da_z", "da", "sci_z", "sci")]
+        "lnm", "z", "hmf", "da_z", "da", "sci_z", "sci", "hmfc")] + [("smooth_hmf", C.c_int)]
 
 
 class Integral(C.Structure):
@@ -56,14 +56,15 @@ class CubOpts(C.Structure):
                 ("max_eval", C.c_longlong), ("max_regions", C.c_int),
                 ("max_split_per_step", C.c_int), ("init_split", C.c_int),
                 ("err_c", C.c_double), ("err_smin", C.c_double),
-                ("tl_c", C.c_double), ("tl_smin", C.c_double)]
+                ("tl_c", C.c_double), ("tl_smin", C.c_double), ("hmf_smooth", C.c_int),
+                ("init_split_lnm", C.c_int), ("err_scale", C.c_double)]
 
 
 class VegasOpts(C.Structure):
     _fields_ = [("eps_rel", C.c_double), ("eps_abs", C.c_double),
                 ("max_eval", C.c_longlong), ("calls_per_iter", C.c_longlong),
                 ("itmax", C.c_int), ("itskip", C.c_int), ("nbins", C.c_int),
-                ("alpha", C.c_double), ("seed", C.c_uint64)]
+                ("alpha", C.c_double), ("seed", C.c_uint64), ("hmf_smooth", C.c_int)]
 
 
 _D = C.c_double
@@ -156,6 +157,11 @@ def _load():
         "y3o_gl_nodes_ab": (None, [C.c_int, _D, _D, _PD, _PD]),
         "y3o_selgl_counts": (_D, [C.POINTER(Params), C.POINTER(Tables), _PD, C.c_int, _PD,
                                   C.c_int, _PD, C.c_int, C.c_int, _D, _D, _D, _D]),
+        "y3o_hmf_smooth": (_D, [C.POINTER(Params), C.POINTER(Tables), _D, _D]),
+        "y3o_hmf_coefs": (None, [_PD, C.c_int, C.c_int, _PD]),
+        "y3o_spline_coefs": (None, [_PD, C.c_int, C.c_long, _PD, C.c_long]),
+        "y3o_b5_weights": (None, [_D, _PD]),
+        "y3o_smooth_bound": (_D, [C.c_int, C.c_int]),
         "y3o_gk21_nodes": (_PD, []),
         "y3o_gk21_wk": (_PD, []),
         "y3o_gk21_wg": (_PD, []),
@@ -258,16 +264,16 @@ def integrand(params, tables, ig, x):
 
 def cub_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000, max_regions=65536,
              max_split_per_step=1024, init_split=0, err_c=0.0, err_smin=1.0, tl_c=0.0,
-             tl_smin=1.0 / 16.0):
+             tl_smin=1.0 / 16.0, hmf_smooth=1, init_split_lnm=4, err_scale=1.5):
     return CubOpts(eps_rel, eps_abs, max_eval, max_regions, max_split_per_step, init_split,
-                   err_c, err_smin, tl_c, tl_smin)
+                   err_c, err_smin, tl_c, tl_smin, hmf_smooth, init_split_lnm, err_scale)
 
 
 def vegas_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000,
                calls_per_iter=200_000, itmax=5, itskip=1, nbins=64, alpha=1.5,
-               seed=0x5933_4331):
+               seed=0x5933_4331, hmf_smooth=1):
     return VegasOpts(eps_rel, eps_abs, max_eval, calls_per_iter, itmax, itskip,
-                     nbins, alpha, seed)
+                     nbins, alpha, seed, hmf_smooth)
 
 
 def cubature(params, tables, ig, opts):

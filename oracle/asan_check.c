@@ -60,11 +60,13 @@ main(void)
   memset(&co, 0, sizeof(co));
   co.eps_rel = 1e-3, co.eps_abs = 1e-300, co.max_eval = 1LL << 40;
   co.max_regions = 512, co.max_split_per_step = 64, co.err_smin = 1.0, co.tl_smin = 1.0 / 16;
+  co.hmf_smooth = 1, co.init_split_lnm = 4, co.err_scale = 1.5; /* smoothed HMF, lnM split */
   y3o_vegas_opts vo;
   memset(&vo, 0, sizeof(vo));
   vo.eps_rel = 1e-3, vo.eps_abs = 1e-300, vo.max_eval = 1LL << 40;
   vo.calls_per_iter = 4000, vo.itmax = 3, vo.itskip = 1, vo.nbins = 32, vo.alpha = 1.5;
   vo.seed = 0x59334331u;
+  vo.hmf_smooth = 1;
   for (int kind = 0; kind < 3; ++kind) {
     y3o_integral ig;
     y3o_make_integral(&p, &s, kind, 20.0, 30.0, 0.2, 0.35, 1.0, &ig);

@@ -70,6 +70,12 @@ typedef struct {
   double *lnm, *z, *hmf;     /* n_lnm, n_z, n_z*n_lnm */
   double *da_z, *da;         /* n_da, Mpc             */
   double *sci_z, *sci;       /* n_sci                 */
+  /* Cubic-spline B-coefficients of the HMF table, (n_z + 4) x (n_lnm + 4),
+   * index (l + 2) (n_lnm + 4) + (k + 2) for l in [-2, n_z + 1], k in
+   * [-2, n_lnm + 1] (y3o_hmf_coefs), and whether the integrands evaluate
+   * the HMF through them (y3o_hmf_smooth) instead of bilinearly. */
+  double *hmfc;
+  int smooth_hmf;
 } y3o_tables;
 
 /* Integrand kinds. */
@@ -158,6 +164,20 @@ int  y3o_tables_build(const y3o_params* p, const y3o_table_spec* s,
                       y3o_tables* t);
 void y3o_tables_free(y3o_tables* t);
 double y3o_hmf(const y3o_params* p, const y3o_tables* t, double lnM, double zt);
+/* The hat-smoothed HMF: the cubic spline through the table convolved with
+ * the bilinear kernel of one cell per axis, as a tensor quintic B-spline
+ * over the spline's coefficients (y3o_hmf_coefs).  Integrated against any
+ * factor smooth on the scale of a table cell it gives the integral of the
+ * bilinear table to a Poisson-summation remainder; see y3o_models.c. */
+double y3o_hmf_smooth(const y3o_params* p, const y3o_tables* t, double lnM, double zt);
+void   y3o_spline_coefs(const double* y, int n, long stride, double* c, long cstride);
+void   y3o_hmf_coefs(const double* hmf, int n_lnm, int n_z, double* c);
+void   y3o_b5_weights(double f, double* w);
+/* Bound on the smoothing identity's remainder relative to the integral
+ * (1e-6, against at most 1.2e-7 measured), added by the cubature and VEGAS
+ * to the error they test and report for the synthetic kinds when these read
+ * the smoothed HMF; 0 otherwise (csrc/y3g_device.hpp smooth_bound). */
+double y3o_smooth_bound(int kind, int smooth);
 double y3o_dvdz(const y3o_params* p, const y3o_tables* t, double zt);
 double y3o_sci(const y3o_tables* t, double zt);
 
@@ -201,6 +221,10 @@ typedef struct {
   int init_split;           /* pieces per axis of the initial partition */
   double err_c, err_smin;   /* region error E1 clamp(c E1/E2, smin, 1)     */
   double tl_c, tl_smin;     /* two-level child scale clamp(c d/e, smin, 1) */
+  int hmf_smooth;           /* integrands read the hat-smoothed HMF        */
+  int init_split_lnm;       /* initial pieces along the synthetic kinds'
+                               lnM axis (0: init_split)                    */
+  double err_scale;         /* safety factor on E (tested and reported)    */
 } y3o_cub_opts;
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,
@@ -233,6 +257,7 @@ typedef struct {
   int itmax, itskip, nbins;
   double alpha;
   uint64_t seed;
+  int hmf_smooth;           /* integrands read the hat-smoothed HMF        */
 } y3o_vegas_opts;
 void y3o_vegas(const y3o_params* p, const y3o_tables* t,
                const y3o_integral* ig, const y3o_vegas_opts* o, y3o_result* r);

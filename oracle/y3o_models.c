@@ -808,8 +808,9 @@ y3o_tables_build(const y3o_params* p, const y3o_table_spec* s, y3o_tables* t)
   t->da = (double*)malloc(sizeof(double) * s->n_da);
   t->sci_z = (double*)malloc(sizeof(double) * s->n_sci);
   t->sci = (double*)malloc(sizeof(double) * s->n_sci);
+  t->hmfc = (double*)malloc(sizeof(double) * (size_t)(s->n_lnm + 4) * (s->n_z + 4));
   if (!t->lnm || !t->z || !t->hmf || !t->da_z || !t->da || !t->sci_z ||
-      !t->sci)
+      !t->sci || !t->hmfc)
     return 1;
   for (int i = 0; i < s->n_lnm; ++i)
     t->lnm[i] = lin_knot(s->lnm_lo, s->lnm_hi, s->n_lnm, i);
@@ -818,6 +819,7 @@ y3o_tables_build(const y3o_params* p, const y3o_table_spec* s, y3o_tables* t)
   for (int j = 0; j < s->n_z; ++j)
     for (int i = 0; i < s->n_lnm; ++i)
       t->hmf[j * s->n_lnm + i] = y3o_hmf_formula(p, t->lnm[i], t->z[j]);
+  if (s->n_lnm >= 6 && s->n_z >= 6) y3o_hmf_coefs(t->hmf, s->n_lnm, s->n_z, t->hmfc);
   for (int i = 0; i < s->n_da; ++i) {
     double const z = lin_knot(0.0, s->da_zmax, s->n_da, i);
     t->da_z[i] = z;
@@ -846,6 +848,7 @@ y3o_tables_free(y3o_tables* t)
   free(t->da);
   free(t->sci_z);
   free(t->sci);
+  free(t->hmfc);
   memset(t, 0, sizeof(*t));
 }
 
@@ -874,6 +877,161 @@ y3o_sci(const y3o_tables* t, double zt)
   return y3o_interp1d_clamp(t->sci_z, t->sci, t->spec.n_sci, zt);
 }
 
+/* ---------------- hat-smoothed HMF ------------------------------------- */
+/*
+ * The integrands' HMF factor is the bilinear interpolant H of the table
+ * H_ij (hmf_t.cuh:54-60).  It is continuous but kinked along every knot
+ * line, and a cubature rule that straddles the kinks sees a sawtooth of
+ * relative size h^2 f''/8 (up to 5e-3 along z at high mass): its errors
+ * there are aliasing, not truncation, and shrink only slowly as the number
+ * of regions grows.
+ *
+ * Smoothing identity.  On a uniform grid x_i = x_0 + i h the hat functions
+ * phi_i give  Int H g = sum_i H_i Int phi_i g = h sum_i S(x_i) gs(x_i)  for
+ * any S with S(x_i) = H_i, where gs = g * k_h and k_h(u) = (1 - |u|/h)/h is
+ * the hat kernel of unit mass.  Also  Int (S * k_h) g = Int S gs.  The two
+ * differ by the error of the knot sum of S gs against its integral, which
+ * Poisson summation puts at the Fourier transform of S gs at the multiples
+ * of 2 pi / h; the kernel's transform sinc^2(k h / 2) vanishes there and gs
+ * is smooth, so the remainder is far below the integrals' tolerances
+ * (1e-9 .. 1e-7 relative in tests/test_oracle_smooth.py, also with weight
+ * at the table's ends).  With S the cubic spline sum_k c_k B3((x - x_k)/h),
+ * S * k_h = sum_k c_k B5((x - x_k)/h), B5 the centred quintic B-spline: a C4
+ * function with no kinks.  In two dimensions the same holds axis by axis
+ * with the tensor spline.
+ *
+ * Coefficients: the not-a-knot cubic spline per axis (y3o_spline_coefs);
+ * tables with fewer than 6 knots on an axis are read bilinearly.
+ */
+
+/* Thomas pivots of the (1, 4, 1) system: beta_1 = 4, beta_i = 4 - 1 /
+ * beta_{i-1}, held at beta_32 beyond i = 32 (converged to double precision;
+ * the device uses the same sequence, csrc/y3g_device.hpp spline_beta). */
+static double
+spline_beta(int i)
+{
+  double b = 4.0;
+  int const m = i < 32 ? i : 32;
+  for (int k = 2; k <= m; ++k) b = 4.0 - 1.0 / b;
+  return b;
+}
+
+/* One axis: y[i * stride], i < n (n >= 6) -> c[(k + 2) * cstride] for k in
+ * [-2, n + 1]; y may alias c with the same stride (y = c + 2 cstride).
+ * Not-a-knot ends: one cubic over [x_0, x_2] (and [x_{n-3}, x_{n-1}]) fixes
+ * c_1 = y_1 - (y_0 - 2 y_1 + y_2) / 6 (S'' at x_1 is the central second
+ * difference); the (1, 4, 1) system over c_2 .. c_{n-3} by Thomas
+ * elimination; c_0, c_{-1} (c_{n-1}, c_n) from the interpolation rows; one
+ * more coefficient per end continuing the end cubic. */
+void
+y3o_spline_coefs(const double* y, int n, long stride, double* c, long cstride)
+{
+#define C_(k) c[((long)(k) + 2) * cstride]
+  double const y0 = y[0], y1 = y[stride], y2 = y[2 * stride];
+  double const ym1 = y[(long)(n - 1) * stride], ym2 = y[(long)(n - 2) * stride],
+               ym3 = y[(long)(n - 3) * stride];
+  double const c1 = y1 - (y0 - 2.0 * y1 + y2) / 6.0;
+  double const cm2 = ym2 - (ym3 - 2.0 * ym2 + ym1) / 6.0;
+  for (int i = 2; i <= n - 3; ++i) {
+    double r = 6.0 * y[(long)i * stride];
+    if (i == 2) r -= c1;
+    if (i == n - 3) r -= cm2;
+    if (i > 2) r -= C_(i - 1) / spline_beta(i - 2);
+    C_(i) = r;
+  }
+  C_(n - 3) = C_(n - 3) / spline_beta(n - 4);
+  for (int i = n - 4; i >= 2; --i) C_(i) = (C_(i) - C_(i + 1)) / spline_beta(i - 1);
+  C_(1) = c1;
+  C_(n - 2) = cm2;
+  C_(0) = 6.0 * y1 - 4.0 * c1 - C_(2);
+  C_(-1) = 6.0 * y0 - 4.0 * C_(0) - c1;
+  C_(n - 1) = 6.0 * ym2 - 4.0 * cm2 - C_(n - 3);
+  C_(n) = 6.0 * ym1 - 4.0 * C_(n - 1) - cm2;
+  C_(-2) = 4.0 * C_(-1) - 6.0 * C_(0) + 4.0 * C_(1) - C_(2);
+  C_(n + 1) = 4.0 * C_(n) - 6.0 * C_(n - 1) + 4.0 * C_(n - 2) - C_(n - 3);
+#undef C_
+}
+
+/* hmf[j * n_lnm + i] -> c[(l + 2) (n_lnm + 4) + (k + 2)]: each row along lnM
+ * into c's rows 2 .. n_z + 1, then each column along z. */
+void
+y3o_hmf_coefs(const double* hmf, int n_lnm, int n_z, double* c)
+{
+  long const ld = n_lnm + 4;
+  for (int j = 0; j < n_z; ++j)
+    y3o_spline_coefs(hmf + (long)j * n_lnm, n_lnm, 1, c + (long)(j + 2) * ld, 1);
+  double* col = (double*)malloc(sizeof(double) * (size_t)n_z);
+  for (long k = 0; k < ld; ++k) {
+    for (int j = 0; j < n_z; ++j) col[j] = c[(long)(j + 2) * ld + k];
+    y3o_spline_coefs(col, n_z, 1, c + k, ld);
+  }
+  free(col);
+}
+
+/* Centred quintic B-spline at f + 2, f + 1, f, f - 1, f - 2, f - 3 (the
+ * weights of coefficients k = i - 2 .. i + 3 in cell i), f in [0, 1]. */
+void
+y3o_b5_weights(double f, double* w)
+{
+  double const g = 1.0 - f, f2 = f * f, f3 = f2 * f, f4 = f3 * f, f5 = f4 * f;
+  w[0] = g * g * g * g * g;
+  w[1] = 26.0 - 50.0 * f + 20.0 * f2 + 20.0 * f3 - 20.0 * f4 + 5.0 * f5;
+  w[2] = 66.0 - 60.0 * f2 + 30.0 * f4 - 10.0 * f5;
+  w[3] = 26.0 + 50.0 * f + 20.0 * f2 - 20.0 * f3 - 20.0 * f4 + 10.0 * f5;
+  w[4] = 1.0 + 5.0 * f + 10.0 * f2 + 10.0 * f3 + 5.0 * f4 - 5.0 * f5;
+  w[5] = f5;
+  for (int k = 0; k < 6; ++k) w[k] *= 1.0 / 120.0;
+}
+
+/* Cell and fraction of x on n uniform knots from x0 (1 / h = ih), clamped
+ * to the table as the bilinear lookup clamps its coordinates. */
+static void
+cell_of(double x, double x0, double ih, int n, int* i, double* f)
+{
+  double u = (x - x0) * ih;
+  u = u < 0.0 ? 0.0 : (u > (double)(n - 1) ? (double)(n - 1) : u);
+  int k = (int)u;
+  if (k > n - 2) k = n - 2;
+  *i = k;
+  *f = u - (double)k;
+}
+
+double
+y3o_hmf_smooth(const y3o_params* p, const y3o_tables* t, double lnM, double zt)
+{
+  const y3o_table_spec* s = &t->spec;
+  int i, j;
+  double fx, fz, wx[6], wz[6];
+  cell_of(lnM, s->lnm_lo, (double)(s->n_lnm - 1) / (s->lnm_hi - s->lnm_lo), s->n_lnm, &i, &fx);
+  cell_of(zt, s->z_lo, (double)(s->n_z - 1) / (s->z_hi - s->z_lo), s->n_z, &j, &fz);
+  y3o_b5_weights(fx, wx);
+  y3o_b5_weights(fz, wz);
+  long const ld = s->n_lnm + 4;
+  double v = 0.0;
+  for (int l = 0; l < 6; ++l) {
+    const double* row = t->hmfc + (long)(j + l) * ld + i;
+    double r = 0.0;
+    for (int k = 0; k < 6; ++k) r += wx[k] * row[k];
+    v += wz[l] * r;
+  }
+  return v * (p->hmf_s * (lnM * 0.4342944819 - 13.8124426028) + p->hmf_q);
+}
+
+double
+y3o_smooth_bound(int kind, int smooth)
+{
+  return smooth && kind <= Y3O_DSIGMA6D ? 1e-6 : 0.0;
+}
+
+/* The HMF factor the integrands read: bilinear, or hat-smoothed. */
+static double
+hmf_eval(const y3o_params* p, const y3o_tables* t, double lnM, double zt)
+{
+  return t->smooth_hmf && t->spec.n_lnm >= 6 && t->spec.n_z >= 6
+           ? y3o_hmf_smooth(p, t, lnM, zt)
+           : y3o_hmf(p, t, lnM, zt);
+}
+
 /* ---------------- integrands -------------------------------------------- */
 
 /* Argument orders follow the reference integrands:
@@ -888,12 +1046,12 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
   switch (ig->kind) {
     case Y3O_COUNT3D: {
       double const lt = x[0], zt = x[1], lnM = x[2];
-      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return hmf_eval(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_bin(p, ig->lo_lo, ig->lo_hi, lt) * y3o_dvdz(p, t, zt);
     }
     case Y3O_COUNT5D: {
       double const lo = x[0], zo = x[1], lt = x[2], zt = x[3], lnM = x[4];
-      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return hmf_eval(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_pdf(p, lo, lt) * y3o_zo_pdf(p, zo, zt) *
              y3o_dvdz(p, t, zt);
     }
@@ -907,7 +1065,7 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
       double const prof =
         (1.0 - p->f_mis) * y3o_nfw_dsigma(ig->radius, lnM, rho, 0.0) +
         p->f_mis * y3o_dsigma_offset(ig->radius, rmis, th, lnM, rho, 0.0);
-      return y3o_hmf(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
+      return hmf_eval(p, t, lnM, zt) * y3o_mor_lognormal(p, lt, lnM, zt) *
              y3o_lo_lt_pdf(p, lo, lt) *
              y3o_zo_bin(p, ig->zo_lo, ig->zo_hi, zt) * y3o_dvdz(p, t, zt) *
              y3o_sci(t, zt) * pr / (2.0 * Y3O_PI) * prof;
@@ -925,8 +1083,12 @@ y3o_integrand(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
  * one over lt in [1, 300] (the Gaussian mass beyond |t| = 7 is 2.6e-12).  For the
  * Delta-Sigma integrand the offset radius is integrated in its Rayleigh CDF
  * u = 1 - exp(-rmis^2 / 2 sigma^2) (flat weight) and the angle on [0, pi]
- * (the integrand is even about theta = pi).  Axis order follows the
- * physical integrands with lt replaced by t, rmis by u. */
+ * (the integrand is even about theta = pi).  For the 5-D count z_obs is
+ * integrated through its standardised photo-z variable w = (zo - zt) / sz,
+ * sz = sigz0 (1 + zt), over the window of the bin clipped smoothly to
+ * |w| <= 7, as v_z in [0, 1] (the zo = zt ridge becomes a fixed bump).  Axis
+ * order follows the physical integrands with lt replaced by t, rmis by u,
+ * zo by v_z. */
 #define Y3O_T_MAX 7.0
 #define Y3O_T_SOFT 4.0
 
@@ -954,6 +1116,19 @@ y3o_integrand_mapped(const y3o_params* p, const y3o_tables* t,
     if (!(rmis > 0.0)) jac = 0.0;
   }
   double const zt = y[iz], lnM = y[im], v = y[it];
+  if (ig->kind == Y3O_COUNT5D) {
+    /* y1 = v_z: z_obs through w = (zo - zt) / sz over the window of
+     * [zo_lo, zo_hi] clipped smoothly to |w| <= T (as t below):
+     * P(zo | zt) dzo = (phi(w) / sz) sz (w_hi - w_lo) dv */
+    double const sz = p->sigz0 * (1.0 + zt);
+    double const wa = (ig->zo_lo - zt) / sz, wb = (ig->zo_hi - zt) / sz;
+    double const qa = Y3O_T_SOFT * (-Y3O_T_MAX - wa), qb = Y3O_T_SOFT * (wb - Y3O_T_MAX);
+    double const wlo = wa + (qa > 30.0 ? qa : log1p(exp(qa))) * (1.0 / Y3O_T_SOFT);
+    double const whi = wb - (qb > 30.0 ? qb : log1p(exp(qb))) * (1.0 / Y3O_T_SOFT);
+    if (!(whi > wlo)) return 0.0;
+    x[1] = zt + sz * (wlo + (whi - wlo) * y[1]);
+    jac *= sz * (whi - wlo);
+  }
   double const s = p->mor_sigma_ln;
   double const lnlbar = p->mor_lnl0 + p->mor_slope * (lnM - Y3O_LN_MPIVOT_MOR) +
                         p->mor_zslope * log((1.0 + zt) / 1.35);
@@ -1002,7 +1177,7 @@ y3o_make_integral(const y3o_params* p, const y3o_table_spec* s, int kind,
     case Y3O_COUNT5D:
       ig->ndim = 5;
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
-      ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
+      ig->lows[1] = 0.0, ig->highs[1] = 1.0; /* standardised photo-z, v_z */
       ig->lows[2] = 0.0, ig->highs[2] = 1.0;
       ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
       ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;

@@ -695,6 +695,8 @@ typedef struct {
   int d;
   double lows[8], highs[8];
   double err_c, err_smin;
+  double sb; /* smoothing bound, relative to |I| (y3o_smooth_bound) */
+  int am;    /* lnM axis of a synthetic kind (init_split_lnm), -1 none */
 } cub_prob;
 
 typedef struct {
@@ -795,6 +797,8 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
   const cub_prob* P = &Pc;
   int const d = P->d;
   int const nn = y3o_gm_num_nodes(d);
+  /* the safety factor applies with the smoothed HMF only (P->sb > 0) */
+  double const escale = (P->sb > 0.0 && o->err_scale > 0.0) ? o->err_scale : 1.0;
   int cap = o->max_regions > 0 ? o->max_regions : 8192;
   int const kmax = o->max_split_per_step > 0 ? o->max_split_per_step : 64;
   region_t* R = (region_t*)malloc(sizeof(region_t) * cap);
@@ -805,16 +809,26 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
    * and 2 otherwise); region r has base-k digits q_a (axis 0 fastest) and
    * spans [L + q w, L + (q+1) w] with w = (H - L) / k, the end pieces ending
    * exactly at L and H */
-  int ks = o->init_split > 0 ? o->init_split : (d <= 3 ? 4 : 2);
+  /* pieces per axis: k everywhere, init_split_lnm along a synthetic
+   * kind's lnM axis (device: init_pieces in y3g_cub_common.hpp) */
+  int kp[8];
   {
+    int const ks = o->init_split > 0 ? o->init_split : (d <= 3 ? 4 : 2);
     long long nn0 = 1;
-    for (int a = 0; a < d; ++a) nn0 *= ks;
-    if (nn0 > cap) ks = 1, nn0 = 1;
+    for (int a = 0; a < d; ++a) {
+      kp[a] = (o->init_split_lnm > 0 && a == P->am) ? o->init_split_lnm : ks;
+      nn0 *= kp[a];
+    }
+    if (nn0 > cap) {
+      for (int a = 0; a < d; ++a) kp[a] = 1;
+      nn0 = 1;
+    }
     n = (int)nn0;
   }
   for (int r = 0; r < n; ++r) {
     int rr = r;
     for (int a = 0; a < d; ++a) {
+      int const ks = kp[a];
       int const q = rr % ks;
       rr /= ks;
       double const L = P->lows[a], H = P->highs[a];
@@ -831,7 +845,7 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
     I = 0.0;
     E = 0.0;
     for (int k = 0; k < n; ++k) I += R[k].est, E += R[k].err;
-    if (E <= fmax(o->eps_abs, o->eps_rel * fabs(I))) {
+    if (escale * E + P->sb * fabs(I) <= fmax(o->eps_abs, o->eps_rel * fabs(I))) {
       status = 0;
       break;
     }
@@ -890,7 +904,7 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
     neval += 2LL * ns * nn;
   }
   res->value = I;
-  res->error = E;
+  res->error = escale * E + P->sb * fabs(I);
   res->prob = 0.0;
   res->neval = neval;
   res->nregions = n;
@@ -909,6 +923,7 @@ y3o_cubature_fn(double (*f)(const double*, void*), void* ctx, int ndim,
   P.f = f;
   P.ctx = ctx;
   P.d = ndim;
+  P.am = -1;
   for (int i = 0; i < ndim; ++i) P.lows[i] = lows[i], P.highs[i] = highs[i];
   cub_run(&P, o, r);
 }
@@ -932,12 +947,16 @@ y3o_cubature(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
 {
   y3o_params pp = *p;
   if (ig->kind == Y3O_COUNT3D) pp.f_prj = 0.0;
-  ig_ctx c = {&pp, t, ig};
+  y3o_tables tt = *t;
+  tt.smooth_hmf = o->hmf_smooth;
+  ig_ctx c = {&pp, &tt, ig};
   cub_prob P;
   memset(&P, 0, sizeof(P));
   P.f = ig_eval;
   P.ctx = &c;
   P.d = ig->ndim;
+  P.sb = y3o_smooth_bound(ig->kind, o->hmf_smooth && t->spec.n_lnm >= 6 && t->spec.n_z >= 6);
+  P.am = ig->kind == Y3O_COUNT3D ? 2 : ig->kind == Y3O_COUNT5D ? 4 : 3;
   for (int i = 0; i < ig->ndim; ++i)
     P.lows[i] = ig->lows[i], P.highs[i] = ig->highs[i];
   cub_run(&P, o, r);
@@ -1017,10 +1036,24 @@ vegas_refine(double* xg, const double* hist, int nb, double alpha)
   for (int i = 0; i <= nb; ++i) xg[i] = xn[i];
 }
 
+static void vegas_run(double (*f)(const double*, void*), void* ctx, int d,
+                      const double* lows, const double* highs,
+                      const y3o_vegas_opts* o, double sb, y3o_result* res);
+
 void
 y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
              const double* lows, const double* highs,
              const y3o_vegas_opts* o, y3o_result* res)
+{
+  vegas_run(f, ctx, d, lows, highs, o, 0.0, res);
+}
+
+/* VEGAS with sb x |I| added to the error it tests and reports (the
+ * smoothing bound of the synthetic kinds, y3o_smooth_bound). */
+static void
+vegas_run(double (*f)(const double*, void*), void* ctx, int d,
+          const double* lows, const double* highs,
+          const y3o_vegas_opts* o, double sb, y3o_result* res)
 {
   int const nb = o->nbins;
   double* xg = (double*)malloc(sizeof(double) * d * (nb + 1));
@@ -1102,7 +1135,7 @@ y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
       sig = sqrt(1.0 / wsum);
       double const dof = nacc - 1;
       double const chi = dof > 0 ? chisum / dof : 0.0;
-      double const sigr = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig);
+      double const sigr = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig) + sb * fabs(I);
       if (sigr <= fmax(o->eps_abs, o->eps_rel * fabs(I))) {
         status = 0;
         break;
@@ -1116,7 +1149,7 @@ y3o_vegas_fn(double (*f)(const double*, void*), void* ctx, int d,
   double const dof = nacc - 1;
   double const chi = dof > 0 ? chisum / dof : 0.0;
   res->value = I;
-  res->error = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig);
+  res->error = Y3O_VEGAS_NSIG * (chi > 1.0 ? sig * sqrt(chi) : sig) + sb * fabs(I);
   res->prob = chi;
   res->neval = neval;
   res->nregions = 0;
@@ -1131,8 +1164,11 @@ y3o_vegas(const y3o_params* p, const y3o_tables* t, const y3o_integral* ig,
 {
   y3o_params pp = *p;
   if (ig->kind == Y3O_COUNT3D) pp.f_prj = 0.0;
-  ig_ctx c = {&pp, t, ig};
-  y3o_vegas_fn(ig_eval, &c, ig->ndim, ig->lows, ig->highs, o, r);
+  y3o_tables tt = *t;
+  tt.smooth_hmf = o->hmf_smooth;
+  ig_ctx c = {&pp, &tt, ig};
+  vegas_run(ig_eval, &c, ig->ndim, ig->lows, ig->highs, o,
+            y3o_smooth_bound(ig->kind, o->hmf_smooth && t->spec.n_lnm >= 6 && t->spec.n_z >= 6), r);
 }
 
 /* ---------------- reference integrator test functions ------------------ */

@@ -293,7 +293,10 @@ namespace {
     CHECK(ig.params().omega_m == 0.25 && ig.params().f_prj == 0.1);
     auto list = build_integrals(ig, m);
     CHECK(list.size() == 3 && list[2].kind == Y3G_COUNT5D && list[2].ndim == 5);
-    CHECK(list[2].lo_lo == 3 && list[2].zo_hi == 4 && list[2].lows[0] == 3 && list[2].highs[1] == 4);
+    // z_obs is integrated as its standardised photo-z variable: the bin goes
+    // to zo_lo / zo_hi, the box axis is [0, 1]
+    CHECK(list[2].lo_lo == 3 && list[2].zo_lo == 3 && list[2].zo_hi == 4 && list[2].lows[0] == 3 &&
+          list[2].lows[1] == 0 && list[2].highs[1] == 1);
     CHECK(close(list[1].lnlt_lo, std::log(2.0)) && close(list[1].lnlt_hi, std::log(3.0)));
     CHECK(list[1].lows[4] == 2 && list[1].highs[4] == 3);
     // the cartesian layout of a grid-less module is [nvol, 1]

@@ -329,7 +329,11 @@ def test_two_level_fp64_equals_cpu_restatement(runtime):
         c = O.cubature(op, ot, oig, O.cub_opts(eps_rel=eps, max_regions=262144, max_eval=1 << 40,
                                                tl_c=2.0, tl_smin=0.125))
         assert (g["nregions"], g["neval"], g["status"]) == (c.nregions, c.neval, c.status)
-        assert abs(g["value"] / c.value - 1) < 1e-11 and abs(g["error"] / c.error - 1) < 1e-9
+        # the two-level estimate takes differences of nearly equal region sums,
+        # which carry the last-bit differences of the integrand (the device
+        # contracts the smoothed HMF's B-spline sums into FMAs): the errors
+        # agree to ~1e-7 relative, the values to rounding
+        assert abs(g["value"] / c.value - 1) < 1e-11 and abs(g["error"] / c.error - 1) < 1e-6
 
 
 def test_large_batch_independence(runtime):
@@ -366,15 +370,16 @@ def test_large_batch_independence(runtime):
     err = np.abs(head["value"] - ref.reshape(-1))
     assert np.all(head["error"] >= err)
     # tolerance on the stream: every converged integral (status 0) is within
-    # its eps_rel; of all, at least 85% are within eps and counted (the
-    # accuracy at the 32768-region budget is limited by the bilinear HMF
-    # table's kinks: median relative error ~1e-5 against eps down to 1e-5)
+    # its eps_rel; of all, at least 94% are within eps and counted (the
+    # integrands read the smoothed HMF: median relative error ~3e-6 against
+    # eps down to 1e-5; the bilinear table's knot aliasing held it at ~1e-5
+    # and 88%)
     rel = err / np.abs(ref.reshape(-1))
     e = eps[:64].reshape(-1)
     conv = head["status"] == 0
     assert np.all(rel[conv] <= e[conv]), rel[conv].max()
     counted = (rel <= e) & (head["error"] >= err)
-    assert counted.mean() >= 0.85, counted.mean()
+    assert counted.mean() >= 0.94, counted.mean()
 
 
 def test_tolerance_bands(runtime):
@@ -389,7 +394,10 @@ def test_tolerance_bands(runtime):
     r = runtime.integrate(igs, y3.options(bands=bands, max_regions=16384))
     nn = 149
     assert r["neval"][0] <= 2 * 8192 * nn and r["nregions"][0] > 4096
-    assert r["neval"][1] <= 2 * 2048 * nn and r["nregions"][1] <= 2048 + 64
+    # the call limit admits the initial regions (2 pieces on five axes, 4
+    # along lnM) plus two per split: at most 2048 + n_init / 2 regions
+    n_init = 2 ** 5 * 4
+    assert r["neval"][1] <= 2 * 2048 * nn and r["nregions"][1] <= 2048 + n_init // 2
     assert r["nregions"][2] > r["nregions"][1] or r["status"][2] == 0  # outside: max_regions
     rp = runtime.integrate(igs[:2], y3.options(bands=bands, max_regions=16384, single=1,
                                                precision=y3.FP64))

@@ -151,7 +151,9 @@ def test_golden_vs_brute_force_count3d():
     t = O.OracleTables(p, s)
     gold = O.golden(p, t, O.COUNT3D, [20., 30.])[0][0, 0]
     ig = O.make_integral(p, s, O.COUNT3D)
-    r = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-7, max_regions=400000, max_eval=10**9))
+    # brute force of the bilinear integrand itself (hmf_smooth=0)
+    r = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-7, max_regions=400000, max_eval=10**9,
+                                        hmf_smooth=0))
     assert abs(r.value - gold) <= r.error
     assert abs(r.value / gold - 1) < 3e-7
 
@@ -218,6 +220,19 @@ def test_mapped_integrand_equals_physical():
                 rm = sg * math.sqrt(-2 * math.log1p(-y[4]))
                 x[4] = rm
                 jac *= 2.0 / (rm / sg ** 2 * math.exp(-0.5 * rm * rm / sg ** 2))
+            if kind == O.COUNT5D:
+                # z_obs through w = (zo - zt) / sz over the bin's window,
+                # clipped like t: zo = zt + sz w, dzo = sz (w_hi - w_lo) dv
+                sz = p.sigz0 * (1 + y[iz])
+                wa, wb = (0.2 - y[iz]) / sz, (0.35 - y[iz]) / sz
+                wlo = wa + sp(4 * (-7.0 - wa)) / 4
+                whi = wb - sp(4 * (wb - 7.0)) / 4
+                x[1] = y[iz] + sz * (wlo + (whi - wlo) * y[1])
+                if whi > wlo:  # an empty window (zt far outside the bin) maps to 0
+                    assert 0.2 <= y[iz] + sz * wlo and y[iz] + sz * whi <= 0.35 + 1e-12
+                    jac *= sz * (whi - wlo)
+                else:
+                    jac = 0.0
             phys = O.integrand(p, t, ig, x)
             mapped = L.y3o_integrand_mapped(C.byref(p), C.byref(t.t), C.byref(ig),
                                             O.ptr(np.ascontiguousarray(y)))

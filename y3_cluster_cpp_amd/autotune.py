@@ -10,15 +10,15 @@ values come from `tests/golden/golden_points.json`. It chooses:
    - Every calibration integral is run to each candidate region budget
      (4096 ... 65536) with its tolerance out of reach, so that it uses the
      whole budget.
-   - One budget B* serves every tolerance <= 1e-4. It is the smallest
-     budget that meets two conditions, over the calibration integrals with
-     their eps_rel capped at 1e-4:
+   - A budget B* for every tolerance <= 1e-4: the smallest budget that
+     meets two conditions, over the calibration integrals with their
+     eps_rel capped at 1e-4:
      - it keeps at least 97% of the integrals that the largest budget
        gets within tolerance;
      - every reported error bounds the true error.
-   - Looser bands (eps_rel <= 2e-4, 5e-4, 1e-3) each get the smallest
-     budget <= B* that meets the same two conditions on their own
-     integrals.
+   - Bands inside it (eps_rel <= 2e-5, 5e-5, 1e-4, capped) and looser
+     bands (eps_rel <= 2e-4, 5e-4, 1e-3) each get the smallest budget
+     <= B* that meets the same two conditions on their own integrals.
    - This keeps accuracy. Budgets chosen for throughput alone would leave
      many more integrals outside their tolerance.
 2. **Launch shape.** The workgroup size (256 or 512 threads) and the
@@ -40,7 +40,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DEFAULT_PATH = os.path.join(HERE, "tuning", "gfx950.json")
-BUDGETS = (4096, 8192, 16384, 32768, 65536)
+BUDGETS = (4096, 8192, 12288, 16384, 24576, 32768, 65536)
+TIGHT_EDGES = (2e-5, 5e-5, 1e-4)
 LOOSE_EDGES = (2e-4, 5e-4, 1e-3)
 KEEP = 0.97  # share of the largest budget's in-tolerance integrals kept
 
@@ -106,11 +107,25 @@ def tune(first=48, last=64, out=DEFAULT_PATH, log=print):
 
     tight = np.ones(n, dtype=bool)
     bstar, frac = pick(tight, np.minimum(eps, 1e-4), BUDGETS[-1])
-    bands = [(1e-4, bstar)]
     table = [{"eps_le": 1e-4, "integrals": n, "budget": bstar,
               "note": "all integrals, eps_rel capped at 1e-4",
               "counted_frac": {str(b): frac[b] for b in BUDGETS}}]
     log(f"tolerances <= 1e-4: budget {bstar}")
+    # bands inside it, on the integrals' capped tolerances (an integral whose
+    # eps_rel exceeds 1e-4 falls in the last one when capped)
+    bands = []
+    capped = np.minimum(eps, 1e-4)
+    lo = 0.0
+    for edge in TIGHT_EDGES:
+        m = (capped > lo) & (capped <= edge)
+        lo = edge
+        if not m.any():
+            continue
+        b, frac = pick(m, capped, bstar)
+        bands.append((edge, b))
+        table.append({"eps_le": edge, "integrals": int(m.sum()), "budget": b,
+                      "counted_frac": {str(k): frac[k] for k in BUDGETS}})
+        log(f"band eps <= {edge:.1e}: {int(m.sum())} integrals, budget {b}")
     lo = 1e-4
     for edge in LOOSE_EDGES:
         m = (eps > lo) & (eps <= edge)

@@ -136,11 +136,13 @@ __device__ int block_sum_i(int v, int* red)
 template <typename T> struct TabSel;
 template <> struct TabSel<float> {
   static __device__ const float* hmf(const KernArgs& A) { return A.hmf32; }
+  static __device__ const float* hmfc(const KernArgs& A) { return A.hmfc32; }
   static __device__ const float* da(const KernArgs& A) { return A.da32; }
   static __device__ const float* sci(const KernArgs& A) { return A.sci32; }
 };
 template <> struct TabSel<double> {
   static __device__ const double* hmf(const KernArgs& A) { return A.hmf64; }
+  static __device__ const double* hmfc(const KernArgs& A) { return A.hmfc64; }
   static __device__ const double* da(const KernArgs& A) { return A.da64; }
   static __device__ const double* sci(const KernArgs& A) { return A.sci64; }
 };
@@ -174,6 +176,25 @@ __device__ void stage_tables(const KernArgs& A, int pt, T* s_hmf, T* s_da,
 // budget would be exceeded (status 1).
 
 constexpr int kChunk = 32;                      // regions per evaluation chunk
+
+// Pieces of the initial partition along axis a: k = init_split (or 4 for
+// d <= 3, 2 above) on every axis, and init_lnm along the lnM axis of the
+// synthetic kinds.  The MOR and the lambda_obs kernel confine their
+// integrands to bands 0.1 - 0.4 wide in lnM, and a region still spanning
+// half the mass range can hold such a band between its nodes, report a tiny
+// |R7 - R5| and never be split.  Which regions miss a band depends on the
+// partition: on the first 64 stream points (12,288 integrals, smoothed HMF)
+// 2, 4, 8 and 16 pieces left 6, 2, 3 and 10 reported errors below the true
+// ones; 4 is the default, and with err_scale 1.5 none remains.
+__host__ __device__ inline int lnm_axis_of(int kind)
+{
+  return kind == Y3G_COUNT3D ? 2 : kind == Y3G_COUNT5D ? 4 : kind == Y3G_DSIGMA6D ? 3 : -1;
+}
+__host__ __device__ inline int init_pieces(int kind, int d, int init_split, int init_lnm, int a)
+{
+  int const k = init_split > 0 ? init_split : (d <= 3 ? 4 : 2);
+  return (init_lnm > 0 && a == lnm_axis_of(kind)) ? init_lnm : k;
+}
 
 // Region error from the degree-7, -5 and -3 rules: E1 = |R7 - R5| is the
 // error of R5; when the rule sequence converges (r = E1 / |R5 - R3| small)
@@ -278,6 +299,7 @@ template <typename T, int BLOCK> struct CubShared {
   static constexpr bool kHmfLds = false;
   T hmf[kHmfLds ? kMaxHmf : 1];
   const T* hmf_tab;  // the table the evaluation reads (LDS or global)
+  const T* hmfc_tab;  // its spline coefficients (global), null: bilinear HMF
   const double* des;  // the point's DES data (real-data kinds): desl or global
   // the point's DES data staged in LDS when it fits, and the MOR_DES tables
   // (FP64 only)
@@ -447,7 +469,7 @@ __device__ void pair_setup(CubShared<T, BLOCK>& S)
 // factor of the 57 (lo, t, zt, lnM) quadruples and, for the counts, the
 // photo-z pdf of the 17 (zo, zt) pairs.  A node's value is then a product of
 // two table entries.
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, bool SM = false>
 __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
 {
   int iz, im, it;
@@ -475,7 +497,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
   for (int q = tid_here(); q < m * (np + ntp); q += BLOCK) {
     if (q < m * np) {
       int const r = div_by(q, S.dv_np), k = q - r * np;
-      S.pc[r][k] = pair_c(pair_combine<T>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
+      S.pc[r][k] = pair_c(pair_combine<T, SM>(S.ig, S.P, tb, zf[r * kMaxAxisOffsets + S.pzi[k]],
                                           mf[r * kMaxAxisOffsets + S.pmi[k]]));
     } else {
       int const qt = q - m * np;
@@ -508,7 +530,7 @@ __device__ void eval_pairs(CubShared<T, BLOCK>& S, int m, Tabs<T> tb)
       int const r = div_by(qz, S.dv_nt), t = qz - r * nz2;
       float const zo = float(S.c[r][1]) + float(S.h[r][1]) * S.tuo[t];
       float const zt = float(S.c[r][iz]) + float(S.h[r][iz]) * S.tto[t];
-      S.prof[r][t] = zo_pdf<float>(zo, zt, kc.sigz0);
+      S.prof[r][t] = zo_std<float>(zo, zt, kc.zo_lo, kc.zo_hi, kc.sigz0);
     }
   }
 }
@@ -681,14 +703,14 @@ __host__ __device__ inline double two_level_scale(double d, double e, double c, 
 
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, bool SM = false>
 __device__ __forceinline__ void eval_chunk(CubShared<T, BLOCK>& S, int m, double err_c, double err_smin)
 {
   int const d = S.ig.ndim;
   int const nn = gm_num_nodes(d);
-  Tabs<T> tb{S.hmf_tab, S.da, S.sci};
+  Tabs<T> tb{S.hmf_tab, S.da, S.sci, S.hmfc_tab};
   if constexpr (sizeof(T) == 4) {
-    eval_pairs<T, BLOCK>(S, m, tb);
+    eval_pairs<T, BLOCK, SM>(S, m, tb);
   } else {
     switch (S.ig.kind) {
       case Y3G_COUNT3D: eval_nodes_direct<T, BLOCK, Y3G_COUNT3D>(S, m, tb); break;

@@ -113,6 +113,10 @@ template <typename T> struct Tabs {
   const T* hmf;
   const T* da;
   const T* sci;
+  // the HMF table's cubic-spline coefficients (hmf_coefs): when set, the
+  // synthetic integrands read the hat-smoothed HMF (hmf_smooth) instead of
+  // the bilinear one
+  const T* hmfc = nullptr;
 };
 
 // ------------------------------------------------------------ functors
@@ -362,6 +366,133 @@ Y3G_HD T hmf(const PointDev& P, const T* tab, double lnM, double zt)
          T(P.p.hmf_s * (lnM * 0.4342944819 - 13.8124426028) + P.p.hmf_q);
 }
 
+// ------------------------------------------------------ hat-smoothed HMF
+// The bilinear HMF is kinked along every knot line, and a cubature rule
+// straddling the kinks sees a sawtooth of relative size h^2 f''/8 (up to
+// 5e-3 along z at high mass): errors that are aliasing rather than
+// truncation.  Smoothing identity: with S any interpolant of the table
+// values and k_h the hat kernel of one cell (unit mass),
+//   Int H g = h sum_i S(x_i) (g * k_h)(x_i)   (exact, H the bilinear form)
+//   Int (S * k_h) g = Int S (g * k_h),
+// which differ by the error of a knot sum of the smooth S (g * k_h) against
+// its integral: by Poisson summation the Fourier transform of that product
+// at multiples of 2 pi / h, where k_h's transform sinc^2(k h / 2) vanishes.
+// With S the cubic spline sum_k c_k B3((x - x_k)/h), S * k_h =
+// sum_k c_k B5((x - x_k)/h) (B5 the centred quintic B-spline): a C4 surface
+// whose integral against the integrands' other factors equals that of the
+// bilinear table to 1e-9 .. 1e-7 relative (tests/test_oracle_smooth.py), and
+// on which the Genz-Malik rules converge as on a smooth function.  The
+// oracle restates it (y3o_hmf_coefs, y3o_hmf_smooth).
+//
+// Coefficients, per axis: the not-a-knot cubic spline (spline_coefs), the
+// (1, 4, 1) system by Thomas elimination with pivots beta_1 = 4, beta_i =
+// 4 - 1 / beta_{i-1}, held at beta_32 beyond i = 32 (converged to double
+// precision).  Layout [(l + 2) (n_lnm + 4) + k + 2], l in [-2, n_z + 1], k in
+// [-2, n_lnm + 1]; tables with fewer than kSmoothMinKnots knots on an axis
+// are read bilinearly.
+constexpr int kSmoothMinKnots = 6;
+constexpr int kSplineBetaCap = 32;
+__host__ __device__ inline double spline_beta(int i)
+{
+  double b = 4.0;
+  int const m = i < kSplineBetaCap ? i : kSplineBetaCap;
+  for (int k = 2; k <= m; ++k) b = 4.0 - 1.0 / b;
+  return b;
+}
+// y[i * ys], i < n (n >= 6) -> c[(k + 2) * cs], k in [-2, n + 1]; y may
+// alias c with the same stride (y = c + 2 cs).  Not-a-knot ends: one cubic
+// over [x_0, x_2] (and [x_{n-3}, x_{n-1}]) fixes c_1 = y_1 - (y_0 - 2 y_1 +
+// y_2) / 6 (S'' at x_1 is the central second difference); the (1, 4, 1)
+// system over c_2 .. c_{n-3} by Thomas elimination; then c_0, c_{-1} (and
+// c_{n-1}, c_n) from the interpolation rows, and one more coefficient per end
+// continuing the end cubic.
+__host__ __device__ inline void spline_coefs(const double* y, int n, long ys, double* c, long cs)
+{
+  auto C = [&](int k) -> double& { return c[(long)(k + 2) * cs]; };
+  double const y0 = y[0], y1 = y[ys], y2 = y[2 * ys];
+  double const ym1 = y[(long)(n - 1) * ys], ym2 = y[(long)(n - 2) * ys],
+               ym3 = y[(long)(n - 3) * ys];
+  double const c1 = y1 - (y0 - 2.0 * y1 + y2) / 6.0;
+  double const cm2 = ym2 - (ym3 - 2.0 * ym2 + ym1) / 6.0;
+  // forward sweep over i = 2 .. n-3 (pivot beta_{i-1}); the rows read y_i
+  // before C(i) overwrites it when y aliases c
+  for (int i = 2; i <= n - 3; ++i) {
+    double r = 6.0 * y[(long)i * ys];
+    if (i == 2) r -= c1;
+    if (i == n - 3) r -= cm2;
+    if (i > 2) r -= C(i - 1) / spline_beta(i - 2);
+    C(i) = r;
+  }
+  C(n - 3) = C(n - 3) / spline_beta(n - 4);
+  for (int i = n - 4; i >= 2; --i) C(i) = (C(i) - C(i + 1)) / spline_beta(i - 1);
+  C(1) = c1;
+  C(n - 2) = cm2;
+  C(0) = 6.0 * y1 - 4.0 * c1 - C(2);
+  C(-1) = 6.0 * y0 - 4.0 * C(0) - c1;
+  C(n - 1) = 6.0 * ym2 - 4.0 * cm2 - C(n - 3);
+  C(n) = 6.0 * ym1 - 4.0 * C(n - 1) - cm2;
+  C(-2) = 4.0 * C(-1) - 6.0 * C(0) + 4.0 * C(1) - C(2);
+  C(n + 1) = 4.0 * C(n) - 6.0 * C(n - 1) + 4.0 * C(n - 2) - C(n - 3);
+}
+
+// The smoothing identity's remainder is below 2e-7 of the integral in every
+// case measured (the 3-D count over exactly the table's z range, with its
+// weight at the table's edge: 1.2e-7); the cubature and VEGAS add this bound,
+// relative to the integral, to the error they report for the synthetic kinds
+// when these read the smoothed HMF (and stop only when the sum meets the
+// tolerance).
+constexpr double kSmoothRel = 1e-6;
+__host__ __device__ inline double smooth_bound(int kind, bool smooth)
+{
+  return smooth && kind <= Y3G_DSIGMA6D ? kSmoothRel : 0.0;
+}
+
+// 120 x the same weights as polynomials in f, lowest power first.
+__device__ constexpr double kB5Poly[6][6] = {{1, -5, 10, -10, 5, -1},  {26, -50, 20, 20, -20, 5},
+                                             {66, 0, -60, 0, 30, -10}, {26, 50, 20, -20, -20, 10},
+                                             {1, 5, 10, 10, 5, -5},    {0, 0, 0, 0, 0, 1}};
+
+// The centred quintic B-spline at f + 2, f + 1, f, f - 1, f - 2, f - 3: the
+// weights of coefficients k = i - 2 .. i + 3 at fraction f of cell i.
+template <typename T> Y3G_HD void b5_weights(T f, T* w)
+{
+  T const g = T(1) - f, f2 = f * f, f3 = f2 * f, f4 = f3 * f, f5 = f4 * f;
+  T const s = T(1.0 / 120.0);
+  w[0] = (g * g) * (g * g) * g * s;
+  w[1] = (T(26) - T(50) * f + T(20) * f2 + T(20) * f3 - T(20) * f4 + T(5) * f5) * s;
+  w[2] = (T(66) - T(60) * f2 + T(30) * f4 - T(10) * f5) * s;
+  w[3] = (T(26) + T(50) * f + T(20) * f2 - T(20) * f3 - T(20) * f4 + T(10) * f5) * s;
+  w[4] = (T(1) + T(5) * f + T(10) * f2 + T(10) * f3 + T(5) * f4 - T(5) * f5) * s;
+  w[5] = f5 * s;
+}
+
+// The hat-smoothed HMF at (lnM, zt): cells and fractions clamped as the
+// bilinear lookup clamps, times s (lnM log10 e - 13.81) + q.
+template <typename T>
+Y3G_HD T hmf_smooth(const PointDev& P, const T* c, double lnM, double zt)
+{
+  double u = (lnM - P.lnm0) * P.inv_dlnm;
+  double v = (zt - P.z0) * P.inv_dz;
+  u = u < 0.0 ? 0.0 : (u > double(P.nm - 1) ? double(P.nm - 1) : u);
+  v = v < 0.0 ? 0.0 : (v > double(P.nz - 1) ? double(P.nz - 1) : v);
+  int i = int(u), j = int(v);
+  if (i > P.nm - 2) i = P.nm - 2;
+  if (j > P.nz - 2) j = P.nz - 2;
+  T wx[6], wz[6];
+  b5_weights<T>(T(u - double(i)), wx);
+  b5_weights<T>(T(v - double(j)), wz);
+  int const ld = P.nm + 4;
+  const T* c0 = c + j * ld + i;
+  T h = T(0);
+  for (int l = 0; l < 6; ++l) {
+    const T* row = c0 + l * ld;
+    T r = T(0);
+    for (int k = 0; k < 6; ++k) r += wx[k] * row[k];
+    h += wz[l] * r;
+  }
+  return h * T(P.p.hmf_s * (lnM * 0.4342944819 - 13.8124426028) + P.p.hmf_q);
+}
+
 // DV_DO_DZ_t::operator() (dv_do_dz_t.cuh:19-27).
 template <typename T>
 Y3G_HD T dv_do_dz(const PointDev& P, const T* da, double zt)
@@ -574,6 +705,23 @@ template <typename T> Y3G_HD void t_window(T a0, T b0, T& tlo, T& thi)
   thi = b0 - sb * T(1.0 / kTSoft);
 }
 
+// z_obs of the 5-D count through its standardised photo-z variable
+// w = (zo - zt) / sz, sz = sigz0 (1 + zt), over the window of [zo_lo, zo_hi]
+// clipped smoothly to |w| <= 7 (t_window), as v in [0, 1]:
+// P(zo | zt) dzo = phi(w) (w_hi - w_lo) dv.  The change of variables is exact
+// for any window inside the bin (the clip drops < 1e-11 of the Gaussian), and
+// it straightens the zo = zt ridge of width sz (0.01 - 0.017, a tenth of the
+// bin) into a fixed bump in v, which axis-aligned regions resolve.
+template <typename T> Y3G_HD T zo_std(T v, T zt, T zlo, T zhi, T sigz0)
+{
+  T const isz = M<T>::div(T(1), sigz0 * (T(1) + zt));
+  T wlo, whi;
+  t_window<T>((zlo - zt) * isz, (zhi - zt) * isz, wlo, whi);
+  if (!(whi > wlo)) return T(0);
+  T const w = wlo + (whi - wlo) * v;
+  return M<T>::exp(T(-0.5) * w * w) * T(0.3989422804014327) * (whi - wlo);
+}
+
 // Integrand of kind K (axis positions known at compile time).
 template <typename T, int K>
 Y3G_HD T integrand_k(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
@@ -592,14 +740,15 @@ Y3G_HD T integrand_k(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
   T const tt = tlo + (thi - tlo) * T(y[it]);
   T const lt = M<T>::exp(lnlbar + sln * tt);
   T const phi = M<T>::exp(T(-0.5) * tt * tt) * T(0.3989422804014327) * (thi - tlo);
-  T const common = hmf<T>(P, tb.hmf, lnM, zt) * phi * dv_do_dz<T>(P, tb.da, zt);
+  T const hm = tb.hmfc ? hmf_smooth<T>(P, tb.hmfc, lnM, zt) : hmf<T>(P, tb.hmf, lnM, zt);
+  T const common = hm * phi * dv_do_dz<T>(P, tb.da, zt);
   T const fprj = T(P.p.f_prj), pscale = T(P.p.prj_scale);
   if constexpr (K == Y3G_COUNT3D) {
     return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    T const lo = T(y[0]), zo = T(y[1]);
+    T const lo = T(y[0]);
     return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
-           zo_pdf<T>(zo, T(zt), T(P.p.sigz0));
+           zo_std<T>(T(y[1]), T(zt), T(ig.zo_lo), T(ig.zo_hi), T(P.p.sigz0));
   } else {
     T const lo = T(y[0]);
     T const s = T(ig.sigma_mis);
@@ -722,7 +871,9 @@ Y3G_HD MFac<T> m_factors(const y3g_integral& ig, const PointDev& P, double lnM,
   return m;
 }
 
-template <typename T>
+// SM: the smoothed HMF (tb.hmfc) at compile time; the FP64 direct integrand
+// and this function's other callers pass false and read the bilinear table.
+template <typename T, bool SM = false>
 Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> tb,
                              const ZFac<T>& z, const MFac<T>& m)
 {
@@ -736,11 +887,36 @@ Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   // HMF: clamped bilinear lookup (hmf_t.cuh:54-60), as hmf() below; the
   // table is in global memory for every caller
   typedef const __attribute__((address_space(1))) T* GPtr;
-  GPtr r0 = (GPtr)(tb.hmf + z.jz * P.nm + m.im);
-  GPtr r1 = r0 + P.nm;
-  T const a = r0[0] + m.tm * (r0[1] - r0[0]);
-  T const b = r1[0] + m.tm * (r1[1] - r1[0]);
-  T const hm = (a + z.sz * (b - a)) * m.hamp;
+  T hm;
+  if constexpr (SM) {
+    // hat-smoothed HMF (hmf_smooth): the 6 x 6 coefficients around the cell
+    // row by row (not unrolled, so that a row's six loads are in flight at
+    // a time), the row's z weight by Horner from kB5Poly
+    T wx[6];
+    b5_weights<T>(m.tm, wx);
+    T const fz = z.sz;
+    int const ld = P.nm + 4;
+    GPtr row = (GPtr)(tb.hmfc + z.jz * ld + m.im);
+    T h = T(0);
+#pragma unroll 1
+    for (int l = 0; l < 6; ++l, row += ld) {
+      T r = wx[0] * row[0];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) r += wx[k] * row[k];
+      const double* a = kB5Poly[l];
+      T w = T(a[5]);
+#pragma unroll
+      for (int q = 4; q >= 0; --q) w = w * fz + T(a[q]);
+      h += w * r;
+    }
+    hm = h * (m.hamp * T(1.0 / 120.0));
+  } else {
+    GPtr r0 = (GPtr)(tb.hmf + z.jz * P.nm + m.im);
+    GPtr r1 = r0 + P.nm;
+    T const a = r0[0] + m.tm * (r0[1] - r0[0]);
+    T const b = r1[0] + m.tm * (r1[1] - r1[0]);
+    hm = (a + z.sz * (b - a)) * m.hamp;
+  }
   f.zfac = hm * z.zz * (T(0.3989422804014327) * f.dt);
   f.irs = m.irs;
   f.amp = m.amp;
@@ -770,9 +946,9 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
   if constexpr (K == Y3G_COUNT3D) {
     return common * lo_lt_bin<T>(T(ig.lo_lo), T(ig.lo_hi), lt, T(0), pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    T const lo = T(y[0]), zo = T(y[1]);
+    T const lo = T(y[0]);
     return common * lo_lt_pdf<T>(lo, lt, fprj, pscale) *
-           zo_pdf<T>(zo, T(y[iz]), T(P.p.sigz0));
+           zo_std<T>(T(y[1]), T(y[iz]), T(ig.zo_lo), T(ig.zo_hi), T(P.p.sigz0));
   } else {
     T const lo = T(y[0]);
     T const s = T(ig.sigma_mis);
@@ -790,7 +966,7 @@ Y3G_HD T integrand_pair_k(const y3g_integral& ig, const PointDev& P, const PairF
 // Per-integral constants of the FP32 node integrand, converted once per
 // chunk of regions and kept in registers across its nodes.
 struct NodeConstF {
-  float sln, fprj, pscale, ips, sigz0, smis, fmis, radius, lo_lo, lo_hi;
+  float sln, fprj, pscale, ips, sigz0, smis, fmis, radius, lo_lo, lo_hi, zo_lo, zo_hi;
   const NfwKnot* nfw;  // the NFW knot table (LDS copy in the cubature)
 };
 Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P, const NfwKnot* nfw)
@@ -806,6 +982,8 @@ Y3G_HD NodeConstF node_const_f(const y3g_integral& ig, const PointDev& P, const 
   k.radius = float(ig.radius);
   k.lo_lo = float(ig.lo_lo);
   k.lo_hi = float(ig.lo_hi);
+  k.zo_lo = float(ig.zo_lo);
+  k.zo_hi = float(ig.zo_hi);
   k.nfw = nfw;
   return k;
 }
@@ -869,7 +1047,7 @@ Y3G_HD float integrand_node_f(const NodeConstF& kc, const PairC& f, const float*
     T const common = f.zfac * M<T>::exp(T(-0.5) * tt * tt);
     return common * lo_lt_bin<T>(kc.lo_lo, kc.lo_hi, lt, T(0), kc.pscale);
   } else if constexpr (K == Y3G_COUNT5D) {
-    return quad_f(kc, f, yf[0], yf[it]) * zo_pdf<T>(yf[1], yf[iz], kc.sigz0);
+    return quad_f(kc, f, yf[0], yf[it]) * zo_std<T>(yf[1], yf[iz], kc.zo_lo, kc.zo_hi, kc.sigz0);
   } else {
     return quad_f(kc, f, yf[0], yf[it]) * prof;
   }

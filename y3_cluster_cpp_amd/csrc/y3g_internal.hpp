@@ -23,6 +23,11 @@ struct TableArgs {
   double* sci64;
   float* sci32;
   int sci_stride;
+  // the HMF table's cubic-spline coefficients (hmf_coefs), (n_z + 4) x
+  // (n_lnm + 4) per point
+  double* hmfc64;
+  float* hmfc32;
+  int hmfc_stride;
 };
 
 struct KernArgs {
@@ -39,6 +44,11 @@ struct KernArgs {
   const float* sci32;
   const double* sci64;
   int sci_stride;
+  // the HMF coefficients; null: the synthetic integrands read the bilinear
+  // HMF (y3g_options.hmf_smooth = 0)
+  const float* hmfc32;
+  const double* hmfc64;
+  int hmfc_stride;
   // DES data of the points (Y3G_DSIGMA6D_DES), point p at des + des_off[p]
   const double* des;
   const long long* des_off;
@@ -52,6 +62,8 @@ struct KernArgs {
   int cap;
   int kmax;
   int init_split;
+  int init_lnm;   // initial pieces along the synthetic kinds' lnM axis (0: init_split)
+  double err_scale;  // safety factor on the summed error with the smoothed HMF
   double err_c, err_smin;
   double tl_c, tl_smin;
   // VEGAS
@@ -155,6 +167,7 @@ hipError_t launch_cubature_stream(const KernArgs& A, const StreamArgs& Z, int pr
                                   int slots, hipStream_t s);
 int cubature_stream_blocks_per_cu(int precision);
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s);
+hipError_t launch_hmf_coefs(const TableArgs& A, int pt0, int n_points, hipStream_t s);
 hipError_t launch_zero_words(void* p, size_t n, hipStream_t s);
 hipError_t launch_nfw_mis_table(double* tab, int nx, int nm, hipStream_t s);
 hipError_t launch_functor(const FunctorArgs& F, int precision, hipStream_t s);

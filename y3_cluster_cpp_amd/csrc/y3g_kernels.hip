@@ -48,6 +48,28 @@ __device__ __forceinline__ double lin_knot(double a, double b, int n, int i)
   return a + double(i) * ((b - a) / double(n - 1));
 }
 
+// The cubic-spline coefficients of point pt's HMF table (spline_coefs), by
+// the nt threads of one workgroup after the table is written: one thread per
+// row along lnM, then one per column along z, then the FP32 copy.
+__device__ void hmf_coefs(const TableArgs& A, int pt, int tid, int nt)
+{
+  int const nm = A.spec.n_lnm, nz = A.spec.n_z, ld = nm + 4;
+  const double* hmf64 = A.hmf64 + (size_t)pt * A.hmf_stride;
+  double* c64 = A.hmfc64 + (size_t)pt * A.hmfc_stride;
+  float* c32 = A.hmfc32 + (size_t)pt * A.hmfc_stride;
+  for (int j = tid; j < nz; j += nt) spline_coefs(hmf64 + (size_t)j * nm, nm, 1, c64 + (size_t)(j + 2) * ld, 1);
+  __syncthreads();
+  for (int k = tid; k < ld; k += nt) spline_coefs(c64 + 2 * ld + k, nz, ld, c64 + k, ld);
+  __syncthreads();
+  for (int k = tid; k < (nz + 4) * ld; k += nt) c32[k] = float(c64[k]);
+}
+
+// One workgroup per point: the HMF coefficients of uploaded tables.
+__global__ __launch_bounds__(256) void k_hmf_coefs(TableArgs A, int pt0)
+{
+  hmf_coefs(A, pt0 + blockIdx.x, threadIdx.x, blockDim.x);
+}
+
 // The tables of one parameter point, by the nt threads of one workgroup:
 // HMF (n_z x n_lnM), D_A (n_da), Sigma_crit^-1 (n_sci), FP64 and FP32
 // copies.
@@ -68,6 +90,10 @@ __device__ void build_point_tables(const TableArgs& A, int pt, const PointDev& P
     hmf64[k] = v;
     hmf32[k] = float(v);
   }
+  // the HMF's cubic-spline coefficients (hmf_coefs): rows along lnM, then
+  // columns along z in place
+  __syncthreads();
+  if (s.n_lnm >= kSmoothMinKnots && s.n_z >= kSmoothMinKnots) hmf_coefs(A, pt, tid, nt);
   double* da64 = A.da64 + (size_t)pt * A.da_stride;
   float* da32 = A.da32 + (size_t)pt * A.da_stride;
   for (int i = tid; i < s.n_da; i += nt) {
@@ -249,7 +275,7 @@ __device__ void stream_publish(const StreamArgs& Z, long long item, const y3g_re
 // The cubature's work loop: integrals from the device queue (k_cubature) or
 // from the host ring (STREAM, k_cubature_stream); the region algorithm of
 // an integral is the same code in both.
-template <typename T, int BLOCK, bool STREAM>
+template <typename T, int BLOCK, bool STREAM, bool SM>
 __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamArgs& Z)
 {
   extern __shared__ char smem_raw[];
@@ -352,6 +378,8 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       if (tid == 0) {
         S.P = A.points[pt];
         S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A) + (size_t)pt * A.hmf_stride;
+        const T* hc = TabSel<T>::hmfc(A);
+        S.hmfc_tab = hc ? hc + (size_t)pt * A.hmfc_stride : nullptr;
       }
       if (A.des) {
         long long const o0 = A.des_off[pt], nd = A.des_off[pt + 1] - o0;
@@ -369,14 +397,15 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
     }
     int const d = S.ig.ndim;
     int const nn = gm_num_nodes(d);
-    int k = A.init_split > 0 ? A.init_split : (d <= 3 ? 4 : 2);
-    int N = 1;
-    for (int a = 0; a < d; ++a) N *= k;
-    if (N > cap) k = 1, N = 1;
-    // initial partition, region r -> digits (base k) per axis
+    long long N0 = 1;
+    for (int a = 0; a < d; ++a) N0 *= init_pieces(S.ig.kind, d, A.init_split, A.init_lnm, a);
+    bool const one = N0 > cap;
+    int N = one ? 1 : (int)N0;
+    // initial partition, region r -> digits per axis (axis 0 fastest)
     for (int r = tid; r < N; r += BLOCK) {
       int rr = r;
       for (int a = 0; a < d; ++a) {
+        int const k = one ? 1 : init_pieces(S.ig.kind, d, A.init_split, A.init_lnm, a);
         int const q = rr % k;
         rr /= k;
         double const L = S.ig.lows[a], H = S.ig.highs[a];
@@ -397,7 +426,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
         S.h[r][a] = 0.5 * (h - l);
       }
       __syncthreads();
-      eval_chunk<T, BLOCK>(S, m, A.err_c, A.err_smin);
+      eval_chunk<T, BLOCK, SM>(S, m, A.err_c, A.err_smin);
       if (tid < m) {
         est[base + tid] = S.est[tid];
         err[base + tid] = S.err[tid];
@@ -411,6 +440,8 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
     // totals: summed once here and at the end, updated by each step's
     // (children - parent) differences in between
     double I = 0.0, E = 0.0;
+    double const sb = smooth_bound(S.ig.kind, S.hmfc_tab != nullptr);
+    double const es = sb > 0.0 ? A.err_scale : 1.0;  // margin with the smoothed HMF
     {
       double a = 0.0, b = 0.0;
       for (int r = tid; r < N; r += BLOCK) a += est[r], b += err[r];
@@ -419,7 +450,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       E = b;
     }
     for (;;) {
-      if (E <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
+      if (es * E + sb * fabs(I) <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
         status = 0;
         break;
       }
@@ -536,7 +567,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
           next = fetch_parents<BLOCK>(S.sel, pb + kChunk / 2, min(kChunk / 2, ns - pb - kChunk / 2),
                                       cap, lo, hi, est, err, raw, axs);
         lds_barrier();
-        eval_chunk<T, BLOCK>(S, 2 * np, A.err_c, A.err_smin);
+        eval_chunk<T, BLOCK, SM>(S, 2 * np, A.err_c, A.err_smin);
         if (more) store_stage<BLOCK>(&stage, next);
         // results; the next chunk's first barrier orders these LDS reads
         // before its writes
@@ -586,7 +617,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
     if (tid == 0) {
       y3g_result res;
       res.value = I;
-      res.error = E;
+      res.error = es * E + sb * fabs(I);
       res.prob = 0.0;
       res.neval = neval;
       res.nregions = N;
@@ -602,17 +633,20 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
   }
 }
 
-template <typename T, int BLOCK>
+// SM: the FP32 pair factors read the smoothed HMF (the launchers pick it
+// when the arguments carry the coefficients); each form has its own
+// register allocation.  The FP64 path reads tb.hmfc at run time.
+template <typename T, int BLOCK, bool SM = false>
 __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature(KernArgs A)
 {
-  cubature_items<T, BLOCK, false>(A, StreamArgs{});
+  cubature_items<T, BLOCK, false, SM>(A, StreamArgs{});
 }
 
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, bool SM = false>
 __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? Y3G_CUB_MIN_WAVES : 1) void k_cubature_stream(
   KernArgs A, StreamArgs Z)
 {
-  cubature_items<T, BLOCK, true>(A, Z);
+  cubature_items<T, BLOCK, true, SM>(A, Z);
 }
 
 // ------------------------------------------------------------------ VEGAS
@@ -640,6 +674,7 @@ template <typename T, int BLOCK> struct VegShared {
   y3g_integral ig;
   PointDev P;
   const double* des;  // the point's DES data (Y3G_DSIGMA6D_DES)
+  const T* hmfc;      // the point's HMF coefficients (global), null: bilinear
   int item, point, stop;
 };
 
@@ -668,7 +703,7 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
   int const tid = threadIdx.x;
   int const nb = A.vegas_nbins;
   constexpr int d = KindAxes<K>::nd;
-  Tabs<T> tb{S.hmf, S.da, S.sci};
+  Tabs<T> tb{S.hmf, S.da, S.sci, S.hmfc};
   for (int q = tid; q < d * (nb + 1); q += BLOCK) {
     int const a = q / (nb + 1), b = q - a * (nb + 1);
     S.xg[a][b] = double(b) / nb;
@@ -753,7 +788,8 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
       sig = sqrt(1.0 / wsum);
       double const dof = nacc - 1;
       double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
-      double const sigr = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
+      double const sigr = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig) +
+                          smooth_bound(S.ig.kind, S.hmfc != nullptr) * fabs(I);
       if (sigr <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
         status = 0;
         break;
@@ -812,7 +848,8 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
     double const chi = dof > 0 ? (i2sum - wsum * I * I) / dof : 0.0;
     y3g_result res;
     res.value = I;
-    res.error = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig);
+    res.error = kVegasNsig * (chi > 1.0 ? sig * sqrt(chi) : sig) +
+                smooth_bound(S.ig.kind, S.hmfc != nullptr) * fabs(I);
     res.prob = chi;
     res.neval = neval;
     res.nregions = 0;
@@ -844,6 +881,8 @@ __global__ __launch_bounds__(BLOCK) void k_vegas(KernArgs A)
       if (tid == 0) {
         S.P = A.points[pt];
         S.des = A.des ? A.des + A.des_off[pt] : nullptr;
+        const T* hc = TabSel<T>::hmfc(A);
+        S.hmfc = hc ? hc + (size_t)pt * A.hmfc_stride : nullptr;
       }
       stage_tables<T>(A, pt, S.hmf, S.da, S.sci);
       __syncthreads();
@@ -1158,17 +1197,24 @@ hipError_t launch_selgl(const SelGLArgs& A, int n_points, hipStream_t s)
 }
 
 // ----------------------------------------------------------- launchers
-template <typename T, int BLOCK>
-static hipError_t launch_cub(const KernArgs& A, int slots, hipStream_t s)
+template <typename T, int BLOCK, bool SM>
+static hipError_t launch_cub_sm(const KernArgs& A, int slots, hipStream_t s)
 {
   size_t const smem = sizeof(CubShared<T, BLOCK>);
   hipError_t e = hipFuncSetAttribute(
-    reinterpret_cast<const void*>(&k_cubature<T, BLOCK>),
+    reinterpret_cast<const void*>(&k_cubature<T, BLOCK, SM>),
     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_cubature<T, BLOCK>), dim3(slots), dim3(BLOCK), smem,
+  hipLaunchKernelGGL((k_cubature<T, BLOCK, SM>), dim3(slots), dim3(BLOCK), smem,
                      s, A);
   return hipGetLastError();
+}
+template <typename T, int BLOCK>
+static hipError_t launch_cub(const KernArgs& A, int slots, hipStream_t s)
+{
+  if constexpr (sizeof(T) == 4)
+    if (A.hmfc32) return launch_cub_sm<T, BLOCK, true>(A, slots, s);
+  return launch_cub_sm<T, BLOCK, false>(A, slots, s);
 }
 
 template <typename T, int BLOCK>
@@ -1220,17 +1266,25 @@ hipError_t launch_cubature(const KernArgs& A, int precision, int block,
                       : launch_cub<float, 512>(A, slots, s);
 }
 
+template <typename T, bool SM>
+static hipError_t launch_cub_stream_sm(const KernArgs& A, const StreamArgs& Z, int slots,
+                                       hipStream_t s)
+{
+  size_t const smem = sizeof(CubShared<T, 512>);
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void*>(&k_cubature_stream<T, 512, SM>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_cubature_stream<T, 512, SM>), dim3(slots), dim3(512), smem, s, A, Z);
+  return hipGetLastError();
+}
 template <typename T>
 static hipError_t launch_cub_stream(const KernArgs& A, const StreamArgs& Z, int slots,
                                     hipStream_t s)
 {
-  size_t const smem = sizeof(CubShared<T, 512>);
-  hipError_t e = hipFuncSetAttribute(
-    reinterpret_cast<const void*>(&k_cubature_stream<T, 512>),
-    hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_cubature_stream<T, 512>), dim3(slots), dim3(512), smem, s, A, Z);
-  return hipGetLastError();
+  if constexpr (sizeof(T) == 4)
+    if (A.hmfc32) return launch_cub_stream_sm<T, true>(A, Z, slots, s);
+  return launch_cub_stream_sm<T, false>(A, Z, slots, s);
 }
 
 hipError_t launch_cubature_stream(const KernArgs& A, const StreamArgs& Z, int precision,
@@ -1318,6 +1372,12 @@ hipError_t launch_nfw_mis_table(double* tab, int nx, int nm, hipStream_t s)
 hipError_t launch_tables(const TableArgs& A, int n_points, hipStream_t s)
 {
   hipLaunchKernelGGL(k_tables_build, dim3(n_points), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_hmf_coefs(const TableArgs& A, int pt0, int n_points, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_hmf_coefs, dim3(n_points), dim3(256), 0, s, A, pt0);
   return hipGetLastError();
 }
 

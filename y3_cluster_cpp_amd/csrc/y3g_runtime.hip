@@ -145,6 +145,9 @@ struct y3g_runtime {
   double *hmf64 = nullptr, *da64 = nullptr, *sci64 = nullptr;
   float *hmf32 = nullptr, *da32 = nullptr, *sci32 = nullptr;
   int hmf_stride = 0, da_stride = 0, sci_stride = 0;
+  double* hmfc64 = nullptr;  // the HMF tables' spline coefficients
+  float* hmfc32 = nullptr;
+  int hmfc_stride = 0;
   // cubature slab
   double *slab_lo = nullptr, *slab_hi = nullptr, *slab_est = nullptr,
          *slab_err = nullptr, *slab_raw = nullptr;
@@ -330,6 +333,9 @@ void y3g_options_default(y3g_options* o)
   for (int k = 0; k < 8; ++k) o->band_eps[k] = 0.0, o->band_regions[k] = 0;
   o->tl_c = 0.0;
   o->tl_smin = 1.0 / 16.0;
+  o->hmf_smooth = 1;
+  o->init_split_lnm = 4;
+  o->err_scale = 1.5;
 }
 
 void y3g_default_params(y3g_params* p)
@@ -404,7 +410,7 @@ void y3g_make_integral(const y3g_params* p, const y3g_table_spec* s, int kind,
     case Y3G_COUNT5D:
       ig->ndim = 5;
       ig->lows[0] = lo_lo, ig->highs[0] = lo_hi;
-      ig->lows[1] = zo_lo, ig->highs[1] = zo_hi;
+      ig->lows[1] = 0.0, ig->highs[1] = 1.0;  // standardised photo-z (zo_std)
       ig->lows[2] = 0.0, ig->highs[2] = 1.0;
       ig->lows[3] = zt_lo, ig->highs[3] = zt_hi;
       ig->lows[4] = s->lnm_lo, ig->highs[4] = s->lnm_hi;
@@ -615,6 +621,7 @@ int y3g_runtime_destroy(y3g_runtime* rt)
   dfree(rt->points);
   dfree(rt->hmf64), dfree(rt->da64), dfree(rt->sci64);
   dfree(rt->hmf32), dfree(rt->da32), dfree(rt->sci32);
+  dfree(rt->hmfc64), dfree(rt->hmfc32);
   dfree(rt->slab_lo), dfree(rt->slab_hi), dfree(rt->slab_est),
     dfree(rt->slab_err), dfree(rt->slab_raw), dfree(rt->slab_ax);
   dfree(rt->d_igs), dfree(rt->d_res), dfree(rt->d_order);
@@ -679,14 +686,24 @@ static int check_spec(const y3g_table_spec* s)
   return 0;
 }
 
+// (n_z + 4) x (n_lnm + 4): the HMF spline coefficients of one point
+static int hmfc_stride_of(const y3g_table_spec& s) { return (s.n_lnm + 4) * (s.n_z + 4); }
+// tables this small on an axis are read bilinearly (no coefficients)
+static bool smooth_ok(const y3g_table_spec& s)
+{
+  return s.n_lnm >= kSmoothMinKnots && s.n_z >= kSmoothMinKnots;
+}
+
 static int alloc_tables(y3g_runtime* rt, const y3g_table_spec* spec, int n)
 {
   dfree(rt->points);
   dfree(rt->hmf64), dfree(rt->da64), dfree(rt->sci64);
   dfree(rt->hmf32), dfree(rt->da32), dfree(rt->sci32);
+  dfree(rt->hmfc64), dfree(rt->hmfc32);
   rt->n_points = 0;
   rt->spec = *spec;
   rt->hmf_stride = spec->n_lnm * spec->n_z;
+  rt->hmfc_stride = hmfc_stride_of(*spec);
   rt->da_stride = spec->n_da;
   rt->sci_stride = spec->n_sci;
   if (dalloc(&rt->points, n) || dalloc(&rt->hmf64, (size_t)n * rt->hmf_stride) ||
@@ -694,7 +711,9 @@ static int alloc_tables(y3g_runtime* rt, const y3g_table_spec* spec, int n)
       dalloc(&rt->da64, (size_t)n * rt->da_stride) ||
       dalloc(&rt->da32, (size_t)n * rt->da_stride) ||
       dalloc(&rt->sci64, (size_t)n * rt->sci_stride) ||
-      dalloc(&rt->sci32, (size_t)n * rt->sci_stride))
+      dalloc(&rt->sci32, (size_t)n * rt->sci_stride) ||
+      dalloc(&rt->hmfc64, (size_t)n * rt->hmfc_stride) ||
+      dalloc(&rt->hmfc32, (size_t)n * rt->hmfc_stride))
     return Y3G_ERR_MEMORY;
   rt->n_points = n;
   rt->des_dirty = true;  // the DES offsets cover every point with tables
@@ -722,6 +741,7 @@ int y3g_tables_build(y3g_runtime* rt, const y3g_params* params,
   A.hmf64 = rt->hmf64, A.hmf32 = rt->hmf32, A.hmf_stride = rt->hmf_stride;
   A.da64 = rt->da64, A.da32 = rt->da32, A.da_stride = rt->da_stride;
   A.sci64 = rt->sci64, A.sci32 = rt->sci32, A.sci_stride = rt->sci_stride;
+  A.hmfc64 = rt->hmfc64, A.hmfc32 = rt->hmfc32, A.hmfc_stride = rt->hmfc_stride;
   HIPCHK(launch_tables(A, n_points, rt->stream));
   HIPCHK(hipStreamSynchronize(rt->stream));
   // the points are new samples: DES data uploaded for the old ones is stale
@@ -746,6 +766,15 @@ int y3g_tables_upload(y3g_runtime* rt, int point, const double* hmf,
     return 0;
   };
   if (int rc = up(hmf, rt->hmf64, rt->hmf32, rt->hmf_stride)) return rc;
+  if (hmf && smooth_ok(rt->spec)) {
+    // the uploaded table's spline coefficients (hat-smoothed HMF)
+    TableArgs A{};
+    A.spec = rt->spec;
+    A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
+    A.hmfc64 = rt->hmfc64, A.hmfc32 = rt->hmfc32, A.hmfc_stride = rt->hmfc_stride;
+    HIPCHK(launch_hmf_coefs(A, point, 1, rt->stream));
+    HIPCHK(hipStreamSynchronize(rt->stream));
+  }
   if (int rc = up(da, rt->da64, rt->da32, rt->da_stride)) return rc;
   if (int rc = up(sci, rt->sci64, rt->sci32, rt->sci_stride)) return rc;
   // a new sample's tables: the point's earlier DES data is stale
@@ -824,13 +853,18 @@ static int ensure_single(y3g_runtime* rt, int cap)
   return 0;
 }
 
-// Largest expected cost first: dimension, then tighter tolerance.
+// Largest expected cost first: dimension, then, point by point, tighter
+// tolerance.  Keeping a point's integrals together keeps the few points the
+// resident workgroups are on at a time in L2 (their HMF coefficients: 29 KB
+// per point, read by every pair factor); measured 1% faster than a
+// tolerance-first order over all points, with the same results.
 static void lpt_order(const y3g_integral* igs, const std::vector<int>& ids,
                       std::vector<int>& order)
 {
   order = ids;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (igs[a].ndim != igs[b].ndim) return igs[a].ndim > igs[b].ndim;
+    if (igs[a].point != igs[b].point) return igs[a].point < igs[b].point;
     return igs[a].eps_rel < igs[b].eps_rel;
   });
 }
@@ -1063,11 +1097,15 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
   A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
   A.sci32 = rt->sci32, A.sci64 = rt->sci64, A.sci_stride = rt->sci_stride;
+  if (o.hmf_smooth && smooth_ok(rt->spec))
+    A.hmfc32 = rt->hmfc32, A.hmfc64 = rt->hmfc64, A.hmfc_stride = rt->hmfc_stride;
   A.des = any_des ? rt->d_des : nullptr;
   A.des_off = any_des ? rt->d_des_off : nullptr;
   A.cap = o.max_regions;
   A.kmax = o.max_split_per_step;
   A.init_split = o.init_split;
+  A.init_lnm = o.init_split_lnm;
+  A.err_scale = o.err_scale;
   A.err_c = o.err_c;
   A.err_smin = o.err_smin;
   A.tl_c = o.tl_c;
@@ -1102,7 +1140,8 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
                             hipMemcpyHostToDevice, rt->stream));
       int const grid = rt->n_cu * (o.precision == Y3G_FP64 ? 1 : 2);
       HIPCHK(run_single(sa, igs[i], o.precision,
-                        std::max(o.max_split_per_step, kSingleKmax), o.init_split, grid,
+                        std::max(o.max_split_per_step, kSingleKmax), o.init_split,
+                        o.init_split_lnm, grid,
                         rt->stream, &host[i], &nev_single));
     }
     // the VEGAS results were written on the device; cubature ones on the host
@@ -1167,7 +1206,8 @@ int y3g_runtime_memory(y3g_runtime* rt, size_t* free_bytes, size_t* total_bytes,
   if (runtime_bytes) {
     size_t const np = (size_t)std::max(rt->n_points, 0);
     size_t b = np * (sizeof(PointDev) + (size_t)rt->hmf_stride * 12 +
-                     (size_t)rt->da_stride * 12 + (size_t)rt->sci_stride * 12);
+                     (size_t)rt->hmfc_stride * 12 + (size_t)rt->da_stride * 12 +
+                     (size_t)rt->sci_stride * 12);
     b += (size_t)rt->slab_slots * rt->slab_cap * (2 * kMaxDim * 8 + 3 * 8 + 1);
     b += rt->batch_cap * (sizeof(y3g_integral) + sizeof(y3g_result) + sizeof(int));
     b += (size_t)rt->sg_cap * (2 * kMaxDim * 8 + 6 * 8);
@@ -1708,12 +1748,16 @@ int stream_open(y3g_runtime* rt, const StreamConfig& c, y3g_stream** out)
   A.hmf32 = rt->hmf32, A.hmf64 = rt->hmf64, A.hmf_stride = rt->hmf_stride;
   A.da32 = rt->da32, A.da64 = rt->da64, A.da_stride = rt->da_stride;
   A.sci32 = rt->sci32, A.sci64 = rt->sci64, A.sci_stride = rt->sci_stride;
+  if (c.options.hmf_smooth && smooth_ok(c.spec))
+    A.hmfc32 = rt->hmfc32, A.hmfc64 = rt->hmfc64, A.hmfc_stride = rt->hmfc_stride;
   A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
   A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
   A.slab_ax = rt->slab_ax;
   A.cap = c.options.max_regions;
   A.kmax = c.options.max_split_per_step;
   A.init_split = c.options.init_split;
+  A.init_lnm = c.options.init_split_lnm;
+  A.err_scale = c.options.err_scale;
   A.err_c = c.options.err_c, A.err_smin = c.options.err_smin;
   A.tl_c = c.options.tl_c, A.tl_smin = c.options.tl_smin;
   A.nevals = rt->d_nevals;
@@ -1731,6 +1775,7 @@ int stream_open(y3g_runtime* rt, const StreamConfig& c, y3g_stream** out)
   Z.tab.hmf64 = rt->hmf64, Z.tab.hmf32 = rt->hmf32, Z.tab.hmf_stride = rt->hmf_stride;
   Z.tab.da64 = rt->da64, Z.tab.da32 = rt->da32, Z.tab.da_stride = rt->da_stride;
   Z.tab.sci64 = rt->sci64, Z.tab.sci32 = rt->sci32, Z.tab.sci_stride = rt->sci_stride;
+  Z.tab.hmfc64 = rt->hmfc64, Z.tab.hmfc32 = rt->hmfc32, Z.tab.hmfc_stride = rt->hmfc_stride;
   Z.ring_cap = (unsigned)c.ring_items;
   Z.idle_ticks = (long long)(c.idle_ms * khz);
   Z.wait_ticks = (long long)(c.table_wait_ms * khz);
@@ -1909,6 +1954,8 @@ struct y3g_graph {
   PointDev* d_points = nullptr;
   double *hmf64 = nullptr, *da64 = nullptr, *sci64 = nullptr;
   float *hmf32 = nullptr, *da32 = nullptr, *sci32 = nullptr;
+  double* hmfc64 = nullptr;
+  float* hmfc32 = nullptr;
   y3g_integral* d_igs = nullptr;
   y3g_result* d_res = nullptr;
   int* d_order = nullptr;
@@ -1929,6 +1976,7 @@ static void graph_free(y3g_graph* g)
   dfree(g->d_points);
   dfree(g->hmf64), dfree(g->da64), dfree(g->sci64);
   dfree(g->hmf32), dfree(g->da32), dfree(g->sci32);
+  dfree(g->hmfc64), dfree(g->hmfc32);
   dfree(g->d_igs), dfree(g->d_res), dfree(g->d_order), dfree(g->d_counters);
   delete g;
 }
@@ -1975,6 +2023,7 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
     return bail(rc);
   g->slab_gen = rt->slab_gen;
   size_t const hs = (size_t)spec->n_lnm * spec->n_z;
+  size_t const hcs = (size_t)hmfc_stride_of(*spec);
   auto pin = [](auto** p, size_t cnt) {
     return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(**p) * cnt, hipHostMallocDefault);
   };
@@ -1987,7 +2036,8 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
       dalloc(&g->da32, (size_t)n_points * spec->n_da) ||
       dalloc(&g->sci64, (size_t)n_points * spec->n_sci) ||
       dalloc(&g->sci32, (size_t)n_points * spec->n_sci) || dalloc(&g->d_igs, n) ||
-      dalloc(&g->d_res, n) || dalloc(&g->d_order, n) || dalloc(&g->d_counters, 4))
+      dalloc(&g->d_res, n) || dalloc(&g->d_order, n) || dalloc(&g->d_counters, 4) ||
+      dalloc(&g->hmfc64, n_points * hcs) || dalloc(&g->hmfc32, n_points * hcs))
     return bail(Y3G_ERR_MEMORY);
   TableArgs T;
   T.points = g->d_points;
@@ -1995,6 +2045,7 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
   T.hmf64 = g->hmf64, T.hmf32 = g->hmf32, T.hmf_stride = (int)hs;
   T.da64 = g->da64, T.da32 = g->da32, T.da_stride = spec->n_da;
   T.sci64 = g->sci64, T.sci32 = g->sci32, T.sci_stride = spec->n_sci;
+  T.hmfc64 = g->hmfc64, T.hmfc32 = g->hmfc32, T.hmfc_stride = (int)hcs;
   KernArgs A;
   std::memset(&A, 0, sizeof(A));
   A.igs = g->d_igs;
@@ -2004,12 +2055,16 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
   A.hmf32 = g->hmf32, A.hmf64 = g->hmf64, A.hmf_stride = (int)hs;
   A.da32 = g->da32, A.da64 = g->da64, A.da_stride = spec->n_da;
   A.sci32 = g->sci32, A.sci64 = g->sci64, A.sci_stride = spec->n_sci;
+  if (o.hmf_smooth && smooth_ok(*spec))
+    A.hmfc32 = g->hmfc32, A.hmfc64 = g->hmfc64, A.hmfc_stride = (int)hcs;
   A.slab_lo = rt->slab_lo, A.slab_hi = rt->slab_hi;
   A.slab_est = rt->slab_est, A.slab_err = rt->slab_err, A.slab_raw = rt->slab_raw;
   A.slab_ax = rt->slab_ax;
   A.cap = o.max_regions;
   A.kmax = o.max_split_per_step;
   A.init_split = o.init_split;
+  A.init_lnm = o.init_split_lnm;
+  A.err_scale = o.err_scale;
   A.err_c = o.err_c, A.err_smin = o.err_smin;
   A.tl_c = o.tl_c, A.tl_smin = o.tl_smin;
   A.queue = reinterpret_cast<int*>(g->d_counters);

@@ -17,14 +17,15 @@ namespace y3g {
 
 constexpr int kTile = 4096;  // regions per block in the compaction kernels
 
-// initial k^d partition
-__global__ void k_s_init(SingleArgs A, int k, int N)
+// initial partition (init_pieces per axis; one region when one == 1)
+__global__ void k_s_init(SingleArgs A, int init_split, int init_lnm, int one, int N)
 {
   int const r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
   y3g_integral const& ig = *A.ig;
   int rr = r;
   for (int a = 0; a < ig.ndim; ++a) {
+    int const k = one ? 1 : init_pieces(ig.kind, ig.ndim, init_split, init_lnm, a);
     int const q = rr % k;
     rr /= k;
     double const L = ig.lows[a], H = ig.highs[a];
@@ -37,7 +38,7 @@ __global__ void k_s_init(SingleArgs A, int k, int N)
 
 // Evaluate the regions A.list[0 .. nlist) (pairs: [A0, B0, A1, B1, ...]
 // with the parents' est/err in A.pest/A.perr).
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, bool SM>
 __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int pairs)
 {
   extern __shared__ char smem_raw[];
@@ -50,8 +51,11 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
     if constexpr (sizeof(T) == 4) pair_setup(S);
   }
   __syncthreads();
-  if (tid == 0)
+  if (tid == 0) {
     S.hmf_tab = S.kHmfLds ? S.hmf : TabSel<T>::hmf(A.K) + (size_t)S.ig.point * A.K.hmf_stride;
+    const T* hc = TabSel<T>::hmfc(A.K);
+    S.hmfc_tab = hc ? hc + (size_t)S.ig.point * A.K.hmfc_stride : nullptr;
+  }
   if constexpr (sizeof(T) == 4)
     for (int k = tid; k < kNfwN; k += BLOCK) S.nfwtab[k] = kNfwTab[k];
   stage_tables<T>(A.K, S.ig.point, S.kHmfLds ? S.hmf : nullptr, S.da, S.sci);
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_s_eval(SingleArgs A, int nlist, int p
       S.h[r][a] = 0.5 * (h - l);
     }
     __syncthreads();
-    eval_chunk<T, BLOCK>(S, m, A.K.err_c, A.K.err_smin);
+    eval_chunk<T, BLOCK, SM>(S, m, A.K.err_c, A.K.err_smin);
     if (!pairs) {
       if (tid < m) {
         int const g = A.list[base + tid];
@@ -237,17 +241,24 @@ __global__ void k_s_split(SingleArgs A, int N, int ns)
   A.list[2 * j + 1] = N + j;
 }
 
-template <typename T>
-static hipError_t eval(const SingleArgs& A, int nlist, int pairs, int grid, hipStream_t s)
+template <typename T, bool SM>
+static hipError_t eval_sm(const SingleArgs& A, int nlist, int pairs, int grid, hipStream_t s)
 {
   size_t const smem = sizeof(CubShared<T, 512>);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s_eval<T, 512>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s_eval<T, 512, SM>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e != hipSuccess) return e;
   int const nch = (nlist + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL((k_s_eval<T, 512>), dim3(std::max(1, std::min(grid, nch))), dim3(512),
+  hipLaunchKernelGGL((k_s_eval<T, 512, SM>), dim3(std::max(1, std::min(grid, nch))), dim3(512),
                      smem, s, A, nlist, pairs);
   return hipGetLastError();
+}
+template <typename T>
+static hipError_t eval(const SingleArgs& A, int nlist, int pairs, int grid, hipStream_t s)
+{
+  if constexpr (sizeof(T) == 4)
+    if (A.K.hmfc32) return eval_sm<T, true>(A, nlist, pairs, grid, s);
+  return eval_sm<T, false>(A, nlist, pairs, grid, s);
 }
 
 #define SCHK(x)                       \
@@ -257,27 +268,29 @@ static hipError_t eval(const SingleArgs& A, int nlist, int pairs, int grid, hipS
   } while (0)
 
 hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int kmax,
-                      int init_split, int eval_grid, hipStream_t s, y3g_result* out,
-                      long long* nevals)
+                      int init_split, int init_lnm, int eval_grid, hipStream_t s,
+                      y3g_result* out, long long* nevals)
 {
   int const d = ig.ndim;
   int const nn = gm_num_nodes(d);
   int const cap = A.cap;
-  int k = init_split > 0 ? init_split : (d <= 3 ? 4 : 2);
   long long N0 = 1;
-  for (int a = 0; a < d; ++a) N0 *= k;
-  if (N0 > cap) k = 1, N0 = 1;
-  int N = (int)N0;
+  for (int a = 0; a < d; ++a) N0 *= init_pieces(ig.kind, d, init_split, init_lnm, a);
+  int const one = N0 > cap;
+  int N = one ? 1 : (int)N0;
   auto ev = [&](int nlist, int pairs) {
     return precision == Y3G_FP64 ? eval<double>(A, nlist, pairs, eval_grid, s)
                                  : eval<float>(A, nlist, pairs, eval_grid, s);
   };
-  hipLaunchKernelGGL(k_s_init, dim3((N + 255) / 256), dim3(256), 0, s, A, k, N);
+  hipLaunchKernelGGL(k_s_init, dim3((N + 255) / 256), dim3(256), 0, s, A, init_split, init_lnm,
+                     one, N);
   SCHK(hipGetLastError());
   SCHK(ev(N, 0));
   long long neval = (long long)N * nn;
   int status = 1;
   double IE[2] = {0.0, 0.0};
+  double const sb = smooth_bound(ig.kind, A.K.hmfc32 != nullptr);
+  double const es = sb > 0.0 ? A.K.err_scale : 1.0;
   int const nred = 256;
   std::vector<int> hc(64), cnt;
   std::vector<unsigned long long> hq(64);
@@ -289,7 +302,8 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
     SCHK(hipMemcpyAsync(IE, A.IE, sizeof(IE), hipMemcpyDeviceToHost, s));
     SCHK(hipStreamSynchronize(s));
     double const I = IE[0], E = IE[1];
-    if (E <= std::max(ig.eps_abs, ig.eps_rel * std::fabs(I))) {
+    if (es * E + sb * std::fabs(I) <=
+        std::max(ig.eps_abs, ig.eps_rel * std::fabs(I))) {
       status = 0;
       break;
     }
@@ -337,7 +351,7 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
     neval += 2LL * ns * nn;
   }
   out->value = IE[0];
-  out->error = IE[1];
+  out->error = es * IE[1] + sb * std::fabs(IE[0]);
   out->prob = 0.0;
   out->neval = neval;
   out->nregions = N;

@@ -27,7 +27,7 @@ struct SingleArgs {
 };
 
 hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int kmax,
-                      int init_split, int eval_grid, hipStream_t s, y3g_result* out,
-                      long long* nevals);
+                      int init_split, int init_lnm, int eval_grid, hipStream_t s,
+                      y3g_result* out, long long* nevals);
 
 }  // namespace y3g
