@@ -63,10 +63,10 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # HBM bytes per integrand evaluation of k_cubature<float,512> on this
 # workload: rocprofv3 FETCH_SIZE x 1.99974 + WRITE_SIZE x 1.000 (factors from
 # a 256 MiB copy read by the same counters) over the evaluations of the same
-# launch, a 205-point step: 459,102,707 KiB fetched, 296,943,521 KiB
-# written, 3.72343e11 evaluations (profiles/k_cubature_prior_stream_pmc.md).
+# launch, a 205-point step: 414,425,802 KiB fetched, 281,454,590 KiB
+# written, 3.41475e11 evaluations (profiles/k_cubature_prior_stream_pmc.md).
 # Traffic per launch = this x the launch's evaluations.
-HBM_BYTES_PER_EVAL = (459102706.75 * 1.99974 + 296943520.95) * 1024 / 3.72343e11
+HBM_BYTES_PER_EVAL = (414425802.12 * 1.99974 + 281454589.66) * 1024 / 3.41474594589e11
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_cubature<float,512>, "
                   "calibrated by a 256 MiB copy; bytes per evaluation x this launch's "
                   "evaluations")

@@ -890,15 +890,15 @@ Y3G_HD PairF<T> pair_combine(const y3g_integral& ig, const PointDev& P, Tabs<T> 
   T hm;
   if constexpr (SM) {
     // hat-smoothed HMF (hmf_smooth): the 6 x 6 coefficients around the cell
-    // row by row (not unrolled, so that a row's six loads are in flight at
-    // a time), the row's z weight by Horner from kB5Poly
+    // row by row, three rows' loads in flight at a time (unrolling all six
+    // spills more and ran slower), the row's z weight by Horner from kB5Poly
     T wx[6];
     b5_weights<T>(m.tm, wx);
     T const fz = z.sz;
     int const ld = P.nm + 4;
     GPtr row = (GPtr)(tb.hmfc + z.jz * ld + m.im);
     T h = T(0);
-#pragma unroll 1
+#pragma unroll 3
     for (int l = 0; l < 6; ++l, row += ld) {
       T r = wx[0] * row[0];
 #pragma unroll
