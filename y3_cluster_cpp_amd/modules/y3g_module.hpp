@@ -386,6 +386,8 @@ namespace y3g_cosmosis {
     m.options.vegas_calls =
       (long long)optional_double(cfg, L, "vegas_calls", double(m.options.vegas_calls));
     m.options.vegas_itmax = int(optional_double(cfg, L, "vegas_itmax", m.options.vegas_itmax));
+    // the synthetic integrands' HMF: hat-smoothed (1, default) or bilinear (0)
+    m.options.hmf_smooth = optional_double(cfg, L, "hmf_smooth", m.options.hmf_smooth) != 0.0;
     // tolerance-band budgets (the autotuner's bands): ascending eps edges
     if (cfg.has_val(L, "band_eps") || cfg.has_val(L, "band_regions")) {
       auto const be = get_vector_double(cfg, L, "band_eps");
