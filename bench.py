@@ -143,7 +143,7 @@ def band_limit(kind_ndim, eps, bands):
     return None
 
 
-def cpu_baseline(pts, eps, sample, max_regions, threads, bands=None):
+def cpu_baseline(pts, eps, sample, max_regions, threads, bands=None, ext=(0, 0.0)):
     """The oracle's FP64 integrands under its CPU restatement of the same
     cubature (same region budget and tolerance-band call limits), one
     integral per OpenMP thread, on `sample`."""
@@ -179,7 +179,8 @@ def cpu_baseline(pts, eps, sample, max_regions, threads, bands=None):
     used = O.lib.y3o_batch_integrate(pa, sa, len(params), arr, (C.c_int * n)(*pt), O.ptr(e),
                                      me, n, 0,
                                      C.byref(O.cub_opts(max_regions=max_regions,
-                                                        max_eval=1 << 40)),
+                                                        max_eval=1 << 40, ext_regions=ext[0],
+                                                        ext_ratio=ext[1])),
                                      C.byref(O.vegas_opts()), threads, out)
     dt = time.perf_counter() - t0
     res = np.array([(r.value, r.error) for r in out], dtype=[("value", "f8"), ("error", "f8")])
@@ -456,6 +457,12 @@ def main():
                     help="per-integral region budget of the cubature (0: the tuned "
                          "bands when y3_cluster_cpp_amd/tuning/gfx950.json exists, "
                          "else %d)" % DEFAULT_REGIONS)
+    ap.add_argument("--ext-regions", type=int, default=-1,
+                    help="budget extension: regions an integral far from its tolerance "
+                         "continues to (-1: the tuned value, 0: off)")
+    ap.add_argument("--ext-ratio", type=float, default=-1.0,
+                    help="budget extension: error / tolerance above which a spent budget "
+                         "is extended (-1: the tuned value)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore the autotuner's file (fixed budget for every integral)")
     ap.add_argument("--dump", default="",
@@ -468,10 +475,12 @@ def main():
                     help="run the steps by eager calls instead of HIP graph replays")
     ap.add_argument("--no-as-specified", action="store_true",
                     help="skip the run at the stream's own (uncapped) eps_rel")
+    ap.add_argument("--no-base-only", action="store_true",
+                    help="skip the run at the base budgets without the extension")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=48)
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--serve-rates", default="4,12,24,48,96")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--no-selgl", action="store_true")
@@ -540,6 +549,15 @@ def main():
     else:
         topts = {"max_regions": a.max_regions or DEFAULT_REGIONS}
         tuning = None
+    if a.ext_regions >= 0:
+        topts["ext_regions"] = a.ext_regions
+    if a.ext_ratio >= 0:
+        topts["ext_ratio"] = a.ext_ratio
+    if topts.get("ext_regions", 0) > topts["max_regions"]:
+        topts["max_regions"] = topts["ext_regions"]  # the slab holds the extended budget
+    if tuning is not None:
+        tuning.update({k: topts[k] for k in ("ext_regions", "ext_ratio", "max_regions")
+                       if k in topts})
     opts = y3.options(precision=y3.FP32SPLIT, **topts)
     bands = topts.get("bands")
 
@@ -661,6 +679,33 @@ def main():
         print(f"as-specified run: {n_s} integrals in {el_s:.2f} s, counted "
               f"{int(ok_s.sum())}/{len(ok_s)}", file=sys.stderr, flush=True)
 
+    # the same stream at the base budgets alone (no extension): the
+    # throughput the extension trades for the integrals it brings within
+    # their tolerance
+    base_only = None
+    if opts.ext_regions > 0 and not a.no_base_only:
+        bo = y3.options(precision=y3.FP32SPLIT,
+                        **{**topts, "ext_regions": 0, "ext_ratio": 0.0,
+                           "max_regions": max([b for _, b in bands] or [DEFAULT_REGIONS])})
+        ok_b = []
+        t1 = time.perf_counter()
+        for s_i, sl in enumerate(timed):
+            params, arr = batches[s_i]
+            rt.tables_build(params, spec)
+            r = rt.integrate(arr, bo)
+            for k in checks[s_i]:
+                loc = k - int(sl[0])
+                rr = r[loc * N_PER_POINT:(loc + 1) * N_PER_POINT]
+                ok_b.append(count_ok(rr, gold[k], eps[k])[0])
+        el_b = time.perf_counter() - t1
+        ok_b = np.concatenate(ok_b)
+        base_only = {"value": float(ok_b.mean()) * n_int / el_b, "unit": "integrals/s",
+                     "counted_frac": float(ok_b.mean()), "integrals_per_s_raw": n_int / el_b,
+                     "seconds": el_b, "launch": "eager calls",
+                     "budgets": "the tolerance bands' budgets, no extension"}
+        print(f"base budgets alone: {n_int} integrals in {el_b:.2f} s, counted "
+              f"{int(ok_b.sum())}/{len(ok_b)}", file=sys.stderr, flush=True)
+
     # samples of the checked integrals for the two baselines
     sample = sample_checked(checked, max(a.plain_sample, a.cpu_sample))
     plain = None
@@ -699,7 +744,8 @@ def main():
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         sub_idx = sample[: a.cpu_sample]
-        res, used, dt = cpu_baseline(pts, eps, sub_idx, opts.max_regions, threads, bands)
+        res, used, dt = cpu_baseline(pts, eps, sub_idx, opts.max_regions, threads, bands,
+                                     (opts.ext_regions, opts.ext_ratio))
         okc, _ = count_ok(res, np.array([gold[k][i] for k, i in sub_idx]),
                           np.array([eps[k, i] for k, i in sub_idx]))
         cpu = {"value": int(okc.sum()) / dt, "unit": "integrals/s", "cores": int(used),
@@ -784,6 +830,7 @@ def main():
                      "flops_per_eval": FLOPS_PER_EVAL, "sfu_per_eval": SFU_PER_EVAL,
                      "evals_per_s": nev_total / (k_tot * 1e-3) if k_tot > 0 else 0.0},
         "as_specified": as_spec,
+        "base_budgets_only": base_only,
         "cpu_baseline": cpu,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (value / plain["counted_per_s"])

@@ -208,6 +208,15 @@ typedef struct {
                              and needs this margin to stay above the
                              true error (0 of 12,288 below it, against 2
                              at 1.0)                                     */
+  int ext_regions;        /* budget extension (cubature; 0 = off): an
+                             integral whose call limit is spent while
+                             its reported error is still above
+                             ext_ratio x its tolerance continues, once,
+                             up to ext_regions regions (<= max_regions).
+                             The base budget (max_regions or its band)
+                             serves the integrals near their tolerance;
+                             the few left far from it get the larger one */
+  double ext_ratio;
 } y3g_options;
 
 typedef struct y3g_runtime y3g_runtime;

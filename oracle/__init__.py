@@ -57,7 +57,8 @@ class CubOpts(C.Structure):
                 ("max_split_per_step", C.c_int), ("init_split", C.c_int),
                 ("err_c", C.c_double), ("err_smin", C.c_double),
                 ("tl_c", C.c_double), ("tl_smin", C.c_double), ("hmf_smooth", C.c_int),
-                ("init_split_lnm", C.c_int), ("err_scale", C.c_double)]
+                ("init_split_lnm", C.c_int), ("err_scale", C.c_double),
+                ("ext_regions", C.c_int), ("ext_ratio", C.c_double)]
 
 
 class VegasOpts(C.Structure):
@@ -264,9 +265,11 @@ def integrand(params, tables, ig, x):
 
 def cub_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000, max_regions=65536,
              max_split_per_step=1024, init_split=0, err_c=0.0, err_smin=1.0, tl_c=0.0,
-             tl_smin=1.0 / 16.0, hmf_smooth=1, init_split_lnm=4, err_scale=1.5):
+             tl_smin=1.0 / 16.0, hmf_smooth=1, init_split_lnm=4, err_scale=1.5,
+             ext_regions=0, ext_ratio=0.0):
     return CubOpts(eps_rel, eps_abs, max_eval, max_regions, max_split_per_step, init_split,
-                   err_c, err_smin, tl_c, tl_smin, hmf_smooth, init_split_lnm, err_scale)
+                   err_c, err_smin, tl_c, tl_smin, hmf_smooth, init_split_lnm, err_scale,
+                   ext_regions, ext_ratio)
 
 
 def vegas_opts(eps_rel=1e-4, eps_abs=1e-300, max_eval=50_000_000,

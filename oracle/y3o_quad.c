@@ -716,6 +716,16 @@ two_level_scale(double d, double e, double c, double smin)
   return s < smin ? smin : (s > 1.0 ? 1.0 : s);
 }
 
+/* Budget extension (device: ext_budget in y3g_cub_common.hpp): a call
+ * limit lim that would cut the next refinement step while the reported
+ * error err is above ratio x tol is raised once to ext_eval; the integral
+ * then refines as a run with ext_eval from the start. */
+static int
+ext_budget(long long lim, long long ext_eval, double ratio, double err, double tol)
+{
+  return ratio > 0.0 && ext_eval > lim && err > ratio * tol;
+}
+
 static void
 eval_region(const cub_prob* P, region_t* r)
 {
@@ -841,17 +851,24 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
   neval += (long long)n * nn;
   int status = 1;
   double I = 0.0, E = 0.0;
+  long long lim = o->max_eval;
+  long long const ext_eval = o->ext_regions > 0 ? 2LL * o->ext_regions * nn : 0;
   for (;;) {
     I = 0.0;
     E = 0.0;
     for (int k = 0; k < n; ++k) I += R[k].est, E += R[k].err;
-    if (escale * E + P->sb * fabs(I) <= fmax(o->eps_abs, o->eps_rel * fabs(I))) {
+    double const tol = fmax(o->eps_abs, o->eps_rel * fabs(I));
+    if (escale * E + P->sb * fabs(I) <= tol) {
       status = 0;
       break;
     }
     /* at most kmax splits, within the capacity and the call limit */
-    long long const left = (o->max_eval - neval) / (2LL * nn);
     int room = (kmax < cap - n) ? kmax : cap - n;
+    long long left = (lim - neval) / (2LL * nn);
+    if (left < room && ext_budget(lim, ext_eval, o->ext_ratio, escale * E + P->sb * fabs(I), tol)) {
+      lim = ext_eval;
+      left = (lim - neval) / (2LL * nn);
+    }
     if (left < room) room = (int)left;
     if (room <= 0) break;
     int eE, hc[64], B, lim0, lim1;
@@ -866,7 +883,7 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
     }
     select_bins(hc, hq, n, room, &B, &lim0, &lim1);
     int const ns = lim0 + lim1;
-    if (ns <= 0 || neval + 2LL * ns * nn > o->max_eval) break;
+    if (ns <= 0 || neval + 2LL * ns * nn > lim) break;
     /* ordered selection: bins < B (up to lim0), then bin B (up to lim1) */
     int m = 0;
     for (int k = 0; k < n && m < lim0; ++k)

@@ -382,6 +382,43 @@ def test_large_batch_independence(runtime):
     assert counted.mean() >= 0.94, counted.mean()
 
 
+def test_budget_extension(runtime):
+    """Budget extension: an integral whose error is still above ext_ratio x
+    its tolerance when its band budget is spent continues to ext_regions;
+    one within that ratio stops at the band budget.  The batched FP64 path
+    equals the oracle's CPU restatement (regions, evaluations, status,
+    value), the plain path honours the same rule, and an ext_regions above
+    max_regions is refused."""
+    p, s = fid(runtime, wide=False)
+    op = O.default_params(f_prj=0.0)
+    ot = O.OracleTables(op, O.spec_config0())
+    oig = O.make_integral(op, O.spec_config0(), O.COUNT3D)
+    nn = 33
+    base, ext = 1024, 4096
+    ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=1e-9)
+    # the error at the base budget is ~1e-5 relative, 1e4 x the tolerance
+    for ratio, extended in ((2.0, True), (1e9, False)):
+        o = dict(bands=[(1e-3, base)], max_regions=ext, ext_regions=ext, ext_ratio=ratio)
+        g = runtime.integrate([ig], y3.options(precision=y3.FP64, **o))[0]
+        c = O.cubature(op, ot, oig, O.cub_opts(eps_rel=1e-9, max_regions=ext,
+                                               max_eval=2 * base * nn, ext_regions=ext,
+                                               ext_ratio=ratio))
+        assert (g["nregions"], g["neval"], g["status"]) == (c.nregions, c.neval, c.status)
+        assert abs(g["value"] / c.value - 1) < 1e-11
+        if extended:
+            assert 2 * base * nn < g["neval"] <= 2 * ext * nn
+        else:
+            assert g["neval"] <= 2 * base * nn
+        # the plain path and the FP32 integrand follow the same rule (their
+        # sums differ from the FP64 batched ones in the last bits)
+        for q in (dict(precision=y3.FP64, single=1), dict(precision=y3.FP32SPLIT)):
+            h = runtime.integrate([ig], y3.options(**q, **o))[0]
+            assert h["neval"] <= 2 * ext * nn
+            assert (h["neval"] > 2 * base * nn) == extended
+    with pytest.raises(RuntimeError, match="ext_regions"):
+        runtime.integrate([ig], y3.options(max_regions=1024, ext_regions=2048, ext_ratio=2.0))
+
+
 def test_tolerance_bands(runtime):
     """Tolerance-band budgets: an integral gets the budget of the first band
     whose edge is >= its eps_rel (per-integral call limit 2 x regions x

@@ -237,3 +237,39 @@ def test_mapped_integrand_equals_physical():
             mapped = L.y3o_integrand_mapped(C.byref(p), C.byref(t.t), C.byref(ig),
                                             O.ptr(np.ascontiguousarray(y)))
             assert abs(mapped - phys * jac) <= 1e-12 * abs(mapped) + 1e-300
+
+
+def test_budget_extension_equals_larger_budget():
+    """The budget extension continues the same refinement: an integral that
+    spends its call limit with the error above ext_ratio x tol ends exactly
+    where a run with the extended limit from the start ends (regions,
+    evaluations, value); with the error within that ratio, or the ratio
+    out of reach, it stops at the base limit."""
+    p = O.default_params(f_prj=0.0)
+    s = O.spec_config0()
+    t = O.OracleTables(p, s)
+    ig = O.make_integral(p, s, O.COUNT3D)
+    nn = 33
+    base, ext = 1024, 4096
+    plain = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=ext,
+                                            max_eval=2 * base * nn))
+    big = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=ext, max_eval=2 * ext * nn))
+    extd = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=ext,
+                                           max_eval=2 * base * nn, ext_regions=ext,
+                                           ext_ratio=2.0))
+    assert plain.neval <= 2 * base * nn < extd.neval
+    assert (extd.neval, extd.nregions, extd.status) == (big.neval, big.nregions, big.status)
+    assert extd.value == big.value and extd.error == big.error
+    # the error at the base limit is ~1e-5 relative (1e4 x tol): a ratio
+    # above that keeps the base limit
+    far = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=ext,
+                                          max_eval=2 * base * nn, ext_regions=ext,
+                                          ext_ratio=1e9))
+    assert (far.neval, far.value) == (plain.neval, plain.value)
+    # an integral that converges within its base limit is untouched
+    loose = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-3, max_regions=ext,
+                                            max_eval=2 * base * nn, ext_regions=ext,
+                                            ext_ratio=2.0))
+    ref = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-3, max_regions=ext, max_eval=2 * base * nn))
+    assert ref.status == 0 and ref.neval < 2 * base * nn
+    assert loose.status == 0 and (loose.neval, loose.value) == (ref.neval, ref.value)

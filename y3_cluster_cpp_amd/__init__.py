@@ -71,7 +71,8 @@ class Options(C.Structure):
                 ("n_bands", C.c_int), ("band_eps", C.c_double * 8),
                 ("band_regions", C.c_int * 8),
                 ("tl_c", C.c_double), ("tl_smin", C.c_double), ("hmf_smooth", C.c_int),
-                ("init_split_lnm", C.c_int), ("err_scale", C.c_double)]
+                ("init_split_lnm", C.c_int), ("err_scale", C.c_double),
+                ("ext_regions", C.c_int), ("ext_ratio", C.c_double)]
 
 
 class SelGLSpec(C.Structure):

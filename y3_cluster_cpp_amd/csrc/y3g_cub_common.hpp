@@ -701,6 +701,19 @@ __host__ __device__ inline double two_level_scale(double d, double e, double c, 
   return s < smin ? smin : (s > 1.0 ? 1.0 : s);
 }
 
+// Budget extension: an integral whose call limit `lim` would cut its next
+// refinement step while its reported error `err` is still above ratio x its
+// tolerance `tol` gets the larger limit ext_eval, once (lim < ext_eval); it
+// then refines exactly as a run with ext_eval from the start.  Such integrals are the ones
+// the base budget leaves far from converged; those near their tolerance,
+// whose |R7-R5| sum over-states the true error, stop at the base budget.
+// Restated by the oracle (y3o_quad.c ext_budget).
+__host__ __device__ inline bool ext_budget(long long lim, long long ext_eval, double ratio,
+                                           double err, double tol)
+{
+  return ratio > 0.0 && ext_eval > lim && err > ratio * tol;
+}
+
 // Evaluate the m regions whose centre / half-widths are in S.c / S.h:
 // S.est, S.err, S.ax.
 template <typename T, int BLOCK, bool SM = false>

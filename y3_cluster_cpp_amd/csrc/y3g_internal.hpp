@@ -64,6 +64,8 @@ struct KernArgs {
   int init_split;
   int init_lnm;   // initial pieces along the synthetic kinds' lnM axis (0: init_split)
   double err_scale;  // safety factor on the summed error with the smoothed HMF
+  double ext_ratio;  // budget extension: error / tolerance above which ...
+  int ext_regions;   // ... a spent budget is raised to this many regions (0: off)
   double err_c, err_smin;
   double tl_c, tl_smin;
   // VEGAS

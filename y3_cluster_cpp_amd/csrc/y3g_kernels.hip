@@ -449,8 +449,13 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       I = a;
       E = b;
     }
+    // the call limit; raised once to ext_eval when the budget runs out with
+    // the estimate still above ext_ratio x the tolerance (ext_budget)
+    long long lim = S.ig.max_eval;
+    long long const ext_eval = A.ext_regions > 0 ? 2LL * A.ext_regions * nn : 0;
     for (;;) {
-      if (es * E + sb * fabs(I) <= fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I))) {
+      double const tol = fmax(S.ig.eps_abs, S.ig.eps_rel * fabs(I));
+      if (es * E + sb * fabs(I) <= tol) {
         status = 0;
         break;
       }
@@ -459,8 +464,13 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       // split bins 0..B-1 and, in index order, bin B, where B covers at
       // least kAlpha of E and at least max(64, N/8) regions, at most `room`
       // at most kmax splits, within the slab and the call limit
-      long long const left = (S.ig.max_eval - neval) / (2LL * nn);
-      int const room = (int)min((long long)min(A.kmax, cap - N), left);
+      int const room0 = min(A.kmax, cap - N);
+      long long left = (lim - neval) / (2LL * nn);
+      if (left < room0 && ext_budget(lim, ext_eval, A.ext_ratio, es * E + sb * fabs(I), tol)) {
+        lim = ext_eval;
+        left = (lim - neval) / (2LL * nn);
+      }
+      int const room = (int)min((long long)room0, left);
       if (room <= 0) break;
       int eE;
       (void)frexp(E, &eE);
@@ -492,7 +502,7 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       __syncthreads();
       int const B = S.selB, lim0 = S.lim0, lim1 = S.lim1;
       int const ns = lim0 + lim1;
-      if (ns <= 0 || neval + 2LL * ns * nn > S.ig.max_eval) break;
+      if (ns <= 0 || neval + 2LL * ns * nn > lim) break;
       // ordered compaction: bins < B in index order to sel[0, lim0), bin B
       // in index order to sel[lim0, lim0 + lim1)
       {

@@ -336,6 +336,8 @@ void y3g_options_default(y3g_options* o)
   o->hmf_smooth = 1;
   o->init_split_lnm = 4;
   o->err_scale = 1.5;
+  o->ext_ratio = 0.0;
+  o->ext_regions = 0;
 }
 
 void y3g_default_params(y3g_params* p)
@@ -853,17 +855,22 @@ static int ensure_single(y3g_runtime* rt, int cap)
   return 0;
 }
 
-// Largest expected cost first: dimension, then, point by point, tighter
-// tolerance.  Keeping a point's integrals together keeps the few points the
+// Largest expected cost first: dimension, then the tight tolerances
+// (eps_rel < 5e-5: the integrals a budget extension most often takes to
+// four times their base budget) before the others, then point by point,
+// tighter tolerance first.  The long integrals start early and the short
+// ones fill the end of the launch, when workgroups run out of work; keeping
+// a point's integrals together within each group keeps the few points the
 // resident workgroups are on at a time in L2 (their HMF coefficients: 29 KB
-// per point, read by every pair factor); measured 1% faster than a
-// tolerance-first order over all points, with the same results.
+// per point, read by every pair factor).
 static void lpt_order(const y3g_integral* igs, const std::vector<int>& ids,
                       std::vector<int>& order)
 {
   order = ids;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (igs[a].ndim != igs[b].ndim) return igs[a].ndim > igs[b].ndim;
+    bool const ta = igs[a].eps_rel < 5e-5, tb = igs[b].eps_rel < 5e-5;
+    if (ta != tb) return ta;
     if (igs[a].point != igs[b].point) return igs[a].point < igs[b].point;
     return igs[a].eps_rel < igs[b].eps_rel;
   });
@@ -1017,6 +1024,9 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
       return fail(Y3G_ERR_ARG, std::string(who) +
                                  ": tolerance bands need ascending edges and "
                                  "budgets in [64, max_regions]");
+  if (o.ext_regions < 0 || o.ext_regions > o.max_regions || !(o.ext_ratio >= 0.0))
+    return fail(Y3G_ERR_ARG, std::string(who) +
+                               ": ext_regions must lie in [0, max_regions] and ext_ratio >= 0");
   if (noalloc && o.single)
     return fail(Y3G_ERR_ARG, "y3g_integrate_noalloc: the plain path (single) allocates its "
                              "own workspace; use y3g_integrate");
@@ -1106,6 +1116,8 @@ static int integrate_call(y3g_runtime* rt, const y3g_integral* igs, size_t n,
   A.init_split = o.init_split;
   A.init_lnm = o.init_split_lnm;
   A.err_scale = o.err_scale;
+  A.ext_ratio = o.ext_ratio;
+  A.ext_regions = o.ext_regions;
   A.err_c = o.err_c;
   A.err_smin = o.err_smin;
   A.tl_c = o.tl_c;
@@ -1661,7 +1673,8 @@ static int stream_shape(y3g_runtime* rt, const StreamConfig& c, int* wg, size_t*
   if (c.options.precision != Y3G_FP32SPLIT && c.options.precision != Y3G_FP64)
     return fail(Y3G_ERR_ARG, "stream: precision must be fp32split or fp64");
   if (c.options.max_regions < 64 || c.options.max_split_per_step < 1 ||
-      c.options.max_split_per_step > 1024)
+      c.options.max_split_per_step > 1024 || c.options.ext_regions < 0 ||
+      c.options.ext_regions > c.options.max_regions || !(c.options.ext_ratio >= 0.0))
     return fail(Y3G_ERR_ARG, "stream: options out of range");
   if (int rc = check_spec(&c.spec)) return rc;
   HIPCHK(hipSetDevice(rt->device));
@@ -1758,6 +1771,8 @@ int stream_open(y3g_runtime* rt, const StreamConfig& c, y3g_stream** out)
   A.init_split = c.options.init_split;
   A.init_lnm = c.options.init_split_lnm;
   A.err_scale = c.options.err_scale;
+  A.ext_ratio = c.options.ext_ratio;
+  A.ext_regions = c.options.ext_regions;
   A.err_c = c.options.err_c, A.err_smin = c.options.err_smin;
   A.tl_c = c.options.tl_c, A.tl_smin = c.options.tl_smin;
   A.nevals = rt->d_nevals;
@@ -1995,6 +2010,7 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
   if (opt) o = *opt;
   if (o.single || o.max_regions < 64 || o.max_split_per_step < 1 ||
       o.max_split_per_step > 1024 || o.n_bands < 0 || o.n_bands > 8 ||
+      o.ext_regions < 0 || o.ext_regions > o.max_regions || !(o.ext_ratio >= 0.0) ||
       (o.precision != Y3G_FP32SPLIT && o.precision != Y3G_FP64))
     return fail(Y3G_ERR_ARG, "y3g_graph_create: options out of range (batched cubature, "
                              "fp32split or fp64)");
@@ -2065,6 +2081,8 @@ int y3g_graph_create(y3g_runtime* rt, const y3g_table_spec* spec, int n_points, 
   A.init_split = o.init_split;
   A.init_lnm = o.init_split_lnm;
   A.err_scale = o.err_scale;
+  A.ext_ratio = o.ext_ratio;
+  A.ext_regions = o.ext_regions;
   A.err_c = o.err_c, A.err_smin = o.err_smin;
   A.tl_c = o.tl_c, A.tl_smin = o.tl_smin;
   A.queue = reinterpret_cast<int*>(g->d_counters);

@@ -291,6 +291,9 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
   double IE[2] = {0.0, 0.0};
   double const sb = smooth_bound(ig.kind, A.K.hmfc32 != nullptr);
   double const es = sb > 0.0 ? A.K.err_scale : 1.0;
+  // call limit, raised once by the budget extension (ext_budget)
+  long long lim = ig.max_eval;
+  long long const ext_eval = A.K.ext_regions > 0 ? 2LL * A.K.ext_regions * nn : 0;
   int const nred = 256;
   std::vector<int> hc(64), cnt;
   std::vector<unsigned long long> hq(64);
@@ -302,13 +305,19 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
     SCHK(hipMemcpyAsync(IE, A.IE, sizeof(IE), hipMemcpyDeviceToHost, s));
     SCHK(hipStreamSynchronize(s));
     double const I = IE[0], E = IE[1];
-    if (es * E + sb * std::fabs(I) <=
-        std::max(ig.eps_abs, ig.eps_rel * std::fabs(I))) {
+    double const tol = std::max(ig.eps_abs, ig.eps_rel * std::fabs(I));
+    if (es * E + sb * std::fabs(I) <= tol) {
       status = 0;
       break;
     }
-    long long const calls_left = (ig.max_eval - neval) / (2LL * nn);
-    int const room = (int)std::min((long long)std::min(kmax, cap - N), calls_left);
+    int const room0 = std::min(kmax, cap - N);
+    long long calls_left = (lim - neval) / (2LL * nn);
+    if (calls_left < room0 &&
+        ext_budget(lim, ext_eval, A.K.ext_ratio, es * E + sb * std::fabs(I), tol)) {
+      lim = ext_eval;
+      calls_left = (lim - neval) / (2LL * nn);
+    }
+    int const room = (int)std::min((long long)room0, calls_left);
     if (room <= 0) break;
     int eE;
     (void)std::frexp(E, &eE);
@@ -322,7 +331,7 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
     int B, lim0, lim1;
     select_bins(hc.data(), hq.data(), N, room, B, lim0, lim1);
     int const ns = lim0 + lim1;
-    if (ns <= 0 || neval + 2LL * ns * nn > ig.max_eval) break;
+    if (ns <= 0 || neval + 2LL * ns * nn > lim) break;
     int const nt = (N + kTile - 1) / kTile;
     hipLaunchKernelGGL(k_s_count<256>, dim3(nt), dim3(256), 0, s, A, N, E, eE, B);
     SCHK(hipGetLastError());
