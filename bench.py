@@ -14,7 +14,11 @@ The timed steps cover the stream exactly once: the 4096 points are cut into
 one slice -- its tables built on the device and its integrals run as one
 persistent cubature launch.  Warm-up steps re-run slice 0 before the clock
 starts.  An integral is counted when |I - I_oracle| <= eps_rel |I_oracle|,
-that error is <= 1e-4, and the reported error bounds it.  I_oracle are the
+that error is <= 1e-4, and the reported error bounds it.  Budgets as tuned
+(y3_cluster_cpp_amd/tuning/gfx950.json): 32,768 regions per integral, and
+the budget extension to 131,072 for the integrals whose reported error is
+still above 10x their tolerance; `base_budgets_only` re-runs the stream
+without the extension.  I_oracle are the
 FP64 oracle's golden values of a stratified subset of the stream -- every
 16th point, 256 points = 49,152 integrals, tests/golden/golden_stream.json
 -- which every step's slice samples; `value` = (counted fraction of that
@@ -61,12 +65,13 @@ FLOPS_PER_EVAL = {0: 51, 1: 83, 2: 136}
 SFU_PER_EVAL = {0: 6, 1: 8, 2: 16}
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # HBM bytes per integrand evaluation of k_cubature<float,512> on this
-# workload: rocprofv3 FETCH_SIZE x 1.99974 + WRITE_SIZE x 1.000 (factors from
+# workload: rocprofv3 FETCH_SIZE x 1.99979 + WRITE_SIZE x 1.000 (factors from
 # a 256 MiB copy read by the same counters) over the evaluations of the same
-# launch, a 205-point step: 414,425,802 KiB fetched, 281,454,590 KiB
-# written, 3.41475e11 evaluations (profiles/k_cubature_prior_stream_pmc.md).
+# launch, a 205-point step with the tuned budgets and budget extension:
+# 1,061,414,897 KiB fetched, 489,082,177 KiB written, 5.53530e11
+# evaluations (profiles/k_cubature_prior_stream_ext.md).
 # Traffic per launch = this x the launch's evaluations.
-HBM_BYTES_PER_EVAL = (414425802.12 * 1.99974 + 281454589.66) * 1024 / 3.41474594589e11
+HBM_BYTES_PER_EVAL = (1061414897.4 * 1.99979 + 489082176.8) * 1024 / 5.53529779267e11
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_cubature<float,512>, "
                   "calibrated by a 256 MiB copy; bytes per evaluation x this launch's "
                   "evaluations")

@@ -764,10 +764,12 @@ err_bin(double err, double E, int eE)
   return b < 0 ? 0 : (b > 62 ? 62 : b);
 }
 
+/* err / E in 2^-40 fixed point, qs = 2^40 / E computed once per step
+ * (device: err_q / err_qscale) */
 static unsigned long long
-err_q(double err, double E)
+err_q(double err, double qs)
 {
-  double const q = floor(err / E * 1099511627776.0);
+  double const q = floor(err * qs);
   return q > 0.0 ? (unsigned long long)q : 0ull;
 }
 
@@ -876,10 +878,11 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
     (void)frexp(E, &eE);
     memset(hc, 0, sizeof(hc));
     memset(hq, 0, sizeof(hq));
+    double const qs = 1099511627776.0 / E;
     for (int k = 0; k < n; ++k) {
       int const b = err_bin(R[k].err, E, eE);
       hc[b] += 1;
-      hq[b] += err_q(R[k].err, E);
+      hq[b] += err_q(R[k].err, qs);
     }
     select_bins(hc, hq, n, room, &B, &lim0, &lim1);
     int const ns = lim0 + lim1;

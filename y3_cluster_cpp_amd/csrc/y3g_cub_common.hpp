@@ -222,10 +222,12 @@ __host__ __device__ inline int err_bin(double err, double E, int eE)
   return b < 0 ? 0 : (b > 62 ? 62 : b);
 }
 
-// err / E in 2^-40 fixed point.
-__host__ __device__ inline unsigned long long err_q(double err, double E)
+// err / E in 2^-40 fixed point, with qs = 2^40 / E (err_qscale) computed
+// once per step: one multiply per region instead of a division.
+__host__ __device__ inline double err_qscale(double E) { return 1099511627776.0 / E; }
+__host__ __device__ inline unsigned long long err_q(double err, double qs)
 {
-  double const q = floor(err / E * 1099511627776.0);
+  double const q = floor(err * qs);
   return q > 0.0 ? (unsigned long long)q : 0ull;
 }
 

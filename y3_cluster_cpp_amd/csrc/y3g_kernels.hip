@@ -485,11 +485,27 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       for (int k = tid; k < NW * 64; k += BLOCK) (&S.wc[0][0])[k] = 0;
       if (tid < 64) S.hq[tid] = 0ull;
       __syncthreads();
-      for (int r = sb + lane; r < se; r += 64) {
-        double const e = err[r];
-        int const b = err_bin(e, E, eE);
-        atomicAdd(&S.wc[wv][b], 1);
-        atomicAdd(&S.hq[b], err_q(e, E));
+      {
+        double const qs = err_qscale(E);
+        int r = sb + lane;
+        // four loads in flight per lane before their bins are counted
+        for (; r + 192 < se; r += 256) {
+          double ev[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) ev[u] = err[r + 64 * u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            int const b = err_bin(ev[u], E, eE);
+            atomicAdd(&S.wc[wv][b], 1);
+            atomicAdd(&S.hq[b], err_q(ev[u], qs));
+          }
+        }
+        for (; r < se; r += 64) {
+          double const e = err[r];
+          int const b = err_bin(e, E, eE);
+          atomicAdd(&S.wc[wv][b], 1);
+          atomicAdd(&S.hq[b], err_q(e, qs));
+        }
       }
       __syncthreads();
       if (tid < 64) {

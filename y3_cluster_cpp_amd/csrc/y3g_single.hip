@@ -142,10 +142,11 @@ __global__ __launch_bounds__(BLOCK) void k_s_hist(SingleArgs A, int N, double E,
   __shared__ unsigned long long hq[64];
   if (threadIdx.x < 64) hc[threadIdx.x] = 0, hq[threadIdx.x] = 0ull;
   __syncthreads();
+  double const qs = err_qscale(E);
   for (int r = blockIdx.x * BLOCK + threadIdx.x; r < N; r += gridDim.x * BLOCK) {
     int const b = err_bin(A.err[r], E, eE);
     atomicAdd(&hc[b], 1);
-    atomicAdd(&hq[b], err_q(A.err[r], E));
+    atomicAdd(&hq[b], err_q(A.err[r], qs));
   }
   __syncthreads();
   if (threadIdx.x < 64) {
