@@ -72,6 +72,16 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # evaluations (profiles/k_cubature_prior_stream_ext.md).
 # Traffic per launch = this x the launch's evaluations.
 HBM_BYTES_PER_EVAL = (1061414897.4 * 1.99979 + 489082176.8) * 1024 / 5.53529779267e11
+# HBM bytes per term of k_selgl (1024 points x 12 bins x 96 x 64 nodes) and
+# per evaluation of k_cubature<double,512> on the real-data Delta-Sigma leg
+# (4 points x 180 integrals, synthetic DES inputs), from the same counters
+# and copy factors (profiles/k_selgl_numcountssel.md,
+# profiles/k_cubature_fp64_des_dsigma.md): k_selgl 219,343 KiB fetched and
+# 384 KiB written per launch of 75,497,472 terms; the Delta-Sigma launch
+# 2,607,762 KiB fetched and 14,250,444 KiB written (register spills) over
+# 1.365657e9 evaluations.
+SELGL_HBM_BYTES_PER_TERM = (219343.0 * 1.99979 + 384.0) * 1024 / 75497472
+DES_HBM_BYTES_PER_EVAL = (2607762.0 * 1.99979 + 14250444.0) * 1024 / 1.365657012e9
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_cubature<float,512>, "
                   "calibrated by a 256 MiB copy; bytes per evaluation x this launch's "
                   "evaluations")
@@ -336,7 +346,10 @@ def selgl_run(y3, rt, spec, n_points, reps, cpu_seconds, with_cpu):
                       "call_ms_p99": float(np.percentile(one_w, 99) * 1e3)},
         "reps": reps, "max_rel_vs_oracle": float(rel), "checked_bins": check * SEL_BINS,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": terms * SELGL_HBM_BYTES_PER_TERM,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
+                                       "launch, calibrated by a 256 MiB copy",
                      "kernel": "k_selgl", "terms_per_launch": terms,
                      "flops_per_term": SELGL_FLOPS_PER_TERM},
         "cpu_baseline": cpu,
@@ -382,7 +395,11 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
         ach = nev * DES_FLOPS_PER_EVAL / (ms * 1e-3) / 1e12
         out["roofline"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP64 / 1e12,
                            "unit": "TFLOP/s", "frac": ach / (PEAK_FP64 / 1e12),
-                           "traffic": None, "kernel": "k_cubature<double,512>",
+                           "traffic": nev * DES_HBM_BYTES_PER_EVAL,
+                           "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this "
+                                             "launch shape, calibrated by a 256 MiB copy; bytes "
+                                             "per evaluation x this launch's evaluations",
+                           "kernel": "k_cubature<double,512>",
                            "flops_per_eval": DES_FLOPS_PER_EVAL,
                            "sfu_per_eval": DES_SFU_PER_EVAL}
     # the oracle on the first integrals (point 0): parity within the errors
