@@ -209,13 +209,16 @@ typedef struct {
                              true error (0 of 12,288 below it, against 2
                              at 1.0)                                     */
   int ext_regions;        /* budget extension (cubature; 0 = off): an
-                             integral whose call limit is spent while
-                             its reported error is still above
-                             ext_ratio x its tolerance continues, once,
-                             up to ext_regions regions (<= max_regions).
+                             integral whose call limit would cut its
+                             next refinement step while its reported
+                             error is still above ext_ratio x its
+                             tolerance gets the limit doubled, as often
+                             as that holds, up to ext_regions regions
+                             (<= max_regions); it then refines exactly
+                             as with its final limit from the start.
                              The base budget (max_regions or its band)
                              serves the integrals near their tolerance;
-                             the few left far from it get the larger one */
+                             those left far from it get larger ones     */
   double ext_ratio;
 } y3g_options;
 

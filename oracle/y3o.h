@@ -225,9 +225,10 @@ typedef struct {
   int init_split_lnm;       /* initial pieces along the synthetic kinds'
                                lnM axis (0: init_split)                    */
   double err_scale;         /* safety factor on E (tested and reported)    */
-  int ext_regions;          /* budget extension: a spent call limit with   */
-  double ext_ratio;         /* the error above ext_ratio x the tolerance is
-                               raised once to 2 ext_regions nodes (0: off) */
+  int ext_regions;          /* budget extension: a call limit about to cut */
+  double ext_ratio;         /* a step with the error above ext_ratio x the
+                               tolerance is doubled, up to 2 ext_regions
+                               nodes (0: off)                              */
 } y3o_cub_opts;
 void y3o_cubature(const y3o_params* p, const y3o_tables* t,
                   const y3o_integral* ig, const y3o_cub_opts* o,

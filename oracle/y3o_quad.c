@@ -718,8 +718,8 @@ two_level_scale(double d, double e, double c, double smin)
 
 /* Budget extension (device: ext_budget in y3g_cub_common.hpp): a call
  * limit lim that would cut the next refinement step while the reported
- * error err is above ratio x tol is raised once to ext_eval; the integral
- * then refines as a run with ext_eval from the start. */
+ * error err is above ratio x tol is doubled, up to ext_eval; the integral
+ * then refines as a run with its final limit from the start. */
 static int
 ext_budget(long long lim, long long ext_eval, double ratio, double err, double tol)
 {
@@ -867,8 +867,8 @@ cub_run(const cub_prob* Pin, const y3o_cub_opts* o, y3o_result* res)
     /* at most kmax splits, within the capacity and the call limit */
     int room = (kmax < cap - n) ? kmax : cap - n;
     long long left = (lim - neval) / (2LL * nn);
-    if (left < room && ext_budget(lim, ext_eval, o->ext_ratio, escale * E + P->sb * fabs(I), tol)) {
-      lim = ext_eval;
+    while (left < room && ext_budget(lim, ext_eval, o->ext_ratio, escale * E + P->sb * fabs(I), tol)) {
+      lim = 2 * lim < ext_eval ? 2 * lim : ext_eval;
       left = (lim - neval) / (2LL * nn);
     }
     if (left < room) room = (int)left;

@@ -396,19 +396,24 @@ def test_budget_extension(runtime):
     nn = 33
     base, ext = 1024, 4096
     ig = y3.make_integral(p, s, y3.COUNT3D, 20., 30., eps_rel=1e-9)
-    # the error at the base budget is ~1e-5 relative, 1e4 x the tolerance
-    for ratio, extended in ((2.0, True), (1e9, False)):
-        o = dict(bands=[(1e-3, base)], max_regions=ext, ext_regions=ext, ext_ratio=ratio)
+    # the error at the base budget is ~1.2e4 x the tolerance, ~3.2e3 x near
+    # twice the budget: ratio 2 doubles the limit up to ext, 1e9 keeps the
+    # base budget, 6e3 (64 splits per step) doubles it once
+    for ratio, kmax, extended in ((2.0, 1024, True), (1e9, 1024, False), (6e3, 64, True)):
+        o = dict(bands=[(1e-3, base)], max_regions=ext, ext_regions=ext, ext_ratio=ratio,
+                 max_split_per_step=kmax)
         g = runtime.integrate([ig], y3.options(precision=y3.FP64, **o))[0]
         c = O.cubature(op, ot, oig, O.cub_opts(eps_rel=1e-9, max_regions=ext,
                                                max_eval=2 * base * nn, ext_regions=ext,
-                                               ext_ratio=ratio))
+                                               ext_ratio=ratio, max_split_per_step=kmax))
         assert (g["nregions"], g["neval"], g["status"]) == (c.nregions, c.neval, c.status)
         assert abs(g["value"] / c.value - 1) < 1e-11
-        if extended:
-            assert 2 * base * nn < g["neval"] <= 2 * ext * nn
-        else:
+        if not extended:
             assert g["neval"] <= 2 * base * nn
+        elif kmax == 64:
+            assert g["neval"] == 4 * base * nn
+        else:
+            assert 2 * base * nn < g["neval"] <= 2 * ext * nn
         # the plain path and the FP32 integrand follow the same rule (their
         # sums differ from the FP64 batched ones in the last bits)
         for q in (dict(precision=y3.FP64, single=1), dict(precision=y3.FP32SPLIT)):

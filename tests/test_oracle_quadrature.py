@@ -240,11 +240,12 @@ def test_mapped_integrand_equals_physical():
 
 
 def test_budget_extension_equals_larger_budget():
-    """The budget extension continues the same refinement: an integral that
-    spends its call limit with the error above ext_ratio x tol ends exactly
-    where a run with the extended limit from the start ends (regions,
-    evaluations, value); with the error within that ratio, or the ratio
-    out of reach, it stops at the base limit."""
+    """The budget extension continues the same refinement: an integral whose
+    call limit would cut a step with the error above ext_ratio x tol gets
+    the limit doubled (up to ext_regions) and ends exactly where a run with
+    its final limit from the start ends (regions, evaluations, value); with
+    the error within that ratio, or the ratio out of reach, it stops at the
+    base limit."""
     p = O.default_params(f_prj=0.0)
     s = O.spec_config0()
     t = O.OracleTables(p, s)
@@ -260,6 +261,17 @@ def test_budget_extension_equals_larger_budget():
     assert plain.neval <= 2 * base * nn < extd.neval
     assert (extd.neval, extd.nregions, extd.status) == (big.neval, big.nregions, big.status)
     assert extd.value == big.value and extd.error == big.error
+    # the limit doubles while the error stays above the ratio: ~1.2e4 x tol
+    # near 1024 regions, ~3.2e3 x tol near 2048 (64 splits per step, so the
+    # decisions fall just before each limit), so a ratio of 6e3 doubles once
+    # and the run equals one with 2048 regions from the start
+    mid = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=2 * ext,
+                                          max_eval=2 * base * nn, ext_regions=2 * ext,
+                                          ext_ratio=6e3, max_split_per_step=64))
+    two = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=2 * ext,
+                                          max_eval=4 * base * nn, max_split_per_step=64))
+    assert mid.neval == 4 * base * nn
+    assert (mid.neval, mid.nregions, mid.value) == (two.neval, two.nregions, two.value)
     # the error at the base limit is ~1e-5 relative (1e4 x tol): a ratio
     # above that keeps the base limit
     far = O.cubature(p, t, ig, O.cub_opts(eps_rel=1e-9, max_regions=ext,

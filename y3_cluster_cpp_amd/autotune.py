@@ -23,8 +23,9 @@ values come from `tests/golden/golden_points.json`. It chooses:
      many more integrals outside their tolerance.
 2. **Budget extension.** The bands' budgets are scaled by f (0.25 ... 1),
    and an integral whose reported error is still above ratio x eps_rel
-   when its budget is spent continues, once, up to ext regions (65536 or
-   131072; y3g_options.ext_regions / ext_ratio).  Each setting runs the
+   when its budget would cut its next step gets the budget doubled, as
+   often as that holds, up to ext regions (65536, 131072 or 262144;
+   y3g_options.ext_regions / ext_ratio).  Each setting runs the
    calibration integrals at their capped tolerances.  Of the settings that
    count at least 99.3% of them (99% and a margin for the sample's size)
    with no reported error below the true error, the one with the most
@@ -56,7 +57,7 @@ BUDGETS = (4096, 8192, 12288, 16384, 24576, 32768, 65536)
 TIGHT_EDGES = (2e-5, 5e-5, 1e-4)
 LOOSE_EDGES = (2e-4, 5e-4, 1e-3)
 KEEP = 0.97  # share of the largest budget's in-tolerance integrals kept
-EXT_REGIONS = (65536, 131072)          # extended budgets tried
+EXT_REGIONS = (65536, 131072, 262144)  # largest extended budgets tried
 BAND_SCALES = (0.25, 0.375, 0.5, 0.75, 1.0)  # base budgets: the bands' x f
 EXT_RATIOS = (3.0, 5.0, 10.0, 20.0)    # reported error / eps that extends
 # counted share of the capped calibration integrals sought: 99% plus a

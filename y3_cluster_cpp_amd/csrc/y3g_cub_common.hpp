@@ -705,8 +705,9 @@ __host__ __device__ inline double two_level_scale(double d, double e, double c, 
 
 // Budget extension: an integral whose call limit `lim` would cut its next
 // refinement step while its reported error `err` is still above ratio x its
-// tolerance `tol` gets the larger limit ext_eval, once (lim < ext_eval); it
-// then refines exactly as a run with ext_eval from the start.  Such integrals are the ones
+// tolerance `tol` gets its limit doubled, up to ext_eval (while lim <
+// ext_eval); it then refines exactly as a run with its final limit from the
+// start.  Such integrals are the ones
 // the base budget leaves far from converged; those near their tolerance,
 // whose |R7-R5| sum over-states the true error, stop at the base budget.
 // Restated by the oracle (y3o_quad.c ext_budget).

@@ -449,8 +449,8 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       I = a;
       E = b;
     }
-    // the call limit; raised once to ext_eval when the budget runs out with
-    // the estimate still above ext_ratio x the tolerance (ext_budget)
+    // the call limit; doubled, up to ext_eval, each time it would cut a step
+    // with the estimate still above ext_ratio x the tolerance (ext_budget)
     long long lim = S.ig.max_eval;
     long long const ext_eval = A.ext_regions > 0 ? 2LL * A.ext_regions * nn : 0;
     for (;;) {
@@ -466,8 +466,8 @@ __device__ __forceinline__ void cubature_items(const KernArgs& A, const StreamAr
       // at most kmax splits, within the slab and the call limit
       int const room0 = min(A.kmax, cap - N);
       long long left = (lim - neval) / (2LL * nn);
-      if (left < room0 && ext_budget(lim, ext_eval, A.ext_ratio, es * E + sb * fabs(I), tol)) {
-        lim = ext_eval;
+      while (left < room0 && ext_budget(lim, ext_eval, A.ext_ratio, es * E + sb * fabs(I), tol)) {
+        lim = min(2 * lim, ext_eval);
         left = (lim - neval) / (2LL * nn);
       }
       int const room = (int)min((long long)room0, left);

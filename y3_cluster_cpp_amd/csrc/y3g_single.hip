@@ -292,7 +292,7 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
   double IE[2] = {0.0, 0.0};
   double const sb = smooth_bound(ig.kind, A.K.hmfc32 != nullptr);
   double const es = sb > 0.0 ? A.K.err_scale : 1.0;
-  // call limit, raised once by the budget extension (ext_budget)
+  // call limit, doubled up to ext_eval by the budget extension (ext_budget)
   long long lim = ig.max_eval;
   long long const ext_eval = A.K.ext_regions > 0 ? 2LL * A.K.ext_regions * nn : 0;
   int const nred = 256;
@@ -313,9 +313,9 @@ hipError_t run_single(SingleArgs A, const y3g_integral& ig, int precision, int k
     }
     int const room0 = std::min(kmax, cap - N);
     long long calls_left = (lim - neval) / (2LL * nn);
-    if (calls_left < room0 &&
-        ext_budget(lim, ext_eval, A.K.ext_ratio, es * E + sb * std::fabs(I), tol)) {
-      lim = ext_eval;
+    while (calls_left < room0 &&
+           ext_budget(lim, ext_eval, A.K.ext_ratio, es * E + sb * std::fabs(I), tol)) {
+      lim = std::min(2 * lim, ext_eval);
       calls_left = (lim - neval) / (2LL * nn);
     }
     int const room = (int)std::min((long long)room0, calls_left);
