@@ -16,8 +16,8 @@ persistent cubature launch.  Warm-up steps re-run slice 0 before the clock
 starts.  An integral is counted when |I - I_oracle| <= eps_rel |I_oracle|,
 that error is <= 1e-4, and the reported error bounds it.  Budgets as tuned
 (y3_cluster_cpp_amd/tuning/gfx950.json): 32,768 regions per integral, and
-the budget extension to 131,072 for the integrals whose reported error is
-still above 10x their tolerance; `base_budgets_only` re-runs the stream
+the budget extension (doubled, up to 131,072) for the integrals whose
+reported error is still above 10x their tolerance; `base_budgets_only` re-runs the stream
 without the extension.  I_oracle are the
 FP64 oracle's golden values of a stratified subset of the stream -- every
 16th point, 256 points = 49,152 integrals, tests/golden/golden_stream.json
@@ -68,10 +68,10 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
 # workload: rocprofv3 FETCH_SIZE x 1.99979 + WRITE_SIZE x 1.000 (factors from
 # a 256 MiB copy read by the same counters) over the evaluations of the same
 # launch, a 205-point step with the tuned budgets and budget extension:
-# 1,061,414,897 KiB fetched, 489,082,177 KiB written, 5.53530e11
+# 766,376,263 KiB fetched, 405,981,544 KiB written, 4.67442e11
 # evaluations (profiles/k_cubature_prior_stream_ext.md).
 # Traffic per launch = this x the launch's evaluations.
-HBM_BYTES_PER_EVAL = (1061414897.4 * 1.99979 + 489082176.8) * 1024 / 5.53529779267e11
+HBM_BYTES_PER_EVAL = (766376263.2 * 1.99979 + 405981544.5) * 1024 / 4.67442080899e11
 # HBM bytes per term of k_selgl (1024 points x 12 bins x 96 x 64 nodes) and
 # per evaluation of k_cubature<double,512> on the real-data Delta-Sigma leg
 # (4 points x 180 integrals, synthetic DES inputs), from the same counters
@@ -505,6 +505,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--serve-rates", default="4,12,24,48,96")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-base-rates", default="4,24,96",
+                    help="rates of the serving run at the base budgets alone ('' skips it)")
     ap.add_argument("--no-selgl", action="store_true")
     ap.add_argument("--selgl-only", action="store_true",
                     help="run the fixed-GL selection counts leg alone and print its object")
@@ -705,10 +707,10 @@ def main():
     # throughput the extension trades for the integrals it brings within
     # their tolerance
     base_only = None
+    bo = y3.options(precision=y3.FP32SPLIT,
+                    **{**topts, "ext_regions": 0, "ext_ratio": 0.0,
+                       "max_regions": max([b for _, b in bands] or [DEFAULT_REGIONS])})
     if opts.ext_regions > 0 and not a.no_base_only:
-        bo = y3.options(precision=y3.FP32SPLIT,
-                        **{**topts, "ext_regions": 0, "ext_ratio": 0.0,
-                           "max_regions": max([b for _, b in bands] or [DEFAULT_REGIONS])})
         ok_b = []
         t1 = time.perf_counter()
         for s_i, sl in enumerate(timed):
@@ -785,6 +787,14 @@ def main():
     if not a.no_serve:
         rates = [float(r) for r in a.serve_rates.split(",") if r]
         serving = serving_run(y3, pts64, ref64, eps_raw[:len(pts64)], rates, a.serve_seconds, opts)
+        if opts.ext_regions > 0 and a.serve_base_rates:
+            # the same requests at the base budgets alone: the latency the
+            # extension's longer integrals add
+            brates = [float(r) for r in a.serve_base_rates.split(",") if r]
+            sb = serving_run(y3, pts64, ref64, eps_raw[:len(pts64)], brates, a.serve_seconds, bo)
+            serving["base_budgets_only"] = {"rates": sb["rates"],
+                                            "budgets": "the tolerance bands' budgets, no "
+                                                       "extension"}
 
     selgl = None
     if not a.no_selgl:
