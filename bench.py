@@ -356,6 +356,85 @@ def selgl_run(y3, rt, spec, n_points, reps, cpu_seconds, with_cpu):
     }
 
 
+def vegas_run(y3, rt, n, calls, with_cpu, cpu_n):
+    """Config 0 by m-Cubes: the 3-D count over lambda_obs in [20, 30] at the
+    fiducial point (tables 200 x 32 over the config-0 box), eps_rel 1e-4,
+    5 iterations of `calls` calls, as n copies with their own Philox seeds
+    in one launch, FP32 integrand and FP64; counted against the oracle's
+    golden value by the headline rule.  CPU baseline: the oracle's VEGAS
+    restatement on cpu_n of the same copies (same seeds), one per thread;
+    its values equal the FP64 device path's."""
+    with open(os.path.join(ROOT, "tests", "golden", "golden_fiducial.json")) as f:
+        gold = json.load(f)["config0"]["value"]
+    p = y3.default_params(f_prj=0.0)
+    s = y3.spec_config0()
+    rt.tables_build(p, s)
+    seed0 = SEED
+    igs = [y3.make_integral(p, s, y3.COUNT3D, 20.0, 30.0, eps_rel=1e-4,
+                            algorithm=y3.MCUBES, seed=seed0 + i) for i in range(n)]
+    arr = (y3.Integral * n)(*igs)
+    out = {"workload": f"config 0 (3-D count, lambda_obs in [20, 30], fiducial point) by "
+                       f"m-Cubes, eps_rel 1e-4, 5 iterations x {calls:,} calls, {n} copies "
+                       f"with their own seeds in one launch",
+           "unit": "integrals/s"}
+    res64 = None
+    for prec, key in ((y3.FP32SPLIT, "fp32split"), (y3.FP64, "fp64")):
+        o = y3.options(precision=prec, vegas_calls=calls, vegas_itmax=5)
+        rt.integrate(arr, o)  # untimed: first launch
+        t0 = time.perf_counter()
+        r = rt.integrate(arr, o)
+        wall = time.perf_counter() - t0
+        ms, nev = rt.last_timing()
+        err = np.abs(r["value"] - gold)
+        ok = (err <= 1e-4 * gold) & (r["error"] >= err)
+        t1 = time.perf_counter()
+        rt.integrate(arr[:1], o)
+        one = time.perf_counter() - t1
+        ach = nev * FLOPS_PER_EVAL[0] / (ms * 1e-3) if ms > 0 else 0.0
+        out[key] = {"value": float(ok.sum()) / wall, "integrals_per_s": n / wall,
+                    "counted_frac": float(ok.mean()), "kernel_ms": ms,
+                    "evals_per_s": nev / (ms * 1e-3) if ms > 0 else 0.0,
+                    "median_rel_err": float(np.median(err / gold)),
+                    "one_integral_ms": one * 1e3,
+                    "roofline": {"bound": "valu", "achieved": ach / 1e12,
+                                 "peak": (PEAK_FP32 if prec == y3.FP32SPLIT else PEAK_FP64) / 1e12,
+                                 "unit": "TFLOP/s",
+                                 "frac": ach / (PEAK_FP32 if prec == y3.FP32SPLIT else PEAK_FP64),
+                                 "traffic": None, "kernel": "k_vegas",
+                                 "flops_per_eval": FLOPS_PER_EVAL[0]}}
+        if prec == y3.FP64:
+            res64 = r
+    out["value"] = out["fp32split"]["value"]
+    if with_cpu:
+        import oracle as O
+        op = O.default_params(f_prj=0.0)
+        osp = O.spec_config0()
+        oig = O.make_integral(op, osp, O.COUNT3D, 20.0, 30.0)
+        m = cpu_n
+        oarr = (O.Integral * m)(*([oig] * m))
+        cres = (O.Result * m)()
+        threads = max(1, min(16, os.cpu_count() or 1, m))
+        ceps = np.full(m, 1e-4)
+        t0 = time.perf_counter()
+        used = O.lib.y3o_batch_integrate(
+            C.byref(op), C.byref(osp), 1, oarr, (C.c_int * m)(*([0] * m)),
+            O.ptr(ceps), (C.c_longlong * m)(*([1 << 40] * m)), m, 1,
+            C.byref(O.cub_opts()),
+            C.byref(O.vegas_opts(eps_rel=1e-4, calls_per_iter=calls, itmax=5, seed=seed0)),
+            threads, cres)
+        dt = time.perf_counter() - t0
+        cv = np.array([c.value for c in cres])
+        ce = np.array([c.error for c in cres])
+        okc = (np.abs(cv - gold) <= 1e-4 * gold) & (ce >= np.abs(cv - gold))
+        out["cpu_baseline"] = {"value": float(okc.sum()) / dt, "unit": "integrals/s",
+                               "cores": int(used), "kind": "port",
+                               "sample": f"{m} of the same copies by the oracle's VEGAS "
+                                         f"restatement, one per thread, {dt:.1f} s",
+                               "integrals_per_s": m / dt}
+        out["max_rel_diff_fp64_vs_oracle"] = float(np.max(np.abs(res64["value"][:m] / cv - 1)))
+    return out
+
+
 def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots=0):
     """The real-data Delta-Sigma integrand (the SigmaMiscentY1MortCUDAScalar-
     Integrand module's device work): n_points parameter points of the prior
@@ -513,6 +592,12 @@ def main():
     ap.add_argument("--selgl-points", type=int, default=1024)
     ap.add_argument("--selgl-reps", type=int, default=200)
     ap.add_argument("--selgl-cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--no-vegas", action="store_true")
+    ap.add_argument("--vegas-only", action="store_true",
+                    help="run the config-0 m-Cubes leg alone and print its object")
+    ap.add_argument("--vegas-copies", type=int, default=768)
+    ap.add_argument("--vegas-calls", type=int, default=6_000_000)
+    ap.add_argument("--vegas-cpu", type=int, default=16)
     ap.add_argument("--no-des", action="store_true")
     ap.add_argument("--des-only", action="store_true",
                     help="run the real-data Delta-Sigma leg alone and print its object")
@@ -549,6 +634,12 @@ def main():
         rt = y3.Runtime(0)
         print(json.dumps(des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
                                  a.des_cpu_integrals, a.des_slots)))
+        rt.close()
+        return
+    if a.vegas_only:
+        rt = y3.Runtime(0)
+        print(json.dumps(vegas_run(y3, rt, a.vegas_copies, a.vegas_calls, not a.no_cpu,
+                                   a.vegas_cpu)))
         rt.close()
         return
     if a.selgl_only:
@@ -803,6 +894,12 @@ def main():
         print(f"fixed-GL counts: {selgl['value']:.3g} bin counts/s", file=sys.stderr,
               flush=True)
 
+    vegas_leg = None
+    if not a.no_vegas:
+        vegas_leg = vegas_run(y3, rt, a.vegas_copies, a.vegas_calls, not a.no_cpu, a.vegas_cpu)
+        print(f"config-0 m-Cubes: {vegas_leg['value']:.3g} counted integrals/s", file=sys.stderr,
+              flush=True)
+
     des_leg = None
     if not a.no_des:
         des_leg = des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
@@ -869,6 +966,7 @@ def main():
         if plain and plain["counted_per_s"] > 0 else None,
         "serving": serving,
         "fixed_gl_counts": selgl,
+        "config0_vegas": vegas_leg,
         "des_dsigma": des_leg,
     }
     print(json.dumps(out))
