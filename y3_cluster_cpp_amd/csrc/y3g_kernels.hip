@@ -696,6 +696,7 @@ template <typename T, int BLOCK> struct VegShared {
   double xg[kMaxDim][kNbMax + 1];
   double hist[kMaxDim][kNbMax];
   double d1[kMaxDim][kNbMax];
+  double xn[kMaxDim][kNbMax + 1];  // the refined grid, before it replaces xg
   double red[2 * (BLOCK / 64)];
   y3g_integral ig;
   PointDev P;
@@ -760,10 +761,36 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
     for (int q = tid; q < d * nb; q += BLOCK) S.hist[q / nb][q % nb] = 0.0;
     __syncthreads();
     double sI = 0.0, sV = 0.0;
-    for (long long c = tid; c < ncubes; c += BLOCK) {
-      int ck[d];
-      long long cc = c;
+    // each wave takes a contiguous run of cubes (axis 0 fastest), its lanes
+    // 64 adjacent cubes at a time: a lane's cube digits advance by 64 with a
+    // carry instead of 64-bit divisions, and its samples stay in the same
+    // grid bin along every axis but the first for long stretches, so a lane
+    // sums its contribution to a bin and adds it to the shared histogram
+    // only when the bin changes
+    // (the FP64 kernel, at its register limit, adds every sample directly:
+    // the lane sums would live in scratch memory there)
+    constexpr bool kLocalHist = sizeof(T) == 4;
+    constexpr int NW = BLOCK / 64;
+    long long const perw = ((ncubes + NW - 1) / NW + 63) / 64 * 64;
+    long long const wbeg = min(ncubes, (long long)(tid >> 6) * perw);
+    long long const wend = min(ncubes, wbeg + perw);
+    int ck[d];
+    {
+      long long cc = wbeg + (tid & 63);
       for (int a = 0; a < d; ++a) ck[a] = (int)(cc % g), cc /= g;
+    }
+    int hb[d];
+    double hs[d];
+    for (int a = 0; a < d; ++a) hb[a] = -1, hs[a] = 0.0;
+    for (long long c = wbeg + (tid & 63); c < wend; c += 64) {
+      if (c >= wbeg + 64) {
+        ck[0] += 64;
+        for (int a = 0; a < d - 1 && ck[a] >= g; ++a) {
+          int const q = ck[a] / g;
+          ck[a] -= q * g;
+          ck[a + 1] += q;
+        }
+      }
       double s1 = 0.0, s2 = 0.0;
       for (long long j = 0; j < p; ++j) {
         uint32_t w[12];
@@ -793,13 +820,27 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
           fx = double(integrand_k<T, K>(S.ig, S.P, tb, y)) * jac;
         s1 += fx;
         s2 += fx * fx;
-        for (int a = 0; a < d; ++a) atomicAdd(&S.hist[a][bins[a]], fx * fx);
+        for (int a = 0; a < d; ++a) {
+          if constexpr (kLocalHist) {
+            if (bins[a] != hb[a]) {
+              if (hb[a] >= 0) atomicAdd(&S.hist[a][hb[a]], hs[a]);
+              hb[a] = bins[a];
+              hs[a] = 0.0;
+            }
+            hs[a] += fx * fx;
+          } else {
+            atomicAdd(&S.hist[a][bins[a]], fx * fx);
+          }
+        }
       }
       double const m = s1 / p;
       double const v = (s2 - s1 * m) / (double(p) * (p - 1));
       sI += m;
       sV += v > 0.0 ? v : 0.0;
     }
+    if constexpr (kLocalHist)
+      for (int a = 0; a < d; ++a)
+        if (hb[a] >= 0) atomicAdd(&S.hist[a][hb[a]], hs[a]);
     block_sum2<BLOCK>(sI, sV, S.red);
     neval += ncubes * p;
     double const Ii = sI / ncubes;
@@ -848,7 +889,7 @@ __device__ __forceinline__ void vegas_one(VegShared<T, BLOCK>& S, const KernArgs
           }
           if (rs > 0.0) {
             double const target = rs / nb;
-            double xn[kNbMax + 1];
+            double* xn = S.xn[a];
             int kk = -1;
             double dr = 0.0;
             xn[0] = 0.0;
