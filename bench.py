@@ -435,7 +435,8 @@ def vegas_run(y3, rt, n, calls, with_cpu, cpu_n):
     return out
 
 
-def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots=0):
+def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots=0,
+            ext_regions=0):
     """The real-data Delta-Sigma integrand (the SigmaMiscentY1MortCUDAScalar-
     Integrand module's device work): n_points parameter points of the prior
     stream, each with the synthetic DES inputs and the reference demo's 180
@@ -458,7 +459,11 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
             ig.point = k
             igs.append(ig)
     arr = (y3.Integral * len(igs))(*igs)
-    opts = y3.options(precision=y3.FP64, max_regions=regions, slots=slots)
+    # integrals not converged at `regions` get the budget doubled up to
+    # ext_regions (the budget extension at ratio 1)
+    opts = y3.options(precision=y3.FP64, max_regions=max(regions, ext_regions), slots=slots,
+                      bands=[(1.0, regions)], ext_regions=ext_regions,
+                      ext_ratio=1.0 if ext_regions > regions else 0.0)
     rt.integrate(arr, opts)  # untimed: uploads
     t0 = time.perf_counter()
     res = rt.integrate(arr, opts)
@@ -466,7 +471,9 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
     ms, nev = rt.last_timing()
     out = {"workload": f"{n_points} parameter points x 180 integrals (4 lambda_obs x 3 z_obs "
                        f"bins x 15 radii), synthetic DES inputs, eps_rel {eps}, "
-                       f"{regions} regions per integral, FP64",
+                       f"{regions} regions per integral"
+                       + (f" (doubled up to {ext_regions} while not converged)"
+                          if ext_regions > regions else "") + ", FP64",
            "value": len(igs) / wall, "unit": "integrals/s", "kernel_ms": ms,
            "evals_per_s": nev / (ms * 1e-3) if ms > 0 else 0.0,
            "converged": int((res["status"] == 0).sum()), "integrals": len(igs)}
@@ -497,9 +504,14 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
         for a in range(5):
             oi.lows[a], oi.highs[a] = lows[a + 1], highs[a + 1]
         r = O.Result()
+        # the same budgets: `regions` (5-D: 93 nodes per region), doubled
+        # up to ext_regions while not converged
         O.lib.y3o_cubature_des(C.byref(op), C.byref(ot.t), O.ptr(tabs), C.byref(oi),
-                               C.byref(O.cub_opts(eps_rel=eps, max_regions=regions,
-                                                  max_eval=1 << 40)), C.byref(r))
+                               C.byref(O.cub_opts(eps_rel=eps,
+                                                  max_regions=max(regions, ext_regions),
+                                                  max_eval=2 * regions * 93,
+                                                  ext_regions=ext_regions,
+                                                  ext_ratio=opts.ext_ratio)), C.byref(r))
         return r.value, r.error
 
     # one integral per thread on the host's cores (the oracle's C code runs
@@ -604,6 +616,8 @@ def main():
     ap.add_argument("--des-points", type=int, default=4)
     ap.add_argument("--des-regions", type=int, default=16384)
     ap.add_argument("--des-cpu-integrals", type=int, default=16)
+    ap.add_argument("--des-ext-regions", type=int, default=65536,
+                    help="real-data leg: double the budget of unconverged integrals up to this")
     ap.add_argument("--des-slots", type=int, default=0,
                     help="workgroups of the real-data launch (0: as many as fit)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -633,7 +647,7 @@ def main():
     if a.des_only:
         rt = y3.Runtime(0)
         print(json.dumps(des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
-                                 a.des_cpu_integrals, a.des_slots)))
+                                 a.des_cpu_integrals, a.des_slots, a.des_ext_regions)))
         rt.close()
         return
     if a.vegas_only:
@@ -903,7 +917,7 @@ def main():
     des_leg = None
     if not a.no_des:
         des_leg = des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
-                          a.des_cpu_integrals)
+                          a.des_cpu_integrals, 0, a.des_ext_regions)
         print(f"real-data Delta-Sigma: {des_leg['value']:.3g} integrals/s", file=sys.stderr,
               flush=True)
 
