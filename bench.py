@@ -483,8 +483,9 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
                            "unit": "TFLOP/s", "frac": ach / (PEAK_FP64 / 1e12),
                            "traffic": nev * DES_HBM_BYTES_PER_EVAL,
                            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this "
-                                             "launch shape, calibrated by a 256 MiB copy; bytes "
-                                             "per evaluation x this launch's evaluations",
+                                             "workload at 16,384 regions (no doubling), "
+                                             "calibrated by a 256 MiB copy; bytes per "
+                                             "evaluation x this launch's evaluations",
                            "kernel": "k_cubature<double,512>",
                            "flops_per_eval": DES_FLOPS_PER_EVAL,
                            "sfu_per_eval": DES_SFU_PER_EVAL}
