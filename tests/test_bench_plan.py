@@ -58,3 +58,22 @@ def test_driver_command_plans_and_builds_descriptors():
     plan = json.loads(r.stdout.strip().splitlines()[-1])
     assert plan["integrals"] == 786432 and plan["warmup"] == 5
     assert len(plan["steps"]) == 20 and sum(plan["steps"]) == 4096
+
+
+def test_tuned_options_are_valid():
+    """The committed tuning file loads into options the library accepts:
+    ascending band edges, budgets within max_regions, an extension that the
+    slab holds, and calibration points outside the benchmark's checked
+    subset."""
+    from y3_cluster_cpp_amd import autotune
+    t = autotune.load()
+    assert t is not None
+    edges = [e for e, _ in t["bands"]]
+    assert edges == sorted(edges) and len(edges) <= 8
+    assert all(64 <= r <= t["max_regions"] for _, r in t["bands"])
+    assert 0 < t["ext_regions"] <= t["max_regions"] and t["ext_ratio"] > 0
+    assert t["block_threads"] in (256, 512) and 1 <= t["max_split_per_step"] <= 1024
+    with open(autotune.DEFAULT_PATH) as f:
+        raw = json.load(f)
+    assert not any(k % 16 == 0 for k in raw["calibration_points"])
+    assert raw["extension_target"] > 0.99
