@@ -436,7 +436,7 @@ def vegas_run(y3, rt, n, calls, with_cpu, cpu_n):
 
 
 def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots=0,
-            ext_regions=0):
+            ext_regions=0, block=512):
     """The real-data Delta-Sigma integrand (the SigmaMiscentY1MortCUDAScalar-
     Integrand module's device work): n_points parameter points of the prior
     stream, each with the synthetic DES inputs and the reference demo's 180
@@ -462,6 +462,7 @@ def des_run(y3, rt, spec, n_points, regions, eps, with_cpu, cpu_integrals, slots
     # integrals not converged at `regions` get the budget doubled up to
     # ext_regions (the budget extension at ratio 1)
     opts = y3.options(precision=y3.FP64, max_regions=max(regions, ext_regions), slots=slots,
+                      block_threads=block,
                       bands=[(1.0, regions)], ext_regions=ext_regions,
                       ext_ratio=1.0 if ext_regions > regions else 0.0)
     rt.integrate(arr, opts)  # untimed: uploads
@@ -617,6 +618,8 @@ def main():
     ap.add_argument("--des-points", type=int, default=4)
     ap.add_argument("--des-regions", type=int, default=16384)
     ap.add_argument("--des-cpu-integrals", type=int, default=16)
+    ap.add_argument("--des-block", type=int, default=512, choices=(256, 512),
+                    help="real-data leg: workgroup size of the FP64 cubature")
     ap.add_argument("--des-ext-regions", type=int, default=65536,
                     help="real-data leg: double the budget of unconverged integrals up to this")
     ap.add_argument("--des-slots", type=int, default=0,
@@ -648,7 +651,8 @@ def main():
     if a.des_only:
         rt = y3.Runtime(0)
         print(json.dumps(des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
-                                 a.des_cpu_integrals, a.des_slots, a.des_ext_regions)))
+                                 a.des_cpu_integrals, a.des_slots, a.des_ext_regions,
+                                 a.des_block)))
         rt.close()
         return
     if a.vegas_only:
@@ -918,7 +922,7 @@ def main():
     des_leg = None
     if not a.no_des:
         des_leg = des_run(y3, rt, spec, a.des_points, a.des_regions, 1e-3, not a.no_cpu,
-                          a.des_cpu_integrals, 0, a.des_ext_regions)
+                          a.des_cpu_integrals, 0, a.des_ext_regions, a.des_block)
         print(f"real-data Delta-Sigma: {des_leg['value']:.3g} integrals/s", file=sys.stderr,
               flush=True)
 
